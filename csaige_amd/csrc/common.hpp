@@ -1,0 +1,137 @@
+// Internal helpers shared by the HIP translation units of libsaige_gpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/saige_gpu.h"
+
+namespace saige {
+
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define SAIGE_HIP(call)                                                              \
+  do {                                                                               \
+    hipError_t _e = (call);                                                          \
+    if (_e != hipSuccess) {                                                          \
+      ::saige::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                   \
+                         hipGetErrorString(_e));                                     \
+      return SAIGE_ERR_DEVICE;                                                       \
+    }                                                                                \
+  } while (0)
+
+#define SAIGE_TRY(expr)                  \
+  do {                                   \
+    int _s = (expr);                     \
+    if (_s != SAIGE_OK) return _s;       \
+  } while (0)
+
+#define SAIGE_ARG(cond, msg)                                        \
+  do {                                                              \
+    if (!(cond)) {                                                  \
+      ::saige::set_error("invalid argument: %s (%s)", msg, #cond);  \
+      return SAIGE_ERR_ARG;                                         \
+    }                                                               \
+  } while (0)
+
+// Grow-only device buffer owned by a handle.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes) return SAIGE_OK;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    if (want == 0) return SAIGE_OK;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      set_error("hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+      p = nullptr;
+      return SAIGE_ERR_NOMEM;
+    }
+    bytes = want;
+    return SAIGE_OK;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+  void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+int use_device(int device);
+
+inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+inline size_t ceil_div(size_t x, size_t m) { return (x + m - 1) / m; }
+
+// ----------------------------------------------------------------------------------
+// Column dot products with a deterministic two-stage reduction: per-block partial sums
+// (fixed tree inside the block) then a fixed-order sequential sum per column.
+// ----------------------------------------------------------------------------------
+constexpr int kDotBlock = 256;      // threads per block
+constexpr int kDotChunk = 4096;     // samples per block
+
+// Repacked genotype matrix for the GRM: SNP-major rows of row_bytes = n_pad / 4 bytes,
+// samples in model order, padding samples and padding markers coded 01 (missing -> 0).
+constexpr size_t kSamplePad = 512;
+constexpr size_t kMarkerPad = 256;
+
+struct GrmDevice {
+  size_t n = 0, m = 0, n_pad = 0, m_pad = 0, row_bytes = 0;
+  uint8_t* geno = nullptr;   // m_pad * row_bytes
+  double* lut = nullptr;     // m_pad * 4 standardized values per code (fp64 path)
+  float* scale = nullptr;    // m_pad: 1/sqrt(2p(1-p)) (0 if monomorphic)   (mixed path)
+  float* shift = nullptr;    // m_pad: 2p * scale                           (mixed path)
+  double* scale64 = nullptr; // m_pad
+  double* shift64 = nullptr; // m_pad
+  uint8_t* geno_t = nullptr; // mixed path: sample-major packed copy (n_pad rows of m_pad/4 bytes)
+  int has_missing = 0;
+};
+
+}  // namespace saige
+
+struct saige_grm {
+  int device = 0;
+  int precision = SAIGE_PREC_FP64;
+  hipStream_t stream = nullptr;
+  std::recursive_mutex mu;
+  saige::GrmDevice g;
+  saige::DevBuf geno, lut, scale, shift, scale64, shift64, geno_t, miss_idx, miss_ptr;
+  // workspaces for products and solves
+  saige::DevBuf ws_vt, ws_tpart, ws_t, ws_ypart, ws_in, ws_out;
+  saige::DevBuf ws_vhi, ws_vlo, ws_tmp;
+  // PCG workspaces
+  saige::DevBuf pcg_vec, pcg_scal, pcg_part, pcg_b, pcg_w;
+  uint64_t products = 0;   // column-products performed (diagnostic)
+};
+
+namespace saige {
+
+// GRM product on device: out[:, c] = scale_out * (Z Z^T v[:, c]) * (1/M) for c < k;
+// v and out have leading dimensions ldv / ldo (>= n).  Rows >= n of out are not written.
+// If w_tau is non-null the epilogue writes Sigma*v instead:
+//   out = tau0 * v / max(w, 1e-30) + tau1 * GRM v.
+int grm_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out, size_t ldo,
+                const double* d_w = nullptr, const double* tau = nullptr);
+
+// Column dots: out[c] = sum_i a[i + c*lda] * b[i + c*ldb] for c < k (host result).
+int col_dots(hipStream_t s, const double* a, size_t lda, const double* b, size_t ldb, size_t n,
+             size_t k, double* h_out, DevBuf& part, DevBuf& res);
+// Gram block: out[a_i + na*b_j] = dot(A[:, a_i], B[:, b_j]) (host result, na x nb col-major).
+int gram(hipStream_t s, const double* A, size_t lda, size_t na, const double* B, size_t ldb,
+         size_t nb, size_t n, double* h_out, DevBuf& part, DevBuf& res);
+
+// Batched PCG on Sigma (device pointers; B, X: n x k with leading dimension ld).
+int pcg_sigma_device(saige_grm* h, const double* d_w, const double tau[2], const double* d_b,
+                     size_t ldb, size_t k, double tol, uint32_t max_iter, double* d_x, size_t ldx,
+                     saige_pcg_info* info);
+
+}  // namespace saige

@@ -1,0 +1,245 @@
+// Genotype decode, allele counts and the resident packed-genotype build.
+//
+// Reference: saige-geno/src/plink.rs:170-267 (decode_genotype, read_marker_raw,
+// set_sample_subset, read_marker) and traits.rs:42-54 (MarkerData::compute_af).
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace saige {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+void clear_error() { g_err.clear(); }
+
+int use_device(int device) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    set_error("no HIP device available (%s)", e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+    return SAIGE_ERR_DEVICE;
+  }
+  if (device < 0 || device >= count) {
+    set_error("device %d out of range (%d devices)", device, count);
+    return SAIGE_ERR_ARG;
+  }
+  SAIGE_HIP(hipSetDevice(device));
+  return SAIGE_OK;
+}
+
+// One 2-bit code of sample s (model order -> file order through idx).
+__device__ __forceinline__ uint32_t code_at(const uint8_t* row, const uint32_t* idx, size_t k) {
+  size_t s = idx ? idx[k] : k;
+  return (row[s >> 2] >> (2 * (s & 3))) & 3u;
+}
+
+// Per marker: allele-1 count (sum of dosages over non-missing samples) and n_valid.
+// The sum of the integer dosages is exact, so ac / af / mac match compute_af bit for bit.
+__global__ void marker_stats_kernel(const uint8_t* __restrict__ bed, size_t bpm,
+                                    const uint32_t* __restrict__ idx, size_t n, size_t m,
+                                    double* __restrict__ ac, uint32_t* __restrict__ nvalid,
+                                    double* __restrict__ af, double* __restrict__ mac) {
+  size_t j = blockIdx.x;
+  if (j >= m) return;
+  const uint8_t* row = bed + j * bpm;
+  uint32_t cnt = 0, sum = 0;
+  for (size_t k = threadIdx.x; k < n; k += blockDim.x) {
+    uint32_t c = code_at(row, idx, k);
+    // 00 -> 2, 10 -> 1, 11 -> 0, 01 -> missing
+    cnt += (c != 1u);
+    sum += (c == 0u) ? 2u : (c == 2u ? 1u : 0u);
+  }
+  __shared__ uint32_t s_cnt[256], s_sum[256];
+  s_cnt[threadIdx.x] = cnt;
+  s_sum[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + w];
+      s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double s = (double)s_sum[0];
+    double k = (double)s_cnt[0];
+    ac[j] = s;
+    nvalid[j] = s_cnt[0];
+    af[j] = s_cnt[0] > 0 ? s / (2.0 * k) : 0.0;
+    mac[j] = fmin(s, 2.0 * k - s);
+  }
+}
+
+__global__ void decode_kernel(const uint8_t* __restrict__ bed, size_t bpm,
+                              const uint32_t* __restrict__ idx, size_t n, size_t m,
+                              double* __restrict__ out, size_t ldo) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t j = blockIdx.y;
+  if (k >= n || j >= m) return;
+  uint32_t c = code_at(bed + j * bpm, idx, k);
+  out[j * ldo + k] = c == 0u ? 2.0 : (c == 2u ? 1.0 : (c == 3u ? 0.0 : __longlong_as_double(0x7ff8000000000000ll)));
+}
+
+// Repack selected markers into padded SNP-major rows of the model-ordered samples.
+// Output byte b of a row holds model samples 4b..4b+3; padding samples get code 01.
+__global__ void repack_kernel(const uint8_t* __restrict__ src, size_t src_bpm,
+                              const uint32_t* __restrict__ idx, size_t n, size_t m_chunk,
+                              uint8_t* __restrict__ dst, size_t row_bytes) {
+  size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t j = blockIdx.y;
+  if (b >= row_bytes || j >= m_chunk) return;
+  const uint8_t* row = src + j * src_bpm;
+  uint32_t out = 0;
+  for (int t = 0; t < 4; ++t) {
+    size_t k = 4 * b + t;
+    uint32_t c = (k < n) ? code_at(row, idx, k) : 1u;
+    out |= c << (2 * t);
+  }
+  dst[j * row_bytes + b] = (uint8_t)out;
+}
+
+// Upload the rows of the selected markers (marker_idx[j], or j) in chunks and repack.
+// dst: device rows of row_bytes.  Returns status.
+int upload_repack(hipStream_t s, const uint8_t* bed_body, size_t bpm, size_t n_total,
+                  const uint32_t* d_idx, size_t n, const uint32_t* marker_idx, size_t m,
+                  uint8_t* d_dst, size_t row_bytes) {
+  const size_t chunk = std::max<size_t>(1, std::min<size_t>(m, (size_t)(256u << 20) / std::max<size_t>(bpm, 1)));
+  DevBuf staging;
+  SAIGE_TRY(staging.ensure(chunk * bpm));
+  std::vector<uint8_t> host;
+  bool contiguous = (marker_idx == nullptr);
+  if (!contiguous) host.resize(chunk * bpm);
+  (void)n_total;
+  for (size_t j0 = 0; j0 < m; j0 += chunk) {
+    size_t mc = std::min(chunk, m - j0);
+    const uint8_t* srcp;
+    if (contiguous) {
+      srcp = bed_body + j0 * bpm;
+    } else {
+      for (size_t j = 0; j < mc; ++j)
+        memcpy(host.data() + j * bpm, bed_body + (size_t)marker_idx[j0 + j] * bpm, bpm);
+      srcp = host.data();
+    }
+    SAIGE_HIP(hipMemcpyAsync(staging.p, srcp, mc * bpm, hipMemcpyHostToDevice, s));
+    dim3 grid((unsigned)ceil_div(row_bytes, 256), (unsigned)mc);
+    repack_kernel<<<grid, 256, 0, s>>>(staging.as<uint8_t>(), bpm, d_idx, n, mc,
+                                       d_dst + j0 * row_bytes, row_bytes);
+    SAIGE_HIP(hipGetLastError());
+    SAIGE_HIP(hipStreamSynchronize(s));
+  }
+  return SAIGE_OK;
+}
+
+static int upload_idx(hipStream_t s, const uint32_t* sample_idx, size_t n, size_t n_total,
+                      DevBuf& d) {
+  if (!sample_idx) return SAIGE_OK;
+  for (size_t k = 0; k < n; ++k)
+    if (sample_idx[k] >= n_total) {
+      set_error("sample_idx[%zu] = %u >= n_total %zu", k, sample_idx[k], n_total);
+      return SAIGE_ERR_ARG;
+    }
+  SAIGE_TRY(d.ensure(n * sizeof(uint32_t)));
+  SAIGE_HIP(hipMemcpyAsync(d.p, sample_idx, n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  return SAIGE_OK;
+}
+
+int upload_sample_idx(hipStream_t s, const uint32_t* sample_idx, size_t n, size_t n_total,
+                      DevBuf& d) {
+  return upload_idx(s, sample_idx, n, n_total, d);
+}
+
+}  // namespace saige
+
+using namespace saige;
+
+extern "C" {
+
+const char* saige_last_error(void) { return g_err.c_str(); }
+const char* saige_version(void) { return "saige_gpu 0.1 (gfx950)"; }
+
+int saige_plink_marker_stats(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,
+                             size_t m, const uint32_t* sample_idx, size_t n, double* ac,
+                             uint32_t* n_valid, double* af, double* mac) {
+  clear_error();
+  SAIGE_ARG(bed_body && ac && n_valid && af && mac, "null pointer");
+  SAIGE_ARG(bpm >= (n_total + 3) / 4, "bytes_per_marker < ceil(n_total/4)");
+  if (!sample_idx) SAIGE_ARG(n == n_total, "n must equal n_total without a subset");
+  if (m == 0) return SAIGE_OK;
+  SAIGE_TRY(use_device(device));
+  hipStream_t s;
+  SAIGE_HIP(hipStreamCreate(&s));
+  int rc = SAIGE_OK;
+  {
+    DevBuf d_idx, d_bed, d_ac, d_nv, d_af, d_mac;
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(m, (size_t)(512u << 20) / bpm));
+    rc = upload_idx(s, sample_idx, n, n_total, d_idx);
+    if (rc == SAIGE_OK) rc = d_bed.ensure(chunk * bpm);
+    if (rc == SAIGE_OK) rc = d_ac.ensure(chunk * 8);
+    if (rc == SAIGE_OK) rc = d_nv.ensure(chunk * 4);
+    if (rc == SAIGE_OK) rc = d_af.ensure(chunk * 8);
+    if (rc == SAIGE_OK) rc = d_mac.ensure(chunk * 8);
+    for (size_t j0 = 0; rc == SAIGE_OK && j0 < m; j0 += chunk) {
+      size_t mc = std::min(chunk, m - j0);
+      hipError_t e = hipMemcpyAsync(d_bed.p, bed_body + j0 * bpm, mc * bpm, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) {
+        marker_stats_kernel<<<(unsigned)mc, 256, 0, s>>>(d_bed.as<uint8_t>(), bpm,
+                                                         d_idx.as<uint32_t>(), n, mc,
+                                                         d_ac.as<double>(), d_nv.as<uint32_t>(),
+                                                         d_af.as<double>(), d_mac.as<double>());
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipMemcpyAsync(ac + j0, d_ac.p, mc * 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(n_valid + j0, d_nv.p, mc * 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(af + j0, d_af.p, mc * 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(mac + j0, d_mac.p, mc * 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) { set_error("marker stats: %s", hipGetErrorString(e)); rc = SAIGE_ERR_DEVICE; }
+    }
+  }
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+int saige_plink_decode(int device, const uint8_t* bed_body, size_t bpm, size_t n_total, size_t m,
+                       const uint32_t* sample_idx, size_t n, double* dosages) {
+  clear_error();
+  SAIGE_ARG(bed_body && dosages, "null pointer");
+  SAIGE_ARG(bpm >= (n_total + 3) / 4, "bytes_per_marker < ceil(n_total/4)");
+  if (!sample_idx) SAIGE_ARG(n == n_total, "n must equal n_total without a subset");
+  if (m == 0 || n == 0) return SAIGE_OK;
+  SAIGE_TRY(use_device(device));
+  hipStream_t s;
+  SAIGE_HIP(hipStreamCreate(&s));
+  int rc = SAIGE_OK;
+  {
+    DevBuf d_idx, d_bed, d_out;
+    rc = upload_idx(s, sample_idx, n, n_total, d_idx);
+    if (rc == SAIGE_OK) rc = d_bed.ensure(m * bpm);
+    if (rc == SAIGE_OK) rc = d_out.ensure(m * n * 8);
+    if (rc == SAIGE_OK) {
+      hipError_t e = hipMemcpyAsync(d_bed.p, bed_body, m * bpm, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) {
+        dim3 grid((unsigned)ceil_div(n, 256), (unsigned)m);
+        decode_kernel<<<grid, 256, 0, s>>>(d_bed.as<uint8_t>(), bpm, d_idx.as<uint32_t>(), n, m,
+                                           d_out.as<double>(), n);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipMemcpyAsync(dosages, d_out.p, m * n * 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) { set_error("decode: %s", hipGetErrorString(e)); rc = SAIGE_ERR_DEVICE; }
+    }
+  }
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,268 @@
+/*
+ * saige_gpu.h — C ABI of the MI355X (gfx950) implementation of saige-rs's two
+ * data-parallel hot paths: the fit-null AI-REML loop with the on-the-fly GRM product,
+ * and the step-2 single-variant score test with variance ratio and saddle-point
+ * approximation.
+ *
+ * Conventions (mirroring the saige-rs Rust API they replace):
+ *   - Inputs are borrowed, caller-owned buffers (Rust `&[f64]` slices); outputs are
+ *     caller-allocated buffers the library fills (Rust returns owned `Vec`s).
+ *   - Matrices are column-major like saige-linalg's DenseMatrix (dense.rs:10-17).
+ *   - Every entry point returns an int status (saige_status).  Nothing panics or throws
+ *     across the ABI; the message of the last failure on the calling thread is
+ *     saige_last_error().  PCG non-convergence is a flag in the result, not an error,
+ *     as in PcgSolver::solve (decomposition.rs:291-297).
+ *   - Host-pointer entry points copy in and out; `_device` entry points take device
+ *     pointers and run on the handle's HIP stream.
+ *   - A handle may be used from several threads; calls on one handle are serialized
+ *     internally (the Send + Sync closure contract of ai_reml.rs:93-95).
+ *   - Reference citations are relative to the saige-rs workspace (saige-rs/...).
+ */
+#ifndef SAIGE_GPU_H
+#define SAIGE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SAIGE_OK = 0,
+  SAIGE_ERR_ARG = 1,            /* bad dimension / null pointer (reference: assert_eq! panics) */
+  SAIGE_ERR_DEVICE = 2,         /* HIP runtime error or no usable GPU */
+  SAIGE_ERR_NOMEM = 3,          /* device allocation failed */
+  SAIGE_ERR_NOT_PD = 4,         /* LinalgError::NotPositiveDefinite (decomposition.rs:17-18) */
+  SAIGE_ERR_NO_VR_MARKERS = 5,  /* "No markers available for variance ratio" (variance_ratio.rs:214-216) */
+  SAIGE_ERR_NO_MARKERS = 6      /* no marker passes QC */
+} saige_status;
+
+enum { SAIGE_PREC_FP64 = 0, SAIGE_PREC_MIXED = 1 };
+enum { SAIGE_TRAIT_BINARY = 0, SAIGE_TRAIT_QUANTITATIVE = 1, SAIGE_TRAIT_SURVIVAL = 2 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* saige_last_error(void);
+/* Library version string. */
+const char* saige_version(void);
+
+/* ------------------------------------------------------------------------------
+ * Genotype decode and allele counts.
+ *   Replaces PlinkReader::read_marker (saige-geno/src/plink.rs:237-267, decode_genotype
+ *   :170-179, read_marker_raw :182-194) and MarkerData::compute_af (traits.rs:42-54).
+ *   bed_body: SNP-major .bed body without the 3 magic bytes; marker j starts at
+ *   j * bytes_per_marker; sample s at bits 2*(s%4) of byte s/4; codes 00->2, 01->NaN,
+ *   10->1, 11->0 (dosage of bim allele 1).  sample_idx (length n, or NULL for all
+ *   n_total samples in file order) is the set_sample_subset index list (plink.rs:223-235).
+ * ---------------------------------------------------------------------------- */
+int saige_plink_marker_stats(int device, const uint8_t* bed_body, size_t bytes_per_marker,
+                             size_t n_total, size_t m, const uint32_t* sample_idx, size_t n,
+                             double* ac, uint32_t* n_valid, double* af, double* mac);
+/* Dosages of m markers (n x m col-major, NaN = missing). */
+int saige_plink_decode(int device, const uint8_t* bed_body, size_t bytes_per_marker,
+                       size_t n_total, size_t m, const uint32_t* sample_idx, size_t n,
+                       double* dosages);
+
+/* ------------------------------------------------------------------------------
+ * On-the-fly GRM operator.
+ *   Replaces OnTheFlyGrm::new (saige-core/src/glmm/pcg.rs:101-122) and
+ *   OnTheFlyGrm::mat_vec (pcg.rs:128-180): GRM * v = (1/M) * sum_m z_m (z_m' v) with
+ *   z_im = (g_im - 2 af_m) / sqrt(2 af_m (1 - af_m)), missing -> 0, monomorphic -> 0.
+ *   Takes the packed 2-bit genotypes (resident in HBM) instead of f64 dosages; af is
+ *   compute_af of each selected marker over the n samples.
+ *   precision SAIGE_PREC_FP64: f64 standardized values, f64 accumulation.
+ *   precision SAIGE_PREC_MIXED: exact genotypes on bf16 MFMA with the standardization
+ *   factored out, v split into bf16 hi + lo, fp32 accumulation (error stated per output).
+ * ---------------------------------------------------------------------------- */
+typedef struct saige_grm saige_grm;
+
+int saige_grm_create(int device, const uint8_t* bed_body, size_t bytes_per_marker,
+                     size_t n_total, const uint32_t* sample_idx, size_t n,
+                     const uint32_t* marker_idx, size_t m, const double* af, int precision,
+                     saige_grm** out);
+void saige_grm_destroy(saige_grm* h);
+size_t saige_grm_n_samples(const saige_grm* h);
+size_t saige_grm_n_markers(const saige_grm* h);
+/* OnTheFlyGrm::mat_vec (pcg.rs:128-180): out = GRM * v, host vectors of length n. */
+int saige_grm_mat_vec(saige_grm* h, const double* v, double* out);
+/* k products at once: out[:, c] = GRM * v[:, c] (col-major, leading dimensions ldv, ldo). */
+int saige_grm_mat_mat(saige_grm* h, const double* v, size_t ldv, size_t k, double* out,
+                      size_t ldo);
+/* Same on device pointers, enqueued on the handle's stream. */
+int saige_grm_mat_mat_device(saige_grm* h, const double* d_v, size_t ldv, size_t k,
+                             double* d_out, size_t ldo);
+/* The handle's HIP stream (hipStream_t) and a device synchronize helper. */
+void* saige_grm_stream(saige_grm* h);
+int saige_grm_synchronize(saige_grm* h);
+
+/* ------------------------------------------------------------------------------
+ * Preconditioned conjugate gradient on Sigma = tau_e * diag(1/W) + tau_g * GRM.
+ *   Replaces PcgSolver::solve (saige-linalg/src/decomposition.rs:212-298) called with
+ *   the Sigma / diagonal-preconditioner closures of ai_reml.rs:142-164 (and
+ *   fit_null.rs:320-341).  k right-hand sides are solved together; each keeps its own
+ *   alpha, beta and stopping test, so every column is iterate-identical to a separate
+ *   PcgSolver::solve (x0 = 0, stop when ||r|| < tol * max(||b||, 1), breakdown when
+ *   |p'Ap| < 1e-30).
+ * ---------------------------------------------------------------------------- */
+typedef struct {
+  uint32_t iterations;   /* PcgResult::iterations */
+  double residual;       /* PcgResult::residual   */
+  uint8_t converged;     /* PcgResult::converged  */
+} saige_pcg_info;
+
+int saige_pcg_solve_sigma(saige_grm* h, const double* w, const double tau[2], const double* b,
+                          size_t ldb, size_t k, double tol, uint32_t max_iter, double* x,
+                          size_t ldx, saige_pcg_info* info);
+
+/* ------------------------------------------------------------------------------
+ * AI-REML null-model fit.
+ *   Replaces fit_ai_reml (saige-core/src/glmm/ai_reml.rs:86-427) with AiRemlConfig
+ *   (ai_reml.rs:22-51) and AiRemlResult (ai_reml.rs:55-74).  The GRM closure is the
+ *   handle.  Hutchinson probes are the ChaCha8Rng(seed) Rademacher draws of
+ *   ai_reml.rs:121-130.  Outputs mu, w, residuals, eta (length n) and alpha (length p)
+ *   may be NULL.
+ * ---------------------------------------------------------------------------- */
+typedef struct {
+  uint32_t max_iter;          /* default 30 */
+  double tol;                 /* default 0.02 (the CLI passes 1e-5, fit_null.rs:78-80) */
+  double pcg_tol;             /* default 1e-5 */
+  uint32_t pcg_max_iter;      /* default 500 */
+  uint32_t n_random_vectors;  /* default 30 */
+  uint64_t seed;              /* default 12345 */
+} saige_ai_reml_config;
+
+typedef struct {
+  double tau[2];
+  uint32_t iterations;
+  uint8_t converged;
+  double log_likelihood;
+  /* diagnostics (no reference counterpart) */
+  uint64_t grm_products;      /* GRM column-products performed */
+  uint64_t pcg_iterations;    /* summed over right-hand sides */
+  uint32_t pcg_not_converged; /* solves that hit max_iter or broke down */
+} saige_ai_reml_result;
+
+int saige_fit_ai_reml(saige_grm* h, const double* y, const double* x, size_t n, size_t p,
+                      int trait_type, const saige_ai_reml_config* cfg,
+                      saige_ai_reml_result* result, double* mu, double* w, double* residuals,
+                      double* eta, double* alpha);
+
+/* ------------------------------------------------------------------------------
+ * (X'WX)^{-1} X'W as produced by fit_null.rs:289-307 (a p*n flat buffer filled
+ * element (j, i) -> [j*n + i], which every consumer reads as col-major p x n,
+ * fit_null.rs:307 and assoc_test.rs:125; for p > 1 this is the reference's layout,
+ * reproduced as is).
+ * ---------------------------------------------------------------------------- */
+int saige_xvx_inv_xv(const double* x, size_t n, size_t p, const double* w, double* out);
+
+/* ------------------------------------------------------------------------------
+ * Variance ratio.
+ *   Replaces estimate_variance_ratio (saige-core/src/glmm/variance_ratio.rs:87-246) with
+ *   VarianceRatioConfig (:30-59, overall ratio; categorical bins not built) and the
+ *   Sigma^{-1} PCG closure of fit_null.rs:320-343.  dosages: n x m col-major f64.
+ * ---------------------------------------------------------------------------- */
+typedef struct {
+  uint32_t n_markers;        /* default 30 */
+  double min_mac;            /* default 20 */
+  double ratio_cv_cutoff;    /* default 1e-3 */
+  uint64_t seed;             /* default 12345 */
+} saige_vr_config;
+
+int saige_estimate_variance_ratio(saige_grm* h, const double* dosages, const double* mac,
+                                  size_t m, const double* mu, const double* w,
+                                  const double tau[2], int trait_type, const double* x,
+                                  const double* xvx_inv_xv, size_t p,
+                                  const saige_vr_config* cfg, double* variance_ratio,
+                                  uint32_t* n_used, double* per_marker_vr);
+
+/* ------------------------------------------------------------------------------
+ * Whole step 1: saige-cli fit_null::run (saige-cli/src/commands/fit_null.rs:199-394)
+ * after phenotype parsing: marker QC (min_maf, max_missing), the ChaCha8 VR/GRM split
+ * (:230-259), GRM build, AI-REML, XVX^{-1}X'V and the variance ratio.  y and x are in
+ * model sample order (the sample_idx order).
+ * ---------------------------------------------------------------------------- */
+typedef struct {
+  saige_ai_reml_config reml;
+  double min_maf;            /* 0.01 */
+  double max_missing_rate;   /* 0.15 */
+  uint32_t n_markers_vr;     /* 30 */
+  int precision;             /* SAIGE_PREC_FP64 | SAIGE_PREC_MIXED */
+} saige_fit_null_config;
+
+typedef struct {
+  saige_ai_reml_result reml;
+  double variance_ratio;
+  uint32_t vr_markers_used;
+  uint32_t n_grm_markers;
+  uint32_t n_vr_markers;
+  double seconds_qc, seconds_reml, seconds_vr, seconds_total;
+} saige_fit_null_result;
+
+int saige_fit_null(int device, const uint8_t* bed_body, size_t bytes_per_marker, size_t n_total,
+                   size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
+                   const double* x, size_t p, int trait_type, const saige_fit_null_config* cfg,
+                   saige_fit_null_result* result, double* mu, double* xvx_inv_xv,
+                   uint8_t* marker_class);
+
+/* ------------------------------------------------------------------------------
+ * Step 2: score test engine.
+ *   Replaces ScoreTestEngine (saige-core/src/score_test/single_variant.rs:77-108),
+ *   test_marker (:112-257) with compute_case_control_af (:260-291), and the fast SPA
+ *   spa_binary_fast (saige-core/src/spa/fast.rs:207-242; spa_binary when use_fast_spa
+ *   is 0 gives the same result, see DESIGN).  x: n x p col-major; xvx_inv_xv: the p*n
+ *   flat buffer of the null model (NullModel::xvx_inv_xv_flat, null_model.rs:46-47);
+ *   categorical ratios are (upper MAC bound, ratio) pairs (variance_ratio.rs:289-301);
+ *   y may be NULL (no case/control AF).
+ * ---------------------------------------------------------------------------- */
+typedef struct saige_engine saige_engine;
+
+typedef struct {
+  double ac, af, mac;
+  uint64_t n;
+  double score, var_t, var_t_star, tstat, pvalue, pvalue_noadj;
+  int32_t is_spa, spa_converged;
+  double beta, se_beta, variance_ratio, af_cases, af_controls;
+  uint64_t n_cases, n_controls;
+  int32_t spa_iters1, spa_iters2;  /* Newton iterations of the two roots (diagnostic) */
+} saige_variant_result;            /* numeric fields of SingleVariantResult (:23-74) */
+
+int saige_engine_create(int device, const double* mu, const double* mu2,
+                        const double* residuals, double tau_e, double tau_g,
+                        const double* xvx_inv_xv, const double* x, size_t n, size_t p,
+                        double variance_ratio, const double* cat_upper, const double* cat_vr,
+                        size_t n_cat, int use_spa, int use_fast_spa, double spa_tol,
+                        double spa_pval_cutoff, const double* y, saige_engine** out);
+void saige_engine_destroy(saige_engine* e);
+void* saige_engine_stream(saige_engine* e);
+
+/* test_marker over nvar dosage vectors g (n x nvar col-major f64, NaN = missing). */
+int saige_test_markers_f64(saige_engine* e, const double* g, size_t ldg, size_t nvar,
+                           saige_variant_result* out);
+/* uint8-quantized dosages (dosage = q * scale), variant-major rows of ldq bytes. */
+int saige_test_markers_u8(saige_engine* e, const uint8_t* q, size_t ldq, size_t nvar,
+                          double scale, saige_variant_result* out);
+int saige_test_markers_u8_device(saige_engine* e, const uint8_t* d_q, size_t ldq, size_t nvar,
+                                 double scale, saige_variant_result* d_out);
+/* Hard calls straight from a .bed body (PlinkReader::read_marker + test_marker). */
+int saige_test_markers_plink(saige_engine* e, const uint8_t* bed_body, size_t bytes_per_marker,
+                             size_t n_total, const uint32_t* sample_idx, size_t nvar,
+                             saige_variant_result* out);
+
+/* ------------------------------------------------------------------------------
+ * SPA on one covariate-adjusted genotype vector.
+ *   Replaces spa_binary_fast (fast.rs:207-242) / spa_binary (binary.rs:194-228).
+ * ---------------------------------------------------------------------------- */
+typedef struct {
+  double pvalue;
+  int32_t converged, is_spa, iters1, iters2;
+  double root1, root2;
+} saige_spa_result;
+
+int saige_spa_binary(int device, const double* mu, const double* g, size_t n, double q,
+                     double qinv, double pval_noadj, double tol, int fast,
+                     saige_spa_result* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SAIGE_GPU_H */
