@@ -1,0 +1,321 @@
+"""csaige_amd — MI355X (gfx950) implementation of saige-rs's fit-null and step-2 hot paths.
+
+This module is the Python host mirror of the saige-rs API for those paths, over the C ABI
+of ``include/saige_gpu.h`` (libsaige_gpu.so):
+
+    OnTheFlyGrm          saige-core/src/glmm/pcg.rs:91-181        (packed genotypes in HBM)
+    solve_sigma          saige-linalg/src/decomposition.rs:212-298 with ai_reml.rs:142-164
+    fit_ai_reml          saige-core/src/glmm/ai_reml.rs:86-427
+    xvx_inv_xv           saige-cli/src/commands/fit_null.rs:289-307
+    estimate_variance_ratio  saige-core/src/glmm/variance_ratio.rs:87-246
+    fit_null             saige-cli/src/commands/fit_null.rs:199-394
+    ScoreTestEngine      saige-core/src/score_test/single_variant.rs:77-257
+    spa_binary_fast      saige-core/src/spa/fast.rs:207-242
+
+Every call runs on the GPU; a missing library raises ImportError (no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (PREC_FP64, PREC_MIXED, TRAIT_BINARY, TRAIT_QUANTITATIVE, TRAIT_SURVIVAL,
+                   SaigeError, check, load)
+
+__all__ = ["OnTheFlyGrm", "fit_ai_reml", "solve_sigma", "xvx_inv_xv", "estimate_variance_ratio",
+           "fit_null", "ScoreTestEngine", "spa_binary_fast", "marker_stats", "decode",
+           "PREC_FP64", "PREC_MIXED", "TRAIT_BINARY", "TRAIT_QUANTITATIVE", "TRAIT_SURVIVAL",
+           "SaigeError", "load", "results_to_dict"]
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def _idx(sample_idx):
+    if sample_idx is None:
+        return None, None
+    si = np.ascontiguousarray(sample_idx, dtype=np.uint32)
+    return si, _p(si, _lib.u32p)
+
+
+def marker_stats(bed_body, bytes_per_marker, n_total, m, sample_idx=None, device=0):
+    """read_marker + compute_af (plink.rs:237-267, traits.rs:42-54) for m markers.
+    Returns (ac, n_valid, af, mac)."""
+    L = load()
+    body = _u8(bed_body)
+    si, sp = _idx(sample_idx)
+    n = n_total if si is None else len(si)
+    ac, af, mac = np.empty(m), np.empty(m), np.empty(m)
+    nv = np.empty(m, np.uint32)
+    check(L.saige_plink_marker_stats(device, _p(body, _lib.u8p), bytes_per_marker, n_total, m, sp,
+                                     n, _p(ac, _lib.dp), _p(nv, _lib.u32p), _p(af, _lib.dp),
+                                     _p(mac, _lib.dp)))
+    return ac, nv, af, mac
+
+
+def decode(bed_body, bytes_per_marker, n_total, m, sample_idx=None, device=0):
+    """Dosages (m, n) with NaN for missing (plink.rs:170-267)."""
+    L = load()
+    body = _u8(bed_body)
+    si, sp = _idx(sample_idx)
+    n = n_total if si is None else len(si)
+    out = np.empty((m, n))
+    check(L.saige_plink_decode(device, _p(body, _lib.u8p), bytes_per_marker, n_total, m, sp, n,
+                               _p(out, _lib.dp)))
+    return out
+
+
+class OnTheFlyGrm:
+    """OnTheFlyGrm (pcg.rs:91-181) on packed genotypes resident in HBM.
+
+    ``bed_body``: SNP-major .bed body without magic; ``marker_idx``: markers used (file
+    indices, or None for all m rows); ``af``: compute_af of each used marker over the
+    sample subset.
+    """
+
+    def __init__(self, bed_body, bytes_per_marker, n_total, af, sample_idx=None, marker_idx=None,
+                 precision=PREC_FP64, device=0):
+        L = load()
+        body = _u8(bed_body)
+        si, sp = _idx(sample_idx)
+        self.n = n_total if si is None else len(si)
+        af = _f64(af)
+        m = len(af)
+        mi = None if marker_idx is None else np.ascontiguousarray(marker_idx, dtype=np.uint32)
+        h = C.c_void_p()
+        check(L.saige_grm_create(device, _p(body, _lib.u8p), bytes_per_marker, n_total, sp,
+                                 self.n, None if mi is None else _p(mi, _lib.u32p), m,
+                                 _p(af, _lib.dp), precision, C.byref(h)))
+        self.h = h
+        self.m = m
+        self.precision = precision
+
+    def mat_vec(self, v):
+        v = _f64(v)
+        assert v.shape == (self.n,)
+        out = np.empty(self.n)
+        check(load().saige_grm_mat_vec(self.h, _p(v, _lib.dp), _p(out, _lib.dp)))
+        return out
+
+    def mat_mat(self, V):
+        """V: (k, n) array of k vectors; returns (k, n)."""
+        V = _f64(V)
+        k = V.shape[0]
+        out = np.empty((k, self.n))
+        check(load().saige_grm_mat_mat(self.h, _p(V, _lib.dp), self.n, k, _p(out, _lib.dp),
+                                       self.n))
+        return out
+
+    def mat_mat_device(self, d_v, ldv, k, d_out, ldo):
+        check(load().saige_grm_mat_mat_device(self.h, d_v, ldv, k, d_out, ldo))
+
+    @property
+    def stream(self):
+        return load().saige_grm_stream(self.h)
+
+    def synchronize(self):
+        check(load().saige_grm_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().saige_grm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_sigma(grm, w, tau, B, tol=1e-5, max_iter=500):
+    """PcgSolver::solve on Sigma = tau0 diag(1/W) + tau1 GRM for k right-hand sides.
+    B: (k, n).  Returns (X (k, n), iterations, residuals, converged)."""
+    B = _f64(np.atleast_2d(B))
+    k, n = B.shape
+    w = _f64(w)
+    t = _f64(tau)
+    X = np.empty((k, n))
+    info = (_lib.PcgInfo * k)()
+    check(load().saige_pcg_solve_sigma(grm.h, _p(w, _lib.dp), _p(t, _lib.dp), _p(B, _lib.dp), n, k,
+                                       tol, max_iter, _p(X, _lib.dp), n, info))
+    return (X, np.array([i.iterations for i in info]), np.array([i.residual for i in info]),
+            np.array([bool(i.converged) for i in info]))
+
+
+def fit_ai_reml(grm, y, x_cm, trait=TRAIT_BINARY, max_iter=30, tol=0.02, pcg_tol=1e-5,
+                pcg_max_iter=500, n_random_vectors=30, seed=12345):
+    """fit_ai_reml (ai_reml.rs:86-427).  x_cm: (p, n), row j = covariate column j."""
+    y = _f64(y)
+    x = _f64(x_cm)
+    p, n = x.shape
+    cfg = _lib.AiRemlConfig(max_iter, tol, pcg_tol, pcg_max_iter, n_random_vectors, seed)
+    r = _lib.AiRemlResult()
+    mu, w, res, eta = (np.empty(n) for _ in range(4))
+    alpha = np.empty(p)
+    check(load().saige_fit_ai_reml(grm.h, _p(y, _lib.dp), _p(x, _lib.dp), n, p, trait,
+                                   C.byref(cfg), C.byref(r), _p(mu, _lib.dp), _p(w, _lib.dp),
+                                   _p(res, _lib.dp), _p(eta, _lib.dp), _p(alpha, _lib.dp)))
+    return dict(tau=np.array(r.tau[:]), iterations=r.iterations, converged=bool(r.converged),
+                log_likelihood=r.log_likelihood, grm_products=r.grm_products,
+                pcg_iterations=r.pcg_iterations, pcg_not_converged=r.pcg_not_converged, mu=mu,
+                w=w, residuals=res, eta=eta, alpha=alpha)
+
+
+def xvx_inv_xv(x_cm, w):
+    """fit_null.rs:289-307 (flat buffer as the reference fills it)."""
+    x = _f64(x_cm)
+    p, n = x.shape
+    w = _f64(w)
+    out = np.empty(p * n)
+    check(load().saige_xvx_inv_xv(_p(x, _lib.dp), n, p, _p(w, _lib.dp), _p(out, _lib.dp)))
+    return out
+
+
+def estimate_variance_ratio(grm, dosages, macs, mu, w, tau, x_cm, a_flat, trait=TRAIT_BINARY,
+                            n_markers=30, min_mac=20.0, cv_cutoff=1e-3, seed=12345):
+    """estimate_variance_ratio (variance_ratio.rs:87-246).  dosages: (m, n)."""
+    d = _f64(dosages)
+    m, n = d.shape
+    x = _f64(x_cm)
+    p = x.shape[0]
+    macs, mu, w, a, t = _f64(macs), _f64(mu), _f64(w), _f64(a_flat), _f64(tau)
+    cfg = _lib.VrConfig(n_markers, min_mac, cv_cutoff, seed)
+    vr = C.c_double()
+    used = C.c_uint32()
+    per = np.empty(max(m, 1))
+    check(load().saige_estimate_variance_ratio(grm.h, _p(d, _lib.dp), _p(macs, _lib.dp), m,
+                                               _p(mu, _lib.dp), _p(w, _lib.dp), _p(t, _lib.dp),
+                                               trait, _p(x, _lib.dp), _p(a, _lib.dp), p,
+                                               C.byref(cfg), C.byref(vr), C.byref(used),
+                                               _p(per, _lib.dp)))
+    return vr.value, used.value, per[:used.value]
+
+
+def fit_null(bed_body, bytes_per_marker, n_total, m_total, y, x_cm, sample_idx=None,
+             trait=TRAIT_BINARY, precision=PREC_FP64, max_iter=30, tol=1e-5, seed=12345,
+             n_random_vectors=30, min_maf=0.01, max_missing=0.15, n_markers_vr=30, device=0):
+    """fit_null::run after phenotype parsing (fit_null.rs:199-394)."""
+    L = load()
+    body = _u8(bed_body)
+    si, sp = _idx(sample_idx)
+    n = n_total if si is None else len(si)
+    y = _f64(y)
+    x = _f64(x_cm)
+    p = x.shape[0]
+    cfg = _lib.FitNullConfig(_lib.AiRemlConfig(max_iter, tol, 1e-5, 500, n_random_vectors, seed),
+                             min_maf, max_missing, n_markers_vr, precision)
+    r = _lib.FitNullResult()
+    mu = np.empty(n)
+    a = np.empty(p * n)
+    cls = np.empty(m_total, np.uint8)
+    check(L.saige_fit_null(device, _p(body, _lib.u8p), bytes_per_marker, n_total, m_total, sp, n,
+                           _p(y, _lib.dp), _p(x, _lib.dp), p, trait, C.byref(cfg), C.byref(r),
+                           _p(mu, _lib.dp), _p(a, _lib.dp), _p(cls, _lib.u8p)))
+    rr = r.reml
+    return dict(tau=np.array(rr.tau[:]), iterations=rr.iterations, converged=bool(rr.converged),
+                log_likelihood=rr.log_likelihood, grm_products=rr.grm_products,
+                pcg_iterations=rr.pcg_iterations, pcg_not_converged=rr.pcg_not_converged,
+                vr=r.variance_ratio, vr_markers_used=r.vr_markers_used,
+                n_grm_markers=r.n_grm_markers, n_vr_markers=r.n_vr_markers, mu=mu,
+                xvx_inv_xv=a, marker_class=cls, seconds_qc=r.seconds_qc,
+                seconds_reml=r.seconds_reml, seconds_vr=r.seconds_vr,
+                seconds_total=r.seconds_total)
+
+
+def results_to_dict(arr):
+    names = [f for f, _ in _lib.VariantResult._fields_]
+    return {f: np.array([getattr(r, f) for r in arr]) for f in names}
+
+
+class ScoreTestEngine:
+    """ScoreTestEngine (single_variant.rs:77-108) with test_marker over batches."""
+
+    def __init__(self, mu, residuals, tau, x_cm, a_flat, variance_ratio, y=None, use_spa=True,
+                 use_fast_spa=True, spa_tol=1e-6, spa_pval_cutoff=0.05, categorical_vr=(),
+                 mu2=None, device=0):
+        mu = _f64(mu)
+        self.n = len(mu)
+        mu2 = _f64(mu * (1.0 - mu) if mu2 is None else mu2)
+        res = _f64(residuals)
+        x = _f64(x_cm)
+        p = x.shape[0]
+        a = _f64(a_flat)
+        yy = None if y is None else _f64(y)
+        cu = _f64([c[0] for c in categorical_vr] or [0.0])
+        cv = _f64([c[1] for c in categorical_vr] or [0.0])
+        h = C.c_void_p()
+        check(load().saige_engine_create(device, _p(mu, _lib.dp), _p(mu2, _lib.dp),
+                                         _p(res, _lib.dp), tau[0], tau[1], _p(a, _lib.dp),
+                                         _p(x, _lib.dp), self.n, p, variance_ratio,
+                                         _p(cu, _lib.dp), _p(cv, _lib.dp), len(categorical_vr),
+                                         int(use_spa), int(use_fast_spa), spa_tol,
+                                         spa_pval_cutoff,
+                                         None if yy is None else _p(yy, _lib.dp), C.byref(h)))
+        self.h = h
+
+    def test_markers(self, G):
+        """G: (nvar, n) f64 dosages (NaN = missing)."""
+        G = _f64(np.atleast_2d(G))
+        nvar = G.shape[0]
+        out = (_lib.VariantResult * nvar)()
+        check(load().saige_test_markers_f64(self.h, _p(G, _lib.dp), self.n, nvar, out))
+        return results_to_dict(out)
+
+    def test_marker(self, g):
+        r = self.test_markers(np.asarray(g)[None, :])
+        return {k: v[0] for k, v in r.items()}
+
+    def test_markers_u8(self, Q, scale=1.0 / 127.0):
+        Q = _u8(np.atleast_2d(Q))
+        nvar, ld = Q.shape
+        out = (_lib.VariantResult * nvar)()
+        check(load().saige_test_markers_u8(self.h, _p(Q, _lib.u8p), ld, nvar, scale, out))
+        return results_to_dict(out)
+
+    def test_markers_plink(self, bed_body, bytes_per_marker, n_total, nvar, sample_idx=None):
+        body = _u8(bed_body)
+        si, sp = _idx(sample_idx)
+        out = (_lib.VariantResult * nvar)()
+        check(load().saige_test_markers_plink(self.h, _p(body, _lib.u8p), bytes_per_marker,
+                                              n_total, sp, nvar, out))
+        return results_to_dict(out)
+
+    def test_markers_u8_device(self, d_q, ldq, nvar, scale, d_out):
+        check(load().saige_test_markers_u8_device(self.h, d_q, ldq, nvar, scale, d_out))
+
+    @property
+    def stream(self):
+        return load().saige_engine_stream(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().saige_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spa_binary_fast(mu, g, q, qinv, pval_noadj, tol=1e-6, fast=True, device=0):
+    """spa_binary_fast (fast.rs:207-242); fast=False gives spa_binary (binary.rs:194-228)."""
+    mu, g = _f64(mu), _f64(g)
+    r = _lib.SpaResult()
+    check(load().saige_spa_binary(device, _p(mu, _lib.dp), _p(g, _lib.dp), len(mu), q, qinv,
+                                  pval_noadj, tol, int(fast), C.byref(r)))
+    return dict(pvalue=r.pvalue, converged=bool(r.converged), is_spa=bool(r.is_spa),
+                iters1=r.iters1, iters2=r.iters2, root1=r.root1, root2=r.root2)

@@ -1,0 +1,297 @@
+// Batched preconditioned conjugate gradient on Sigma = tau0 * diag(1/W) + tau1 * GRM.
+//
+// Reference: PcgSolver::solve (saige-linalg/src/decomposition.rs:212-298) with the Sigma
+// operator and diagonal preconditioner closures of saige-core/src/glmm/ai_reml.rs:142-164.
+// k right-hand sides share one multi-column GRM product per iteration; every column keeps
+// its own rz, alpha, beta, residual norm and stop flag, so each column follows exactly the
+// iteration of a separate PcgSolver::solve:
+//   x0 = 0 (the product A*x0 is exactly zero and is not formed), r = b, z = M^-1 r, p = z;
+//   loop: stop if ||r|| < tol * max(||b||, 1); Ap; breakdown if |p'Ap| < 1e-30;
+//         alpha = rz / p'Ap; x += alpha p; r -= alpha Ap; z = M^-1 r; beta = rz'/rz; p = z + beta p.
+// Dot products and norms are fp64 with a deterministic two-stage reduction.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace saige {
+
+struct ColState {
+  double rz, tol_abs, alpha, beta, rnorm;
+  int active, iters, converged, pad;
+};
+
+__device__ __forceinline__ double precond_apply(double r, double w, double tau0, double tau1) {
+  double diag = tau0 / fmax(w, 1e-30) + tau1;
+  return fabs(diag) > 1e-30 ? r / diag : r;
+}
+
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
+  __shared__ double sa[kDotBlock], sb[kDotBlock], sc[kDotBlock];
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = kDotBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sa[threadIdx.x] += sa[threadIdx.x + w];
+      sb[threadIdx.x] += sb[threadIdx.x + w];
+      sc[threadIdx.x] += sc[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  a = sa[0];
+  b = sb[0];
+  c = sc[0];
+}
+
+// r = b, z = M^-1 r, p = z, x = 0; partial sums of r'z, b'b, r'r.
+__global__ __launch_bounds__(kDotBlock) void pcg_init_kernel(
+    const double* __restrict__ b, size_t ldb, double* __restrict__ x, size_t ldx,
+    double* __restrict__ r, double* __restrict__ z, double* __restrict__ p, size_t ld,
+    const double* __restrict__ w, double tau0, double tau1, size_t n, size_t nchunks,
+    double* __restrict__ part) {
+  const size_t c = blockIdx.y;
+  const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, n);
+  double s_rz = 0.0, s_bb = 0.0, s_rr = 0.0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) {
+    double bi = b[i + c * ldb];
+    double zi = precond_apply(bi, w[i], tau0, tau1);
+    r[i + c * ld] = bi;
+    z[i + c * ld] = zi;
+    p[i + c * ld] = zi;
+    x[i + c * ldx] = 0.0;
+    s_rz = fma(bi, zi, s_rz);
+    s_bb = fma(bi, bi, s_bb);
+  }
+  s_rr = s_bb;
+  block_sum3(s_rz, s_bb, s_rr);
+  if (threadIdx.x == 0) {
+    size_t k = gridDim.y;
+    part[(0 * k + c) * nchunks + blockIdx.x] = s_rz;
+    part[(1 * k + c) * nchunks + blockIdx.x] = s_bb;
+    part[(2 * k + c) * nchunks + blockIdx.x] = s_rr;
+  }
+}
+
+__global__ void pcg_init_fin(const double* __restrict__ part, size_t nchunks, size_t k,
+                             double tol, ColState* __restrict__ st) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double rz = 0.0, bb = 0.0, rr = 0.0;
+  for (size_t q = 0; q < nchunks; ++q) {
+    rz += part[(0 * k + c) * nchunks + q];
+    bb += part[(1 * k + c) * nchunks + q];
+    rr += part[(2 * k + c) * nchunks + q];
+  }
+  ColState s;
+  s.rz = rz;
+  s.tol_abs = tol * fmax(sqrt(bb), 1.0);
+  s.alpha = 0.0;
+  s.beta = 0.0;
+  s.rnorm = sqrt(rr);
+  s.iters = 0;
+  s.pad = 0;
+  if (s.rnorm < s.tol_abs) {
+    s.active = 0;
+    s.converged = 1;
+  } else {
+    s.active = 1;
+    s.converged = 0;
+  }
+  st[c] = s;
+}
+
+// partial p'Ap for active columns
+__global__ __launch_bounds__(kDotBlock) void pcg_pap_kernel(const double* __restrict__ p,
+                                                            const double* __restrict__ ap,
+                                                            size_t ld, size_t n, size_t nchunks,
+                                                            const ColState* __restrict__ st,
+                                                            double* __restrict__ part) {
+  const size_t c = blockIdx.y;
+  if (!st[c].active) return;
+  const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, n);
+  double s = 0.0, d1 = 0.0, d2 = 0.0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) s = fma(p[i + c * ld], ap[i + c * ld], s);
+  block_sum3(s, d1, d2);
+  if (threadIdx.x == 0) part[c * nchunks + blockIdx.x] = s;
+}
+
+__global__ void pcg_alpha_fin(const double* __restrict__ part, size_t nchunks, size_t k,
+                              int iter, ColState* __restrict__ st) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k || !st[c].active) return;
+  double pap = 0.0;
+  for (size_t q = 0; q < nchunks; ++q) pap += part[c * nchunks + q];
+  if (fabs(pap) < 1e-30) {  // breakdown: decomposition.rs:257-264
+    st[c].active = 0;
+    st[c].converged = 0;
+    st[c].iters = iter;
+    return;
+  }
+  st[c].alpha = st[c].rz / pap;
+}
+
+// x += alpha p; r -= alpha Ap; z = M^-1 r; partial r'z and r'r.
+__global__ __launch_bounds__(kDotBlock) void pcg_update_kernel(
+    double* __restrict__ x, size_t ldx, double* __restrict__ r, double* __restrict__ z,
+    const double* __restrict__ p, const double* __restrict__ ap, size_t ld,
+    const double* __restrict__ w, double tau0, double tau1, size_t n, size_t nchunks,
+    const ColState* __restrict__ st, double* __restrict__ part) {
+  const size_t c = blockIdx.y;
+  if (!st[c].active) return;
+  const double alpha = st[c].alpha;
+  const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, n);
+  double s_rz = 0.0, s_rr = 0.0, d = 0.0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) {
+    double pi = p[i + c * ld];
+    x[i + c * ldx] += alpha * pi;
+    double ri = r[i + c * ld] - alpha * ap[i + c * ld];
+    r[i + c * ld] = ri;
+    double zi = precond_apply(ri, w[i], tau0, tau1);
+    z[i + c * ld] = zi;
+    s_rz = fma(ri, zi, s_rz);
+    s_rr = fma(ri, ri, s_rr);
+  }
+  block_sum3(s_rz, s_rr, d);
+  if (threadIdx.x == 0) {
+    size_t k = gridDim.y;
+    part[(0 * k + c) * nchunks + blockIdx.x] = s_rz;
+    part[(1 * k + c) * nchunks + blockIdx.x] = s_rr;
+  }
+}
+
+__global__ void pcg_beta_fin(const double* __restrict__ part, size_t nchunks, size_t k, int iter,
+                             ColState* __restrict__ st, int* __restrict__ n_active) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  if (st[c].active) {
+    double rz = 0.0, rr = 0.0;
+    for (size_t q = 0; q < nchunks; ++q) {
+      rz += part[(0 * k + c) * nchunks + q];
+      rr += part[(1 * k + c) * nchunks + q];
+    }
+    st[c].beta = rz / st[c].rz;
+    st[c].rz = rz;
+    st[c].rnorm = sqrt(rr);
+    if (st[c].rnorm < st[c].tol_abs) {  // next top-of-loop test, decomposition.rs:242-250
+      st[c].active = 0;
+      st[c].converged = 1;
+      st[c].iters = iter + 1;
+    }
+  }
+  // count (single block of <= 1024 threads handles all columns)
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  if (c < k && st[c].active) atomicAdd(&cnt, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) *n_active = cnt;
+}
+
+__global__ void pcg_p_kernel(double* __restrict__ p, const double* __restrict__ z, size_t ld,
+                             size_t n, const ColState* __restrict__ st) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t c = blockIdx.y;
+  if (i >= n || !st[c].active) return;
+  p[i + c * ld] = z[i + c * ld] + st[c].beta * p[i + c * ld];
+}
+
+__global__ void pcg_final(size_t k, uint32_t max_iter, ColState* __restrict__ st) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k || !st[c].active) return;
+  st[c].iters = (int)max_iter;
+  st[c].converged = st[c].rnorm < st[c].tol_abs;
+  st[c].active = 0;
+}
+
+int pcg_sigma_device(saige_grm* h, const double* d_w, const double tau[2], const double* d_b,
+                     size_t ldb, size_t k, double tol, uint32_t max_iter, double* d_x, size_t ldx,
+                     saige_pcg_info* info) {
+  if (k == 0) return SAIGE_OK;
+  SAIGE_ARG(k <= 1024, "k <= 1024 right-hand sides per solve");
+  const size_t n = h->g.n;
+  const size_t ld = round_up(n, 64);
+  hipStream_t s = h->stream;
+  const size_t nchunks = std::max<size_t>(1, ceil_div(n, kDotChunk));
+  SAIGE_TRY(h->pcg_vec.ensure(4 * ld * k * 8));
+  SAIGE_TRY(h->pcg_part.ensure(3 * k * nchunks * 8));
+  SAIGE_TRY(h->pcg_scal.ensure(k * sizeof(ColState) + 64));
+  double* R = h->pcg_vec.as<double>();
+  double* Z = R + ld * k;
+  double* P = Z + ld * k;
+  double* AP = P + ld * k;
+  ColState* st = reinterpret_cast<ColState*>(h->pcg_scal.as<char>() + 64);
+  int* d_nact = h->pcg_scal.as<int>();
+  double* part = h->pcg_part.as<double>();
+  const double tau0 = tau[0], tau1 = tau[1];
+  const unsigned kb = (unsigned)ceil_div(k, 256);
+  dim3 gchunk((unsigned)nchunks, (unsigned)k);
+
+  pcg_init_kernel<<<gchunk, kDotBlock, 0, s>>>(d_b, ldb, d_x, ldx, R, Z, P, ld, d_w, tau0, tau1, n,
+                                               nchunks, part);
+  pcg_init_fin<<<kb, 256, 0, s>>>(part, nchunks, k, tol, st);
+  SAIGE_HIP(hipGetLastError());
+  int n_active = 0;
+  {
+    std::vector<ColState> hs(k);
+    SAIGE_HIP(hipMemcpyAsync(hs.data(), st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipStreamSynchronize(s));
+    for (size_t c = 0; c < k; ++c) n_active += hs[c].active;
+  }
+  int pinned_nact = 0;
+  for (uint32_t iter = 0; iter < max_iter && n_active > 0; ++iter) {
+    SAIGE_TRY(grm_product(h, P, ld, k, AP, ld, d_w, tau));
+    pcg_pap_kernel<<<gchunk, kDotBlock, 0, s>>>(P, AP, ld, n, nchunks, st, part);
+    pcg_alpha_fin<<<kb, 256, 0, s>>>(part, nchunks, k, (int)iter, st);
+    pcg_update_kernel<<<gchunk, kDotBlock, 0, s>>>(d_x, ldx, R, Z, P, AP, ld, d_w, tau0, tau1, n,
+                                                   nchunks, st, part);
+    pcg_beta_fin<<<1, 1024, 0, s>>>(part, nchunks, k, (int)iter, st, d_nact);
+    pcg_p_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(P, Z, ld, n, st);
+    SAIGE_HIP(hipGetLastError());
+    SAIGE_HIP(hipMemcpyAsync(&pinned_nact, d_nact, sizeof(int), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipStreamSynchronize(s));
+    n_active = pinned_nact;
+  }
+  pcg_final<<<kb, 256, 0, s>>>(k, max_iter, st);
+  SAIGE_HIP(hipGetLastError());
+  if (info) {
+    std::vector<ColState> hs(k);
+    SAIGE_HIP(hipMemcpyAsync(hs.data(), st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipStreamSynchronize(s));
+    for (size_t c = 0; c < k; ++c) {
+      info[c].iterations = (uint32_t)hs[c].iters;
+      info[c].residual = hs[c].rnorm;
+      info[c].converged = (uint8_t)hs[c].converged;
+    }
+  }
+  return SAIGE_OK;
+}
+
+}  // namespace saige
+
+using namespace saige;
+
+extern "C" int saige_pcg_solve_sigma(saige_grm* h, const double* w, const double tau[2],
+                                     const double* b, size_t ldb, size_t k, double tol,
+                                     uint32_t max_iter, double* x, size_t ldx,
+                                     saige_pcg_info* info) {
+  clear_error();
+  SAIGE_ARG(h && w && tau && (k == 0 || (b && x)), "null pointer");
+  SAIGE_ARG(ldb >= h->g.n && ldx >= h->g.n, "leading dimension < n");
+  if (k == 0) return SAIGE_OK;
+  std::lock_guard<std::recursive_mutex> lk(h->mu);
+  SAIGE_HIP(hipSetDevice(h->device));
+  const size_t n = h->g.n;
+  SAIGE_TRY(h->pcg_w.ensure(n * 8));
+  SAIGE_TRY(h->pcg_b.ensure(2 * n * k * 8));
+  double* dw = h->pcg_w.as<double>();
+  double* db = h->pcg_b.as<double>();
+  double* dx = db + n * k;
+  SAIGE_HIP(hipMemcpyAsync(dw, w, n * 8, hipMemcpyHostToDevice, h->stream));
+  SAIGE_HIP(hipMemcpy2DAsync(db, n * 8, b, ldb * 8, n * 8, k, hipMemcpyHostToDevice, h->stream));
+  SAIGE_TRY(pcg_sigma_device(h, dw, tau, db, n, k, tol, max_iter, dx, n, info));
+  SAIGE_HIP(hipMemcpy2DAsync(x, ldx * 8, dx, n * 8, n * 8, k, hipMemcpyDeviceToHost, h->stream));
+  SAIGE_HIP(hipStreamSynchronize(h->stream));
+  return SAIGE_OK;
+}

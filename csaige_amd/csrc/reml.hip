@@ -1,0 +1,818 @@
+// Fit-null on the GPU: AI-REML (ai_reml.rs:86-427), (X'WX)^-1 X'W (fit_null.rs:289-307),
+// the variance ratio (variance_ratio.rs:87-246) and the whole fit_null::run flow
+// (saige-cli/src/commands/fit_null.rs:199-394).
+//
+// All n-length work (Sigma solves, GRM products, projections, dot products) runs on the
+// device; the host keeps only the p x p algebra (Cholesky of X' Sigma^-1 X, its solves)
+// and the scalar tau update, exactly as the reference does them.
+//
+// Batching (iterate-identical to the reference): per AI-REML iteration the reference
+// runs 1 + p + R separate PCG solves on the same Sigma (y~, the p columns of X, the R
+// Hutchinson probes; ai_reml.rs:175-187, 237-258); here they are one block solve with
+// k = 1 + p + R columns.  GRM * u of every probe does not change between iterations
+// (ai_reml.rs:254 recomputes it) and is formed once.  The second-stage solves
+// Sigma^-1 (GRM P y) (and, for quantitative traits, Sigma^-1 (P y / W)) form one block of
+// 1 or 2 columns.
+#include "common.hpp"
+#include "rng.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+namespace saige {
+
+int upload_repack(hipStream_t s, const uint8_t* bed_body, size_t bpm, size_t n_total,
+                  const uint32_t* d_idx, size_t n, const uint32_t* marker_idx, size_t m,
+                  uint8_t* d_dst, size_t row_bytes);
+int upload_sample_idx(hipStream_t s, const uint32_t* sample_idx, size_t n, size_t n_total,
+                      DevBuf& d);
+
+// ---------------------------------------------------------------------------------
+// p x p host algebra: CholeskyDecomp (saige-linalg/src/decomposition.rs:35-102).
+// ---------------------------------------------------------------------------------
+static bool chol(const std::vector<double>& a, size_t n, std::vector<double>& l) {
+  l.assign(n * n, 0.0);
+  for (size_t j = 0; j < n; ++j) {
+    double sum = 0.0;
+    for (size_t k = 0; k < j; ++k) sum += l[k * n + j] * l[k * n + j];
+    double d = a[j * n + j] - sum;
+    if (d <= 0.0) return false;
+    l[j * n + j] = std::sqrt(d);
+    for (size_t i = j + 1; i < n; ++i) {
+      double s = 0.0;
+      for (size_t k = 0; k < j; ++k) s += l[k * n + i] * l[k * n + j];
+      l[j * n + i] = (a[j * n + i] - s) / l[j * n + j];
+    }
+  }
+  return true;
+}
+static void chol_solve(const std::vector<double>& l, size_t n, const double* b, double* x) {
+  std::vector<double> y(n);
+  for (size_t i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (size_t j = 0; j < i; ++j) s += l[j * n + i] * y[j];
+    y[i] = (b[i] - s) / l[i * n + i];
+  }
+  for (size_t i = n; i-- > 0;) {
+    double s = 0.0;
+    for (size_t j = i + 1; j < n; ++j) s += l[i * n + j] * x[j];
+    x[i] = (y[i] - s) / l[i * n + i];
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Elementwise kernels.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double inv_logit(double e) { return 1.0 / (1.0 + exp(-e)); }
+
+// family.rs:55-90 working weights / residuals and the working response y~ = eta + r / W
+__global__ void reml_prep_kernel(const double* __restrict__ y, const double* __restrict__ eta,
+                                 const double* __restrict__ mu, int trait, size_t n,
+                                 double* __restrict__ w, double* __restrict__ ytil) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m = mu[i];
+  double v = (trait == SAIGE_TRAIT_QUANTITATIVE) ? 1.0 : m * (1.0 - m);
+  double wi = v > 1e-30 ? v : 1e-30;
+  w[i] = wi;
+  ytil[i] = eta[i] + (y[i] - m) / fmax(wi, 1e-30);
+}
+
+// eta = X alpha (dense.rs:102-113 order), mu = clamp(inv_logit(eta)) (family.rs:82-90)
+__global__ void eta_mu_kernel(const double* __restrict__ x, size_t ldx, size_t p,
+                              const double* __restrict__ alpha, int trait, size_t n,
+                              double* __restrict__ eta, double* __restrict__ mu) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double e = 0.0;
+  for (size_t j = 0; j < p; ++j) e += x[i + j * ldx] * alpha[j];
+  eta[i] = e;
+  if (trait == SAIGE_TRAIT_QUANTITATIVE) {
+    mu[i] = e;
+  } else {
+    double m = inv_logit(e);
+    mu[i] = fmin(fmax(m, 1e-10), 1.0 - 1e-10);
+  }
+}
+
+// out[:, c] = a[:, c] - sum_j six[:, j] * coef[j + p*c]  (ai_reml.rs:221-226, 246-251)
+__global__ void project_kernel(const double* __restrict__ a, size_t lda,
+                               const double* __restrict__ six, size_t lds, size_t p,
+                               const double* __restrict__ coef, size_t n, size_t k,
+                               double* __restrict__ out, size_t ldo) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t c = blockIdx.y;
+  if (i >= n || c >= k) return;
+  double v = a[i + c * lda];
+  for (size_t j = 0; j < p; ++j) v -= six[i + j * lds] * coef[j + p * c];
+  out[i + c * ldo] = v;
+}
+
+// out[:, c] = a[:, c] / max(w, 1e-30)
+__global__ void div_w_kernel(const double* __restrict__ a, size_t lda,
+                             const double* __restrict__ w, size_t n, size_t k,
+                             double* __restrict__ out, size_t ldo) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t c = blockIdx.y;
+  if (i >= n || c >= k) return;
+  out[i + c * ldo] = a[i + c * lda] / fmax(w[i], 1e-30);
+}
+
+__global__ void fill_kernel(double* __restrict__ a, size_t n, double v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+static inline dim3 grid1(size_t n) { return dim3((unsigned)ceil_div(n, 256)); }
+static inline dim3 grid2(size_t n, size_t k) { return dim3((unsigned)ceil_div(n, 256), (unsigned)k); }
+
+// IRLS for the GLM start (ai_reml.rs:431-482), host fp64 in the reference's order.
+static bool fit_glm_irls(const double* y, const double* x, size_t n, size_t p, int trait,
+                         int max_iter, std::vector<double>& alpha) {
+  std::vector<double> mu(n), eta(n), w(n), z(n), xw(p * p), xz(p), an(p), l;
+  for (size_t i = 0; i < n; ++i) mu[i] = (trait == SAIGE_TRAIT_QUANTITATIVE) ? y[i] : (y[i] + 0.5) / 2.0;
+  for (size_t i = 0; i < n; ++i)
+    eta[i] = (trait == SAIGE_TRAIT_QUANTITATIVE) ? mu[i] : std::log(mu[i] / (1.0 - mu[i]));
+  alpha.assign(p, 0.0);
+  for (int it = 0; it < max_iter; ++it) {
+    for (size_t i = 0; i < n; ++i) {
+      double v = (trait == SAIGE_TRAIT_QUANTITATIVE) ? 1.0 : mu[i] * (1.0 - mu[i]);
+      w[i] = v > 1e-30 ? v : 1e-30;
+      z[i] = eta[i] + (y[i] - mu[i]) / std::max(w[i], 1e-30);
+    }
+    for (size_t j = 0; j < p; ++j)
+      for (size_t k = j; k < p; ++k) {
+        double s = 0.0;
+        for (size_t i = 0; i < n; ++i) s += x[j * n + i] * w[i] * x[k * n + i];
+        xw[k * p + j] = s;
+        xw[j * p + k] = s;
+      }
+    for (size_t j = 0; j < p; ++j) {
+      double s = 0.0;
+      for (size_t i = 0; i < n; ++i) s += x[j * n + i] * w[i] * z[i];
+      xz[j] = s;
+    }
+    if (!chol(xw, p, l)) {
+      for (size_t i = 0; i < p; ++i) xw[i * p + i] += 1e-6;
+      if (!chol(xw, p, l)) return false;
+    }
+    chol_solve(l, p, xz.data(), an.data());
+    for (size_t i = 0; i < n; ++i) eta[i] = 0.0;
+    for (size_t j = 0; j < p; ++j)
+      for (size_t i = 0; i < n; ++i) eta[i] += x[j * n + i] * an[j];
+    for (size_t i = 0; i < n; ++i) {
+      if (trait == SAIGE_TRAIT_QUANTITATIVE) {
+        mu[i] = eta[i];
+      } else {
+        double m = 1.0 / (1.0 + std::exp(-eta[i]));
+        mu[i] = std::min(std::max(m, 1e-10), 1.0 - 1e-10);
+      }
+    }
+    double change = 0.0;
+    for (size_t j = 0; j < p; ++j) change += std::fabs(an[j] - alpha[j]);
+    alpha = an;
+    if (change < 1e-8) break;
+  }
+  return true;
+}
+
+struct RemlState {
+  DevBuf x, y, eta, mu, w, rhs1, sol1, gu, py, gpy, pu, rhs2, sol2, tmp, coef, part, res, ones;
+};
+
+// fit_ai_reml with X (n x p, host col-major) and y (host); outputs stay on device in st.
+static int fit_ai_reml_dev(saige_grm* h, const double* y, const double* x, size_t n, size_t p,
+                           int trait, const saige_ai_reml_config* cfg, saige_ai_reml_result* res,
+                           RemlState& st) {
+  hipStream_t s = h->stream;
+  const size_t ld = round_up(n, 64);
+  const size_t R = cfg->n_random_vectors;
+  const bool fix_tau_e = (trait != SAIGE_TRAIT_QUANTITATIVE);
+  const size_t k1 = 1 + p + R;
+  *res = saige_ai_reml_result{};
+  const uint64_t prod0 = h->products;
+
+  std::vector<double> alpha;
+  if (!fit_glm_irls(y, x, n, p, trait, 25, alpha)) {
+    set_error("GLM IRLS: X'WX not positive definite");
+    return SAIGE_ERR_NOT_PD;
+  }
+  SAIGE_TRY(st.x.ensure(ld * p * 8));
+  SAIGE_TRY(st.y.ensure(n * 8));
+  SAIGE_TRY(st.eta.ensure(n * 8));
+  SAIGE_TRY(st.mu.ensure(n * 8));
+  SAIGE_TRY(st.w.ensure(n * 8));
+  SAIGE_TRY(st.rhs1.ensure(ld * k1 * 8));
+  SAIGE_TRY(st.sol1.ensure(ld * k1 * 8));
+  SAIGE_TRY(st.gu.ensure(ld * std::max<size_t>(R, 1) * 8));
+  SAIGE_TRY(st.py.ensure(ld * 8));
+  SAIGE_TRY(st.gpy.ensure(ld * 8));
+  SAIGE_TRY(st.pu.ensure(ld * std::max<size_t>(R, 1) * 8));
+  SAIGE_TRY(st.rhs2.ensure(ld * 2 * 8));
+  SAIGE_TRY(st.sol2.ensure(ld * 2 * 8));
+  SAIGE_TRY(st.tmp.ensure(ld * (R + 2) * 8));
+  SAIGE_TRY(st.coef.ensure(std::max<size_t>(p * std::max<size_t>(R, 2), 8) * 8));
+  double* dX = st.x.as<double>();
+  double* dY = st.y.as<double>();
+  double* dEta = st.eta.as<double>();
+  double* dMu = st.mu.as<double>();
+  double* dW = st.w.as<double>();
+  double* RHS1 = st.rhs1.as<double>();
+  double* SOL1 = st.sol1.as<double>();
+  double* GU = st.gu.as<double>();
+  double* PY = st.py.as<double>();
+  double* GPY = st.gpy.as<double>();
+  double* PU = st.pu.as<double>();
+  double* RHS2 = st.rhs2.as<double>();
+  double* SOL2 = st.sol2.as<double>();
+  double* TMP = st.tmp.as<double>();
+  double* COEF = st.coef.as<double>();
+
+  SAIGE_HIP(hipMemcpy2DAsync(dX, ld * 8, x, n * 8, n * 8, p, hipMemcpyHostToDevice, s));
+  SAIGE_HIP(hipMemcpyAsync(dY, y, n * 8, hipMemcpyHostToDevice, s));
+  // X columns and the Rademacher probes are fixed right-hand sides (ai_reml.rs:121-130).
+  SAIGE_HIP(hipMemcpy2DAsync(RHS1 + ld, ld * 8, x, n * 8, n * 8, p, hipMemcpyHostToDevice, s));
+  {
+    ChaCha8 rng(cfg->seed);
+    std::vector<double> probes(n * R);
+    for (size_t r = 0; r < R; ++r)
+      for (size_t i = 0; i < n; ++i) probes[r * n + i] = rng.gen_bool() ? 1.0 : -1.0;
+    if (R > 0)
+      SAIGE_HIP(hipMemcpy2DAsync(RHS1 + (1 + p) * ld, ld * 8, probes.data(), n * 8, n * 8, R,
+                                 hipMemcpyHostToDevice, s));
+    SAIGE_HIP(hipStreamSynchronize(s));
+  }
+  const double* U = RHS1 + (1 + p) * ld;
+  if (R > 0) SAIGE_TRY(grm_product(h, U, ld, R, GU, ld));  // GRM * u, constant over iterations
+
+  // eta, mu from the GLM fit
+  SAIGE_HIP(hipMemcpyAsync(COEF, alpha.data(), p * 8, hipMemcpyHostToDevice, s));
+  eta_mu_kernel<<<grid1(n), 256, 0, s>>>(dX, ld, p, COEF, trait, n, dEta, dMu);
+  SAIGE_HIP(hipGetLastError());
+
+  double tau[2] = {1.0, 0.0};
+  bool converged = false;
+  uint32_t n_iter = 0;
+  uint64_t pcg_iters = 0;
+  uint32_t pcg_fail = 0;
+  std::vector<saige_pcg_info> info(k1);
+  std::vector<double> G((1 + p) * (p + R)), xsx(p * p), xsy(p), l, tcol(p), corr(p * std::max<size_t>(R, 2));
+  std::vector<double> tr(std::max<size_t>(R, 1)), misc(4);
+
+  for (uint32_t iter = 0; iter < cfg->max_iter; ++iter) {
+    n_iter = iter + 1;
+    reml_prep_kernel<<<grid1(n), 256, 0, s>>>(dY, dEta, dMu, trait, n, dW, RHS1);
+    SAIGE_HIP(hipGetLastError());
+    SAIGE_TRY(pcg_sigma_device(h, dW, tau, RHS1, ld, k1, cfg->pcg_tol, cfg->pcg_max_iter, SOL1, ld,
+                               info.data()));
+    for (size_t c = 0; c < k1; ++c) {
+      pcg_iters += info[c].iterations;
+      pcg_fail += info[c].converged ? 0 : 1;
+    }
+    const double* SIY = SOL1;
+    const double* SIX = SOL1 + ld;
+    const double* SIU = SOL1 + (1 + p) * ld;
+    // [Sigma^-1 y~ | Sigma^-1 X]' [X | U]: X'Sigma^-1 y, X'Sigma^-1 X, (Sigma^-1 X)'u
+    SAIGE_TRY(gram(s, SOL1, ld, 1 + p, RHS1 + ld, ld, p + R, n, G.data(), st.part, st.res));
+    auto Gat = [&](size_t a, size_t b) { return G[a + (1 + p) * b]; };
+    for (size_t j = 0; j < p; ++j)
+      for (size_t k = j; k < p; ++k) {
+        double d = Gat(1 + j, k);  // dot(x.col(k), sigma_inv_x[j]) (ai_reml.rs:194)
+        xsx[k * p + j] = d;
+        xsx[j * p + k] = d;
+      }
+    for (size_t j = 0; j < p; ++j) xsy[j] = Gat(0, j);
+    if (!chol(xsx, p, l)) break;  // ai_reml.rs:207-213
+    chol_solve(l, p, xsy.data(), alpha.data());
+    for (size_t r = 0; r < R; ++r) {
+      for (size_t j = 0; j < p; ++j) tcol[j] = Gat(1 + j, p + r);
+      chol_solve(l, p, tcol.data(), corr.data() + p * r);
+    }
+    // alpha, eta, mu (ai_reml.rs:214-218)
+    SAIGE_HIP(hipMemcpyAsync(COEF, alpha.data(), p * 8, hipMemcpyHostToDevice, s));
+    eta_mu_kernel<<<grid1(n), 256, 0, s>>>(dX, ld, p, COEF, trait, n, dEta, dMu);
+    // P y = Sigma^-1 y - Sigma^-1 X alpha (ai_reml.rs:221-226)
+    project_kernel<<<grid2(n, 1), 256, 0, s>>>(SIY, ld, SIX, ld, p, COEF, n, 1, PY, ld);
+    SAIGE_HIP(hipGetLastError());
+    SAIGE_TRY(grm_product(h, PY, ld, 1, GPY, ld));
+    double ypapy;
+    SAIGE_TRY(col_dots(s, PY, ld, GPY, ld, n, 1, &ypapy, st.part, st.res));
+    double trace = 0.0;
+    if (R > 0) {
+      // P u_r = Sigma^-1 u_r - Sigma^-1 X corr_r (ai_reml.rs:242-251); trace terms (GRM u)'Pu
+      SAIGE_HIP(hipMemcpyAsync(COEF, corr.data(), p * R * 8, hipMemcpyHostToDevice, s));
+      project_kernel<<<grid2(n, R), 256, 0, s>>>(SIU, ld, SIX, ld, p, COEF, n, R, PU, ld);
+      SAIGE_HIP(hipGetLastError());
+      SAIGE_TRY(col_dots(s, GU, ld, PU, ld, n, R, tr.data(), st.part, st.res));
+      for (size_t r = 0; r < R; ++r) trace += tr[r];
+      trace /= (double)R;
+    }
+    const double score_g = ypapy - trace;
+
+    // second-stage block: Sigma^-1 (GRM P y) [and Sigma^-1 (P y / W)]
+    size_t k2 = fix_tau_e ? 1 : 2;
+    SAIGE_HIP(hipMemcpyAsync(RHS2, GPY, n * 8, hipMemcpyDeviceToDevice, s));
+    if (!fix_tau_e) {
+      div_w_kernel<<<grid2(n, 1), 256, 0, s>>>(PY, ld, dW, n, 1, RHS2 + ld, ld);
+      SAIGE_HIP(hipGetLastError());
+    }
+    SAIGE_TRY(pcg_sigma_device(h, dW, tau, RHS2, ld, k2, cfg->pcg_tol, cfg->pcg_max_iter, SOL2,
+                               ld, info.data()));
+    for (size_t c = 0; c < k2; ++c) {
+      pcg_iters += info[c].iterations;
+      pcg_fail += info[c].converged ? 0 : 1;
+    }
+    // corrections (Sigma^-1 X)' rhs2 -> chol.solve (ai_reml.rs:263-272, 306-315)
+    std::vector<double> g2(p * 2);
+    SAIGE_TRY(gram(s, SIX, ld, p, RHS2, ld, k2, n, g2.data(), st.part, st.res));
+    for (size_t c = 0; c < k2; ++c) chol_solve(l, p, g2.data() + p * c, corr.data() + p * c);
+    SAIGE_HIP(hipMemcpyAsync(COEF, corr.data(), p * k2 * 8, hipMemcpyHostToDevice, s));
+    project_kernel<<<grid2(n, k2), 256, 0, s>>>(SOL2, ld, SIX, ld, p, COEF, n, k2, TMP, ld);
+    SAIGE_HIP(hipGetLastError());
+    double ai11;
+    SAIGE_TRY(col_dots(s, GPY, ld, TMP, ld, n, 1, &ai11, st.part, st.res));
+    double score_e = 0.0, ai00 = 0.0, ai01 = 0.0;
+    if (!fix_tau_e) {
+      // TMP = [P GRM P y | P (P y / W) | U / W]; trace(P diag(1/W)) reuses P u
+      // (ai_reml.rs:281-317 recomputes the same solves)
+      double trw = 0.0;
+      if (R > 0) {
+        div_w_kernel<<<grid2(n, R), 256, 0, s>>>(U, ld, dW, n, R, TMP + 2 * ld, ld);
+        SAIGE_HIP(hipGetLastError());
+        SAIGE_TRY(col_dots(s, TMP + 2 * ld, ld, PU, ld, n, R, tr.data(), st.part, st.res));
+        for (size_t r = 0; r < R; ++r) trw += tr[r];
+        trw /= (double)R;
+      }
+      double ypa0py;
+      SAIGE_TRY(col_dots(s, PY, ld, RHS2 + ld, ld, n, 1, &ypa0py, st.part, st.res));
+      score_e = ypa0py - trw;
+      SAIGE_TRY(col_dots(s, RHS2 + ld, ld, TMP + ld, ld, n, 1, &ai00, st.part, st.res));
+      SAIGE_TRY(col_dots(s, RHS2 + ld, ld, TMP, ld, n, 1, &ai01, st.part, st.res));
+    }
+    // tau update with step halving (ai_reml.rs:329-369)
+    double tn[2], max_change;
+    if (fix_tau_e) {
+      if (std::fabs(ai11) < 1e-30) break;
+      double d1 = score_g / ai11;
+      double step = 1.0;
+      double tg = tau[1] + step * d1;
+      while (tg < 0.0 && step > 1e-10) {
+        step *= 0.5;
+        tg = tau[1] + step * d1;
+      }
+      tg = std::max(tg, 1e-10);
+      max_change = std::fabs(tg - tau[1]) / (std::fabs(tg) + std::fabs(tau[1]) + cfg->tol);
+      tn[0] = 1.0;
+      tn[1] = tg;
+    } else {
+      double det = ai00 * ai11 - ai01 * ai01;
+      if (std::fabs(det) < 1e-30) break;
+      double d0 = (ai11 * score_e - ai01 * score_g) / det;
+      double d1 = (-ai01 * score_e + ai00 * score_g) / det;
+      double step = 1.0;
+      double t0 = tau[0] + step * d0, t1 = tau[1] + step * d1;
+      while ((t0 < 0.0 || t1 < 0.0) && step > 1e-10) {
+        step *= 0.5;
+        t0 = tau[0] + step * d0;
+        t1 = tau[1] + step * d1;
+      }
+      t0 = std::max(t0, 1e-10);
+      t1 = std::max(t1, 1e-10);
+      double c0 = std::fabs(t0 - tau[0]) / (std::fabs(t0) + std::fabs(tau[0]) + cfg->tol);
+      double c1 = std::fabs(t1 - tau[1]) / (std::fabs(t1) + std::fabs(tau[1]) + cfg->tol);
+      max_change = std::max(c0, c1);
+      tn[0] = t0;
+      tn[1] = t1;
+    }
+    tau[0] = tn[0];
+    tau[1] = tn[1];
+    if (max_change < cfg->tol) {
+      converged = true;
+      break;
+    }
+  }
+  // final W and log-likelihood (ai_reml.rs:393-414)
+  std::vector<double> mu(n);
+  SAIGE_HIP(hipMemcpyAsync(mu.data(), dMu, n * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  double ll = 0.0;
+  if (trait != SAIGE_TRAIT_QUANTITATIVE) {
+    for (size_t i = 0; i < n; ++i) {
+      double mi = std::min(std::max(mu[i], 1e-15), 1.0 - 1e-15);
+      ll += y[i] * std::log(mi) + (1.0 - y[i]) * std::log(1.0 - mi);
+    }
+  } else {
+    double ss = 0.0;
+    for (size_t i = 0; i < n; ++i) ss += (y[i] - mu[i]) * (y[i] - mu[i]);
+    ll = -0.5 * (double)n * std::log(2.0 * M_PI * tau[0]) - 0.5 * ss / tau[0];
+  }
+  // final working weights on device (family.rs:55-66)
+  reml_prep_kernel<<<grid1(n), 256, 0, s>>>(dY, dEta, dMu, trait, n, dW, RHS1);
+  SAIGE_HIP(hipGetLastError());
+  res->tau[0] = tau[0];
+  res->tau[1] = tau[1];
+  res->iterations = n_iter;
+  res->converged = converged ? 1 : 0;
+  res->log_likelihood = ll;
+  res->grm_products = h->products - prod0;
+  res->pcg_iterations = pcg_iters;
+  res->pcg_not_converged = pcg_fail;
+  st.coef.ensure(std::max<size_t>(p, 8) * 8);
+  SAIGE_HIP(hipMemcpyAsync(st.coef.p, alpha.data(), p * 8, hipMemcpyHostToDevice, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  return SAIGE_OK;
+}
+
+// (X'WX)^-1 X'W, flat buffer filled as fit_null.rs:293-305 does.
+static int xvx_inv_xv_host(const double* x, size_t n, size_t p, const double* w, double* out) {
+  std::vector<double> xvx(p * p), l, inv(p * p), e(p), col(p);
+  for (size_t j = 0; j < p; ++j)
+    for (size_t k = j; k < p; ++k) {
+      double s = 0.0;
+      for (size_t i = 0; i < n; ++i) s += x[j * n + i] * w[i] * x[k * n + i];
+      xvx[k * p + j] = s;
+      xvx[j * p + k] = s;
+    }
+  if (!chol(xvx, p, l)) {
+    set_error("X'WX is not positive definite");
+    return SAIGE_ERR_NOT_PD;
+  }
+  for (size_t j = 0; j < p; ++j) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[j] = 1.0;
+    chol_solve(l, p, e.data(), col.data());
+    for (size_t i = 0; i < p; ++i) inv[j * p + i] = col[i];
+  }
+  for (size_t j = 0; j < p; ++j)
+    for (size_t i = 0; i < n; ++i) {
+      double v = 0.0;
+      for (size_t k = 0; k < p; ++k) v += inv[k * p + j] * x[k * n + i] * w[i];
+      out[j * n + i] = v;
+    }
+  return SAIGE_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// Variance ratio (variance_ratio.rs:87-246).
+// ---------------------------------------------------------------------------------
+// Flip to the minor allele per column where flip[c] (variance_ratio.rs:131-137).
+__global__ void flip_kernel(double* __restrict__ g, size_t ld, size_t n, size_t k,
+                            const int* __restrict__ flip) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t c = blockIdx.y;
+  if (i >= n || c >= k || !flip[c]) return;
+  g[i + c * ld] = 2.0 - g[i + c * ld];
+}
+// g~ = g0 - X (A g0) (variance_ratio.rs:144-151; A g0 computed as p dots per column)
+__global__ void adjust_kernel(const double* __restrict__ g0, const double* __restrict__ x,
+                              size_t ld, size_t p, const double* __restrict__ ag, size_t n,
+                              size_t k, double* __restrict__ gt) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t c = blockIdx.y;
+  if (i >= n || c >= k) return;
+  double xa = 0.0;
+  for (size_t j = 0; j < p; ++j) xa += x[i + j * ld] * ag[j + p * c];
+  gt[i + c * ld] = g0[i + c * ld] - xa;
+}
+// weight * g~ / ac  (var2 summand without the last g~ factor)
+__global__ void var2_weight_kernel(const double* __restrict__ gt, size_t ld,
+                                   const double* __restrict__ mu, int trait,
+                                   const double* __restrict__ ac, size_t n, size_t k,
+                                   double* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t c = blockIdx.y;
+  if (i >= n || c >= k) return;
+  double m = mu[i], g = gt[i + c * ld];
+  double wgt = trait == SAIGE_TRAIT_BINARY ? m * (1.0 - m) : (trait == SAIGE_TRAIT_QUANTITATIVE ? 1.0 : m);
+  out[i + c * ld] = wgt * g / ac[c];
+}
+// decode marker rows (model order, padded rows) into f64 columns
+__global__ void decode_rows_kernel(const uint8_t* __restrict__ rows, size_t row_bytes,
+                                   const uint32_t* __restrict__ which, size_t n, size_t k,
+                                   double* __restrict__ out, size_t ldo) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t c = blockIdx.y;
+  if (i >= n || c >= k) return;
+  uint32_t code = (rows[(size_t)which[c] * row_bytes + (i >> 2)] >> (2 * (i & 3))) & 3u;
+  out[i + c * ldo] = code == 0u ? 2.0 : (code == 2u ? 1.0 : (code == 3u ? 0.0 : __longlong_as_double(0x7ff8000000000000ll)));
+}
+
+static double cal_cv(const std::vector<double>& v) {  // variance_ratio.rs:249-261
+  size_t n = v.size();
+  if (n <= 1) return INFINITY;
+  double m = 0.0;
+  for (double x : v) m += x;
+  m /= (double)n;
+  if (std::fabs(m) < 1e-30) return INFINITY;
+  double var = 0.0;
+  for (double x : v) var += (x - m) * (x - m);
+  var /= ((double)n - 1.0);
+  return (std::sqrt(var) / m) / (double)n;
+}
+
+// Column source: fills G (ld x k) with the raw dosages of candidate markers `ids`.
+struct VrSource {
+  const double* host_dosages = nullptr;  // n x m col-major (host)
+  const uint8_t* d_rows = nullptr;       // device packed rows (model order)
+  size_t row_bytes = 0;
+};
+
+static int vr_fill(hipStream_t s, const VrSource& src, size_t n, const std::vector<uint32_t>& ids,
+                   double* G, size_t ld, DevBuf& d_ids) {
+  size_t k = ids.size();
+  if (src.host_dosages) {
+    for (size_t c = 0; c < k; ++c)
+      SAIGE_HIP(hipMemcpyAsync(G + c * ld, src.host_dosages + (size_t)ids[c] * n, n * 8,
+                               hipMemcpyHostToDevice, s));
+    return SAIGE_OK;
+  }
+  SAIGE_TRY(d_ids.ensure(k * 4));
+  SAIGE_HIP(hipMemcpyAsync(d_ids.p, ids.data(), k * 4, hipMemcpyHostToDevice, s));
+  decode_rows_kernel<<<grid2(n, k), 256, 0, s>>>(src.d_rows, src.row_bytes, d_ids.as<uint32_t>(),
+                                                 n, k, G, ld);
+  SAIGE_HIP(hipGetLastError());
+  return SAIGE_OK;
+}
+
+static int estimate_vr_dev(saige_grm* h, const VrSource& src, const double* mac, size_t m,
+                           const double* d_mu, const double* d_w, const double tau[2], int trait,
+                           const double* d_x, size_t ldx, const double* a_flat, size_t p,
+                           const saige_vr_config* cfg, double* vr_out, uint32_t* n_used,
+                           double* per_marker, RemlState& st) {
+  hipStream_t s = h->stream;
+  const size_t n = h->g.n;
+  const size_t ld = round_up(n, 64);
+  ChaCha8 rng(cfg->seed);
+  std::vector<uint32_t> order(m);
+  std::iota(order.begin(), order.end(), 0u);
+  rng.shuffle(order);
+  // A^T as n x p col-major: element (r, i) of the col-major p x n view is a_flat[i*p + r]
+  DevBuf d_at, G, GT, SG, WG, acb, agb, flipb, ids, onesb;
+  SAIGE_TRY(d_at.ensure(ld * p * 8));
+  {
+    std::vector<double> at(n * p);
+    for (size_t i = 0; i < n; ++i)
+      for (size_t r = 0; r < p; ++r) at[r * n + i] = a_flat[i * p + r];
+    SAIGE_HIP(hipMemcpy2DAsync(d_at.p, ld * 8, at.data(), n * 8, n * 8, p, hipMemcpyHostToDevice, s));
+    SAIGE_HIP(hipStreamSynchronize(s));
+  }
+  SAIGE_TRY(onesb.ensure(ld * 8));
+  fill_kernel<<<grid1(n), 256, 0, s>>>(onesb.as<double>(), n, 1.0);
+  std::vector<double> vr;
+  size_t target = cfg->n_markers, tested = 0, cursor = 0;
+  bool done = false;
+  while (!done && cursor < m) {
+    size_t want = (tested < target ? target - tested : 0) + 2;
+    want = std::max<size_t>(want, 4);
+    std::vector<uint32_t> chunk;
+    while (cursor < m && chunk.size() < want) {
+      uint32_t id = order[cursor++];
+      if (mac[id] < cfg->min_mac) continue;  // variance_ratio.rs:124-126
+      chunk.push_back(id);
+    }
+    size_t k = chunk.size();
+    if (k == 0) break;
+    SAIGE_TRY(G.ensure(ld * k * 8));
+    SAIGE_TRY(GT.ensure(ld * k * 8));
+    SAIGE_TRY(SG.ensure(ld * k * 8));
+    SAIGE_TRY(WG.ensure(ld * k * 8));
+    SAIGE_TRY(acb.ensure(k * 8));
+    SAIGE_TRY(agb.ensure(p * k * 8));
+    SAIGE_TRY(flipb.ensure(k * 4));
+    double* dG = G.as<double>();
+    SAIGE_TRY(vr_fill(s, src, n, chunk, dG, ld, ids));
+    std::vector<double> sums(k), acs(k), ag(p * k), v1(k), v2(k);
+    SAIGE_TRY(col_dots(s, dG, ld, onesb.as<double>(), 0, n, k, sums.data(), st.part, st.res));
+    std::vector<int> flip(k);
+    for (size_t c = 0; c < k; ++c) flip[c] = (sums[c] / (2.0 * (double)n)) > 0.5;
+    SAIGE_HIP(hipMemcpyAsync(flipb.p, flip.data(), k * 4, hipMemcpyHostToDevice, s));
+    flip_kernel<<<grid2(n, k), 256, 0, s>>>(dG, ld, n, k, flipb.as<int>());
+    SAIGE_TRY(col_dots(s, dG, ld, onesb.as<double>(), 0, n, k, acs.data(), st.part, st.res));
+    SAIGE_TRY(gram(s, d_at.as<double>(), ld, p, dG, ld, k, n, ag.data(), st.part, st.res));
+    SAIGE_HIP(hipMemcpyAsync(agb.p, ag.data(), p * k * 8, hipMemcpyHostToDevice, s));
+    adjust_kernel<<<grid2(n, k), 256, 0, s>>>(dG, d_x, ldx, p, agb.as<double>(), n, k, GT.as<double>());
+    SAIGE_HIP(hipGetLastError());
+    std::vector<saige_pcg_info> info(k);
+    SAIGE_TRY(pcg_sigma_device(h, d_w, tau, GT.as<double>(), ld, k, 1e-5, 500, SG.as<double>(), ld,
+                               info.data()));
+    SAIGE_TRY(col_dots(s, GT.as<double>(), ld, SG.as<double>(), ld, n, k, v1.data(), st.part, st.res));
+    SAIGE_HIP(hipMemcpyAsync(acb.p, acs.data(), k * 8, hipMemcpyHostToDevice, s));
+    var2_weight_kernel<<<grid2(n, k), 256, 0, s>>>(GT.as<double>(), ld, d_mu, trait, acb.as<double>(),
+                                                   n, k, WG.as<double>());
+    SAIGE_TRY(col_dots(s, WG.as<double>(), ld, GT.as<double>(), ld, n, k, v2.data(), st.part, st.res));
+    // replay the sequential selection (variance_ratio.rs:120-212)
+    for (size_t c = 0; c < k && !done; ++c) {
+      double ac = acs[c];
+      if (ac < 1.0) continue;
+      double var1 = v1[c] / ac;
+      double var2 = v2[c];
+      if (var2 < 1e-30) continue;
+      vr.push_back(var1 / var2);
+      ++tested;
+      if (tested >= target) {
+        if (cal_cv(vr) <= cfg->ratio_cv_cutoff) done = true;
+        else target += 10;
+      }
+    }
+  }
+  if (vr.empty()) {
+    set_error("No markers available for variance ratio estimation");
+    return SAIGE_ERR_NO_VR_MARKERS;
+  }
+  double mean = 0.0;
+  for (double v : vr) mean += v;
+  mean /= (double)vr.size();
+  *vr_out = mean;
+  if (n_used) *n_used = (uint32_t)vr.size();
+  if (per_marker) std::copy(vr.begin(), vr.end(), per_marker);
+  (void)d_x;
+  return SAIGE_OK;
+}
+
+}  // namespace saige
+
+using namespace saige;
+
+extern "C" {
+
+int saige_fit_ai_reml(saige_grm* h, const double* y, const double* x, size_t n, size_t p,
+                      int trait_type, const saige_ai_reml_config* cfg,
+                      saige_ai_reml_result* result, double* mu, double* w, double* residuals,
+                      double* eta, double* alpha) {
+  clear_error();
+  SAIGE_ARG(h && y && x && cfg && result, "null pointer");
+  SAIGE_ARG(n == h->g.n, "y length != GRM samples");
+  SAIGE_ARG(p >= 1, "p >= 1");
+  SAIGE_ARG(trait_type >= 0 && trait_type <= 2, "trait_type");
+  std::lock_guard<std::recursive_mutex> lk(h->mu);
+  SAIGE_HIP(hipSetDevice(h->device));
+  RemlState st;
+  SAIGE_TRY(fit_ai_reml_dev(h, y, x, n, p, trait_type, cfg, result, st));
+  hipStream_t s = h->stream;
+  std::vector<double> hm(n), hw(n);
+  SAIGE_HIP(hipMemcpyAsync(hm.data(), st.mu.p, n * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipMemcpyAsync(hw.data(), st.w.p, n * 8, hipMemcpyDeviceToHost, s));
+  if (eta) SAIGE_HIP(hipMemcpyAsync(eta, st.eta.p, n * 8, hipMemcpyDeviceToHost, s));
+  if (alpha) SAIGE_HIP(hipMemcpyAsync(alpha, st.coef.p, p * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  if (mu) std::copy(hm.begin(), hm.end(), mu);
+  if (w) std::copy(hw.begin(), hw.end(), w);
+  if (residuals)
+    for (size_t i = 0; i < n; ++i) residuals[i] = y[i] - hm[i];
+  return SAIGE_OK;
+}
+
+int saige_xvx_inv_xv(const double* x, size_t n, size_t p, const double* w, double* out) {
+  clear_error();
+  SAIGE_ARG(x && w && out && p >= 1, "null pointer");
+  return xvx_inv_xv_host(x, n, p, w, out);
+}
+
+int saige_estimate_variance_ratio(saige_grm* h, const double* dosages, const double* mac, size_t m,
+                                  const double* mu, const double* w, const double tau[2],
+                                  int trait_type, const double* x, const double* xvx_inv_xv,
+                                  size_t p, const saige_vr_config* cfg, double* variance_ratio,
+                                  uint32_t* n_used, double* per_marker_vr) {
+  clear_error();
+  SAIGE_ARG(h && mac && mu && w && tau && x && xvx_inv_xv && cfg && variance_ratio, "null pointer");
+  SAIGE_ARG(m == 0 || dosages, "dosages");
+  std::lock_guard<std::recursive_mutex> lk(h->mu);
+  SAIGE_HIP(hipSetDevice(h->device));
+  const size_t n = h->g.n, ld = round_up(n, 64);
+  hipStream_t s = h->stream;
+  DevBuf dmu, dw, dx;
+  SAIGE_TRY(dmu.ensure(n * 8));
+  SAIGE_TRY(dw.ensure(n * 8));
+  SAIGE_TRY(dx.ensure(ld * p * 8));
+  SAIGE_HIP(hipMemcpyAsync(dmu.p, mu, n * 8, hipMemcpyHostToDevice, s));
+  SAIGE_HIP(hipMemcpyAsync(dw.p, w, n * 8, hipMemcpyHostToDevice, s));
+  SAIGE_HIP(hipMemcpy2DAsync(dx.p, ld * 8, x, n * 8, n * 8, p, hipMemcpyHostToDevice, s));
+  VrSource src;
+  src.host_dosages = dosages;
+  RemlState st;
+  if (m == 0) {
+    set_error("No markers available for variance ratio estimation");
+    return SAIGE_ERR_NO_VR_MARKERS;
+  }
+  return estimate_vr_dev(h, src, mac, m, dmu.as<double>(), dw.as<double>(), tau, trait_type,
+                         dx.as<double>(), ld, xvx_inv_xv, p, cfg, variance_ratio, n_used,
+                         per_marker_vr, st);
+}
+
+int saige_fit_null(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,
+                   size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
+                   const double* x, size_t p, int trait_type, const saige_fit_null_config* cfg,
+                   saige_fit_null_result* result, double* mu_out, double* xvx_inv_xv,
+                   uint8_t* marker_class) {
+  clear_error();
+  SAIGE_ARG(bed_body && y && x && cfg && result, "null pointer");
+  SAIGE_ARG(p >= 1 && n > 0, "dimensions");
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  *result = saige_fit_null_result{};
+  // QC over all markers (fit_null.rs:209-222)
+  std::vector<double> ac(m_total), af(m_total), mac(m_total);
+  std::vector<uint32_t> nv(m_total);
+  SAIGE_TRY(saige_plink_marker_stats(device, bed_body, bpm, n_total, m_total, sample_idx, n,
+                                     ac.data(), nv.data(), af.data(), mac.data()));
+  std::vector<uint32_t> passing;
+  for (size_t j = 0; j < m_total; ++j) {
+    double miss = 1.0 - ((double)nv[j] / (double)n);
+    if (af[j] >= cfg->min_maf && af[j] <= 1.0 - cfg->min_maf && miss <= cfg->max_missing_rate)
+      passing.push_back((uint32_t)j);
+  }
+  if (passing.empty()) {
+    set_error("no marker passes QC");
+    return SAIGE_ERR_NO_MARKERS;
+  }
+  // VR candidates: 1000 draws of gen_range(0..n_passing), sort, dedup, MAC >= 20 (:230-243)
+  const size_t np = passing.size();
+  std::vector<uint8_t> in_vr(np, 0);
+  {
+    ChaCha8 rng(cfg->reml.seed);
+    std::vector<uint64_t> cand(1000);
+    for (auto& c : cand) c = rng.gen_range64(np);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    for (uint64_t c : cand)
+      if (mac[passing[c]] >= 20.0) in_vr[c] = 1;
+  }
+  std::vector<uint32_t> grm_ids, vr_ids;
+  std::vector<double> grm_af, vr_mac;
+  if (marker_class) std::fill(marker_class, marker_class + m_total, 0);
+  for (size_t i = 0; i < np; ++i) {
+    uint32_t j = passing[i];
+    if (in_vr[i]) {
+      vr_ids.push_back(j);
+      vr_mac.push_back(mac[j]);
+      if (marker_class) marker_class[j] = 2;
+    } else {
+      grm_ids.push_back(j);
+      grm_af.push_back(af[j]);
+      if (marker_class) marker_class[j] = 1;
+    }
+  }
+  auto t1 = clk::now();
+  saige_grm* h = nullptr;
+  SAIGE_TRY(saige_grm_create(device, bed_body, bpm, n_total, sample_idx, n, grm_ids.data(),
+                             grm_ids.size(), grm_af.data(), cfg->precision, &h));
+  int rc = SAIGE_OK;
+  {
+    std::lock_guard<std::recursive_mutex> lk(h->mu);
+    RemlState st;
+    hipStream_t s = h->stream;
+    const size_t ld = round_up(n, 64);
+    rc = fit_ai_reml_dev(h, y, x, n, p, trait_type, &cfg->reml, &result->reml, st);
+    auto t2 = clk::now();
+    std::vector<double> hm(n), hw(n), a(p * n);
+    if (rc == SAIGE_OK) {
+      if (hipMemcpyAsync(hm.data(), st.mu.p, n * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(hw.data(), st.w.p, n * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("copy of mu/w failed");
+        rc = SAIGE_ERR_DEVICE;
+      }
+    }
+    if (rc == SAIGE_OK) rc = xvx_inv_xv_host(x, n, p, hw.data(), a.data());
+    double vr = 1.0;
+    uint32_t used = 0;
+    if (rc == SAIGE_OK && !vr_ids.empty()) {
+      // VR markers resident as packed rows in model order, decoded per chunk
+      DevBuf rows, d_idx;
+      const size_t row_bytes = h->g.row_bytes;
+      rc = rows.ensure(vr_ids.size() * row_bytes);
+      if (rc == SAIGE_OK) rc = upload_sample_idx(s, sample_idx, n, n_total, d_idx);
+      if (rc == SAIGE_OK)
+        rc = upload_repack(s, bed_body, bpm, n_total, d_idx.as<uint32_t>(), n, vr_ids.data(),
+                           vr_ids.size(), rows.as<uint8_t>(), row_bytes);
+      VrSource src;
+      src.d_rows = rows.as<uint8_t>();
+      src.row_bytes = row_bytes;
+      saige_vr_config vc{cfg->n_markers_vr, 20.0, 0.001, cfg->reml.seed};
+      if (rc == SAIGE_OK)
+        rc = estimate_vr_dev(h, src, vr_mac.data(), vr_ids.size(), st.mu.as<double>(),
+                             st.w.as<double>(), result->reml.tau, trait_type, st.x.as<double>(), ld,
+                             a.data(), p, &vc, &vr, &used, nullptr, st);
+    }
+    auto t3 = clk::now();
+    if (rc == SAIGE_OK) {
+      result->variance_ratio = vr;
+      result->vr_markers_used = used;
+      result->n_grm_markers = (uint32_t)grm_ids.size();
+      result->n_vr_markers = (uint32_t)vr_ids.size();
+      result->seconds_qc = std::chrono::duration<double>(t1 - t0).count();
+      result->seconds_reml = std::chrono::duration<double>(t2 - t1).count();
+      result->seconds_vr = std::chrono::duration<double>(t3 - t2).count();
+      result->seconds_total = std::chrono::duration<double>(t3 - t0).count();
+      result->reml.grm_products = h->products;
+      if (mu_out) std::copy(hm.begin(), hm.end(), mu_out);
+      if (xvx_inv_xv) std::copy(a.begin(), a.end(), xvx_inv_xv);
+    }
+  }
+  saige_grm_destroy(h);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,755 @@
+// Step 2 on the GPU: the single-variant score test with variance ratio and the
+// saddle-point approximation.
+//
+// Reference: ScoreTestEngine::test_marker (saige-core/src/score_test/single_variant.rs:
+// 112-257), compute_case_control_af (:260-291), select_variance_ratio
+// (glmm/variance_ratio.rs:289-301), spa_binary_fast (spa/fast.rs:207-242) with
+// find_root_k1_fast (:53-132), saddle_probability_fast (:135-180) and
+// partition_for_fast_spa (:185-204).
+//
+// Kernel 1 (score_sums): a batch of variants is a (variants x samples) matrix times a
+// (samples x features) matrix.  One lane per variant, 64 variants per wave; the lanes of
+// a wave walk the same sample index, so the per-sample feature row
+//   [A(:, i) (p values), res_i, mu2_i, mu2_i * x_i (p values), mu_i, case_i]
+// is wave-uniform and comes from the scalar cache.  The sample range is split over the
+// grid; partial sums are combined in a fixed order (deterministic).
+// Kernel 2 (score_finalize): one thread per variant turns the sums into S, varT, varT*,
+// T, the chi-square(1) p-value and the SPA gate, using
+//   g~ = g - X (A g)                       (single_variant.rs:137-144)
+//   g~'res      = g'res - (A g)'(X'res)
+//   sum mu2 g~^2 = g'M2g - 2 (A g)'(X'M2 g) + (A g)'(X'M2X)(A g)
+//   sum mu g~    = g'mu - (A g)'(X'mu)
+// which hold for any p x n matrix A (see the layout note on xvx_inv_xv_flat).
+// Kernel 3 (spa): one work-group per gated variant runs both safeguarded Newton roots in
+// lockstep, recomputing g~_i from the dosage, X and A g in every pass, each pass fusing
+// K'(t) and K''(t) (the values the reference's loop needs at the same point), then the
+// Lugannani-Rice tail.  Control flow per root is exactly find_root_k1_fast.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace saige {
+
+constexpr int kMaxP = 8;
+constexpr int kMaxCat = 16;
+
+struct EngineConst {
+  double tau_e;
+  double vr;
+  double cutoff;
+  double spa_tol;
+  int use_spa;
+  int fast;
+  int n_cat;
+  int has_y;
+  double xtr[kMaxP];
+  double xtm[kMaxP];
+  double xm2x[kMaxP * kMaxP];
+  double cat_upper[kMaxCat];
+  double cat_vr[kMaxCat];
+};
+
+struct GateInfo {
+  uint32_t var;
+  uint32_t pad;
+  double q, qinv, pn;
+  double ag[kMaxP];
+};
+
+__device__ __forceinline__ double dnan() { return __longlong_as_double(0x7ff8000000000000ll); }
+
+// ---- dosage loaders: value of sample i of variant v (NaN = missing) ----
+struct LoadU8 {
+  const uint8_t* q;
+  size_t ld;
+  double scale;
+  __device__ __forceinline__ double operator()(size_t v, size_t i) const {
+    return (double)q[v * ld + i] * scale;
+  }
+};
+struct LoadF64 {
+  const double* g;
+  size_t ld;
+  __device__ __forceinline__ double operator()(size_t v, size_t i) const { return g[v * ld + i]; }
+};
+struct LoadPacked {  // plink.rs:170-194, 237-247
+  const uint8_t* bed;
+  size_t bpm;
+  const uint32_t* idx;
+  __device__ __forceinline__ double operator()(size_t v, size_t i) const {
+    size_t s = idx ? idx[i] : i;
+    uint32_t c = (bed[v * bpm + (s >> 2)] >> (2 * (s & 3))) & 3u;
+    return c == 0u ? 2.0 : (c == 2u ? 1.0 : (c == 3u ? 0.0 : dnan()));
+  }
+};
+
+// feature row layout
+template <int P> struct Layout {
+  static constexpr int NF = 2 * P + 4;  // A(p), res, mu2, mu2*x(p), mu, case
+  static constexpr int NS = 2 * P + 9;  // sums: g, n, gA(p), g res, g^2 mu2, g mu2 x(p), g mu, g case, n case, g ctrl
+};
+
+template <int P, class L>
+__global__ __launch_bounds__(64) void score_sums_kernel(L load, const double* __restrict__ feat,
+                                                        size_t n, size_t nvar, size_t sps,
+                                                        double* __restrict__ part,
+                                                        size_t nvar_pad) {
+  constexpr int NF = Layout<P>::NF, NS = Layout<P>::NS;
+  const size_t v = (size_t)blockIdx.x * 64 + threadIdx.x;
+  const size_t vv = v < nvar ? v : nvar - 1;
+  const size_t i0 = (size_t)blockIdx.y * sps;
+  const size_t i1 = min(i0 + sps, n);
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  for (size_t i = i0; i < i1; ++i) {
+    const double* f = feat + i * NF;
+    const double g = load(vv, i);
+    const bool ok = !isnan(g);
+    const double gz = ok ? g : 0.0;
+    const double one = ok ? 1.0 : 0.0;
+    acc[0] += gz;
+    acc[1] += one;
+#pragma unroll
+    for (int r = 0; r < P; ++r) acc[2 + r] = fma(gz, f[r], acc[2 + r]);
+    acc[2 + P] = fma(gz, f[P], acc[2 + P]);
+    acc[3 + P] = fma(gz * gz, f[P + 1], acc[3 + P]);
+#pragma unroll
+    for (int j = 0; j < P; ++j) acc[4 + P + j] = fma(gz, f[P + 2 + j], acc[4 + P + j]);
+    acc[4 + 2 * P] = fma(gz, f[2 * P + 2], acc[4 + 2 * P]);
+    const double cs = f[2 * P + 3];
+    acc[5 + 2 * P] = fma(gz, cs, acc[5 + 2 * P]);
+    acc[6 + 2 * P] = fma(one, cs, acc[6 + 2 * P]);
+    acc[7 + 2 * P] = fma(gz, 1.0 - cs, acc[7 + 2 * P]);
+  }
+  if (v >= nvar) return;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) part[((size_t)blockIdx.y * NS + s) * nvar_pad + v] = acc[s];
+}
+
+__device__ __forceinline__ double chisq1_cdf(double t) { return t <= 0.0 ? 0.0 : erf(sqrt(0.5 * t)); }
+__device__ __forceinline__ double norm_cdf(double z) { return 0.5 * erfc(-z / sqrt(2.0)); }
+
+template <int P>
+__global__ void score_finalize_kernel(const double* __restrict__ part, size_t splits,
+                                      size_t nvar, size_t nvar_pad, size_t n, EngineConst ec,
+                                      saige_variant_result* __restrict__ out,
+                                      GateInfo* __restrict__ gate, uint32_t* __restrict__ n_gate) {
+  constexpr int NS = Layout<P>::NS;
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvar) return;
+  double s[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    double a = 0.0;
+    for (size_t q = 0; q < splits; ++q) a += part[(q * NS + k) * nvar_pad + v];
+    s[k] = a;
+  }
+  const double sum = s[0], nv = s[1];
+  saige_variant_result r;
+  r.n = (uint64_t)nv;
+  r.ac = sum;
+  r.af = sum / (2.0 * nv);
+  r.mac = fmin(sum, 2.0 * nv - sum);
+  const bool missing = nv < (double)n;  // a NaN dosage makes A g, and so g~, NaN
+  double ag[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) ag[j] = missing ? dnan() : s[2 + j];
+  double gres = s[2 + P];
+  double q2 = s[3 + P];
+  double gm = s[4 + 2 * P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    gres -= ag[j] * ec.xtr[j];
+    q2 -= 2.0 * ag[j] * s[4 + P + j];
+    gm -= ag[j] * ec.xtm[j];
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) t += ec.xm2x[j * P + k] * ag[k];
+    q2 += ag[j] * t;
+  }
+  const double score = gres / ec.tau_e;
+  double vr = ec.vr;
+  for (int c = 0; c < ec.n_cat; ++c)
+    if (r.mac <= ec.cat_upper[c]) { vr = ec.cat_vr[c]; break; }
+  const double vt = q2 * ec.tau_e;
+  const double vts = vt * vr;
+  const double tstat = vts > 1e-30 ? score * score / vts : 0.0;
+  const double pn = 1.0 - chisq1_cdf(tstat);
+  r.score = score;
+  r.var_t = vt;
+  r.var_t_star = vts;
+  r.tstat = tstat;
+  r.pvalue = pn;
+  r.pvalue_noadj = pn;
+  r.is_spa = 0;
+  r.spa_converged = 1;
+  r.spa_iters1 = 0;
+  r.spa_iters2 = 0;
+  r.variance_ratio = vr;
+  r.beta = vts > 1e-30 ? score / vts : 0.0;
+  r.se_beta = vts > 1e-30 ? 1.0 / sqrt(vts) : __longlong_as_double(0x7ff0000000000000ll);
+  if (ec.has_y) {
+    const double sc = s[5 + 2 * P], nc = s[6 + 2 * P], sn = s[7 + 2 * P];
+    const double nn = nv - nc;
+    r.af_cases = nc > 0.0 ? sc / (2.0 * nc) : dnan();
+    r.af_controls = nn > 0.0 ? sn / (2.0 * nn) : dnan();
+    r.n_cases = (uint64_t)nc;
+    r.n_controls = (uint64_t)nn;
+  } else {
+    r.af_cases = dnan();
+    r.af_controls = dnan();
+    r.n_cases = 0;
+    r.n_controls = 0;
+  }
+  if (ec.use_spa && pn < ec.cutoff) {  // single_variant.rs:183-207
+    const double q = score / sqrt(vr) + gm;
+    const double qinv = 2.0 * gm - q;
+    uint32_t slot = atomicAdd(n_gate, 1u);
+    GateInfo gi;
+    gi.var = (uint32_t)v;
+    gi.pad = 0;
+    gi.q = q;
+    gi.qinv = qinv;
+    gi.pn = pn;
+#pragma unroll
+    for (int j = 0; j < kMaxP; ++j) gi.ag[j] = j < P ? ag[j] : 0.0;
+    gate[slot] = gi;
+  }
+  out[v] = r;
+}
+
+// ---------------------------------------------------------------------------------
+// SPA
+// ---------------------------------------------------------------------------------
+constexpr int kSpaBlock = 256;
+
+__device__ __forceinline__ void block_sum4(double* v, double* sh) {
+  // sh: 4 * kSpaBlock doubles; deterministic tree
+  for (int k = 0; k < 4; ++k) sh[k * kSpaBlock + threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = kSpaBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 4; ++k) sh[k * kSpaBlock + threadIdx.x] += sh[k * kSpaBlock + threadIdx.x + w];
+    __syncthreads();
+  }
+  for (int k = 0; k < 4; ++k) v[k] = sh[k * kSpaBlock];
+  __syncthreads();
+}
+
+struct RootState {
+  double t, k1e, k2e, prev_jump, pend, root;
+  int phase;  // 0 init-eval, 1 at tn, 2 at th, 3 done
+  int it, converged;
+};
+
+// After an evaluation at the current point t (k1e = K'(t) - q, k2e = K''(t)), take the
+// Newton step of find_root_k1_fast (fast.rs:79-105); returns with either a new pending
+// evaluation point or the root decided.
+__device__ __forceinline__ void root_step(RootState& s, double tol, int max_iter) {
+  const double tn = s.t - s.k1e / s.k2e;
+  if (isnan(tn) || isinf(tn)) { s.root = s.t; s.converged = 0; s.phase = 3; return; }
+  if (fabs(tn - s.t) < tol) { s.root = tn; s.converged = 1; s.phase = 3; return; }
+  if (s.it == max_iter) { s.root = tn; s.converged = 0; s.phase = 3; return; }
+  s.pend = tn;
+  s.phase = 1;
+}
+
+__device__ __forceinline__ void root_after_eval(RootState& s, double nk1, double nk2, double tol,
+                                                int max_iter) {
+  if (s.phase == 0) {
+    s.k1e = nk1;
+    s.k2e = nk2;
+    s.it = 1;
+    root_step(s, tol, max_iter);
+    return;
+  }
+  if (s.phase == 1) {  // evaluated at tn (fast.rs:107-124)
+    const double tn = s.pend;
+    if (s.k1e * nk1 < 0.0) {
+      if (fabs(tn - s.t) > (s.prev_jump - tol)) {
+        const double dir = (nk1 - s.k1e > 0.0) ? 1.0 : -1.0;
+        const double th = s.t + dir * s.prev_jump / 2.0;
+        s.prev_jump /= 2.0;
+        s.pend = th;
+        s.phase = 2;
+        return;
+      }
+      s.prev_jump = fabs(tn - s.t);
+    }
+    s.t = tn;
+    s.k1e = nk1;
+    s.k2e = nk2;
+    s.it += 1;
+    root_step(s, tol, max_iter);
+    return;
+  }
+  // phase 2: evaluated at the halved point
+  s.t = s.pend;
+  s.k1e = nk1;
+  s.k2e = nk2;
+  s.it += 1;
+  root_step(s, tol, max_iter);
+}
+
+// g~_i for sample i of the gated variant: g_i - sum_j x_ij (A g)_j (dense.rs:102-113 order).
+template <class L, int P>
+struct GTilde {
+  L load;
+  const double* x;  // n x P col-major
+  size_t n;
+  __device__ __forceinline__ double operator()(size_t v, size_t i, const double* ag) const {
+    double xa = 0.0;
+#pragma unroll
+    for (int j = 0; j < P; ++j) xa += x[i + j * n] * ag[j];
+    return load(v, i) - xa;
+  }
+};
+
+template <class G>
+__global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __restrict__ mu,
+                                                        size_t n, const GateInfo* __restrict__ gate,
+                                                        const uint32_t* __restrict__ n_gate,
+                                                        double tol, int fast,
+                                                        saige_variant_result* __restrict__ out,
+                                                        saige_spa_result* __restrict__ single) {
+  const uint32_t slot = blockIdx.x;
+  if (slot >= *n_gate) return;
+  __shared__ double sh[4 * kSpaBlock];
+  __shared__ GateInfo g;
+  if (threadIdx.x == 0) g = gate[slot];
+  __syncthreads();
+  const size_t v = g.var;
+  const double q[2] = {g.q, g.qinv};
+  const double thr = fast ? 1e-10 : -1.0;  // fast: |g~| > 1e-10 enters K (fast.rs:191-200)
+  // setup pass: g_pos, g_neg over all samples; NA moments over the excluded samples
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (size_t i = threadIdx.x; i < n; i += kSpaBlock) {
+    const double gi = gt(v, i, g.ag);
+    const double m = mu[i];
+    if (gi > 0.0) a[0] += gi;
+    if (gi < 0.0) a[1] += gi;
+    if (!(fabs(gi) > thr)) {
+      a[2] += m * gi;
+      a[3] += m * (1.0 - m) * gi * gi;
+    }
+  }
+  block_sum4(a, sh);
+  const double gpos = a[0], gneg = a[1], na_mu = a[2], na_sigma = a[3];
+  RootState st[2];
+  for (int r = 0; r < 2; ++r) {
+    st[r].t = 0.0;
+    st[r].prev_jump = __longlong_as_double(0x7ff0000000000000ll);
+    st[r].pend = 0.0;
+    st[r].it = 0;
+    st[r].k1e = st[r].k2e = 0.0;
+    if (q[r] >= gpos || q[r] <= gneg) {  // fast.rs:67-73: root at infinity
+      st[r].root = __longlong_as_double(0x7ff0000000000000ll);
+      st[r].converged = 1;
+      st[r].phase = 3;
+    } else {
+      st[r].phase = 0;
+      st[r].converged = 0;
+      st[r].root = 0.0;
+    }
+  }
+  const int max_iter = 1000;
+  // Each pass evaluates K'(t) - q and K''(t) at both roots' pending points.
+  for (int guard = 0; guard < 4 * max_iter + 8; ++guard) {
+    if (st[0].phase == 3 && st[1].phase == 3) break;
+    const double t0 = st[0].pend, t1 = st[1].pend;
+    double b[4] = {0.0, 0.0, 0.0, 0.0};
+    for (size_t i = threadIdx.x; i < n; i += kSpaBlock) {
+      const double gi = gt(v, i, g.ag);
+      if (!(fabs(gi) > thr)) continue;
+      const double m = mu[i];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const double tt = r == 0 ? t0 : t1;
+        const double e = exp(-gi * tt);
+        const double den = (1.0 - m) * e + m;
+        b[2 * r] += m * gi / den;
+        b[2 * r + 1] += (1.0 - m) * m * gi * gi * e / (den * den);
+      }
+    }
+    block_sum4(b, sh);
+    for (int r = 0; r < 2; ++r) {
+      if (st[r].phase == 3) continue;
+      const double tt = r == 0 ? t0 : t1;
+      const double k1 = b[2 * r] + na_mu + na_sigma * tt - q[r];
+      const double k2 = b[2 * r + 1] + na_sigma;
+      root_after_eval(st[r], k1, k2, tol, max_iter);
+    }
+  }
+  const bool both = st[0].converged && st[1].converged;
+  double p1 = 0.0, p2 = 0.0;
+  bool ok1 = false, ok2 = false;
+  if (both) {
+    // saddle_probability_fast (fast.rs:135-180): K(zeta), K''(zeta) at both roots
+    const double z0 = st[0].root, z1 = st[1].root;
+    double c[4] = {0.0, 0.0, 0.0, 0.0};
+    for (size_t i = threadIdx.x; i < n; i += kSpaBlock) {
+      const double gi = gt(v, i, g.ag);
+      if (!(fabs(gi) > thr)) continue;
+      const double m = mu[i];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const double zz = r == 0 ? z0 : z1;
+        c[2 * r] += log(1.0 - m + m * exp(gi * zz));
+        const double e = exp(-gi * zz);
+        const double den = (1.0 - m) * e + m;
+        c[2 * r + 1] += (1.0 - m) * m * gi * gi * e / (den * den);
+      }
+    }
+    block_sum4(c, sh);
+    for (int r = 0; r < 2; ++r) {
+      const double zeta = r == 0 ? z0 : z1;
+      const double k0 = c[2 * r] + na_mu * zeta + 0.5 * na_sigma * zeta * zeta;
+      const double k2 = c[2 * r + 1] + na_sigma;
+      const double temp1 = zeta * q[r] - k0;
+      double pv = 0.0;
+      bool ok = false;
+      if (isfinite(k0) && isfinite(k2) && !(temp1 < 0.0) && !(k2 < 0.0)) {
+        const double sg = zeta > 0.0 ? 1.0 : (zeta < 0.0 ? -1.0 : (signbit(zeta) ? -1.0 : 1.0));
+        const double w = (isnan(zeta) ? zeta : sg) * sqrt(2.0 * temp1);
+        const double vv = zeta * sqrt(k2);
+        if (w != 0.0) {
+          const double z = w + (1.0 / w) * log(vv / w);
+          pv = z > 0.0 ? 1.0 - norm_cdf(z) : -norm_cdf(z);
+          ok = true;
+        }
+      }
+      if (r == 0) { p1 = pv; ok1 = ok; } else { p2 = pv; ok2 = ok; }
+    }
+  }
+  if (threadIdx.x == 0) {
+    double pval;
+    int conv, is_spa;
+    if (both) {
+      const double a1 = ok1 ? p1 : g.pn / 2.0;
+      const double a2 = ok2 ? p2 : g.pn / 2.0;
+      pval = fabs(a1) + fabs(a2);
+      conv = 1;
+      is_spa = (ok1 || ok2) ? 1 : 0;
+    } else {
+      pval = g.pn;
+      conv = 0;
+      is_spa = 0;
+    }
+    if (out) {
+      out[v].pvalue = pval;
+      out[v].is_spa = is_spa;
+      out[v].spa_converged = conv;
+      out[v].spa_iters1 = st[0].it;
+      out[v].spa_iters2 = st[1].it;
+    }
+    if (single) {
+      single->pvalue = pval;
+      single->converged = conv;
+      single->is_spa = is_spa;
+      single->iters1 = st[0].it;
+      single->iters2 = st[1].it;
+      single->root1 = st[0].root;
+      single->root2 = st[1].root;
+    }
+  }
+}
+
+}  // namespace saige
+
+using namespace saige;
+
+struct saige_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  size_t n = 0, p = 0;
+  EngineConst ec{};
+  DevBuf feat, mu_d, x_d, idx, part, res, gate, ngate, in;
+};
+
+namespace saige {
+
+template <int P, class L>
+static int run_batch_p(saige_engine* e, L load, size_t nvar, saige_variant_result* d_out) {
+  hipStream_t s = e->stream;
+  const size_t n = e->n;
+  constexpr int NS = Layout<P>::NS;
+  const size_t vblocks = ceil_div(nvar, 64);
+  size_t splits = std::max<size_t>(1, std::min<size_t>(ceil_div(2048, vblocks), ceil_div(n, 512)));
+  size_t sps = ceil_div(n, splits);
+  splits = ceil_div(n, sps);
+  const size_t nvar_pad = round_up(nvar, 64);
+  SAIGE_TRY(e->part.ensure(splits * NS * nvar_pad * 8));
+  SAIGE_TRY(e->gate.ensure(nvar * sizeof(GateInfo)));
+  SAIGE_TRY(e->ngate.ensure(64));
+  SAIGE_HIP(hipMemsetAsync(e->ngate.p, 0, 4, s));
+  score_sums_kernel<P, L><<<dim3((unsigned)vblocks, (unsigned)splits), 64, 0, s>>>(
+      load, e->feat.as<double>(), n, nvar, sps, e->part.as<double>(), nvar_pad);
+  score_finalize_kernel<P><<<(unsigned)ceil_div(nvar, 128), 128, 0, s>>>(
+      e->part.as<double>(), splits, nvar, nvar_pad, n, e->ec, d_out, e->gate.as<GateInfo>(),
+      e->ngate.as<uint32_t>());
+  SAIGE_HIP(hipGetLastError());
+  if (e->ec.use_spa) {
+    GTilde<L, P> gt{load, e->x_d.as<double>(), n};
+    spa_kernel<GTilde<L, P>><<<(unsigned)nvar, kSpaBlock, 0, s>>>(
+        gt, e->mu_d.as<double>(), n, e->gate.as<GateInfo>(), e->ngate.as<uint32_t>(),
+        e->ec.spa_tol, e->ec.fast, d_out, nullptr);
+    SAIGE_HIP(hipGetLastError());
+  }
+  return SAIGE_OK;
+}
+
+template <class L>
+static int run_batch(saige_engine* e, L load, size_t nvar, saige_variant_result* d_out) {
+  switch (e->p) {
+    case 1: return run_batch_p<1>(e, load, nvar, d_out);
+    case 2: return run_batch_p<2>(e, load, nvar, d_out);
+    case 3: return run_batch_p<3>(e, load, nvar, d_out);
+    case 4: return run_batch_p<4>(e, load, nvar, d_out);
+    case 5: return run_batch_p<5>(e, load, nvar, d_out);
+    case 6: return run_batch_p<6>(e, load, nvar, d_out);
+    case 7: return run_batch_p<7>(e, load, nvar, d_out);
+    case 8: return run_batch_p<8>(e, load, nvar, d_out);
+  }
+  set_error("p = %zu covariates not supported (1..8)", e->p);
+  return SAIGE_ERR_ARG;
+}
+
+constexpr size_t kHostBatch = 4096;
+
+}  // namespace saige
+
+extern "C" {
+
+int saige_engine_create(int device, const double* mu, const double* mu2, const double* residuals,
+                        double tau_e, double tau_g, const double* a_flat, const double* x,
+                        size_t n, size_t p, double variance_ratio, const double* cat_upper,
+                        const double* cat_vr, size_t n_cat, int use_spa, int use_fast_spa,
+                        double spa_tol, double spa_pval_cutoff, const double* y,
+                        saige_engine** out) {
+  clear_error();
+  SAIGE_ARG(out && mu && mu2 && residuals && a_flat && x, "null pointer");
+  *out = nullptr;
+  SAIGE_ARG(n > 0, "n > 0");
+  SAIGE_ARG(p >= 1 && p <= (size_t)kMaxP, "1 <= p <= 8");
+  SAIGE_ARG(n_cat <= (size_t)kMaxCat, "at most 16 categorical ratios");
+  SAIGE_ARG(n_cat == 0 || (cat_upper && cat_vr), "categorical arrays");
+  (void)tau_g;
+  SAIGE_TRY(use_device(device));
+  saige_engine* e = new saige_engine();
+  e->device = device;
+  e->n = n;
+  e->p = p;
+  if (hipStreamCreate(&e->stream) != hipSuccess) {
+    delete e;
+    set_error("hipStreamCreate failed");
+    return SAIGE_ERR_DEVICE;
+  }
+  EngineConst& ec = e->ec;
+  ec.tau_e = tau_e;
+  ec.vr = variance_ratio;
+  ec.cutoff = spa_pval_cutoff;
+  ec.spa_tol = spa_tol;
+  ec.use_spa = use_spa ? 1 : 0;
+  ec.fast = use_fast_spa ? 1 : 0;
+  ec.n_cat = (int)n_cat;
+  ec.has_y = y ? 1 : 0;
+  for (size_t c = 0; c < n_cat; ++c) {
+    ec.cat_upper[c] = cat_upper[c];
+    ec.cat_vr[c] = cat_vr[c];
+  }
+  // engine constants X'res, X'mu, X'M2X
+  for (size_t j = 0; j < p; ++j) {
+    double a = 0.0, b = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      a += x[j * n + i] * residuals[i];
+      b += x[j * n + i] * mu[i];
+    }
+    ec.xtr[j] = a;
+    ec.xtm[j] = b;
+    for (size_t k = 0; k < p; ++k) {
+      double c = 0.0;
+      for (size_t i = 0; i < n; ++i) c += x[j * n + i] * mu2[i] * x[k * n + i];
+      ec.xm2x[j * p + k] = c;
+    }
+  }
+  const size_t nf = 2 * p + 4;
+  std::vector<double> feat(n * nf);
+  for (size_t i = 0; i < n; ++i) {
+    double* f = &feat[i * nf];
+    for (size_t r = 0; r < p; ++r) f[r] = a_flat[i * p + r];  // col-major p x n view
+    f[p] = residuals[i];
+    f[p + 1] = mu2[i];
+    for (size_t j = 0; j < p; ++j) f[p + 2 + j] = mu2[i] * x[j * n + i];
+    f[2 * p + 2] = mu[i];
+    f[2 * p + 3] = (y && y[i] > 0.5) ? 1.0 : 0.0;
+  }
+  int rc = SAIGE_OK;
+  auto fail = [&](int code) {
+    saige_engine_destroy(e);
+    return code;
+  };
+  if ((rc = e->feat.ensure(feat.size() * 8)) != SAIGE_OK) return fail(rc);
+  if ((rc = e->mu_d.ensure(n * 8)) != SAIGE_OK) return fail(rc);
+  if ((rc = e->x_d.ensure(n * p * 8)) != SAIGE_OK) return fail(rc);
+  if (hipMemcpy(e->feat.p, feat.data(), feat.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->mu_d.p, mu, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->x_d.p, x, n * p * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("engine upload failed");
+    return fail(SAIGE_ERR_DEVICE);
+  }
+  *out = e;
+  return SAIGE_OK;
+}
+
+void saige_engine_destroy(saige_engine* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) {
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipStreamDestroy(e->stream);
+  }
+  delete e;
+}
+
+void* saige_engine_stream(saige_engine* e) { return e ? (void*)e->stream : nullptr; }
+
+int saige_test_markers_u8_device(saige_engine* e, const uint8_t* d_q, size_t ldq, size_t nvar,
+                                 double scale, saige_variant_result* d_out) {
+  clear_error();
+  SAIGE_ARG(e && (nvar == 0 || (d_q && d_out)), "null pointer");
+  SAIGE_ARG(ldq >= e->n, "ldq < n");
+  if (nvar == 0) return SAIGE_OK;
+  std::lock_guard<std::mutex> lk(e->mu);
+  SAIGE_HIP(hipSetDevice(e->device));
+  return run_batch(e, LoadU8{d_q, ldq, scale}, nvar, d_out);
+}
+
+int saige_test_markers_u8(saige_engine* e, const uint8_t* q, size_t ldq, size_t nvar, double scale,
+                          saige_variant_result* out) {
+  clear_error();
+  SAIGE_ARG(e && (nvar == 0 || (q && out)), "null pointer");
+  SAIGE_ARG(ldq >= e->n, "ldq < n");
+  std::lock_guard<std::mutex> lk(e->mu);
+  SAIGE_HIP(hipSetDevice(e->device));
+  hipStream_t s = e->stream;
+  for (size_t v0 = 0; v0 < nvar; v0 += kHostBatch) {
+    size_t nb = std::min(kHostBatch, nvar - v0);
+    SAIGE_TRY(e->in.ensure(nb * e->n + nb * sizeof(saige_variant_result) + 256));
+    uint8_t* dq = e->in.as<uint8_t>();
+    saige_variant_result* dr =
+        reinterpret_cast<saige_variant_result*>(e->in.as<uint8_t>() + round_up(nb * e->n, 256));
+    SAIGE_HIP(hipMemcpy2DAsync(dq, e->n, q + v0 * ldq, ldq, e->n, nb, hipMemcpyHostToDevice, s));
+    SAIGE_TRY(run_batch(e, LoadU8{dq, e->n, scale}, nb, dr));
+    SAIGE_HIP(hipMemcpyAsync(out + v0, dr, nb * sizeof(saige_variant_result), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipStreamSynchronize(s));
+  }
+  return SAIGE_OK;
+}
+
+int saige_test_markers_f64(saige_engine* e, const double* g, size_t ldg, size_t nvar,
+                           saige_variant_result* out) {
+  clear_error();
+  SAIGE_ARG(e && (nvar == 0 || (g && out)), "null pointer");
+  SAIGE_ARG(ldg >= e->n, "ldg < n");
+  std::lock_guard<std::mutex> lk(e->mu);
+  SAIGE_HIP(hipSetDevice(e->device));
+  hipStream_t s = e->stream;
+  const size_t n = e->n;
+  for (size_t v0 = 0; v0 < nvar; v0 += kHostBatch) {
+    size_t nb = std::min(kHostBatch, nvar - v0);
+    SAIGE_TRY(e->in.ensure(nb * n * 8 + nb * sizeof(saige_variant_result) + 256));
+    double* dg = e->in.as<double>();
+    saige_variant_result* dr =
+        reinterpret_cast<saige_variant_result*>(e->in.as<uint8_t>() + round_up(nb * n * 8, 256));
+    SAIGE_HIP(hipMemcpy2DAsync(dg, n * 8, g + v0 * ldg, ldg * 8, n * 8, nb, hipMemcpyHostToDevice, s));
+    SAIGE_TRY(run_batch(e, LoadF64{dg, n}, nb, dr));
+    SAIGE_HIP(hipMemcpyAsync(out + v0, dr, nb * sizeof(saige_variant_result), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipStreamSynchronize(s));
+  }
+  return SAIGE_OK;
+}
+
+int saige_test_markers_plink(saige_engine* e, const uint8_t* bed_body, size_t bpm, size_t n_total,
+                             const uint32_t* sample_idx, size_t nvar,
+                             saige_variant_result* out) {
+  clear_error();
+  SAIGE_ARG(e && (nvar == 0 || (bed_body && out)), "null pointer");
+  SAIGE_ARG(bpm >= (n_total + 3) / 4, "bytes_per_marker < ceil(n_total/4)");
+  if (!sample_idx) SAIGE_ARG(e->n == n_total, "n must equal n_total without a subset");
+  std::lock_guard<std::mutex> lk(e->mu);
+  SAIGE_HIP(hipSetDevice(e->device));
+  hipStream_t s = e->stream;
+  if (sample_idx) {
+    for (size_t k = 0; k < e->n; ++k)
+      SAIGE_ARG(sample_idx[k] < n_total, "sample_idx out of range");
+    SAIGE_TRY(e->idx.ensure(e->n * 4));
+    SAIGE_HIP(hipMemcpyAsync(e->idx.p, sample_idx, e->n * 4, hipMemcpyHostToDevice, s));
+  }
+  for (size_t v0 = 0; v0 < nvar; v0 += kHostBatch) {
+    size_t nb = std::min(kHostBatch, nvar - v0);
+    SAIGE_TRY(e->in.ensure(nb * bpm + nb * sizeof(saige_variant_result) + 256));
+    uint8_t* db = e->in.as<uint8_t>();
+    saige_variant_result* dr =
+        reinterpret_cast<saige_variant_result*>(e->in.as<uint8_t>() + round_up(nb * bpm, 256));
+    SAIGE_HIP(hipMemcpyAsync(db, bed_body + v0 * bpm, nb * bpm, hipMemcpyHostToDevice, s));
+    SAIGE_TRY(run_batch(e, LoadPacked{db, bpm, sample_idx ? e->idx.as<uint32_t>() : nullptr}, nb, dr));
+    SAIGE_HIP(hipMemcpyAsync(out + v0, dr, nb * sizeof(saige_variant_result), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipStreamSynchronize(s));
+  }
+  return SAIGE_OK;
+}
+
+int saige_spa_binary(int device, const double* mu, const double* g, size_t n, double q,
+                     double qinv, double pval_noadj, double tol, int fast,
+                     saige_spa_result* out) {
+  clear_error();
+  SAIGE_ARG(mu && g && out && n > 0, "null pointer");
+  SAIGE_TRY(use_device(device));
+  hipStream_t s;
+  SAIGE_HIP(hipStreamCreate(&s));
+  int rc = SAIGE_OK;
+  {
+    DevBuf dmu, dg, dgate, dn, dres;
+    rc = dmu.ensure(n * 8);
+    if (rc == SAIGE_OK) rc = dg.ensure(n * 8);
+    if (rc == SAIGE_OK) rc = dgate.ensure(sizeof(GateInfo));
+    if (rc == SAIGE_OK) rc = dn.ensure(8);
+    if (rc == SAIGE_OK) rc = dres.ensure(sizeof(saige_spa_result));
+    if (rc == SAIGE_OK) {
+      GateInfo gi{};
+      gi.var = 0;
+      gi.q = q;
+      gi.qinv = qinv;
+      gi.pn = pval_noadj;
+      uint32_t one = 1;
+      hipError_t err = hipMemcpyAsync(dmu.p, mu, n * 8, hipMemcpyHostToDevice, s);
+      if (err == hipSuccess) err = hipMemcpyAsync(dg.p, g, n * 8, hipMemcpyHostToDevice, s);
+      if (err == hipSuccess) err = hipMemcpyAsync(dgate.p, &gi, sizeof(gi), hipMemcpyHostToDevice, s);
+      if (err == hipSuccess) err = hipMemcpyAsync(dn.p, &one, 4, hipMemcpyHostToDevice, s);
+      if (err == hipSuccess) {
+        GTilde<LoadF64, 0> gt{LoadF64{dg.as<double>(), n}, nullptr, n};
+        spa_kernel<GTilde<LoadF64, 0>><<<1, kSpaBlock, 0, s>>>(gt, dmu.as<double>(), n,
+                                                               dgate.as<GateInfo>(), dn.as<uint32_t>(),
+                                                               tol, fast, nullptr,
+                                                               dres.as<saige_spa_result>());
+        err = hipGetLastError();
+      }
+      if (err == hipSuccess) err = hipMemcpyAsync(out, dres.p, sizeof(saige_spa_result), hipMemcpyDeviceToHost, s);
+      if (err == hipSuccess) err = hipStreamSynchronize(s);
+      if (err != hipSuccess) {
+        set_error("spa: %s", hipGetErrorString(err));
+        rc = SAIGE_ERR_DEVICE;
+      }
+    }
+  }
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+"""Seeded synthetic cohorts (harness): ctypes wrapper of libsaige_synth.so."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "libsaige_synth.so")
+_lib = None
+
+dp = C.POINTER(C.c_double)
+u8p = C.POINTER(C.c_uint8)
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            raise ImportError(f"{_SO} missing: run python csaige_amd/build.py")
+        L = C.CDLL(_SO)
+        L.synth_unrelated.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_double,
+                                      u8p, u8p, dp, dp, dp, dp, dp]
+        L.synth_families.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_double,
+                                     u8p, dp, dp, dp]
+        L.synth_null_fit.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, C.c_double, C.c_double,
+                                     dp, dp, dp]
+        L.synth_dosage_u8.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                      C.c_double, C.c_double, u8p, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Cohort(dict):
+    """bed: (m, bpm) uint8 .bed body; x: (p, n); y: (n,)."""
+
+    __getattr__ = dict.__getitem__
+
+
+def unrelated(n=2000, m_grm=5000, m_test=2000, seed=1, maf=(0.01, 0.5), h2=0.3, prevalence=0.10,
+              n_cov=2, cov_beta=0.5):
+    """Unrelated HWE samples (the 2,000-sample workload)."""
+    L = _load()
+    bpm = (n + 3) // 4
+    bed = np.empty((m_grm, bpm), np.uint8)
+    bed_t = np.empty((max(m_test, 1), bpm), np.uint8)
+    mg, mt = np.empty(m_grm), np.empty(max(m_test, 1))
+    y = np.empty(n)
+    p = 1 + n_cov
+    x = np.empty((p, n))
+    liab = np.empty(n)
+    L.synth_unrelated(seed, n, m_grm, m_test, maf[0], maf[1], h2, prevalence, n_cov, cov_beta,
+                      _p(bed, u8p), _p(bed_t, u8p) if m_test else None, _p(mg, dp),
+                      _p(mt, dp) if m_test else None, _p(y, dp), _p(x, dp), _p(liab, dp))
+    return Cohort(n=n, bpm=bpm, bed=bed, bed_test=bed_t[:m_test], maf=mg, maf_test=mt[:m_test],
+                  y=y, x=x, liability=liab)
+
+
+def families(n_fam=10000, kids=3, m=100000, seed=2, maf=(0.01, 0.5), h2=0.4, prevalence=0.05,
+             n_cov=2, cov_beta=0.5):
+    """Nuclear families (2 founders + kids offspring each)."""
+    L = _load()
+    n = n_fam * (2 + kids)
+    bpm = (n + 3) // 4
+    bed = np.empty((m, bpm), np.uint8)
+    mf = np.empty(m)
+    y = np.empty(n)
+    x = np.empty((1 + n_cov, n))
+    L.synth_families(seed, n_fam, kids, m, maf[0], maf[1], h2, prevalence, n_cov, cov_beta,
+                     _p(bed, u8p), _p(mf, dp), _p(y, dp), _p(x, dp))
+    return Cohort(n=n, bpm=bpm, bed=bed, maf=mf, y=y, x=x)
+
+
+def null_fit(n=400000, p=1, eta_mean=-4.6, eta_sd=0.5, seed=3):
+    L = _load()
+    mu, y = np.empty(n), np.empty(n)
+    x = np.empty((p, n))
+    L.synth_null_fit(seed, n, p, eta_mean, eta_sd, _p(mu, dp), _p(y, dp), _p(x, dp))
+    return Cohort(n=n, mu=mu, y=y, x=x)
+
+
+def dosage_u8(n, first, nvar, seed=4, maf=(1e-4, 0.5), ld=None, out=None):
+    L = _load()
+    ld = n if ld is None else ld
+    q = np.empty((nvar, ld), np.uint8) if out is None else out
+    mafs = np.empty(nvar)
+    L.synth_dosage_u8(seed, n, first, nvar, ld, maf[0], maf[1], _p(q, u8p), _p(mafs, dp))
+    return q, mafs

@@ -1,0 +1,275 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same seeded inputs.
+
+Bars: integer work (decode, allele counts, MAC, marker split) bit-exact; fp64 GRM products
+to 1e-12 relative; PCG iterate-identical (same iteration counts) with x to 1e-9; tau and
+the variance ratio to 1e-6 relative; step-2 -log10 p to 1e-6 on the parity mask p >= 1e-10.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import csaige_amd as sg
+import oracle
+from csaige_amd import synth
+from tests.helpers import decode_np, neglog10, random_bed, rel
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+# --------------------------------------------------------------------------- decode / AF
+def test_marker_stats_golden_frq():
+    f = np.load(os.path.join(GOLD, "frq_subset.npz"))
+    n = int(f["n"])
+    ac, nv, af, mac = sg.marker_stats(f["bed"], f["bed"].shape[1], n, len(f["bed"]))
+    np.testing.assert_array_equal(ac, np.round(f["maf"] * f["nchrobs"]))
+    np.testing.assert_array_equal(nv * 2, f["nchrobs"])
+    np.testing.assert_array_equal(mac, np.minimum(ac, 2.0 * nv - ac))
+
+
+def test_marker_stats_golden_r_saige_ac_subset():
+    d = np.load(os.path.join(GOLD, "ac100.npz"))
+    ac, nv, af, mac = sg.marker_stats(d["bed"], d["bed"].shape[1], int(d["n_total"]), len(d["bed"]),
+                                      sample_idx=d["sample_idx"])
+    np.testing.assert_array_equal(ac, d["ac"])
+    assert (nv == 1000).all()
+
+
+@pytest.mark.parametrize("n,subset", [(37, False), (1031, True), (4, False)])
+def test_decode_and_stats_bitexact(n, subset):
+    body, bpm = random_bed(n, 9, seed=n, p_missing=0.1)
+    idx = np.random.default_rng(n).permutation(n)[: max(1, n - 5)].astype(np.uint32) if subset else None
+    got = sg.decode(body, bpm, n, 9, sample_idx=idx)
+    for j in range(9):
+        want = oracle.decode_marker(body[j], n, idx)
+        np.testing.assert_array_equal(got[j], want)
+    ac, nv, af, mac = sg.marker_stats(body, bpm, n, 9, sample_idx=idx)
+    for j in range(9):
+        s, a, m_, k = oracle.compute_af(oracle.decode_marker(body[j], n, idx))
+        assert (ac[j], nv[j], af[j], mac[j]) == (s, k, a, m_)
+
+
+# --------------------------------------------------------------------------- GRM product
+def _grm_pair(n, m, seed, p_missing=0.03, mono=False):
+    body, bpm = random_bed(n, m, seed, p_missing)
+    if mono:
+        body[m // 2] = 0xFF  # all 0 dosage -> af 0 -> monomorphic
+    d = decode_np(body, n)
+    af = np.array([oracle.compute_af(r)[1] for r in d])
+    return body, bpm, d, af
+
+
+@pytest.mark.parametrize("n,m,k", [(600, 777, 1), (1000, 300, 5), (513, 1100, 34), (300, 257, 70)])
+def test_grm_fp64_vs_oracle(n, m, k):
+    _say(f"grm n={n} m={m} k={k}")
+    body, bpm, d, af = _grm_pair(n, m, seed=n + m, mono=True)
+    g = sg.OnTheFlyGrm(body, bpm, n, af)
+    ref = oracle.Grm(d, af, oracle.GRM_F64)
+    V = np.random.default_rng(k).standard_normal((k, n))
+    got = g.mat_mat(V)
+    for c in range(k):
+        want = ref.mat_vec(V[c])
+        np.testing.assert_allclose(got[c], want, rtol=0, atol=1e-12 * np.abs(want).max())
+    np.testing.assert_allclose(g.mat_vec(V[0]), got[0], rtol=0, atol=0)  # deterministic
+
+
+def test_grm_subset_and_marker_selection():
+    n_total, m_total = 700, 400
+    body, bpm = random_bed(n_total, m_total, seed=5, p_missing=0.02)
+    idx = np.sort(np.random.default_rng(1).choice(n_total, 650, replace=False)).astype(np.uint32)[::-1].copy()
+    markers = np.arange(3, m_total, 3, dtype=np.uint32)
+    d = np.stack([oracle.decode_marker(body[j], n_total, idx) for j in markers])
+    af = np.array([oracle.compute_af(r)[1] for r in d])
+    g = sg.OnTheFlyGrm(body, bpm, n_total, af, sample_idx=idx, marker_idx=markers)
+    v = np.random.default_rng(2).standard_normal(len(idx))
+    want = oracle.Grm(d, af, oracle.GRM_F64).mat_vec(v)
+    np.testing.assert_allclose(g.mat_vec(v), want, atol=1e-12 * np.abs(want).max(), rtol=0)
+
+
+def test_grm_symmetric():
+    n = 200
+    body, bpm, d, af = _grm_pair(n, 300, seed=9)
+    g = sg.OnTheFlyGrm(body, bpm, n, af)
+    K = g.mat_mat(np.eye(n))
+    np.testing.assert_allclose(K, K.T, atol=1e-13)
+
+
+# --------------------------------------------------------------------------- PCG
+def test_pcg_block_iterate_identical():
+    n, m = 800, 1200
+    body, bpm, d, af = _grm_pair(n, m, seed=17, p_missing=0)
+    g = sg.OnTheFlyGrm(body, bpm, n, af)
+    ref = oracle.Grm(d, af, oracle.GRM_F64)
+    rng = np.random.default_rng(3)
+    w = rng.uniform(0.02, 0.25, n)
+    tau = (1.0, 0.8)
+    B = rng.standard_normal((6, n))
+    B[2] = 0.0  # converges at iteration 0
+    X, its, res, conv = sg.solve_sigma(g, w, tau, B, 1e-5, 500)
+    for c in range(6):
+        x, it, r, cv = oracle.pcg_solve_sigma(ref, w, tau, B[c], 1e-5, 500)
+        assert its[c] == it and conv[c] == cv, (c, its[c], it)
+        np.testing.assert_allclose(X[c], x, rtol=0, atol=1e-9 * max(np.abs(x).max(), 1e-30))
+    # max_iter stop: not converged, iterations = max_iter
+    X, its, res, conv = sg.solve_sigma(g, w, tau, B[:1], 1e-12, 3)
+    x, it, r, cv = oracle.pcg_solve_sigma(ref, w, tau, B[0], 1e-12, 3)
+    assert its[0] == it == 3 and not conv[0] and not cv
+    np.testing.assert_allclose(X[0], x, atol=1e-10 * np.abs(x).max(), rtol=0)
+
+
+# --------------------------------------------------------------------------- AI-REML
+@pytest.mark.parametrize("trait", [sg.TRAIT_BINARY, sg.TRAIT_QUANTITATIVE])
+def test_fit_ai_reml_vs_oracle(trait):
+    c = synth.unrelated(n=500, m_grm=1500, m_test=0, seed=7, h2=0.5, prevalence=0.2)
+    d = decode_np(c.bed, c.n)
+    af = np.array([oracle.compute_af(r)[1] for r in d])
+    y = c.y if trait == sg.TRAIT_BINARY else c.liability
+    g = sg.OnTheFlyGrm(c.bed, c.bpm, c.n, af)
+    got = sg.fit_ai_reml(g, y, c.x, trait=trait, tol=1e-5, n_random_vectors=30)
+    want = oracle.fit_ai_reml(oracle.Grm(d, af, oracle.GRM_F64), y, c.x, trait=trait, tol=1e-5)
+    _say(f"tau gpu={got['tau']} oracle={want['tau']} it={got['iterations']}/{want['iterations']}")
+    assert got["iterations"] == want["iterations"]
+    assert got["converged"] == want["converged"]
+    assert rel(got["tau"], want["tau"]).max() < 1e-6
+    np.testing.assert_allclose(got["mu"], want["mu"], rtol=1e-6)
+    np.testing.assert_allclose(got["alpha"], want["alpha"], rtol=1e-6, atol=1e-9)
+    assert abs(got["log_likelihood"] - want["log_likelihood"]) < 1e-6 * abs(want["log_likelihood"])
+
+
+# --------------------------------------------------------------------------- fit-null flow
+def test_fit_null_vs_oracle():
+    c = synth.unrelated(n=600, m_grm=2000, m_test=0, seed=11, h2=0.5, prevalence=0.2)
+    idx = np.arange(c.n, dtype=np.uint32)
+    got = sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], c.y, c.x, sample_idx=idx)
+    want = oracle.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], idx, c.y, c.x)
+    _say(f"fit_null gpu tau={got['tau']} vr={got['vr']} oracle tau={want['tau']} vr={want['vr']}")
+    np.testing.assert_array_equal(got["marker_class"], want["marker_class"])
+    assert (got["n_grm_markers"], got["n_vr_markers"]) == (want["n_grm_markers"], want["n_vr_markers"])
+    assert got["iterations"] == want["iterations"]
+    assert rel(got["tau"], want["tau"]).max() < 1e-6
+    assert got["vr_markers_used"] == want["vr_markers_used"]
+    assert abs(got["vr"] - want["vr"]) < 1e-6 * want["vr"]
+    np.testing.assert_allclose(got["mu"], want["mu"], rtol=1e-6)
+    np.testing.assert_allclose(got["xvx_inv_xv"], want["xvx_inv_xv"], rtol=1e-5, atol=1e-12)
+
+
+# --------------------------------------------------------------------------- step 2
+def _null(n, p, seed, eta=-2.5):
+    rng = np.random.default_rng(seed)
+    X = np.column_stack([np.ones(n)] + [rng.standard_normal(n) for _ in range(p - 1)])
+    mu = 1 / (1 + np.exp(-(eta + 0.3 * X[:, -1] + 0.4 * rng.standard_normal(n))))
+    y = (rng.random(n) < mu).astype(float)
+    a = oracle.xvx_inv_xv(X.T, mu * (1 - mu))
+    return X, mu, y, a
+
+
+def _compare(got, want, mask_p=1e-10):
+    for f in ("ac", "af", "mac", "n", "n_cases", "n_controls", "af_cases", "af_controls"):
+        np.testing.assert_array_equal(got[f], want[f], err_msg=f)
+    for f in ("score", "var_t", "var_t_star", "beta"):
+        np.testing.assert_allclose(got[f], want[f], rtol=1e-8, atol=1e-12, err_msg=f)
+    np.testing.assert_array_equal(got["is_spa"], want["is_spa"])
+    np.testing.assert_array_equal(got["spa_converged"], want["spa_converged"])
+    m = want["pvalue"] >= mask_p
+    err = np.abs(neglog10(got["pvalue"][m]) - neglog10(want["pvalue"][m]))
+    assert err.max() < 1e-6, err.max()
+    m2 = want["pvalue_noadj"] >= mask_p
+    err2 = np.abs(neglog10(got["pvalue_noadj"][m2]) - neglog10(want["pvalue_noadj"][m2]))
+    assert err2.max() < 1e-6
+
+
+@pytest.mark.parametrize("p", [1, 3])
+def test_score_u8_vs_oracle(p):
+    n = 3000
+    X, mu, y, a = _null(n, p, seed=p)
+    q, _ = synth.dosage_u8(n, 0, 300, seed=5, maf=(2e-3, 0.5))
+    # plant strong signals so that SPA is exercised on both tails
+    rng = np.random.default_rng(0)
+    for v in range(0, 300, 7):
+        q[v, (y > 0.5) & (rng.random(n) < 0.3)] = 254
+    vr = 1.05
+    e = sg.ScoreTestEngine(mu, y - mu, (1.0, 0.3), X.T, a, vr, y=y, spa_pval_cutoff=0.05)
+    got = e.test_markers_u8(q, 1.0 / 127.0)
+    eo = oracle.ScoreEngine(mu, y - mu, (1.0, 0.3), X.T, a, vr, y=y, spa_cutoff=0.05)
+    want = eo.test_batch_u8(q, 1.0 / 127.0)
+    _say(f"p={p} spa'd {int(want['is_spa'].sum())} of {len(q)}; min p {want['pvalue'].min():.3g}")
+    assert want["is_spa"].sum() > 10
+    _compare(got, want)
+
+
+def test_score_f64_missing_and_categorical_vr():
+    n = 700
+    X, mu, y, a = _null(n, 2, seed=9)
+    rng = np.random.default_rng(1)
+    G = rng.binomial(2, rng.uniform(0.001, 0.3, (40, 1)), (40, n)).astype(float)
+    G[5, 10] = np.nan
+    G[6] = 0.0
+    G[7, y > 0.5] = 2.0
+    cat = [(1.5, 0.9), (10.5, 1.2), (math.inf, 1.0)]
+    e = sg.ScoreTestEngine(mu, y - mu, (1.0, 0.3), X.T, a, 1.0, y=y, categorical_vr=cat)
+    got = e.test_markers(G)
+    eo = oracle.ScoreEngine(mu, y - mu, (1.0, 0.3), X.T, a, 1.0, y=y, cat_vr=cat)
+    want_rows = [eo.test_marker(g) for g in G]
+    want = {k: np.array([r[k] for r in want_rows]) for k in want_rows[0]}
+    assert np.isnan(got["score"][5]) and got["pvalue"][5] == 1.0
+    got["score"][5] = want["score"][5] = 0.0
+    got["var_t"][5] = want["var_t"][5] = 0.0
+    got["var_t_star"][5] = want["var_t_star"][5] = 0.0
+    _compare(got, want)
+
+
+def test_score_plink_hard_calls():
+    n_total = 900
+    body, bpm = random_bed(n_total, 120, seed=4)
+    idx = np.arange(n_total - 1, -1, -1, dtype=np.uint32)[:850].copy()
+    X, mu, y, a = _null(len(idx), 3, seed=8)
+    e = sg.ScoreTestEngine(mu, y - mu, (1.0, 0.3), X.T, a, 1.1, y=y)
+    got = e.test_markers_plink(body, bpm, n_total, 120, sample_idx=idx)
+    eo = oracle.ScoreEngine(mu, y - mu, (1.0, 0.3), X.T, a, 1.1, y=y)
+    rows = [eo.test_marker(oracle.decode_marker(body[j], n_total, idx)) for j in range(120)]
+    want = {k: np.array([r[k] for r in rows]) for k in rows[0]}
+    _compare(got, want)
+
+
+def test_spa_single_vs_oracle():
+    rng = np.random.default_rng(6)
+    n = 2000
+    mu = rng.uniform(0.005, 0.1, n)
+    g = rng.binomial(2, 0.01, n) - 0.02 * rng.random(n)
+    m1 = mu @ g
+    sd = math.sqrt(np.sum(mu * (1 - mu) * g * g))
+    for z in (2.5, 4.0, 8.0, -3.0):
+        q = m1 + z * sd
+        for fast in (True, False):
+            a = sg.spa_binary_fast(mu, g, q, 2 * m1 - q, 0.01, fast=fast)
+            b = oracle.spa_binary(mu, g, q, 2 * m1 - q, 0.01, fast=fast)
+            assert (a["converged"], a["is_spa"], a["iters1"], a["iters2"]) == \
+                   (b["converged"], b["is_spa"], b["iters1"], b["iters2"])
+            assert abs(neglog10(a["pvalue"]) - neglog10(b["pvalue"])) < 1e-6
+    # q outside the support: root at infinity, both tails fall back to p_noadj / 2
+    a = sg.spa_binary_fast(np.full(10, 0.5), np.ones(10), 20.0, -15.0, 0.01)
+    assert a["converged"] and math.isinf(a["root1"]) and a["pvalue"] == 0.01
+
+
+def test_pipeline_fit_null_then_score():
+    """fit-null on the GPU then step 2 on the GPU vs the same two steps of the oracle."""
+    c = synth.unrelated(n=800, m_grm=2000, m_test=300, seed=21, h2=0.4, prevalence=0.1)
+    idx = np.arange(c.n, dtype=np.uint32)
+    fg = sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], c.y, c.x, sample_idx=idx)
+    fo = oracle.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], idx, c.y, c.x)
+    tau = fg["tau"]
+    e = sg.ScoreTestEngine(fg["mu"], c.y - fg["mu"], tau, c.x, fg["xvx_inv_xv"], fg["vr"], y=c.y)
+    got = e.test_markers_plink(c.bed_test, c.bpm, c.n, c.bed_test.shape[0])
+    eo = oracle.ScoreEngine(fo["mu"], c.y - fo["mu"], fo["tau"], c.x, fo["xvx_inv_xv"], fo["vr"], y=c.y)
+    rows = [eo.test_marker(oracle.decode_marker(b, c.n)) for b in c.bed_test]
+    want = {k: np.array([r[k] for r in rows]) for k in rows[0]}
+    m = want["pvalue"] >= 1e-10
+    err = np.abs(neglog10(got["pvalue"][m]) - neglog10(want["pvalue"][m]))
+    _say(f"pipeline max |d -log10 p| = {err.max():.3g}")
+    assert err.max() < 1e-5
