@@ -205,10 +205,12 @@ def estimate_variance_ratio(grm, dosages, macs, mu, w, tau, x_cm, a_flat, trait=
 
 def fit_null(bed_body, bytes_per_marker, n_total, m_total, y, x_cm, sample_idx=None,
              trait=TRAIT_BINARY, precision=PREC_FP64, max_iter=30, tol=1e-5, seed=12345,
-             n_random_vectors=30, min_maf=0.01, max_missing=0.15, n_markers_vr=30, device=0):
-    """fit_null::run after phenotype parsing (fit_null.rs:199-394)."""
+             n_random_vectors=30, min_maf=0.01, max_missing=0.15, n_markers_vr=30, device=0,
+             d_bed=None):
+    """fit_null::run after phenotype parsing (fit_null.rs:199-394).  With ``d_bed`` (a device
+    pointer holding the .bed body) the genotypes are read from HBM instead of host memory."""
     L = load()
-    body = _u8(bed_body)
+    body = _u8(bed_body) if d_bed is None else None
     si, sp = _idx(sample_idx)
     n = n_total if si is None else len(si)
     y = _f64(y)
@@ -220,9 +222,12 @@ def fit_null(bed_body, bytes_per_marker, n_total, m_total, y, x_cm, sample_idx=N
     mu = np.empty(n)
     a = np.empty(p * n)
     cls = np.empty(m_total, np.uint8)
-    check(L.saige_fit_null(device, _p(body, _lib.u8p), bytes_per_marker, n_total, m_total, sp, n,
-                           _p(y, _lib.dp), _p(x, _lib.dp), p, trait, C.byref(cfg), C.byref(r),
-                           _p(mu, _lib.dp), _p(a, _lib.dp), _p(cls, _lib.u8p)))
+    args = (bytes_per_marker, n_total, m_total, sp, n, _p(y, _lib.dp), _p(x, _lib.dp), p, trait,
+            C.byref(cfg), C.byref(r), _p(mu, _lib.dp), _p(a, _lib.dp), _p(cls, _lib.u8p))
+    if d_bed is None:
+        check(L.saige_fit_null(device, _p(body, _lib.u8p), *args))
+    else:
+        check(L.saige_fit_null_device(device, C.c_void_p(d_bed), *args))
     rr = r.reml
     return dict(tau=np.array(rr.tau[:]), iterations=rr.iterations, converged=bool(rr.converged),
                 log_likelihood=rr.log_likelihood, grm_products=rr.grm_products,

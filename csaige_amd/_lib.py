@@ -92,6 +92,8 @@ SIGNATURES = [
     ("saige_plink_decode", C.c_int, [C.c_int, u8p, sz, sz, sz, u32p, sz, dp]),
     ("saige_grm_create", C.c_int, [C.c_int, u8p, sz, sz, u32p, sz, u32p, sz, dp, C.c_int,
                                    C.POINTER(vp)]),
+    ("saige_grm_create_device", C.c_int, [C.c_int, vp, sz, sz, u32p, sz, u32p, sz, dp, C.c_int,
+                                          C.POINTER(vp)]),
     ("saige_grm_destroy", None, [vp]),
     ("saige_grm_n_samples", sz, [vp]),
     ("saige_grm_n_markers", sz, [vp]),
@@ -110,6 +112,9 @@ SIGNATURES = [
     ("saige_fit_null", C.c_int, [C.c_int, u8p, sz, sz, sz, u32p, sz, dp, dp, sz, C.c_int,
                                  C.POINTER(FitNullConfig), C.POINTER(FitNullResult), dp, dp,
                                  u8p]),
+    ("saige_fit_null_device", C.c_int, [C.c_int, vp, sz, sz, sz, u32p, sz, dp, dp, sz, C.c_int,
+                                        C.POINTER(FitNullConfig), C.POINTER(FitNullResult), dp,
+                                        dp, u8p]),
     ("saige_engine_create", C.c_int, [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, sz,
                                       sz, C.c_double, dp, dp, sz, C.c_int, C.c_int, C.c_double,
                                       C.c_double, dp, C.POINTER(vp)]),

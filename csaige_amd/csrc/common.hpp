@@ -111,6 +111,7 @@ struct saige_grm {
   // PCG workspaces
   saige::DevBuf pcg_vec, pcg_scal, pcg_part, pcg_b, pcg_w;
   uint64_t products = 0;   // column-products performed (diagnostic)
+  void* mixed = nullptr;   // mixed-precision state (grm_mixed.hip)
 };
 
 namespace saige {

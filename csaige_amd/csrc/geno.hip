@@ -107,6 +107,72 @@ __global__ void repack_kernel(const uint8_t* __restrict__ src, size_t src_bpm,
   dst[j * row_bytes + b] = (uint8_t)out;
 }
 
+// Same, reading the source rows from device memory through a device marker index list.
+__global__ void repack_gather_kernel(const uint8_t* __restrict__ src, size_t src_bpm,
+                                     const uint32_t* __restrict__ midx,
+                                     const uint32_t* __restrict__ idx, size_t n, size_t m,
+                                     uint8_t* __restrict__ dst, size_t row_bytes) {
+  size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t j = blockIdx.y;
+  if (b >= row_bytes || j >= m) return;
+  const uint8_t* row = src + (size_t)(midx ? midx[j] : j) * src_bpm;
+  uint32_t out = 0;
+  for (int t = 0; t < 4; ++t) {
+    size_t k = 4 * b + t;
+    uint32_t c = (k < n) ? code_at(row, idx, k) : 1u;
+    out |= c << (2 * t);
+  }
+  dst[j * row_bytes + b] = (uint8_t)out;
+}
+
+int repack_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const uint32_t* d_idx,
+                       size_t n, const uint32_t* marker_idx, size_t m, uint8_t* d_dst,
+                       size_t row_bytes) {
+  if (m == 0) return SAIGE_OK;
+  DevBuf d_mi;
+  if (marker_idx) {
+    SAIGE_TRY(d_mi.ensure(m * 4));
+    SAIGE_HIP(hipMemcpyAsync(d_mi.p, marker_idx, m * 4, hipMemcpyHostToDevice, s));
+  }
+  const size_t chunk = 65535;
+  for (size_t j0 = 0; j0 < m; j0 += chunk) {
+    size_t mc = std::min(chunk, m - j0);
+    dim3 grid((unsigned)ceil_div(row_bytes, 256), (unsigned)mc);
+    repack_gather_kernel<<<grid, 256, 0, s>>>(
+        marker_idx ? d_bed : d_bed + j0 * bpm, bpm, marker_idx ? d_mi.as<uint32_t>() + j0 : nullptr,
+        d_idx, n, mc, d_dst + j0 * row_bytes, row_bytes);
+    SAIGE_HIP(hipGetLastError());
+  }
+  SAIGE_HIP(hipStreamSynchronize(s));
+  return SAIGE_OK;
+}
+
+// Allele counts of m markers of a device-resident .bed body (host outputs).
+int marker_stats_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const uint32_t* d_idx,
+                             size_t n, size_t m, double* ac, uint32_t* n_valid, double* af,
+                             double* mac) {
+  if (m == 0) return SAIGE_OK;
+  DevBuf d_ac, d_nv, d_af, d_mac;
+  SAIGE_TRY(d_ac.ensure(m * 8));
+  SAIGE_TRY(d_nv.ensure(m * 4));
+  SAIGE_TRY(d_af.ensure(m * 8));
+  SAIGE_TRY(d_mac.ensure(m * 8));
+  const size_t chunk = 1u << 30;
+  for (size_t j0 = 0; j0 < m; j0 += chunk) {
+    size_t mc = std::min(chunk, m - j0);
+    marker_stats_kernel<<<(unsigned)mc, 256, 0, s>>>(d_bed + j0 * bpm, bpm, d_idx, n, mc,
+                                                     d_ac.as<double>() + j0, d_nv.as<uint32_t>() + j0,
+                                                     d_af.as<double>() + j0, d_mac.as<double>() + j0);
+    SAIGE_HIP(hipGetLastError());
+  }
+  SAIGE_HIP(hipMemcpyAsync(ac, d_ac.p, m * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipMemcpyAsync(n_valid, d_nv.p, m * 4, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipMemcpyAsync(af, d_af.p, m * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipMemcpyAsync(mac, d_mac.p, m * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  return SAIGE_OK;
+}
+
 // Upload the rows of the selected markers (marker_idx[j], or j) in chunks and repack.
 // dst: device rows of row_bytes.  Returns status.
 int upload_repack(hipStream_t s, const uint8_t* bed_body, size_t bpm, size_t n_total,

@@ -27,7 +27,11 @@ int upload_repack(hipStream_t s, const uint8_t* bed_body, size_t bpm, size_t n_t
                   uint8_t* d_dst, size_t row_bytes);
 int upload_sample_idx(hipStream_t s, const uint32_t* sample_idx, size_t n, size_t n_total,
                       DevBuf& d);
+int repack_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const uint32_t* d_idx,
+                       size_t n, const uint32_t* marker_idx, size_t m, uint8_t* d_dst,
+                       size_t row_bytes);
 int grm_mixed_prepare(saige_grm* h);
+void grm_mixed_release(saige_grm* h);
 int grm_mixed_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
                       size_t ldo, const double* d_w, const double* tau);
 
@@ -323,12 +327,12 @@ static void build_lut(const double* af, size_t m, size_t m_pad, std::vector<doub
 
 using namespace saige;
 
-extern "C" {
+namespace saige {
 
-int saige_grm_create(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,
-                     const uint32_t* sample_idx, size_t n, const uint32_t* marker_idx, size_t m,
-                     const double* af, int precision, saige_grm** out) {
-  clear_error();
+int grm_create_impl(int device, const uint8_t* bed_body, bool on_device, size_t bpm,
+                    size_t n_total, const uint32_t* sample_idx, size_t n,
+                    const uint32_t* marker_idx, size_t m, const double* af, int precision,
+                    saige_grm** out) {
   SAIGE_ARG(out != nullptr, "out");
   *out = nullptr;
   SAIGE_ARG(n > 0, "n > 0");
@@ -364,14 +368,13 @@ int saige_grm_create(int device, const uint8_t* bed_body, size_t bpm, size_t n_t
   }
   DevBuf d_idx;
   if ((rc = upload_sample_idx(h->stream, sample_idx, n, n_total, d_idx)) != SAIGE_OK) return fail(rc);
-  if (marker_idx) {
-    size_t m_total_bound = SIZE_MAX;
-    (void)m_total_bound;
+  if (m > 0) {
+    rc = on_device ? repack_from_device(h->stream, bed_body, bpm, d_idx.as<uint32_t>(), n,
+                                        marker_idx, m, g.geno, g.row_bytes)
+                   : upload_repack(h->stream, bed_body, bpm, n_total, d_idx.as<uint32_t>(), n,
+                                   marker_idx, m, g.geno, g.row_bytes);
+    if (rc != SAIGE_OK) return fail(rc);
   }
-  if (m > 0 &&
-      (rc = upload_repack(h->stream, bed_body, bpm, n_total, d_idx.as<uint32_t>(), n, marker_idx, m,
-                          g.geno, g.row_bytes)) != SAIGE_OK)
-    return fail(rc);
   std::vector<double> lut, sc, sh;
   build_lut(af, m, g.m_pad, lut, sc, sh);
   if ((rc = h->lut.ensure(lut.size() * 8)) != SAIGE_OK) return fail(rc);
@@ -397,11 +400,32 @@ int saige_grm_create(int device, const uint8_t* bed_body, size_t bpm, size_t n_t
   return SAIGE_OK;
 }
 
+}  // namespace saige
+
+extern "C" {
+
+int saige_grm_create(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,
+                     const uint32_t* sample_idx, size_t n, const uint32_t* marker_idx, size_t m,
+                     const double* af, int precision, saige_grm** out) {
+  clear_error();
+  return grm_create_impl(device, bed_body, false, bpm, n_total, sample_idx, n, marker_idx, m, af,
+                         precision, out);
+}
+
+int saige_grm_create_device(int device, const uint8_t* d_bed_body, size_t bpm, size_t n_total,
+                            const uint32_t* sample_idx, size_t n, const uint32_t* marker_idx,
+                            size_t m, const double* af, int precision, saige_grm** out) {
+  clear_error();
+  return grm_create_impl(device, d_bed_body, true, bpm, n_total, sample_idx, n, marker_idx, m, af,
+                         precision, out);
+}
+
 void saige_grm_destroy(saige_grm* h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+  grm_mixed_release(h);
   delete h;
 }
 

@@ -1,20 +1,437 @@
-// Mixed-precision GRM product (bf16 MFMA on exact genotypes).  Built in a later step;
-// until then a handle created with SAIGE_PREC_MIXED reports that the path is absent.
+// Mixed-precision GRM product: exact genotypes on bf16 MFMA (gfx950), standardization
+// factored out, fp32 accumulation, fp64 epilogues.
+//
+// Same operator as OnTheFlyGrm::mat_vec (saige-core/src/glmm/pcg.rs:128-180):
+//   GRM V = (1/M) Z Z^T V,  z_im = (g_im - 2 p_m) s_m  with s_m = 1/sqrt(2 p_m (1 - p_m)).
+// Factored:  Z^T V = diag(s) (G^T V - 2p * 1^T V)  and  Z W = G (s*W) - 1 (2p)^T (s*W),
+// so the only n x M work is the two integer-matrix products G^T V and G W', run as bf16
+// MFMA with G in {0, 1, 2} exact and the dense operand split into bf16 hi + lo.
+//
+// Conditioning: the dense operand is centred per column before the split (V' = V - mean,
+// W' = W - mean over markers) so that the fp32 MFMA accumulators never carry the large
+// common component that the factored form subtracts again; the means re-enter exactly
+// through fp64 rank-one terms:
+//   T_mc   = s_m (U'_mc - 2 p_m cs'_c) + zs_m mean_c,     U' = G^T V'_split,
+//            cs'_c = sum_i V'_split_ic,  zs_m = sum_i z_im (0 up to rounding)
+//   W_mc   = s_m T_mc;  W' = W - nu_c (nu_c the mean over real markers)
+//   M Y_ic = (G W'_split)_ic - sum_m 2 p_m W'_split_mc + nu_c (r_i - P2)
+//            with r_i = sum_m g_im and P2 = sum_m 2 p_m.
+// Missing genotypes (code 01) decode to 0 in the MFMA operand; handles whose markers have
+// missing calls are refused for this precision (the sparse correction is not built).
+//
+// Layouts: G is resident twice, SNP-major (stage 1 reads a marker's samples) and
+// sample-major (stage 2 reads a sample's markers), 2 bits per genotype in both.  One
+// MFMA kernel serves both stages: rows (markers, then samples) x 16*KT columns, reducing
+// over the other dimension in 32-long steps (v_mfma_f32_16x16x32_bf16).
 #include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
 
 namespace saige {
 
-int grm_mixed_prepare(saige_grm* h) {
-  (void)h;
-  set_error("SAIGE_PREC_MIXED GRM product is not available in this build");
-  return SAIGE_ERR_ARG;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+int marker_stats_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const uint32_t* d_idx,
+                             size_t n, size_t m, double* ac, uint32_t* n_valid, double* af,
+                             double* mac);
+
+constexpr int kMixRows = 256;   // rows per work-group (4 waves x 64)
+constexpr int kMixStep = 32;    // reduction step of one MFMA
+
+// ---------------------------------------------------------------------------------
+// Prepare: sample-major copy, per-sample row sums, missing check.
+// ---------------------------------------------------------------------------------
+// geno: m_pad rows of rb bytes (n_pad/4); geno_t: n_pad rows of rbt bytes (m_pad/4).
+__global__ void transpose_codes_kernel(const uint8_t* __restrict__ geno, size_t rb,
+                                       uint8_t* __restrict__ geno_t, size_t rbt, size_t n_pad) {
+  size_t i = (size_t)blockIdx.y * blockDim.y + threadIdx.y;  // sample
+  size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // output byte (4 markers)
+  if (i >= n_pad || b >= rbt) return;
+  uint32_t out = 0;
+  const size_t byte = i >> 2;
+  const uint32_t sh = 2u * (uint32_t)(i & 3);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) out |= ((geno[(4 * b + t) * rb + byte] >> sh) & 3u) << (2 * t);
+  geno_t[i * rbt + b] = (uint8_t)out;
 }
 
-int grm_mixed_product(saige_grm* h, const double*, size_t, size_t, double*, size_t,
-                      const double*, const double*) {
-  (void)h;
-  set_error("SAIGE_PREC_MIXED GRM product is not available in this build");
-  return SAIGE_ERR_ARG;
+// r_i = sum of dosages over real markers; missing count over real (sample, marker) pairs.
+__global__ void sample_sums_kernel(const uint8_t* __restrict__ geno_t, size_t rbt, size_t n,
+                                   size_t m, double* __restrict__ r, unsigned* __restrict__ n_miss) {
+  size_t i = blockIdx.x;
+  if (i >= n) return;
+  const uint8_t* row = geno_t + i * rbt;
+  uint32_t s = 0, miss = 0;
+  for (size_t mm = threadIdx.x; mm < m; mm += blockDim.x) {
+    uint32_t c = (row[mm >> 2] >> (2 * (mm & 3))) & 3u;
+    s += (c == 0u) ? 2u : (c == 2u ? 1u : 0u);
+    miss += (c == 1u);
+  }
+  __shared__ uint32_t sh_s[256], sh_m[256];
+  sh_s[threadIdx.x] = s;
+  sh_m[threadIdx.x] = miss;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh_s[threadIdx.x] += sh_s[threadIdx.x + w];
+      sh_m[threadIdx.x] += sh_m[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    r[i] = (double)sh_s[0];
+    if (sh_m[0]) atomicAdd(n_miss, sh_m[0]);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// MFMA kernel: part[split][row][16*KT] = sum over the split's reduction range of
+//   code(row, red) * (hi + lo)[col][red]
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ bf16x8 decode8(uint32_t bits16, const uint32_t* lut) {
+  // 8 two-bit codes -> 8 bf16 dosages (00 -> 2, 10 -> 1, 11 -> 0, 01 missing -> 0)
+  union { uint32_t u[4]; bf16x8 v; } o;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) o.u[p] = lut[(bits16 >> (4 * p)) & 15u];
+  return o.v;
+}
+
+template <int KT>
+__global__ __launch_bounds__(256) void mix_gemm_kernel(
+    const uint8_t* __restrict__ codes, size_t code_stride, const __bf16* __restrict__ bhi,
+    const __bf16* __restrict__ blo, size_t b_stride, size_t red_per_split, size_t red_len,
+    float* __restrict__ part, size_t rows_pad) {
+  __shared__ uint32_t lut[16];
+  if (threadIdx.x < 16) {
+    const uint32_t c0 = threadIdx.x & 3u, c1 = threadIdx.x >> 2;
+    auto val = [](uint32_t c) -> uint32_t { return c == 0u ? 0x4000u : (c == 2u ? 0x3F80u : 0u); };
+    lut[threadIdx.x] = val(c0) | (val(c1) << 16);
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const size_t row_base = (size_t)blockIdx.x * kMixRows + (size_t)wave * 64;
+  const size_t split = blockIdx.y;
+  const size_t k0 = split * red_per_split;
+  const size_t k1 = min(k0 + red_per_split, red_len);
+  floatx4 acc[4][KT];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[g][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const uint8_t* arow[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) arow[g] = codes + (row_base + 16 * g + r) * code_stride;
+  for (size_t kk = k0; kk < k1; kk += kMixStep) {
+    const size_t kq = kk + 8 * q;  // this lane's 8 reduction indices
+    bf16x8 a[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const uint32_t bits = *reinterpret_cast<const uint16_t*>(arow[g] + (kq >> 2));
+      a[g] = decode8(bits, lut);
+    }
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+      const size_t off = (size_t)(16 * t + r) * b_stride + kq;
+      const bf16x8 h = *reinterpret_cast<const bf16x8*>(bhi + off);
+      const bf16x8 l = *reinterpret_cast<const bf16x8*>(blo + off);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], h, acc[g][t], 0, 0, 0);
+        acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], l, acc[g][t], 0, 0, 0);
+      }
+    }
+  }
+  // D[i][n]: lane holds rows 4q..4q+3 of column r
+  constexpr int KP = 16 * KT;
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int t = 0; t < KT; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        part[(split * rows_pad + row_base + 16 * g + 4 * q + j) * KP + 16 * t + r] = acc[g][t][j];
+}
+
+// ---------------------------------------------------------------------------------
+// Small fp64 kernels (deterministic fixed-order reductions)
+// ---------------------------------------------------------------------------------
+// per-column block partial sums of a col-major (ld) matrix over rows [0, len)
+__global__ void col_partial_kernel(const double* __restrict__ v, size_t ld, size_t len,
+                                   size_t nchunks, double* __restrict__ part) {
+  const size_t c = blockIdx.y;
+  const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, len);
+  double s = 0.0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) s += v[i + c * ld];
+  __shared__ double sh[kDotBlock];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kDotBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[c * nchunks + blockIdx.x] = sh[0];
+}
+__global__ void col_finalize_kernel(const double* __restrict__ part, size_t nchunks, size_t k,
+                                    double scale, double* __restrict__ out) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double s = 0.0;
+  for (size_t q = 0; q < nchunks; ++q) s += part[c * nchunks + q];
+  out[c] = s * scale;
+}
+
+// centre and split: x' = x - mean_c; hi = bf16(x'), lo = bf16(x' - hi); zero for rows >= len
+// or columns >= k.  Also writes x'_split = hi + lo (fp64) for the exact rank-one terms.
+__global__ void centre_split_kernel(const double* __restrict__ x, size_t ldx, size_t len,
+                                    size_t k, const double* __restrict__ mean, size_t pad_len,
+                                    __bf16* __restrict__ hi, __bf16* __restrict__ lo,
+                                    double* __restrict__ xs, size_t ldxs) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (i >= pad_len) return;
+  double v = 0.0;
+  if (i < len && c < k) v = x[i + c * ldx] - mean[c];
+  const __bf16 h = (__bf16)(float)v;
+  const double rem = v - (double)(float)h;
+  const __bf16 l = (__bf16)(float)rem;
+  hi[c * pad_len + i] = h;
+  lo[c * pad_len + i] = l;
+  if (xs && i < len) xs[i + c * ldxs] = (double)(float)h + (double)(float)l;
+}
+
+// Stage-1 epilogue: T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c; W_mc = s_m T_mc (col-major)
+template <int KP>
+__global__ void stage1_epilogue_kernel(const float* __restrict__ part, size_t splits, size_t m_pad,
+                                       size_t m, size_t k, const double* __restrict__ sc,
+                                       const double* __restrict__ p2, const double* __restrict__ zs,
+                                       const double* __restrict__ cs, const double* __restrict__ mean,
+                                       double* __restrict__ w, size_t ldw) {
+  const size_t mm = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (mm >= m || c >= k) return;
+  double u = 0.0;
+  for (size_t q = 0; q < splits; ++q) u += (double)part[(q * m_pad + mm) * KP + c];
+  const double s = sc[mm];
+  const double t = s * (u - p2[mm] * cs[c]) + zs[mm] * mean[c];
+  w[mm + c * ldw] = s * t;
+}
+
+// per-column dot with a fixed weight vector over [0, len): partials
+__global__ void wdot_partial_kernel(const double* __restrict__ x, size_t ld,
+                                    const double* __restrict__ wt, size_t len, size_t nchunks,
+                                    double* __restrict__ part) {
+  const size_t c = blockIdx.y;
+  const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, len);
+  double s = 0.0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) s = fma(wt[i], x[i + c * ld], s);
+  __shared__ double sh[kDotBlock];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kDotBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[c * nchunks + blockIdx.x] = sh[0];
+}
+
+// Stage-2 epilogue: M Y_ic = sum_s part + nu_c (r_i - P2) - S2p_c; optional Sigma form.
+template <int KP>
+__global__ void stage2_epilogue_kernel(const float* __restrict__ part, size_t splits,
+                                       size_t n_pad, size_t n, size_t k,
+                                       const double* __restrict__ rsum, double P2,
+                                       const double* __restrict__ nu,
+                                       const double* __restrict__ s2p, double inv_m,
+                                       double* __restrict__ out, size_t ldo,
+                                       const double* __restrict__ v, size_t ldv,
+                                       const double* __restrict__ wts, double tau0, double tau1) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (i >= n || c >= k) return;
+  double y = 0.0;
+  for (size_t q = 0; q < splits; ++q) y += (double)part[(q * n_pad + i) * KP + c];
+  y = (y + nu[c] * (rsum[i] - P2) - s2p[c]) * inv_m;
+  if (wts) y = tau0 * v[i + c * ldv] / fmax(wts[i], 1e-30) + tau1 * y;
+  out[i + c * ldo] = y;
+}
+
+// ---------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------
+struct MixedState {
+  DevBuf geno_t, rsum, sc, p2, zs, nmiss;
+  double P2 = 0.0;
+  size_t rbt = 0;
+  // per-product workspaces
+  DevBuf vhi, vlo, vs, whi, wlo, ws_, w, part1, part2, red, cs, mean, nu, s2p;
+};
+
+static MixedState* mstate(saige_grm* h);
+
+int grm_mixed_prepare(saige_grm* h) {
+  GrmDevice& g = h->g;
+  hipStream_t s = h->stream;
+  MixedState* ms = mstate(h);
+  ms->rbt = g.m_pad / 4;
+  SAIGE_TRY(ms->geno_t.ensure(g.n_pad * ms->rbt));
+  SAIGE_TRY(ms->rsum.ensure(g.n_pad * 8));
+  SAIGE_TRY(ms->nmiss.ensure(64));
+  dim3 blk(64, 4);
+  dim3 grd((unsigned)ceil_div(ms->rbt, 64), (unsigned)ceil_div(g.n_pad, 4));
+  transpose_codes_kernel<<<grd, blk, 0, s>>>(g.geno, g.row_bytes, ms->geno_t.as<uint8_t>(), ms->rbt,
+                                             g.n_pad);
+  SAIGE_HIP(hipMemsetAsync(ms->nmiss.p, 0, 4, s));
+  sample_sums_kernel<<<(unsigned)g.n, 256, 0, s>>>(ms->geno_t.as<uint8_t>(), ms->rbt, g.n, g.m,
+                                                   ms->rsum.as<double>(), ms->nmiss.as<unsigned>());
+  SAIGE_HIP(hipGetLastError());
+  unsigned nmiss = 0;
+  SAIGE_HIP(hipMemcpyAsync(&nmiss, ms->nmiss.p, 4, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  if (nmiss) {
+    set_error("SAIGE_PREC_MIXED: %u missing genotype calls among the GRM markers; the mixed "
+              "path requires complete calls (use SAIGE_PREC_FP64)", nmiss);
+    return SAIGE_ERR_ARG;
+  }
+  // per-marker s_m, 2 p_m, zs_m = s_m (ac_m - 2 p_m n) with ac from the stored codes
+  std::vector<double> sc(g.m_pad, 0.0), p2(g.m_pad, 0.0), zs(g.m_pad, 0.0), lut(g.m_pad * 4);
+  std::vector<double> sc64(g.m_pad), sh64(g.m_pad);
+  SAIGE_HIP(hipMemcpy(sc64.data(), g.scale64, g.m_pad * 8, hipMemcpyDeviceToHost));
+  SAIGE_HIP(hipMemcpy(sh64.data(), g.shift64, g.m_pad * 8, hipMemcpyDeviceToHost));
+  std::vector<uint8_t> row(g.row_bytes);
+  double P2 = 0.0;
+  for (size_t j = 0; j < g.m; ++j) {
+    sc[j] = sc64[j];
+    p2[j] = sc64[j] > 0.0 ? sh64[j] / sc64[j] : 0.0;  // shift = 2p * s
+    P2 += p2[j];
+  }
+  // zs_m needs the allele count of each marker: one pass over the SNP-major rows
+  {
+    std::vector<double> ac(g.m), af(g.m), mac(g.m);
+    std::vector<uint32_t> nv(g.m);
+    SAIGE_TRY(marker_stats_from_device(s, g.geno, g.row_bytes, nullptr, g.n, g.m, ac.data(),
+                                       nv.data(), af.data(), mac.data()));
+    for (size_t j = 0; j < g.m; ++j) zs[j] = sc[j] * (ac[j] - p2[j] * (double)g.n);
+  }
+  ms->P2 = P2;
+  SAIGE_TRY(ms->sc.ensure(g.m_pad * 8));
+  SAIGE_TRY(ms->p2.ensure(g.m_pad * 8));
+  SAIGE_TRY(ms->zs.ensure(g.m_pad * 8));
+  SAIGE_HIP(hipMemcpy(ms->sc.p, sc.data(), g.m_pad * 8, hipMemcpyHostToDevice));
+  SAIGE_HIP(hipMemcpy(ms->p2.p, p2.data(), g.m_pad * 8, hipMemcpyHostToDevice));
+  SAIGE_HIP(hipMemcpy(ms->zs.p, zs.data(), g.m_pad * 8, hipMemcpyHostToDevice));
+  return SAIGE_OK;
+}
+
+template <int KT>
+static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
+                            size_t ldo, const double* d_w, const double* tau) {
+  constexpr int KP = 16 * KT;
+  GrmDevice& g = h->g;
+  MixedState* ms = mstate(h);
+  hipStream_t s = h->stream;
+  const size_t n = g.n, m = g.m, n_pad = g.n_pad, m_pad = g.m_pad;
+  const size_t nch_n = std::max<size_t>(1, ceil_div(n, kDotChunk));
+  const size_t nch_m = std::max<size_t>(1, ceil_div(m, kDotChunk));
+  SAIGE_TRY(ms->vhi.ensure(KP * n_pad * 2));
+  SAIGE_TRY(ms->vlo.ensure(KP * n_pad * 2));
+  SAIGE_TRY(ms->vs.ensure(n * KP * 8));
+  SAIGE_TRY(ms->whi.ensure(KP * m_pad * 2));
+  SAIGE_TRY(ms->wlo.ensure(KP * m_pad * 2));
+  SAIGE_TRY(ms->ws_.ensure(m * KP * 8));
+  SAIGE_TRY(ms->w.ensure(m * KP * 8));
+  SAIGE_TRY(ms->red.ensure(std::max(nch_n, nch_m) * KP * 8));
+  SAIGE_TRY(ms->cs.ensure(KP * 8));
+  SAIGE_TRY(ms->mean.ensure(KP * 8));
+  SAIGE_TRY(ms->nu.ensure(KP * 8));
+  SAIGE_TRY(ms->s2p.ensure(KP * 8));
+  double* red = ms->red.as<double>();
+  double* mean = ms->mean.as<double>();
+  double* cs = ms->cs.as<double>();
+  double* nu = ms->nu.as<double>();
+  double* s2p = ms->s2p.as<double>();
+  const unsigned kb = 1;
+
+  // ---- V: column means, centre + split, cs' = colsum of the split ----
+  col_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n, red);
+  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_n, k, 1.0 / (double)n, mean);
+  centre_split_kernel<<<dim3((unsigned)ceil_div(n_pad, 256), (unsigned)KP), 256, 0, s>>>(
+      d_v, ldv, n, k, mean, n_pad, ms->vhi.as<__bf16>(), ms->vlo.as<__bf16>(), ms->vs.as<double>(), n);
+  col_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(ms->vs.as<double>(), n,
+                                                                               n, nch_n, red);
+  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_n, k, 1.0, cs);
+
+  // ---- stage 1: U' = G^T V'_split (rows = markers, reduction over samples) ----
+  const size_t rblk1 = m_pad / kMixRows;
+  size_t splits1 = std::max<size_t>(1, std::min(ceil_div(1024, rblk1), n_pad / 512));
+  size_t rps1 = round_up(ceil_div(n_pad, splits1), kMixStep);
+  splits1 = ceil_div(n_pad, rps1);
+  SAIGE_TRY(ms->part1.ensure(splits1 * m_pad * KP * 4));
+  mix_gemm_kernel<KT><<<dim3((unsigned)rblk1, (unsigned)splits1), 256, 0, s>>>(
+      g.geno, g.row_bytes, ms->vhi.as<__bf16>(), ms->vlo.as<__bf16>(), n_pad, rps1, n_pad,
+      ms->part1.as<float>(), m_pad);
+  stage1_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(m, 256), (unsigned)k), 256, 0, s>>>(
+      ms->part1.as<float>(), splits1, m_pad, m, k, ms->sc.as<double>(), ms->p2.as<double>(),
+      ms->zs.as<double>(), cs, mean, ms->w.as<double>(), m);
+
+  // ---- W: mean over real markers, centre + split, S2p = sum_m 2p_m W'_split ----
+  col_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(ms->w.as<double>(), m,
+                                                                               m, nch_m, red);
+  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_m, k, 1.0 / (double)m, nu);
+  centre_split_kernel<<<dim3((unsigned)ceil_div(m_pad, 256), (unsigned)KP), 256, 0, s>>>(
+      ms->w.as<double>(), m, m, k, nu, m_pad, ms->whi.as<__bf16>(), ms->wlo.as<__bf16>(),
+      ms->ws_.as<double>(), m);
+  wdot_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
+      ms->ws_.as<double>(), m, ms->p2.as<double>(), m, nch_m, red);
+  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_m, k, 1.0, s2p);
+
+  // ---- stage 2: G W'_split (rows = samples, reduction over markers) ----
+  const size_t rblk2 = n_pad / kMixRows;
+  size_t splits2 = std::max<size_t>(1, std::min(ceil_div(1024, rblk2), m_pad / 256));
+  size_t rps2 = round_up(ceil_div(m_pad, splits2), kMixStep);
+  splits2 = ceil_div(m_pad, rps2);
+  SAIGE_TRY(ms->part2.ensure(splits2 * n_pad * KP * 4));
+  mix_gemm_kernel<KT><<<dim3((unsigned)rblk2, (unsigned)splits2), 256, 0, s>>>(
+      ms->geno_t.as<uint8_t>(), ms->rbt, ms->whi.as<__bf16>(), ms->wlo.as<__bf16>(), m_pad, rps2,
+      m_pad, ms->part2.as<float>(), n_pad);
+  stage2_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(
+      ms->part2.as<float>(), splits2, n_pad, n, k, ms->rsum.as<double>(), ms->P2, nu, s2p,
+      1.0 / (double)m, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0, tau ? tau[1] : 0.0);
+  SAIGE_HIP(hipGetLastError());
+  h->products += k;
+  return SAIGE_OK;
+}
+
+int grm_mixed_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
+                      size_t ldo, const double* d_w, const double* tau) {
+  for (size_t c0 = 0; c0 < k; c0 += 128) {
+    const size_t kc = std::min<size_t>(128, k - c0);
+    const double* v = d_v + c0 * ldv;
+    double* o = d_out + c0 * ldo;
+    int rc;
+    if (kc <= 16) rc = mixed_product_kt<1>(h, v, ldv, kc, o, ldo, d_w, tau);
+    else if (kc <= 32) rc = mixed_product_kt<2>(h, v, ldv, kc, o, ldo, d_w, tau);
+    else if (kc <= 48) rc = mixed_product_kt<3>(h, v, ldv, kc, o, ldo, d_w, tau);
+    else if (kc <= 64) rc = mixed_product_kt<4>(h, v, ldv, kc, o, ldo, d_w, tau);
+    else if (kc <= 96) rc = mixed_product_kt<6>(h, v, ldv, kc, o, ldo, d_w, tau);
+    else rc = mixed_product_kt<8>(h, v, ldv, kc, o, ldo, d_w, tau);
+    if (rc != SAIGE_OK) return rc;
+  }
+  return SAIGE_OK;
+}
+
+// one MixedState per handle, owned through the handle's geno_t buffer slot
+static MixedState* mstate(saige_grm* h) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!h->mixed) h->mixed = new MixedState();
+  return static_cast<MixedState*>(h->mixed);
+}
+
+void grm_mixed_release(saige_grm* h) {
+  delete static_cast<MixedState*>(h->mixed);
+  h->mixed = nullptr;
 }
 
 }  // namespace saige

@@ -28,6 +28,16 @@ int upload_repack(hipStream_t s, const uint8_t* bed_body, size_t bpm, size_t n_t
                   uint8_t* d_dst, size_t row_bytes);
 int upload_sample_idx(hipStream_t s, const uint32_t* sample_idx, size_t n, size_t n_total,
                       DevBuf& d);
+int repack_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const uint32_t* d_idx,
+                       size_t n, const uint32_t* marker_idx, size_t m, uint8_t* d_dst,
+                       size_t row_bytes);
+int marker_stats_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const uint32_t* d_idx,
+                             size_t n, size_t m, double* ac, uint32_t* n_valid, double* af,
+                             double* mac);
+int grm_create_impl(int device, const uint8_t* bed_body, bool on_device, size_t bpm,
+                    size_t n_total, const uint32_t* sample_idx, size_t n,
+                    const uint32_t* marker_idx, size_t m, const double* af, int precision,
+                    saige_grm** out);
 
 // ---------------------------------------------------------------------------------
 // p x p host algebra: CholeskyDecomp (saige-linalg/src/decomposition.rs:35-102).
@@ -701,12 +711,15 @@ int saige_estimate_variance_ratio(saige_grm* h, const double* dosages, const dou
                          per_marker_vr, st);
 }
 
-int saige_fit_null(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,
-                   size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
-                   const double* x, size_t p, int trait_type, const saige_fit_null_config* cfg,
-                   saige_fit_null_result* result, double* mu_out, double* xvx_inv_xv,
-                   uint8_t* marker_class) {
-  clear_error();
+}  // extern "C"
+
+namespace saige {
+
+static int fit_null_impl(int device, const uint8_t* bed_body, bool on_device, size_t bpm,
+                         size_t n_total, size_t m_total, const uint32_t* sample_idx, size_t n,
+                         const double* y, const double* x, size_t p, int trait_type,
+                         const saige_fit_null_config* cfg, saige_fit_null_result* result,
+                         double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
   SAIGE_ARG(bed_body && y && x && cfg && result, "null pointer");
   SAIGE_ARG(p >= 1 && n > 0, "dimensions");
   using clk = std::chrono::steady_clock;
@@ -715,8 +728,23 @@ int saige_fit_null(int device, const uint8_t* bed_body, size_t bpm, size_t n_tot
   // QC over all markers (fit_null.rs:209-222)
   std::vector<double> ac(m_total), af(m_total), mac(m_total);
   std::vector<uint32_t> nv(m_total);
-  SAIGE_TRY(saige_plink_marker_stats(device, bed_body, bpm, n_total, m_total, sample_idx, n,
-                                     ac.data(), nv.data(), af.data(), mac.data()));
+  if (on_device) {
+    SAIGE_ARG(bpm >= (n_total + 3) / 4, "bytes_per_marker < ceil(n_total/4)");
+    if (!sample_idx) SAIGE_ARG(n == n_total, "n must equal n_total without a subset");
+    SAIGE_TRY(use_device(device));
+    hipStream_t qs;
+    SAIGE_HIP(hipStreamCreate(&qs));
+    DevBuf d_idx;
+    int qrc = upload_sample_idx(qs, sample_idx, n, n_total, d_idx);
+    if (qrc == SAIGE_OK)
+      qrc = marker_stats_from_device(qs, bed_body, bpm, d_idx.as<uint32_t>(), n, m_total, ac.data(),
+                                     nv.data(), af.data(), mac.data());
+    (void)hipStreamDestroy(qs);
+    SAIGE_TRY(qrc);
+  } else {
+    SAIGE_TRY(saige_plink_marker_stats(device, bed_body, bpm, n_total, m_total, sample_idx, n,
+                                       ac.data(), nv.data(), af.data(), mac.data()));
+  }
   std::vector<uint32_t> passing;
   for (size_t j = 0; j < m_total; ++j) {
     double miss = 1.0 - ((double)nv[j] / (double)n);
@@ -756,8 +784,8 @@ int saige_fit_null(int device, const uint8_t* bed_body, size_t bpm, size_t n_tot
   }
   auto t1 = clk::now();
   saige_grm* h = nullptr;
-  SAIGE_TRY(saige_grm_create(device, bed_body, bpm, n_total, sample_idx, n, grm_ids.data(),
-                             grm_ids.size(), grm_af.data(), cfg->precision, &h));
+  SAIGE_TRY(grm_create_impl(device, bed_body, on_device, bpm, n_total, sample_idx, n,
+                            grm_ids.data(), grm_ids.size(), grm_af.data(), cfg->precision, &h));
   int rc = SAIGE_OK;
   {
     std::lock_guard<std::recursive_mutex> lk(h->mu);
@@ -785,8 +813,11 @@ int saige_fit_null(int device, const uint8_t* bed_body, size_t bpm, size_t n_tot
       rc = rows.ensure(vr_ids.size() * row_bytes);
       if (rc == SAIGE_OK) rc = upload_sample_idx(s, sample_idx, n, n_total, d_idx);
       if (rc == SAIGE_OK)
-        rc = upload_repack(s, bed_body, bpm, n_total, d_idx.as<uint32_t>(), n, vr_ids.data(),
-                           vr_ids.size(), rows.as<uint8_t>(), row_bytes);
+        rc = on_device ? repack_from_device(s, bed_body, bpm, d_idx.as<uint32_t>(), n,
+                                            vr_ids.data(), vr_ids.size(), rows.as<uint8_t>(),
+                                            row_bytes)
+                       : upload_repack(s, bed_body, bpm, n_total, d_idx.as<uint32_t>(), n,
+                                       vr_ids.data(), vr_ids.size(), rows.as<uint8_t>(), row_bytes);
       VrSource src;
       src.d_rows = rows.as<uint8_t>();
       src.row_bytes = row_bytes;
@@ -813,6 +844,30 @@ int saige_fit_null(int device, const uint8_t* bed_body, size_t bpm, size_t n_tot
   }
   saige_grm_destroy(h);
   return rc;
+}
+
+}  // namespace saige
+
+extern "C" {
+
+int saige_fit_null(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,
+                   size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
+                   const double* x, size_t p, int trait_type, const saige_fit_null_config* cfg,
+                   saige_fit_null_result* result, double* mu_out, double* xvx_inv_xv,
+                   uint8_t* marker_class) {
+  clear_error();
+  return fit_null_impl(device, bed_body, false, bpm, n_total, m_total, sample_idx, n, y, x, p,
+                       trait_type, cfg, result, mu_out, xvx_inv_xv, marker_class);
+}
+
+int saige_fit_null_device(int device, const uint8_t* d_bed_body, size_t bpm, size_t n_total,
+                          size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
+                          const double* x, size_t p, int trait_type,
+                          const saige_fit_null_config* cfg, saige_fit_null_result* result,
+                          double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
+  clear_error();
+  return fit_null_impl(device, d_bed_body, true, bpm, n_total, m_total, sample_idx, n, y, x, p,
+                       trait_type, cfg, result, mu_out, xvx_inv_xv, marker_class);
 }
 
 }  // extern "C"

@@ -80,6 +80,11 @@ int saige_grm_create(int device, const uint8_t* bed_body, size_t bytes_per_marke
                      size_t n_total, const uint32_t* sample_idx, size_t n,
                      const uint32_t* marker_idx, size_t m, const double* af, int precision,
                      saige_grm** out);
+/* Same, from a .bed body already resident in device memory (d_bed_body). */
+int saige_grm_create_device(int device, const uint8_t* d_bed_body, size_t bytes_per_marker,
+                            size_t n_total, const uint32_t* sample_idx, size_t n,
+                            const uint32_t* marker_idx, size_t m, const double* af,
+                            int precision, saige_grm** out);
 void saige_grm_destroy(saige_grm* h);
 size_t saige_grm_n_samples(const saige_grm* h);
 size_t saige_grm_n_markers(const saige_grm* h);
@@ -203,6 +208,13 @@ int saige_fit_null(int device, const uint8_t* bed_body, size_t bytes_per_marker,
                    const double* x, size_t p, int trait_type, const saige_fit_null_config* cfg,
                    saige_fit_null_result* result, double* mu, double* xvx_inv_xv,
                    uint8_t* marker_class);
+
+/* Same, with the .bed body already resident in device memory. */
+int saige_fit_null_device(int device, const uint8_t* d_bed_body, size_t bytes_per_marker,
+                          size_t n_total, size_t m_total, const uint32_t* sample_idx, size_t n,
+                          const double* y, const double* x, size_t p, int trait_type,
+                          const saige_fit_null_config* cfg, saige_fit_null_result* result,
+                          double* mu, double* xvx_inv_xv, uint8_t* marker_class);
 
 /* ------------------------------------------------------------------------------
  * Step 2: score test engine.

@@ -273,3 +273,41 @@ def test_pipeline_fit_null_then_score():
     err = np.abs(neglog10(got["pvalue"][m]) - neglog10(want["pvalue"][m]))
     _say(f"pipeline max |d -log10 p| = {err.max():.3g}")
     assert err.max() < 1e-5
+
+
+# --------------------------------------------------------------------------- mixed precision
+@pytest.mark.parametrize("n,m,k", [(1000, 1300, 1), (2048, 3000, 34), (700, 513, 100)])
+def test_grm_mixed_error_bound(n, m, k):
+    """bf16 MFMA on exact genotypes, bf16 hi+lo operand, fp32 accumulation: stated error
+    against the fp64 oracle, relative to the column norm (bound 2e-5)."""
+    _say(f"mixed grm n={n} m={m} k={k}")
+    body, bpm, d, af = _grm_pair(n, m, seed=3 * n + m, p_missing=0.0, mono=True)
+    g = sg.OnTheFlyGrm(body, bpm, n, af, precision=sg.PREC_MIXED)
+    ref = oracle.Grm(d, af, oracle.GRM_F64)
+    rng = np.random.default_rng(k)
+    V = rng.standard_normal((k, n)) + 3.0  # a common offset exercises the centring
+    got = g.mat_mat(V)
+    worst = 0.0
+    for c in range(k):
+        want = ref.mat_vec(V[c])
+        worst = max(worst, np.linalg.norm(got[c] - want) / np.linalg.norm(want))
+    _say(f"mixed grm worst relative error {worst:.2e}")
+    assert worst < 2e-5
+
+
+def test_grm_mixed_refuses_missing_calls():
+    body, bpm, d, af = _grm_pair(300, 200, seed=1, p_missing=0.01)
+    with pytest.raises(sg.SaigeError):
+        sg.OnTheFlyGrm(body, bpm, 300, af, precision=sg.PREC_MIXED)
+
+
+def test_fit_null_mixed_vs_fp64_oracle():
+    c = synth.unrelated(n=1500, m_grm=3000, m_test=0, seed=12, h2=0.5, prevalence=0.2)
+    idx = np.arange(c.n, dtype=np.uint32)
+    got = sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], c.y, c.x, sample_idx=idx,
+                      precision=sg.PREC_MIXED)
+    want = oracle.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], idx, c.y, c.x)
+    _say(f"mixed tau={got['tau'][1]:.8g} oracle={want['tau'][1]:.8g} vr {got['vr']:.8g}/{want['vr']:.8g}")
+    np.testing.assert_array_equal(got["marker_class"], want["marker_class"])
+    assert rel(got["tau"], want["tau"]).max() < 1e-4
+    assert abs(got["vr"] - want["vr"]) < 1e-4 * want["vr"]
