@@ -1,20 +1,22 @@
 #!/usr/bin/env python3
-"""Benchmark: fit-null wall-seconds on one MI355X (the headline workload), plus the GRM
-product roofline, step-2 variants/s and a CPU baseline.
+"""Benchmark: fit-null wall-seconds on one MI355X, mixed precision vs plain FP64 at equal
+accuracy, plus the GRM product roofline, step-2 variants/s and a CPU baseline.
 
 Workload (default): 10,000 nuclear families (2 HWE founders + 3 Mendelian offspring,
 n = 50,000), 100,000 markers with MAF ~ U(0.01, 0.5), binary trait at 5% prevalence from a
 liability with h2 = 0.4 plus two N(0,1) covariates; fit-null with 30 Hutchinson probes, so
-every AI-REML iteration solves 1 + p + 30 = 34 right-hand sides together.  Synthetic,
+every AI-REML iteration solves 1 + p + 30 = 34 right-hand sides as one block.  Synthetic,
 seeded data; the .bed body is resident in HBM before the timed region.
 
 A "step" is one whole fit-null (QC, VR/GRM split, GRM build, AI-REML, XVX^-1X'V and the
-variance ratio) over that cohort.  `value` is the wall-seconds of one step with the
-plain FP64 path (lower is better).
+variance ratio) over that cohort.  `value` is the wall-seconds of one step of the mixed-
+precision path (bf16 MFMA GRM product); the plain FP64 path is timed beside it, and both
+null models are checked to give the same tau, variance ratio and step-2 p-values.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -28,10 +30,34 @@ sys.path.insert(0, ROOT)
 METRIC = "fit-null wall-s + step-2 variants/s, 1 GPU: mixed vs FP64 at equal-accuracy p"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_product(sg, hip, d_bed, bpm, n, markers, af, precision, k, reps):
+    """Average device time of one k-column GRM product, HIP events on the handle's stream."""
+    L = sg.load()
+    h = C.c_void_p()
+    sg.check(L.saige_grm_create_device(0, C.c_void_p(d_bed.value), bpm, n, None, n,
+                                       markers.ctypes.data_as(sg._lib.u32p), len(markers),
+                                       np.ascontiguousarray(af).ctypes.data_as(sg._lib.dp),
+                                       precision, C.byref(h)))
+    d_v = hip.DeviceBuffer(n * k * 8)
+    d_o = hip.DeviceBuffer(n * k * 8)
+    d_v.upload(np.random.default_rng(0).standard_normal((k, n)))
+    stream = L.saige_grm_stream(h)
+    sg.check(L.saige_grm_mat_mat_device(h, d_v.value, n, k, d_o.value, n))
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        sg.check(L.saige_grm_mat_mat_device(h, d_v.value, n, k, d_o.value, n))
+    e1.record(stream)
+    ms = e0.elapsed_ms(e1) / reps
+    L.saige_grm_destroy(h)
+    return ms
 
 
 def main():
@@ -43,10 +69,11 @@ def main():
     ap.add_argument("--markers", type=int, default=100000)
     ap.add_argument("--product-reps", type=int, default=5)
     ap.add_argument("--step2-n", type=int, default=400000)
-    ap.add_argument("--step2-batches", type=int, default=8)
+    ap.add_argument("--step2-batches", type=int, default=4)
     ap.add_argument("--cpu-markers", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-step2", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -74,58 +101,66 @@ def main():
         hip.device_synchronize()
         return time.perf_counter() - t, r
 
+    def describe(tag, dt, r):
+        log(f"{tag}: {dt:.3f}s tau={r['tau'][1]:.8g} iters={r['iterations']} vr={r['vr']:.8g} "
+            f"products={r['grm_products']} pcg_iters={r['pcg_iterations']} "
+            f"(qc {r['seconds_qc']:.3f} reml {r['seconds_reml']:.3f} vr {r['seconds_vr']:.3f})")
+
     for w in range(args.warmup):
         log(f"warmup {w}")
-        one_fit(sg.PREC_FP64)
-    times, res = [], None
+        one_fit(sg.PREC_MIXED)
+    times, rm = [], None
     for s in range(args.steps):
-        dt, res = one_fit(sg.PREC_FP64)
+        dt, rm = one_fit(sg.PREC_MIXED)
         times.append(dt)
-        log(f"step {s}: fit-null FP64 {dt:.2f}s tau={res['tau'][1]:.6g} iters={res['iterations']} "
-            f"vr={res['vr']:.6g} products={res['grm_products']} pcg_iters={res['pcg_iterations']}")
-    fit_s = max(times)
+        describe(f"step {s} mixed", dt, rm)
+    mixed_s = max(times)
+    fp64_s, rf = None, None
+    if not args.no_fp64:
+        fp64_s, rf = one_fit(sg.PREC_FP64)
+        describe("fp64", fp64_s, rf)
+    ref = rf if rf is not None else rm
 
-    # ---- GRM product roofline (34 right-hand sides, the block-PCG shape) ----
-    grm_markers = np.where(res["marker_class"] == 1)[0].astype(np.uint32)
+    # ---- equal accuracy: step-2 p-values under the two null models ----
+    acc = {}
+    if rf is not None:
+        acc["tau_rel_diff"] = float(abs(rm["tau"][1] - rf["tau"][1]) / rf["tau"][1])
+        acc["vr_rel_diff"] = float(abs(rm["vr"] - rf["vr"]) / rf["vr"])
+        ntest = min(4096, m)
+        ps = []
+        for r in (rf, rm):
+            e = sg.ScoreTestEngine(r["mu"], c.y - r["mu"], r["tau"], c.x, r["xvx_inv_xv"], r["vr"],
+                                   y=c.y)
+            ps.append(e.test_markers_plink(c.bed[:ntest], c.bpm, n, ntest)["pvalue"])
+            e.close()
+        mask = ps[0] >= 1e-10
+        acc["max_abs_dlog10p"] = float(np.max(np.abs(np.log10(ps[0][mask]) - np.log10(ps[1][mask]))))
+        acc["variants_compared"] = int(mask.sum())
+        log(f"accuracy mixed vs fp64: {acc}")
+
+    # ---- GRM product roofline at the block-PCG shape ----
+    grm_markers = np.where(ref["marker_class"] == 1)[0].astype(np.uint32)
     ac, nv, af, mac = sg.marker_stats(c.bed, c.bpm, n, m)
-    L = sg.load()
-    import ctypes as C
-    h = C.c_void_p()
-    sg.check(L.saige_grm_create_device(0, C.c_void_p(d_bed.value), c.bpm, n, None, n,
-                                       grm_markers.ctypes.data_as(sg._lib.u32p), len(grm_markers),
-                                       np.ascontiguousarray(af[grm_markers]).ctypes.data_as(sg._lib.dp),
-                                       sg.PREC_FP64, C.byref(h)))
-    k = 34
-    ld = n
-    d_v = hip.DeviceBuffer(ld * k * 8)
-    d_o = hip.DeviceBuffer(ld * k * 8)
-    d_v.upload(np.random.default_rng(0).standard_normal((k, ld)))
-    stream = L.saige_grm_stream(h)
-    sg.check(L.saige_grm_mat_mat_device(h, d_v.value, ld, k, d_o.value, ld))  # warm
-    e0, e1 = hip.Event(), hip.Event()
-    e0.record(stream)
-    for _ in range(args.product_reps):
-        sg.check(L.saige_grm_mat_mat_device(h, d_v.value, ld, k, d_o.value, ld))
-    e1.record(stream)
-    prod_ms = e0.elapsed_ms(e1) / args.product_reps
     mg = len(grm_markers)
+    k = 34
     flops = 4.0 * n * mg * k
-    bytes_alg = 2.0 * ((n + 3) // 4) * mg
-    achieved_tf = flops / (prod_ms * 1e-3) / 1e12
-    log(f"GRM product k={k}: {prod_ms:.3f} ms, {achieved_tf:.2f} TFLOP/s fp64, "
-        f"{bytes_alg / (prod_ms * 1e-3) / 1e9:.1f} GB/s algorithmic")
-    L.saige_grm_destroy(h)
+    prod = {}
+    for tag, prec in (("mixed", sg.PREC_MIXED), ("fp64", sg.PREC_FP64)):
+        ms = time_product(sg, hip, d_bed, c.bpm, n, grm_markers, af[grm_markers], prec, k,
+                          args.product_reps)
+        prod[tag] = ms
+        log(f"GRM product {tag} k={k}: {ms:.3f} ms, {flops / (ms * 1e-3) / 1e12:.2f} TFLOP/s "
+            f"(useful), {2.0 * ((n + 3) // 4) * mg / (ms * 1e-3) / 1e9:.1f} GB/s packed G")
+    mixed_tf = flops / (prod["mixed"] * 1e-3) / 1e12
 
     # ---- step 2: streamed score test + SPA (n = 400,000, 4,096-variant uint8 batches) ----
     step2 = None
     if not args.no_step2:
         n2 = args.step2_n
         nf = synth.null_fit(n=n2, p=1, eta_mean=-4.6, eta_sd=0.5, seed=3)
-        x = nf.x
-        mu = nf.mu
-        a = sg.xvx_inv_xv(x, mu * (1 - mu))
+        a = sg.xvx_inv_xv(nf.x, nf.mu * (1 - nf.mu))
         cutoff = 0.04550026389635842  # 2 * (1 - Phi(2)): SPA where |score/sd| > 2
-        eng = sg.ScoreTestEngine(mu, nf.y - mu, (1.0, 0.0), x, a, 1.05, y=nf.y,
+        eng = sg.ScoreTestEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
                                  spa_pval_cutoff=cutoff)
         B = 4096
         nb = args.step2_batches
@@ -163,7 +198,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle
-        mc = args.cpu_markers
+        mc = min(args.cpu_markers, mg)
         sub = grm_markers[:mc]
         dos = sg.decode(c.bed[sub], c.bpm, n, len(sub))
         g32 = oracle.Grm(dos, af[sub], oracle.GRM_F32)
@@ -174,37 +209,42 @@ def main():
             g32.mat_vec(v)
             reps += 1
         per_prod = (time.perf_counter() - t) / reps * (mg / mc)
-        cpu_fit = per_prod * res["grm_products"]
+        cpu_fit = per_prod * ref["grm_products"]
         cpu = {"value": cpu_fit, "unit": "s (fit-null, extrapolated)", "cores": 1, "kind": "port",
                "sample": f"{reps} f32 OnTheFlyGrm products (saige-rs pcg.rs:128-180 restated) at "
                          f"n={n} x {mc} markers, scaled by {mg}/{mc} markers and by the "
-                         f"{res['grm_products']} column-products of the GPU fit-null; "
+                         f"{ref['grm_products']} column-products of the GPU fit-null; "
                          f"one product = {per_prod:.2f} s"}
         log(f"cpu baseline: {per_prod:.2f} s/product -> {cpu_fit:.0f} s fit-null (extrapolated)")
 
     line = {
         "metric": METRIC,
-        "value": fit_s,
+        "value": mixed_s,
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": fit_s * 1e3,
+        "ms_per_step": mixed_s * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16 MFMA (exact genotypes, hi+lo operand) / fp32 accumulate / fp64 solver",
         "data": "synthetic (seeded families cohort; packed genotypes resident in HBM)",
         "config": {"workload": f"fit-null binary, {args.n_fam} nuclear families x 5 = {n} samples, "
-                               f"{m} markers ({mg} GRM after the VR split), 30 probes, FP64",
-                   "precision": "fp64", "tau_g": float(res["tau"][1]),
-                   "ai_reml_iterations": int(res["iterations"]), "variance_ratio": float(res["vr"]),
-                   "grm_column_products": int(res["grm_products"]),
-                   "pcg_iterations": int(res["pcg_iterations"])},
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": f"GRM product (grm_s1_f64 + grm_s2_f64 + reductions), k={k}, "
-                               f"{prod_ms:.3f} ms/launch, algorithmic {flops:.3g} flop/launch"},
+                               f"{m} markers ({mg} GRM after the VR split), 30 probes",
+                   "precision": "mixed", "fp64_wall_s": fp64_s,
+                   "speedup_vs_fp64": (fp64_s / mixed_s) if fp64_s else None,
+                   "tau_g": float(rm["tau"][1]), "ai_reml_iterations": int(rm["iterations"]),
+                   "variance_ratio": float(rm["vr"]),
+                   "grm_column_products": int(rm["grm_products"]),
+                   "pcg_iterations": int(rm["pcg_iterations"]), "accuracy": acc,
+                   "product_ms": prod},
+        "roofline": {"bound": "mfma", "achieved": mixed_tf, "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": mixed_tf / BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": f"mixed GRM product (2 x mix_gemm_kernel + epilogues), k={k}, "
+                               f"{prod['mixed']:.3f} ms/launch, algorithmic 4*n*M*k = "
+                               f"{flops:.3g} flop/launch; fp64 product {prod['fp64']:.3f} ms = "
+                               f"{flops / (prod['fp64'] * 1e-3) / 1e12:.2f} TFLOP/s of 78.6"},
         "cpu_baseline": cpu,
         "step2": step2,
     }

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace saige {
 
@@ -43,6 +44,7 @@ struct EngineConst {
   int fast;
   int n_cat;
   int has_y;
+  double n_cases;  // cases among all samples (the count when no dosage can be missing)
   double xtr[kMaxP];
   double xtm[kMaxP];
   double xm2x[kMaxP * kMaxP];
@@ -61,26 +63,49 @@ __device__ __forceinline__ double dnan() { return __longlong_as_double(0x7ff8000
 
 // ---- dosage loaders: value of sample i of variant v (NaN = missing) ----
 struct LoadU8 {
+  static constexpr bool kMayMiss = false;
   const uint8_t* q;
   size_t ld;
   double scale;
+  int vec;  // rows 16-byte aligned: 16 samples per load
   __device__ __forceinline__ double operator()(size_t v, size_t i) const {
     return (double)q[v * ld + i] * scale;
   }
+  __device__ __forceinline__ void load16(size_t v, size_t i, double* g) const {
+    const uint4 w = *reinterpret_cast<const uint4*>(q + v * ld + i);
+    const uint32_t u[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) g[k] = (double)((u[k >> 2] >> (8 * (k & 3))) & 0xffu) * scale;
+  }
 };
 struct LoadF64 {
+  static constexpr bool kMayMiss = true;
   const double* g;
   size_t ld;
+  int vec;
   __device__ __forceinline__ double operator()(size_t v, size_t i) const { return g[v * ld + i]; }
+  __device__ __forceinline__ void load16(size_t v, size_t i, double* out) const {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) out[k] = g[v * ld + i + k];
+  }
 };
 struct LoadPacked {  // plink.rs:170-194, 237-247
+  static constexpr bool kMayMiss = true;
   const uint8_t* bed;
   size_t bpm;
   const uint32_t* idx;
+  int vec;  // no subset and 4-byte aligned rows: 16 samples per 32-bit load
+  __device__ __forceinline__ static double dec(uint32_t c) {
+    return c == 0u ? 2.0 : (c == 2u ? 1.0 : (c == 3u ? 0.0 : dnan()));
+  }
   __device__ __forceinline__ double operator()(size_t v, size_t i) const {
     size_t s = idx ? idx[i] : i;
-    uint32_t c = (bed[v * bpm + (s >> 2)] >> (2 * (s & 3))) & 3u;
-    return c == 0u ? 2.0 : (c == 2u ? 1.0 : (c == 3u ? 0.0 : dnan()));
+    return dec((bed[v * bpm + (s >> 2)] >> (2 * (s & 3))) & 3u);
+  }
+  __device__ __forceinline__ void load16(size_t v, size_t i, double* g) const {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(bed + v * bpm + (i >> 2));
+#pragma unroll
+    for (int k = 0; k < 16; ++k) g[k] = dec((w >> (2 * k)) & 3u);
   }
 };
 
@@ -90,12 +115,35 @@ template <int P> struct Layout {
   static constexpr int NS = 2 * P + 9;  // sums: g, n, gA(p), g res, g^2 mu2, g mu2 x(p), g mu, g case, n case, g ctrl
 };
 
+template <int P, bool MISS>
+__device__ __forceinline__ void accumulate(double g, const double* __restrict__ f, double* acc) {
+  bool ok = true;
+  if (MISS) ok = !isnan(g);
+  const double gz = ok ? g : 0.0;
+  acc[0] += gz;
+  if (MISS) acc[1] += ok ? 1.0 : 0.0;
+#pragma unroll
+  for (int r = 0; r < P; ++r) acc[2 + r] = fma(gz, f[r], acc[2 + r]);
+  acc[2 + P] = fma(gz, f[P], acc[2 + P]);
+  acc[3 + P] = fma(gz * gz, f[P + 1], acc[3 + P]);
+#pragma unroll
+  for (int j = 0; j < P; ++j) acc[4 + P + j] = fma(gz, f[P + 2 + j], acc[4 + P + j]);
+  acc[4 + 2 * P] = fma(gz, f[2 * P + 2], acc[4 + 2 * P]);
+  const double cs = f[2 * P + 3];
+  acc[5 + 2 * P] = fma(gz, cs, acc[5 + 2 * P]);
+  if (MISS) acc[6 + 2 * P] += ok ? cs : 0.0;
+  acc[7 + 2 * P] = fma(gz, 1.0 - cs, acc[7 + 2 * P]);
+}
+
+// Without possible missing values the counts n_valid and n_cases are the engine's
+// constants and are filled in by the finalize kernel.
 template <int P, class L>
 __global__ __launch_bounds__(64) void score_sums_kernel(L load, const double* __restrict__ feat,
                                                         size_t n, size_t nvar, size_t sps,
                                                         double* __restrict__ part,
                                                         size_t nvar_pad) {
   constexpr int NF = Layout<P>::NF, NS = Layout<P>::NS;
+  constexpr bool MISS = L::kMayMiss;
   const size_t v = (size_t)blockIdx.x * 64 + threadIdx.x;
   const size_t vv = v < nvar ? v : nvar - 1;
   const size_t i0 = (size_t)blockIdx.y * sps;
@@ -103,26 +151,16 @@ __global__ __launch_bounds__(64) void score_sums_kernel(L load, const double* __
   double acc[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) acc[s] = 0.0;
-  for (size_t i = i0; i < i1; ++i) {
-    const double* f = feat + i * NF;
-    const double g = load(vv, i);
-    const bool ok = !isnan(g);
-    const double gz = ok ? g : 0.0;
-    const double one = ok ? 1.0 : 0.0;
-    acc[0] += gz;
-    acc[1] += one;
+  size_t i = i0;
+  if (load.vec) {
+    for (; i + 16 <= i1; i += 16) {
+      double g[16];
+      load.load16(vv, i, g);
 #pragma unroll
-    for (int r = 0; r < P; ++r) acc[2 + r] = fma(gz, f[r], acc[2 + r]);
-    acc[2 + P] = fma(gz, f[P], acc[2 + P]);
-    acc[3 + P] = fma(gz * gz, f[P + 1], acc[3 + P]);
-#pragma unroll
-    for (int j = 0; j < P; ++j) acc[4 + P + j] = fma(gz, f[P + 2 + j], acc[4 + P + j]);
-    acc[4 + 2 * P] = fma(gz, f[2 * P + 2], acc[4 + 2 * P]);
-    const double cs = f[2 * P + 3];
-    acc[5 + 2 * P] = fma(gz, cs, acc[5 + 2 * P]);
-    acc[6 + 2 * P] = fma(one, cs, acc[6 + 2 * P]);
-    acc[7 + 2 * P] = fma(gz, 1.0 - cs, acc[7 + 2 * P]);
+      for (int k = 0; k < 16; ++k) accumulate<P, MISS>(g[k], feat + (i + k) * NF, acc);
+    }
   }
+  for (; i < i1; ++i) accumulate<P, MISS>(load(vv, i), feat + i * NF, acc);
   if (v >= nvar) return;
 #pragma unroll
   for (int s = 0; s < NS; ++s) part[((size_t)blockIdx.y * NS + s) * nvar_pad + v] = acc[s];
@@ -134,7 +172,7 @@ __device__ __forceinline__ double norm_cdf(double z) { return 0.5 * erfc(-z / sq
 template <int P>
 __global__ void score_finalize_kernel(const double* __restrict__ part, size_t splits,
                                       size_t nvar, size_t nvar_pad, size_t n, EngineConst ec,
-                                      saige_variant_result* __restrict__ out,
+                                      int may_miss, saige_variant_result* __restrict__ out,
                                       GateInfo* __restrict__ gate, uint32_t* __restrict__ n_gate) {
   constexpr int NS = Layout<P>::NS;
   const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -145,6 +183,10 @@ __global__ void score_finalize_kernel(const double* __restrict__ part, size_t sp
     double a = 0.0;
     for (size_t q = 0; q < splits; ++q) a += part[(q * NS + k) * nvar_pad + v];
     s[k] = a;
+  }
+  if (!may_miss) {
+    s[1] = (double)n;
+    s[6 + 2 * P] = ec.n_cases;
   }
   const double sum = s[0], nv = s[1];
   saige_variant_result r;
@@ -310,7 +352,15 @@ struct GTilde {
   }
 };
 
-template <class G>
+// g~ for an intercept-only model from uint8 dosages: g~ = q * scale - (A g)_0 depends only
+// on the dosage byte, so every exp of a pass is one of 256 table entries (the same values
+// the per-sample evaluation would produce).
+struct GTildeU8Tab {
+  LoadU8 load;
+  __device__ __forceinline__ uint32_t byte(size_t v, size_t i) const { return load.q[v * load.ld + i]; }
+};
+
+template <class G, bool TAB>
 __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __restrict__ mu,
                                                         size_t n, const GateInfo* __restrict__ gate,
                                                         const uint32_t* __restrict__ n_gate,
@@ -321,15 +371,24 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
   if (slot >= *n_gate) return;
   __shared__ double sh[4 * kSpaBlock];
   __shared__ GateInfo g;
+  __shared__ double tg[TAB ? 256 : 1], te0[TAB ? 256 : 1], te1[TAB ? 256 : 1];
   if (threadIdx.x == 0) g = gate[slot];
   __syncthreads();
   const size_t v = g.var;
   const double q[2] = {g.q, g.qinv};
   const double thr = fast ? 1e-10 : -1.0;  // fast: |g~| > 1e-10 enters K (fast.rs:191-200)
+  if constexpr (TAB) {
+    for (int b = threadIdx.x; b < 256; b += kSpaBlock) tg[b] = (double)b * gt.load.scale - g.ag[0];
+    __syncthreads();
+  }
+  auto gval = [&](size_t i) -> double {
+    if constexpr (TAB) return tg[gt.byte(v, i)];
+    else return gt(v, i, g.ag);
+  };
   // setup pass: g_pos, g_neg over all samples; NA moments over the excluded samples
   double a[4] = {0.0, 0.0, 0.0, 0.0};
   for (size_t i = threadIdx.x; i < n; i += kSpaBlock) {
-    const double gi = gt(v, i, g.ag);
+    const double gi = gval(i);
     const double m = mu[i];
     if (gi > 0.0) a[0] += gi;
     if (gi < 0.0) a[1] += gi;
@@ -362,19 +421,34 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
   for (int guard = 0; guard < 4 * max_iter + 8; ++guard) {
     if (st[0].phase == 3 && st[1].phase == 3) break;
     const double t0 = st[0].pend, t1 = st[1].pend;
+    if constexpr (TAB) {
+      for (int b = threadIdx.x; b < 256; b += kSpaBlock) {
+        te0[b] = exp(-tg[b] * t0);
+        te1[b] = exp(-tg[b] * t1);
+      }
+      __syncthreads();
+    }
     double b[4] = {0.0, 0.0, 0.0, 0.0};
     for (size_t i = threadIdx.x; i < n; i += kSpaBlock) {
-      const double gi = gt(v, i, g.ag);
+      double gi, e0, e1;
+      if constexpr (TAB) {
+        const uint32_t by = gt.byte(v, i);
+        gi = tg[by];
+        e0 = te0[by];
+        e1 = te1[by];
+      } else {
+        gi = gt(v, i, g.ag);
+        e0 = exp(-gi * t0);
+        e1 = exp(-gi * t1);
+      }
       if (!(fabs(gi) > thr)) continue;
       const double m = mu[i];
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const double tt = r == 0 ? t0 : t1;
-        const double e = exp(-gi * tt);
-        const double den = (1.0 - m) * e + m;
-        b[2 * r] += m * gi / den;
-        b[2 * r + 1] += (1.0 - m) * m * gi * gi * e / (den * den);
-      }
+      const double d0 = (1.0 - m) * e0 + m;
+      const double d1 = (1.0 - m) * e1 + m;
+      b[0] += m * gi / d0;
+      b[1] += (1.0 - m) * m * gi * gi * e0 / (d0 * d0);
+      b[2] += m * gi / d1;
+      b[3] += (1.0 - m) * m * gi * gi * e1 / (d1 * d1);
     }
     block_sum4(b, sh);
     for (int r = 0; r < 2; ++r) {
@@ -393,7 +467,7 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
     const double z0 = st[0].root, z1 = st[1].root;
     double c[4] = {0.0, 0.0, 0.0, 0.0};
     for (size_t i = threadIdx.x; i < n; i += kSpaBlock) {
-      const double gi = gt(v, i, g.ag);
+      const double gi = gval(i);
       if (!(fabs(gi) > thr)) continue;
       const double m = mu[i];
 #pragma unroll
@@ -469,6 +543,7 @@ struct saige_engine {
   std::mutex mu;
   size_t n = 0, p = 0;
   EngineConst ec{};
+  bool intercept_only = false;
   DevBuf feat, mu_d, x_d, idx, part, res, gate, ngate, in;
 };
 
@@ -481,7 +556,7 @@ static int run_batch_p(saige_engine* e, L load, size_t nvar, saige_variant_resul
   constexpr int NS = Layout<P>::NS;
   const size_t vblocks = ceil_div(nvar, 64);
   size_t splits = std::max<size_t>(1, std::min<size_t>(ceil_div(2048, vblocks), ceil_div(n, 512)));
-  size_t sps = ceil_div(n, splits);
+  size_t sps = round_up(ceil_div(n, splits), 16);
   splits = ceil_div(n, sps);
   const size_t nvar_pad = round_up(nvar, 64);
   SAIGE_TRY(e->part.ensure(splits * NS * nvar_pad * 8));
@@ -491,14 +566,25 @@ static int run_batch_p(saige_engine* e, L load, size_t nvar, saige_variant_resul
   score_sums_kernel<P, L><<<dim3((unsigned)vblocks, (unsigned)splits), 64, 0, s>>>(
       load, e->feat.as<double>(), n, nvar, sps, e->part.as<double>(), nvar_pad);
   score_finalize_kernel<P><<<(unsigned)ceil_div(nvar, 128), 128, 0, s>>>(
-      e->part.as<double>(), splits, nvar, nvar_pad, n, e->ec, d_out, e->gate.as<GateInfo>(),
-      e->ngate.as<uint32_t>());
+      e->part.as<double>(), splits, nvar, nvar_pad, n, e->ec, L::kMayMiss ? 1 : 0, d_out,
+      e->gate.as<GateInfo>(), e->ngate.as<uint32_t>());
   SAIGE_HIP(hipGetLastError());
   if (e->ec.use_spa) {
-    GTilde<L, P> gt{load, e->x_d.as<double>(), n};
-    spa_kernel<GTilde<L, P>><<<(unsigned)nvar, kSpaBlock, 0, s>>>(
-        gt, e->mu_d.as<double>(), n, e->gate.as<GateInfo>(), e->ngate.as<uint32_t>(),
-        e->ec.spa_tol, e->ec.fast, d_out, nullptr);
+    bool tab = false;
+    if constexpr (std::is_same<L, LoadU8>::value) tab = (P == 1 && e->intercept_only);
+    if (tab) {
+      if constexpr (std::is_same<L, LoadU8>::value) {
+        GTildeU8Tab gt{load};
+        spa_kernel<GTildeU8Tab, true><<<(unsigned)nvar, kSpaBlock, 0, s>>>(
+            gt, e->mu_d.as<double>(), n, e->gate.as<GateInfo>(), e->ngate.as<uint32_t>(),
+            e->ec.spa_tol, e->ec.fast, d_out, nullptr);
+      }
+    } else {
+      GTilde<L, P> gt{load, e->x_d.as<double>(), n};
+      spa_kernel<GTilde<L, P>, false><<<(unsigned)nvar, kSpaBlock, 0, s>>>(
+          gt, e->mu_d.as<double>(), n, e->gate.as<GateInfo>(), e->ngate.as<uint32_t>(),
+          e->ec.spa_tol, e->ec.fast, d_out, nullptr);
+    }
     SAIGE_HIP(hipGetLastError());
   }
   return SAIGE_OK;
@@ -559,6 +645,11 @@ int saige_engine_create(int device, const double* mu, const double* mu2, const d
   ec.fast = use_fast_spa ? 1 : 0;
   ec.n_cat = (int)n_cat;
   ec.has_y = y ? 1 : 0;
+  ec.n_cases = 0.0;
+  if (y)
+    for (size_t i = 0; i < n; ++i) ec.n_cases += (y[i] > 0.5) ? 1.0 : 0.0;
+  e->intercept_only = (p == 1);
+  for (size_t i = 0; i < n && e->intercept_only; ++i) e->intercept_only = (x[i] == 1.0);
   for (size_t c = 0; c < n_cat; ++c) {
     ec.cat_upper[c] = cat_upper[c];
     ec.cat_vr[c] = cat_vr[c];
@@ -627,7 +718,8 @@ int saige_test_markers_u8_device(saige_engine* e, const uint8_t* d_q, size_t ldq
   if (nvar == 0) return SAIGE_OK;
   std::lock_guard<std::mutex> lk(e->mu);
   SAIGE_HIP(hipSetDevice(e->device));
-  return run_batch(e, LoadU8{d_q, ldq, scale}, nvar, d_out);
+  const int vec = (ldq % 16 == 0) && ((uintptr_t)d_q % 16 == 0);
+  return run_batch(e, LoadU8{d_q, ldq, scale, vec}, nvar, d_out);
 }
 
 int saige_test_markers_u8(saige_engine* e, const uint8_t* q, size_t ldq, size_t nvar, double scale,
@@ -645,7 +737,7 @@ int saige_test_markers_u8(saige_engine* e, const uint8_t* q, size_t ldq, size_t 
     saige_variant_result* dr =
         reinterpret_cast<saige_variant_result*>(e->in.as<uint8_t>() + round_up(nb * e->n, 256));
     SAIGE_HIP(hipMemcpy2DAsync(dq, e->n, q + v0 * ldq, ldq, e->n, nb, hipMemcpyHostToDevice, s));
-    SAIGE_TRY(run_batch(e, LoadU8{dq, e->n, scale}, nb, dr));
+    SAIGE_TRY(run_batch(e, LoadU8{dq, e->n, scale, (int)(e->n % 16 == 0)}, nb, dr));
     SAIGE_HIP(hipMemcpyAsync(out + v0, dr, nb * sizeof(saige_variant_result), hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
   }
@@ -668,7 +760,7 @@ int saige_test_markers_f64(saige_engine* e, const double* g, size_t ldg, size_t 
     saige_variant_result* dr =
         reinterpret_cast<saige_variant_result*>(e->in.as<uint8_t>() + round_up(nb * n * 8, 256));
     SAIGE_HIP(hipMemcpy2DAsync(dg, n * 8, g + v0 * ldg, ldg * 8, n * 8, nb, hipMemcpyHostToDevice, s));
-    SAIGE_TRY(run_batch(e, LoadF64{dg, n}, nb, dr));
+    SAIGE_TRY(run_batch(e, LoadF64{dg, n, 1}, nb, dr));
     SAIGE_HIP(hipMemcpyAsync(out + v0, dr, nb * sizeof(saige_variant_result), hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
   }
@@ -698,7 +790,8 @@ int saige_test_markers_plink(saige_engine* e, const uint8_t* bed_body, size_t bp
     saige_variant_result* dr =
         reinterpret_cast<saige_variant_result*>(e->in.as<uint8_t>() + round_up(nb * bpm, 256));
     SAIGE_HIP(hipMemcpyAsync(db, bed_body + v0 * bpm, nb * bpm, hipMemcpyHostToDevice, s));
-    SAIGE_TRY(run_batch(e, LoadPacked{db, bpm, sample_idx ? e->idx.as<uint32_t>() : nullptr}, nb, dr));
+    SAIGE_TRY(run_batch(e, LoadPacked{db, bpm, sample_idx ? e->idx.as<uint32_t>() : nullptr,
+                                       (int)(!sample_idx && bpm % 4 == 0)}, nb, dr));
     SAIGE_HIP(hipMemcpyAsync(out + v0, dr, nb * sizeof(saige_variant_result), hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
   }
@@ -733,8 +826,8 @@ int saige_spa_binary(int device, const double* mu, const double* g, size_t n, do
       if (err == hipSuccess) err = hipMemcpyAsync(dgate.p, &gi, sizeof(gi), hipMemcpyHostToDevice, s);
       if (err == hipSuccess) err = hipMemcpyAsync(dn.p, &one, 4, hipMemcpyHostToDevice, s);
       if (err == hipSuccess) {
-        GTilde<LoadF64, 0> gt{LoadF64{dg.as<double>(), n}, nullptr, n};
-        spa_kernel<GTilde<LoadF64, 0>><<<1, kSpaBlock, 0, s>>>(gt, dmu.as<double>(), n,
+        GTilde<LoadF64, 0> gt{LoadF64{dg.as<double>(), n, 0}, nullptr, n};
+        spa_kernel<GTilde<LoadF64, 0>, false><<<1, kSpaBlock, 0, s>>>(gt, dmu.as<double>(), n,
                                                                dgate.as<GateInfo>(), dn.as<uint32_t>(),
                                                                tol, fast, nullptr,
                                                                dres.as<saige_spa_result>());
