@@ -76,8 +76,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "sharding is described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the multi-GPU "
+              "sharding of the sample axis is a design, not built code", file=sys.stderr)
         sys.exit(2)
 
     import csaige_amd as sg

@@ -27,7 +27,7 @@ from ._lib import (PREC_FP64, PREC_MIXED, TRAIT_BINARY, TRAIT_QUANTITATIVE, TRAI
 __all__ = ["OnTheFlyGrm", "fit_ai_reml", "solve_sigma", "xvx_inv_xv", "estimate_variance_ratio",
            "fit_null", "ScoreTestEngine", "spa_binary_fast", "marker_stats", "decode",
            "PREC_FP64", "PREC_MIXED", "TRAIT_BINARY", "TRAIT_QUANTITATIVE", "TRAIT_SURVIVAL",
-           "SaigeError", "load", "results_to_dict"]
+           "SaigeError", "load", "results_to_dict", "SpaRootMlp", "spa_training_set"]
 
 
 def _f64(a):
@@ -240,8 +240,27 @@ def fit_null(bed_body, bytes_per_marker, n_total, m_total, y, x_cm, sample_idx=N
 
 
 def results_to_dict(arr):
-    names = [f for f, _ in _lib.VariantResult._fields_]
-    return {f: np.array([getattr(r, f) for r in arr]) for f in names}
+    """Columns of a ctypes array of saige_variant_result as numpy arrays."""
+    raw = np.frombuffer(arr, dtype=_RESULT_DTYPE)
+    return {f: (raw[f].astype(np.float64) if f == "spa_features" else raw[f].copy())
+            for f in _RESULT_DTYPE.names}
+
+
+def _result_dtype():
+    fields = []
+    for name, ct in _lib.VariantResult._fields_:
+        if name == "spa_features":
+            fields.append((name, np.float32, (6,)))
+        else:
+            fields.append((name, np.dtype(ct)))
+    dt = np.dtype({"names": [f[0] for f in fields],
+                   "formats": [f[1] if len(f) == 2 else (f[1], f[2]) for f in fields],
+                   "offsets": [getattr(_lib.VariantResult, f[0]).offset for f in fields],
+                   "itemsize": C.sizeof(_lib.VariantResult)})
+    return dt
+
+
+_RESULT_DTYPE = _result_dtype()
 
 
 class ScoreTestEngine:
@@ -297,6 +316,11 @@ class ScoreTestEngine:
                                               n_total, sp, nvar, out))
         return results_to_dict(out)
 
+    def set_root_proposal(self, mlp):
+        """Start the SPA Newton iterations from the learned proposal (None: from 0)."""
+        self._mlp = mlp
+        check(load().saige_engine_set_root_proposal(self.h, None if mlp is None else mlp.h))
+
     def test_markers_u8_device(self, d_q, ldq, nvar, scale, d_out):
         check(load().saige_test_markers_u8_device(self.h, d_q, ldq, nvar, scale, d_out))
 
@@ -324,3 +348,80 @@ def spa_binary_fast(mu, g, q, qinv, pval_noadj, tol=1e-6, fast=True, device=0):
                                   pval_noadj, tol, int(fast), C.byref(r)))
     return dict(pvalue=r.pvalue, converged=bool(r.converged), is_spa=bool(r.is_spa),
                 iters1=r.iters1, iters2=r.iters2, root1=r.root1, root2=r.root2)
+
+
+class SpaRootMlp:
+    """Learned saddle-point root proposal (no counterpart in saige-rs): a 6 -> 64 -> 64 -> 1
+    GELU network trained and run on the GPU; see include/saige_gpu.h."""
+
+    N_PARAMS = 4673
+
+    def __init__(self, seed=0, device=0):
+        h = C.c_void_p()
+        check(load().saige_mlp_create(device, seed, C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def _x(x):
+        return np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+
+    def params(self):
+        p = np.empty(self.N_PARAMS, np.float32)
+        nm = np.empty(13, np.float32)
+        check(load().saige_mlp_get_params(self.h, _p(p, _lib.fp), _p(nm, _lib.fp)))
+        return p, nm
+
+    def set_params(self, p, norm):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        nm = np.ascontiguousarray(norm, dtype=np.float32)
+        check(load().saige_mlp_set_params(self.h, _p(p, _lib.fp), _p(nm, _lib.fp)))
+
+    def predict(self, x):
+        x = self._x(x)
+        y = np.empty(len(x), np.float32)
+        check(load().saige_mlp_predict(self.h, _p(x, _lib.fp), len(x), _p(y, _lib.fp)))
+        return y
+
+    def loss_grad(self, x, y):
+        x = self._x(x)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = C.c_float()
+        g = np.empty(self.N_PARAMS, np.float32)
+        check(load().saige_mlp_loss_grad(self.h, _p(x, _lib.fp), _p(y, _lib.fp), len(x),
+                                         C.byref(loss), _p(g, _lib.fp)))
+        return loss.value, g
+
+    def train(self, x, y, epochs=20, batch=4096, lr=2e-3, seed=1):
+        x = self._x(x)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        hist = np.empty(epochs, np.float32)
+        check(load().saige_mlp_train(self.h, _p(x, _lib.fp), _p(y, _lib.fp), len(x), epochs, batch,
+                                     lr, seed, _p(hist, _lib.fp)))
+        return hist
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().saige_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spa_training_set(results, tau_e=1.0):
+    """(features, targets) of the learned proposal from step-2 results of SPA'd variants:
+    both roots of each variant whose root finding converged to a finite saddle point."""
+    ok = results["is_spa"].astype(bool) & results["spa_converged"].astype(bool)
+    sd = np.sqrt(results["var_t"] / tau_e)
+    xs, ys = [], []
+    for root, sign in (("spa_root1", 1.0), ("spa_root2", -1.0)):
+        t = results[root]
+        m = ok & np.isfinite(t) & (sd > 0)
+        f = results["spa_features"][m].copy()
+        f[:, 0] *= sign
+        xs.append(f)
+        ys.append(t[m] * sd[m])
+    return np.concatenate(xs).astype(np.float32), np.concatenate(ys).astype(np.float32)

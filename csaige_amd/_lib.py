@@ -23,6 +23,7 @@ dp = C.POINTER(C.c_double)
 u8p = C.POINTER(C.c_uint8)
 u32p = C.POINTER(C.c_uint32)
 vp = C.c_void_p
+fp = C.POINTER(C.c_float)
 sz = C.c_size_t
 
 
@@ -75,7 +76,9 @@ class VariantResult(C.Structure):
                 ("se_beta", C.c_double), ("variance_ratio", C.c_double),
                 ("af_cases", C.c_double), ("af_controls", C.c_double),
                 ("n_cases", C.c_uint64), ("n_controls", C.c_uint64),
-                ("spa_iters1", C.c_int32), ("spa_iters2", C.c_int32)]
+                ("spa_iters1", C.c_int32), ("spa_iters2", C.c_int32),
+                ("spa_root1", C.c_double), ("spa_root2", C.c_double),
+                ("spa_features", C.c_float * 6)]
 
 
 class SpaResult(C.Structure):
@@ -125,6 +128,15 @@ SIGNATURES = [
     ("saige_test_markers_u8_device", C.c_int, [vp, vp, sz, sz, C.c_double, vp]),
     ("saige_test_markers_plink", C.c_int, [vp, u8p, sz, sz, u32p, sz,
                                            C.POINTER(VariantResult)]),
+    ("saige_mlp_create", C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
+    ("saige_mlp_destroy", None, [vp]),
+    ("saige_mlp_set_params", C.c_int, [vp, fp, fp]),
+    ("saige_mlp_get_params", C.c_int, [vp, fp, fp]),
+    ("saige_mlp_predict", C.c_int, [vp, fp, sz, fp]),
+    ("saige_mlp_loss_grad", C.c_int, [vp, fp, fp, sz, fp, fp]),
+    ("saige_mlp_train", C.c_int, [vp, fp, fp, sz, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
+                                  fp]),
+    ("saige_engine_set_root_proposal", C.c_int, [vp, vp]),
     ("saige_spa_binary", C.c_int, [C.c_int, dp, dp, sz, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_int, C.POINTER(SpaResult)]),
 ]

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["geno.hip", "grm.hip", "grm_mixed.hip", "pcg.hip", "reml.hip", "score.hip"]
+SOURCES = ["geno.hip", "grm.hip", "grm_mixed.hip", "pcg.hip", "reml.hip", "score.hip", "mlp.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
     "-Wno-unused-result",

@@ -82,7 +82,7 @@ constexpr int kDotChunk = 4096;     // samples per block
 // Repacked genotype matrix for the GRM: SNP-major rows of row_bytes = n_pad / 4 bytes,
 // samples in model order, padding samples and padding markers coded 01 (missing -> 0).
 constexpr size_t kSamplePad = 512;
-constexpr size_t kMarkerPad = 256;
+constexpr size_t kMarkerPad = 512;
 
 struct GrmDevice {
   size_t n = 0, m = 0, n_pad = 0, m_pad = 0, row_bytes = 0;

@@ -79,6 +79,53 @@ __global__ void marker_stats_kernel(const uint8_t* __restrict__ bed, size_t bpm,
   }
 }
 
+// Same counts without a sample subset: 16 codes per 32-bit word.  With lo / hi the low and
+// high bits of each code: missing = lo & ~hi (01), dosage 2 = ~lo & ~hi (00), 1 = hi & ~lo (10).
+__global__ void marker_stats_packed_kernel(const uint8_t* __restrict__ bed, size_t bpm, size_t n,
+                                           size_t m, double* __restrict__ ac,
+                                           uint32_t* __restrict__ nvalid, double* __restrict__ af,
+                                           double* __restrict__ mac) {
+  size_t j = blockIdx.x;
+  if (j >= m) return;
+  const uint8_t* row = bed + j * bpm;
+  const size_t nwords = n / 16;
+  uint32_t miss = 0, sum = 0;
+  const bool aligned = ((reinterpret_cast<uintptr_t>(row) & 3u) == 0);
+  for (size_t k = threadIdx.x; k < nwords; k += blockDim.x) {
+    uint32_t w;
+    if (aligned) w = reinterpret_cast<const uint32_t*>(row)[k];
+    else w = (uint32_t)row[4 * k] | ((uint32_t)row[4 * k + 1] << 8) | ((uint32_t)row[4 * k + 2] << 16) |
+             ((uint32_t)row[4 * k + 3] << 24);
+    const uint32_t lo = w & 0x55555555u, hi = (w >> 1) & 0x55555555u;
+    miss += __popc(lo & ~hi);
+    sum += 2u * __popc(~lo & ~hi & 0x55555555u) + __popc(hi & ~lo);
+  }
+  for (size_t i = nwords * 16 + threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t c = (row[i >> 2] >> (2 * (i & 3))) & 3u;
+    miss += (c == 1u);
+    sum += (c == 0u) ? 2u : (c == 2u ? 1u : 0u);
+  }
+  __shared__ uint32_t s_m[256], s_s[256];
+  s_m[threadIdx.x] = miss;
+  s_s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_m[threadIdx.x] += s_m[threadIdx.x + w];
+      s_s[threadIdx.x] += s_s[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t cnt = (uint32_t)n - s_m[0];
+    const double sd = (double)s_s[0], k = (double)cnt;
+    ac[j] = sd;
+    nvalid[j] = cnt;
+    af[j] = cnt > 0 ? sd / (2.0 * k) : 0.0;
+    mac[j] = fmin(sd, 2.0 * k - sd);
+  }
+}
+
 __global__ void decode_kernel(const uint8_t* __restrict__ bed, size_t bpm,
                               const uint32_t* __restrict__ idx, size_t n, size_t m,
                               double* __restrict__ out, size_t ldo) {
@@ -160,9 +207,14 @@ int marker_stats_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, co
   const size_t chunk = 1u << 30;
   for (size_t j0 = 0; j0 < m; j0 += chunk) {
     size_t mc = std::min(chunk, m - j0);
-    marker_stats_kernel<<<(unsigned)mc, 256, 0, s>>>(d_bed + j0 * bpm, bpm, d_idx, n, mc,
-                                                     d_ac.as<double>() + j0, d_nv.as<uint32_t>() + j0,
-                                                     d_af.as<double>() + j0, d_mac.as<double>() + j0);
+    if (d_idx)
+      marker_stats_kernel<<<(unsigned)mc, 256, 0, s>>>(d_bed + j0 * bpm, bpm, d_idx, n, mc,
+                                                       d_ac.as<double>() + j0, d_nv.as<uint32_t>() + j0,
+                                                       d_af.as<double>() + j0, d_mac.as<double>() + j0);
+    else
+      marker_stats_packed_kernel<<<(unsigned)mc, 256, 0, s>>>(
+          d_bed + j0 * bpm, bpm, n, mc, d_ac.as<double>() + j0, d_nv.as<uint32_t>() + j0,
+          d_af.as<double>() + j0, d_mac.as<double>() + j0);
     SAIGE_HIP(hipGetLastError());
   }
   SAIGE_HIP(hipMemcpyAsync(ac, d_ac.p, m * 8, hipMemcpyDeviceToHost, s));
@@ -205,9 +257,16 @@ int upload_repack(hipStream_t s, const uint8_t* bed_body, size_t bpm, size_t n_t
   return SAIGE_OK;
 }
 
+// The index list is uploaded unless it is the identity over all samples (then the kernels
+// take their faster no-subset paths; the result is the same).
 static int upload_idx(hipStream_t s, const uint32_t* sample_idx, size_t n, size_t n_total,
                       DevBuf& d) {
   if (!sample_idx) return SAIGE_OK;
+  if (n == n_total) {
+    bool ident = true;
+    for (size_t k = 0; k < n && ident; ++k) ident = (sample_idx[k] == k);
+    if (ident) return SAIGE_OK;
+  }
   for (size_t k = 0; k < n; ++k)
     if (sample_idx[k] >= n_total) {
       set_error("sample_idx[%zu] = %u >= n_total %zu", k, sample_idx[k], n_total);
@@ -257,10 +316,15 @@ int saige_plink_marker_stats(int device, const uint8_t* bed_body, size_t bpm, si
       size_t mc = std::min(chunk, m - j0);
       hipError_t e = hipMemcpyAsync(d_bed.p, bed_body + j0 * bpm, mc * bpm, hipMemcpyHostToDevice, s);
       if (e == hipSuccess) {
-        marker_stats_kernel<<<(unsigned)mc, 256, 0, s>>>(d_bed.as<uint8_t>(), bpm,
-                                                         d_idx.as<uint32_t>(), n, mc,
-                                                         d_ac.as<double>(), d_nv.as<uint32_t>(),
-                                                         d_af.as<double>(), d_mac.as<double>());
+        if (d_idx.p)
+          marker_stats_kernel<<<(unsigned)mc, 256, 0, s>>>(d_bed.as<uint8_t>(), bpm,
+                                                           d_idx.as<uint32_t>(), n, mc,
+                                                           d_ac.as<double>(), d_nv.as<uint32_t>(),
+                                                           d_af.as<double>(), d_mac.as<double>());
+        else
+          marker_stats_packed_kernel<<<(unsigned)mc, 256, 0, s>>>(
+              d_bed.as<uint8_t>(), bpm, n, mc, d_ac.as<double>(), d_nv.as<uint32_t>(),
+              d_af.as<double>(), d_mac.as<double>());
         e = hipGetLastError();
       }
       if (e == hipSuccess) e = hipMemcpyAsync(ac + j0, d_ac.p, mc * 8, hipMemcpyDeviceToHost, s);

@@ -37,53 +37,52 @@ int marker_stats_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, co
                              size_t n, size_t m, double* ac, uint32_t* n_valid, double* af,
                              double* mac);
 
-constexpr int kMixRows = 256;   // rows per work-group (4 waves x 64)
-constexpr int kMixStep = 32;    // reduction step of one MFMA
+constexpr int kMixRT = 8;                     // 16-row MFMA tiles per wave
+constexpr int kMixRows = 4 * 16 * kMixRT;      // rows per work-group (4 waves)
 
 // ---------------------------------------------------------------------------------
 // Prepare: sample-major copy, per-sample row sums, missing check.
 // ---------------------------------------------------------------------------------
 // geno: m_pad rows of rb bytes (n_pad/4); geno_t: n_pad rows of rbt bytes (m_pad/4).
-__global__ void transpose_codes_kernel(const uint8_t* __restrict__ geno, size_t rb,
-                                       uint8_t* __restrict__ geno_t, size_t rbt, size_t n_pad) {
-  size_t i = (size_t)blockIdx.y * blockDim.y + threadIdx.y;  // sample
-  size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // output byte (4 markers)
-  if (i >= n_pad || b >= rbt) return;
-  uint32_t out = 0;
-  const size_t byte = i >> 2;
-  const uint32_t sh = 2u * (uint32_t)(i & 3);
-#pragma unroll
-  for (int t = 0; t < 4; ++t) out |= ((geno[(4 * b + t) * rb + byte] >> sh) & 3u) << (2 * t);
-  geno_t[i * rbt + b] = (uint8_t)out;
+// One work-group transposes a 64-marker x 256-sample tile through LDS: 64 coalesced row
+// segments of 64 bytes in, 256 sample rows of 16 bytes out.  The same pass accumulates,
+// per sample, the dosage sum over real markers and the count of missing calls (integer
+// atomics, so the totals do not depend on the order).
+__global__ __launch_bounds__(256) void transpose_codes_kernel(
+    const uint8_t* __restrict__ geno, size_t rb, uint8_t* __restrict__ geno_t, size_t rbt,
+    size_t n, size_t m, unsigned* __restrict__ rsum, unsigned* __restrict__ n_miss) {
+  __shared__ __attribute__((aligned(16))) uint8_t tile[64][64 + 4];
+  const int t = threadIdx.x;
+  const size_t m0 = (size_t)blockIdx.x * 64, s0 = (size_t)blockIdx.y * 256;
+  {
+    const int row = t >> 2, seg = t & 3;
+    const uint4 v = *reinterpret_cast<const uint4*>(geno + (m0 + row) * rb + (s0 >> 2) + 16 * seg);
+    uint32_t* d = reinterpret_cast<uint32_t*>(&tile[row][16 * seg]);
+    d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+  }
+  __syncthreads();
+  const size_t i = s0 + t;
+  const int byte = t >> 2, sh = 2 * (t & 3);
+  uint32_t out[4] = {0u, 0u, 0u, 0u};
+  uint32_t sum = 0, miss = 0;
+#pragma unroll 8
+  for (int j = 0; j < 64; ++j) {
+    const uint32_t c = (tile[j][byte] >> sh) & 3u;
+    out[j >> 4] |= c << (2 * (j & 15));
+    const bool real = (m0 + j) < m;
+    sum += real ? ((c == 0u) ? 2u : (c == 2u ? 1u : 0u)) : 0u;
+    miss += (real && c == 1u) ? 1u : 0u;
+  }
+  *reinterpret_cast<uint4*>(geno_t + i * rbt + (m0 >> 2)) = make_uint4(out[0], out[1], out[2], out[3]);
+  if (i < n) {
+    if (sum) atomicAdd(&rsum[i], sum);
+    if (miss) atomicAdd(n_miss, miss);
+  }
 }
 
-// r_i = sum of dosages over real markers; missing count over real (sample, marker) pairs.
-__global__ void sample_sums_kernel(const uint8_t* __restrict__ geno_t, size_t rbt, size_t n,
-                                   size_t m, double* __restrict__ r, unsigned* __restrict__ n_miss) {
-  size_t i = blockIdx.x;
-  if (i >= n) return;
-  const uint8_t* row = geno_t + i * rbt;
-  uint32_t s = 0, miss = 0;
-  for (size_t mm = threadIdx.x; mm < m; mm += blockDim.x) {
-    uint32_t c = (row[mm >> 2] >> (2 * (mm & 3))) & 3u;
-    s += (c == 0u) ? 2u : (c == 2u ? 1u : 0u);
-    miss += (c == 1u);
-  }
-  __shared__ uint32_t sh_s[256], sh_m[256];
-  sh_s[threadIdx.x] = s;
-  sh_m[threadIdx.x] = miss;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      sh_s[threadIdx.x] += sh_s[threadIdx.x + w];
-      sh_m[threadIdx.x] += sh_m[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    r[i] = (double)sh_s[0];
-    if (sh_m[0]) atomicAdd(n_miss, sh_m[0]);
-  }
+__global__ void u32_to_f64_kernel(const unsigned* __restrict__ a, size_t n, double* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (double)a[i];
 }
 
 // ---------------------------------------------------------------------------------
@@ -98,61 +97,124 @@ __device__ __forceinline__ bf16x8 decode8(uint32_t bits16, const uint32_t* lut) 
   return o.v;
 }
 
+// Per chunk of KC reduction indices the work-group copies its 512
+// code rows (KC/4 bytes each) and the KP x KC hi/lo operand tile into LDS with 16-byte
+// coalesced loads (the next chunk is prefetched into registers while the MFMAs of the
+// current one run).  Each wave holds its B fragments of one 32-step in registers and
+// reuses them for its 8 row tiles.  Strides are padded so that the 2-byte code reads
+// (rows at 40 bytes) and the 16-byte operand reads (columns at 272 bytes) hit distinct
+// LDS banks.
+constexpr int kKC = 128;
+constexpr int kArow = 40;              // bytes per staged code row (32 used)
+constexpr int kBcol = kKC + 8;         // bf16 per staged operand column (kKC used)
+
 template <int KT>
-__global__ __launch_bounds__(256) void mix_gemm_kernel(
+__global__ __launch_bounds__(256) void mix_gemm_lds_kernel(
     const uint8_t* __restrict__ codes, size_t code_stride, const __bf16* __restrict__ bhi,
     const __bf16* __restrict__ blo, size_t b_stride, size_t red_per_split, size_t red_len,
     float* __restrict__ part, size_t rows_pad) {
+  constexpr int KP = 16 * KT;
+  constexpr int A_LOADS = kMixRows * (kKC / 4) / 16 / 256;      // 16-byte loads per thread
+  constexpr int B_SEGS = kKC * 2 / 16;                           // 16-byte segments per column
+  constexpr int B_LOADS = (2 * KP * B_SEGS + 255) / 256;         // per thread
   __shared__ uint32_t lut[16];
+  __shared__ __attribute__((aligned(16))) uint8_t sA[kMixRows * kArow];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2 * KP * kBcol];
   if (threadIdx.x < 16) {
     const uint32_t c0 = threadIdx.x & 3u, c1 = threadIdx.x >> 2;
     auto val = [](uint32_t c) -> uint32_t { return c == 0u ? 0x4000u : (c == 2u ? 0x3F80u : 0u); };
     lut[threadIdx.x] = val(c0) | (val(c1) << 16);
   }
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
   const int r = lane & 15, q = lane >> 4;
-  const size_t row_base = (size_t)blockIdx.x * kMixRows + (size_t)wave * 64;
+  const size_t row0 = (size_t)blockIdx.x * kMixRows;
   const size_t split = blockIdx.y;
   const size_t k0 = split * red_per_split;
   const size_t k1 = min(k0 + red_per_split, red_len);
-  floatx4 acc[4][KT];
+
+  uint4 pa[A_LOADS];
+  uint4 pb[B_LOADS];
+  auto gload = [&](size_t kk) {
 #pragma unroll
-  for (int g = 0; g < 4; ++g)
-#pragma unroll
-    for (int t = 0; t < KT; ++t) acc[g][t] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const uint8_t* arow[4];
-#pragma unroll
-  for (int g = 0; g < 4; ++g) arow[g] = codes + (row_base + 16 * g + r) * code_stride;
-  for (size_t kk = k0; kk < k1; kk += kMixStep) {
-    const size_t kq = kk + 8 * q;  // this lane's 8 reduction indices
-    bf16x8 a[4];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const uint32_t bits = *reinterpret_cast<const uint16_t*>(arow[g] + (kq >> 2));
-      a[g] = decode8(bits, lut);
+    for (int j = 0; j < A_LOADS; ++j) {
+      const int e = tid + 256 * j;
+      const int row = e >> 1, half = e & 1;
+      pa[j] = *reinterpret_cast<const uint4*>(codes + (row0 + row) * code_stride + (kk >> 2) + 16 * half);
     }
 #pragma unroll
-    for (int t = 0; t < KT; ++t) {
-      const size_t off = (size_t)(16 * t + r) * b_stride + kq;
-      const bf16x8 h = *reinterpret_cast<const bf16x8*>(bhi + off);
-      const bf16x8 l = *reinterpret_cast<const bf16x8*>(blo + off);
+    for (int j = 0; j < B_LOADS; ++j) {
+      const int e = tid + 256 * j;
+      if (e < 2 * KP * B_SEGS) {
+        const int hl = e / (KP * B_SEGS), rem = e % (KP * B_SEGS);
+        const int col = rem / B_SEGS, seg = rem % B_SEGS;
+        const __bf16* src = (hl ? blo : bhi) + (size_t)col * b_stride + kk + 8 * seg;
+        pb[j] = *reinterpret_cast<const uint4*>(src);
+      }
+    }
+  };
+  auto lstore = [&]() {
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], h, acc[g][t], 0, 0, 0);
-        acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], l, acc[g][t], 0, 0, 0);
+    for (int j = 0; j < A_LOADS; ++j) {
+      const int e = tid + 256 * j;
+      const int row = e >> 1, half = e & 1;
+      uint2* d = reinterpret_cast<uint2*>(sA + row * kArow + 16 * half);
+      d[0] = make_uint2(pa[j].x, pa[j].y);
+      d[1] = make_uint2(pa[j].z, pa[j].w);
+    }
+#pragma unroll
+    for (int j = 0; j < B_LOADS; ++j) {
+      const int e = tid + 256 * j;
+      if (e < 2 * KP * B_SEGS) {
+        const int hl = e / (KP * B_SEGS), rem = e % (KP * B_SEGS);
+        const int col = rem / B_SEGS, seg = rem % B_SEGS;
+        *reinterpret_cast<uint4*>(sB + (hl * KP + col) * kBcol + 8 * seg) = pb[j];
+      }
+    }
+  };
+
+  floatx4 acc[kMixRT][KT];
+#pragma unroll
+  for (int g = 0; g < kMixRT; ++g)
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[g][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  if (k0 < k1) gload(k0);
+  for (size_t kk = k0; kk < k1; kk += kKC) {
+    __syncthreads();  // previous chunk fully consumed (and lut ready on the first pass)
+    lstore();
+    __syncthreads();
+    if (kk + kKC < k1) gload(kk + kKC);
+#pragma unroll
+    for (int ks = 0; ks < kKC / 32; ++ks) {
+      bf16x8 bh[KT], bl[KT];
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        const int off = (16 * t + r) * kBcol + ks * 32 + 8 * q;
+        bh[t] = *reinterpret_cast<const bf16x8*>(sB + off);
+        bl[t] = *reinterpret_cast<const bf16x8*>(sB + KP * kBcol + off);
+      }
+#pragma unroll
+      for (int g = 0; g < kMixRT; ++g) {
+        const int row = wave * 16 * kMixRT + 16 * g + r;
+        const uint32_t bits = *reinterpret_cast<const uint16_t*>(sA + row * kArow + ks * 8 + 2 * q);
+        const bf16x8 a = decode8(bits, lut);
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+          acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[t], acc[g][t], 0, 0, 0);
+          acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[t], acc[g][t], 0, 0, 0);
+        }
       }
     }
   }
-  // D[i][n]: lane holds rows 4q..4q+3 of column r
-  constexpr int KP = 16 * KT;
 #pragma unroll
-  for (int g = 0; g < 4; ++g)
+  for (int g = 0; g < kMixRT; ++g)
 #pragma unroll
     for (int t = 0; t < KT; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        part[(split * rows_pad + row_base + 16 * g + 4 * q + j) * KP + 16 * t + r] = acc[g][t][j];
+        part[(split * rows_pad + row0 + wave * 16 * kMixRT + 16 * g + 4 * q + j) * KP + 16 * t + r] =
+            acc[g][t][j];
 }
 
 // ---------------------------------------------------------------------------------
@@ -277,14 +339,14 @@ int grm_mixed_prepare(saige_grm* h) {
   ms->rbt = g.m_pad / 4;
   SAIGE_TRY(ms->geno_t.ensure(g.n_pad * ms->rbt));
   SAIGE_TRY(ms->rsum.ensure(g.n_pad * 8));
-  SAIGE_TRY(ms->nmiss.ensure(64));
-  dim3 blk(64, 4);
-  dim3 grd((unsigned)ceil_div(ms->rbt, 64), (unsigned)ceil_div(g.n_pad, 4));
-  transpose_codes_kernel<<<grd, blk, 0, s>>>(g.geno, g.row_bytes, ms->geno_t.as<uint8_t>(), ms->rbt,
-                                             g.n_pad);
-  SAIGE_HIP(hipMemsetAsync(ms->nmiss.p, 0, 4, s));
-  sample_sums_kernel<<<(unsigned)g.n, 256, 0, s>>>(ms->geno_t.as<uint8_t>(), ms->rbt, g.n, g.m,
-                                                   ms->rsum.as<double>(), ms->nmiss.as<unsigned>());
+  SAIGE_TRY(ms->nmiss.ensure(64 + g.n_pad * 4));
+  unsigned* d_rsum_u = reinterpret_cast<unsigned*>(ms->nmiss.as<char>() + 64);
+  SAIGE_HIP(hipMemsetAsync(ms->nmiss.p, 0, 64 + g.n_pad * 4, s));
+  transpose_codes_kernel<<<dim3((unsigned)(g.m_pad / 64), (unsigned)(g.n_pad / 256)), 256, 0, s>>>(
+      g.geno, g.row_bytes, ms->geno_t.as<uint8_t>(), ms->rbt, g.n, g.m, d_rsum_u,
+      ms->nmiss.as<unsigned>());
+  u32_to_f64_kernel<<<(unsigned)ceil_div(g.n_pad, 256), 256, 0, s>>>(d_rsum_u, g.n_pad,
+                                                                     ms->rsum.as<double>());
   SAIGE_HIP(hipGetLastError());
   unsigned nmiss = 0;
   SAIGE_HIP(hipMemcpyAsync(&nmiss, ms->nmiss.p, 4, hipMemcpyDeviceToHost, s));
@@ -365,10 +427,10 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   // ---- stage 1: U' = G^T V'_split (rows = markers, reduction over samples) ----
   const size_t rblk1 = m_pad / kMixRows;
   size_t splits1 = std::max<size_t>(1, std::min(ceil_div(1024, rblk1), n_pad / 512));
-  size_t rps1 = round_up(ceil_div(n_pad, splits1), kMixStep);
+  size_t rps1 = round_up(ceil_div(n_pad, splits1), kKC);
   splits1 = ceil_div(n_pad, rps1);
   SAIGE_TRY(ms->part1.ensure(splits1 * m_pad * KP * 4));
-  mix_gemm_kernel<KT><<<dim3((unsigned)rblk1, (unsigned)splits1), 256, 0, s>>>(
+  mix_gemm_lds_kernel<KT><<<dim3((unsigned)rblk1, (unsigned)splits1), 256, 0, s>>>(
       g.geno, g.row_bytes, ms->vhi.as<__bf16>(), ms->vlo.as<__bf16>(), n_pad, rps1, n_pad,
       ms->part1.as<float>(), m_pad);
   stage1_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(m, 256), (unsigned)k), 256, 0, s>>>(
@@ -389,10 +451,10 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   // ---- stage 2: G W'_split (rows = samples, reduction over markers) ----
   const size_t rblk2 = n_pad / kMixRows;
   size_t splits2 = std::max<size_t>(1, std::min(ceil_div(1024, rblk2), m_pad / 256));
-  size_t rps2 = round_up(ceil_div(m_pad, splits2), kMixStep);
+  size_t rps2 = round_up(ceil_div(m_pad, splits2), kKC);
   splits2 = ceil_div(m_pad, rps2);
   SAIGE_TRY(ms->part2.ensure(splits2 * n_pad * KP * 4));
-  mix_gemm_kernel<KT><<<dim3((unsigned)rblk2, (unsigned)splits2), 256, 0, s>>>(
+  mix_gemm_lds_kernel<KT><<<dim3((unsigned)rblk2, (unsigned)splits2), 256, 0, s>>>(
       ms->geno_t.as<uint8_t>(), ms->rbt, ms->whi.as<__bf16>(), ms->wlo.as<__bf16>(), m_pad, rps2,
       m_pad, ms->part2.as<float>(), n_pad);
   stage2_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(

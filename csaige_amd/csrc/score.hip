@@ -25,6 +25,7 @@
 // K'(t) and K''(t) (the values the reference's loop needs at the same point), then the
 // Lugannani-Rice tail.  Control flow per root is exactly find_root_k1_fast.
 #include "common.hpp"
+#include "mlp_common.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -56,8 +57,55 @@ struct GateInfo {
   uint32_t var;
   uint32_t pad;
   double q, qinv, pn;
+  double sd;       // sqrt(var_t / tau_e) = sqrt(K''(0)): the scale of the saddle point
+  double feat[6];  // features of the learned root proposal (root 1; root 2 negates feat[0])
   double ag[kMaxP];
 };
+
+// Learned root proposal used as the Newton starting point (mlp.hip).
+struct MlpView {
+  const float* w;  // fp32 copy of the bf16 weights (exactly bf16-representable values)
+  float mean[6], inv_std[6];
+  float y_scale;
+  int enabled;
+};
+void mlp_view(const saige_mlp* h, const float** w, float* mean, float* inv_std, float* y_scale);
+
+// Both roots' starting points from the learned proposal; all threads of the block call it
+// and all receive the result.  128 threads: thread (r, j) owns hidden unit j of root r.
+__device__ void mlp_propose2(const MlpView& mv, const GateInfo& g, float* sh, double* tinit) {
+  const int tid = threadIdx.x, r = tid >> 6, j = tid & 63;
+  const float* w = mv.w;
+  if (tid < 128) {
+    float a = w[kMlpB1 + j];
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) {
+      const float f = (float)((i == 0 && r == 1) ? -g.feat[0] : g.feat[i]);
+      a = fmaf(w[kMlpW1 + j * kMlpIn + i], (f - mv.mean[i]) * mv.inv_std[i], a);
+    }
+    sh[r * kMlpH + j] = mlp_gelu(a);
+  }
+  __syncthreads();
+  float h2 = 0.0f;
+  if (tid < 128) {
+    float a = w[kMlpB2 + j];
+    for (int k = 0; k < kMlpH; ++k) a = fmaf(w[kMlpW2 + j * kMlpH + k], sh[r * kMlpH + k], a);
+    h2 = w[kMlpW3 + j] * mlp_gelu(a);
+  }
+  __syncthreads();
+  if (tid < 128) sh[r * kMlpH + j] = h2;
+  __syncthreads();
+  if (tid < 2) {
+    float o = w[kMlpB3];
+    for (int k = 0; k < kMlpH; ++k) o += sh[tid * kMlpH + k];
+    const double t = sinh((double)o * (double)mv.y_scale) / g.sd;
+    sh[2 * kMlpH + tid] = isfinite(t) ? (float)t : 0.0f;
+  }
+  __syncthreads();
+  tinit[0] = (double)sh[2 * kMlpH];
+  tinit[1] = (double)sh[2 * kMlpH + 1];
+  __syncthreads();
+}
 
 __device__ __forceinline__ double dnan() { return __longlong_as_double(0x7ff8000000000000ll); }
 
@@ -112,7 +160,9 @@ struct LoadPacked {  // plink.rs:170-194, 237-247
 // feature row layout
 template <int P> struct Layout {
   static constexpr int NF = 2 * P + 4;  // A(p), res, mu2, mu2*x(p), mu, case
-  static constexpr int NS = 2 * P + 9;  // sums: g, n, gA(p), g res, g^2 mu2, g mu2 x(p), g mu, g case, n case, g ctrl
+  // sums: g, n, gA(p), g res, g^2 mu2, g mu2 x(p), g mu, g case, n case, g ctrl, and over
+  // carriers (g > 0): mu, mu2, mu * mu2 (the moments the learned root proposal reads)
+  static constexpr int NS = 2 * P + 12;
 };
 
 template <int P, bool MISS>
@@ -133,6 +183,12 @@ __device__ __forceinline__ void accumulate(double g, const double* __restrict__ 
   acc[5 + 2 * P] = fma(gz, cs, acc[5 + 2 * P]);
   if (MISS) acc[6 + 2 * P] += ok ? cs : 0.0;
   acc[7 + 2 * P] = fma(gz, 1.0 - cs, acc[7 + 2 * P]);
+  if (gz > 0.0) {
+    const double m = f[2 * P + 2], m2 = f[P + 1];
+    acc[8 + 2 * P] += m;
+    acc[9 + 2 * P] += m2;
+    acc[10 + 2 * P] = fma(m, m2, acc[10 + 2 * P]);
+  }
 }
 
 // Without possible missing values the counts n_valid and n_cases are the engine's
@@ -232,6 +288,16 @@ __global__ void score_finalize_kernel(const double* __restrict__ part, size_t sp
   r.spa_converged = 1;
   r.spa_iters1 = 0;
   r.spa_iters2 = 0;
+  r.spa_root1 = 0.0;
+  r.spa_root2 = 0.0;
+  {
+    const double sd = sqrt(vt / ec.tau_e);
+    const double f[6] = {vts > 0.0 ? score / sqrt(vts) : 0.0, log(fmax(vt, 1e-300)),
+                         log1p(fmax(r.mac, 0.0)), log(s[8 + 2 * P] + 1e-8),
+                         log(s[9 + 2 * P] + 1e-8), log(s[10 + 2 * P] + 1e-8)};
+    for (int j = 0; j < 6; ++j) r.spa_features[j] = (float)f[j];
+    (void)sd;
+  }
   r.variance_ratio = vr;
   r.beta = vts > 1e-30 ? score / vts : 0.0;
   r.se_beta = vts > 1e-30 ? 1.0 / sqrt(vts) : __longlong_as_double(0x7ff0000000000000ll);
@@ -258,6 +324,8 @@ __global__ void score_finalize_kernel(const double* __restrict__ part, size_t sp
     gi.q = q;
     gi.qinv = qinv;
     gi.pn = pn;
+    gi.sd = sqrt(vt / ec.tau_e);
+    for (int j = 0; j < 6; ++j) gi.feat[j] = (double)r.spa_features[j];
 #pragma unroll
     for (int j = 0; j < kMaxP; ++j) gi.ag[j] = j < P ? ag[j] : 0.0;
     gate[slot] = gi;
@@ -287,7 +355,20 @@ struct RootState {
   double t, k1e, k2e, prev_jump, pend, root;
   int phase;  // 0 init-eval, 1 at tn, 2 at th, 3 done
   int it, converged;
+  int spent;  // iterations of an abandoned start (learned proposal that did not converge)
 };
+
+// Begin (or restart) the search at t0 with the state find_root_k1_fast starts from.
+__device__ __forceinline__ void root_start(RootState& s, double t0) {
+  s.t = t0;
+  s.pend = t0;
+  s.prev_jump = __longlong_as_double(0x7ff0000000000000ll);
+  s.it = 0;
+  s.k1e = s.k2e = 0.0;
+  s.phase = 0;
+  s.converged = 0;
+  s.root = 0.0;
+}
 
 // After an evaluation at the current point t (k1e = K'(t) - q, k2e = K''(t)), take the
 // Newton step of find_root_k1_fast (fast.rs:79-105); returns with either a new pending
@@ -364,7 +445,7 @@ template <class G, bool TAB>
 __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __restrict__ mu,
                                                         size_t n, const GateInfo* __restrict__ gate,
                                                         const uint32_t* __restrict__ n_gate,
-                                                        double tol, int fast,
+                                                        double tol, int fast, MlpView mv,
                                                         saige_variant_result* __restrict__ out,
                                                         saige_spa_result* __restrict__ single) {
   const uint32_t slot = blockIdx.x;
@@ -399,26 +480,24 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
   }
   block_sum4(a, sh);
   const double gpos = a[0], gneg = a[1], na_mu = a[2], na_sigma = a[3];
+  // starting points: 0 as in fast.rs:217-218, or the learned proposal
+  double tinit[2] = {0.0, 0.0};
+  if (mv.enabled) mlp_propose2(mv, g, reinterpret_cast<float*>(sh), tinit);
   RootState st[2];
   for (int r = 0; r < 2; ++r) {
-    st[r].t = 0.0;
-    st[r].prev_jump = __longlong_as_double(0x7ff0000000000000ll);
-    st[r].pend = 0.0;
-    st[r].it = 0;
-    st[r].k1e = st[r].k2e = 0.0;
+    root_start(st[r], tinit[r]);
+    st[r].spent = 0;
     if (q[r] >= gpos || q[r] <= gneg) {  // fast.rs:67-73: root at infinity
       st[r].root = __longlong_as_double(0x7ff0000000000000ll);
       st[r].converged = 1;
       st[r].phase = 3;
-    } else {
-      st[r].phase = 0;
-      st[r].converged = 0;
-      st[r].root = 0.0;
     }
   }
   const int max_iter = 1000;
-  // Each pass evaluates K'(t) - q and K''(t) at both roots' pending points.
-  for (int guard = 0; guard < 4 * max_iter + 8; ++guard) {
+  // Each pass evaluates K'(t) - q and K''(t) at both roots' pending points.  A search from
+  // a learned proposal that fails is restarted once from 0, the plain method's start, so
+  // the proposal can only add roots, never lose one the plain method finds.
+  for (int guard = 0; guard < 8 * max_iter + 16; ++guard) {
     if (st[0].phase == 3 && st[1].phase == 3) break;
     const double t0 = st[0].pend, t1 = st[1].pend;
     if constexpr (TAB) {
@@ -457,6 +536,10 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
       const double k1 = b[2 * r] + na_mu + na_sigma * tt - q[r];
       const double k2 = b[2 * r + 1] + na_sigma;
       root_after_eval(st[r], k1, k2, tol, max_iter);
+      if (st[r].phase == 3 && !st[r].converged && tinit[r] != 0.0 && st[r].spent == 0) {
+        st[r].spent = st[r].it;
+        root_start(st[r], 0.0);
+      }
     }
   }
   const bool both = st[0].converged && st[1].converged;
@@ -518,15 +601,17 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
       out[v].pvalue = pval;
       out[v].is_spa = is_spa;
       out[v].spa_converged = conv;
-      out[v].spa_iters1 = st[0].it;
-      out[v].spa_iters2 = st[1].it;
+      out[v].spa_iters1 = st[0].spent + st[0].it;
+      out[v].spa_iters2 = st[1].spent + st[1].it;
+      out[v].spa_root1 = st[0].root;
+      out[v].spa_root2 = st[1].root;
     }
     if (single) {
       single->pvalue = pval;
       single->converged = conv;
       single->is_spa = is_spa;
-      single->iters1 = st[0].it;
-      single->iters2 = st[1].it;
+      single->iters1 = st[0].spent + st[0].it;
+      single->iters2 = st[1].spent + st[1].it;
       single->root1 = st[0].root;
       single->root2 = st[1].root;
     }
@@ -544,7 +629,16 @@ struct saige_engine {
   size_t n = 0, p = 0;
   EngineConst ec{};
   bool intercept_only = false;
+  const saige_mlp* mlp = nullptr;
   DevBuf feat, mu_d, x_d, idx, part, res, gate, ngate, in;
+  MlpView mlp_view() const {
+    MlpView v{};
+    if (mlp) {
+      saige::mlp_view(mlp, &v.w, v.mean, v.inv_std, &v.y_scale);
+      v.enabled = 1;
+    }
+    return v;
+  }
 };
 
 namespace saige {
@@ -577,13 +671,13 @@ static int run_batch_p(saige_engine* e, L load, size_t nvar, saige_variant_resul
         GTildeU8Tab gt{load};
         spa_kernel<GTildeU8Tab, true><<<(unsigned)nvar, kSpaBlock, 0, s>>>(
             gt, e->mu_d.as<double>(), n, e->gate.as<GateInfo>(), e->ngate.as<uint32_t>(),
-            e->ec.spa_tol, e->ec.fast, d_out, nullptr);
+            e->ec.spa_tol, e->ec.fast, e->mlp_view(), d_out, nullptr);
       }
     } else {
       GTilde<L, P> gt{load, e->x_d.as<double>(), n};
       spa_kernel<GTilde<L, P>, false><<<(unsigned)nvar, kSpaBlock, 0, s>>>(
           gt, e->mu_d.as<double>(), n, e->gate.as<GateInfo>(), e->ngate.as<uint32_t>(),
-          e->ec.spa_tol, e->ec.fast, d_out, nullptr);
+          e->ec.spa_tol, e->ec.fast, e->mlp_view(), d_out, nullptr);
     }
     SAIGE_HIP(hipGetLastError());
   }
@@ -798,6 +892,14 @@ int saige_test_markers_plink(saige_engine* e, const uint8_t* bed_body, size_t bp
   return SAIGE_OK;
 }
 
+int saige_engine_set_root_proposal(saige_engine* e, const saige_mlp* mlp) {
+  clear_error();
+  SAIGE_ARG(e, "engine");
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->mlp = mlp;
+  return SAIGE_OK;
+}
+
 int saige_spa_binary(int device, const double* mu, const double* g, size_t n, double q,
                      double qinv, double pval_noadj, double tol, int fast,
                      saige_spa_result* out) {
@@ -829,7 +931,7 @@ int saige_spa_binary(int device, const double* mu, const double* g, size_t n, do
         GTilde<LoadF64, 0> gt{LoadF64{dg.as<double>(), n, 0}, nullptr, n};
         spa_kernel<GTilde<LoadF64, 0>, false><<<1, kSpaBlock, 0, s>>>(gt, dmu.as<double>(), n,
                                                                dgate.as<GateInfo>(), dn.as<uint32_t>(),
-                                                               tol, fast, nullptr,
+                                                               tol, fast, MlpView{}, nullptr,
                                                                dres.as<saige_spa_result>());
         err = hipGetLastError();
       }

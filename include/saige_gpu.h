@@ -221,7 +221,9 @@ int saige_fit_null_device(int device, const uint8_t* d_bed_body, size_t bytes_pe
  *   Replaces ScoreTestEngine (saige-core/src/score_test/single_variant.rs:77-108),
  *   test_marker (:112-257) with compute_case_control_af (:260-291), and the fast SPA
  *   spa_binary_fast (saige-core/src/spa/fast.rs:207-242; spa_binary when use_fast_spa
- *   is 0 gives the same result, see DESIGN).  x: n x p col-major; xvx_inv_xv: the p*n
+ *   is 0 gives the same result: fast.rs:192 sets apart samples with |g~| <= 1e-10,
+ *   and the covariate-adjusted g~ of single_variant.rs:200 is practically never that
+ *   small; the kernel applies the test literally).  x: n x p col-major; xvx_inv_xv: the p*n
  *   flat buffer of the null model (NullModel::xvx_inv_xv_flat, null_model.rs:46-47);
  *   categorical ratios are (upper MAC bound, ratio) pairs (variance_ratio.rs:289-301);
  *   y may be NULL (no case/control AF).
@@ -236,6 +238,8 @@ typedef struct {
   double beta, se_beta, variance_ratio, af_cases, af_controls;
   uint64_t n_cases, n_controls;
   int32_t spa_iters1, spa_iters2;  /* Newton iterations of the two roots (diagnostic) */
+  double spa_root1, spa_root2;     /* saddle points of q and qinv (diagnostic)          */
+  float spa_features[6];           /* inputs of the learned root proposal (diagnostic)  */
 } saige_variant_result;            /* numeric fields of SingleVariantResult (:23-74) */
 
 int saige_engine_create(int device, const double* mu, const double* mu2,
@@ -259,6 +263,35 @@ int saige_test_markers_u8_device(saige_engine* e, const uint8_t* d_q, size_t ldq
 int saige_test_markers_plink(saige_engine* e, const uint8_t* bed_body, size_t bytes_per_marker,
                              size_t n_total, const uint32_t* sample_idx, size_t nvar,
                              saige_variant_result* out);
+
+/* ------------------------------------------------------------------------------
+ * Learned saddle-point root proposal (no counterpart in saige-rs).
+ *   A 6 -> 64 -> 64 -> 1 GELU network proposes the Newton starting points of the two SPA
+ *   roots from per-variant features (saige_variant_result::spa_features); the
+ *   safeguarded Newton of find_root_k1_fast (fast.rs:53-132) still refines every root to
+ *   the engine's tolerance.  Parameters: 4,673 fp32 master weights (layout in
+ *   csaige_amd/csrc/mlp_common.hpp), inference with bf16-rounded weights and fp32
+ *   accumulation.  norm[13] = feature means (6), inverse standard deviations (6), y_scale.
+ *   Inputs x are n x 6 row-major raw features; targets y are t * sqrt(varT / tau_e),
+ *   fitted through asinh(y) / y_scale (predict returns y).
+ * ---------------------------------------------------------------------------- */
+typedef struct saige_mlp saige_mlp;
+
+int saige_mlp_create(int device, uint64_t seed, saige_mlp** out);
+void saige_mlp_destroy(saige_mlp* h);
+int saige_mlp_set_params(saige_mlp* h, const float* params, const float* norm);
+int saige_mlp_get_params(saige_mlp* h, float* params, float* norm);
+/* y = network output in target units (bf16 weights, fp32 accumulation). */
+int saige_mlp_predict(saige_mlp* h, const float* x, size_t n, float* y);
+/* Mean squared error on asinh(y) / y_scale and its gradient w.r.t. the fp32 parameters. */
+int saige_mlp_loss_grad(saige_mlp* h, const float* x, const float* y, size_t n, float* loss,
+                        float* grad);
+/* Adam training on the GPU (standardization fitted to x, y); loss_history[epochs]. */
+int saige_mlp_train(saige_mlp* h, const float* x, const float* y, size_t n, uint32_t epochs,
+                    uint32_t batch, float lr, uint64_t seed, float* loss_history);
+/* Use the proposal as the SPA starting point of an engine (NULL: start at 0 as
+ * fast.rs:217-218 does).  The model must outlive its use by the engine. */
+int saige_engine_set_root_proposal(saige_engine* e, const saige_mlp* mlp);
 
 /* ------------------------------------------------------------------------------
  * SPA on one covariate-adjusted genotype vector.
