@@ -69,6 +69,49 @@ struct DevBuf {
 
 int use_device(int device);
 
+// Optional per-handle timing of a handle's GEMM-class kernel launches: HIP events recorded
+// on the handle's stream immediately before and after each timed launch, read back (and
+// reset) by read().  At most kMaxPairs launches are recorded between reads.
+struct KernelTimer {
+  static constexpr size_t kMaxPairs = 4096;
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;  // events recorded since the last read (always even after end())
+  int begin(hipStream_t s) {
+    if (!on || used / 2 >= kMaxPairs) return SAIGE_OK;
+    while (ev.size() < used + 2) {
+      hipEvent_t e;
+      SAIGE_HIP(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+    SAIGE_HIP(hipEventRecord(ev[used], s));
+    ++used;
+    return SAIGE_OK;
+  }
+  int end(hipStream_t s) {
+    if (!on || (used & 1) == 0) return SAIGE_OK;
+    SAIGE_HIP(hipEventRecord(ev[used], s));
+    ++used;
+    return SAIGE_OK;
+  }
+  int read(double* total_ms, uint64_t* launches) {
+    double t = 0.0;
+    if (used) SAIGE_HIP(hipEventSynchronize(ev[used - 1]));
+    for (size_t i = 0; i + 1 < used; i += 2) {
+      float ms = 0.f;
+      SAIGE_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      t += ms;
+    }
+    if (total_ms) *total_ms = t;
+    if (launches) *launches = used / 2;
+    used = 0;
+    return SAIGE_OK;
+  }
+  ~KernelTimer() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
+
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 inline size_t ceil_div(size_t x, size_t m) { return (x + m - 1) / m; }
 
@@ -111,6 +154,7 @@ struct saige_grm {
   // PCG workspaces
   saige::DevBuf pcg_vec, pcg_scal, pcg_part, pcg_b, pcg_w;
   uint64_t products = 0;   // column-products performed (diagnostic)
+  saige::KernelTimer timer; // GEMM kernel launch timing (diagnostic)
   void* mixed = nullptr;   // mixed-precision state (grm_mixed.hip)
 };
 

@@ -180,13 +180,17 @@ static int product_k(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
 
   transpose_v_kernel<K><<<(unsigned)ceil_div(g.n_pad, 256), 256, 0, s>>>(
       d_v, ldv, g.n, k, h->ws_vt.as<double>(), g.n_pad);
+  SAIGE_TRY(h->timer.begin(s));
   grm_s1_f64<K><<<dim3((unsigned)nblk_m, (unsigned)splits1), 256, 0, s>>>(
       g.geno, g.row_bytes, g.lut, h->ws_vt.as<double>(), bps, h->ws_tpart.as<double>(), g.m_pad);
+  SAIGE_TRY(h->timer.end(s));
   grm_s1_reduce<K><<<(unsigned)ceil_div(g.m_pad * K, 256), 256, 0, s>>>(
       h->ws_tpart.as<double>(), splits1, g.m_pad, h->ws_t.as<double>());
+  SAIGE_TRY(h->timer.begin(s));
   grm_s2_f64<K><<<dim3((unsigned)nblk_n, (unsigned)splits2), 256, 0, s>>>(
       g.geno, g.row_bytes, g.lut, h->ws_t.as<double>(), mps, g.m, h->ws_ypart.as<double>(),
       g.n_pad);
+  SAIGE_TRY(h->timer.end(s));
   grm_s2_reduce<K><<<dim3((unsigned)ceil_div(g.n, 256), (unsigned)k), 256, 0, s>>>(
       h->ws_ypart.as<double>(), splits2, g.n_pad, g.n, k, 1.0 / (double)g.m, d_out, ldo, d_v,
       ldv, d_w, tau ? tau[0] : 0.0, tau ? tau[1] : 0.0);
@@ -432,6 +436,24 @@ void saige_grm_destroy(saige_grm* h) {
 size_t saige_grm_n_samples(const saige_grm* h) { return h ? h->g.n : 0; }
 size_t saige_grm_n_markers(const saige_grm* h) { return h ? h->g.m : 0; }
 void* saige_grm_stream(saige_grm* h) { return h ? (void*)h->stream : nullptr; }
+
+int saige_grm_set_kernel_timing(saige_grm* h, int enable) {
+  clear_error();
+  SAIGE_ARG(h, "handle");
+  SAIGE_HIP(hipSetDevice(h->device));
+  std::lock_guard<std::recursive_mutex> lk(h->mu);
+  SAIGE_TRY(h->timer.read(nullptr, nullptr));
+  h->timer.on = enable != 0;
+  return SAIGE_OK;
+}
+
+int saige_grm_kernel_timing(saige_grm* h, double* total_ms, uint64_t* launches) {
+  clear_error();
+  SAIGE_ARG(h, "handle");
+  SAIGE_HIP(hipSetDevice(h->device));
+  std::lock_guard<std::recursive_mutex> lk(h->mu);
+  return h->timer.read(total_ms, launches);
+}
 
 int saige_grm_synchronize(saige_grm* h) {
   clear_error();

@@ -430,9 +430,11 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   size_t rps1 = round_up(ceil_div(n_pad, splits1), kKC);
   splits1 = ceil_div(n_pad, rps1);
   SAIGE_TRY(ms->part1.ensure(splits1 * m_pad * KP * 4));
+  SAIGE_TRY(h->timer.begin(s));
   mix_gemm_lds_kernel<KT><<<dim3((unsigned)rblk1, (unsigned)splits1), 256, 0, s>>>(
       g.geno, g.row_bytes, ms->vhi.as<__bf16>(), ms->vlo.as<__bf16>(), n_pad, rps1, n_pad,
       ms->part1.as<float>(), m_pad);
+  SAIGE_TRY(h->timer.end(s));
   stage1_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(m, 256), (unsigned)k), 256, 0, s>>>(
       ms->part1.as<float>(), splits1, m_pad, m, k, ms->sc.as<double>(), ms->p2.as<double>(),
       ms->zs.as<double>(), cs, mean, ms->w.as<double>(), m);
@@ -454,9 +456,11 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   size_t rps2 = round_up(ceil_div(m_pad, splits2), kKC);
   splits2 = ceil_div(m_pad, rps2);
   SAIGE_TRY(ms->part2.ensure(splits2 * n_pad * KP * 4));
+  SAIGE_TRY(h->timer.begin(s));
   mix_gemm_lds_kernel<KT><<<dim3((unsigned)rblk2, (unsigned)splits2), 256, 0, s>>>(
       ms->geno_t.as<uint8_t>(), ms->rbt, ms->whi.as<__bf16>(), ms->wlo.as<__bf16>(), m_pad, rps2,
       m_pad, ms->part2.as<float>(), n_pad);
+  SAIGE_TRY(h->timer.end(s));
   stage2_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(
       ms->part2.as<float>(), splits2, n_pad, n, k, ms->rsum.as<double>(), ms->P2, nu, s2p,
       1.0 / (double)m, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0, tau ? tau[1] : 0.0);

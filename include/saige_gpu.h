@@ -99,6 +99,14 @@ int saige_grm_mat_mat_device(saige_grm* h, const double* d_v, size_t ldv, size_t
 /* The handle's HIP stream (hipStream_t) and a device synchronize helper. */
 void* saige_grm_stream(saige_grm* h);
 int saige_grm_synchronize(saige_grm* h);
+/* Kernel timing (diagnostic, no counterpart in saige-rs): while enabled, the handle records
+ * HIP events on its stream around every launch of its genotype GEMM kernels (two per
+ * product: the marker-side and the sample-side pass, both precisions).
+ * saige_grm_kernel_timing waits for the last recorded launch and returns the summed
+ * launch time and the number of launches since the previous read (or enable), then resets.
+ * At most 4096 launches are recorded between reads. */
+int saige_grm_set_kernel_timing(saige_grm* h, int enable);
+int saige_grm_kernel_timing(saige_grm* h, double* total_ms, uint64_t* launches);
 
 /* ------------------------------------------------------------------------------
  * Preconditioned conjugate gradient on Sigma = tau_e * diag(1/W) + tau_g * GRM.

@@ -104,7 +104,7 @@ def test_trained_proposal_cuts_newton_iterations():
             assert r1["pvalue"][v] == r0["pvalue"][v]
     x = np.asarray(nf.x, np.float64).reshape(-1, len(nf.mu))
     for v in rescued:
-        g = q[v].astype(np.float64) * (2.0 / 255.0)
+        g = q[v].astype(np.float64) / 127.0
         gt = g - x[0] * float(np.dot(a, g))
         m1 = float(np.dot(nf.mu, gt))
         qs = r1["score"][v] + m1
