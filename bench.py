@@ -10,8 +10,9 @@ seeded data; the .bed body is resident in HBM before the timed region.
 
 A "step" is one whole fit-null (QC, VR/GRM split, GRM build, AI-REML, XVX^-1X'V and the
 variance ratio) over that cohort.  `value` is the wall-seconds of one step of the mixed-
-precision path (bf16 MFMA GRM product); the plain FP64 path is timed beside it, and both
-null models are checked to give the same tau, variance ratio and step-2 p-values.
+precision path with the int8 MFMA GRM product; the bf16 hi+lo form and the plain FP64 path
+are timed beside it, and the null models are checked to give the same tau, variance ratio
+and step-2 p-values.
 """
 from __future__ import annotations
 
@@ -31,6 +32,7 @@ METRIC = "fit-null wall-s + step-2 variants/s, 1 GPU: mixed vs FP64 at equal-acc
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
+I8_PEAK_TOPS = 5000.0
 
 
 def log(msg):
@@ -113,35 +115,44 @@ def main():
 
     for w in range(args.warmup):
         log(f"warmup {w}")
-        one_fit(sg.PREC_MIXED)
+        one_fit(sg.PREC_INT8)
     times, rm = [], None
     for s in range(args.steps):
-        dt, rm = one_fit(sg.PREC_MIXED)
+        dt, rm = one_fit(sg.PREC_INT8)
         times.append(dt)
-        describe(f"step {s} mixed", dt, rm)
+        describe(f"step {s} mixed int8", dt, rm)
     mixed_s = max(times)
+    bf16_s, rb = one_fit(sg.PREC_MIXED)
+    describe("mixed bf16", bf16_s, rb)
     fp64_s, rf = None, None
     if not args.no_fp64:
         fp64_s, rf = one_fit(sg.PREC_FP64)
         describe("fp64", fp64_s, rf)
     ref = rf if rf is not None else rm
 
-    # ---- equal accuracy: step-2 p-values under the two null models ----
+    # ---- equal accuracy: step-2 p-values under the null models ----
     acc = {}
     if rf is not None:
-        acc["tau_rel_diff"] = float(abs(rm["tau"][1] - rf["tau"][1]) / rf["tau"][1])
-        acc["vr_rel_diff"] = float(abs(rm["vr"] - rf["vr"]) / rf["vr"])
         ntest = min(4096, m)
-        ps = []
-        for r in (rf, rm):
+
+        def pvals(r):
             e = sg.ScoreTestEngine(r["mu"], c.y - r["mu"], r["tau"], c.x, r["xvx_inv_xv"], r["vr"],
                                    y=c.y)
-            ps.append(e.test_markers_plink(c.bed[:ntest], c.bpm, n, ntest)["pvalue"])
+            p = e.test_markers_plink(c.bed[:ntest], c.bpm, n, ntest)["pvalue"]
             e.close()
-        mask = ps[0] >= 1e-10
-        acc["max_abs_dlog10p"] = float(np.max(np.abs(np.log10(ps[0][mask]) - np.log10(ps[1][mask]))))
+            return p
+
+        pf = pvals(rf)
+        mask = pf >= 1e-10
+        for tag, r in (("int8", rm), ("bf16", rb)):
+            pm = pvals(r)
+            acc[tag] = {
+                "tau_rel_diff": float(abs(r["tau"][1] - rf["tau"][1]) / rf["tau"][1]),
+                "vr_rel_diff": float(abs(r["vr"] - rf["vr"]) / rf["vr"]),
+                "max_abs_dlog10p": float(np.max(np.abs(np.log10(pf[mask]) - np.log10(pm[mask])))),
+            }
         acc["variants_compared"] = int(mask.sum())
-        log(f"accuracy mixed vs fp64: {acc}")
+        log(f"accuracy vs fp64: {acc}")
 
     # ---- GRM product roofline at the block-PCG shape ----
     grm_markers = np.where(ref["marker_class"] == 1)[0].astype(np.uint32)
@@ -150,7 +161,7 @@ def main():
     k = 34
     flops = 4.0 * n * mg * k
     prod, kern = {}, {}
-    for tag, prec in (("mixed", sg.PREC_MIXED), ("fp64", sg.PREC_FP64)):
+    for tag, prec in (("int8", sg.PREC_INT8), ("bf16", sg.PREC_MIXED), ("fp64", sg.PREC_FP64)):
         ms, kms, kl = time_product(sg, hip, d_bed, c.bpm, n, grm_markers, af[grm_markers], prec,
                                    k, args.product_reps)
         prod[tag] = ms
@@ -161,7 +172,7 @@ def main():
     # the dominant kernel: one genotype GEMM launch = one of the two passes of a product,
     # 2 * n * M * k useful flop (exact genotypes times the k-column operand)
     launch_flops = 2.0 * n * mg * k
-    mixed_tf = launch_flops / (kern["mixed"] * 1e-3) / 1e12
+    mixed_tf = launch_flops / (kern["int8"] * 1e-3) / 1e12
 
     # ---- step 2: streamed score test + SPA (n = 400,000, 4,096-variant uint8 batches) ----
     step2 = None
@@ -238,27 +249,29 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16 MFMA (exact genotypes, hi+lo operand) / fp32 accumulate / fp64 solver",
+        "dtype": "int8 MFMA (exact genotypes x 3 int8 operand slices) / int32 accumulate / "
+                 "fp64 solver",
         "data": "synthetic (seeded families cohort; packed genotypes resident in HBM)",
         "config": {"workload": f"fit-null binary, {args.n_fam} nuclear families x 5 = {n} samples, "
                                f"{m} markers ({mg} GRM after the VR split), 30 probes",
-                   "precision": "mixed", "fp64_wall_s": fp64_s,
+                   "precision": "int8", "bf16_wall_s": bf16_s, "fp64_wall_s": fp64_s,
                    "speedup_vs_fp64": (fp64_s / mixed_s) if fp64_s else None,
                    "tau_g": float(rm["tau"][1]), "ai_reml_iterations": int(rm["iterations"]),
                    "variance_ratio": float(rm["vr"]),
                    "grm_column_products": int(rm["grm_products"]),
                    "pcg_iterations": int(rm["pcg_iterations"]), "accuracy": acc,
-                   "product_ms": prod},
-        "roofline": {"bound": "mfma", "achieved": mixed_tf, "peak": BF16_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": mixed_tf / BF16_PEAK_TFLOPS,
+                   "product_ms": prod, "gemm_launch_ms": kern},
+        "roofline": {"bound": "mfma", "achieved": mixed_tf, "peak": I8_PEAK_TOPS,
+                     "unit": "TOP/s", "frac": mixed_tf / I8_PEAK_TOPS,
                      "traffic": args.traffic_bytes,
-                     "kernel": f"mix_gemm_lds_kernel (bf16 MFMA genotype GEMM), k={k}: "
-                               f"{kern['mixed']:.3f} ms/launch (HIP events on the handle stream), "
-                               f"algorithmic 2*n*M*k = {launch_flops:.3g} flop/launch; "
-                               f"hi+lo operand halves executes 2x that; whole product "
-                               f"{prod['mixed']:.3f} ms; fp64 GEMM {kern['fp64']:.3f} ms/launch = "
-                               f"{launch_flops / (kern['fp64'] * 1e-3) / 1e12:.2f} TFLOP/s of "
-                               f"{FP64_PEAK_TFLOPS}"},
+                     "kernel": f"i8_gemm_kernel (int8 MFMA genotype GEMM), k={k}: "
+                               f"{kern['int8']:.3f} ms/launch (HIP events on the handle stream), "
+                               f"algorithmic 2*n*M*k = {launch_flops:.3g} op/launch; the three "
+                               f"operand slices execute 3x that (padded to 16-column tiles); "
+                               f"whole product {prod['int8']:.3f} ms; bf16 GEMM "
+                               f"{kern['bf16']:.3f} ms/launch; fp64 GEMM {kern['fp64']:.3f} "
+                               f"ms/launch = {launch_flops / (kern['fp64'] * 1e-3) / 1e12:.2f} "
+                               f"TFLOP/s of {FP64_PEAK_TFLOPS}"},
         "cpu_baseline": cpu,
         "step2": step2,
     }

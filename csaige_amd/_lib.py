@@ -15,6 +15,7 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_NOMEM", 4: "ERR_NOT_PD
           5: "ERR_NO_VR_MARKERS", 6: "ERR_NO_MARKERS"}
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_INT8 = 2
 TRAIT_BINARY = 0
 TRAIT_QUANTITATIVE = 1
 TRAIT_SURVIVAL = 2

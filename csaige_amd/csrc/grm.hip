@@ -224,7 +224,7 @@ int grm_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d
     SAIGE_HIP(hipGetLastError());
     return SAIGE_OK;
   }
-  if (h->precision == SAIGE_PREC_MIXED) return grm_mixed_product(h, d_v, ldv, k, d_out, ldo, d_w, tau);
+  if (h->precision != SAIGE_PREC_FP64) return grm_mixed_product(h, d_v, ldv, k, d_out, ldo, d_w, tau);
   for (size_t c0 = 0; c0 < k; c0 += 64) {
     size_t kc = std::min<size_t>(64, k - c0);
     const double* v = d_v + c0 * ldv;
@@ -342,7 +342,10 @@ int grm_create_impl(int device, const uint8_t* bed_body, bool on_device, size_t 
   SAIGE_ARG(n > 0, "n > 0");
   SAIGE_ARG(m == 0 || (bed_body && af), "bed_body/af");
   SAIGE_ARG(bpm >= (n_total + 3) / 4, "bytes_per_marker < ceil(n_total/4)");
-  SAIGE_ARG(precision == SAIGE_PREC_FP64 || precision == SAIGE_PREC_MIXED, "precision");
+  SAIGE_ARG(precision == SAIGE_PREC_FP64 || precision == SAIGE_PREC_MIXED ||
+                precision == SAIGE_PREC_INT8, "precision");
+  SAIGE_ARG(precision != SAIGE_PREC_INT8 || (n <= 8000000 && m <= 8000000),
+            "SAIGE_PREC_INT8 needs n and M <= 8,000,000 (exact int32 accumulation)");
   if (!sample_idx) SAIGE_ARG(n == n_total, "n must equal n_total without a subset");
   SAIGE_TRY(use_device(device));
   saige_grm* h = new saige_grm();
@@ -388,7 +391,7 @@ int grm_create_impl(int device, const uint8_t* bed_body, bool on_device, size_t 
     set_error("lut upload failed");
     return fail(SAIGE_ERR_DEVICE);
   }
-  if (precision == SAIGE_PREC_MIXED) {
+  if (precision != SAIGE_PREC_FP64) {
     if ((rc = h->scale64.ensure(g.m_pad * 8)) != SAIGE_OK) return fail(rc);
     if ((rc = h->shift64.ensure(g.m_pad * 8)) != SAIGE_OK) return fail(rc);
     g.scale64 = h->scale64.as<double>();

@@ -328,9 +328,12 @@ struct MixedState {
   size_t rbt = 0;
   // per-product workspaces
   DevBuf vhi, vlo, vs, whi, wlo, ws_, w, part1, part2, red, cs, mean, nu, s2p;
+  // int8 path: sliced operands, per-column slice scales, integer partials
+  DevBuf vq, wq, vscale, wscale, ipart1, ipart2;
 };
 
 static MixedState* mstate(saige_grm* h);
+MixedState* mixed_state(saige_grm* h) { return mstate(h); }
 
 int grm_mixed_prepare(saige_grm* h) {
   GrmDevice& g = h->g;
@@ -352,8 +355,8 @@ int grm_mixed_prepare(saige_grm* h) {
   SAIGE_HIP(hipMemcpyAsync(&nmiss, ms->nmiss.p, 4, hipMemcpyDeviceToHost, s));
   SAIGE_HIP(hipStreamSynchronize(s));
   if (nmiss) {
-    set_error("SAIGE_PREC_MIXED: %u missing genotype calls among the GRM markers; the mixed "
-              "path requires complete calls (use SAIGE_PREC_FP64)", nmiss);
+    set_error("reduced precision: %u missing genotype calls among the GRM markers; the "
+              "factored MFMA paths require complete calls (use SAIGE_PREC_FP64)", nmiss);
     return SAIGE_ERR_ARG;
   }
   // per-marker s_m, 2 p_m, zs_m = s_m (ac_m - 2 p_m n) with ac from the stored codes
@@ -469,8 +472,12 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   return SAIGE_OK;
 }
 
+int grm_int8_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
+                     size_t ldo, const double* d_w, const double* tau);
+
 int grm_mixed_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
                       size_t ldo, const double* d_w, const double* tau) {
+  if (h->precision == SAIGE_PREC_INT8) return grm_int8_product(h, d_v, ldv, k, d_out, ldo, d_w, tau);
   for (size_t c0 = 0; c0 < k; c0 += 128) {
     const size_t kc = std::min<size_t>(128, k - c0);
     const double* v = d_v + c0 * ldv;
@@ -498,6 +505,405 @@ static MixedState* mstate(saige_grm* h) {
 void grm_mixed_release(saige_grm* h) {
   delete static_cast<MixedState*>(h->mixed);
   h->mixed = nullptr;
+}
+
+
+// =================================================================================
+// Int8 path (SAIGE_PREC_INT8): exact genotypes times int8 operand slices on integer MFMA.
+// =================================================================================
+// The centred operand x' of each column is split into three int8 slices with per-column
+// scales s0 = max|x'| / 127, s1 = s0 / 254, s2 = s1 / 254:
+//   q0 = rint(x'/s0),  q1 = rint((x' - s0 q0)/s1),  q2 = rint((x' - s0 q0 - s1 q1)/s2),
+// each in [-127, 127], so |x' - sum_j s_j q_j| <= s2/2 = max|x'| / (2 * 127 * 254^2), about
+// 6.1e-8 of the column's largest entry.  G^T Q_j is computed exactly with
+// v_mfma_i32_16x16x64_i8 (int32 accumulation: |sum| <= 2 * 127 * n < 2^31 for n <= 8e6),
+// the slices are recombined in fp64, and x'_split = sum_j s_j q_j enters the rank-one
+// terms of the factored form exactly as in the bf16 path.  Integer partial sums make the
+// result independent of how the reduction is split.
+//
+// Operand layout: slice j of vector c is column j*k + c of a KP-column int8 matrix (columns
+// from 3k up to KP are zero), each column pad_len long in reduction order.  Within every
+// group of 16 reduction indices the bytes are stored 4x4-transposed (index 4b+u at byte
+// 4u+b): a lane expands one 32-bit word of packed codes (code 4b+u at bits 8b+2u) with
+// v_perm into four words, word u holding codes {u, 4+u, 8+u, 12+u} one per byte, and the
+// operand's bytes then pair with the same reduction indices.
+constexpr int kI8KC = 256;        // reduction indices per staged chunk
+constexpr int kI8MaxKT = 8;       // at most 128 slice columns (42 vectors) per launch
+constexpr size_t kI8MaxRed = 8000000;
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ size_t perm16(size_t i) {
+  return (i & ~(size_t)15) | ((i & 3) << 2) | ((i >> 2) & 3);
+}
+
+// per-column partial max of |x - mean| over chunks of kDotChunk rows
+__global__ void col_absmax_partial_kernel(const double* __restrict__ x, size_t ld, size_t len,
+                                          const double* __restrict__ mean, size_t nchunks,
+                                          double* __restrict__ part) {
+  const size_t c = blockIdx.y;
+  const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, len);
+  const double mu = mean[c];
+  double m = 0.0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) m = fmax(m, fabs(x[i + c * ld] - mu));
+  __shared__ double sh[kDotBlock];
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kDotBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[c * nchunks + blockIdx.x] = sh[0];
+}
+
+// slice scales of each column from its partial maxima: scale[3c + j] = s_j
+__global__ void slice_scales_kernel(const double* __restrict__ part, size_t nchunks, size_t k,
+                                    double* __restrict__ scale) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double m = 0.0;
+  for (size_t q = 0; q < nchunks; ++q) m = fmax(m, part[c * nchunks + q]);
+  const double s0 = m > 0.0 ? m / 127.0 : 1.0;
+  const double s1 = s0 / 254.0;
+  scale[3 * c] = s0;
+  scale[3 * c + 1] = s1;
+  scale[3 * c + 2] = s1 / 254.0;
+}
+
+__device__ __forceinline__ double q127(double t) { return fmin(fmax(rint(t), -127.0), 127.0); }
+
+// x' = x - mean -> three int8 slices (stored in the permuted reduction order, zero for
+// rows >= len and for the padding columns) and the fp64 value they represent (xs).
+// grid.y = k + (kp - 3k): the last kp - 3k rows of the grid clear the padding columns.
+__global__ void quantize3_kernel(const double* __restrict__ x, size_t ldx, size_t len, size_t k,
+                                 const double* __restrict__ mean, const double* __restrict__ scale,
+                                 size_t pad_len, int8_t* __restrict__ q, double* __restrict__ xs,
+                                 size_t ldxs) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (i >= pad_len) return;
+  const size_t pos = perm16(i);
+  if (c >= k) {
+    q[(2 * k + c) * pad_len + pos] = 0;  // padding column 3k + (c - k)
+    return;
+  }
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+  if (i < len) {
+    const double s0 = scale[3 * c], s1 = scale[3 * c + 1], s2 = scale[3 * c + 2];
+    const double xv = x[i + c * ldx] - mean[c];
+    t0 = q127(xv / s0);
+    double r = fma(-s0, t0, xv);
+    t1 = q127(r / s1);
+    r = fma(-s1, t1, r);
+    t2 = q127(r / s2);
+    xs[i + c * ldxs] = s0 * t0 + s1 * t1 + s2 * t2;
+  }
+  q[c * pad_len + pos] = (int8_t)t0;
+  q[(k + c) * pad_len + pos] = (int8_t)t1;
+  q[(2 * k + c) * pad_len + pos] = (int8_t)t2;
+}
+
+// part[split][row][16*KT] = sum over the split's reduction range of g(row, red) * q[col][red].
+// 4 waves; each wave owns 16*RT rows and reads their packed codes straight from global
+// memory (16 bytes = 64 genotypes per lane and chunk, the next chunk prefetched in
+// registers); the KP x 256 operand chunk is staged in LDS (column stride 272 bytes, so the
+// 16-byte fragment reads of 8 consecutive columns fall in distinct banks).
+template <int RT, int KT>
+__global__ __launch_bounds__(256) void i8_gemm_kernel(
+    const uint8_t* __restrict__ codes, size_t code_stride, const int8_t* __restrict__ bq,
+    size_t b_stride, size_t red_per_split, size_t red_len, int* __restrict__ part,
+    size_t rows_pad) {
+  constexpr int KP = 16 * KT;
+  constexpr int ROWS = 64 * RT;
+  constexpr int SB = kI8KC + 16;
+  constexpr int SEGS = kI8KC / 16;
+  constexpr int B_LOADS = KP * SEGS / 256;
+  constexpr int BCSTEP = 256 / SEGS;
+  __shared__ __attribute__((aligned(16))) int8_t sB[KP * SB];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 15, q = lane >> 4;
+  const size_t rowb = (size_t)blockIdx.x * ROWS + (size_t)wave * 16 * RT;
+  const size_t split = blockIdx.y;
+  const size_t k0 = split * red_per_split;
+  const size_t k1 = min(k0 + red_per_split, red_len);
+  const int bcol0 = tid / SEGS, bseg = tid % SEGS;
+  const uint8_t* abase = codes + (rowb + r) * code_stride + 16 * q;
+  const int8_t* bbase = bq + (size_t)bcol0 * b_stride + 16 * bseg;
+  i32x4 acc[RT][KT];
+#pragma unroll
+  for (int g = 0; g < RT; ++g)
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[g][t] = i32x4{0, 0, 0, 0};
+  u32x4 pa[RT], pb[B_LOADS], a_cur[RT];
+  if (k0 < k1) {
+#pragma unroll
+    for (int g = 0; g < RT; ++g)
+      pa[g] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + k0 / 4);
+#pragma unroll
+    for (int j = 0; j < B_LOADS; ++j)
+      pb[j] = *reinterpret_cast<const u32x4*>(bbase + (size_t)(BCSTEP * j) * b_stride + k0);
+  }
+  for (size_t kk = k0; kk < k1; kk += kI8KC) {
+    __syncthreads();  // the previous chunk's operand is no longer read
+#pragma unroll
+    for (int j = 0; j < B_LOADS; ++j)
+      *reinterpret_cast<u32x4*>(sB + (bcol0 + BCSTEP * j) * SB + 16 * bseg) = pb[j];
+#pragma unroll
+    for (int g = 0; g < RT; ++g) a_cur[g] = pa[g];
+    __syncthreads();
+    {
+      // next chunk; the last pass reloads its own chunk (in bounds, unused) so that the
+      // prefetch stays in registers
+      const size_t kn = (kk + kI8KC < k1) ? kk + kI8KC : kk;
+#pragma unroll
+      for (int g = 0; g < RT; ++g)
+        pa[g] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + kn / 4);
+#pragma unroll
+      for (int j = 0; j < B_LOADS; ++j)
+        pb[j] = *reinterpret_cast<const u32x4*>(bbase + (size_t)(BCSTEP * j) * b_stride + kn);
+    }
+#pragma unroll
+    for (int ks = 0; ks < kI8KC / 64; ++ks) {
+      i32x4 b[KT];
+#pragma unroll
+      for (int t = 0; t < KT; ++t)
+        b[t] = *reinterpret_cast<const i32x4*>(sB + (16 * t + r) * SB + 64 * q + 16 * ks);
+#pragma unroll
+      for (int g = 0; g < RT; ++g) {
+        const uint32_t w = a_cur[g][ks];
+        i32x4 a;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)  // codes 00, 01, 10, 11 -> 2, 0, 1, 0
+          a[u] = (int)__builtin_amdgcn_perm(0u, 0x00010002u, (w >> (2 * u)) & 0x03030303u);
+#pragma unroll
+        for (int t = 0; t < KT; ++t)
+          acc[g][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[t], acc[g][t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < RT; ++g)
+#pragma unroll
+    for (int t = 0; t < KT; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        part[(split * rows_pad + rowb + 16 * g + 4 * q + j) * KP + 16 * t + r] = acc[g][t][j];
+}
+
+// Sum the integer partials of 64 rows over the splits into an LDS tile (reads follow the
+// partial layout); the caller's epilogue then writes column-major outputs.
+template <int KP>
+__device__ __forceinline__ void i8_tile_sum(const int* __restrict__ part, size_t splits,
+                                            size_t rows_pad, size_t row0, int (*tile)[KP + 1]) {
+  for (int e = threadIdx.x; e < 64 * KP; e += 256) {
+    int acc = 0;
+    for (size_t s = 0; s < splits; ++s) acc += part[(s * rows_pad + row0) * KP + e];
+    tile[e / KP][e % KP] = acc;
+  }
+  __syncthreads();
+}
+
+// Stage-1 epilogue: U'_mc = sum_j s_jc I_jmc; T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c;
+// W_mc = s_m T_mc (column-major, ldw).
+template <int KP>
+__global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
+    const int* __restrict__ part, size_t splits, size_t m_pad, size_t m, size_t k,
+    const double* __restrict__ vscale, const double* __restrict__ sc,
+    const double* __restrict__ p2, const double* __restrict__ zs, const double* __restrict__ cs,
+    const double* __restrict__ mean, double* __restrict__ w, size_t ldw) {
+  __shared__ int tile[64][KP + 1];
+  const size_t row0 = (size_t)blockIdx.x * 64;
+  i8_tile_sum<KP>(part, splits, m_pad, row0, tile);
+  const int r = threadIdx.x & 63;
+  const size_t mm = row0 + r;
+  if (mm >= m) return;
+  const double s = sc[mm], pp = p2[mm], z = zs[mm];
+  for (size_t c = threadIdx.x >> 6; c < k; c += 4) {
+    const double u = vscale[3 * c] * (double)tile[r][c] +
+                     vscale[3 * c + 1] * (double)tile[r][k + c] +
+                     vscale[3 * c + 2] * (double)tile[r][2 * k + c];
+    const double t = s * (u - pp * cs[c]) + z * mean[c];
+    w[mm + c * ldw] = s * t;
+  }
+}
+
+// Stage-2 epilogue: M Y_ic = sum_j s_jc I_jic + nu_c (r_i - P2) - S2p_c; optional Sigma form.
+template <int KP>
+__global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
+    const int* __restrict__ part, size_t splits, size_t n_pad, size_t n, size_t k,
+    const double* __restrict__ wscale, const double* __restrict__ rsum, double P2,
+    const double* __restrict__ nu, const double* __restrict__ s2p, double inv_m,
+    double* __restrict__ out, size_t ldo, const double* __restrict__ v, size_t ldv,
+    const double* __restrict__ wts, double tau0, double tau1) {
+  __shared__ int tile[64][KP + 1];
+  const size_t row0 = (size_t)blockIdx.x * 64;
+  i8_tile_sum<KP>(part, splits, n_pad, row0, tile);
+  const int r = threadIdx.x & 63;
+  const size_t i = row0 + r;
+  if (i >= n) return;
+  const double ri = rsum[i];
+  for (size_t c = threadIdx.x >> 6; c < k; c += 4) {
+    double y = wscale[3 * c] * (double)tile[r][c] + wscale[3 * c + 1] * (double)tile[r][k + c] +
+               wscale[3 * c + 2] * (double)tile[r][2 * k + c];
+    y = (y + nu[c] * (ri - P2) - s2p[c]) * inv_m;
+    if (wts) y = tau0 * v[i + c * ldv] / fmax(wts[i], 1e-30) + tau1 * y;
+    out[i + c * ldo] = y;
+  }
+}
+
+// Work-groups of i8_gemm_kernel<RT, KT> resident at once on the device (cached per
+// instantiation).
+template <int RT, int KT>
+static size_t resident_slots() {
+  static size_t slots = 0;
+  if (!slots) {
+    int dev = 0, cus = 0, occ = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, i8_gemm_kernel<RT, KT>, 256, 0);
+    slots = (size_t)std::max(1, cus) * (size_t)std::max(1, occ);
+  }
+  return slots;
+}
+
+// Reduction splits: the fewest rounds of resident work-groups times chunks per split, with
+// a 2% charge per extra split for the partial-sum traffic.
+static size_t pick_splits(size_t rblk, size_t nchunks, size_t slots) {
+  size_t best = 1;
+  double best_cost = 1e300;
+  for (size_t sp = 1; sp <= std::min<size_t>(nchunks, 32); ++sp) {
+    const size_t per = ceil_div(nchunks, sp);
+    const size_t eff = ceil_div(nchunks, per);
+    const double rounds = (double)ceil_div(rblk * eff, slots);
+    const double cost = rounds * (double)per * (1.0 + 0.02 * (double)(eff - 1));
+    if (cost < best_cost - 1e-9) { best_cost = cost; best = eff; }
+  }
+  return best;
+}
+
+template <int RT, int KT>
+static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const int8_t* bq,
+                   size_t b_stride, size_t rows_pad, size_t red_len, DevBuf& part,
+                   size_t* splits_out) {
+  constexpr int KP = 16 * KT;
+  constexpr int ROWS = 64 * RT;
+  hipStream_t s = h->stream;
+  const size_t rblk = rows_pad / ROWS;
+  const size_t nch = red_len / kI8KC;
+  const size_t splits = pick_splits(rblk, nch, resident_slots<RT, KT>());
+  const size_t rps = ceil_div(nch, splits) * kI8KC;
+  const size_t sp = ceil_div(red_len, rps);
+  SAIGE_TRY(part.ensure(sp * rows_pad * KP * 4));
+  SAIGE_TRY(h->timer.begin(s));
+  i8_gemm_kernel<RT, KT><<<dim3((unsigned)rblk, (unsigned)sp), 256, 0, s>>>(
+      codes, code_stride, bq, b_stride, rps, red_len, part.as<int>(), rows_pad);
+  SAIGE_TRY(h->timer.end(s));
+  *splits_out = sp;
+  return SAIGE_OK;
+}
+
+template <int KT>
+static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
+                           size_t ldo, const double* d_w, const double* tau) {
+  constexpr int KP = 16 * KT;
+  constexpr int RT = KT <= 2 ? 4 : 8;
+  GrmDevice& g = h->g;
+  MixedState* ms = mstate(h);
+  hipStream_t s = h->stream;
+  const size_t n = g.n, m = g.m, n_pad = g.n_pad, m_pad = g.m_pad;
+  const size_t nch_n = std::max<size_t>(1, ceil_div(n, kDotChunk));
+  const size_t nch_m = std::max<size_t>(1, ceil_div(m, kDotChunk));
+  const unsigned pad_cols = (unsigned)(KP - 3 * k);
+  SAIGE_TRY(ms->vq.ensure(KP * n_pad));
+  SAIGE_TRY(ms->wq.ensure(KP * m_pad));
+  SAIGE_TRY(ms->vs.ensure(n * k * 8));
+  SAIGE_TRY(ms->ws_.ensure(m * k * 8));
+  SAIGE_TRY(ms->w.ensure(m * k * 8));
+  SAIGE_TRY(ms->vscale.ensure(3 * k * 8));
+  SAIGE_TRY(ms->wscale.ensure(3 * k * 8));
+  SAIGE_TRY(ms->red.ensure(std::max(nch_n, nch_m) * k * 8));
+  SAIGE_TRY(ms->cs.ensure(k * 8));
+  SAIGE_TRY(ms->mean.ensure(k * 8));
+  SAIGE_TRY(ms->nu.ensure(k * 8));
+  SAIGE_TRY(ms->s2p.ensure(k * 8));
+  double* red = ms->red.as<double>();
+  double* mean = ms->mean.as<double>();
+  double* cs = ms->cs.as<double>();
+  double* nu = ms->nu.as<double>();
+  double* s2p = ms->s2p.as<double>();
+  const unsigned kb = (unsigned)ceil_div(k, 256);
+
+  // ---- V: column means, slice scales, slices, cs' = colsum of the sliced value ----
+  col_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n, red);
+  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_n, k, 1.0 / (double)n, mean);
+  col_absmax_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(
+      d_v, ldv, n, mean, nch_n, red);
+  slice_scales_kernel<<<kb, 256, 0, s>>>(red, nch_n, k, ms->vscale.as<double>());
+  quantize3_kernel<<<dim3((unsigned)ceil_div(n_pad, 256), (unsigned)k + pad_cols), 256, 0, s>>>(
+      d_v, ldv, n, k, mean, ms->vscale.as<double>(), n_pad, ms->vq.as<int8_t>(),
+      ms->vs.as<double>(), n);
+  col_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(ms->vs.as<double>(), n,
+                                                                               n, nch_n, red);
+  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_n, k, 1.0, cs);
+
+  // ---- stage 1: I = G^T Q_v (rows = markers, reduction over samples) ----
+  size_t splits1 = 0;
+  SAIGE_TRY((i8_gemm<RT, KT>(h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad, n_pad,
+                              ms->ipart1, &splits1)));
+  i8_stage1_epilogue_kernel<KP><<<(unsigned)(m_pad / 64), 256, 0, s>>>(
+      ms->ipart1.as<int>(), splits1, m_pad, m, k, ms->vscale.as<double>(), ms->sc.as<double>(),
+      ms->p2.as<double>(), ms->zs.as<double>(), cs, mean, ms->w.as<double>(), m);
+
+  // ---- W: mean over real markers, slices, S2p = sum_m 2p_m W'_split ----
+  col_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(ms->w.as<double>(), m,
+                                                                               m, nch_m, red);
+  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_m, k, 1.0 / (double)m, nu);
+  col_absmax_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
+      ms->w.as<double>(), m, m, nu, nch_m, red);
+  slice_scales_kernel<<<kb, 256, 0, s>>>(red, nch_m, k, ms->wscale.as<double>());
+  quantize3_kernel<<<dim3((unsigned)ceil_div(m_pad, 256), (unsigned)k + pad_cols), 256, 0, s>>>(
+      ms->w.as<double>(), m, m, k, nu, ms->wscale.as<double>(), m_pad, ms->wq.as<int8_t>(),
+      ms->ws_.as<double>(), m);
+  wdot_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
+      ms->ws_.as<double>(), m, ms->p2.as<double>(), m, nch_m, red);
+  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_m, k, 1.0, s2p);
+
+  // ---- stage 2: I = G Q_w (rows = samples, reduction over markers) ----
+  size_t splits2 = 0;
+  SAIGE_TRY((i8_gemm<RT, KT>(h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(), m_pad,
+                              n_pad, m_pad, ms->ipart2, &splits2)));
+  i8_stage2_epilogue_kernel<KP><<<(unsigned)(n_pad / 64), 256, 0, s>>>(
+      ms->ipart2.as<int>(), splits2, n_pad, n, k, ms->wscale.as<double>(), ms->rsum.as<double>(),
+      ms->P2, nu, s2p, 1.0 / (double)m, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0,
+      tau ? tau[1] : 0.0);
+  SAIGE_HIP(hipGetLastError());
+  h->products += k;
+  return SAIGE_OK;
+}
+
+int grm_int8_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
+                     size_t ldo, const double* d_w, const double* tau) {
+  SAIGE_ARG(h->g.n_pad <= kI8MaxRed && h->g.m_pad <= kI8MaxRed,
+            "SAIGE_PREC_INT8 needs n and M <= 8,000,000 (exact int32 accumulation)");
+  constexpr size_t kMaxVec = 16 * kI8MaxKT / 3;  // 42 vectors per launch
+  for (size_t c0 = 0; c0 < k; c0 += kMaxVec) {
+    const size_t kc = std::min(kMaxVec, k - c0);
+    const double* v = d_v + c0 * ldv;
+    double* o = d_out + c0 * ldo;
+    int rc;
+    switch (ceil_div(3 * kc, 16)) {
+      case 1: rc = int8_product_kt<1>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+      case 2: rc = int8_product_kt<2>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+      case 3: rc = int8_product_kt<3>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+      case 4: rc = int8_product_kt<4>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+      case 5: rc = int8_product_kt<5>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+      case 6: rc = int8_product_kt<6>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+      case 7: rc = int8_product_kt<7>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+      default: rc = int8_product_kt<8>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+    }
+    if (rc != SAIGE_OK) return rc;
+  }
+  return SAIGE_OK;
 }
 
 }  // namespace saige

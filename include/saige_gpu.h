@@ -38,7 +38,7 @@ typedef enum {
   SAIGE_ERR_NO_MARKERS = 6      /* no marker passes QC */
 } saige_status;
 
-enum { SAIGE_PREC_FP64 = 0, SAIGE_PREC_MIXED = 1 };
+enum { SAIGE_PREC_FP64 = 0, SAIGE_PREC_MIXED = 1, SAIGE_PREC_INT8 = 2 };
 enum { SAIGE_TRAIT_BINARY = 0, SAIGE_TRAIT_QUANTITATIVE = 1, SAIGE_TRAIT_SURVIVAL = 2 };
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -73,6 +73,10 @@ int saige_plink_decode(int device, const uint8_t* bed_body, size_t bytes_per_mar
  *   precision SAIGE_PREC_FP64: f64 standardized values, f64 accumulation.
  *   precision SAIGE_PREC_MIXED: exact genotypes on bf16 MFMA with the standardization
  *   factored out, v split into bf16 hi + lo, fp32 accumulation (error stated per output).
+ *   precision SAIGE_PREC_INT8: exact genotypes on int8 MFMA with the standardization
+ *   factored out, the centred v split into three int8 slices with per-column scales
+ *   (|v - split| <= 6.1e-8 max|v - mean|), exact int32 accumulation; n, M <= 8,000,000.
+ *   Both reduced-precision modes refuse markers with missing calls.
  * ---------------------------------------------------------------------------- */
 typedef struct saige_grm saige_grm;
 
@@ -199,7 +203,7 @@ typedef struct {
   double min_maf;            /* 0.01 */
   double max_missing_rate;   /* 0.15 */
   uint32_t n_markers_vr;     /* 30 */
-  int precision;             /* SAIGE_PREC_FP64 | SAIGE_PREC_MIXED */
+  int precision;             /* SAIGE_PREC_FP64 | SAIGE_PREC_MIXED | SAIGE_PREC_INT8 */
 } saige_fit_null_config;
 
 typedef struct {

@@ -283,13 +283,20 @@ def test_pipeline_fit_null_then_score():
 
 
 # --------------------------------------------------------------------------- mixed precision
+# Stated error bounds of the reduced-precision products against the fp64 oracle, relative
+# to the column norm: bf16 hi+lo with fp32 accumulation 2e-5; three int8 slices with exact
+# int32 accumulation 1e-6 (per-entry operand error <= 6.1e-8 of the column's largest entry).
+_BOUND = {sg.PREC_MIXED: 2e-5, sg.PREC_INT8: 1e-6}
+
+
+@pytest.mark.parametrize("prec", [sg.PREC_MIXED, sg.PREC_INT8], ids=["bf16", "int8"])
 @pytest.mark.parametrize("n,m,k", [(1000, 1300, 1), (2048, 3000, 34), (700, 513, 100)])
-def test_grm_mixed_error_bound(n, m, k):
-    """bf16 MFMA on exact genotypes, bf16 hi+lo operand, fp32 accumulation: stated error
-    against the fp64 oracle, relative to the column norm (bound 2e-5)."""
-    _say(f"mixed grm n={n} m={m} k={k}")
+def test_grm_mixed_error_bound(n, m, k, prec):
+    """Exact genotypes on MFMA with the standardization factored out: stated error against
+    the fp64 oracle (k = 100 runs the int8 path as three column groups)."""
+    _say(f"mixed grm prec={prec} n={n} m={m} k={k}")
     body, bpm, d, af = _grm_pair(n, m, seed=3 * n + m, p_missing=0.0, mono=True)
-    g = sg.OnTheFlyGrm(body, bpm, n, af, precision=sg.PREC_MIXED)
+    g = sg.OnTheFlyGrm(body, bpm, n, af, precision=prec)
     ref = oracle.Grm(d, af, oracle.GRM_F64)
     rng = np.random.default_rng(k)
     V = rng.standard_normal((k, n)) + 3.0  # a common offset exercises the centring
@@ -298,23 +305,33 @@ def test_grm_mixed_error_bound(n, m, k):
     for c in range(k):
         want = ref.mat_vec(V[c])
         worst = max(worst, np.linalg.norm(got[c] - want) / np.linalg.norm(want))
-    _say(f"mixed grm worst relative error {worst:.2e}")
-    assert worst < 2e-5
+    _say(f"prec {prec} grm worst relative error {worst:.2e}")
+    assert worst < _BOUND[prec]
+    # one column alone gives the same values as inside the block (int8: bit for bit,
+    # the integer partial sums do not depend on how the work is split)
+    one = g.mat_mat(V[k // 2:k // 2 + 1])[0]
+    if prec == sg.PREC_INT8:
+        np.testing.assert_array_equal(one, got[k // 2])
+    else:
+        assert np.linalg.norm(one - got[k // 2]) < 1e-5 * np.linalg.norm(one)
 
 
-def test_grm_mixed_refuses_missing_calls():
+@pytest.mark.parametrize("prec", [sg.PREC_MIXED, sg.PREC_INT8], ids=["bf16", "int8"])
+def test_grm_mixed_refuses_missing_calls(prec):
     body, bpm, d, af = _grm_pair(300, 200, seed=1, p_missing=0.01)
     with pytest.raises(sg.SaigeError):
-        sg.OnTheFlyGrm(body, bpm, 300, af, precision=sg.PREC_MIXED)
+        sg.OnTheFlyGrm(body, bpm, 300, af, precision=prec)
 
 
-def test_fit_null_mixed_vs_fp64_oracle():
+@pytest.mark.parametrize("prec", [sg.PREC_MIXED, sg.PREC_INT8], ids=["bf16", "int8"])
+def test_fit_null_mixed_vs_fp64_oracle(prec):
     c = synth.unrelated(n=1500, m_grm=3000, m_test=0, seed=12, h2=0.5, prevalence=0.2)
     idx = np.arange(c.n, dtype=np.uint32)
     got = sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], c.y, c.x, sample_idx=idx,
-                      precision=sg.PREC_MIXED)
+                      precision=prec)
     want = oracle.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], idx, c.y, c.x)
-    _say(f"mixed tau={got['tau'][1]:.8g} oracle={want['tau'][1]:.8g} vr {got['vr']:.8g}/{want['vr']:.8g}")
+    _say(f"prec {prec} tau={got['tau'][1]:.8g} oracle={want['tau'][1]:.8g} "
+         f"vr {got['vr']:.8g}/{want['vr']:.8g}")
     np.testing.assert_array_equal(got["marker_class"], want["marker_class"])
     assert rel(got["tau"], want["tau"]).max() < 1e-4
     assert abs(got["vr"] - want["vr"]) < 1e-4 * want["vr"]
