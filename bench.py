@@ -33,6 +33,10 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 I8_PEAK_TOPS = 5000.0
+# HBM bytes per int8 GEMM launch at k = 34, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (one counter per run; FETCH_SIZE scaled by 2.0 from a 1 GiB copy) for the default families
+# workload: mean of the marker-side and sample-side launches.  Keyed by (families, markers).
+MEASURED_TRAFFIC = {(10000, 100000): 1.51e9}
 
 
 def log(msg):
@@ -81,7 +85,18 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="HBM bytes per GEMM launch measured with rocprofv3 counters")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device 3 times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from csaige_amd import hip
+        a, b = hip.DeviceBuffer(1 << 30), hip.DeviceBuffer(1 << 30)
+        a.zero()
+        for _ in range(3):
+            b.copy_from(a)
+        hip.device_synchronize()
+        log("copied 3 x 1 GiB device to device")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the multi-GPU "
               "sharding of the sample axis is a design, not built code", file=sys.stderr)
@@ -263,7 +278,8 @@ def main():
                    "product_ms": prod, "gemm_launch_ms": kern},
         "roofline": {"bound": "mfma", "achieved": mixed_tf, "peak": I8_PEAK_TOPS,
                      "unit": "TOP/s", "frac": mixed_tf / I8_PEAK_TOPS,
-                     "traffic": args.traffic_bytes,
+                     "traffic": (args.traffic_bytes if args.traffic_bytes is not None else
+                                 MEASURED_TRAFFIC.get((args.n_fam, args.markers))),
                      "kernel": f"i8_gemm_kernel (int8 MFMA genotype GEMM), k={k}: "
                                f"{kern['int8']:.3f} ms/launch (HIP events on the handle stream), "
                                f"algorithmic 2*n*M*k = {launch_flops:.3g} op/launch; the three "

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdarg>
 #include <cstdint>
@@ -69,29 +70,27 @@ struct DevBuf {
 
 int use_device(int device);
 
-// Optional per-handle timing of a handle's GEMM-class kernel launches: HIP events recorded
-// on the handle's stream immediately before and after each timed launch, read back (and
-// reset) by read().  At most kMaxPairs launches are recorded between reads.
+// Optional per-handle timing of a handle's GEMM-class kernel launches.  Timed launches go
+// through hipExtLaunchKernel with a start/stop event pair, so the events carry the
+// dispatch's own begin and end timestamps; read() sums them (and resets).  At most
+// kMaxPairs launches are recorded between reads.
 struct KernelTimer {
   static constexpr size_t kMaxPairs = 4096;
   bool on = false;
   std::vector<hipEvent_t> ev;
-  size_t used = 0;  // events recorded since the last read (always even after end())
-  int begin(hipStream_t s) {
+  size_t used = 0;  // events handed out since the last read (always even)
+  // Event pair for the next timed launch; null events when timing is off or full.
+  int pair(hipEvent_t* a, hipEvent_t* b) {
+    *a = *b = nullptr;
     if (!on || used / 2 >= kMaxPairs) return SAIGE_OK;
     while (ev.size() < used + 2) {
       hipEvent_t e;
       SAIGE_HIP(hipEventCreate(&e));
       ev.push_back(e);
     }
-    SAIGE_HIP(hipEventRecord(ev[used], s));
-    ++used;
-    return SAIGE_OK;
-  }
-  int end(hipStream_t s) {
-    if (!on || (used & 1) == 0) return SAIGE_OK;
-    SAIGE_HIP(hipEventRecord(ev[used], s));
-    ++used;
+    *a = ev[used];
+    *b = ev[used + 1];
+    used += 2;
     return SAIGE_OK;
   }
   int read(double* total_ms, uint64_t* launches) {
@@ -111,6 +110,16 @@ struct KernelTimer {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   }
 };
+
+// Launch a kernel on stream s, timed by t when its timing is on.
+template <class F, class... Args>
+int timed_launch(KernelTimer& t, F kernel, dim3 grid, dim3 block, hipStream_t s, Args... args) {
+  hipEvent_t a, b;
+  SAIGE_TRY(t.pair(&a, &b));
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, s, a, b, 0, args...);
+  SAIGE_HIP(hipGetLastError());
+  return SAIGE_OK;
+}
 
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 inline size_t ceil_div(size_t x, size_t m) { return (x + m - 1) / m; }

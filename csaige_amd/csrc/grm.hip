@@ -180,17 +180,14 @@ static int product_k(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
 
   transpose_v_kernel<K><<<(unsigned)ceil_div(g.n_pad, 256), 256, 0, s>>>(
       d_v, ldv, g.n, k, h->ws_vt.as<double>(), g.n_pad);
-  SAIGE_TRY(h->timer.begin(s));
-  grm_s1_f64<K><<<dim3((unsigned)nblk_m, (unsigned)splits1), 256, 0, s>>>(
-      g.geno, g.row_bytes, g.lut, h->ws_vt.as<double>(), bps, h->ws_tpart.as<double>(), g.m_pad);
-  SAIGE_TRY(h->timer.end(s));
+  SAIGE_TRY(timed_launch(h->timer, grm_s1_f64<K>, dim3((unsigned)nblk_m, (unsigned)splits1),
+                         dim3(256), s, g.geno, g.row_bytes, g.lut, h->ws_vt.as<double>(), bps,
+                         h->ws_tpart.as<double>(), g.m_pad));
   grm_s1_reduce<K><<<(unsigned)ceil_div(g.m_pad * K, 256), 256, 0, s>>>(
       h->ws_tpart.as<double>(), splits1, g.m_pad, h->ws_t.as<double>());
-  SAIGE_TRY(h->timer.begin(s));
-  grm_s2_f64<K><<<dim3((unsigned)nblk_n, (unsigned)splits2), 256, 0, s>>>(
-      g.geno, g.row_bytes, g.lut, h->ws_t.as<double>(), mps, g.m, h->ws_ypart.as<double>(),
-      g.n_pad);
-  SAIGE_TRY(h->timer.end(s));
+  SAIGE_TRY(timed_launch(h->timer, grm_s2_f64<K>, dim3((unsigned)nblk_n, (unsigned)splits2),
+                         dim3(256), s, g.geno, g.row_bytes, g.lut, h->ws_t.as<double>(), mps,
+                         g.m, h->ws_ypart.as<double>(), g.n_pad));
   grm_s2_reduce<K><<<dim3((unsigned)ceil_div(g.n, 256), (unsigned)k), 256, 0, s>>>(
       h->ws_ypart.as<double>(), splits2, g.n_pad, g.n, k, 1.0 / (double)g.m, d_out, ldo, d_v,
       ldv, d_w, tau ? tau[0] : 0.0, tau ? tau[1] : 0.0);

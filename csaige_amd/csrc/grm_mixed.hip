@@ -433,11 +433,9 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   size_t rps1 = round_up(ceil_div(n_pad, splits1), kKC);
   splits1 = ceil_div(n_pad, rps1);
   SAIGE_TRY(ms->part1.ensure(splits1 * m_pad * KP * 4));
-  SAIGE_TRY(h->timer.begin(s));
-  mix_gemm_lds_kernel<KT><<<dim3((unsigned)rblk1, (unsigned)splits1), 256, 0, s>>>(
-      g.geno, g.row_bytes, ms->vhi.as<__bf16>(), ms->vlo.as<__bf16>(), n_pad, rps1, n_pad,
-      ms->part1.as<float>(), m_pad);
-  SAIGE_TRY(h->timer.end(s));
+  SAIGE_TRY(timed_launch(h->timer, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk1, (unsigned)splits1),
+                         dim3(256), s, g.geno, g.row_bytes, ms->vhi.as<__bf16>(),
+                         ms->vlo.as<__bf16>(), n_pad, rps1, n_pad, ms->part1.as<float>(), m_pad));
   stage1_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(m, 256), (unsigned)k), 256, 0, s>>>(
       ms->part1.as<float>(), splits1, m_pad, m, k, ms->sc.as<double>(), ms->p2.as<double>(),
       ms->zs.as<double>(), cs, mean, ms->w.as<double>(), m);
@@ -459,11 +457,9 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   size_t rps2 = round_up(ceil_div(m_pad, splits2), kKC);
   splits2 = ceil_div(m_pad, rps2);
   SAIGE_TRY(ms->part2.ensure(splits2 * n_pad * KP * 4));
-  SAIGE_TRY(h->timer.begin(s));
-  mix_gemm_lds_kernel<KT><<<dim3((unsigned)rblk2, (unsigned)splits2), 256, 0, s>>>(
-      ms->geno_t.as<uint8_t>(), ms->rbt, ms->whi.as<__bf16>(), ms->wlo.as<__bf16>(), m_pad, rps2,
-      m_pad, ms->part2.as<float>(), n_pad);
-  SAIGE_TRY(h->timer.end(s));
+  SAIGE_TRY(timed_launch(h->timer, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk2, (unsigned)splits2),
+                         dim3(256), s, ms->geno_t.as<uint8_t>(), ms->rbt, ms->whi.as<__bf16>(),
+                         ms->wlo.as<__bf16>(), m_pad, rps2, m_pad, ms->part2.as<float>(), n_pad));
   stage2_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(
       ms->part2.as<float>(), splits2, n_pad, n, k, ms->rsum.as<double>(), ms->P2, nu, s2p,
       1.0 / (double)m, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0, tau ? tau[1] : 0.0);
@@ -794,10 +790,9 @@ static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const
   const size_t rps = ceil_div(nch, splits) * kI8KC;
   const size_t sp = ceil_div(red_len, rps);
   SAIGE_TRY(part.ensure(sp * rows_pad * KP * 4));
-  SAIGE_TRY(h->timer.begin(s));
-  i8_gemm_kernel<RT, KT><<<dim3((unsigned)rblk, (unsigned)sp), 256, 0, s>>>(
-      codes, code_stride, bq, b_stride, rps, red_len, part.as<int>(), rows_pad);
-  SAIGE_TRY(h->timer.end(s));
+  SAIGE_TRY(timed_launch(h->timer, i8_gemm_kernel<RT, KT>, dim3((unsigned)rblk, (unsigned)sp),
+                         dim3(256), s, codes, code_stride, bq, b_stride, rps, red_len,
+                         part.as<int>(), rows_pad));
   *splits_out = sp;
   return SAIGE_OK;
 }

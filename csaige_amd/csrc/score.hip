@@ -336,7 +336,7 @@ __global__ void score_finalize_kernel(const double* __restrict__ part, size_t sp
 // ---------------------------------------------------------------------------------
 // SPA
 // ---------------------------------------------------------------------------------
-constexpr int kSpaBlock = 256;
+constexpr int kSpaBlock = 1024;  // 16 waves: 4 per SIMD hide the fp64 latency of one variant
 
 __device__ __forceinline__ void block_sum4(double* v, double* sh) {
   // sh: 4 * kSpaBlock doubles; deterministic tree

@@ -59,6 +59,10 @@ class DeviceBuffer:
         _chk(lib().hipMemcpy(arr.ctypes.data, C.c_void_p(self.ptr.value + offset), arr.nbytes, D2H), "D2H")
         return arr
 
+    def copy_from(self, other, nbytes=None):
+        nb = self.nbytes if nbytes is None else int(nbytes)
+        _chk(lib().hipMemcpy(self.ptr, other.ptr, nb, D2D), "D2D")
+
     def zero(self):
         _chk(lib().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
 
