@@ -25,7 +25,7 @@ from ._lib import (PREC_FP64, PREC_MIXED, PREC_INT8, TRAIT_BINARY, TRAIT_QUANTIT
                    SaigeError, check, load)
 
 __all__ = ["OnTheFlyGrm", "fit_ai_reml", "solve_sigma", "xvx_inv_xv", "estimate_variance_ratio",
-           "fit_null", "ScoreTestEngine", "spa_binary_fast", "marker_stats", "decode",
+           "fit_null", "fit_null_multi", "ScoreTestEngine", "spa_binary_fast", "marker_stats", "decode",
            "PREC_FP64", "PREC_MIXED", "PREC_INT8", "TRAIT_BINARY", "TRAIT_QUANTITATIVE", "TRAIT_SURVIVAL",
            "SaigeError", "load", "results_to_dict", "SpaRootMlp", "spa_training_set"]
 
@@ -228,6 +228,10 @@ def fit_null(bed_body, bytes_per_marker, n_total, m_total, y, x_cm, sample_idx=N
         check(L.saige_fit_null(device, _p(body, _lib.u8p), *args))
     else:
         check(L.saige_fit_null_device(device, C.c_void_p(d_bed), *args))
+    return _fit_null_dict(r, mu, a, cls)
+
+
+def _fit_null_dict(r, mu, a, cls):
     rr = r.reml
     return dict(tau=np.array(rr.tau[:]), iterations=rr.iterations, converged=bool(rr.converged),
                 log_likelihood=rr.log_likelihood, grm_products=rr.grm_products,
@@ -237,6 +241,40 @@ def fit_null(bed_body, bytes_per_marker, n_total, m_total, y, x_cm, sample_idx=N
                 xvx_inv_xv=a, marker_class=cls, seconds_qc=r.seconds_qc,
                 seconds_reml=r.seconds_reml, seconds_vr=r.seconds_vr,
                 seconds_total=r.seconds_total)
+
+
+def fit_null_multi(bed_body, bytes_per_marker, n_total, m_total, Y, x_cm, traits=None,
+                   sample_idx=None, precision=PREC_FP64, max_iter=30, tol=1e-5, seed=12345,
+                   n_random_vectors=30, min_maf=0.01, max_missing=0.15, n_markers_vr=30,
+                   device=0, d_bed=None):
+    """Several phenotypes (rows of ``Y``, model sample order) fitted together over one GRM:
+    each trait follows its own fit_null, the GRM products of all traits are joined into one
+    multi-column product.  Returns one fit_null dict per trait."""
+    L = load()
+    body = _u8(bed_body) if d_bed is None else None
+    si, sp = _idx(sample_idx)
+    n = n_total if si is None else len(si)
+    Y = np.ascontiguousarray(np.atleast_2d(Y), dtype=np.float64)
+    T = Y.shape[0]
+    if Y.shape[1] != n:
+        raise ValueError("Y must be n_traits x n")
+    x = _f64(x_cm)
+    p = x.shape[0]
+    tt = np.ascontiguousarray([TRAIT_BINARY] * T if traits is None else traits, dtype=np.int32)
+    cfg = _lib.FitNullConfig(_lib.AiRemlConfig(max_iter, tol, 1e-5, 500, n_random_vectors, seed),
+                             min_maf, max_missing, n_markers_vr, precision)
+    res = (_lib.FitNullResult * T)()
+    mu = np.empty((T, n))
+    a = np.empty((T, p * n))
+    cls = np.empty(m_total, np.uint8)
+    args = (bytes_per_marker, n_total, m_total, sp, n, _p(Y, _lib.dp), T, _p(x, _lib.dp), p,
+            tt.ctypes.data_as(C.POINTER(C.c_int)), C.byref(cfg), res, _p(mu, _lib.dp),
+            _p(a, _lib.dp), _p(cls, _lib.u8p))
+    if d_bed is None:
+        check(L.saige_fit_null_multi(device, _p(body, _lib.u8p), *args))
+    else:
+        check(L.saige_fit_null_multi_device(device, C.c_void_p(d_bed), *args))
+    return [_fit_null_dict(res[t], mu[t], a[t], cls) for t in range(T)]
 
 
 def results_to_dict(arr):

@@ -150,6 +150,26 @@ struct GrmDevice {
 
 }  // namespace saige
 
+namespace saige {
+
+// Where a fit sends its GRM products: straight to the handle, or to a batcher that joins
+// the products of several concurrent fits (phenotypes sharing the GRM) into one
+// multi-column product.  out = GRM v, or Sigma v = tau0 v / max(w, 1e-30) + tau1 GRM v when
+// w is non-null.  `columns` counts the column-products this sink has served.
+struct ProductSink {
+  uint64_t columns = 0;
+  virtual int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo,
+                      const double* w, const double* tau) = 0;
+  virtual ~ProductSink() = default;
+};
+
+// Workspace of one batched PCG solve (one per concurrent fit).
+struct PcgWork {
+  DevBuf vec, scal, part;
+};
+
+}  // namespace saige
+
 struct saige_grm {
   int device = 0;
   int precision = SAIGE_PREC_FP64;
@@ -160,8 +180,9 @@ struct saige_grm {
   // workspaces for products and solves
   saige::DevBuf ws_vt, ws_tpart, ws_t, ws_ypart, ws_in, ws_out;
   saige::DevBuf ws_vhi, ws_vlo, ws_tmp;
-  // PCG workspaces
-  saige::DevBuf pcg_vec, pcg_scal, pcg_part, pcg_b, pcg_w;
+  // PCG workspaces of the handle's own solves
+  saige::PcgWork pcg;
+  saige::DevBuf pcg_b, pcg_w;
   uint64_t products = 0;   // column-products performed (diagnostic)
   saige::KernelTimer timer; // GEMM kernel launch timing (diagnostic)
   void* mixed = nullptr;   // mixed-precision state (grm_mixed.hip)
@@ -183,9 +204,21 @@ int col_dots(hipStream_t s, const double* a, size_t lda, const double* b, size_t
 int gram(hipStream_t s, const double* A, size_t lda, size_t na, const double* B, size_t ldb,
          size_t nb, size_t n, double* h_out, DevBuf& part, DevBuf& res);
 
-// Batched PCG on Sigma (device pointers; B, X: n x k with leading dimension ld).
-int pcg_sigma_device(saige_grm* h, const double* d_w, const double tau[2], const double* d_b,
-                     size_t ldb, size_t k, double tol, uint32_t max_iter, double* d_x, size_t ldx,
-                     saige_pcg_info* info);
+// The handle itself as a product sink.
+struct DirectSink : ProductSink {
+  saige_grm* h;
+  explicit DirectSink(saige_grm* hh) : h(hh) {}
+  int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
+              const double* tau) override {
+    columns += k;
+    return grm_product(h, v, ldv, k, out, ldo, w, tau);
+  }
+};
+
+// Batched PCG on Sigma (device pointers; B, X: n x k with leading dimension ld), products
+// through `sink`, vectors in `ws`.
+int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double* d_w,
+                     const double tau[2], const double* d_b, size_t ldb, size_t k, double tol,
+                     uint32_t max_iter, double* d_x, size_t ldx, saige_pcg_info* info);
 
 }  // namespace saige

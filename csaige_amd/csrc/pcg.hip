@@ -205,25 +205,25 @@ __global__ void pcg_final(size_t k, uint32_t max_iter, ColState* __restrict__ st
   st[c].active = 0;
 }
 
-int pcg_sigma_device(saige_grm* h, const double* d_w, const double tau[2], const double* d_b,
-                     size_t ldb, size_t k, double tol, uint32_t max_iter, double* d_x, size_t ldx,
-                     saige_pcg_info* info) {
+int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double* d_w,
+                     const double tau[2], const double* d_b, size_t ldb, size_t k, double tol,
+                     uint32_t max_iter, double* d_x, size_t ldx, saige_pcg_info* info) {
   if (k == 0) return SAIGE_OK;
   SAIGE_ARG(k <= 1024, "k <= 1024 right-hand sides per solve");
   const size_t n = h->g.n;
   const size_t ld = round_up(n, 64);
   hipStream_t s = h->stream;
   const size_t nchunks = std::max<size_t>(1, ceil_div(n, kDotChunk));
-  SAIGE_TRY(h->pcg_vec.ensure(4 * ld * k * 8));
-  SAIGE_TRY(h->pcg_part.ensure(3 * k * nchunks * 8));
-  SAIGE_TRY(h->pcg_scal.ensure(k * sizeof(ColState) + 64));
-  double* R = h->pcg_vec.as<double>();
+  SAIGE_TRY(ws.vec.ensure(4 * ld * k * 8));
+  SAIGE_TRY(ws.part.ensure(3 * k * nchunks * 8));
+  SAIGE_TRY(ws.scal.ensure(k * sizeof(ColState) + 64));
+  double* R = ws.vec.as<double>();
   double* Z = R + ld * k;
   double* P = Z + ld * k;
   double* AP = P + ld * k;
-  ColState* st = reinterpret_cast<ColState*>(h->pcg_scal.as<char>() + 64);
-  int* d_nact = h->pcg_scal.as<int>();
-  double* part = h->pcg_part.as<double>();
+  ColState* st = reinterpret_cast<ColState*>(ws.scal.as<char>() + 64);
+  int* d_nact = ws.scal.as<int>();
+  double* part = ws.part.as<double>();
   const double tau0 = tau[0], tau1 = tau[1];
   const unsigned kb = (unsigned)ceil_div(k, 256);
   dim3 gchunk((unsigned)nchunks, (unsigned)k);
@@ -241,7 +241,7 @@ int pcg_sigma_device(saige_grm* h, const double* d_w, const double tau[2], const
   }
   int pinned_nact = 0;
   for (uint32_t iter = 0; iter < max_iter && n_active > 0; ++iter) {
-    SAIGE_TRY(grm_product(h, P, ld, k, AP, ld, d_w, tau));
+    SAIGE_TRY(sink->product(P, ld, k, AP, ld, d_w, tau));
     pcg_pap_kernel<<<gchunk, kDotBlock, 0, s>>>(P, AP, ld, n, nchunks, st, part);
     pcg_alpha_fin<<<kb, 256, 0, s>>>(part, nchunks, k, (int)iter, st);
     pcg_update_kernel<<<gchunk, kDotBlock, 0, s>>>(d_x, ldx, R, Z, P, AP, ld, d_w, tau0, tau1, n,
@@ -290,7 +290,8 @@ extern "C" int saige_pcg_solve_sigma(saige_grm* h, const double* w, const double
   double* dx = db + n * k;
   SAIGE_HIP(hipMemcpyAsync(dw, w, n * 8, hipMemcpyHostToDevice, h->stream));
   SAIGE_HIP(hipMemcpy2DAsync(db, n * 8, b, ldb * 8, n * 8, k, hipMemcpyHostToDevice, h->stream));
-  SAIGE_TRY(pcg_sigma_device(h, dw, tau, db, n, k, tol, max_iter, dx, n, info));
+  DirectSink sink(h);
+  SAIGE_TRY(pcg_sigma_device(h, &sink, h->pcg, dw, tau, db, n, k, tol, max_iter, dx, n, info));
   SAIGE_HIP(hipMemcpy2DAsync(x, ldx * 8, dx, n * 8, n * 8, k, hipMemcpyDeviceToHost, h->stream));
   SAIGE_HIP(hipStreamSynchronize(h->stream));
   return SAIGE_OK;

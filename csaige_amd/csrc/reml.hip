@@ -19,7 +19,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <mutex>
 #include <numeric>
+#include <string>
+#include <thread>
 
 namespace saige {
 
@@ -190,19 +194,20 @@ static bool fit_glm_irls(const double* y, const double* x, size_t n, size_t p, i
 
 struct RemlState {
   DevBuf x, y, eta, mu, w, rhs1, sol1, gu, py, gpy, pu, rhs2, sol2, tmp, coef, part, res, ones;
+  PcgWork pcg;
 };
 
 // fit_ai_reml with X (n x p, host col-major) and y (host); outputs stay on device in st.
-static int fit_ai_reml_dev(saige_grm* h, const double* y, const double* x, size_t n, size_t p,
-                           int trait, const saige_ai_reml_config* cfg, saige_ai_reml_result* res,
-                           RemlState& st) {
+static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, const double* x,
+                           size_t n, size_t p, int trait, const saige_ai_reml_config* cfg,
+                           saige_ai_reml_result* res, RemlState& st) {
   hipStream_t s = h->stream;
   const size_t ld = round_up(n, 64);
   const size_t R = cfg->n_random_vectors;
   const bool fix_tau_e = (trait != SAIGE_TRAIT_QUANTITATIVE);
   const size_t k1 = 1 + p + R;
   *res = saige_ai_reml_result{};
-  const uint64_t prod0 = h->products;
+  const uint64_t prod0 = sink->columns;
 
   std::vector<double> alpha;
   if (!fit_glm_irls(y, x, n, p, trait, 25, alpha)) {
@@ -255,7 +260,7 @@ static int fit_ai_reml_dev(saige_grm* h, const double* y, const double* x, size_
     SAIGE_HIP(hipStreamSynchronize(s));
   }
   const double* U = RHS1 + (1 + p) * ld;
-  if (R > 0) SAIGE_TRY(grm_product(h, U, ld, R, GU, ld));  // GRM * u, constant over iterations
+  if (R > 0) SAIGE_TRY(sink->product(U, ld, R, GU, ld, nullptr, nullptr));  // constant GRM * u
 
   // eta, mu from the GLM fit
   SAIGE_HIP(hipMemcpyAsync(COEF, alpha.data(), p * 8, hipMemcpyHostToDevice, s));
@@ -275,8 +280,8 @@ static int fit_ai_reml_dev(saige_grm* h, const double* y, const double* x, size_
     n_iter = iter + 1;
     reml_prep_kernel<<<grid1(n), 256, 0, s>>>(dY, dEta, dMu, trait, n, dW, RHS1);
     SAIGE_HIP(hipGetLastError());
-    SAIGE_TRY(pcg_sigma_device(h, dW, tau, RHS1, ld, k1, cfg->pcg_tol, cfg->pcg_max_iter, SOL1, ld,
-                               info.data()));
+    SAIGE_TRY(pcg_sigma_device(h, sink, st.pcg, dW, tau, RHS1, ld, k1, cfg->pcg_tol,
+                               cfg->pcg_max_iter, SOL1, ld, info.data()));
     for (size_t c = 0; c < k1; ++c) {
       pcg_iters += info[c].iterations;
       pcg_fail += info[c].converged ? 0 : 1;
@@ -306,7 +311,7 @@ static int fit_ai_reml_dev(saige_grm* h, const double* y, const double* x, size_
     // P y = Sigma^-1 y - Sigma^-1 X alpha (ai_reml.rs:221-226)
     project_kernel<<<grid2(n, 1), 256, 0, s>>>(SIY, ld, SIX, ld, p, COEF, n, 1, PY, ld);
     SAIGE_HIP(hipGetLastError());
-    SAIGE_TRY(grm_product(h, PY, ld, 1, GPY, ld));
+    SAIGE_TRY(sink->product(PY, ld, 1, GPY, ld, nullptr, nullptr));
     double ypapy;
     SAIGE_TRY(col_dots(s, PY, ld, GPY, ld, n, 1, &ypapy, st.part, st.res));
     double trace = 0.0;
@@ -328,8 +333,8 @@ static int fit_ai_reml_dev(saige_grm* h, const double* y, const double* x, size_
       div_w_kernel<<<grid2(n, 1), 256, 0, s>>>(PY, ld, dW, n, 1, RHS2 + ld, ld);
       SAIGE_HIP(hipGetLastError());
     }
-    SAIGE_TRY(pcg_sigma_device(h, dW, tau, RHS2, ld, k2, cfg->pcg_tol, cfg->pcg_max_iter, SOL2,
-                               ld, info.data()));
+    SAIGE_TRY(pcg_sigma_device(h, sink, st.pcg, dW, tau, RHS2, ld, k2, cfg->pcg_tol,
+                               cfg->pcg_max_iter, SOL2, ld, info.data()));
     for (size_t c = 0; c < k2; ++c) {
       pcg_iters += info[c].iterations;
       pcg_fail += info[c].converged ? 0 : 1;
@@ -426,7 +431,7 @@ static int fit_ai_reml_dev(saige_grm* h, const double* y, const double* x, size_
   res->iterations = n_iter;
   res->converged = converged ? 1 : 0;
   res->log_likelihood = ll;
-  res->grm_products = h->products - prod0;
+  res->grm_products = sink->columns - prod0;
   res->pcg_iterations = pcg_iters;
   res->pcg_not_converged = pcg_fail;
   st.coef.ensure(std::max<size_t>(p, 8) * 8);
@@ -546,7 +551,8 @@ static int vr_fill(hipStream_t s, const VrSource& src, size_t n, const std::vect
   return SAIGE_OK;
 }
 
-static int estimate_vr_dev(saige_grm* h, const VrSource& src, const double* mac, size_t m,
+static int estimate_vr_dev(saige_grm* h, ProductSink* sink, const VrSource& src,
+                           const double* mac, size_t m,
                            const double* d_mu, const double* d_w, const double tau[2], int trait,
                            const double* d_x, size_t ldx, const double* a_flat, size_t p,
                            const saige_vr_config* cfg, double* vr_out, uint32_t* n_used,
@@ -605,8 +611,8 @@ static int estimate_vr_dev(saige_grm* h, const VrSource& src, const double* mac,
     adjust_kernel<<<grid2(n, k), 256, 0, s>>>(dG, d_x, ldx, p, agb.as<double>(), n, k, GT.as<double>());
     SAIGE_HIP(hipGetLastError());
     std::vector<saige_pcg_info> info(k);
-    SAIGE_TRY(pcg_sigma_device(h, d_w, tau, GT.as<double>(), ld, k, 1e-5, 500, SG.as<double>(), ld,
-                               info.data()));
+    SAIGE_TRY(pcg_sigma_device(h, sink, st.pcg, d_w, tau, GT.as<double>(), ld, k, 1e-5, 500,
+                               SG.as<double>(), ld, info.data()));
     SAIGE_TRY(col_dots(s, GT.as<double>(), ld, SG.as<double>(), ld, n, k, v1.data(), st.part, st.res));
     SAIGE_HIP(hipMemcpyAsync(acb.p, acs.data(), k * 8, hipMemcpyHostToDevice, s));
     var2_weight_kernel<<<grid2(n, k), 256, 0, s>>>(GT.as<double>(), ld, d_mu, trait, acb.as<double>(),
@@ -659,7 +665,8 @@ int saige_fit_ai_reml(saige_grm* h, const double* y, const double* x, size_t n, 
   std::lock_guard<std::recursive_mutex> lk(h->mu);
   SAIGE_HIP(hipSetDevice(h->device));
   RemlState st;
-  SAIGE_TRY(fit_ai_reml_dev(h, y, x, n, p, trait_type, cfg, result, st));
+  DirectSink sink(h);
+  SAIGE_TRY(fit_ai_reml_dev(h, &sink, y, x, n, p, trait_type, cfg, result, st));
   hipStream_t s = h->stream;
   std::vector<double> hm(n), hw(n);
   SAIGE_HIP(hipMemcpyAsync(hm.data(), st.mu.p, n * 8, hipMemcpyDeviceToHost, s));
@@ -706,7 +713,8 @@ int saige_estimate_variance_ratio(saige_grm* h, const double* dosages, const dou
     set_error("No markers available for variance ratio estimation");
     return SAIGE_ERR_NO_VR_MARKERS;
   }
-  return estimate_vr_dev(h, src, mac, m, dmu.as<double>(), dw.as<double>(), tau, trait_type,
+  DirectSink sink(h);
+  return estimate_vr_dev(h, &sink, src, mac, m, dmu.as<double>(), dw.as<double>(), tau, trait_type,
                          dx.as<double>(), ld, xvx_inv_xv, p, cfg, variance_ratio, n_used,
                          per_marker_vr, st);
 }
@@ -715,16 +723,15 @@ int saige_estimate_variance_ratio(saige_grm* h, const double* dosages, const dou
 
 namespace saige {
 
-static int fit_null_impl(int device, const uint8_t* bed_body, bool on_device, size_t bpm,
-                         size_t n_total, size_t m_total, const uint32_t* sample_idx, size_t n,
-                         const double* y, const double* x, size_t p, int trait_type,
-                         const saige_fit_null_config* cfg, saige_fit_null_result* result,
-                         double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
-  SAIGE_ARG(bed_body && y && x && cfg && result, "null pointer");
-  SAIGE_ARG(p >= 1 && n > 0, "dimensions");
-  using clk = std::chrono::steady_clock;
-  auto t0 = clk::now();
-  *result = saige_fit_null_result{};
+// QC and the VR/GRM marker split shared by every trait of a fit (fit_null.rs:199-264).
+struct MarkerSplit {
+  std::vector<uint32_t> grm_ids, vr_ids;
+  std::vector<double> grm_af, vr_mac;
+};
+
+static int qc_split(int device, const uint8_t* bed_body, bool on_device, size_t bpm, size_t n_total,
+                    size_t m_total, const uint32_t* sample_idx, size_t n,
+                    const saige_fit_null_config* cfg, MarkerSplit& ms, uint8_t* marker_class) {
   // QC over all markers (fit_null.rs:209-222)
   std::vector<double> ac(m_total), af(m_total), mac(m_total);
   std::vector<uint32_t> nv(m_total);
@@ -767,82 +774,310 @@ static int fit_null_impl(int device, const uint8_t* bed_body, bool on_device, si
     for (uint64_t c : cand)
       if (mac[passing[c]] >= 20.0) in_vr[c] = 1;
   }
-  std::vector<uint32_t> grm_ids, vr_ids;
-  std::vector<double> grm_af, vr_mac;
   if (marker_class) std::fill(marker_class, marker_class + m_total, 0);
   for (size_t i = 0; i < np; ++i) {
     uint32_t j = passing[i];
     if (in_vr[i]) {
-      vr_ids.push_back(j);
-      vr_mac.push_back(mac[j]);
+      ms.vr_ids.push_back(j);
+      ms.vr_mac.push_back(mac[j]);
       if (marker_class) marker_class[j] = 2;
     } else {
-      grm_ids.push_back(j);
-      grm_af.push_back(af[j]);
+      ms.grm_ids.push_back(j);
+      ms.grm_af.push_back(af[j]);
       if (marker_class) marker_class[j] = 1;
     }
   }
+  return SAIGE_OK;
+}
+
+// One trait after the GRM exists: AI-REML, XVX^-1X'V and the variance ratio
+// (fit_null.rs:267-373), every GRM product through `sink`.  vr_rows: the VR markers as
+// packed rows in model order (shared, read-only).
+static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const double* x, size_t p,
+                     int trait_type, const saige_fit_null_config* cfg, const MarkerSplit& ms,
+                     const uint8_t* vr_rows, saige_fit_null_result* result, double* mu_out,
+                     double* xvx_inv_xv) {
+  using clk = std::chrono::steady_clock;
+  const size_t n = h->g.n;
+  hipStream_t s = h->stream;
+  const size_t ld = round_up(n, 64);
+  RemlState st;
+  auto t1 = clk::now();
+  SAIGE_TRY(fit_ai_reml_dev(h, sink, y, x, n, p, trait_type, &cfg->reml, &result->reml, st));
+  auto t2 = clk::now();
+  std::vector<double> hm(n), hw(n), a(p * n);
+  SAIGE_HIP(hipMemcpyAsync(hm.data(), st.mu.p, n * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipMemcpyAsync(hw.data(), st.w.p, n * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  SAIGE_TRY(xvx_inv_xv_host(x, n, p, hw.data(), a.data()));
+  double vr = 1.0;
+  uint32_t used = 0;
+  if (!ms.vr_ids.empty()) {
+    VrSource src;
+    src.d_rows = vr_rows;
+    src.row_bytes = h->g.row_bytes;
+    saige_vr_config vc{cfg->n_markers_vr, 20.0, 0.001, cfg->reml.seed};
+    SAIGE_TRY(estimate_vr_dev(h, sink, src, ms.vr_mac.data(), ms.vr_ids.size(),
+                              st.mu.as<double>(), st.w.as<double>(), result->reml.tau, trait_type,
+                              st.x.as<double>(), ld, a.data(), p, &vc, &vr, &used, nullptr, st));
+  }
+  auto t3 = clk::now();
+  result->variance_ratio = vr;
+  result->vr_markers_used = used;
+  result->n_grm_markers = (uint32_t)ms.grm_ids.size();
+  result->n_vr_markers = (uint32_t)ms.vr_ids.size();
+  result->seconds_reml = std::chrono::duration<double>(t2 - t1).count();
+  result->seconds_vr = std::chrono::duration<double>(t3 - t2).count();
+  result->reml.grm_products = sink->columns;
+  if (mu_out) std::copy(hm.begin(), hm.end(), mu_out);
+  if (xvx_inv_xv) std::copy(a.begin(), a.end(), xvx_inv_xv);
+  return SAIGE_OK;
+}
+
+// VR markers resident as packed rows in model order (decoded per chunk by the VR solves).
+static int vr_rows_upload(saige_grm* h, const uint8_t* bed_body, bool on_device, size_t bpm,
+                          size_t n_total, const uint32_t* sample_idx, size_t n,
+                          const MarkerSplit& ms, DevBuf& rows) {
+  if (ms.vr_ids.empty()) return SAIGE_OK;
+  hipStream_t s = h->stream;
+  DevBuf d_idx;
+  const size_t row_bytes = h->g.row_bytes;
+  SAIGE_TRY(rows.ensure(ms.vr_ids.size() * row_bytes));
+  SAIGE_TRY(upload_sample_idx(s, sample_idx, n, n_total, d_idx));
+  if (on_device)
+    SAIGE_TRY(repack_from_device(s, bed_body, bpm, d_idx.as<uint32_t>(), n, ms.vr_ids.data(),
+                                 ms.vr_ids.size(), rows.as<uint8_t>(), row_bytes));
+  else
+    SAIGE_TRY(upload_repack(s, bed_body, bpm, n_total, d_idx.as<uint32_t>(), n, ms.vr_ids.data(),
+                            ms.vr_ids.size(), rows.as<uint8_t>(), row_bytes));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  return SAIGE_OK;
+}
+
+static int fit_null_impl(int device, const uint8_t* bed_body, bool on_device, size_t bpm,
+                         size_t n_total, size_t m_total, const uint32_t* sample_idx, size_t n,
+                         const double* y, const double* x, size_t p, int trait_type,
+                         const saige_fit_null_config* cfg, saige_fit_null_result* result,
+                         double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
+  SAIGE_ARG(bed_body && y && x && cfg && result, "null pointer");
+  SAIGE_ARG(p >= 1 && n > 0, "dimensions");
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  *result = saige_fit_null_result{};
+  MarkerSplit ms;
+  SAIGE_TRY(qc_split(device, bed_body, on_device, bpm, n_total, m_total, sample_idx, n, cfg, ms,
+                     marker_class));
   auto t1 = clk::now();
   saige_grm* h = nullptr;
   SAIGE_TRY(grm_create_impl(device, bed_body, on_device, bpm, n_total, sample_idx, n,
-                            grm_ids.data(), grm_ids.size(), grm_af.data(), cfg->precision, &h));
-  int rc = SAIGE_OK;
+                            ms.grm_ids.data(), ms.grm_ids.size(), ms.grm_af.data(), cfg->precision,
+                            &h));
+  int rc;
   {
     std::lock_guard<std::recursive_mutex> lk(h->mu);
-    RemlState st;
-    hipStream_t s = h->stream;
-    const size_t ld = round_up(n, 64);
-    rc = fit_ai_reml_dev(h, y, x, n, p, trait_type, &cfg->reml, &result->reml, st);
-    auto t2 = clk::now();
-    std::vector<double> hm(n), hw(n), a(p * n);
+    DevBuf rows;
+    rc = vr_rows_upload(h, bed_body, on_device, bpm, n_total, sample_idx, n, ms, rows);
+    DirectSink sink(h);
+    if (rc == SAIGE_OK)
+      rc = fit_trait(h, &sink, y, x, p, trait_type, cfg, ms, rows.as<uint8_t>(), result, mu_out,
+                     xvx_inv_xv);
+  }
+  saige_grm_destroy(h);
+  if (rc == SAIGE_OK) {
+    // the REML clock of fit_trait starts after the GRM build; charge the build to it
+    result->seconds_qc = std::chrono::duration<double>(t1 - t0).count();
+    result->seconds_total = std::chrono::duration<double>(clk::now() - t0).count();
+    result->seconds_reml = result->seconds_total - result->seconds_qc - result->seconds_vr;
+  }
+  return rc;
+}
+
+// ---------------------------------------------------------------------------------
+// Several phenotypes over one GRM (the sample set and covariates are shared).
+// ---------------------------------------------------------------------------------
+// Every trait runs its own fit (its own W, tau, PCG iterations and stop rules) on its own
+// host thread; the GRM products they request meet in a ProductBatcher, which waits until
+// every trait still running has a product pending, runs one multi-column product over all
+// of their columns and applies each request's own Sigma epilogue.  The per-column
+// arithmetic of the product does not depend on which other columns share it, so each trait
+// follows the iteration of a single-trait fit.
+
+// out = tau0 * v / max(w, 1e-30) + tau1 * g  (or out = g without w)
+__global__ void sigma_epilogue_kernel(const double* __restrict__ g, size_t ldg,
+                                      const double* __restrict__ v, size_t ldv,
+                                      const double* __restrict__ w, double tau0, double tau1,
+                                      size_t n, size_t k, double* __restrict__ out, size_t ldo) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (i >= n || c >= k) return;
+  const double y = g[i + c * ldg];
+  out[i + c * ldo] = w ? tau0 * v[i + c * ldv] / fmax(w[i], 1e-30) + tau1 * y : y;
+}
+
+class ProductBatcher {
+ public:
+  ProductBatcher(saige_grm* h, size_t clients) : h_(h), live_(clients) {}
+
+  struct Request {
+    const double* v;
+    size_t ldv, k;
+    double* out;
+    size_t ldo;
+    const double* w;
+    double tau[2];
+    int rc = SAIGE_OK;
+    bool done = false;
+  };
+
+  // Blocks until the request's product has been enqueued on the handle's stream.
+  int submit(Request& r) {
+    std::unique_lock<std::mutex> lk(mu_);
+    r.done = false;
+    pending_.push_back(&r);
+    if (pending_.size() == live_) run_locked();
+    else cv_.wait(lk, [&] { return r.done; });
+    return r.rc;
+  }
+  // A client has no more requests.
+  void leave() {
+    std::unique_lock<std::mutex> lk(mu_);
+    --live_;
+    if (live_ > 0 && pending_.size() == live_) run_locked();
+  }
+  uint64_t joined_products = 0;  // multi-column products run
+  uint64_t joined_columns = 0;
+
+ private:
+  void run_locked() {
+    const int rc = run(pending_);
+    for (Request* q : pending_) {
+      q->rc = rc;
+      q->done = true;
+    }
+    pending_.clear();
+    cv_.notify_all();
+  }
+  int run(const std::vector<Request*>& reqs) {
+    ++joined_products;
+    if (reqs.size() == 1) {
+      const Request& q = *reqs[0];
+      joined_columns += q.k;
+      return grm_product(h_, q.v, q.ldv, q.k, q.out, q.ldo, q.w, q.w ? q.tau : nullptr);
+    }
+    const size_t n = h_->g.n;
+    hipStream_t s = h_->stream;
+    size_t K = 0;
+    for (const Request* q : reqs) K += q->k;
+    joined_columns += K;
+    SAIGE_TRY(vin_.ensure(n * K * 8));
+    SAIGE_TRY(vout_.ensure(n * K * 8));
+    double* vin = vin_.as<double>();
+    double* vout = vout_.as<double>();
+    size_t off = 0;
+    for (const Request* q : reqs) {
+      SAIGE_HIP(hipMemcpy2DAsync(vin + off * n, n * 8, q->v, q->ldv * 8, n * 8, q->k,
+                                 hipMemcpyDeviceToDevice, s));
+      off += q->k;
+    }
+    SAIGE_TRY(grm_product(h_, vin, n, K, vout, n, nullptr, nullptr));
+    off = 0;
+    for (const Request* q : reqs) {
+      sigma_epilogue_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)q->k), 256, 0, s>>>(
+          vout + off * n, n, q->v, q->ldv, q->w, q->tau[0], q->tau[1], n, q->k, q->out, q->ldo);
+      off += q->k;
+    }
+    SAIGE_HIP(hipGetLastError());
+    return SAIGE_OK;
+  }
+
+  saige_grm* h_;
+  size_t live_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::vector<Request*> pending_;
+  DevBuf vin_, vout_;
+};
+
+struct BatchedSink : ProductSink {
+  ProductBatcher* b;
+  explicit BatchedSink(ProductBatcher* bb) : b(bb) {}
+  int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
+              const double* tau) override {
+    if (k == 0) return SAIGE_OK;
+    columns += k;
+    ProductBatcher::Request r;
+    r.v = v;
+    r.ldv = ldv;
+    r.k = k;
+    r.out = out;
+    r.ldo = ldo;
+    r.w = w;
+    r.tau[0] = tau ? tau[0] : 0.0;
+    r.tau[1] = tau ? tau[1] : 0.0;
+    return b->submit(r);
+  }
+};
+
+static int fit_null_multi_impl(int device, const uint8_t* bed_body, bool on_device, size_t bpm,
+                               size_t n_total, size_t m_total, const uint32_t* sample_idx,
+                               size_t n, const double* y, size_t n_traits, const double* x,
+                               size_t p, const int* trait_types, const saige_fit_null_config* cfg,
+                               saige_fit_null_result* results, double* mu_out,
+                               double* xvx_inv_xv, uint8_t* marker_class) {
+  SAIGE_ARG(bed_body && y && x && cfg && results && trait_types, "null pointer");
+  SAIGE_ARG(p >= 1 && n > 0 && n_traits >= 1 && n_traits <= 64, "dimensions");
+  for (size_t t = 0; t < n_traits; ++t)
+    SAIGE_ARG(trait_types[t] >= 0 && trait_types[t] <= 2, "trait_type");
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  for (size_t t = 0; t < n_traits; ++t) results[t] = saige_fit_null_result{};
+  MarkerSplit ms;
+  SAIGE_TRY(qc_split(device, bed_body, on_device, bpm, n_total, m_total, sample_idx, n, cfg, ms,
+                     marker_class));
+  auto t1 = clk::now();
+  saige_grm* h = nullptr;
+  SAIGE_TRY(grm_create_impl(device, bed_body, on_device, bpm, n_total, sample_idx, n,
+                            ms.grm_ids.data(), ms.grm_ids.size(), ms.grm_af.data(), cfg->precision,
+                            &h));
+  int rc;
+  {
+    std::lock_guard<std::recursive_mutex> lk(h->mu);
+    DevBuf rows;
+    rc = vr_rows_upload(h, bed_body, on_device, bpm, n_total, sample_idx, n, ms, rows);
     if (rc == SAIGE_OK) {
-      if (hipMemcpyAsync(hm.data(), st.mu.p, n * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipMemcpyAsync(hw.data(), st.w.p, n * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess) {
-        set_error("copy of mu/w failed");
-        rc = SAIGE_ERR_DEVICE;
+      ProductBatcher batcher(h, n_traits);
+      std::vector<int> rcs(n_traits, SAIGE_OK);
+      std::vector<std::string> errs(n_traits);
+      std::vector<std::thread> threads;
+      for (size_t t = 0; t < n_traits; ++t) {
+        threads.emplace_back([&, t]() {
+          BatchedSink sink(&batcher);
+          int r = hipSetDevice(h->device) == hipSuccess ? SAIGE_OK : SAIGE_ERR_DEVICE;
+          if (r == SAIGE_OK)
+            r = fit_trait(h, &sink, y + t * n, x, p, trait_types[t], cfg, ms, rows.as<uint8_t>(),
+                          &results[t], mu_out ? mu_out + t * n : nullptr,
+                          xvx_inv_xv ? xvx_inv_xv + t * p * n : nullptr);
+          if (r != SAIGE_OK) errs[t] = saige_last_error();
+          rcs[t] = r;
+          batcher.leave();
+        });
       }
-    }
-    if (rc == SAIGE_OK) rc = xvx_inv_xv_host(x, n, p, hw.data(), a.data());
-    double vr = 1.0;
-    uint32_t used = 0;
-    if (rc == SAIGE_OK && !vr_ids.empty()) {
-      // VR markers resident as packed rows in model order, decoded per chunk
-      DevBuf rows, d_idx;
-      const size_t row_bytes = h->g.row_bytes;
-      rc = rows.ensure(vr_ids.size() * row_bytes);
-      if (rc == SAIGE_OK) rc = upload_sample_idx(s, sample_idx, n, n_total, d_idx);
-      if (rc == SAIGE_OK)
-        rc = on_device ? repack_from_device(s, bed_body, bpm, d_idx.as<uint32_t>(), n,
-                                            vr_ids.data(), vr_ids.size(), rows.as<uint8_t>(),
-                                            row_bytes)
-                       : upload_repack(s, bed_body, bpm, n_total, d_idx.as<uint32_t>(), n,
-                                       vr_ids.data(), vr_ids.size(), rows.as<uint8_t>(), row_bytes);
-      VrSource src;
-      src.d_rows = rows.as<uint8_t>();
-      src.row_bytes = row_bytes;
-      saige_vr_config vc{cfg->n_markers_vr, 20.0, 0.001, cfg->reml.seed};
-      if (rc == SAIGE_OK)
-        rc = estimate_vr_dev(h, src, vr_mac.data(), vr_ids.size(), st.mu.as<double>(),
-                             st.w.as<double>(), result->reml.tau, trait_type, st.x.as<double>(), ld,
-                             a.data(), p, &vc, &vr, &used, nullptr, st);
-    }
-    auto t3 = clk::now();
-    if (rc == SAIGE_OK) {
-      result->variance_ratio = vr;
-      result->vr_markers_used = used;
-      result->n_grm_markers = (uint32_t)grm_ids.size();
-      result->n_vr_markers = (uint32_t)vr_ids.size();
-      result->seconds_qc = std::chrono::duration<double>(t1 - t0).count();
-      result->seconds_reml = std::chrono::duration<double>(t2 - t1).count();
-      result->seconds_vr = std::chrono::duration<double>(t3 - t2).count();
-      result->seconds_total = std::chrono::duration<double>(t3 - t0).count();
-      result->reml.grm_products = h->products;
-      if (mu_out) std::copy(hm.begin(), hm.end(), mu_out);
-      if (xvx_inv_xv) std::copy(a.begin(), a.end(), xvx_inv_xv);
+      for (auto& th : threads) th.join();
+      for (size_t t = 0; t < n_traits && rc == SAIGE_OK; ++t)
+        if (rcs[t] != SAIGE_OK) {
+          rc = rcs[t];
+          set_error("trait %zu: %s", t, errs[t].c_str());
+        }
     }
   }
   saige_grm_destroy(h);
+  if (rc == SAIGE_OK) {
+    const double total = std::chrono::duration<double>(clk::now() - t0).count();
+    for (size_t t = 0; t < n_traits; ++t) {
+      results[t].seconds_qc = std::chrono::duration<double>(t1 - t0).count();
+      results[t].seconds_total = total;
+    }
+  }
   return rc;
 }
 
@@ -868,6 +1103,30 @@ int saige_fit_null_device(int device, const uint8_t* d_bed_body, size_t bpm, siz
   clear_error();
   return fit_null_impl(device, d_bed_body, true, bpm, n_total, m_total, sample_idx, n, y, x, p,
                        trait_type, cfg, result, mu_out, xvx_inv_xv, marker_class);
+}
+
+
+int saige_fit_null_multi(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,
+                         size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
+                         size_t n_traits, const double* x, size_t p, const int* trait_types,
+                         const saige_fit_null_config* cfg, saige_fit_null_result* results,
+                         double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
+  clear_error();
+  return fit_null_multi_impl(device, bed_body, false, bpm, n_total, m_total, sample_idx, n, y,
+                             n_traits, x, p, trait_types, cfg, results, mu_out, xvx_inv_xv,
+                             marker_class);
+}
+
+int saige_fit_null_multi_device(int device, const uint8_t* d_bed_body, size_t bpm,
+                                size_t n_total, size_t m_total, const uint32_t* sample_idx,
+                                size_t n, const double* y, size_t n_traits, const double* x,
+                                size_t p, const int* trait_types,
+                                const saige_fit_null_config* cfg, saige_fit_null_result* results,
+                                double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
+  clear_error();
+  return fit_null_multi_impl(device, d_bed_body, true, bpm, n_total, m_total, sample_idx, n, y,
+                             n_traits, x, p, trait_types, cfg, results, mu_out, xvx_inv_xv,
+                             marker_class);
 }
 
 }  // extern "C"

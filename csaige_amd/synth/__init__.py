@@ -30,6 +30,10 @@ def _load():
                                      dp, dp, dp]
         L.synth_dosage_u8.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
                                       C.c_double, C.c_double, u8p, dp]
+        L.synth_biobank.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                    dp, C.c_double, C.c_size_t, C.c_double, C.c_double,
+                                    C.c_size_t, C.c_double, u8p, dp, dp, dp]
+        L.synth_biobank.restype = C.c_int
         _lib = L
     return _lib
 
@@ -93,3 +97,25 @@ def dosage_u8(n, first, nvar, seed=4, maf=(1e-4, 0.5), ld=None, out=None):
     mafs = np.empty(nvar)
     L.synth_dosage_u8(seed, n, first, nvar, ld, maf[0], maf[1], _p(q, u8p), _p(mafs, dp))
     return q, mafs
+
+
+def biobank(n=400000, m=150000, frac_related=0.15, prevalences=(0.005, 0.01, 0.05, 0.20), seed=5,
+            maf=(0.01, 0.5), h2=0.4, n_causal=2000, n_cov=2, cov_beta=0.5, bed=None):
+    """Several binary phenotypes over one cohort with `frac_related` of the samples in
+    half-sibling (2nd-degree) pairs.  ``bed`` may be a preallocated (m, ceil(n/4)) uint8
+    array (for instance in pinned memory)."""
+    L = _load()
+    bpm = (n + 3) // 4
+    if bed is None:
+        bed = np.empty((m, bpm), np.uint8)
+    prev = np.ascontiguousarray(prevalences, dtype=np.float64)
+    T = len(prev)
+    mf = np.empty(m)
+    y = np.empty((T, n))
+    x = np.empty((1 + n_cov, n))
+    n_pairs = int(round(frac_related * n / 2))
+    rc = L.synth_biobank(seed, n, m, n_pairs, T, _p(prev, dp), h2, n_causal, maf[0], maf[1], n_cov,
+                         cov_beta, _p(bed, u8p), _p(mf, dp), _p(y, dp), _p(x, dp))
+    if rc != 0:
+        raise ValueError("synth_biobank: bad arguments")
+    return Cohort(n=n, bpm=bpm, bed=bed, maf=mf, y=y, x=x, n_pairs=n_pairs, prevalences=prev)

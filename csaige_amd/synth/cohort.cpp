@@ -34,7 +34,7 @@ inline uint8_t code_of(int c) { return c == 2 ? 0u : (c == 1 ? 2u : 3u); }
 enum Stream : uint64_t {
   S_MAF = 1, S_GENO = 2, S_BETA = 3, S_COV = 4, S_NOISE = 5, S_TRANSMIT = 6, S_MAF_TEST = 7,
   S_GENO_TEST = 8, S_ETA = 9, S_Y = 10, S_DOSAGE = 11, S_DOSAGE_MAF = 12, S_DOSAGE_NOISE = 13,
-  S_X2 = 14
+  S_X2 = 14, S_CAUSAL = 15, S_PAIR = 16, S_PAIR_T = 17, S_BETA_T = 18, S_NOISE_T = 19
 };
 
 // Liability -> binary phenotype: the top `prevalence` fraction are cases.
@@ -182,6 +182,83 @@ int synth_families(uint64_t seed, size_t n_fam, size_t kids, size_t m, double ma
     liab[i] = l;
   }
   threshold_cases(liab, prevalence, y);
+  return 0;
+}
+
+// Biobank-style cohort for several phenotypes: n samples, the first 2 * n_pairs of them in
+// half-sibling pairs (2nd-degree relatives: one shared parent, two unrelated mates; pair k
+// is samples 2k and 2k + 1), the rest unrelated, all under HWE with MAF ~ U(lo, hi) and no
+// LD.  n_traits liabilities share the covariates but have their own causal effects
+// (markers causal with probability n_causal / m, beta ~ N(0, h2 / n_causal)) and noise;
+// trait t is binary with the top prevalence[t] fraction as cases.  y: n_traits x n.
+int synth_biobank(uint64_t seed, size_t n, size_t m, size_t n_pairs, size_t n_traits,
+                  const double* prevalence, double h2, size_t n_causal, double maf_lo,
+                  double maf_hi, size_t n_cov, double cov_beta, uint8_t* bed, double* maf_out,
+                  double* y, double* x) {
+  if (2 * n_pairs > n || n_traits == 0) return 1;
+  const size_t bpm = (n + 3) / 4;
+  const size_t nrel = 2 * n_pairs;
+  const double pc = std::min(1.0, (double)n_causal / (double)std::max<size_t>(m, 1));
+  const double bsd = std::sqrt(h2 / (double)std::max<size_t>(n_causal, 1));
+  std::vector<double> pgs(n * n_traits, 0.0);
+#pragma omp parallel
+  {
+    std::vector<double> local;
+    std::vector<uint8_t> dose(n);
+#pragma omp for schedule(dynamic, 64)
+    for (long long jj = 0; jj < (long long)m; ++jj) {
+      const size_t j = (size_t)jj;
+      const double maf = maf_lo + (maf_hi - maf_lo) * u01(h3(seed, S_MAF, j));
+      maf_out[j] = maf;
+      const uint32_t thr = (uint32_t)std::min(65536.0, maf * 65536.0);
+      auto allele = [&](uint64_t h, int k) -> int { return ((h >> (16 * k)) & 0xFFFFu) < thr; };
+      for (size_t k = 0; k < n_pairs; ++k) {
+        const uint64_t a = h3(seed, S_PAIR, j * n_pairs + k);       // parent, mate 1
+        const uint64_t b = h3(seed, S_PAIR + 100, j * n_pairs + k); // mate 2
+        const uint64_t t = h3(seed, S_PAIR_T, j * n_pairs + k);
+        const int par[2] = {allele(a, 0), allele(a, 1)};
+        const int m1[2] = {allele(a, 2), allele(a, 3)};
+        const int m2[2] = {allele(b, 0), allele(b, 1)};
+        dose[2 * k] = (uint8_t)(par[t & 1] + m1[(t >> 1) & 1]);
+        dose[2 * k + 1] = (uint8_t)(par[(t >> 2) & 1] + m2[(t >> 3) & 1]);
+      }
+      for (size_t i = nrel; i < n; i += 2) {
+        const uint64_t h = h3(seed, S_GENO, j * n + i);
+        dose[i] = (uint8_t)(allele(h, 0) + allele(h, 1));
+        if (i + 1 < n) dose[i + 1] = (uint8_t)(allele(h, 2) + allele(h, 3));
+      }
+      uint8_t* row = bed + j * bpm;
+      std::memset(row, 0, bpm);
+      for (size_t i = 0; i < n; ++i) row[i / 4] |= (uint8_t)(code_of(dose[i]) << (2 * (i % 4)));
+      if (u01(h3(seed, S_CAUSAL, j)) < pc) {
+        if (local.empty()) local.assign(n * n_traits, 0.0);
+        const double mean = 2.0 * maf, sd = std::sqrt(2.0 * maf * (1.0 - maf));
+        for (size_t t = 0; t < n_traits; ++t) {
+          const double beta = bsd * normal(seed, S_BETA_T + 100 * t, j);
+          double* lt = local.data() + t * n;
+          for (size_t i = 0; i < n; ++i) lt[i] += beta * ((double)dose[i] - mean) / sd;
+        }
+      }
+    }
+#pragma omp critical
+    if (!local.empty())
+      for (size_t i = 0; i < n * n_traits; ++i) pgs[i] += local[i];
+  }
+  std::vector<double> cov(n, 0.0);
+  for (size_t i = 0; i < n; ++i) {
+    x[i] = 1.0;
+    for (size_t c = 0; c < n_cov; ++c) {
+      const double v = normal(seed, S_COV + 100 * (c + 1), i);
+      x[(c + 1) * n + i] = v;
+      cov[i] += cov_beta * v;
+    }
+  }
+  std::vector<double> liab(n);
+  for (size_t t = 0; t < n_traits; ++t) {
+    for (size_t i = 0; i < n; ++i)
+      liab[i] = pgs[t * n + i] + cov[i] + std::sqrt(1.0 - h2) * normal(seed, S_NOISE_T + 100 * t, i);
+    threshold_cases(liab, prevalence[t], y + t * n);
+  }
   return 0;
 }
 

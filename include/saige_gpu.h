@@ -228,6 +228,28 @@ int saige_fit_null_device(int device, const uint8_t* d_bed_body, size_t bytes_pe
                           const saige_fit_null_config* cfg, saige_fit_null_result* result,
                           double* mu, double* xvx_inv_xv, uint8_t* marker_class);
 
+/* Several phenotypes over the same samples, covariates and GRM: n_traits runs of
+ * fit_null::run (fit_null.rs:199-394, one per trait, each with its own W, tau, PCG
+ * iterations and variance ratio) fitted together.  Each trait runs on its own host thread;
+ * whenever every trait still running has a GRM product pending, the products are joined
+ * into one multi-column product (its own Sigma epilogue per trait), so each trait follows
+ * its single-trait iteration while the genotypes are streamed once per joined product.
+ * y: n x n_traits col-major; trait_types[n_traits]; results[n_traits];
+ * mu: n x n_traits; xvx_inv_xv: (p*n) x n_traits; marker_class: m_total (shared).
+ * No counterpart in saige-rs beyond calling fit_null once per phenotype. */
+int saige_fit_null_multi(int device, const uint8_t* bed_body, size_t bytes_per_marker,
+                         size_t n_total, size_t m_total, const uint32_t* sample_idx, size_t n,
+                         const double* y, size_t n_traits, const double* x, size_t p,
+                         const int* trait_types, const saige_fit_null_config* cfg,
+                         saige_fit_null_result* results, double* mu, double* xvx_inv_xv,
+                         uint8_t* marker_class);
+int saige_fit_null_multi_device(int device, const uint8_t* d_bed_body, size_t bytes_per_marker,
+                                size_t n_total, size_t m_total, const uint32_t* sample_idx,
+                                size_t n, const double* y, size_t n_traits, const double* x,
+                                size_t p, const int* trait_types,
+                                const saige_fit_null_config* cfg, saige_fit_null_result* results,
+                                double* mu, double* xvx_inv_xv, uint8_t* marker_class);
+
 /* ------------------------------------------------------------------------------
  * Step 2: score test engine.
  *   Replaces ScoreTestEngine (saige-core/src/score_test/single_variant.rs:77-108),

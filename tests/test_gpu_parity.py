@@ -335,3 +335,36 @@ def test_fit_null_mixed_vs_fp64_oracle(prec):
     np.testing.assert_array_equal(got["marker_class"], want["marker_class"])
     assert rel(got["tau"], want["tau"]).max() < 1e-4
     assert abs(got["vr"] - want["vr"]) < 1e-4 * want["vr"]
+
+
+# --------------------------------------------------------------------------- several phenotypes
+@pytest.mark.parametrize("prec", [sg.PREC_FP64, sg.PREC_INT8], ids=["fp64", "int8"])
+def test_fit_null_multi_matches_single_trait(prec):
+    """Three phenotypes fitted together (joined GRM products) follow their single-trait
+    fits: same AI-REML and PCG iteration counts, same tau / VR / mu to 1e-10."""
+    c = synth.biobank(n=1600, m=3000, frac_related=0.15, prevalences=(0.05, 0.1, 0.3), seed=21,
+                      h2=0.5, n_causal=500)
+    idx = np.arange(c.n, dtype=np.uint32)
+    rng = np.random.default_rng(4)
+    Y = c.y.copy()
+    Y[2] = Y[2] + 0.5 * rng.standard_normal(c.n)  # a quantitative trait among binary ones
+    traits = [sg.TRAIT_BINARY, sg.TRAIT_BINARY, sg.TRAIT_QUANTITATIVE]
+    multi = sg.fit_null_multi(c.bed, c.bpm, c.n, c.bed.shape[0], Y, c.x, traits=traits,
+                              sample_idx=idx, precision=prec)
+    for t in range(3):
+        one = sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], Y[t], c.x, sample_idx=idx,
+                          trait=traits[t], precision=prec)
+        _say(f"prec {prec} trait {t}: tau multi={multi[t]['tau']} single={one['tau']} "
+             f"it {multi[t]['iterations']}/{one['iterations']} pcg {multi[t]['pcg_iterations']}"
+             f"/{one['pcg_iterations']}")
+        assert multi[t]["iterations"] == one["iterations"]
+        assert multi[t]["pcg_iterations"] == one["pcg_iterations"]
+        assert multi[t]["grm_products"] == one["grm_products"]
+        assert rel(multi[t]["tau"], one["tau"]).max() < 1e-10
+        assert abs(multi[t]["vr"] - one["vr"]) < 1e-10 * one["vr"]
+        np.testing.assert_allclose(multi[t]["mu"], one["mu"], rtol=1e-10)
+        np.testing.assert_array_equal(multi[t]["marker_class"], one["marker_class"])
+    if prec == sg.PREC_FP64:
+        want = oracle.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], idx, Y[0], c.x)
+        assert rel(multi[0]["tau"], want["tau"]).max() < 1e-6
+        assert abs(multi[0]["vr"] - want["vr"]) < 1e-6 * want["vr"]
