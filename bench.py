@@ -148,6 +148,107 @@ def run_biobank(args):
     print(json.dumps(line), flush=True)
 
 
+def run_learned(args):
+    """Learned SPA root proposal: train the MLP on ~2M saddle-point roots computed by the SPA
+    kernel itself (n = 50,000, 1% and 10% prevalence, every variant SPA'd), then compare
+    Newton iterations, p-values and SPA time with and without the proposal on held-out
+    variants; one JSON line."""
+    import csaige_amd as sg
+    from csaige_amd import hip, synth
+    n, B = args.ls_n, 4096
+    nb = max(1, args.ls_variants // B)
+    xs, ys = [], []
+    t0 = time.time()
+    q = np.empty((B, n), np.uint8)
+    for prev, seed in ((0.01, 11), (0.10, 21)):
+        eta = float(np.log(prev / (1 - prev)))
+        nf = synth.null_fit(n=n, p=1, eta_mean=eta, eta_sd=0.5, seed=seed)
+        a = sg.xvx_inv_xv(nf.x, nf.mu * (1 - nf.mu))
+        e = sg.ScoreTestEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.0, y=nf.y,
+                               spa_pval_cutoff=1.0, spa_tol=1e-10)
+        for b in range(nb):
+            synth.dosage_u8(n, b * B, B, seed=seed + 1, maf=(5e-4, 0.5), out=q)
+            x, y = sg.spa_training_set(e.test_markers_u8(q))
+            xs.append(x)
+            ys.append(y)
+            if b % 32 == 31:
+                log(f"prevalence {prev}: {b + 1}/{nb} batches, {sum(len(v) for v in ys)} roots")
+        e.close()
+    X, Y = np.concatenate(xs), np.concatenate(ys)
+    t_gen = time.time() - t0
+    log(f"{len(Y)} training roots in {t_gen:.1f}s")
+    m = sg.SpaRootMlp(seed=5)
+    t1 = time.time()
+    hist = m.train(X, Y, epochs=args.ls_epochs, batch=4096, lr=2e-3, seed=7)
+    t_train = time.time() - t1
+    log(f"trained {args.ls_epochs} epochs in {t_train:.1f}s: loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+
+    def evaluate(tag, n_e, prev, seed, nbat, cutoff):
+        eta = float(np.log(prev / (1 - prev)))
+        nf = synth.null_fit(n=n_e, p=1, eta_mean=eta, eta_sd=0.5, seed=seed)
+        a = sg.xvx_inv_xv(nf.x, nf.mu * (1 - nf.mu))
+        qe = np.empty((nbat * B, n_e), np.uint8)
+        synth.dosage_u8(n_e, 0, nbat * B, seed=seed + 1, maf=(1e-4, 0.5), out=qe)
+        d_q = hip.DeviceBuffer(qe.nbytes)
+        d_q.upload(qe)
+        del qe
+        rsize = C.sizeof(sg._lib.VariantResult)
+        d_r = hip.DeviceBuffer(nbat * B * rsize)
+        out = {}
+        for mode in ("plain", "mlp"):
+            e = sg.ScoreTestEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
+                                   spa_pval_cutoff=cutoff, spa_tol=1e-10)
+            if mode == "mlp":
+                e.set_root_proposal(m)
+            e.test_markers_u8_device(d_q.value, n_e, B, 1.0 / 127, d_r.value)  # warm
+            hip.device_synchronize()
+            s0, s1 = hip.Event(), hip.Event()
+            s0.record(e.stream)
+            for b in range(nbat):  # resident batches, timed on the engine's stream
+                e.test_markers_u8_device(d_q.value + b * B * n_e, n_e, B, 1.0 / 127,
+                                         d_r.value + b * B * rsize)
+            s1.record(e.stream)
+            ms = s0.elapsed_ms(s1)
+            res = (sg._lib.VariantResult * (nbat * B))()
+            host = np.empty(nbat * B * rsize, np.uint8)
+            d_r.download(host)
+            C.memmove(C.addressof(res), host.ctypes.data, host.nbytes)
+            out[mode] = (sg.results_to_dict(res), ms * 1e-3)
+            e.close()
+        d_q.free()
+        r0, r1 = out["plain"][0], out["mlp"][0]
+        gated = r0["pvalue_noadj"] < cutoff
+        ok = gated & (r0["spa_converged"] == 1)
+        it0 = float((r0["spa_iters1"][ok] + r0["spa_iters2"][ok]).mean())
+        it1 = float((r1["spa_iters1"][ok] + r1["spa_iters2"][ok]).mean())
+        d = float(np.abs(np.log10(r0["pvalue"][ok]) - np.log10(r1["pvalue"][ok])).max())
+        lost = gated & (r0["spa_converged"] == 0)
+        res = {"workload": tag, "variants": int(nbat * B), "spa_variants": int(gated.sum()),
+               "mean_newton_iterations_plain": it0, "mean_newton_iterations_mlp": it1,
+               "max_abs_dlog10p_vs_plain": d,
+               "plain_diverged": int(lost.sum()),
+               "rescued_by_proposal": int((lost & (r1["spa_converged"] == 1)).sum()),
+               "step2_device_s_plain": out["plain"][1], "step2_device_s_mlp": out["mlp"][1]}
+        log(f"{tag}: {res}")
+        return res
+
+    ev = [evaluate("held-out n=50,000 at 3% prevalence, SPA where p < 0.05", n, 0.03, 31, 16, 0.05),
+          evaluate("step-2 shape n=400,000 at 1% prevalence, SPA where |z| > 2", 400000, 0.01, 3, 4,
+                   0.04550026389635842)]
+    line = {
+        "metric": "learned SPA root proposal: mean Newton iterations and max |d -log10 p| vs plain",
+        "value": ev[0]["mean_newton_iterations_mlp"], "unit": "Newton iterations per variant",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": None, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 weights / fp32 MLP; fp64 Newton",
+        "data": "synthetic (roots computed by the SPA kernel on seeded null fits and dosages)",
+        "config": {"workload": f"MLP 6-64-64-1 trained on {len(Y)} roots (n={n}, prevalence 1% "
+                               f"and 10%, every variant SPA'd, tol 1e-10), {args.ls_epochs} epochs",
+                   "roots_seconds": t_gen, "train_seconds": t_train,
+                   "loss_first_last": [float(hist[0]), float(hist[-1])], "evaluation": ev},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,9 +267,13 @@ def main():
                     help="HBM bytes per GEMM launch measured with rocprofv3 counters")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (counter calibration)")
-    ap.add_argument("--workload", default="families", choices=["families", "biobank"],
+    ap.add_argument("--workload", default="families", choices=["families", "biobank", "learned"],
                     help="families: the default fit-null benchmark; biobank: several "
-                         "phenotypes at 400,000 x 150,000")
+                         "phenotypes at 400,000 x 150,000; learned: the SPA root proposal")
+    ap.add_argument("--ls-n", type=int, default=50000)
+    ap.add_argument("--ls-variants", type=int, default=524288,
+                    help="learned: variants per prevalence for training (2 roots each)")
+    ap.add_argument("--ls-epochs", type=int, default=20)
     ap.add_argument("--bb-n", type=int, default=400000)
     ap.add_argument("--bb-m", type=int, default=150000)
     ap.add_argument("--bb-separate", action="store_true",
@@ -189,6 +294,9 @@ def main():
         sys.exit(2)
     if args.workload == "biobank":
         run_biobank(args)
+        return
+    if args.workload == "learned":
+        run_learned(args)
         return
 
     import csaige_amd as sg
