@@ -252,8 +252,8 @@ def run_learned(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-fam", type=int, default=10000)
     ap.add_argument("--markers", type=int, default=100000)
     ap.add_argument("--product-reps", type=int, default=5)
@@ -325,15 +325,20 @@ def main():
             f"products={r['grm_products']} pcg_iters={r['pcg_iterations']} "
             f"(qc {r['seconds_qc']:.3f} reml {r['seconds_reml']:.3f} vr {r['seconds_vr']:.3f})")
 
+    # untimed warm-up fits load the kernels and fill the device memory cache, for each
+    # precision that is timed below
     for w in range(args.warmup):
         log(f"warmup {w}")
         one_fit(sg.PREC_INT8)
+        one_fit(sg.PREC_MIXED)
+        if not args.no_fp64:
+            one_fit(sg.PREC_FP64)
     times, rm = [], None
     for s in range(args.steps):
         dt, rm = one_fit(sg.PREC_INT8)
         times.append(dt)
         describe(f"step {s} mixed int8", dt, rm)
-    mixed_s = max(times)
+    mixed_s = sum(times) / len(times)
     bf16_s, rb = one_fit(sg.PREC_MIXED)
     describe("mixed bf16", bf16_s, rb)
     fp64_s, rf = None, None

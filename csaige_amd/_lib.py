@@ -91,6 +91,7 @@ class SpaResult(C.Structure):
 # (name, restype, argtypes) for every symbol of include/saige_gpu.h
 SIGNATURES = [
     ("saige_last_error", C.c_char_p, []),
+    ("saige_release_cached_memory", None, []),
     ("saige_version", C.c_char_p, []),
     ("saige_plink_marker_stats", C.c_int, [C.c_int, u8p, sz, sz, sz, u32p, sz, dp, u32p, dp, dp]),
     ("saige_plink_decode", C.c_int, [C.c_int, u8p, sz, sz, sz, u32p, sz, dp]),

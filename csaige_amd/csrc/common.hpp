@@ -43,25 +43,37 @@ void clear_error();
     }                                                               \
   } while (0)
 
+// Device memory cache.  A freed block is kept for its device and handed back to the next
+// request of the same rounded size, so repeated fits, products and batches do not pay
+// hipMalloc/hipFree (a 1.25 GB hipFree costs milliseconds).  Sizes round up to 64 KiB below
+// 64 MiB and to 2 MiB above.  Handing out a cached block first waits for the device, so
+// work enqueued before the block was freed cannot overlap its new use.  When hipMalloc
+// fails, the device's cached blocks are released and the allocation is retried once.
+// saige_release_cached_memory() empties the cache.
+void* cache_alloc(size_t bytes, size_t* rounded, int* device);
+void cache_free(void* p, size_t rounded, int device);
+
 // Grow-only device buffer owned by a handle.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  int device = 0;
   int ensure(size_t want) {
     if (want <= bytes) return SAIGE_OK;
-    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    release();
     if (want == 0) return SAIGE_OK;
-    hipError_t e = hipMalloc(&p, want);
-    if (e != hipSuccess) {
-      set_error("hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
-      p = nullptr;
-      return SAIGE_ERR_NOMEM;
-    }
-    bytes = want;
+    size_t got = 0;
+    p = cache_alloc(want, &got, &device);
+    if (!p) return SAIGE_ERR_NOMEM;
+    bytes = got;
     return SAIGE_OK;
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
-  void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+  void release() {
+    if (p) cache_free(p, bytes, device);
+    p = nullptr;
+    bytes = 0;
+  }
   ~DevBuf() { release(); }
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;

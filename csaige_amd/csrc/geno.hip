@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 
 namespace saige {
 
@@ -20,6 +22,105 @@ void set_error(const char* fmt, ...) {
   g_err = buf;
 }
 void clear_error() { g_err.clear(); }
+
+// ---- device memory cache (see common.hpp) ----
+namespace {
+struct MemCache {
+  std::mutex mu;
+  std::map<int, std::multimap<size_t, void*>> blocks;  // device -> size -> free block
+  size_t cached = 0;
+};
+MemCache& mem_cache() {
+  static MemCache* c = new MemCache();  // never destroyed: blocks may be returned at exit
+  return *c;
+}
+constexpr size_t kCacheLimit = size_t(32) << 30;  // cached bytes kept at most
+size_t alloc_round(size_t b) {
+  return b < (size_t(64) << 20) ? round_up(b, size_t(64) << 10) : round_up(b, size_t(2) << 20);
+}
+// frees the cached blocks of `device` (all devices if < 0); caller holds the lock
+void cache_release_locked(MemCache& c, int device) {
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto& kv : c.blocks) {
+    if (device >= 0 && kv.first != device) continue;
+    (void)hipSetDevice(kv.first);
+    for (auto& b : kv.second) {
+      (void)hipFree(b.second);
+      c.cached -= b.first;
+    }
+    kv.second.clear();
+  }
+  (void)hipSetDevice(cur);
+}
+}  // namespace
+
+void* cache_alloc(size_t bytes, size_t* rounded, int* device) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const size_t r = alloc_round(bytes);
+  MemCache& c = mem_cache();
+  void* reuse = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto& m = c.blocks[dev];
+    auto it = m.lower_bound(r);  // smallest free block that fits, if not more than twice as big
+    if (it != m.end() && it->first <= 2 * r) {
+      reuse = it->second;
+      *rounded = it->first;
+      c.cached -= it->first;
+      m.erase(it);
+    }
+  }
+  if (reuse) {
+    // work enqueued on any stream before the block was freed may still read or write it
+    (void)hipDeviceSynchronize();
+    *device = dev;
+    return reuse;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, r);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    {
+      std::lock_guard<std::mutex> lk(c.mu);
+      cache_release_locked(c, dev);
+    }
+    e = hipMalloc(&p, r);
+  }
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%zu) failed: %s", r, hipGetErrorString(e));
+    return nullptr;
+  }
+  *rounded = r;
+  *device = dev;
+  return p;
+}
+
+void cache_free(void* p, size_t rounded, int device) {
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.blocks[device].emplace(rounded, p);
+  c.cached += rounded;
+  while (c.cached > kCacheLimit) {  // drop the largest cached blocks first
+    int bd = -1;
+    size_t bs = 0;
+    for (auto& kv : c.blocks)
+      if (!kv.second.empty() && kv.second.rbegin()->first >= bs) {
+        bs = kv.second.rbegin()->first;
+        bd = kv.first;
+      }
+    if (bd < 0) break;
+    auto it = std::prev(c.blocks[bd].end());
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(bd);
+    (void)hipFree(it->second);
+    (void)hipSetDevice(cur);
+    c.cached -= it->first;
+    c.blocks[bd].erase(it);
+  }
+}
 
 int use_device(int device) {
   int count = 0;
@@ -289,6 +390,12 @@ using namespace saige;
 extern "C" {
 
 const char* saige_last_error(void) { return g_err.c_str(); }
+
+void saige_release_cached_memory(void) {
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  cache_release_locked(c, -1);
+}
 const char* saige_version(void) { return "saige_gpu 0.1 (gfx950)"; }
 
 int saige_plink_marker_stats(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,

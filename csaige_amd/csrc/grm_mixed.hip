@@ -524,7 +524,7 @@ void grm_mixed_release(saige_grm* h) {
 // v_perm into four words, word u holding codes {u, 4+u, 8+u, 12+u} one per byte, and the
 // operand's bytes then pair with the same reduction indices.
 constexpr int kI8KC = 256;        // reduction indices per staged chunk
-constexpr int kI8MaxKT = 8;       // at most 128 slice columns (42 vectors) per launch
+constexpr int kI8MaxKT = 7;       // at most 112 slice columns (37 vectors) per launch
 constexpr size_t kI8MaxRed = 8000000;
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
@@ -601,27 +601,33 @@ __global__ void quantize3_kernel(const double* __restrict__ x, size_t ldx, size_
 }
 
 // part[split][row][16*KT] = sum over the split's reduction range of g(row, red) * q[col][red].
-// 4 waves; each wave owns 16*RT rows and reads their packed codes straight from global
+// NW waves; each wave owns 16*RT rows and reads their packed codes straight from global
 // memory (16 bytes = 64 genotypes per lane and chunk, the next chunk prefetched in
-// registers); the KP x 256 operand chunk is staged in LDS (column stride 272 bytes, so the
-// 16-byte fragment reads of 8 consecutive columns fall in distinct banks).
-template <int RT, int KT>
-__global__ __launch_bounds__(256) void i8_gemm_kernel(
+// registers).  The KP x 256 operand chunk is staged in LDS, double-buffered so that one
+// barrier per chunk separates the reads of one buffer from the refill of the other (column
+// stride 272 bytes: the 16-byte fragment reads of 8 consecutive columns fall in distinct
+// banks).  Eight waves of RT = 4 keep the 112 accumulators of k = 34 in VGPRs at two waves
+// per SIMD; four waves of RT = 8 spill them into AGPR shuffles.
+template <int RT, int KT, int NW>
+__global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
     const uint8_t* __restrict__ codes, size_t code_stride, const int8_t* __restrict__ bq,
     size_t b_stride, size_t red_per_split, size_t red_len, int* __restrict__ part,
     size_t rows_pad) {
   constexpr int KP = 16 * KT;
-  constexpr int ROWS = 64 * RT;
+  constexpr int ROWS = NW * 16 * RT;
+  constexpr int NT = NW * 64;
   constexpr int SB = kI8KC + 16;
   constexpr int SEGS = kI8KC / 16;
-  constexpr int B_LOADS = KP * SEGS / 256;
-  constexpr int BCSTEP = 256 / SEGS;
-  __shared__ __attribute__((aligned(16))) int8_t sB[KP * SB];
+  constexpr int B_LOADS = (KP * SEGS + NT - 1) / NT;
+  constexpr int BCSTEP = NT / SEGS;
+  constexpr int BUF = KP * SB;
+  __shared__ __attribute__((aligned(16))) int8_t sB[2 * BUF];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 15, q = lane >> 4;
   const size_t rowb = (size_t)blockIdx.x * ROWS + (size_t)wave * 16 * RT;
   const size_t split = blockIdx.y;
   const size_t k0 = split * red_per_split;
   const size_t k1 = min(k0 + red_per_split, red_len);
+  if (k0 >= k1) return;  // uniform over the work-group
   const int bcol0 = tid / SEGS, bseg = tid % SEGS;
   const uint8_t* abase = codes + (rowb + r) * code_stride + 16 * q;
   const int8_t* bbase = bq + (size_t)bcol0 * b_stride + 16 * bseg;
@@ -631,22 +637,22 @@ __global__ __launch_bounds__(256) void i8_gemm_kernel(
 #pragma unroll
     for (int t = 0; t < KT; ++t) acc[g][t] = i32x4{0, 0, 0, 0};
   u32x4 pa[RT], pb[B_LOADS], a_cur[RT];
-  if (k0 < k1) {
 #pragma unroll
-    for (int g = 0; g < RT; ++g)
-      pa[g] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + k0 / 4);
+  for (int g = 0; g < RT; ++g)
+    pa[g] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + k0 / 4);
 #pragma unroll
-    for (int j = 0; j < B_LOADS; ++j)
+  for (int j = 0; j < B_LOADS; ++j)
+    if (bcol0 + BCSTEP * j < KP)
       pb[j] = *reinterpret_cast<const u32x4*>(bbase + (size_t)(BCSTEP * j) * b_stride + k0);
-  }
-  for (size_t kk = k0; kk < k1; kk += kI8KC) {
-    __syncthreads();  // the previous chunk's operand is no longer read
 #pragma unroll
-    for (int j = 0; j < B_LOADS; ++j)
+  for (int j = 0; j < B_LOADS; ++j)
+    if (bcol0 + BCSTEP * j < KP)
       *reinterpret_cast<u32x4*>(sB + (bcol0 + BCSTEP * j) * SB + 16 * bseg) = pb[j];
+  __syncthreads();
+  int buf = 0;
+  for (size_t kk = k0; kk < k1; kk += kI8KC) {
 #pragma unroll
     for (int g = 0; g < RT; ++g) a_cur[g] = pa[g];
-    __syncthreads();
     {
       // next chunk; the last pass reloads its own chunk (in bounds, unused) so that the
       // prefetch stays in registers
@@ -656,14 +662,16 @@ __global__ __launch_bounds__(256) void i8_gemm_kernel(
         pa[g] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + kn / 4);
 #pragma unroll
       for (int j = 0; j < B_LOADS; ++j)
-        pb[j] = *reinterpret_cast<const u32x4*>(bbase + (size_t)(BCSTEP * j) * b_stride + kn);
+        if (bcol0 + BCSTEP * j < KP)
+          pb[j] = *reinterpret_cast<const u32x4*>(bbase + (size_t)(BCSTEP * j) * b_stride + kn);
     }
+    const int8_t* sb = sB + buf * BUF;
 #pragma unroll
     for (int ks = 0; ks < kI8KC / 64; ++ks) {
       i32x4 b[KT];
 #pragma unroll
       for (int t = 0; t < KT; ++t)
-        b[t] = *reinterpret_cast<const i32x4*>(sB + (16 * t + r) * SB + 64 * q + 16 * ks);
+        b[t] = *reinterpret_cast<const i32x4*>(sb + (16 * t + r) * SB + 64 * q + 16 * ks);
 #pragma unroll
       for (int g = 0; g < RT; ++g) {
         const uint32_t w = a_cur[g][ks];
@@ -676,6 +684,14 @@ __global__ __launch_bounds__(256) void i8_gemm_kernel(
           acc[g][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[t], acc[g][t], 0, 0, 0);
       }
     }
+    // the other buffer was last read before the previous barrier
+#pragma unroll
+    for (int j = 0; j < B_LOADS; ++j)
+      if (bcol0 + BCSTEP * j < KP)
+        *reinterpret_cast<u32x4*>(sB + (1 - buf) * BUF + (bcol0 + BCSTEP * j) * SB + 16 * bseg) =
+            pb[j];
+    __syncthreads();
+    buf = 1 - buf;
   }
 #pragma unroll
   for (int g = 0; g < RT; ++g)
@@ -747,16 +763,17 @@ __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
   }
 }
 
-// Work-groups of i8_gemm_kernel<RT, KT> resident at once on the device (cached per
+// Work-groups of i8_gemm_kernel<RT, KT, NW> resident at once on the device (cached per
 // instantiation).
-template <int RT, int KT>
+template <int RT, int KT, int NW>
 static size_t resident_slots() {
   static size_t slots = 0;
   if (!slots) {
     int dev = 0, cus = 0, occ = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, i8_gemm_kernel<RT, KT>, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, i8_gemm_kernel<RT, KT, NW>, NW * 64,
+                                                       0);
     slots = (size_t)std::max(1, cus) * (size_t)std::max(1, occ);
   }
   return slots;
@@ -777,21 +794,22 @@ static size_t pick_splits(size_t rblk, size_t nchunks, size_t slots) {
   return best;
 }
 
-template <int RT, int KT>
+template <int RT, int KT, int NW>
 static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const int8_t* bq,
                    size_t b_stride, size_t rows_pad, size_t red_len, DevBuf& part,
                    size_t* splits_out) {
   constexpr int KP = 16 * KT;
-  constexpr int ROWS = 64 * RT;
+  constexpr int ROWS = NW * 16 * RT;
+  static_assert(kMarkerPad % ROWS == 0 && kSamplePad % ROWS == 0, "row tiles must divide padding");
   hipStream_t s = h->stream;
   const size_t rblk = rows_pad / ROWS;
   const size_t nch = red_len / kI8KC;
-  const size_t splits = pick_splits(rblk, nch, resident_slots<RT, KT>());
+  const size_t splits = pick_splits(rblk, nch, resident_slots<RT, KT, NW>());
   const size_t rps = ceil_div(nch, splits) * kI8KC;
   const size_t sp = ceil_div(red_len, rps);
   SAIGE_TRY(part.ensure(sp * rows_pad * KP * 4));
-  SAIGE_TRY(timed_launch(h->timer, i8_gemm_kernel<RT, KT>, dim3((unsigned)rblk, (unsigned)sp),
-                         dim3(256), s, codes, code_stride, bq, b_stride, rps, red_len,
+  SAIGE_TRY(timed_launch(h->timer, i8_gemm_kernel<RT, KT, NW>, dim3((unsigned)rblk, (unsigned)sp),
+                         dim3(NW * 64), s, codes, code_stride, bq, b_stride, rps, red_len,
                          part.as<int>(), rows_pad));
   *splits_out = sp;
   return SAIGE_OK;
@@ -801,7 +819,8 @@ template <int KT>
 static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
                            size_t ldo, const double* d_w, const double* tau) {
   constexpr int KP = 16 * KT;
-  constexpr int RT = KT <= 2 ? 4 : 8;
+  constexpr int RT = 4;
+  constexpr int NW = KT <= 2 ? 4 : 8;
   GrmDevice& g = h->g;
   MixedState* ms = mstate(h);
   hipStream_t s = h->stream;
@@ -843,8 +862,8 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
 
   // ---- stage 1: I = G^T Q_v (rows = markers, reduction over samples) ----
   size_t splits1 = 0;
-  SAIGE_TRY((i8_gemm<RT, KT>(h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad, n_pad,
-                              ms->ipart1, &splits1)));
+  SAIGE_TRY((i8_gemm<RT, KT, NW>(h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
+                                  n_pad, ms->ipart1, &splits1)));
   i8_stage1_epilogue_kernel<KP><<<(unsigned)(m_pad / 64), 256, 0, s>>>(
       ms->ipart1.as<int>(), splits1, m_pad, m, k, ms->vscale.as<double>(), ms->sc.as<double>(),
       ms->p2.as<double>(), ms->zs.as<double>(), cs, mean, ms->w.as<double>(), m);
@@ -865,8 +884,8 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
 
   // ---- stage 2: I = G Q_w (rows = samples, reduction over markers) ----
   size_t splits2 = 0;
-  SAIGE_TRY((i8_gemm<RT, KT>(h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(), m_pad,
-                              n_pad, m_pad, ms->ipart2, &splits2)));
+  SAIGE_TRY((i8_gemm<RT, KT, NW>(h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
+                                  m_pad, n_pad, m_pad, ms->ipart2, &splits2)));
   i8_stage2_epilogue_kernel<KP><<<(unsigned)(n_pad / 64), 256, 0, s>>>(
       ms->ipart2.as<int>(), splits2, n_pad, n, k, ms->wscale.as<double>(), ms->rsum.as<double>(),
       ms->P2, nu, s2p, 1.0 / (double)m, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0,
@@ -880,7 +899,7 @@ int grm_int8_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
                      size_t ldo, const double* d_w, const double* tau) {
   SAIGE_ARG(h->g.n_pad <= kI8MaxRed && h->g.m_pad <= kI8MaxRed,
             "SAIGE_PREC_INT8 needs n and M <= 8,000,000 (exact int32 accumulation)");
-  constexpr size_t kMaxVec = 16 * kI8MaxKT / 3;  // 42 vectors per launch
+  constexpr size_t kMaxVec = 16 * kI8MaxKT / 3;  // 37 vectors per launch
   for (size_t c0 = 0; c0 < k; c0 += kMaxVec) {
     const size_t kc = std::min(kMaxVec, k - c0);
     const double* v = d_v + c0 * ldv;
@@ -893,8 +912,7 @@ int grm_int8_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
       case 4: rc = int8_product_kt<4>(h, v, ldv, kc, o, ldo, d_w, tau); break;
       case 5: rc = int8_product_kt<5>(h, v, ldv, kc, o, ldo, d_w, tau); break;
       case 6: rc = int8_product_kt<6>(h, v, ldv, kc, o, ldo, d_w, tau); break;
-      case 7: rc = int8_product_kt<7>(h, v, ldv, kc, o, ldo, d_w, tau); break;
-      default: rc = int8_product_kt<8>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+      default: rc = int8_product_kt<7>(h, v, ldv, kc, o, ldo, d_w, tau); break;
     }
     if (rc != SAIGE_OK) return rc;
   }

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <condition_variable>
 #include <mutex>
 #include <numeric>
@@ -192,6 +193,26 @@ static bool fit_glm_irls(const double* y, const double* x, size_t n, size_t p, i
   return true;
 }
 
+struct ChaChaKey {
+  uint32_t k[8];
+};
+
+// Rademacher probes from the ChaCha8 stream: thread b expands block b (stream words
+// 16b..16b+15); word j is probe j / n, sample j % n, +1 when its sign bit is set.
+__global__ void rademacher_kernel(ChaChaKey key, size_t n, size_t R, double* __restrict__ out,
+                                  size_t ld) {
+  const size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t words = n * R;
+  if (16 * b >= words) return;
+  uint32_t w[16];
+  chacha8_block(key.k, (uint64_t)b, w);
+  for (int t = 0; t < 16; ++t) {
+    const size_t j = 16 * b + t;
+    if (j >= words) break;
+    out[(j % n) + (j / n) * ld] = (w[t] >> 31) ? 1.0 : -1.0;
+  }
+}
+
 struct RemlState {
   DevBuf x, y, eta, mu, w, rhs1, sol1, gu, py, gpy, pu, rhs2, sol2, tmp, coef, part, res, ones;
   PcgWork pcg;
@@ -249,15 +270,16 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
   SAIGE_HIP(hipMemcpyAsync(dY, y, n * 8, hipMemcpyHostToDevice, s));
   // X columns and the Rademacher probes are fixed right-hand sides (ai_reml.rs:121-130).
   SAIGE_HIP(hipMemcpy2DAsync(RHS1 + ld, ld * 8, x, n * 8, n * 8, p, hipMemcpyHostToDevice, s));
-  {
+  if (R > 0) {
+    // probe r, sample i is the sign bit of stream word r*n + i (ai_reml.rs:121-130),
+    // generated on the device from the same key
     ChaCha8 rng(cfg->seed);
-    std::vector<double> probes(n * R);
-    for (size_t r = 0; r < R; ++r)
-      for (size_t i = 0; i < n; ++i) probes[r * n + i] = rng.gen_bool() ? 1.0 : -1.0;
-    if (R > 0)
-      SAIGE_HIP(hipMemcpy2DAsync(RHS1 + (1 + p) * ld, ld * 8, probes.data(), n * 8, n * 8, R,
-                                 hipMemcpyHostToDevice, s));
-    SAIGE_HIP(hipStreamSynchronize(s));
+    ChaChaKey key;
+    std::memcpy(key.k, rng.key(), sizeof(key.k));
+    const size_t words = n * R;
+    rademacher_kernel<<<(unsigned)ceil_div(ceil_div(words, 16), 256), 256, 0, s>>>(
+        key, n, R, RHS1 + (1 + p) * ld, ld);
+    SAIGE_HIP(hipGetLastError());
   }
   const double* U = RHS1 + (1 + p) * ld;
   if (R > 0) SAIGE_TRY(sink->product(U, ld, R, GU, ld, nullptr, nullptr));  // constant GRM * u

@@ -5,12 +5,37 @@
 //   rng.gen_range(0..n) for usize       fit_null.rs:237  (64-bit widening-multiply rejection)
 //   indices.shuffle(&mut rng)           variance_ratio.rs:115 (Fisher-Yates from the end)
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <cstring>
 #include <utility>
 #include <vector>
 
 namespace saige {
+
+// One 64-byte ChaCha block with 8 rounds, 64-bit block counter and stream 0 (the
+// rand_chacha 0.3.1 layout); the same code serves the host stream and the device twin.
+__host__ __device__ inline void chacha8_block(const uint32_t key[8], uint64_t ctr, uint32_t out[16]) {
+  uint32_t s[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, key[0], key[1], key[2],
+                    key[3], key[4], key[5], key[6], key[7], (uint32_t)ctr, (uint32_t)(ctr >> 32),
+                    0u, 0u};
+  uint32_t x[16];
+  for (int i = 0; i < 16; ++i) x[i] = s[i];
+#define SAIGE_RL(v, r) (((v) << (r)) | ((v) >> (32 - (r))))
+#define SAIGE_QR(a, b, c, d)                                  \
+  x[a] += x[b]; x[d] = SAIGE_RL(x[d] ^ x[a], 16);             \
+  x[c] += x[d]; x[b] = SAIGE_RL(x[b] ^ x[c], 12);             \
+  x[a] += x[b]; x[d] = SAIGE_RL(x[d] ^ x[a], 8);              \
+  x[c] += x[d]; x[b] = SAIGE_RL(x[b] ^ x[c], 7);
+  for (int dr = 0; dr < 4; ++dr) {
+    SAIGE_QR(0, 4, 8, 12) SAIGE_QR(1, 5, 9, 13) SAIGE_QR(2, 6, 10, 14) SAIGE_QR(3, 7, 11, 15)
+    SAIGE_QR(0, 5, 10, 15) SAIGE_QR(1, 6, 11, 12) SAIGE_QR(2, 7, 8, 13) SAIGE_QR(3, 4, 9, 14)
+  }
+#undef SAIGE_QR
+#undef SAIGE_RL
+  for (int i = 0; i < 16; ++i) out[i] = x[i] + s[i];
+}
 
 class ChaCha8 {
  public:
@@ -56,31 +81,17 @@ class ChaCha8 {
   }
 
  private:
-  // Four consecutive 64-byte ChaCha blocks (8 rounds, 64-bit counter, stream 0).
+  // Four consecutive 64-byte ChaCha blocks.
   void refill() {
-    for (int b = 0; b < 4; ++b) block(ctr_ + b, buf_ + 16 * b);
+    for (int b = 0; b < 4; ++b) chacha8_block(key_, ctr_ + b, buf_ + 16 * b);
     ctr_ += 4;
     pos_ = 0;
   }
-  static uint32_t rl(uint32_t v, int r) { return (v << r) | (v >> (32 - r)); }
-  void block(uint64_t ctr, uint32_t* out) const {
-    uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, key_[0], key_[1],
-                       key_[2], key_[3], key_[4], key_[5], key_[6], key_[7], (uint32_t)ctr,
-                       (uint32_t)(ctr >> 32), 0u, 0u};
-    uint32_t s[16];
-    std::memcpy(s, in, sizeof(s));
-    auto q = [&](int a, int b, int c, int d) {
-      s[a] += s[b]; s[d] = rl(s[d] ^ s[a], 16);
-      s[c] += s[d]; s[b] = rl(s[b] ^ s[c], 12);
-      s[a] += s[b]; s[d] = rl(s[d] ^ s[a], 8);
-      s[c] += s[d]; s[b] = rl(s[b] ^ s[c], 7);
-    };
-    for (int dr = 0; dr < 4; ++dr) {
-      q(0, 4, 8, 12); q(1, 5, 9, 13); q(2, 6, 10, 14); q(3, 7, 11, 15);
-      q(0, 5, 10, 15); q(1, 6, 11, 12); q(2, 7, 8, 13); q(3, 4, 9, 14);
-    }
-    for (int i = 0; i < 16; ++i) out[i] = s[i] + in[i];
-  }
+
+ public:
+  const uint32_t* key() const { return key_; }
+
+ private:
   uint32_t key_[8];
   uint32_t buf_[64];
   int pos_ = 64;

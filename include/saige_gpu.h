@@ -43,6 +43,12 @@ enum { SAIGE_TRAIT_BINARY = 0, SAIGE_TRAIT_QUANTITATIVE = 1, SAIGE_TRAIT_SURVIVA
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* saige_last_error(void);
+
+/* Device memory is allocated through a per-device cache: buffers a handle or a call frees
+ * are kept (up to 32 GiB) and reused by later calls of a similar size, and the cache is
+ * emptied whenever an allocation would otherwise fail.  This releases every cached block
+ * (no counterpart in saige-rs). */
+void saige_release_cached_memory(void);
 /* Library version string. */
 const char* saige_version(void);
 
