@@ -329,7 +329,7 @@ struct MixedState {
   // per-product workspaces
   DevBuf vhi, vlo, vs, whi, wlo, ws_, w, part1, part2, red, cs, mean, nu, s2p;
   // int8 path: sliced operands, per-column slice scales, integer partials
-  DevBuf vq, wq, vscale, wscale, ipart1, ipart2;
+  DevBuf vq, wq, ipart1, ipart2;
 };
 
 static MixedState* mstate(saige_grm* h);
@@ -530,17 +530,36 @@ constexpr size_t kI8MaxRed = 8000000;
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ size_t perm16(size_t i) {
-  return (i & ~(size_t)15) | ((i & 3) << 2) | ((i >> 2) & 3);
+// Column scalars rebuilt from fixed-order partial sums (one value per kDotChunk rows, added
+// in chunk order): the mean of column c, and the three slice scales from the partial maxima
+// of |x - mean| (s0 = max / 127, s1 = s0 / 254, s2 = s1 / 254; 1 for an all-zero column).
+__device__ __forceinline__ double chunk_sum(const double* __restrict__ part, size_t c,
+                                            size_t nchunks) {
+  double s = 0.0;
+  for (size_t q = 0; q < nchunks; ++q) s += part[c * nchunks + q];
+  return s;
+}
+__device__ __forceinline__ void slice_scales(const double* __restrict__ pmax, size_t c,
+                                             size_t nchunks, double* sc3) {
+  double m = 0.0;
+  for (size_t q = 0; q < nchunks; ++q) m = fmax(m, pmax[c * nchunks + q]);
+  const double s0 = m > 0.0 ? m / 127.0 : 1.0;
+  sc3[0] = s0;
+  sc3[1] = s0 / 254.0;
+  sc3[2] = sc3[1] / 254.0;
 }
 
-// per-column partial max of |x - mean| over chunks of kDotChunk rows
-__global__ void col_absmax_partial_kernel(const double* __restrict__ x, size_t ld, size_t len,
-                                          const double* __restrict__ mean, size_t nchunks,
-                                          double* __restrict__ part) {
+// per-column partial max of |x - mean| over chunks of kDotChunk rows, the mean taken from
+// the column's partial sums (psum) times inv_len
+__global__ __launch_bounds__(kDotBlock) void col_absmax_partial_kernel(
+    const double* __restrict__ x, size_t ld, size_t len, const double* __restrict__ psum,
+    double inv_len, size_t nchunks, double* __restrict__ part) {
   const size_t c = blockIdx.y;
+  __shared__ double mu_s;
+  if (threadIdx.x == 0) mu_s = chunk_sum(psum, c, nchunks) * inv_len;
+  __syncthreads();
+  const double mu = mu_s;
   const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, len);
-  const double mu = mean[c];
   double m = 0.0;
   for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) m = fmax(m, fabs(x[i + c * ld] - mu));
   __shared__ double sh[kDotBlock];
@@ -553,51 +572,60 @@ __global__ void col_absmax_partial_kernel(const double* __restrict__ x, size_t l
   if (threadIdx.x == 0) part[c * nchunks + blockIdx.x] = sh[0];
 }
 
-// slice scales of each column from its partial maxima: scale[3c + j] = s_j
-__global__ void slice_scales_kernel(const double* __restrict__ part, size_t nchunks, size_t k,
-                                    double* __restrict__ scale) {
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= k) return;
-  double m = 0.0;
-  for (size_t q = 0; q < nchunks; ++q) m = fmax(m, part[c * nchunks + q]);
-  const double s0 = m > 0.0 ? m / 127.0 : 1.0;
-  const double s1 = s0 / 254.0;
-  scale[3 * c] = s0;
-  scale[3 * c + 1] = s1;
-  scale[3 * c + 2] = s1 / 254.0;
+__device__ __forceinline__ size_t perm16(size_t i) {
+  return (i & ~(size_t)15) | ((i & 3) << 2) | ((i >> 2) & 3);
 }
-
 __device__ __forceinline__ double q127(double t) { return fmin(fmax(rint(t), -127.0), 127.0); }
 
-// x' = x - mean -> three int8 slices (stored in the permuted reduction order, zero for
-// rows >= len and for the padding columns) and the fp64 value they represent (xs).
-// grid.y = k + (kp - 3k): the last kp - 3k rows of the grid clear the padding columns.
-__global__ void quantize3_kernel(const double* __restrict__ x, size_t ldx, size_t len, size_t k,
-                                 const double* __restrict__ mean, const double* __restrict__ scale,
-                                 size_t pad_len, int8_t* __restrict__ q, double* __restrict__ xs,
-                                 size_t ldxs) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// x' = x - mean -> three int8 slices (stored in the permuted reduction order, zero for rows
+// >= len), the fp64 value they represent (xs), and per chunk of kDotChunk rows the partial
+// sum of wt * xs (wt = 1 when null) for the rank-one terms.  One block per (chunk, column);
+// grid.y = k + (KP - 3k): the last rows of the grid clear the padding columns.
+__global__ __launch_bounds__(kDotBlock) void quantize3_kernel(
+    const double* __restrict__ x, size_t ldx, size_t len, size_t k,
+    const double* __restrict__ psum, const double* __restrict__ pmax, double inv_len,
+    size_t nchunks, size_t pad_len, int8_t* __restrict__ q, double* __restrict__ xs, size_t ldxs,
+    const double* __restrict__ wt, double* __restrict__ pdot) {
   const size_t c = blockIdx.y;
-  if (i >= pad_len) return;
-  const size_t pos = perm16(i);
-  if (c >= k) {
-    q[(2 * k + c) * pad_len + pos] = 0;  // padding column 3k + (c - k)
+  const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, pad_len);
+  if (c >= k) {  // padding column 3k + (c - k)
+    for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) q[(2 * k + c) * pad_len + perm16(i)] = 0;
     return;
   }
-  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-  if (i < len) {
-    const double s0 = scale[3 * c], s1 = scale[3 * c + 1], s2 = scale[3 * c + 2];
-    const double xv = x[i + c * ldx] - mean[c];
-    t0 = q127(xv / s0);
-    double r = fma(-s0, t0, xv);
-    t1 = q127(r / s1);
-    r = fma(-s1, t1, r);
-    t2 = q127(r / s2);
-    xs[i + c * ldxs] = s0 * t0 + s1 * t1 + s2 * t2;
+  __shared__ double cs_s[4];
+  if (threadIdx.x == 0) {
+    cs_s[0] = chunk_sum(psum, c, nchunks) * inv_len;
+    slice_scales(pmax, c, nchunks, cs_s + 1);
   }
-  q[c * pad_len + pos] = (int8_t)t0;
-  q[(k + c) * pad_len + pos] = (int8_t)t1;
-  q[(2 * k + c) * pad_len + pos] = (int8_t)t2;
+  __syncthreads();
+  const double mu = cs_s[0], s0 = cs_s[1], s1 = cs_s[2], s2 = cs_s[3];
+  double acc = 0.0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (i < len) {
+      const double xv = x[i + c * ldx] - mu;
+      t0 = q127(xv / s0);
+      double r = fma(-s0, t0, xv);
+      t1 = q127(r / s1);
+      r = fma(-s1, t1, r);
+      t2 = q127(r / s2);
+      const double v = s0 * t0 + s1 * t1 + s2 * t2;
+      xs[i + c * ldxs] = v;
+      acc = wt ? fma(wt[i], v, acc) : acc + v;
+    }
+    const size_t pos = perm16(i);
+    q[c * pad_len + pos] = (int8_t)t0;
+    q[(k + c) * pad_len + pos] = (int8_t)t1;
+    q[(2 * k + c) * pad_len + pos] = (int8_t)t2;
+  }
+  __shared__ double sh[kDotBlock];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kDotBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && i0 < len) pdot[c * nchunks + blockIdx.x] = sh[0];
 }
 
 // part[split][row][16*KT] = sum over the split's reduction range of g(row, red) * q[col][red].
@@ -716,14 +744,22 @@ __device__ __forceinline__ void i8_tile_sum(const int* __restrict__ part, size_t
 }
 
 // Stage-1 epilogue: U'_mc = sum_j s_jc I_jmc; T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c;
-// W_mc = s_m T_mc (column-major, ldw).
+// W_mc = s_m T_mc (column-major, ldw).  mean_c, the slice scales and cs_c are rebuilt from
+// the partials of the operand pass (column sums psum, maxima pmax, sliced sums pcs).
 template <int KP>
 __global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
     const int* __restrict__ part, size_t splits, size_t m_pad, size_t m, size_t k,
-    const double* __restrict__ vscale, const double* __restrict__ sc,
-    const double* __restrict__ p2, const double* __restrict__ zs, const double* __restrict__ cs,
-    const double* __restrict__ mean, double* __restrict__ w, size_t ldw) {
+    const double* __restrict__ psum, const double* __restrict__ pmax,
+    const double* __restrict__ pcs, double inv_n, size_t nch, const double* __restrict__ sc,
+    const double* __restrict__ p2, const double* __restrict__ zs, double* __restrict__ w,
+    size_t ldw) {
   __shared__ int tile[64][KP + 1];
+  __shared__ double col[KP / 3 + 1][5];  // mean, s0, s1, s2, cs
+  for (size_t c = threadIdx.x; c < k; c += 256) {
+    col[c][0] = chunk_sum(psum, c, nch) * inv_n;
+    slice_scales(pmax, c, nch, &col[c][1]);
+    col[c][4] = chunk_sum(pcs, c, nch);
+  }
   const size_t row0 = (size_t)blockIdx.x * 64;
   i8_tile_sum<KP>(part, splits, m_pad, row0, tile);
   const int r = threadIdx.x & 63;
@@ -731,23 +767,29 @@ __global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
   if (mm >= m) return;
   const double s = sc[mm], pp = p2[mm], z = zs[mm];
   for (size_t c = threadIdx.x >> 6; c < k; c += 4) {
-    const double u = vscale[3 * c] * (double)tile[r][c] +
-                     vscale[3 * c + 1] * (double)tile[r][k + c] +
-                     vscale[3 * c + 2] * (double)tile[r][2 * k + c];
-    const double t = s * (u - pp * cs[c]) + z * mean[c];
+    const double u = col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c] +
+                     col[c][3] * (double)tile[r][2 * k + c];
+    const double t = s * (u - pp * col[c][4]) + z * col[c][0];
     w[mm + c * ldw] = s * t;
   }
 }
 
 // Stage-2 epilogue: M Y_ic = sum_j s_jc I_jic + nu_c (r_i - P2) - S2p_c; optional Sigma form.
+// nu_c, the slice scales and S2p_c are rebuilt from the partials of the W pass.
 template <int KP>
 __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
     const int* __restrict__ part, size_t splits, size_t n_pad, size_t n, size_t k,
-    const double* __restrict__ wscale, const double* __restrict__ rsum, double P2,
-    const double* __restrict__ nu, const double* __restrict__ s2p, double inv_m,
-    double* __restrict__ out, size_t ldo, const double* __restrict__ v, size_t ldv,
+    const double* __restrict__ psum, const double* __restrict__ pmax,
+    const double* __restrict__ ps2p, double inv_m, size_t nch, const double* __restrict__ rsum,
+    double P2, double* __restrict__ out, size_t ldo, const double* __restrict__ v, size_t ldv,
     const double* __restrict__ wts, double tau0, double tau1) {
   __shared__ int tile[64][KP + 1];
+  __shared__ double col[KP / 3 + 1][5];  // nu, s0, s1, s2, S2p
+  for (size_t c = threadIdx.x; c < k; c += 256) {
+    col[c][0] = chunk_sum(psum, c, nch) * inv_m;
+    slice_scales(pmax, c, nch, &col[c][1]);
+    col[c][4] = chunk_sum(ps2p, c, nch);
+  }
   const size_t row0 = (size_t)blockIdx.x * 64;
   i8_tile_sum<KP>(part, splits, n_pad, row0, tile);
   const int r = threadIdx.x & 63;
@@ -755,9 +797,9 @@ __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
   if (i >= n) return;
   const double ri = rsum[i];
   for (size_t c = threadIdx.x >> 6; c < k; c += 4) {
-    double y = wscale[3 * c] * (double)tile[r][c] + wscale[3 * c + 1] * (double)tile[r][k + c] +
-               wscale[3 * c + 2] * (double)tile[r][2 * k + c];
-    y = (y + nu[c] * (ri - P2) - s2p[c]) * inv_m;
+    double y = col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c] +
+               col[c][3] * (double)tile[r][2 * k + c];
+    y = (y + col[c][0] * (ri - P2) - col[c][4]) * inv_m;
     if (wts) y = tau0 * v[i + c * ldv] / fmax(wts[i], 1e-30) + tau1 * y;
     out[i + c * ldo] = y;
   }
@@ -833,62 +875,49 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
   SAIGE_TRY(ms->vs.ensure(n * k * 8));
   SAIGE_TRY(ms->ws_.ensure(m * k * 8));
   SAIGE_TRY(ms->w.ensure(m * k * 8));
-  SAIGE_TRY(ms->vscale.ensure(3 * k * 8));
-  SAIGE_TRY(ms->wscale.ensure(3 * k * 8));
-  SAIGE_TRY(ms->red.ensure(std::max(nch_n, nch_m) * k * 8));
-  SAIGE_TRY(ms->cs.ensure(k * 8));
-  SAIGE_TRY(ms->mean.ensure(k * 8));
-  SAIGE_TRY(ms->nu.ensure(k * 8));
-  SAIGE_TRY(ms->s2p.ensure(k * 8));
-  double* red = ms->red.as<double>();
-  double* mean = ms->mean.as<double>();
-  double* cs = ms->cs.as<double>();
-  double* nu = ms->nu.as<double>();
-  double* s2p = ms->s2p.as<double>();
-  const unsigned kb = (unsigned)ceil_div(k, 256);
+  SAIGE_TRY(ms->red.ensure(3 * std::max(nch_n, nch_m) * k * 8));
 
-  // ---- V: column means, slice scales, slices, cs' = colsum of the sliced value ----
-  col_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n, red);
-  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_n, k, 1.0 / (double)n, mean);
+  // Ten launches per product.  Column scalars (means, slice scales, sums of the sliced
+  // values) stay as fixed-order partials per kDotChunk rows and are finished by the kernel
+  // that needs them.
+  double* psum = ms->red.as<double>();                // column sums of V, then of W
+  double* pmax = psum + std::max(nch_n, nch_m) * k;   // partial maxima
+  double* pdot = pmax + std::max(nch_n, nch_m) * k;   // sums of the sliced values
+  const dim3 gq_n((unsigned)ceil_div(n_pad, kDotChunk), (unsigned)k + pad_cols);
+  const dim3 gq_m((unsigned)ceil_div(m_pad, kDotChunk), (unsigned)k + pad_cols);
+
+  // ---- V: column sums, maxima of |V - mean|, slices + sums of the sliced values ----
+  col_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n, psum);
   col_absmax_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(
-      d_v, ldv, n, mean, nch_n, red);
-  slice_scales_kernel<<<kb, 256, 0, s>>>(red, nch_n, k, ms->vscale.as<double>());
-  quantize3_kernel<<<dim3((unsigned)ceil_div(n_pad, 256), (unsigned)k + pad_cols), 256, 0, s>>>(
-      d_v, ldv, n, k, mean, ms->vscale.as<double>(), n_pad, ms->vq.as<int8_t>(),
-      ms->vs.as<double>(), n);
-  col_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(ms->vs.as<double>(), n,
-                                                                               n, nch_n, red);
-  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_n, k, 1.0, cs);
+      d_v, ldv, n, psum, 1.0 / (double)n, nch_n, pmax);
+  quantize3_kernel<<<gq_n, kDotBlock, 0, s>>>(d_v, ldv, n, k, psum, pmax, 1.0 / (double)n, nch_n,
+                                              n_pad, ms->vq.as<int8_t>(), ms->vs.as<double>(), n,
+                                              nullptr, pdot);
 
   // ---- stage 1: I = G^T Q_v (rows = markers, reduction over samples) ----
   size_t splits1 = 0;
   SAIGE_TRY((i8_gemm<RT, KT, NW>(h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
                                   n_pad, ms->ipart1, &splits1)));
   i8_stage1_epilogue_kernel<KP><<<(unsigned)(m_pad / 64), 256, 0, s>>>(
-      ms->ipart1.as<int>(), splits1, m_pad, m, k, ms->vscale.as<double>(), ms->sc.as<double>(),
-      ms->p2.as<double>(), ms->zs.as<double>(), cs, mean, ms->w.as<double>(), m);
+      ms->ipart1.as<int>(), splits1, m_pad, m, k, psum, pmax, pdot, 1.0 / (double)n, nch_n,
+      ms->sc.as<double>(), ms->p2.as<double>(), ms->zs.as<double>(), ms->w.as<double>(), m);
 
-  // ---- W: mean over real markers, slices, S2p = sum_m 2p_m W'_split ----
+  // ---- W: column sums over real markers, maxima, slices + S2p partials ----
   col_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(ms->w.as<double>(), m,
-                                                                               m, nch_m, red);
-  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_m, k, 1.0 / (double)m, nu);
+                                                                               m, nch_m, psum);
   col_absmax_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
-      ms->w.as<double>(), m, m, nu, nch_m, red);
-  slice_scales_kernel<<<kb, 256, 0, s>>>(red, nch_m, k, ms->wscale.as<double>());
-  quantize3_kernel<<<dim3((unsigned)ceil_div(m_pad, 256), (unsigned)k + pad_cols), 256, 0, s>>>(
-      ms->w.as<double>(), m, m, k, nu, ms->wscale.as<double>(), m_pad, ms->wq.as<int8_t>(),
-      ms->ws_.as<double>(), m);
-  wdot_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
-      ms->ws_.as<double>(), m, ms->p2.as<double>(), m, nch_m, red);
-  col_finalize_kernel<<<kb, 256, 0, s>>>(red, nch_m, k, 1.0, s2p);
+      ms->w.as<double>(), m, m, psum, 1.0 / (double)m, nch_m, pmax);
+  quantize3_kernel<<<gq_m, kDotBlock, 0, s>>>(ms->w.as<double>(), m, m, k, psum, pmax,
+                                              1.0 / (double)m, nch_m, m_pad, ms->wq.as<int8_t>(),
+                                              ms->ws_.as<double>(), m, ms->p2.as<double>(), pdot);
 
   // ---- stage 2: I = G Q_w (rows = samples, reduction over markers) ----
   size_t splits2 = 0;
   SAIGE_TRY((i8_gemm<RT, KT, NW>(h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
                                   m_pad, n_pad, m_pad, ms->ipart2, &splits2)));
   i8_stage2_epilogue_kernel<KP><<<(unsigned)(n_pad / 64), 256, 0, s>>>(
-      ms->ipart2.as<int>(), splits2, n_pad, n, k, ms->wscale.as<double>(), ms->rsum.as<double>(),
-      ms->P2, nu, s2p, 1.0 / (double)m, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0,
+      ms->ipart2.as<int>(), splits2, n_pad, n, k, psum, pmax, pdot, 1.0 / (double)m, nch_m,
+      ms->rsum.as<double>(), ms->P2, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0,
       tau ? tau[1] : 0.0);
   SAIGE_HIP(hipGetLastError());
   h->products += k;

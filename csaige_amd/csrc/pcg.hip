@@ -117,30 +117,36 @@ __global__ __launch_bounds__(kDotBlock) void pcg_pap_kernel(const double* __rest
   if (threadIdx.x == 0) part[c * nchunks + blockIdx.x] = s;
 }
 
-__global__ void pcg_alpha_fin(const double* __restrict__ part, size_t nchunks, size_t k,
-                              int iter, ColState* __restrict__ st) {
-  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= k || !st[c].active) return;
-  double pap = 0.0;
-  for (size_t q = 0; q < nchunks; ++q) pap += part[c * nchunks + q];
-  if (fabs(pap) < 1e-30) {  // breakdown: decomposition.rs:257-264
-    st[c].active = 0;
-    st[c].converged = 0;
-    st[c].iters = iter;
-    return;
-  }
-  st[c].alpha = st[c].rz / pap;
-}
-
-// x += alpha p; r -= alpha Ap; z = M^-1 r; partial r'z and r'r.
+// x += alpha p; r -= alpha Ap; z = M^-1 r; partial r'z and r'r.  Every block of column c
+// first finishes p'Ap from the partials (same fixed order in every block) and takes
+// alpha = r'z / p'Ap; on breakdown (|p'Ap| < 1e-30, decomposition.rs:257-264) the column
+// stops, recorded by block 0.
 __global__ __launch_bounds__(kDotBlock) void pcg_update_kernel(
     double* __restrict__ x, size_t ldx, double* __restrict__ r, double* __restrict__ z,
     const double* __restrict__ p, const double* __restrict__ ap, size_t ld,
     const double* __restrict__ w, double tau0, double tau1, size_t n, size_t nchunks,
-    const ColState* __restrict__ st, double* __restrict__ part) {
+    const double* __restrict__ pap_part, int iter, ColState* __restrict__ st,
+    double* __restrict__ part) {
   const size_t c = blockIdx.y;
   if (!st[c].active) return;
-  const double alpha = st[c].alpha;
+  __shared__ double alpha_s;
+  __shared__ int stop_s;
+  if (threadIdx.x == 0) {
+    double pap = 0.0;
+    for (size_t q = 0; q < nchunks; ++q) pap += pap_part[c * nchunks + q];
+    stop_s = fabs(pap) < 1e-30;
+    alpha_s = stop_s ? 0.0 : st[c].rz / pap;
+  }
+  __syncthreads();
+  if (stop_s) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st[c].active = 0;
+      st[c].converged = 0;
+      st[c].iters = iter;
+    }
+    return;
+  }
+  const double alpha = alpha_s;
   const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, n);
   double s_rz = 0.0, s_rr = 0.0, d = 0.0;
   for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) {
@@ -161,11 +167,17 @@ __global__ __launch_bounds__(kDotBlock) void pcg_update_kernel(
   }
 }
 
-__global__ void pcg_beta_fin(const double* __restrict__ part, size_t nchunks, size_t k, int iter,
-                             ColState* __restrict__ st, int* __restrict__ n_active) {
-  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= k) return;
-  if (st[c].active) {
+// beta, new r'z and ||r|| per active column, the stop test of the next iteration, and the
+// number of columns still active (one block of 1024 threads covers all columns).
+__global__ __launch_bounds__(1024) void pcg_beta_fin(const double* __restrict__ part,
+                                                     size_t nchunks, size_t k, int iter,
+                                                     ColState* __restrict__ st,
+                                                     int* __restrict__ n_active) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const size_t c = threadIdx.x;
+  if (c < k && st[c].active) {
     double rz = 0.0, rr = 0.0;
     for (size_t q = 0; q < nchunks; ++q) {
       rz += part[(0 * k + c) * nchunks + q];
@@ -180,10 +192,6 @@ __global__ void pcg_beta_fin(const double* __restrict__ part, size_t nchunks, si
       st[c].iters = iter + 1;
     }
   }
-  // count (single block of <= 1024 threads handles all columns)
-  __shared__ int cnt;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
   if (c < k && st[c].active) atomicAdd(&cnt, 1);
   __syncthreads();
   if (threadIdx.x == 0) *n_active = cnt;
@@ -215,7 +223,7 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   hipStream_t s = h->stream;
   const size_t nchunks = std::max<size_t>(1, ceil_div(n, kDotChunk));
   SAIGE_TRY(ws.vec.ensure(4 * ld * k * 8));
-  SAIGE_TRY(ws.part.ensure(3 * k * nchunks * 8));
+  SAIGE_TRY(ws.part.ensure(4 * k * nchunks * 8));
   SAIGE_TRY(ws.scal.ensure(k * sizeof(ColState) + 64));
   double* R = ws.vec.as<double>();
   double* Z = R + ld * k;
@@ -224,6 +232,7 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   ColState* st = reinterpret_cast<ColState*>(ws.scal.as<char>() + 64);
   int* d_nact = ws.scal.as<int>();
   double* part = ws.part.as<double>();
+  double* pap_part = part + 3 * k * nchunks;  // p'Ap partials, apart from the update's
   const double tau0 = tau[0], tau1 = tau[1];
   const unsigned kb = (unsigned)ceil_div(k, 256);
   dim3 gchunk((unsigned)nchunks, (unsigned)k);
@@ -242,10 +251,9 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   int pinned_nact = 0;
   for (uint32_t iter = 0; iter < max_iter && n_active > 0; ++iter) {
     SAIGE_TRY(sink->product(P, ld, k, AP, ld, d_w, tau));
-    pcg_pap_kernel<<<gchunk, kDotBlock, 0, s>>>(P, AP, ld, n, nchunks, st, part);
-    pcg_alpha_fin<<<kb, 256, 0, s>>>(part, nchunks, k, (int)iter, st);
+    pcg_pap_kernel<<<gchunk, kDotBlock, 0, s>>>(P, AP, ld, n, nchunks, st, pap_part);
     pcg_update_kernel<<<gchunk, kDotBlock, 0, s>>>(d_x, ldx, R, Z, P, AP, ld, d_w, tau0, tau1, n,
-                                                   nchunks, st, part);
+                                                   nchunks, pap_part, (int)iter, st, part);
     pcg_beta_fin<<<1, 1024, 0, s>>>(part, nchunks, k, (int)iter, st, d_nact);
     pcg_p_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(P, Z, ld, n, st);
     SAIGE_HIP(hipGetLastError());
