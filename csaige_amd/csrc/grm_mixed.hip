@@ -523,7 +523,6 @@ void grm_mixed_release(saige_grm* h) {
 // 4u+b): a lane expands one 32-bit word of packed codes (code 4b+u at bits 8b+2u) with
 // v_perm into four words, word u holding codes {u, 4+u, 8+u, 12+u} one per byte, and the
 // operand's bytes then pair with the same reduction indices.
-constexpr int kI8KC = 256;        // reduction indices per staged chunk
 constexpr int kI8MaxKT = 7;       // at most 112 slice columns (37 vectors) per launch
 constexpr size_t kI8MaxRed = 8000000;
 
@@ -630,13 +629,15 @@ __global__ __launch_bounds__(kDotBlock) void quantize3_kernel(
 
 // part[split][row][16*KT] = sum over the split's reduction range of g(row, red) * q[col][red].
 // NW waves; each wave owns 16*RT rows and reads their packed codes straight from global
-// memory (16 bytes = 64 genotypes per lane and chunk, the next chunk prefetched in
-// registers).  The KP x 256 operand chunk is staged in LDS, double-buffered so that one
+// memory (KCH/16 bytes = KCH/4 genotypes per lane and chunk of KCH reduction indices, the
+// next chunk prefetched in registers; lane q takes the q-th quarter of a row's chunk, so a
+// row's chunk is one 64-byte or 128-byte segment).  The KP x KCH operand chunk is staged in
+// LDS, double-buffered so that one
 // barrier per chunk separates the reads of one buffer from the refill of the other (column
 // stride 272 bytes: the 16-byte fragment reads of 8 consecutive columns fall in distinct
 // banks).  Eight waves of RT = 4 keep the 112 accumulators of k = 34 in VGPRs at two waves
 // per SIMD; four waves of RT = 8 spill them into AGPR shuffles.
-template <int RT, int KT, int NW>
+template <int RT, int KT, int NW, int KCH>
 __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
     const uint8_t* __restrict__ codes, size_t code_stride, const int8_t* __restrict__ bq,
     size_t b_stride, size_t red_per_split, size_t red_len, int* __restrict__ part,
@@ -644,8 +645,9 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
   constexpr int KP = 16 * KT;
   constexpr int ROWS = NW * 16 * RT;
   constexpr int NT = NW * 64;
-  constexpr int SB = kI8KC + 16;
-  constexpr int SEGS = kI8KC / 16;
+  constexpr int SB = KCH + 16;
+  constexpr int SEGS = KCH / 16;
+  constexpr int AV = KCH / 256;  // 16-byte code loads per row tile and chunk
   constexpr int B_LOADS = (KP * SEGS + NT - 1) / NT;
   constexpr int BCSTEP = NT / SEGS;
   constexpr int BUF = KP * SB;
@@ -657,17 +659,19 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
   const size_t k1 = min(k0 + red_per_split, red_len);
   if (k0 >= k1) return;  // uniform over the work-group
   const int bcol0 = tid / SEGS, bseg = tid % SEGS;
-  const uint8_t* abase = codes + (rowb + r) * code_stride + 16 * q;
+  const uint8_t* abase = codes + (rowb + r) * code_stride + (KCH / 16) * q;
   const int8_t* bbase = bq + (size_t)bcol0 * b_stride + 16 * bseg;
   i32x4 acc[RT][KT];
 #pragma unroll
   for (int g = 0; g < RT; ++g)
 #pragma unroll
     for (int t = 0; t < KT; ++t) acc[g][t] = i32x4{0, 0, 0, 0};
-  u32x4 pa[RT], pb[B_LOADS], a_cur[RT];
+  u32x4 pa[RT][AV], pb[B_LOADS], a_cur[RT][AV];
 #pragma unroll
   for (int g = 0; g < RT; ++g)
-    pa[g] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + k0 / 4);
+#pragma unroll
+    for (int a = 0; a < AV; ++a)
+      pa[g][a] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + k0 / 4 + 16 * a);
 #pragma unroll
   for (int j = 0; j < B_LOADS; ++j)
     if (bcol0 + BCSTEP * j < KP)
@@ -678,16 +682,20 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
       *reinterpret_cast<u32x4*>(sB + (bcol0 + BCSTEP * j) * SB + 16 * bseg) = pb[j];
   __syncthreads();
   int buf = 0;
-  for (size_t kk = k0; kk < k1; kk += kI8KC) {
+  for (size_t kk = k0; kk < k1; kk += KCH) {
 #pragma unroll
-    for (int g = 0; g < RT; ++g) a_cur[g] = pa[g];
+    for (int g = 0; g < RT; ++g)
+#pragma unroll
+      for (int a = 0; a < AV; ++a) a_cur[g][a] = pa[g][a];
     {
       // next chunk; the last pass reloads its own chunk (in bounds, unused) so that the
       // prefetch stays in registers
-      const size_t kn = (kk + kI8KC < k1) ? kk + kI8KC : kk;
+      const size_t kn = (kk + KCH < k1) ? kk + KCH : kk;
 #pragma unroll
       for (int g = 0; g < RT; ++g)
-        pa[g] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + kn / 4);
+#pragma unroll
+        for (int a = 0; a < AV; ++a)
+          pa[g][a] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + kn / 4 + 16 * a);
 #pragma unroll
       for (int j = 0; j < B_LOADS; ++j)
         if (bcol0 + BCSTEP * j < KP)
@@ -695,14 +703,14 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
     }
     const int8_t* sb = sB + buf * BUF;
 #pragma unroll
-    for (int ks = 0; ks < kI8KC / 64; ++ks) {
+    for (int ks = 0; ks < KCH / 64; ++ks) {
       i32x4 b[KT];
 #pragma unroll
       for (int t = 0; t < KT; ++t)
-        b[t] = *reinterpret_cast<const i32x4*>(sb + (16 * t + r) * SB + 64 * q + 16 * ks);
+        b[t] = *reinterpret_cast<const i32x4*>(sb + (16 * t + r) * SB + (KCH / 4) * q + 16 * ks);
 #pragma unroll
       for (int g = 0; g < RT; ++g) {
-        const uint32_t w = a_cur[g][ks];
+        const uint32_t w = a_cur[g][ks >> 2][ks & 3];
         i32x4 a;
 #pragma unroll
         for (int u = 0; u < 4; ++u)  // codes 00, 01, 10, 11 -> 2, 0, 1, 0
@@ -805,16 +813,16 @@ __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
   }
 }
 
-// Work-groups of i8_gemm_kernel<RT, KT, NW> resident at once on the device (cached per
+// Work-groups of i8_gemm_kernel<RT, KT, NW, KCH> resident at once on the device (cached per
 // instantiation).
-template <int RT, int KT, int NW>
+template <int RT, int KT, int NW, int KCH>
 static size_t resident_slots() {
   static size_t slots = 0;
   if (!slots) {
     int dev = 0, cus = 0, occ = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, i8_gemm_kernel<RT, KT, NW>, NW * 64,
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, i8_gemm_kernel<RT, KT, NW, KCH>, NW * 64,
                                                        0);
     slots = (size_t)std::max(1, cus) * (size_t)std::max(1, occ);
   }
@@ -836,7 +844,7 @@ static size_t pick_splits(size_t rblk, size_t nchunks, size_t slots) {
   return best;
 }
 
-template <int RT, int KT, int NW>
+template <int RT, int KT, int NW, int KCH>
 static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const int8_t* bq,
                    size_t b_stride, size_t rows_pad, size_t red_len, DevBuf& part,
                    size_t* splits_out) {
@@ -845,12 +853,12 @@ static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const
   static_assert(kMarkerPad % ROWS == 0 && kSamplePad % ROWS == 0, "row tiles must divide padding");
   hipStream_t s = h->stream;
   const size_t rblk = rows_pad / ROWS;
-  const size_t nch = red_len / kI8KC;
-  const size_t splits = pick_splits(rblk, nch, resident_slots<RT, KT, NW>());
-  const size_t rps = ceil_div(nch, splits) * kI8KC;
+  const size_t nch = red_len / KCH;
+  const size_t splits = pick_splits(rblk, nch, resident_slots<RT, KT, NW, KCH>());
+  const size_t rps = ceil_div(nch, splits) * KCH;
   const size_t sp = ceil_div(red_len, rps);
   SAIGE_TRY(part.ensure(sp * rows_pad * KP * 4));
-  SAIGE_TRY(timed_launch(h->timer, i8_gemm_kernel<RT, KT, NW>, dim3((unsigned)rblk, (unsigned)sp),
+  SAIGE_TRY(timed_launch(h->timer, i8_gemm_kernel<RT, KT, NW, KCH>, dim3((unsigned)rblk, (unsigned)sp),
                          dim3(NW * 64), s, codes, code_stride, bq, b_stride, rps, red_len,
                          part.as<int>(), rows_pad));
   *splits_out = sp;
@@ -863,6 +871,9 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
   constexpr int KP = 16 * KT;
   constexpr int RT = 4;
   constexpr int NW = KT <= 2 ? 4 : 8;
+  // few columns: the pass is bound by the genotype stream, so each row's chunk is a whole
+  // 128-byte line; more columns: 64-byte chunks keep the prefetch registers in budget
+  constexpr int KCH = KT <= 2 ? 512 : 256;
   GrmDevice& g = h->g;
   MixedState* ms = mstate(h);
   hipStream_t s = h->stream;
@@ -896,7 +907,7 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
 
   // ---- stage 1: I = G^T Q_v (rows = markers, reduction over samples) ----
   size_t splits1 = 0;
-  SAIGE_TRY((i8_gemm<RT, KT, NW>(h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
+  SAIGE_TRY((i8_gemm<RT, KT, NW, KCH>(h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
                                   n_pad, ms->ipart1, &splits1)));
   i8_stage1_epilogue_kernel<KP><<<(unsigned)(m_pad / 64), 256, 0, s>>>(
       ms->ipart1.as<int>(), splits1, m_pad, m, k, psum, pmax, pdot, 1.0 / (double)n, nch_n,
@@ -913,7 +924,7 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
 
   // ---- stage 2: I = G Q_w (rows = samples, reduction over markers) ----
   size_t splits2 = 0;
-  SAIGE_TRY((i8_gemm<RT, KT, NW>(h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
+  SAIGE_TRY((i8_gemm<RT, KT, NW, KCH>(h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
                                   m_pad, n_pad, m_pad, ms->ipart2, &splits2)));
   i8_stage2_epilogue_kernel<KP><<<(unsigned)(n_pad / 64), 256, 0, s>>>(
       ms->ipart2.as<int>(), splits2, n_pad, n, k, psum, pmax, pdot, 1.0 / (double)m, nch_m,
