@@ -261,6 +261,9 @@ def main():
     ap.add_argument("--step2-batches", type=int, default=4)
     ap.add_argument("--cpu-markers", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int,
+                    default=max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "16")))),
+                    help="host threads of the CPU baseline (the box's CPU share)")
     ap.add_argument("--no-step2", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--traffic-bytes", type=float, default=None,
@@ -432,7 +435,8 @@ def main():
         log(f"step2: {nb * B} variants in {ms:.1f} ms -> {vps:.3g} variants/s, "
             f"SPA'd {r['is_spa'].mean() * 100:.2f}%")
 
-    # ---- CPU baseline: the oracle's f32 OnTheFlyGrm product (saige-rs restated), one thread ----
+    # ---- CPU baseline: the oracle's f32 OnTheFlyGrm product (saige-rs restated) on the
+    # host cores, split over the thread pool the way rayon splits it, and on one core ----
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -441,19 +445,28 @@ def main():
         dos = sg.decode(c.bed[sub], c.bpm, n, len(sub))
         g32 = oracle.Grm(dos, af[sub], oracle.GRM_F32)
         v = np.random.default_rng(1).standard_normal(n)
-        t = time.perf_counter()
-        reps = 0
-        while time.perf_counter() - t < 8.0 or reps < 1:
-            g32.mat_vec(v)
-            reps += 1
-        per_prod = (time.perf_counter() - t) / reps * (mg / mc)
-        cpu_fit = per_prod * ref["grm_products"]
-        cpu = {"value": cpu_fit, "unit": "s (fit-null, extrapolated)", "cores": 1, "kind": "port",
-               "sample": f"{reps} f32 OnTheFlyGrm products (saige-rs pcg.rs:128-180 restated) at "
-                         f"n={n} x {mc} markers, scaled by {mg}/{mc} markers and by the "
-                         f"{ref['grm_products']} column-products of the GPU fit-null; "
-                         f"one product = {per_prod:.2f} s"}
-        log(f"cpu baseline: {per_prod:.2f} s/product -> {cpu_fit:.0f} s fit-null (extrapolated)")
+
+        def per_product(threads, seconds):
+            t = time.perf_counter()
+            reps = 0
+            while time.perf_counter() - t < seconds or reps < 1:
+                g32.mat_vec_threads(v, threads)
+                reps += 1
+            return (time.perf_counter() - t) / reps * (mg / mc), reps
+
+        pt, reps_t = per_product(args.cpu_threads, 8.0)
+        p1, reps_1 = per_product(1, 4.0)
+        cpu_fit = pt * ref["grm_products"]
+        cpu = {"value": cpu_fit, "unit": "s (fit-null, extrapolated)", "cores": args.cpu_threads,
+               "kind": "port",
+               "sample": f"f32 OnTheFlyGrm products (saige-rs pcg.rs:128-180 restated, rayon-"
+                         f"shaped chunks) at n={n} x {mc} markers: {reps_t} products on "
+                         f"{args.cpu_threads} threads, {reps_1} on 1; scaled by {mg}/{mc} markers "
+                         f"and by the {ref['grm_products']} column-products of the GPU fit-null; "
+                         f"one full product = {pt:.3f} s on {args.cpu_threads} threads, "
+                         f"{p1:.2f} s on 1 thread (fit-null {p1 * ref['grm_products']:.0f} s)"}
+        log(f"cpu baseline: {pt:.3f} s/product on {args.cpu_threads} threads, {p1:.2f} s on 1 -> "
+            f"{cpu_fit:.0f} s fit-null (extrapolated)")
 
     line = {
         "metric": METRIC,

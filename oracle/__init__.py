@@ -114,6 +114,7 @@ def _declare(L):
     L.orc_grm_identity.restype = C.c_void_p
     L.orc_grm_free.argtypes = [C.c_void_p]
     L.orc_grm_mat_vec.argtypes = [C.c_void_p, _dp, _dp]
+    L.orc_grm_mat_vec_threads.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
     L.orc_pcg_solve_sigma.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp, C.c_size_t,
                                       C.c_double, C.c_int, _dp, _ip, _dp]
     L.orc_pcg_solve_dense.argtypes = [_dp, C.c_size_t, _dp, C.c_double, C.c_int, _dp, _ip, _dp]
@@ -249,6 +250,13 @@ class Grm:
         v = _f64(v)
         out = np.empty(self.n, np.float64)
         lib().orc_grm_mat_vec(self.h, _p(v, _dp), _p(out, _dp))
+        return out
+
+    def mat_vec_threads(self, v, threads):
+        """The f32 product split as saige-rs splits it over a rayon pool (timing)."""
+        v = _f64(v)
+        out = np.empty(self.n, np.float64)
+        lib().orc_grm_mat_vec_threads(self.h, _p(v, _dp), _p(out, _dp), int(threads))
         return out
 
     def __del__(self):

@@ -472,6 +472,36 @@ void* orc_grm_identity(size_t n) {
 void orc_grm_free(void* h) { delete (Grm*)h; }
 void orc_grm_mat_vec(void* h, const double* v, double* out) { grm_mat_vec((Grm*)h, v, out); }
 
+// The same f32 product split the way pcg.rs:139-177 splits it over a rayon pool of
+// `threads` workers: chunks of max(M / threads, 64) markers, each folded into its own f32
+// accumulator, the accumulators added in chunk order.  Used to time the CPU path at the
+// pool size; with threads > 1 the f32 sums are in a different order from threads = 1.
+void orc_grm_mat_vec_threads(void* h, const double* v, double* out, int threads) {
+  const Grm* g = (const Grm*)h;
+  size_t n = g->n;
+  if (g->mode != 0 || g->identity || g->m == 0 || threads <= 1) { grm_mat_vec(g, v, out); return; }
+  std::vector<float> vf(n);
+  for (size_t i = 0; i < n; ++i) vf[i] = (float)v[i];
+  const size_t per = std::max<size_t>(g->m / (size_t)threads, 64);
+  const size_t nchunks = (g->m + per - 1) / per;
+  std::vector<std::vector<float>> accs(nchunks, std::vector<float>(n, 0.0f));
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (long long c = 0; c < (long long)nchunks; ++c) {
+    std::vector<float>& acc = accs[(size_t)c];
+    const size_t j1 = std::min(g->m, ((size_t)c + 1) * per);
+    for (size_t j = (size_t)c * per; j < j1; ++j) {
+      const float* col = &g->z32[j * n];
+      float d = 0.0f;
+      for (size_t i = 0; i < n; ++i) d += col[i] * vf[i];
+      for (size_t i = 0; i < n; ++i) acc[i] += col[i] * d;
+    }
+  }
+  float scale = 1.0f / (float)g->m;
+  for (size_t c = 1; c < nchunks; ++c)
+    for (size_t i = 0; i < n; ++i) accs[0][i] += accs[c][i];
+  for (size_t i = 0; i < n; ++i) out[i] = (double)(accs[0][i] * scale);
+}
+
 // --- a7 PcgSolver::solve with the SAIGE Sigma closures (one RHS).
 int orc_pcg_solve_sigma(void* grm, const double* w, double tau0, double tau1, const double* b,
                         size_t n, double tol, int max_iter, double* x, int* iters, double* resid) {
