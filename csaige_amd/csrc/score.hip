@@ -29,6 +29,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <type_traits>
 
 namespace saige {
@@ -120,7 +122,12 @@ struct LoadU8 {
     return (double)q[v * ld + i] * scale;
   }
   __device__ __forceinline__ void load16(size_t v, size_t i, double* g) const {
-    const uint4 w = *reinterpret_cast<const uint4*>(q + v * ld + i);
+    expand16(raw16(v, i), g);
+  }
+  __device__ __forceinline__ uint4 raw16(size_t v, size_t i) const {
+    return *reinterpret_cast<const uint4*>(q + v * ld + i);
+  }
+  __device__ __forceinline__ void expand16(uint4 w, double* g) const {
     const uint32_t u[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
     for (int k = 0; k < 16; ++k) g[k] = (double)((u[k >> 2] >> (8 * (k & 3))) & 0xffu) * scale;
@@ -208,7 +215,23 @@ __global__ __launch_bounds__(64) void score_sums_kernel(L load, const double* __
 #pragma unroll
   for (int s = 0; s < NS; ++s) acc[s] = 0.0;
   size_t i = i0;
-  if (load.vec) {
+  if constexpr (std::is_same<L, LoadU8>::value) {
+    if (load.vec && i + 16 <= i1) {
+      // the uint8 stream: the next two 16-sample segments of the lane's row are in flight
+      // while the current one is accumulated
+      uint4 w0 = load.raw16(vv, i);
+      uint4 w1 = i + 32 <= i1 ? load.raw16(vv, i + 16) : w0;
+      for (; i + 16 <= i1; i += 16) {
+        const uint4 cur = w0;
+        w0 = w1;
+        if (i + 48 <= i1) w1 = load.raw16(vv, i + 32);
+        double g[16];
+        load.expand16(cur, g);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) accumulate<P, MISS>(g[k], feat + (i + k) * NF, acc);
+      }
+    }
+  } else if (load.vec) {
     for (; i + 16 <= i1; i += 16) {
       double g[16];
       load.load16(vv, i, g);
@@ -524,10 +547,13 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
       const double m = mu[i];
       const double d0 = (1.0 - m) * e0 + m;
       const double d1 = (1.0 - m) * e1 + m;
-      b[0] += m * gi / d0;
-      b[1] += (1.0 - m) * m * gi * gi * e0 / (d0 * d0);
-      b[2] += m * gi / d1;
-      b[3] += (1.0 - m) * m * gi * gi * e1 / (d1 * d1);
+      // one reciprocal per root: K' term m g / d, K'' term (1 - m) m g^2 e / d^2
+      const double r0 = 1.0 / d0, r1 = 1.0 / d1;
+      const double mg = m * gi, w2 = (1.0 - m) * mg * gi;
+      b[0] += mg * r0;
+      b[1] += w2 * e0 * r0 * r0;
+      b[2] += mg * r1;
+      b[3] += w2 * e1 * r1 * r1;
     }
     block_sum4(b, sh);
     for (int r = 0; r < 2; ++r) {
@@ -618,6 +644,260 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
   }
 }
 
+
+// ---------------------------------------------------------------------------------
+// Batched SPA: the passes of spa_kernel with each gated variant's samples split over S
+// work-groups.  A pass kernel writes per-slice partial sums; a step kernel (one thread per
+// variant) adds them in slice order and advances both root searches with the same state
+// machine (root_after_eval, restart from 0 after a failed proposal).  A batch's few
+// hundred gated variants then fill the GPU instead of occupying one work-group each.
+// ---------------------------------------------------------------------------------
+struct SpaVar {
+  RootState st[2];
+  double gpos, gneg, na_mu, na_sigma;
+  double tinit[2];
+  int done;  // both searches finished
+  int pad;
+};
+constexpr int kSpaPassBlock = 256;
+constexpr int kSpaMaxPasses = 8 * 1000 + 16;  // the guard of spa_kernel's pass loop
+
+template <int NT>
+__device__ __forceinline__ void block_sum4t(double* v, double* sh) {
+  for (int k = 0; k < 4; ++k) sh[k * NT + threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 4; ++k) sh[k * NT + threadIdx.x] += sh[k * NT + threadIdx.x + w];
+    __syncthreads();
+  }
+  for (int k = 0; k < 4; ++k) v[k] = sh[k * NT];
+  __syncthreads();
+}
+
+// KIND 0: g_pos, g_neg and the NA moments; 1: K'(t) and K''(t) sums at both pending points;
+// 2: K(zeta) and K''(zeta) sums at both roots (variants whose two searches converged).
+template <class G, bool TAB, int KIND>
+__global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
+    G gt, const double* __restrict__ mu, size_t n, const GateInfo* __restrict__ gate,
+    const uint32_t* __restrict__ n_gate, int fast, size_t slice,
+    const SpaVar* __restrict__ vars, double* __restrict__ part) {
+  const uint32_t slot = blockIdx.x;  // grid: (gated variants, slices)
+  if (slot >= *n_gate) return;
+  if (KIND == 1 && vars[slot].done) return;
+  if (KIND == 2 && !(vars[slot].st[0].converged && vars[slot].st[1].converged)) return;
+  __shared__ double sh[4 * kSpaPassBlock];
+  __shared__ GateInfo g;
+  __shared__ double tg[TAB ? 256 : 1], te0[TAB ? 256 : 1], te1[TAB ? 256 : 1];
+  if (threadIdx.x == 0) g = gate[slot];
+  __syncthreads();
+  const size_t v = g.var;
+  const double thr = fast ? 1e-10 : -1.0;  // fast: |g~| > 1e-10 enters K (fast.rs:191-200)
+  if constexpr (TAB) {
+    for (int b = threadIdx.x; b < 256; b += kSpaPassBlock) tg[b] = (double)b * gt.load.scale - g.ag[0];
+  }
+  double t0 = 0.0, t1 = 0.0;
+  if (KIND == 1) {
+    t0 = vars[slot].st[0].pend;
+    t1 = vars[slot].st[1].pend;
+  } else if (KIND == 2) {
+    t0 = vars[slot].st[0].root;
+    t1 = vars[slot].st[1].root;
+  }
+  if constexpr (TAB) {
+    if (KIND != 0) {
+      __syncthreads();
+      for (int b = threadIdx.x; b < 256; b += kSpaPassBlock) {
+        te0[b] = exp(-tg[b] * t0);
+        te1[b] = exp(-tg[b] * t1);
+      }
+    }
+    __syncthreads();
+  }
+  const size_t i0 = (size_t)blockIdx.y * slice, i1 = min(i0 + slice, n);
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kSpaPassBlock) {
+    double gi, e0 = 1.0, e1 = 1.0;
+    if constexpr (TAB) {
+      const uint32_t by = gt.byte(v, i);
+      gi = tg[by];
+      if (KIND != 0) {
+        e0 = te0[by];
+        e1 = te1[by];
+      }
+    } else {
+      gi = gt(v, i, g.ag);
+      if (KIND != 0) {
+        e0 = exp(-gi * t0);
+        e1 = exp(-gi * t1);
+      }
+    }
+    const double m = mu[i];
+    if (KIND == 0) {
+      if (gi > 0.0) a[0] += gi;
+      if (gi < 0.0) a[1] += gi;
+      if (!(fabs(gi) > thr)) {
+        a[2] += m * gi;
+        a[3] += m * (1.0 - m) * gi * gi;
+      }
+      continue;
+    }
+    if (!(fabs(gi) > thr)) continue;
+    const double d0 = (1.0 - m) * e0 + m;
+    const double d1 = (1.0 - m) * e1 + m;
+    const double r0 = 1.0 / d0, r1 = 1.0 / d1;
+    if (KIND == 1) {
+      const double mg = m * gi, w2 = (1.0 - m) * mg * gi;
+      a[0] += mg * r0;
+      a[1] += w2 * e0 * r0 * r0;
+      a[2] += mg * r1;
+      a[3] += w2 * e1 * r1 * r1;
+    } else {
+      // K at the roots is ln(1 - m + m e^{g zeta}) (binary.rs:148-151), K'' as above
+      const double w2 = (1.0 - m) * m * gi * gi;
+      a[0] += log(1.0 - m + m * exp(gi * t0));
+      a[1] += w2 * e0 * r0 * r0;
+      a[2] += log(1.0 - m + m * exp(gi * t1));
+      a[3] += w2 * e1 * r1 * r1;
+    }
+  }
+  block_sum4t<kSpaPassBlock>(a, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) part[((size_t)slot * gridDim.y + blockIdx.y) * 4 + k] = a[k];
+}
+
+__device__ __forceinline__ double slice_sum(const double* __restrict__ part, uint32_t slot, int S,
+                                            int k) {
+  double s = 0.0;
+  for (int q = 0; q < S; ++q) s += part[((size_t)slot * S + q) * 4 + k];
+  return s;
+}
+
+// learned starting points of both searches (one 128-thread group per gated variant)
+__global__ __launch_bounds__(128) void spa_propose_kernel(const GateInfo* __restrict__ gate,
+                                                          const uint32_t* __restrict__ n_gate,
+                                                          MlpView mv, SpaVar* __restrict__ vars) {
+  const uint32_t slot = blockIdx.x;
+  if (slot >= *n_gate) return;
+  __shared__ GateInfo g;
+  __shared__ float sh[2 * kMlpH + 2];
+  if (threadIdx.x == 0) g = gate[slot];
+  __syncthreads();
+  double tinit[2];
+  mlp_propose2(mv, g, sh, tinit);
+  if (threadIdx.x == 0) {
+    vars[slot].tinit[0] = tinit[0];
+    vars[slot].tinit[1] = tinit[1];
+  }
+}
+
+__global__ void spa_setup_step(const GateInfo* __restrict__ gate, const uint32_t* __restrict__ n_gate,
+                               int S, const double* __restrict__ part, int use_tinit,
+                               SpaVar* __restrict__ vars) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= *n_gate) return;
+  SpaVar V = vars[slot];
+  if (!use_tinit) V.tinit[0] = V.tinit[1] = 0.0;  // fast.rs:217-218
+  V.gpos = slice_sum(part, slot, S, 0);
+  V.gneg = slice_sum(part, slot, S, 1);
+  V.na_mu = slice_sum(part, slot, S, 2);
+  V.na_sigma = slice_sum(part, slot, S, 3);
+  const double q[2] = {gate[slot].q, gate[slot].qinv};
+  for (int r = 0; r < 2; ++r) {
+    root_start(V.st[r], V.tinit[r]);
+    V.st[r].spent = 0;
+    if (q[r] >= V.gpos || q[r] <= V.gneg) {  // fast.rs:67-73: root at infinity
+      V.st[r].root = __longlong_as_double(0x7ff0000000000000ll);
+      V.st[r].converged = 1;
+      V.st[r].phase = 3;
+    }
+  }
+  V.done = (V.st[0].phase == 3 && V.st[1].phase == 3) ? 1 : 0;
+  vars[slot] = V;
+}
+
+// active[pass & 1] counts the variants still searching after this pass; the other counter
+// is cleared for the next pass.
+__global__ void spa_eval_step(const GateInfo* __restrict__ gate, const uint32_t* __restrict__ n_gate,
+                              int S, const double* __restrict__ part, double tol, int pass,
+                              SpaVar* __restrict__ vars, uint32_t* __restrict__ active) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot == 0) active[(pass + 1) & 1] = 0u;
+  if (slot >= *n_gate) return;
+  SpaVar V = vars[slot];
+  if (V.done) return;
+  const double q[2] = {gate[slot].q, gate[slot].qinv};
+  const int max_iter = 1000;
+  for (int r = 0; r < 2; ++r) {
+    if (V.st[r].phase == 3) continue;
+    const double tt = V.st[r].pend;
+    const double k1 = slice_sum(part, slot, S, 2 * r) + V.na_mu + V.na_sigma * tt - q[r];
+    const double k2 = slice_sum(part, slot, S, 2 * r + 1) + V.na_sigma;
+    root_after_eval(V.st[r], k1, k2, tol, max_iter);
+    if (V.st[r].phase == 3 && !V.st[r].converged && V.tinit[r] != 0.0 && V.st[r].spent == 0) {
+      V.st[r].spent = V.st[r].it;
+      root_start(V.st[r], 0.0);
+    }
+  }
+  V.done = (V.st[0].phase == 3 && V.st[1].phase == 3) ? 1 : 0;
+  vars[slot] = V;
+  if (!V.done) atomicAdd(&active[pass & 1], 1u);
+}
+
+// saddle-point probabilities (fast.rs:135-180 and :220-241) and the outputs
+__global__ void spa_final_step(const GateInfo* __restrict__ gate, const uint32_t* __restrict__ n_gate,
+                               int S, const double* __restrict__ part,
+                               const SpaVar* __restrict__ vars,
+                               saige_variant_result* __restrict__ out) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= *n_gate) return;
+  const SpaVar V = vars[slot];
+  const GateInfo g = gate[slot];
+  const double q[2] = {g.q, g.qinv};
+  const bool both = V.st[0].converged && V.st[1].converged;
+  double p[2] = {0.0, 0.0};
+  bool ok[2] = {false, false};
+  if (both) {
+    for (int r = 0; r < 2; ++r) {
+      const double zeta = V.st[r].root;
+      const double k0 = slice_sum(part, slot, S, 2 * r) + V.na_mu * zeta + 0.5 * V.na_sigma * zeta * zeta;
+      const double k2 = slice_sum(part, slot, S, 2 * r + 1) + V.na_sigma;
+      const double temp1 = zeta * q[r] - k0;
+      if (isfinite(k0) && isfinite(k2) && !(temp1 < 0.0) && !(k2 < 0.0)) {
+        const double sg = zeta > 0.0 ? 1.0 : (zeta < 0.0 ? -1.0 : (signbit(zeta) ? -1.0 : 1.0));
+        const double w = (isnan(zeta) ? zeta : sg) * sqrt(2.0 * temp1);
+        const double vv = zeta * sqrt(k2);
+        if (w != 0.0) {
+          const double z = w + (1.0 / w) * log(vv / w);
+          p[r] = z > 0.0 ? 1.0 - norm_cdf(z) : -norm_cdf(z);
+          ok[r] = true;
+        }
+      }
+    }
+  }
+  double pval;
+  int conv, is_spa;
+  if (both) {
+    const double a1 = ok[0] ? p[0] : g.pn / 2.0;
+    const double a2 = ok[1] ? p[1] : g.pn / 2.0;
+    pval = fabs(a1) + fabs(a2);
+    conv = 1;
+    is_spa = (ok[0] || ok[1]) ? 1 : 0;
+  } else {
+    pval = g.pn;
+    conv = 0;
+    is_spa = 0;
+  }
+  saige_variant_result& o = out[g.var];
+  o.pvalue = pval;
+  o.is_spa = is_spa;
+  o.spa_converged = conv;
+  o.spa_iters1 = V.st[0].spent + V.st[0].it;
+  o.spa_iters2 = V.st[1].spent + V.st[1].it;
+  o.spa_root1 = V.st[0].root;
+  o.spa_root2 = V.st[1].root;
+}
+
 }  // namespace saige
 
 using namespace saige;
@@ -631,6 +911,9 @@ struct saige_engine {
   bool intercept_only = false;
   const saige_mlp* mlp = nullptr;
   DevBuf feat, mu_d, x_d, idx, part, res, gate, ngate, in;
+  DevBuf spa_vars, spa_part, spa_active;  // batched SPA state
+  // split counts of the sums kernel chosen by timing, per (loader, covariates, variant blocks)
+  std::map<std::tuple<int, int, size_t>, size_t> tuned_splits;
   MlpView mlp_view() const {
     MlpView v{};
     if (mlp) {
@@ -643,43 +926,166 @@ struct saige_engine {
 
 namespace saige {
 
+__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
+
+// Sum the per-split partials of the sums kernel in split order (one thread per variant and
+// sum), so the finalize kernel reads one row of sums per variant.
+__global__ void sums_reduce_kernel(const double* __restrict__ part, size_t splits, int ns,
+                                   size_t nvar_pad, double* __restrict__ out) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (v >= nvar_pad) return;
+  double a = 0.0;
+  for (size_t q = 0; q < splits; ++q) a += part[(q * ns + k) * nvar_pad + v];
+  out[(size_t)k * nvar_pad + v] = a;
+}
+
+template <class L> constexpr int loader_id() {
+  return std::is_same<L, LoadU8>::value ? 1 : (std::is_same<L, LoadF64>::value ? 2 : 3);
+}
+
+// Tile-shape autotuner of the sums kernel: the first batch of a shape times the kernel and
+// the reduction of its partials at 16 to 256 sample splits (every candidate computes the
+// complete, correct sums; the run that follows recomputes them with the choice) and the
+// fastest split count serves the later batches of that shape.  Shapes with fewer than 64 * 1024 samples use the default split count.
+template <int P, class L>
+static int sums_splits(saige_engine* e, L load, size_t nvar, size_t nvar_pad, size_t* out) {
+  constexpr int NS = Layout<P>::NS;
+  const size_t n = e->n;
+  const size_t vblocks = ceil_div(nvar, 64);
+  auto norm = [&](size_t sp) {
+    const size_t sps = round_up(ceil_div(n, std::max<size_t>(sp, 1)), 16);
+    return ceil_div(n, sps);
+  };
+  const size_t dflt = norm(std::max<size_t>(1, std::min<size_t>(ceil_div(2048, vblocks),
+                                                                 ceil_div(n, 512))));
+  const auto key = std::make_tuple(loader_id<L>(), P, vblocks);
+  auto it = e->tuned_splits.find(key);
+  if (it != e->tuned_splits.end()) {
+    *out = it->second;
+    return SAIGE_OK;
+  }
+  if (n < 65536) {
+    e->tuned_splits[key] = dflt;
+    *out = dflt;
+    return SAIGE_OK;
+  }
+  hipStream_t s = e->stream;
+  size_t best = dflt;
+  float best_ms = 1e30f;
+  hipEvent_t a, b;
+  SAIGE_HIP(hipEventCreate(&a));
+  SAIGE_HIP(hipEventCreate(&b));
+  for (size_t cand : {size_t(16), size_t(32), size_t(64), size_t(128), size_t(256)}) {
+    const size_t sp = norm(cand);
+    const size_t sps = round_up(ceil_div(n, sp), 16);
+    SAIGE_TRY(e->part.ensure((sp + 1) * NS * nvar_pad * 8));
+    SAIGE_HIP(hipEventRecord(a, s));
+    score_sums_kernel<P, L><<<dim3((unsigned)vblocks, (unsigned)sp), 64, 0, s>>>(
+        load, e->feat.as<double>(), n, nvar, sps, e->part.as<double>(), nvar_pad);
+    sums_reduce_kernel<<<dim3((unsigned)ceil_div(nvar_pad, 256), (unsigned)NS), 256, 0, s>>>(
+        e->part.as<double>(), sp, NS, nvar_pad, e->part.as<double>() + sp * NS * nvar_pad);
+    SAIGE_HIP(hipEventRecord(b, s));
+    SAIGE_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    SAIGE_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = sp;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  e->tuned_splits[key] = best;
+  *out = best;
+  return SAIGE_OK;
+}
+
+// SPA of a batch's gated variants with S work-groups per variant and pass (about 1,024
+// work-groups per pass, slices of at least 2,048 samples); the host reads the number of
+// variants still searching after every fourth pass.
+template <class G, bool TAB>
+static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out) {
+  hipStream_t s = e->stream;
+  const size_t n = e->n;
+  uint32_t ng = 0;
+  SAIGE_HIP(hipMemcpyAsync(&ng, e->ngate.p, 4, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  if (ng == 0) return SAIGE_OK;
+  size_t S = std::max<size_t>(1, std::min<size_t>(ceil_div(1024, ng), ceil_div(n, 2048)));
+  S = std::min<size_t>(S, 64);
+  const size_t slice = round_up(ceil_div(n, S), 256);
+  S = ceil_div(n, slice);
+  SAIGE_TRY(e->spa_vars.ensure(ng * sizeof(SpaVar)));
+  SAIGE_TRY(e->spa_part.ensure(ng * S * 4 * 8));
+  SAIGE_TRY(e->spa_active.ensure(64));
+  uint32_t* active = e->spa_active.as<uint32_t>();
+  SpaVar* vars = e->spa_vars.as<SpaVar>();
+  double* part = e->spa_part.as<double>();
+  const GateInfo* gate = e->gate.as<GateInfo>();
+  const uint32_t* ngp = e->ngate.as<uint32_t>();
+  const double* mu = e->mu_d.as<double>();
+  const MlpView mv = e->mlp_view();
+  const dim3 grid(ng, (unsigned)S);
+  const unsigned sb = (unsigned)ceil_div(ng, 256);
+  const int fast = e->ec.fast, iS = (int)S;
+  zero_u32_kernel<<<1, 1, 0, s>>>(active);
+  zero_u32_kernel<<<1, 1, 0, s>>>(active + 1);
+  if (mv.enabled) spa_propose_kernel<<<ng, 128, 0, s>>>(gate, ngp, mv, vars);
+  spa_pass_kernel<G, TAB, 0><<<grid, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
+                                                             vars, part);
+  spa_setup_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, mv.enabled, vars);
+  for (int pass = 0; pass < kSpaMaxPasses; ++pass) {
+    spa_pass_kernel<G, TAB, 1><<<grid, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
+                                                               vars, part);
+    spa_eval_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, e->ec.spa_tol, pass, vars, active);
+    if (pass % 4 == 3) {
+      uint32_t cnt = 0;
+      SAIGE_HIP(hipMemcpyAsync(&cnt, active + (pass & 1), 4, hipMemcpyDeviceToHost, s));
+      SAIGE_HIP(hipStreamSynchronize(s));
+      if (cnt == 0) break;
+    }
+  }
+  spa_pass_kernel<G, TAB, 2><<<grid, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
+                                                             vars, part);
+  spa_final_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, vars, d_out);
+  SAIGE_HIP(hipGetLastError());
+  return SAIGE_OK;
+}
+
 template <int P, class L>
 static int run_batch_p(saige_engine* e, L load, size_t nvar, saige_variant_result* d_out) {
   hipStream_t s = e->stream;
   const size_t n = e->n;
   constexpr int NS = Layout<P>::NS;
   const size_t vblocks = ceil_div(nvar, 64);
-  size_t splits = std::max<size_t>(1, std::min<size_t>(ceil_div(2048, vblocks), ceil_div(n, 512)));
-  size_t sps = round_up(ceil_div(n, splits), 16);
-  splits = ceil_div(n, sps);
   const size_t nvar_pad = round_up(nvar, 64);
-  SAIGE_TRY(e->part.ensure(splits * NS * nvar_pad * 8));
+  size_t splits = 1;
+  SAIGE_TRY((sums_splits<P, L>(e, load, nvar, nvar_pad, &splits)));
+  const size_t sps = round_up(ceil_div(n, splits), 16);
+  SAIGE_TRY(e->part.ensure((splits + 1) * NS * nvar_pad * 8));
   SAIGE_TRY(e->gate.ensure(nvar * sizeof(GateInfo)));
   SAIGE_TRY(e->ngate.ensure(64));
-  SAIGE_HIP(hipMemsetAsync(e->ngate.p, 0, 4, s));
+  zero_u32_kernel<<<1, 1, 0, s>>>(e->ngate.as<uint32_t>());  // a kernel, not a memset node
   score_sums_kernel<P, L><<<dim3((unsigned)vblocks, (unsigned)splits), 64, 0, s>>>(
       load, e->feat.as<double>(), n, nvar, sps, e->part.as<double>(), nvar_pad);
+  double* sums = e->part.as<double>() + splits * NS * nvar_pad;
+  sums_reduce_kernel<<<dim3((unsigned)ceil_div(nvar_pad, 256), (unsigned)NS), 256, 0, s>>>(
+      e->part.as<double>(), splits, NS, nvar_pad, sums);
   score_finalize_kernel<P><<<(unsigned)ceil_div(nvar, 128), 128, 0, s>>>(
-      e->part.as<double>(), splits, nvar, nvar_pad, n, e->ec, L::kMayMiss ? 1 : 0, d_out,
+      sums, 1, nvar, nvar_pad, n, e->ec, L::kMayMiss ? 1 : 0, d_out,
       e->gate.as<GateInfo>(), e->ngate.as<uint32_t>());
   SAIGE_HIP(hipGetLastError());
   if (e->ec.use_spa) {
     bool tab = false;
     if constexpr (std::is_same<L, LoadU8>::value) tab = (P == 1 && e->intercept_only);
     if (tab) {
-      if constexpr (std::is_same<L, LoadU8>::value) {
-        GTildeU8Tab gt{load};
-        spa_kernel<GTildeU8Tab, true><<<(unsigned)nvar, kSpaBlock, 0, s>>>(
-            gt, e->mu_d.as<double>(), n, e->gate.as<GateInfo>(), e->ngate.as<uint32_t>(),
-            e->ec.spa_tol, e->ec.fast, e->mlp_view(), d_out, nullptr);
-      }
+      if constexpr (std::is_same<L, LoadU8>::value)
+        SAIGE_TRY((spa_batched<GTildeU8Tab, true>(e, GTildeU8Tab{load}, d_out)));
     } else {
-      GTilde<L, P> gt{load, e->x_d.as<double>(), n};
-      spa_kernel<GTilde<L, P>, false><<<(unsigned)nvar, kSpaBlock, 0, s>>>(
-          gt, e->mu_d.as<double>(), n, e->gate.as<GateInfo>(), e->ngate.as<uint32_t>(),
-          e->ec.spa_tol, e->ec.fast, e->mlp_view(), d_out, nullptr);
+      SAIGE_TRY((spa_batched<GTilde<L, P>, false>(e, GTilde<L, P>{load, e->x_d.as<double>(), n},
+                                                   d_out)));
     }
-    SAIGE_HIP(hipGetLastError());
   }
   return SAIGE_OK;
 }
