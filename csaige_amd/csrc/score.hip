@@ -112,6 +112,9 @@ __device__ void mlp_propose2(const MlpView& mv, const GateInfo& g, float* sh, do
 __device__ __forceinline__ double dnan() { return __longlong_as_double(0x7ff8000000000000ll); }
 
 // ---- dosage loaders: value of sample i of variant v (NaN = missing) ----
+typedef uint32_t u32x4w __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
 struct LoadU8 {
   static constexpr bool kMayMiss = false;
   const uint8_t* q;
@@ -126,6 +129,9 @@ struct LoadU8 {
   }
   __device__ __forceinline__ uint4 raw16(size_t v, size_t i) const {
     return *reinterpret_cast<const uint4*>(q + v * ld + i);
+  }
+  __device__ __forceinline__ u32x4w raw16w(size_t v, size_t i) const {
+    return *reinterpret_cast<const u32x4w*>(q + v * ld + i);
   }
   __device__ __forceinline__ void expand16(uint4 w, double* g) const {
     const uint32_t u[4] = {w.x, w.y, w.z, w.w};
@@ -166,10 +172,10 @@ struct LoadPacked {  // plink.rs:170-194, 237-247
 
 // feature row layout
 template <int P> struct Layout {
-  static constexpr int NF = 2 * P + 4;  // A(p), res, mu2, mu2*x(p), mu, case
-  // sums: g, n, gA(p), g res, g^2 mu2, g mu2 x(p), g mu, g case, n case, g ctrl, and over
-  // carriers (g > 0): mu, mu2, mu * mu2 (the moments the learned root proposal reads)
-  static constexpr int NS = 2 * P + 12;
+  static constexpr int NF = 2 * P + 3;  // A(p), res, mu2, mu2*x(p), case
+  // sums: g, n, gA(p), g res, g^2 mu2, g mu2 x(p), g case, n case.  The moments that only
+  // SPA'd variants need (sum of mu g~, carrier moments) come from the SPA setup pass.
+  static constexpr int NS = 2 * P + 6;
 };
 
 template <int P, bool MISS>
@@ -185,16 +191,247 @@ __device__ __forceinline__ void accumulate(double g, const double* __restrict__ 
   acc[3 + P] = fma(gz * gz, f[P + 1], acc[3 + P]);
 #pragma unroll
   for (int j = 0; j < P; ++j) acc[4 + P + j] = fma(gz, f[P + 2 + j], acc[4 + P + j]);
-  acc[4 + 2 * P] = fma(gz, f[2 * P + 2], acc[4 + 2 * P]);
-  const double cs = f[2 * P + 3];
-  acc[5 + 2 * P] = fma(gz, cs, acc[5 + 2 * P]);
-  if (MISS) acc[6 + 2 * P] += ok ? cs : 0.0;
-  acc[7 + 2 * P] = fma(gz, 1.0 - cs, acc[7 + 2 * P]);
-  if (gz > 0.0) {
-    const double m = f[2 * P + 2], m2 = f[P + 1];
-    acc[8 + 2 * P] += m;
-    acc[9 + 2 * P] += m2;
-    acc[10 + 2 * P] = fma(m, m2, acc[10 + 2 * P]);
+  const double cs = f[2 * P + 2];
+  acc[4 + 2 * P] = fma(gz, cs, acc[4 + 2 * P]);
+  if (MISS) acc[5 + 2 * P] += ok ? cs : 0.0;
+}
+
+// Dosage bytes q of one 16-sample segment against the feature rows: the products use the
+// integer-valued q (the dosage scale multiplies the sums once at the end), sum q and the
+// case-sample sum of q are exact uint32 dot products with the byte masks.
+template <int P>
+__device__ __forceinline__ void accumulate_u8(u32x4w w, const double* __restrict__ f,
+                                              const uint32_t* __restrict__ cmask, double* acc,
+                                              uint32_t& sq, uint32_t& sc) {
+  constexpr int NF = Layout<P>::NF;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    sq = __builtin_amdgcn_udot4(w[c], 0x01010101u, sq, false);
+    sc = __builtin_amdgcn_udot4(w[c], cmask[c], sc, false);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const double q = (double)((w[k >> 2] >> (8 * (k & 3))) & 0xffu);
+    const double* fk = f + k * NF;
+#pragma unroll
+    for (int r = 0; r < P; ++r) acc[2 + r] = fma(q, fk[r], acc[2 + r]);
+    acc[2 + P] = fma(q, fk[P], acc[2 + P]);
+    acc[3 + P] = fma(q * q, fk[P + 1], acc[3 + P]);
+#pragma unroll
+    for (int j = 0; j < P; ++j) acc[4 + P + j] = fma(q, fk[P + 2 + j], acc[4 + P + j]);
+  }
+}
+
+// uint8 dosages, VPL variants per lane (64 * VPL per work-group), one sample range per
+// grid row; the next two 16-sample segments of each row are in flight while the current
+// one is accumulated.  The feature rows and case masks are wave-uniform (scalar loads).
+template <int P, int VPL>
+__global__ __launch_bounds__(64) void score_sums_u8_kernel(LoadU8 load,
+                                                           const double* __restrict__ feat,
+                                                           const uint32_t* __restrict__ cmask,
+                                                           size_t n, size_t nvar, size_t sps,
+                                                           double* __restrict__ part,
+                                                           size_t nvar_pad) {
+  constexpr int NF = Layout<P>::NF, NS = Layout<P>::NS;
+  size_t vv[VPL];
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) {
+    const size_t v = (size_t)blockIdx.x * 64 * VPL + u * 64 + threadIdx.x;
+    vv[u] = v < nvar ? v : nvar - 1;
+  }
+  const size_t i0 = (size_t)blockIdx.y * sps;
+  const size_t i1 = min(i0 + sps, n);
+  double acc[VPL][NS];
+  uint32_t sq[VPL], sc[VPL];
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) {
+    sq[u] = sc[u] = 0u;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[u][s] = 0.0;
+  }
+  size_t i = i0;
+  if (i + 16 <= i1) {
+    u32x4w w0[VPL], w1[VPL];
+#pragma unroll
+    for (int u = 0; u < VPL; ++u) {
+      w0[u] = load.raw16w(vv[u], i);
+      w1[u] = load.raw16w(vv[u], i + 32 <= i1 ? i + 16 : i);
+    }
+    for (; i + 16 <= i1; i += 16) {
+      u32x4w cur[VPL];
+      const size_t nx = i + 48 <= i1 ? i + 32 : i;  // in bounds; unused at the tail
+#pragma unroll
+      for (int u = 0; u < VPL; ++u) {
+        cur[u] = w0[u];
+        w0[u] = w1[u];
+        w1[u] = load.raw16w(vv[u], nx);
+      }
+#pragma unroll
+      for (int u = 0; u < VPL; ++u)
+        accumulate_u8<P>(cur[u], feat + i * NF, cmask + i / 4, acc[u], sq[u], sc[u]);
+    }
+  }
+  for (; i < i1; ++i) {
+    const uint32_t cs = (cmask[i >> 2] >> (8 * (i & 3))) & 0xffu;
+#pragma unroll
+    for (int u = 0; u < VPL; ++u) {
+      const uint32_t qb = load.q[vv[u] * load.ld + i];
+      const double q = (double)qb;
+      const double* f = feat + i * NF;
+      sq[u] += qb;
+      sc[u] += qb * cs;
+#pragma unroll
+      for (int r = 0; r < P; ++r) acc[u][2 + r] = fma(q, f[r], acc[u][2 + r]);
+      acc[u][2 + P] = fma(q, f[P], acc[u][2 + P]);
+      acc[u][3 + P] = fma(q * q, f[P + 1], acc[u][3 + P]);
+#pragma unroll
+      for (int j = 0; j < P; ++j) acc[u][4 + P + j] = fma(q, f[P + 2 + j], acc[u][4 + P + j]);
+    }
+  }
+  const double sc1 = load.scale, sc2 = load.scale * load.scale;
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) {
+    const size_t v = (size_t)blockIdx.x * 64 * VPL + u * 64 + threadIdx.x;
+    if (v >= nvar) continue;
+    acc[u][0] = (double)sq[u] * sc1;
+    acc[u][4 + 2 * P] = (double)sc[u] * sc1;
+#pragma unroll
+    for (int s = 2; s < 3 + P; ++s) acc[u][s] *= sc1;
+    acc[u][3 + P] *= sc2;
+#pragma unroll
+    for (int j = 0; j < P; ++j) acc[u][4 + P + j] *= sc1;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) part[((size_t)blockIdx.y * NS + s) * nvar_pad + v] = acc[u][s];
+  }
+}
+
+// uint8 dosages with lanes over samples: a wave sums VB variants over its work-group's
+// sample range in blocks of 256 samples, four consecutive samples per lane (one 32-bit
+// dosage load per variant row, 256 contiguous bytes per wave).  The features of a block are
+// staged once per work-group in LDS as [feature][sample] (each thread loads one sample's
+// row, the next block's row in flight), so the four waves (4 VB variants) share them; every
+// lane keeps per-variant partial sums that are added over the lanes at the end.
+template <int P> struct SumsT {
+  static constexpr int NFU = 2 * P + 2;             // A(p), res, mu2, mu2*x(p)
+  static constexpr int VB = P == 1 ? 8 : 4;         // variants per wave
+};
+
+template <int P>
+__global__ __launch_bounds__(256) void score_sums_u8t_kernel(LoadU8 load,
+                                                             const double* __restrict__ feat,
+                                                             const uint32_t* __restrict__ cmask,
+                                                             size_t n, size_t nvar, size_t sps,
+                                                             double* __restrict__ part,
+                                                             size_t nvar_pad) {
+  constexpr int NF = Layout<P>::NF, NS = Layout<P>::NS;
+  constexpr int NFU = SumsT<P>::NFU, VB = SumsT<P>::VB;
+  __shared__ __attribute__((aligned(16))) double fl[2][NFU][256];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const size_t vg = ((size_t)blockIdx.x * 4 + wave) * VB;
+  const size_t i0 = (size_t)blockIdx.y * sps, i1 = min(i0 + sps, n);
+  size_t rows[VB];
+#pragma unroll
+  for (int u = 0; u < VB; ++u) rows[u] = (vg + u < nvar ? vg + u : nvar - 1) * load.ld;
+  double acc[VB][NFU];
+  uint32_t sq[VB], sc[VB];
+#pragma unroll
+  for (int u = 0; u < VB; ++u) {
+    sq[u] = sc[u] = 0u;
+#pragma unroll
+    for (int s = 0; s < NFU; ++s) acc[u][s] = 0.0;
+  }
+  double pf[NFU];
+  uint32_t wn[VB], cmn = 0u;
+  auto fetch = [&](size_t blk) {
+#pragma unroll
+    for (int j = 0; j < NFU; ++j) pf[j] = feat[(blk + t) * NF + j];
+    const size_t s0 = blk + 4 * lane;
+    cmn = cmask[s0 / 4];
+#pragma unroll
+    for (int u = 0; u < VB; ++u) wn[u] = *reinterpret_cast<const uint32_t*>(load.q + rows[u] + s0);
+  };
+  auto add = [&](int u, double q, const double* f) {
+#pragma unroll
+    for (int r = 0; r < P; ++r) acc[u][r] = fma(q, f[r], acc[u][r]);
+    acc[u][P] = fma(q, f[P], acc[u][P]);
+    acc[u][P + 1] = fma(q * q, f[P + 1], acc[u][P + 1]);
+#pragma unroll
+    for (int j = 0; j < P; ++j) acc[u][P + 2 + j] = fma(q, f[P + 2 + j], acc[u][P + 2 + j]);
+  };
+  const size_t iend = i0 + ((i1 - i0) & ~size_t(255));
+  int buf = 0;
+  if (i0 < iend) fetch(i0);
+  for (size_t blk = i0; blk < iend; blk += 256, buf ^= 1) {
+#pragma unroll
+    for (int j = 0; j < NFU; ++j) fl[buf][j][t] = pf[j];
+    uint32_t w[VB];
+#pragma unroll
+    for (int u = 0; u < VB; ++u) w[u] = wn[u];
+    const uint32_t cm = cmn;
+    __syncthreads();
+    if (blk + 256 < iend) fetch(blk + 256);
+    double f[4][NFU];
+#pragma unroll
+    for (int j = 0; j < NFU; ++j) {
+      const f64x2 a = *reinterpret_cast<const f64x2*>(&fl[buf][j][4 * lane]);
+      const f64x2 b = *reinterpret_cast<const f64x2*>(&fl[buf][j][4 * lane + 2]);
+      f[0][j] = a.x;
+      f[1][j] = a.y;
+      f[2][j] = b.x;
+      f[3][j] = b.y;
+    }
+#pragma unroll
+    for (int u = 0; u < VB; ++u) {
+      sq[u] = __builtin_amdgcn_udot4(w[u], 0x01010101u, sq[u], false);
+      sc[u] = __builtin_amdgcn_udot4(w[u], cm, sc[u], false);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) add(u, (double)((w[u] >> (8 * k)) & 0xffu), f[k]);
+    }
+  }
+  // the last < 256 samples of the range, one per lane
+  for (size_t i = iend + lane; i < i1; i += 64) {
+    double f[NFU];
+#pragma unroll
+    for (int j = 0; j < NFU; ++j) f[j] = feat[i * NF + j];
+    const uint32_t cs = (cmask[i >> 2] >> (8 * (i & 3))) & 0xffu;
+#pragma unroll
+    for (int u = 0; u < VB; ++u) {
+      const uint32_t qb = load.q[rows[u] + i];
+      sq[u] += qb;
+      sc[u] += qb * cs;
+      add(u, (double)qb, f);
+    }
+  }
+  // totals over the 64 lanes (butterfly, the same order in every lane)
+#pragma unroll
+  for (int u = 0; u < VB; ++u) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+      for (int s = 0; s < NFU; ++s) acc[u][s] += __shfl_xor(acc[u][s], m);
+      sq[u] += __shfl_xor(sq[u], m);
+      sc[u] += __shfl_xor(sc[u], m);
+    }
+  }
+  if (lane == 0) {
+    const double sc1 = load.scale, sc2 = load.scale * load.scale;
+#pragma unroll
+    for (int u = 0; u < VB; ++u) {
+      const size_t v = vg + u;
+      if (v >= nvar) continue;
+      double o[NS];
+      o[0] = (double)sq[u] * sc1;
+      o[1] = 0.0;
+#pragma unroll
+      for (int r = 0; r < P + 1; ++r) o[2 + r] = acc[u][r] * sc1;
+      o[3 + P] = acc[u][P + 1] * sc2;
+#pragma unroll
+      for (int j = 0; j < P; ++j) o[4 + P + j] = acc[u][P + 2 + j] * sc1;
+      o[4 + 2 * P] = (double)sc[u] * sc1;
+      o[5 + 2 * P] = 0.0;
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) part[((size_t)blockIdx.y * NS + s2) * nvar_pad + v] = o[s2];
+    }
   }
 }
 
@@ -215,23 +452,7 @@ __global__ __launch_bounds__(64) void score_sums_kernel(L load, const double* __
 #pragma unroll
   for (int s = 0; s < NS; ++s) acc[s] = 0.0;
   size_t i = i0;
-  if constexpr (std::is_same<L, LoadU8>::value) {
-    if (load.vec && i + 16 <= i1) {
-      // the uint8 stream: the next two 16-sample segments of the lane's row are in flight
-      // while the current one is accumulated
-      uint4 w0 = load.raw16(vv, i);
-      uint4 w1 = i + 32 <= i1 ? load.raw16(vv, i + 16) : w0;
-      for (; i + 16 <= i1; i += 16) {
-        const uint4 cur = w0;
-        w0 = w1;
-        if (i + 48 <= i1) w1 = load.raw16(vv, i + 32);
-        double g[16];
-        load.expand16(cur, g);
-#pragma unroll
-        for (int k = 0; k < 16; ++k) accumulate<P, MISS>(g[k], feat + (i + k) * NF, acc);
-      }
-    }
-  } else if (load.vec) {
+  if (load.vec) {
     for (; i + 16 <= i1; i += 16) {
       double g[16];
       load.load16(vv, i, g);
@@ -265,7 +486,7 @@ __global__ void score_finalize_kernel(const double* __restrict__ part, size_t sp
   }
   if (!may_miss) {
     s[1] = (double)n;
-    s[6 + 2 * P] = ec.n_cases;
+    s[5 + 2 * P] = ec.n_cases;
   }
   const double sum = s[0], nv = s[1];
   saige_variant_result r;
@@ -279,12 +500,10 @@ __global__ void score_finalize_kernel(const double* __restrict__ part, size_t sp
   for (int j = 0; j < P; ++j) ag[j] = missing ? dnan() : s[2 + j];
   double gres = s[2 + P];
   double q2 = s[3 + P];
-  double gm = s[4 + 2 * P];
 #pragma unroll
   for (int j = 0; j < P; ++j) {
     gres -= ag[j] * ec.xtr[j];
     q2 -= 2.0 * ag[j] * s[4 + P + j];
-    gm -= ag[j] * ec.xtm[j];
   }
 #pragma unroll
   for (int j = 0; j < P; ++j) {
@@ -313,19 +532,17 @@ __global__ void score_finalize_kernel(const double* __restrict__ part, size_t sp
   r.spa_iters2 = 0;
   r.spa_root1 = 0.0;
   r.spa_root2 = 0.0;
-  {
-    const double sd = sqrt(vt / ec.tau_e);
-    const double f[6] = {vts > 0.0 ? score / sqrt(vts) : 0.0, log(fmax(vt, 1e-300)),
-                         log1p(fmax(r.mac, 0.0)), log(s[8 + 2 * P] + 1e-8),
-                         log(s[9 + 2 * P] + 1e-8), log(s[10 + 2 * P] + 1e-8)};
-    for (int j = 0; j < 6; ++j) r.spa_features[j] = (float)f[j];
-    (void)sd;
-  }
+  // features of the learned root proposal; the carrier moments (3-5) are filled in by the
+  // SPA setup for the variants that go to SPA
+  r.spa_features[0] = (float)(vts > 0.0 ? score / sqrt(vts) : 0.0);
+  r.spa_features[1] = (float)log(fmax(vt, 1e-300));
+  r.spa_features[2] = (float)log1p(fmax(r.mac, 0.0));
+  r.spa_features[3] = r.spa_features[4] = r.spa_features[5] = 0.0f;
   r.variance_ratio = vr;
   r.beta = vts > 1e-30 ? score / vts : 0.0;
   r.se_beta = vts > 1e-30 ? 1.0 / sqrt(vts) : __longlong_as_double(0x7ff0000000000000ll);
   if (ec.has_y) {
-    const double sc = s[5 + 2 * P], nc = s[6 + 2 * P], sn = s[7 + 2 * P];
+    const double sc = s[4 + 2 * P], nc = s[5 + 2 * P], sn = sum - sc;
     const double nn = nv - nc;
     r.af_cases = nc > 0.0 ? sc / (2.0 * nc) : dnan();
     r.af_controls = nn > 0.0 ? sn / (2.0 * nn) : dnan();
@@ -338,14 +555,14 @@ __global__ void score_finalize_kernel(const double* __restrict__ part, size_t sp
     r.n_controls = 0;
   }
   if (ec.use_spa && pn < ec.cutoff) {  // single_variant.rs:183-207
-    const double q = score / sqrt(vr) + gm;
-    const double qinv = 2.0 * gm - q;
+    // q = S / sqrt(VR) + m1 and qinv = 2 m1 - q are completed by the SPA setup, which
+    // sums m1 = sum mu g~ over the samples
     uint32_t slot = atomicAdd(n_gate, 1u);
     GateInfo gi;
     gi.var = (uint32_t)v;
     gi.pad = 0;
-    gi.q = q;
-    gi.qinv = qinv;
+    gi.q = score / sqrt(vr);
+    gi.qinv = 0.0;
     gi.pn = pn;
     gi.sd = sqrt(vt / ec.tau_e);
     for (int j = 0; j < 6; ++j) gi.feat[j] = (double)r.spa_features[j];
@@ -660,42 +877,54 @@ struct SpaVar {
   int pad;
 };
 constexpr int kSpaPassBlock = 256;
+constexpr int kSpaPart = 8;                   // partial sums per (variant, slice)
 constexpr int kSpaMaxPasses = 8 * 1000 + 16;  // the guard of spa_kernel's pass loop
 
-template <int NT>
-__device__ __forceinline__ void block_sum4t(double* v, double* sh) {
-  for (int k = 0; k < 4; ++k) sh[k * NT + threadIdx.x] = v[k];
+template <int NT, int K>
+__device__ __forceinline__ void block_sum_t(double* v, double* sh) {
+  for (int k = 0; k < K; ++k) sh[k * NT + threadIdx.x] = v[k];
   __syncthreads();
   for (int w = NT / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
-      for (int k = 0; k < 4; ++k) sh[k * NT + threadIdx.x] += sh[k * NT + threadIdx.x + w];
+      for (int k = 0; k < K; ++k) sh[k * NT + threadIdx.x] += sh[k * NT + threadIdx.x + w];
     __syncthreads();
   }
-  for (int k = 0; k < 4; ++k) v[k] = sh[k * NT];
+  for (int k = 0; k < K; ++k) v[k] = sh[k * NT];
   __syncthreads();
 }
 
-// KIND 0: g_pos, g_neg and the NA moments; 1: K'(t) and K''(t) sums at both pending points;
-// 2: K(zeta) and K''(zeta) sums at both roots (variants whose two searches converged).
-template <class G, bool TAB, int KIND>
+// 1/d for d > 0: the hardware estimate refined by two Newton steps (within an ulp of the
+// rounded quotient over the range d takes here, d >= mu >= 1e-10); 1/inf = 0 as in IEEE
+// division.
+__device__ __forceinline__ double recip_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  return isinf(d) ? 0.0 : r;
+}
+
+// KIND 0: g_pos, g_neg, the NA moments, m1 = sum mu g~ (single_variant.rs:189-192) and the
+// carrier moments sum mu, sum mu(1-mu), sum mu^2(1-mu) over samples with dosage > 0 (the
+// learned proposal's inputs); 1: K'(t) and K''(t) sums at both pending points; 2: K(zeta)
+// and K''(zeta) sums at both roots (variants whose two searches converged).
+template <class G, int KIND>
 __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
     G gt, const double* __restrict__ mu, size_t n, const GateInfo* __restrict__ gate,
     const uint32_t* __restrict__ n_gate, int fast, size_t slice,
     const SpaVar* __restrict__ vars, double* __restrict__ part) {
+  constexpr int K = KIND == 0 ? 8 : 4;
   const uint32_t slot = blockIdx.x;  // grid: (gated variants, slices)
   if (slot >= *n_gate) return;
   if (KIND == 1 && vars[slot].done) return;
   if (KIND == 2 && !(vars[slot].st[0].converged && vars[slot].st[1].converged)) return;
-  __shared__ double sh[4 * kSpaPassBlock];
+  __shared__ double sh[K * kSpaPassBlock];
   __shared__ GateInfo g;
-  __shared__ double tg[TAB ? 256 : 1], te0[TAB ? 256 : 1], te1[TAB ? 256 : 1];
   if (threadIdx.x == 0) g = gate[slot];
   __syncthreads();
   const size_t v = g.var;
   const double thr = fast ? 1e-10 : -1.0;  // fast: |g~| > 1e-10 enters K (fast.rs:191-200)
-  if constexpr (TAB) {
-    for (int b = threadIdx.x; b < 256; b += kSpaPassBlock) tg[b] = (double)b * gt.load.scale - g.ag[0];
-  }
   double t0 = 0.0, t1 = 0.0;
   if (KIND == 1) {
     t0 = vars[slot].st[0].pend;
@@ -704,72 +933,175 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
     t0 = vars[slot].st[0].root;
     t1 = vars[slot].st[1].root;
   }
-  if constexpr (TAB) {
-    if (KIND != 0) {
-      __syncthreads();
-      for (int b = threadIdx.x; b < 256; b += kSpaPassBlock) {
-        te0[b] = exp(-tg[b] * t0);
-        te1[b] = exp(-tg[b] * t1);
-      }
-    }
-    __syncthreads();
-  }
   const size_t i0 = (size_t)blockIdx.y * slice, i1 = min(i0 + slice, n);
-  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  double a[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = 0.0;
   for (size_t i = i0 + threadIdx.x; i < i1; i += kSpaPassBlock) {
-    double gi, e0 = 1.0, e1 = 1.0;
-    if constexpr (TAB) {
-      const uint32_t by = gt.byte(v, i);
-      gi = tg[by];
-      if (KIND != 0) {
-        e0 = te0[by];
-        e1 = te1[by];
-      }
-    } else {
-      gi = gt(v, i, g.ag);
-      if (KIND != 0) {
-        e0 = exp(-gi * t0);
-        e1 = exp(-gi * t1);
-      }
-    }
     const double m = mu[i];
     if (KIND == 0) {
+      const bool carrier = gt.load(v, i) > 0.0;
+      const double gi = gt(v, i, g.ag);
       if (gi > 0.0) a[0] += gi;
       if (gi < 0.0) a[1] += gi;
+      const double m2 = m * (1.0 - m);
       if (!(fabs(gi) > thr)) {
         a[2] += m * gi;
-        a[3] += m * (1.0 - m) * gi * gi;
+        a[3] += m2 * gi * gi;
+      }
+      a[4] = fma(m, gi, a[4]);
+      if (carrier) {
+        a[5] += m;
+        a[6] += m2;
+        a[7] = fma(m, m2, a[7]);
       }
       continue;
     }
+    const double gi = gt(v, i, g.ag);
     if (!(fabs(gi) > thr)) continue;
-    const double d0 = (1.0 - m) * e0 + m;
-    const double d1 = (1.0 - m) * e1 + m;
-    const double r0 = 1.0 / d0, r1 = 1.0 / d1;
+    const double e0 = exp(-gi * t0), e1 = exp(-gi * t1);
+    const double om = 1.0 - m;
+    const double mg = m * gi, w2 = om * mg * gi;
+    const double r0 = recip_nr(fma(om, e0, m)), r1 = recip_nr(fma(om, e1, m));
     if (KIND == 1) {
-      const double mg = m * gi, w2 = (1.0 - m) * mg * gi;
-      a[0] += mg * r0;
-      a[1] += w2 * e0 * r0 * r0;
-      a[2] += mg * r1;
-      a[3] += w2 * e1 * r1 * r1;
+      a[0] = fma(mg, r0, a[0]);
+      a[2] = fma(mg, r1, a[2]);
     } else {
-      // K at the roots is ln(1 - m + m e^{g zeta}) (binary.rs:148-151), K'' as above
-      const double w2 = (1.0 - m) * m * gi * gi;
-      a[0] += log(1.0 - m + m * exp(gi * t0));
-      a[1] += w2 * e0 * r0 * r0;
-      a[2] += log(1.0 - m + m * exp(gi * t1));
-      a[3] += w2 * e1 * r1 * r1;
+      // K at the roots is ln(1 - m + m e^{g zeta}) (binary.rs:148-151)
+      a[0] += log(fma(m, exp(gi * t0), om));
+      a[2] += log(fma(m, exp(gi * t1), om));
+    }
+    a[1] = fma(w2 * e0, r0 * r0, a[1]);
+    a[3] = fma(w2 * e1, r1 * r1, a[3]);
+  }
+  block_sum_t<kSpaPassBlock, K>(a, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < K; ++k) part[((size_t)slot * gridDim.y + blockIdx.y) * kSpaPart + k] = a[k];
+}
+
+// The passes of spa_pass_kernel for uint8 dosages of an intercept-only model, where g~
+// depends only on the dosage byte: per byte, LDS holds g~ and the exponentials of the pass
+// (g~ = 0 and unit exponentials for bytes whose samples do not enter K under the fast
+// rule), and each thread takes four consecutive samples per step (one 32-bit dosage load and
+// two 16-byte mu loads), with the next step's loads in flight.
+template <int KIND>
+__global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
+    LoadU8 ld8, const double* __restrict__ mu, size_t n, const GateInfo* __restrict__ gate,
+    const uint32_t* __restrict__ n_gate, int fast, size_t slice,
+    const SpaVar* __restrict__ vars, double* __restrict__ part) {
+  constexpr int K = KIND == 0 ? 8 : 4;
+  constexpr int TW = KIND == 0 ? 2 : (KIND == 1 ? 4 : 8);  // doubles per byte entry
+  const uint32_t slot = blockIdx.x;  // grid: (gated variants, slices)
+  if (slot >= *n_gate) return;
+  if (KIND == 1 && vars[slot].done) return;
+  if (KIND == 2 && !(vars[slot].st[0].converged && vars[slot].st[1].converged)) return;
+  __shared__ double sh[K * kSpaPassBlock];
+  __shared__ __attribute__((aligned(16))) double tab[256 * TW];
+  __shared__ GateInfo g;
+  if (threadIdx.x == 0) g = gate[slot];
+  __syncthreads();
+  const double thr = fast ? 1e-10 : -1.0;  // fast: |g~| > 1e-10 enters K (fast.rs:191-200)
+  {
+    static_assert(kSpaPassBlock == 256, "one table entry per thread");
+    const int b = threadIdx.x;
+    const double gb = (double)b * ld8.scale - g.ag[0];
+    const bool incl = fabs(gb) > thr;
+    double* e = tab + b * TW;
+    if (KIND == 0) {
+      e[0] = gb;
+      e[1] = incl ? 1.0 : 0.0;
+    } else {
+      const double t0 = KIND == 1 ? vars[slot].st[0].pend : vars[slot].st[0].root;
+      const double t1 = KIND == 1 ? vars[slot].st[1].pend : vars[slot].st[1].root;
+      e[0] = incl ? gb : 0.0;
+      e[1] = incl ? exp(-gb * t0) : 1.0;
+      e[2] = incl ? exp(-gb * t1) : 1.0;
+      e[3] = 0.0;
+      if (KIND == 2) {
+        e[3] = incl ? exp(gb * t0) : 1.0;
+        e[4] = incl ? exp(gb * t1) : 1.0;
+        e[5] = incl ? 1.0 : 0.0;
+        e[6] = e[7] = 0.0;
+      }
     }
   }
-  block_sum4t<kSpaPassBlock>(a, sh);
+  __syncthreads();
+  double a[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = 0.0;
+  auto body = [&](uint32_t by, double m) {
+    const double* e = tab + by * TW;
+    const double om = 1.0 - m;
+    if (KIND == 0) {
+      const double gi = e[0];
+      if (gi > 0.0) a[0] += gi;
+      if (gi < 0.0) a[1] += gi;
+      const double m2 = m * om;
+      if (e[1] == 0.0) {
+        a[2] += m * gi;
+        a[3] += m2 * gi * gi;
+      }
+      a[4] = fma(m, gi, a[4]);
+      if (by != 0u) {
+        a[5] += m;
+        a[6] += m2;
+        a[7] = fma(m, m2, a[7]);
+      }
+    } else {
+      const double gk = e[0], e0 = e[1], e1 = e[2];
+      const double mg = m * gk, w2 = om * mg * gk;
+      const double r0 = recip_nr(fma(om, e0, m)), r1 = recip_nr(fma(om, e1, m));
+      if (KIND == 1) {
+        a[0] = fma(mg, r0, a[0]);
+        a[2] = fma(mg, r1, a[2]);
+      } else {
+        // K at the roots is ln(1 - m + m e^{g zeta}) (binary.rs:148-151)
+        a[0] = fma(e[5], log(fma(m, e[3], om)), a[0]);
+        a[2] = fma(e[5], log(fma(m, e[4], om)), a[2]);
+      }
+      a[1] = fma(w2 * e0, r0 * r0, a[1]);
+      a[3] = fma(w2 * e1, r1 * r1, a[3]);
+    }
+  };
+  const uint8_t* row = ld8.q + (size_t)g.var * ld8.ld;
+  const size_t i0 = (size_t)blockIdx.y * slice, i1 = min(i0 + slice, n);
+  const size_t i4 = i0 + ((i1 - i0) & ~size_t(3));
+  constexpr size_t kStep = 4 * kSpaPassBlock;
+  size_t tail = i0;
+  if (ld8.vec) {  // 16-byte aligned rows; i0 is a multiple of 256
+    size_t i = i0 + 4 * threadIdx.x;
+    uint32_t wn = 0u;
+    f64x2 ma = {0.0, 0.0}, mb = {0.0, 0.0};
+    if (i < i4) {
+      wn = *reinterpret_cast<const uint32_t*>(row + i);
+      ma = *reinterpret_cast<const f64x2*>(mu + i);
+      mb = *reinterpret_cast<const f64x2*>(mu + i + 2);
+    }
+    for (; i < i4; i += kStep) {
+      const uint32_t w = wn;
+      const f64x2 ca = ma, cb = mb;
+      if (i + kStep < i4) {
+        wn = *reinterpret_cast<const uint32_t*>(row + i + kStep);
+        ma = *reinterpret_cast<const f64x2*>(mu + i + kStep);
+        mb = *reinterpret_cast<const f64x2*>(mu + i + kStep + 2);
+      }
+      body(w & 0xffu, ca.x);
+      body((w >> 8) & 0xffu, ca.y);
+      body((w >> 16) & 0xffu, cb.x);
+      body(w >> 24, cb.y);
+    }
+    tail = i4;
+  }
+  for (size_t j = tail + threadIdx.x; j < i1; j += kSpaPassBlock) body(row[j], mu[j]);
+  block_sum_t<kSpaPassBlock, K>(a, sh);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 4; ++k) part[((size_t)slot * gridDim.y + blockIdx.y) * 4 + k] = a[k];
+    for (int k = 0; k < K; ++k) part[((size_t)slot * gridDim.y + blockIdx.y) * kSpaPart + k] = a[k];
 }
 
 __device__ __forceinline__ double slice_sum(const double* __restrict__ part, uint32_t slot, int S,
                                             int k) {
   double s = 0.0;
-  for (int q = 0; q < S; ++q) s += part[((size_t)slot * S + q) * 4 + k];
+  for (int q = 0; q < S; ++q) s += part[((size_t)slot * S + q) * kSpaPart + k];
   return s;
 }
 
@@ -791,22 +1123,43 @@ __global__ __launch_bounds__(128) void spa_propose_kernel(const GateInfo* __rest
   }
 }
 
-__global__ void spa_setup_step(const GateInfo* __restrict__ gate, const uint32_t* __restrict__ n_gate,
-                               int S, const double* __restrict__ part, int use_tinit,
-                               SpaVar* __restrict__ vars) {
+// Totals of the setup pass: q = S / sqrt(VR) + m1, qinv = 2 m1 - q (single_variant.rs:
+// 189-196), the support bounds, the NA moments and the carrier-moment features.
+__global__ void spa_setup_step(GateInfo* __restrict__ gate, const uint32_t* __restrict__ n_gate,
+                               int S, const double* __restrict__ part, SpaVar* __restrict__ vars,
+                               saige_variant_result* __restrict__ out) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= *n_gate) return;
   SpaVar V = vars[slot];
-  if (!use_tinit) V.tinit[0] = V.tinit[1] = 0.0;  // fast.rs:217-218
   V.gpos = slice_sum(part, slot, S, 0);
   V.gneg = slice_sum(part, slot, S, 1);
   V.na_mu = slice_sum(part, slot, S, 2);
   V.na_sigma = slice_sum(part, slot, S, 3);
+  V.tinit[0] = V.tinit[1] = 0.0;  // fast.rs:217-218
+  vars[slot] = V;
+  const double m1 = slice_sum(part, slot, S, 4);
+  GateInfo& g = gate[slot];
+  g.q = g.q + m1;
+  g.qinv = 2.0 * m1 - g.q;
+  for (int j = 0; j < 3; ++j) {
+    const float f = (float)log(slice_sum(part, slot, S, 5 + j) + 1e-8);
+    g.feat[3 + j] = (double)f;
+    if (out) out[g.var].spa_features[3 + j] = f;
+  }
+}
+
+// Begin both searches at 0 (fast.rs:217-218) or at the learned proposal; a q outside the
+// support (fast.rs:67-73) has its root at infinity.
+__global__ void spa_start_step(const GateInfo* __restrict__ gate,
+                               const uint32_t* __restrict__ n_gate, SpaVar* __restrict__ vars) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= *n_gate) return;
+  SpaVar V = vars[slot];
   const double q[2] = {gate[slot].q, gate[slot].qinv};
   for (int r = 0; r < 2; ++r) {
     root_start(V.st[r], V.tinit[r]);
     V.st[r].spent = 0;
-    if (q[r] >= V.gpos || q[r] <= V.gneg) {  // fast.rs:67-73: root at infinity
+    if (q[r] >= V.gpos || q[r] <= V.gneg) {
       V.st[r].root = __longlong_as_double(0x7ff0000000000000ll);
       V.st[r].converged = 1;
       V.st[r].phase = 3;
@@ -910,10 +1263,11 @@ struct saige_engine {
   EngineConst ec{};
   bool intercept_only = false;
   const saige_mlp* mlp = nullptr;
-  DevBuf feat, mu_d, x_d, idx, part, res, gate, ngate, in;
+  DevBuf feat, cmask, mu_d, x_d, idx, part, res, gate, ngate, in;
   DevBuf spa_vars, spa_part, spa_active;  // batched SPA state
-  // split counts of the sums kernel chosen by timing, per (loader, covariates, variant blocks)
-  std::map<std::tuple<int, int, size_t>, size_t> tuned_splits;
+  // sums-kernel shape (variants per lane, sample splits) chosen by timing, per (loader,
+  // covariates, variants)
+  std::map<std::tuple<int, int, size_t>, std::pair<int, size_t>> tuned_shape;
   MlpView mlp_view() const {
     MlpView v{};
     if (mlp) {
@@ -944,63 +1298,6 @@ template <class L> constexpr int loader_id() {
   return std::is_same<L, LoadU8>::value ? 1 : (std::is_same<L, LoadF64>::value ? 2 : 3);
 }
 
-// Tile-shape autotuner of the sums kernel: the first batch of a shape times the kernel and
-// the reduction of its partials at 16 to 256 sample splits (every candidate computes the
-// complete, correct sums; the run that follows recomputes them with the choice) and the
-// fastest split count serves the later batches of that shape.  Shapes with fewer than 64 * 1024 samples use the default split count.
-template <int P, class L>
-static int sums_splits(saige_engine* e, L load, size_t nvar, size_t nvar_pad, size_t* out) {
-  constexpr int NS = Layout<P>::NS;
-  const size_t n = e->n;
-  const size_t vblocks = ceil_div(nvar, 64);
-  auto norm = [&](size_t sp) {
-    const size_t sps = round_up(ceil_div(n, std::max<size_t>(sp, 1)), 16);
-    return ceil_div(n, sps);
-  };
-  const size_t dflt = norm(std::max<size_t>(1, std::min<size_t>(ceil_div(2048, vblocks),
-                                                                 ceil_div(n, 512))));
-  const auto key = std::make_tuple(loader_id<L>(), P, vblocks);
-  auto it = e->tuned_splits.find(key);
-  if (it != e->tuned_splits.end()) {
-    *out = it->second;
-    return SAIGE_OK;
-  }
-  if (n < 65536) {
-    e->tuned_splits[key] = dflt;
-    *out = dflt;
-    return SAIGE_OK;
-  }
-  hipStream_t s = e->stream;
-  size_t best = dflt;
-  float best_ms = 1e30f;
-  hipEvent_t a, b;
-  SAIGE_HIP(hipEventCreate(&a));
-  SAIGE_HIP(hipEventCreate(&b));
-  for (size_t cand : {size_t(16), size_t(32), size_t(64), size_t(128), size_t(256)}) {
-    const size_t sp = norm(cand);
-    const size_t sps = round_up(ceil_div(n, sp), 16);
-    SAIGE_TRY(e->part.ensure((sp + 1) * NS * nvar_pad * 8));
-    SAIGE_HIP(hipEventRecord(a, s));
-    score_sums_kernel<P, L><<<dim3((unsigned)vblocks, (unsigned)sp), 64, 0, s>>>(
-        load, e->feat.as<double>(), n, nvar, sps, e->part.as<double>(), nvar_pad);
-    sums_reduce_kernel<<<dim3((unsigned)ceil_div(nvar_pad, 256), (unsigned)NS), 256, 0, s>>>(
-        e->part.as<double>(), sp, NS, nvar_pad, e->part.as<double>() + sp * NS * nvar_pad);
-    SAIGE_HIP(hipEventRecord(b, s));
-    SAIGE_HIP(hipEventSynchronize(b));
-    float ms = 0.f;
-    SAIGE_HIP(hipEventElapsedTime(&ms, a, b));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = sp;
-    }
-  }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  e->tuned_splits[key] = best;
-  *out = best;
-  return SAIGE_OK;
-}
-
 // SPA of a batch's gated variants with S work-groups per variant and pass (about 1,024
 // work-groups per pass, slices of at least 2,048 samples); the host reads the number of
 // variants still searching after every fourth pass.
@@ -1017,27 +1314,35 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out) {
   const size_t slice = round_up(ceil_div(n, S), 256);
   S = ceil_div(n, slice);
   SAIGE_TRY(e->spa_vars.ensure(ng * sizeof(SpaVar)));
-  SAIGE_TRY(e->spa_part.ensure(ng * S * 4 * 8));
+  SAIGE_TRY(e->spa_part.ensure(ng * S * kSpaPart * 8));
   SAIGE_TRY(e->spa_active.ensure(64));
   uint32_t* active = e->spa_active.as<uint32_t>();
   SpaVar* vars = e->spa_vars.as<SpaVar>();
   double* part = e->spa_part.as<double>();
-  const GateInfo* gate = e->gate.as<GateInfo>();
+  GateInfo* gate = e->gate.as<GateInfo>();
   const uint32_t* ngp = e->ngate.as<uint32_t>();
   const double* mu = e->mu_d.as<double>();
   const MlpView mv = e->mlp_view();
   const dim3 grid(ng, (unsigned)S);
   const unsigned sb = (unsigned)ceil_div(ng, 256);
   const int fast = e->ec.fast, iS = (int)S;
+  auto launch_pass = [&](auto kind) {
+    constexpr int KIND = decltype(kind)::value;
+    if constexpr (TAB)
+      spa_pass_tab_kernel<KIND><<<grid, kSpaPassBlock, 0, s>>>(gt.load, mu, n, gate, ngp, fast,
+                                                               slice, vars, part);
+    else
+      spa_pass_kernel<G, KIND><<<grid, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
+                                                              vars, part);
+  };
   zero_u32_kernel<<<1, 1, 0, s>>>(active);
   zero_u32_kernel<<<1, 1, 0, s>>>(active + 1);
+  launch_pass(std::integral_constant<int, 0>{});
+  spa_setup_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, vars, d_out);
   if (mv.enabled) spa_propose_kernel<<<ng, 128, 0, s>>>(gate, ngp, mv, vars);
-  spa_pass_kernel<G, TAB, 0><<<grid, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
-                                                             vars, part);
-  spa_setup_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, mv.enabled, vars);
+  spa_start_step<<<sb, 256, 0, s>>>(gate, ngp, vars);
   for (int pass = 0; pass < kSpaMaxPasses; ++pass) {
-    spa_pass_kernel<G, TAB, 1><<<grid, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
-                                                               vars, part);
+    launch_pass(std::integral_constant<int, 1>{});
     spa_eval_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, e->ec.spa_tol, pass, vars, active);
     if (pass % 4 == 3) {
       uint32_t cnt = 0;
@@ -1046,10 +1351,111 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out) {
       if (cnt == 0) break;
     }
   }
-  spa_pass_kernel<G, TAB, 2><<<grid, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
-                                                             vars, part);
+  launch_pass(std::integral_constant<int, 2>{});
   spa_final_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, vars, d_out);
   SAIGE_HIP(hipGetLastError());
+  return SAIGE_OK;
+}
+
+
+// One launch of the sums kernel.  uint8 dosages with 16-byte rows take a byte kernel: form 0
+// has lanes over samples (up to 3 covariates), form 1 a lane per variant; the other loaders
+// take the generic kernel.
+template <int P>
+constexpr bool has_u8t() { return P <= 3; }
+
+template <int P, class L>
+static void launch_sums(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int form,
+                        size_t splits, size_t sps, double* part) {
+  hipStream_t s = e->stream;
+  const size_t n = e->n;
+  if constexpr (std::is_same<L, LoadU8>::value) {
+    if (load.vec) {
+      const uint32_t* cm = e->cmask.as<uint32_t>();
+      if constexpr (has_u8t<P>()) {
+        if (form == 0) {
+          const size_t vpw = 4 * SumsT<P>::VB;
+          score_sums_u8t_kernel<P><<<dim3((unsigned)ceil_div(nvar, vpw), (unsigned)splits), 256, 0,
+                                     s>>>(load, e->feat.as<double>(), cm, n, nvar, sps, part,
+                                          nvar_pad);
+          return;
+        }
+      }
+      score_sums_u8_kernel<P, 1><<<dim3((unsigned)ceil_div(nvar, 64), (unsigned)splits), 64, 0,
+                                   s>>>(load, e->feat.as<double>(), cm, n, nvar, sps, part,
+                                        nvar_pad);
+      return;
+    }
+  }
+  score_sums_kernel<P, L><<<dim3((unsigned)ceil_div(nvar, 64), (unsigned)splits), 64, 0, s>>>(
+      load, e->feat.as<double>(), n, nvar, sps, part, nvar_pad);
+}
+
+// Sample-range length of a split: whole 256-sample blocks for form 0, 16-sample segments
+// otherwise.
+static size_t split_len(size_t n, size_t splits, int form) {
+  const size_t g = form == 0 ? 256 : 16;
+  return round_up(ceil_div(n, std::max<size_t>(splits, 1)), g);
+}
+
+// Tile-shape autotuner of the sums kernel: the first batch of a shape times the kernel and
+// the reduction of its partials for each form and 16 to 256 sample splits (every candidate
+// computes the complete, correct sums; the run that follows recomputes them with the
+// choice), and the fastest (form, splits) serves the later batches of that shape.  Fewer
+// than 64 * 1024 samples use the default shape.
+template <int P, class L>
+static int sums_shape(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int* form_out,
+                      size_t* splits_out) {
+  constexpr int NS = Layout<P>::NS;
+  const size_t n = e->n;
+  const bool u8 = std::is_same<L, LoadU8>::value && load.vec;
+  const int form0 = (u8 && has_u8t<P>()) ? 0 : 1;
+  auto norm = [&](size_t sp, int form) { return ceil_div(n, split_len(n, sp, form)); };
+  const size_t vblocks = ceil_div(nvar, 64);
+  const size_t dflt_sp = std::max<size_t>(1, std::min<size_t>(ceil_div(2048, vblocks),
+                                                               ceil_div(n, 512)));
+  const auto key = std::make_tuple(loader_id<L>() * 2 + (int)load.vec, P, nvar);
+  auto it = e->tuned_shape.find(key);
+  if (it != e->tuned_shape.end()) {
+    *form_out = it->second.first;
+    *splits_out = it->second.second;
+    return SAIGE_OK;
+  }
+  int best_form = form0;
+  size_t best = norm(form0 == 0 ? 16 : dflt_sp, form0);
+  if (n >= 65536) {
+    hipStream_t s = e->stream;
+    float best_ms = 1e30f;
+    hipEvent_t a, b;
+    SAIGE_HIP(hipEventCreate(&a));
+    SAIGE_HIP(hipEventCreate(&b));
+    for (int form = form0; form <= 1; ++form) {
+      for (size_t cand : {size_t(16), size_t(32), size_t(64), size_t(128), size_t(256)}) {
+        const size_t sp = norm(cand, form);
+        const size_t sps = split_len(n, sp, form);
+        SAIGE_TRY(e->part.ensure((sp + 1) * NS * nvar_pad * 8));
+        double* part = e->part.as<double>();
+        SAIGE_HIP(hipEventRecord(a, s));
+        launch_sums<P, L>(e, load, nvar, nvar_pad, form, sp, sps, part);
+        sums_reduce_kernel<<<dim3((unsigned)ceil_div(nvar_pad, 256), (unsigned)NS), 256, 0, s>>>(
+            part, sp, NS, nvar_pad, part + sp * NS * nvar_pad);
+        SAIGE_HIP(hipEventRecord(b, s));
+        SAIGE_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        SAIGE_HIP(hipEventElapsedTime(&ms, a, b));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = sp;
+          best_form = form;
+        }
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  e->tuned_shape[key] = std::make_pair(best_form, best);
+  *form_out = best_form;
+  *splits_out = best;
   return SAIGE_OK;
 }
 
@@ -1058,17 +1464,16 @@ static int run_batch_p(saige_engine* e, L load, size_t nvar, saige_variant_resul
   hipStream_t s = e->stream;
   const size_t n = e->n;
   constexpr int NS = Layout<P>::NS;
-  const size_t vblocks = ceil_div(nvar, 64);
-  const size_t nvar_pad = round_up(nvar, 64);
+  const size_t nvar_pad = round_up(nvar, 128);
   size_t splits = 1;
-  SAIGE_TRY((sums_splits<P, L>(e, load, nvar, nvar_pad, &splits)));
-  const size_t sps = round_up(ceil_div(n, splits), 16);
+  int form = 1;
+  SAIGE_TRY((sums_shape<P, L>(e, load, nvar, nvar_pad, &form, &splits)));
+  const size_t sps = split_len(n, splits, form);
   SAIGE_TRY(e->part.ensure((splits + 1) * NS * nvar_pad * 8));
   SAIGE_TRY(e->gate.ensure(nvar * sizeof(GateInfo)));
   SAIGE_TRY(e->ngate.ensure(64));
   zero_u32_kernel<<<1, 1, 0, s>>>(e->ngate.as<uint32_t>());  // a kernel, not a memset node
-  score_sums_kernel<P, L><<<dim3((unsigned)vblocks, (unsigned)splits), 64, 0, s>>>(
-      load, e->feat.as<double>(), n, nvar, sps, e->part.as<double>(), nvar_pad);
+  launch_sums<P, L>(e, load, nvar, nvar_pad, form, splits, sps, e->part.as<double>());
   double* sums = e->part.as<double>() + splits * NS * nvar_pad;
   sums_reduce_kernel<<<dim3((unsigned)ceil_div(nvar_pad, 256), (unsigned)NS), 256, 0, s>>>(
       e->part.as<double>(), splits, NS, nvar_pad, sums);
@@ -1169,16 +1574,18 @@ int saige_engine_create(int device, const double* mu, const double* mu2, const d
       ec.xm2x[j * p + k] = c;
     }
   }
-  const size_t nf = 2 * p + 4;
+  const size_t nf = 2 * p + 3;
   std::vector<double> feat(n * nf);
+  // case flags as bytes (0/1), zero-padded to whole 16-sample segments
+  std::vector<uint8_t> cmask(round_up(n, 16) + 16, 0);
   for (size_t i = 0; i < n; ++i) {
     double* f = &feat[i * nf];
     for (size_t r = 0; r < p; ++r) f[r] = a_flat[i * p + r];  // col-major p x n view
     f[p] = residuals[i];
     f[p + 1] = mu2[i];
     for (size_t j = 0; j < p; ++j) f[p + 2 + j] = mu2[i] * x[j * n + i];
-    f[2 * p + 2] = mu[i];
-    f[2 * p + 3] = (y && y[i] > 0.5) ? 1.0 : 0.0;
+    f[2 * p + 2] = (y && y[i] > 0.5) ? 1.0 : 0.0;
+    cmask[i] = (y && y[i] > 0.5) ? 1 : 0;
   }
   int rc = SAIGE_OK;
   auto fail = [&](int code) {
@@ -1188,7 +1595,9 @@ int saige_engine_create(int device, const double* mu, const double* mu2, const d
   if ((rc = e->feat.ensure(feat.size() * 8)) != SAIGE_OK) return fail(rc);
   if ((rc = e->mu_d.ensure(n * 8)) != SAIGE_OK) return fail(rc);
   if ((rc = e->x_d.ensure(n * p * 8)) != SAIGE_OK) return fail(rc);
+  if ((rc = e->cmask.ensure(cmask.size())) != SAIGE_OK) return fail(rc);
   if (hipMemcpy(e->feat.p, feat.data(), feat.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->cmask.p, cmask.data(), cmask.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->mu_d.p, mu, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->x_d.p, x, n * p * 8, hipMemcpyHostToDevice) != hipSuccess) {
     set_error("engine upload failed");

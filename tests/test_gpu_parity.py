@@ -191,7 +191,7 @@ def _compare(got, want, mask_p=1e-10, integer_dosages=True):
     assert err2.max() < 1e-6
 
 
-@pytest.mark.parametrize("p", [1, 3])
+@pytest.mark.parametrize("p", [1, 3, 5])
 def test_score_u8_vs_oracle(p):
     n = 3000
     X, mu, y, a = _null(n, p, seed=p)
