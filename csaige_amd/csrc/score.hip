@@ -7,23 +7,28 @@
 // find_root_k1_fast (:53-132), saddle_probability_fast (:135-180) and
 // partition_for_fast_spa (:185-204).
 //
-// Kernel 1 (score_sums): a batch of variants is a (variants x samples) matrix times a
-// (samples x features) matrix.  One lane per variant, 64 variants per wave; the lanes of
-// a wave walk the same sample index, so the per-sample feature row
-//   [A(:, i) (p values), res_i, mu2_i, mu2_i * x_i (p values), mu_i, case_i]
-// is wave-uniform and comes from the scalar cache.  The sample range is split over the
-// grid; partial sums are combined in a fixed order (deterministic).
-// Kernel 2 (score_finalize): one thread per variant turns the sums into S, varT, varT*,
+// Sums (score_sums_*): a batch of variants is a (variants x samples) dosage matrix against
+// the per-sample feature rows [A(:, i) (p values), res_i, mu2_i, mu2_i * x_i (p values),
+// case_i].  uint8 dosages take score_sums_u8t_kernel (lanes over samples, several variants
+// per wave, feature rows shared through LDS) or score_sums_u8_kernel (a lane per variant,
+// feature rows wave-uniform from the scalar cache); the shape is picked by timing the first
+// batch of each shape.  Other dosage forms take score_sums_kernel.  The sample range is
+// split over the grid; the partial sums are added in a fixed order (deterministic).
+// Finalize (score_finalize): one thread per variant turns the sums into S, varT, varT*,
 // T, the chi-square(1) p-value and the SPA gate, using
 //   g~ = g - X (A g)                       (single_variant.rs:137-144)
 //   g~'res      = g'res - (A g)'(X'res)
 //   sum mu2 g~^2 = g'M2g - 2 (A g)'(X'M2 g) + (A g)'(X'M2X)(A g)
-//   sum mu g~    = g'mu - (A g)'(X'mu)
 // which hold for any p x n matrix A (see the layout note on xvx_inv_xv_flat).
-// Kernel 3 (spa): one work-group per gated variant runs both safeguarded Newton roots in
-// lockstep, recomputing g~_i from the dosage, X and A g in every pass, each pass fusing
-// K'(t) and K''(t) (the values the reference's loop needs at the same point), then the
-// Lugannani-Rice tail.  Control flow per root is exactly find_root_k1_fast.
+// SPA of a batch (spa_pass_*, spa_*_step): the gated variants' passes over the samples are
+// split over several work-groups each; a setup pass gives m1 = sum mu g~, the support and
+// the carrier moments, then every pass evaluates K'(t) and K''(t) of both roots at their
+// pending points and a one-thread-per-variant step advances both safeguarded Newton
+// searches with exactly the control flow of find_root_k1_fast, until every search ended;
+// a last pass gives K and K'' at the roots for the Lugannani-Rice tails.  spa_kernel (one
+// work-group per variant, all passes in one launch) serves single saige_spa_binary calls.
+#include <cfloat>
+
 #include "common.hpp"
 #include "mlp_common.hpp"
 
@@ -327,7 +332,8 @@ __global__ __launch_bounds__(256) void score_sums_u8t_kernel(LoadU8 load,
   constexpr int NFU = SumsT<P>::NFU, VB = SumsT<P>::VB;
   __shared__ __attribute__((aligned(16))) double fl[2][NFU][256];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const size_t vg = ((size_t)blockIdx.x * 4 + wave) * VB;
+  // the wave's variant group is wave-uniform: row offsets live in scalar registers
+  const size_t vg = ((size_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(wave)) * VB;
   const size_t i0 = (size_t)blockIdx.y * sps, i1 = min(i0 + sps, n);
   size_t rows[VB];
 #pragma unroll
@@ -873,11 +879,12 @@ struct SpaVar {
   RootState st[2];
   double gpos, gneg, na_mu, na_sigma;
   double tinit[2];
-  int done;  // both searches finished
+  int done;          // both searches finished
+  int bmin, bmax;    // smallest and largest dosage byte of the variant (byte-table passes)
   int pad;
 };
 constexpr int kSpaPassBlock = 256;
-constexpr int kSpaPart = 8;                   // partial sums per (variant, slice)
+constexpr int kSpaPart = 10;                  // partials per (variant, slice): 8 sums, byte range
 constexpr int kSpaMaxPasses = 8 * 1000 + 16;  // the guard of spa_kernel's pass loop
 
 template <int NT, int K>
@@ -893,15 +900,18 @@ __device__ __forceinline__ void block_sum_t(double* v, double* sh) {
   __syncthreads();
 }
 
-// 1/d for d > 0: the hardware estimate refined by two Newton steps (within an ulp of the
-// rounded quotient over the range d takes here, d >= mu >= 1e-10); 1/inf = 0 as in IEEE
-// division.
-__device__ __forceinline__ double recip_nr(double d) {
+// 1/d for finite d > 0: the hardware estimate refined by two Newton steps (d >= mu >= 1e-10
+// here; over 2^28 sampled d = mu + (1 - mu) e^x it equals the IEEE quotient in every case);
+// recip_nr also gives 1/inf = 0 as IEEE division does.
+__device__ __forceinline__ double recip_nr_finite(double d) {
   double r = __builtin_amdgcn_rcp(d);
   double e = fma(-d, r, 1.0);
   r = fma(r, e, r);
   e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ double recip_nr(double d) {
+  const double r = recip_nr_finite(d);
   return isinf(d) ? 0.0 : r;
 }
 
@@ -975,8 +985,14 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
     a[3] = fma(w2 * e1, r1 * r1, a[3]);
   }
   block_sum_t<kSpaPassBlock, K>(a, sh);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < K; ++k) part[((size_t)slot * gridDim.y + blockIdx.y) * kSpaPart + k] = a[k];
+  if (threadIdx.x == 0) {
+    double* o = part + ((size_t)slot * gridDim.y + blockIdx.y) * kSpaPart;
+    for (int k = 0; k < K; ++k) o[k] = a[k];
+    if (KIND == 0) {
+      o[8] = 0.0;
+      o[9] = 255.0;
+    }
+  }
 }
 
 // The passes of spa_pass_kernel for uint8 dosages of an intercept-only model, where g~
@@ -1017,6 +1033,13 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
       e[1] = incl ? exp(-gb * t0) : 1.0;
       e[2] = incl ? exp(-gb * t1) : 1.0;
       e[3] = 0.0;
+      if (KIND == 1) {
+        // an overflowing e^{-g t} makes that sample's K' term 0 and its K'' term NaN
+        // (m g / inf, inf / inf): the capped value gives K' and K'' terms below 1e-300,
+        // and the NaN is added once per variant below
+        e[1] = fmin(e[1], DBL_MAX);
+        e[2] = fmin(e[2], DBL_MAX);
+      }
       if (KIND == 2) {
         e[3] = incl ? exp(gb * t0) : 1.0;
         e[4] = incl ? exp(gb * t1) : 1.0;
@@ -1029,6 +1052,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
   double a[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) a[k] = 0.0;
+  uint32_t bmin = 255u, bmax = 0u;
   auto body = [&](uint32_t by, double m) {
     const double* e = tab + by * TW;
     const double om = 1.0 - m;
@@ -1047,10 +1071,19 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
         a[6] += m2;
         a[7] = fma(m, m2, a[7]);
       }
+      bmin = min(bmin, by);
+      bmax = max(bmax, by);
     } else {
       const double gk = e[0], e0 = e[1], e1 = e[2];
       const double mg = m * gk, w2 = om * mg * gk;
-      const double r0 = recip_nr(fma(om, e0, m)), r1 = recip_nr(fma(om, e1, m));
+      double r0, r1;
+      if (KIND == 1) {
+        r0 = recip_nr_finite(fma(om, e0, m));
+        r1 = recip_nr_finite(fma(om, e1, m));
+      } else {
+        r0 = recip_nr(fma(om, e0, m));
+        r1 = recip_nr(fma(om, e1, m));
+      }
       if (KIND == 1) {
         a[0] = fma(mg, r0, a[0]);
         a[2] = fma(mg, r1, a[2]);
@@ -1093,9 +1126,39 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
     tail = i4;
   }
   for (size_t j = tail + threadIdx.x; j < i1; j += kSpaPassBlock) body(row[j], mu[j]);
+  __shared__ uint32_t brange[2];
+  if (KIND == 0) {
+    if (threadIdx.x == 0) {
+      brange[0] = 255u;
+      brange[1] = 0u;
+    }
+    __syncthreads();
+    atomicMin(&brange[0], bmin);
+    atomicMax(&brange[1], bmax);
+  }
   block_sum_t<kSpaPassBlock, K>(a, sh);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < K; ++k) part[((size_t)slot * gridDim.y + blockIdx.y) * kSpaPart + k] = a[k];
+  if (threadIdx.x == 0) {
+    double* o = part + ((size_t)slot * gridDim.y + blockIdx.y) * kSpaPart;
+    if (KIND == 1 && blockIdx.y == 0) {
+      // g~ is increasing in the byte, so e^{-g t} is largest at the smallest or the largest
+      // byte of the variant: K'' is NaN when either overflows (an included byte)
+      const SpaVar& V = vars[slot];
+      for (int r = 0; r < 2; ++r) {
+        const double t = V.st[r].pend;
+        bool nan = false;
+        for (int b : {V.bmin, V.bmax}) {
+          const double gb = (double)b * ld8.scale - g.ag[0];
+          if (fabs(gb) > thr && isinf(exp(-gb * t))) nan = true;
+        }
+        if (nan) a[2 * r + 1] += __longlong_as_double(0x7ff8000000000000ll);
+      }
+    }
+    for (int k = 0; k < K; ++k) o[k] = a[k];
+    if (KIND == 0) {
+      o[8] = (double)brange[0];
+      o[9] = (double)brange[1];
+    }
+  }
 }
 
 __device__ __forceinline__ double slice_sum(const double* __restrict__ part, uint32_t slot, int S,
@@ -1136,6 +1199,13 @@ __global__ void spa_setup_step(GateInfo* __restrict__ gate, const uint32_t* __re
   V.na_mu = slice_sum(part, slot, S, 2);
   V.na_sigma = slice_sum(part, slot, S, 3);
   V.tinit[0] = V.tinit[1] = 0.0;  // fast.rs:217-218
+  double bmn = 255.0, bmx = 0.0;
+  for (int q = 0; q < S; ++q) {
+    bmn = fmin(bmn, part[((size_t)slot * S + q) * kSpaPart + 8]);
+    bmx = fmax(bmx, part[((size_t)slot * S + q) * kSpaPart + 9]);
+  }
+  V.bmin = (int)bmn;
+  V.bmax = (int)bmx;
   vars[slot] = V;
   const double m1 = slice_sum(part, slot, S, 4);
   GateInfo& g = gate[slot];

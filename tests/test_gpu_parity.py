@@ -210,6 +210,30 @@ def test_score_u8_vs_oracle(p):
     _compare(got, want, integer_dosages=False)
 
 
+@pytest.mark.parametrize("prev,p", [(0.03, 1), (0.01, 1), (0.03, 2)])
+def test_score_u8_spa_failed_roots_vs_oracle(prev, p):
+    """Rare variants under extreme case imbalance, where the safeguarded Newton of many SPA
+    roots fails (its iterates run off to where e^{-g t} overflows): flags, counts and
+    p-values against the oracle, for the byte-table passes (intercept only) and the
+    per-sample passes (a covariate)."""
+    n = 4000
+    eta = math.log(prev / (1 - prev))
+    nf = synth.null_fit(n=n, p=p, eta_mean=eta, eta_sd=0.5, seed=3)
+    a = oracle.xvx_inv_xv(nf.x, nf.mu * (1 - nf.mu))
+    q = np.zeros((2000, n), np.uint8)
+    synth.dosage_u8(n, 0, 2000, seed=4, maf=(1e-4, 0.5), out=q)
+    e = sg.ScoreTestEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
+                           spa_pval_cutoff=0.05)
+    got = e.test_markers_u8(q, 1.0 / 127.0)
+    eo = oracle.ScoreEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
+                            spa_cutoff=0.05)
+    want = eo.test_batch_u8(q, 1.0 / 127.0)
+    failed = int((want["spa_converged"] == 0).sum())
+    _say(f"prev={prev} p={p}: {int(want['is_spa'].sum())} SPA'd, {failed} failed roots")
+    assert failed >= 10
+    _compare(got, want, integer_dosages=False)
+
+
 def test_score_f64_missing_and_categorical_vr():
     n = 700
     X, mu, y, a = _null(n, 2, seed=9)
