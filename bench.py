@@ -261,6 +261,10 @@ def main():
     ap.add_argument("--step2-batches", type=int, default=4)
     ap.add_argument("--cpu-markers", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-step2-variants", type=int, default=1024,
+                    help="variants of the step-2 batches the oracle tests on one core")
+    ap.add_argument("--step2-reps", type=int, default=5,
+                    help="passes over the resident step-2 batches inside the timed region")
     ap.add_argument("--cpu-threads", type=int,
                     default=max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "16")))),
                     help="host threads of the CPU baseline (the box's CPU share)")
@@ -410,30 +414,52 @@ def main():
         synth.dosage_u8(n2, 0, nb * B, seed=4, out=q)
         d_q = hip.DeviceBuffer(q.nbytes)
         d_q.upload(q)
+        q_cpu = q[:args.cpu_step2_variants].copy() if not args.no_cpu else None
         del q
         rsize = C.sizeof(sg._lib.VariantResult)
         d_r = hip.DeviceBuffer(nb * B * rsize)
         es = eng.stream
         eng.test_markers_u8_device(d_q.value, n2, B, 1.0 / 127, d_r.value)  # warm
         hip.device_synchronize()
+        reps = args.step2_reps
         s0, s1 = hip.Event(), hip.Event()
         s0.record(es)
-        for b in range(nb):
-            eng.test_markers_u8_device(d_q.value + b * B * n2, n2, B, 1.0 / 127,
-                                       d_r.value + b * B * rsize)
+        for _ in range(reps):
+            for b in range(nb):
+                eng.test_markers_u8_device(d_q.value + b * B * n2, n2, B, 1.0 / 127,
+                                           d_r.value + b * B * rsize)
         s1.record(es)
-        ms = s0.elapsed_ms(s1)
+        ms = s0.elapsed_ms(s1) / reps
         out = (sg._lib.VariantResult * (nb * B))()
         host = np.empty(nb * B * rsize, np.uint8)
         d_r.download(host)
         C.memmove(C.addressof(out), host.ctypes.data, host.nbytes)
         r = sg.results_to_dict(out)
         vps = nb * B / (ms * 1e-3)
-        step2 = dict(n=n2, variants=nb * B, ms=ms, variants_per_s=vps,
+        step2 = dict(n=n2, variants=nb * B, ms=ms, repetitions=reps, variants_per_s=vps,
                      spa_fraction=float(r["is_spa"].mean()),
                      dosage_gbs=nb * B * n2 / (ms * 1e-3) / 1e9)
-        log(f"step2: {nb * B} variants in {ms:.1f} ms -> {vps:.3g} variants/s, "
+        log(f"step2: {nb * B} variants in {ms:.2f} ms (mean of {reps}) -> {vps:.3g} variants/s, "
             f"SPA'd {r['is_spa'].mean() * 100:.2f}%")
+        if q_cpu is not None:
+            # the oracle's score test + fast SPA (saige-rs single_variant.rs / fast.rs
+            # restated) on one core, as saige-rs runs step 2 (assoc_test.rs:165), on the
+            # first variants of the same batches; its p-values also check the GPU's
+            import oracle
+            eo = oracle.ScoreEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
+                                    spa_cutoff=cutoff)
+            t = time.perf_counter()
+            want = eo.test_batch_u8(q_cpu, 1.0 / 127)
+            dt = time.perf_counter() - t
+            nv = len(q_cpu)
+            got_p = r["pvalue"][:nv]
+            m = want["pvalue"] >= 1e-300
+            dlp = float(np.max(np.abs(np.log10(got_p[m]) - np.log10(want["pvalue"][m]))))
+            step2["cpu_variants_per_s"] = nv / dt
+            step2["max_abs_dlog10p_vs_oracle"] = dlp
+            step2["is_spa_mismatches_vs_oracle"] = int(np.sum(r["is_spa"][:nv] != want["is_spa"]))
+            log(f"step2 cpu: {nv} variants in {dt:.2f} s on 1 thread -> {nv / dt:.1f} variants/s; "
+                f"GPU vs oracle max |d log10 p| = {dlp:.2e}")
 
     # ---- CPU baseline: the oracle's f32 OnTheFlyGrm product (saige-rs restated) on the
     # host cores, split over the thread pool the way rayon splits it, and on one core ----
@@ -467,6 +493,13 @@ def main():
                          f"{p1:.2f} s on 1 thread (fit-null {p1 * ref['grm_products']:.0f} s)"}
         log(f"cpu baseline: {pt:.3f} s/product on {args.cpu_threads} threads, {p1:.2f} s on 1 -> "
             f"{cpu_fit:.0f} s fit-null (extrapolated)")
+        if step2 is not None and "cpu_variants_per_s" in step2:
+            cpu["step2"] = {"value": step2["cpu_variants_per_s"], "unit": "variants/s",
+                            "cores": 1, "kind": "port",
+                            "sample": f"oracle score test + fast SPA of the first "
+                                      f"{args.cpu_step2_variants} variants of the step-2 "
+                                      f"batches (n={step2['n']}), single-threaded as saige-rs "
+                                      f"runs step 2"}
 
     line = {
         "metric": METRIC,
