@@ -143,54 +143,69 @@ __global__ void fill_kernel(double* __restrict__ a, size_t n, double v) {
 static inline dim3 grid1(size_t n) { return dim3((unsigned)ceil_div(n, 256)); }
 static inline dim3 grid2(size_t n, size_t k) { return dim3((unsigned)ceil_div(n, 256), (unsigned)k); }
 
-// IRLS for the GLM start (ai_reml.rs:431-482), host fp64 in the reference's order.
-static bool fit_glm_irls(const double* y, const double* x, size_t n, size_t p, int trait,
-                         int max_iter, std::vector<double>& alpha) {
-  std::vector<double> mu(n), eta(n), w(n), z(n), xw(p * p), xz(p), an(p), l;
-  for (size_t i = 0; i < n; ++i) mu[i] = (trait == SAIGE_TRAIT_QUANTITATIVE) ? y[i] : (y[i] + 0.5) / 2.0;
-  for (size_t i = 0; i < n; ++i)
-    eta[i] = (trait == SAIGE_TRAIT_QUANTITATIVE) ? mu[i] : std::log(mu[i] / (1.0 - mu[i]));
+// IRLS working weights and response (ai_reml.rs:441-447) from the current (eta, mu):
+// column j < p of out is w x_j and column p is w z.  With alpha null (eta, mu) are the
+// start values from y (:436-439), otherwise eta = X alpha and mu its clamped inverse logit
+// (:466-474).
+__global__ void irls_prep_kernel(const double* __restrict__ x, size_t ldx, size_t p,
+                                 const double* __restrict__ y, const double* __restrict__ alpha,
+                                 int trait, size_t n, double* __restrict__ out, size_t ldo) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool quant = trait == SAIGE_TRAIT_QUANTITATIVE;
+  double eta, mu;
+  if (!alpha) {
+    mu = quant ? y[i] : (y[i] + 0.5) / 2.0;
+    eta = quant ? mu : log(mu / (1.0 - mu));
+  } else {
+    eta = 0.0;
+    for (size_t j = 0; j < p; ++j) eta += x[i + j * ldx] * alpha[j];
+    mu = quant ? eta : fmin(fmax(1.0 / (1.0 + exp(-eta)), 1e-10), 1.0 - 1e-10);
+  }
+  const double v = quant ? 1.0 : mu * (1.0 - mu);
+  const double w = v > 1e-30 ? v : 1e-30;
+  const double z = eta + (y[i] - mu) / fmax(w, 1e-30);
+  for (size_t j = 0; j < p; ++j) out[i + j * ldo] = w * x[i + j * ldx];
+  out[i + p * ldo] = w * z;
+}
+
+// IRLS for the GLM start (ai_reml.rs:431-482) on the device: per iteration one kernel for
+// w and z, the deterministic Gram X'[WX | Wz], and the p x p Cholesky solve on the host.
+static int fit_glm_irls_dev(hipStream_t s, const double* dX, size_t ldx, const double* dY,
+                            size_t n, size_t p, int trait, int max_iter,
+                            std::vector<double>& alpha, DevBuf& tmp, DevBuf& coef, DevBuf& part,
+                            DevBuf& res) {
+  const size_t ld = round_up(n, 64);
+  SAIGE_TRY(tmp.ensure(ld * (p + 1) * 8));
+  SAIGE_TRY(coef.ensure(std::max<size_t>(p, 8) * 8));
+  std::vector<double> G(p * (p + 1)), xw(p * p), xz(p), an(p), l;
   alpha.assign(p, 0.0);
   for (int it = 0; it < max_iter; ++it) {
-    for (size_t i = 0; i < n; ++i) {
-      double v = (trait == SAIGE_TRAIT_QUANTITATIVE) ? 1.0 : mu[i] * (1.0 - mu[i]);
-      w[i] = v > 1e-30 ? v : 1e-30;
-      z[i] = eta[i] + (y[i] - mu[i]) / std::max(w[i], 1e-30);
-    }
+    irls_prep_kernel<<<dim3((unsigned)ceil_div(n, 256)), 256, 0, s>>>(
+        dX, ldx, p, dY, it == 0 ? nullptr : coef.as<double>(), trait, n, tmp.as<double>(), ld);
+    SAIGE_HIP(hipGetLastError());
+    SAIGE_TRY(gram(s, dX, ldx, p, tmp.as<double>(), ld, p + 1, n, G.data(), part, res));
     for (size_t j = 0; j < p; ++j)
       for (size_t k = j; k < p; ++k) {
-        double s = 0.0;
-        for (size_t i = 0; i < n; ++i) s += x[j * n + i] * w[i] * x[k * n + i];
-        xw[k * p + j] = s;
-        xw[j * p + k] = s;
+        xw[k * p + j] = G[j + p * k];  // sum_i x_j (w x_k)
+        xw[j * p + k] = G[j + p * k];
       }
-    for (size_t j = 0; j < p; ++j) {
-      double s = 0.0;
-      for (size_t i = 0; i < n; ++i) s += x[j * n + i] * w[i] * z[i];
-      xz[j] = s;
-    }
+    for (size_t j = 0; j < p; ++j) xz[j] = G[j + p * p];
     if (!chol(xw, p, l)) {
       for (size_t i = 0; i < p; ++i) xw[i * p + i] += 1e-6;
-      if (!chol(xw, p, l)) return false;
-    }
-    chol_solve(l, p, xz.data(), an.data());
-    for (size_t i = 0; i < n; ++i) eta[i] = 0.0;
-    for (size_t j = 0; j < p; ++j)
-      for (size_t i = 0; i < n; ++i) eta[i] += x[j * n + i] * an[j];
-    for (size_t i = 0; i < n; ++i) {
-      if (trait == SAIGE_TRAIT_QUANTITATIVE) {
-        mu[i] = eta[i];
-      } else {
-        double m = 1.0 / (1.0 + std::exp(-eta[i]));
-        mu[i] = std::min(std::max(m, 1e-10), 1.0 - 1e-10);
+      if (!chol(xw, p, l)) {
+        set_error("GLM IRLS: X'WX not positive definite");
+        return SAIGE_ERR_NOT_PD;
       }
     }
+    chol_solve(l, p, xz.data(), an.data());
     double change = 0.0;
     for (size_t j = 0; j < p; ++j) change += std::fabs(an[j] - alpha[j]);
     alpha = an;
+    SAIGE_HIP(hipMemcpyAsync(coef.p, alpha.data(), p * 8, hipMemcpyHostToDevice, s));
     if (change < 1e-8) break;
   }
-  return true;
+  return SAIGE_OK;
 }
 
 struct ChaChaKey {
@@ -215,6 +230,7 @@ __global__ void rademacher_kernel(ChaChaKey key, size_t n, size_t R, double* __r
 
 struct RemlState {
   DevBuf x, y, eta, mu, w, rhs1, sol1, gu, py, gpy, pu, rhs2, sol2, tmp, coef, part, res, ones;
+  DevBuf glm, xinv, a_flat, at;  // IRLS work columns; (X'WX)^-1 and X'W-side products
   PcgWork pcg;
 };
 
@@ -231,10 +247,6 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
   const uint64_t prod0 = sink->columns;
 
   std::vector<double> alpha;
-  if (!fit_glm_irls(y, x, n, p, trait, 25, alpha)) {
-    set_error("GLM IRLS: X'WX not positive definite");
-    return SAIGE_ERR_NOT_PD;
-  }
   SAIGE_TRY(st.x.ensure(ld * p * 8));
   SAIGE_TRY(st.y.ensure(n * 8));
   SAIGE_TRY(st.eta.ensure(n * 8));
@@ -268,6 +280,8 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
 
   SAIGE_HIP(hipMemcpy2DAsync(dX, ld * 8, x, n * 8, n * 8, p, hipMemcpyHostToDevice, s));
   SAIGE_HIP(hipMemcpyAsync(dY, y, n * 8, hipMemcpyHostToDevice, s));
+  SAIGE_TRY(fit_glm_irls_dev(s, dX, ld, dY, n, p, trait, 25, alpha, st.glm, st.coef, st.part,
+                             st.res));
   // X columns and the Rademacher probes are fixed right-hand sides (ai_reml.rs:121-130).
   SAIGE_HIP(hipMemcpy2DAsync(RHS1 + ld, ld * 8, x, n * 8, n * 8, p, hipMemcpyHostToDevice, s));
   if (R > 0) {
@@ -491,6 +505,78 @@ static int xvx_inv_xv_host(const double* x, size_t n, size_t p, const double* w,
   return SAIGE_OK;
 }
 
+// (X'WX)^-1 X'W from the final W (fit_null.rs:289-307) on the device, in the flat layout
+// the reference fills: a_flat[j*n + i] = sum_k inv[k*p + j] x_ik w_i.
+__global__ void xvx_apply_kernel(const double* __restrict__ x, size_t ldx, size_t p,
+                                 const double* __restrict__ w, const double* __restrict__ inv,
+                                 size_t n, double* __restrict__ a_flat) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (size_t j = 0; j < p; ++j) {
+    double v = 0.0;
+    for (size_t k = 0; k < p; ++k) v += inv[k * p + j] * x[i + k * ldx] * w[i];
+    a_flat[j * n + i] = v;
+  }
+}
+
+// A^T as n x p col-major from the flat layout read as the col-major p x n view (the view
+// estimate_variance_ratio takes, variance_ratio.rs:144-151): at[r*ld + i] = a_flat[i*p + r].
+__global__ void at_gather_kernel(const double* __restrict__ a_flat, size_t n, size_t p,
+                                 double* __restrict__ at, size_t ld) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t r = blockIdx.y;
+  if (i >= n || r >= p) return;
+  at[r * ld + i] = a_flat[i * p + r];
+}
+
+// columns j < p: w x_j
+__global__ void wx_kernel(const double* __restrict__ x, size_t ldx, size_t p,
+                          const double* __restrict__ w, size_t n, double* __restrict__ out,
+                          size_t ldo) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t j = blockIdx.y;
+  if (i >= n || j >= p) return;
+  out[i + j * ldo] = w[i] * x[i + j * ldx];
+}
+
+// st.a_flat and st.at from the device X and W; X'WX comes from the deterministic Gram, its
+// inverse from the p x p Cholesky on the host.
+static int xvx_inv_xv_dev(hipStream_t s, const double* dX, size_t ldx, const double* dW,
+                          size_t n, size_t p, RemlState& st) {
+  const size_t ld = round_up(n, 64);
+  SAIGE_TRY(st.glm.ensure(ld * (p + 1) * 8));
+  SAIGE_TRY(st.xinv.ensure(p * p * 8));
+  SAIGE_TRY(st.a_flat.ensure(n * p * 8));
+  SAIGE_TRY(st.at.ensure(ld * p * 8));
+  wx_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)p), 256, 0, s>>>(dX, ldx, p, dW, n,
+                                                                           st.glm.as<double>(), ld);
+  SAIGE_HIP(hipGetLastError());
+  std::vector<double> G(p * p), xvx(p * p), l, inv(p * p), e(p), col(p);
+  SAIGE_TRY(gram(s, dX, ldx, p, st.glm.as<double>(), ld, p, n, G.data(), st.part, st.res));
+  for (size_t j = 0; j < p; ++j)
+    for (size_t k = j; k < p; ++k) {
+      xvx[k * p + j] = G[j + p * k];
+      xvx[j * p + k] = G[j + p * k];
+    }
+  if (!chol(xvx, p, l)) {
+    set_error("X'WX is not positive definite");
+    return SAIGE_ERR_NOT_PD;
+  }
+  for (size_t j = 0; j < p; ++j) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[j] = 1.0;
+    chol_solve(l, p, e.data(), col.data());
+    for (size_t i = 0; i < p; ++i) inv[j * p + i] = col[i];
+  }
+  SAIGE_HIP(hipMemcpyAsync(st.xinv.p, inv.data(), p * p * 8, hipMemcpyHostToDevice, s));
+  xvx_apply_kernel<<<dim3((unsigned)ceil_div(n, 256)), 256, 0, s>>>(
+      dX, ldx, p, dW, st.xinv.as<double>(), n, st.a_flat.as<double>());
+  at_gather_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)p), 256, 0, s>>>(
+      st.a_flat.as<double>(), n, p, st.at.as<double>(), ld);
+  SAIGE_HIP(hipGetLastError());
+  return SAIGE_OK;
+}
+
 // ---------------------------------------------------------------------------------
 // Variance ratio (variance_ratio.rs:87-246).
 // ---------------------------------------------------------------------------------
@@ -576,7 +662,7 @@ static int vr_fill(hipStream_t s, const VrSource& src, size_t n, const std::vect
 static int estimate_vr_dev(saige_grm* h, ProductSink* sink, const VrSource& src,
                            const double* mac, size_t m,
                            const double* d_mu, const double* d_w, const double tau[2], int trait,
-                           const double* d_x, size_t ldx, const double* a_flat, size_t p,
+                           const double* d_x, size_t ldx, const double* d_at, size_t p,
                            const saige_vr_config* cfg, double* vr_out, uint32_t* n_used,
                            double* per_marker, RemlState& st) {
   hipStream_t s = h->stream;
@@ -586,16 +672,9 @@ static int estimate_vr_dev(saige_grm* h, ProductSink* sink, const VrSource& src,
   std::vector<uint32_t> order(m);
   std::iota(order.begin(), order.end(), 0u);
   rng.shuffle(order);
-  // A^T as n x p col-major: element (r, i) of the col-major p x n view is a_flat[i*p + r]
-  DevBuf d_at, G, GT, SG, WG, acb, agb, flipb, ids, onesb;
-  SAIGE_TRY(d_at.ensure(ld * p * 8));
-  {
-    std::vector<double> at(n * p);
-    for (size_t i = 0; i < n; ++i)
-      for (size_t r = 0; r < p; ++r) at[r * n + i] = a_flat[i * p + r];
-    SAIGE_HIP(hipMemcpy2DAsync(d_at.p, ld * 8, at.data(), n * 8, n * 8, p, hipMemcpyHostToDevice, s));
-    SAIGE_HIP(hipStreamSynchronize(s));
-  }
+  // d_at: A^T as n x p col-major (leading dimension ld), element (r, i) of the col-major
+  // p x n view of the flat A
+  DevBuf G, GT, SG, WG, acb, agb, flipb, ids, onesb;
   SAIGE_TRY(onesb.ensure(ld * 8));
   fill_kernel<<<grid1(n), 256, 0, s>>>(onesb.as<double>(), n, 1.0);
   std::vector<double> vr;
@@ -628,7 +707,7 @@ static int estimate_vr_dev(saige_grm* h, ProductSink* sink, const VrSource& src,
     SAIGE_HIP(hipMemcpyAsync(flipb.p, flip.data(), k * 4, hipMemcpyHostToDevice, s));
     flip_kernel<<<grid2(n, k), 256, 0, s>>>(dG, ld, n, k, flipb.as<int>());
     SAIGE_TRY(col_dots(s, dG, ld, onesb.as<double>(), 0, n, k, acs.data(), st.part, st.res));
-    SAIGE_TRY(gram(s, d_at.as<double>(), ld, p, dG, ld, k, n, ag.data(), st.part, st.res));
+    SAIGE_TRY(gram(s, d_at, ld, p, dG, ld, k, n, ag.data(), st.part, st.res));
     SAIGE_HIP(hipMemcpyAsync(agb.p, ag.data(), p * k * 8, hipMemcpyHostToDevice, s));
     adjust_kernel<<<grid2(n, k), 256, 0, s>>>(dG, d_x, ldx, p, agb.as<double>(), n, k, GT.as<double>());
     SAIGE_HIP(hipGetLastError());
@@ -735,9 +814,16 @@ int saige_estimate_variance_ratio(saige_grm* h, const double* dosages, const dou
     set_error("No markers available for variance ratio estimation");
     return SAIGE_ERR_NO_VR_MARKERS;
   }
+  // A^T as n x p col-major: element (r, i) of the col-major p x n view is xvx_inv_xv[i*p + r]
+  DevBuf d_at;
+  SAIGE_TRY(d_at.ensure(ld * p * 8));
+  std::vector<double> at(n * p);
+  for (size_t i = 0; i < n; ++i)
+    for (size_t r = 0; r < p; ++r) at[r * n + i] = xvx_inv_xv[i * p + r];
+  SAIGE_HIP(hipMemcpy2DAsync(d_at.p, ld * 8, at.data(), n * 8, n * 8, p, hipMemcpyHostToDevice, s));
   DirectSink sink(h);
   return estimate_vr_dev(h, &sink, src, mac, m, dmu.as<double>(), dw.as<double>(), tau, trait_type,
-                         dx.as<double>(), ld, xvx_inv_xv, p, cfg, variance_ratio, n_used,
+                         dx.as<double>(), ld, d_at.as<double>(), p, cfg, variance_ratio, n_used,
                          per_marker_vr, st);
 }
 
@@ -827,11 +913,10 @@ static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const dou
   auto t1 = clk::now();
   SAIGE_TRY(fit_ai_reml_dev(h, sink, y, x, n, p, trait_type, &cfg->reml, &result->reml, st));
   auto t2 = clk::now();
-  std::vector<double> hm(n), hw(n), a(p * n);
-  SAIGE_HIP(hipMemcpyAsync(hm.data(), st.mu.p, n * 8, hipMemcpyDeviceToHost, s));
-  SAIGE_HIP(hipMemcpyAsync(hw.data(), st.w.p, n * 8, hipMemcpyDeviceToHost, s));
-  SAIGE_HIP(hipStreamSynchronize(s));
-  SAIGE_TRY(xvx_inv_xv_host(x, n, p, hw.data(), a.data()));
+  SAIGE_TRY(xvx_inv_xv_dev(s, st.x.as<double>(), ld, st.w.as<double>(), n, p, st));
+  if (mu_out) SAIGE_HIP(hipMemcpyAsync(mu_out, st.mu.p, n * 8, hipMemcpyDeviceToHost, s));
+  if (xvx_inv_xv)
+    SAIGE_HIP(hipMemcpyAsync(xvx_inv_xv, st.a_flat.p, n * p * 8, hipMemcpyDeviceToHost, s));
   double vr = 1.0;
   uint32_t used = 0;
   if (!ms.vr_ids.empty()) {
@@ -841,7 +926,8 @@ static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const dou
     saige_vr_config vc{cfg->n_markers_vr, 20.0, 0.001, cfg->reml.seed};
     SAIGE_TRY(estimate_vr_dev(h, sink, src, ms.vr_mac.data(), ms.vr_ids.size(),
                               st.mu.as<double>(), st.w.as<double>(), result->reml.tau, trait_type,
-                              st.x.as<double>(), ld, a.data(), p, &vc, &vr, &used, nullptr, st));
+                              st.x.as<double>(), ld, st.at.as<double>(), p, &vc, &vr, &used,
+                              nullptr, st));
   }
   auto t3 = clk::now();
   result->variance_ratio = vr;
@@ -851,8 +937,7 @@ static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const dou
   result->seconds_reml = std::chrono::duration<double>(t2 - t1).count();
   result->seconds_vr = std::chrono::duration<double>(t3 - t2).count();
   result->reml.grm_products = sink->columns;
-  if (mu_out) std::copy(hm.begin(), hm.end(), mu_out);
-  if (xvx_inv_xv) std::copy(a.begin(), a.end(), xvx_inv_xv);
+  SAIGE_HIP(hipStreamSynchronize(s));  // mu_out and xvx_inv_xv copies
   return SAIGE_OK;
 }
 
