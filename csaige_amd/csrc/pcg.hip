@@ -168,14 +168,13 @@ __global__ __launch_bounds__(kDotBlock) void pcg_update_kernel(
 }
 
 // beta, new r'z and ||r|| per active column, the stop test of the next iteration, and the
-// number of columns still active (one block of 1024 threads covers all columns).
+// columns still active: n_active[0] is their number, n_active[1..] their indices in
+// increasing order (one block of 1024 threads covers all columns).
 __global__ __launch_bounds__(1024) void pcg_beta_fin(const double* __restrict__ part,
                                                      size_t nchunks, size_t k, int iter,
                                                      ColState* __restrict__ st,
                                                      int* __restrict__ n_active) {
-  __shared__ int cnt;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
+  __shared__ int flag[1024];
   const size_t c = threadIdx.x;
   if (c < k && st[c].active) {
     double rz = 0.0, rr = 0.0;
@@ -192,9 +191,27 @@ __global__ __launch_bounds__(1024) void pcg_beta_fin(const double* __restrict__ 
       st[c].iters = iter + 1;
     }
   }
-  if (c < k && st[c].active) atomicAdd(&cnt, 1);
+  flag[threadIdx.x] = (c < k && st[c].active) ? 1 : 0;
   __syncthreads();
-  if (threadIdx.x == 0) *n_active = cnt;
+  if (threadIdx.x == 0) {
+    int cnt = 0;
+    for (size_t j = 0; j < k; ++j)
+      if (flag[j]) n_active[1 + cnt++] = (int)j;
+    n_active[0] = cnt;
+  }
+}
+
+// Active columns gathered into a contiguous block, and the product of the block scattered
+// back: out[:, j] = in[:, idx[j]] (gather) or out[:, idx[j]] = in[:, j] (scatter).
+__global__ void pcg_gather_kernel(const double* __restrict__ in, size_t ld, size_t n,
+                                  const int* __restrict__ idx, int scatter,
+                                  double* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t j = blockIdx.y;
+  if (i >= n) return;
+  const size_t c = (size_t)idx[j];
+  if (scatter) out[i + c * ld] = in[i + j * ld];
+  else out[i + j * ld] = in[i + c * ld];
 }
 
 __global__ void pcg_p_kernel(double* __restrict__ p, const double* __restrict__ z, size_t ld,
@@ -222,15 +239,18 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   const size_t ld = round_up(n, 64);
   hipStream_t s = h->stream;
   const size_t nchunks = std::max<size_t>(1, ceil_div(n, kDotChunk));
-  SAIGE_TRY(ws.vec.ensure(4 * ld * k * 8));
+  SAIGE_TRY(ws.vec.ensure(6 * ld * k * 8));
   SAIGE_TRY(ws.part.ensure(4 * k * nchunks * 8));
-  SAIGE_TRY(ws.scal.ensure(k * sizeof(ColState) + 64));
+  const size_t list_bytes = round_up((k + 1) * sizeof(int), 256);
+  SAIGE_TRY(ws.scal.ensure(list_bytes + k * sizeof(ColState)));
   double* R = ws.vec.as<double>();
   double* Z = R + ld * k;
   double* P = Z + ld * k;
   double* AP = P + ld * k;
-  ColState* st = reinterpret_cast<ColState*>(ws.scal.as<char>() + 64);
-  int* d_nact = ws.scal.as<int>();
+  double* PC = AP + ld * k;   // active columns of P, gathered
+  double* APC = PC + ld * k;  // their product
+  ColState* st = reinterpret_cast<ColState*>(ws.scal.as<char>() + list_bytes);
+  int* d_nact = ws.scal.as<int>();  // [count, active column indices]
   double* part = ws.part.as<double>();
   double* pap_part = part + 3 * k * nchunks;  // p'Ap partials, apart from the update's
   const double tau0 = tau[0], tau1 = tau[1];
@@ -242,24 +262,37 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   pcg_init_fin<<<kb, 256, 0, s>>>(part, nchunks, k, tol, st);
   SAIGE_HIP(hipGetLastError());
   int n_active = 0;
+  std::vector<int> list(k + 1);
   {
     std::vector<ColState> hs(k);
     SAIGE_HIP(hipMemcpyAsync(hs.data(), st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
-    for (size_t c = 0; c < k; ++c) n_active += hs[c].active;
+    for (size_t c = 0; c < k; ++c)
+      if (hs[c].active) list[1 + n_active++] = (int)c;
+    list[0] = n_active;
+    if (n_active > 0 && (size_t)n_active < k)
+      SAIGE_HIP(hipMemcpyAsync(d_nact, list.data(), (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
   }
-  int pinned_nact = 0;
   for (uint32_t iter = 0; iter < max_iter && n_active > 0; ++iter) {
-    SAIGE_TRY(sink->product(P, ld, k, AP, ld, d_w, tau));
+    if ((size_t)n_active == k) {
+      SAIGE_TRY(sink->product(P, ld, k, AP, ld, d_w, tau));
+    } else {
+      // only the columns still iterating: their results do not depend on which other
+      // columns share the product
+      const dim3 gg((unsigned)ceil_div(n, 256), (unsigned)n_active);
+      pcg_gather_kernel<<<gg, 256, 0, s>>>(P, ld, n, d_nact + 1, 0, PC);
+      SAIGE_TRY(sink->product(PC, ld, (size_t)n_active, APC, ld, d_w, tau));
+      pcg_gather_kernel<<<gg, 256, 0, s>>>(APC, ld, n, d_nact + 1, 1, AP);
+    }
     pcg_pap_kernel<<<gchunk, kDotBlock, 0, s>>>(P, AP, ld, n, nchunks, st, pap_part);
     pcg_update_kernel<<<gchunk, kDotBlock, 0, s>>>(d_x, ldx, R, Z, P, AP, ld, d_w, tau0, tau1, n,
                                                    nchunks, pap_part, (int)iter, st, part);
     pcg_beta_fin<<<1, 1024, 0, s>>>(part, nchunks, k, (int)iter, st, d_nact);
     pcg_p_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(P, Z, ld, n, st);
     SAIGE_HIP(hipGetLastError());
-    SAIGE_HIP(hipMemcpyAsync(&pinned_nact, d_nact, sizeof(int), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipMemcpyAsync(list.data(), d_nact, (k + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
-    n_active = pinned_nact;
+    n_active = list[0];
   }
   pcg_final<<<kb, 256, 0, s>>>(k, max_iter, st);
   SAIGE_HIP(hipGetLastError());
