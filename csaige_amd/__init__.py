@@ -222,21 +222,25 @@ def fit_null(bed_body, bytes_per_marker, n_total, m_total, y, x_cm, sample_idx=N
     mu = np.empty(n)
     a = np.empty(p * n)
     cls = np.empty(m_total, np.uint8)
+    alpha = np.empty(p)
+    pm = np.empty(max(m_total, 1))
     args = (bytes_per_marker, n_total, m_total, sp, n, _p(y, _lib.dp), _p(x, _lib.dp), p, trait,
-            C.byref(cfg), C.byref(r), _p(mu, _lib.dp), _p(a, _lib.dp), _p(cls, _lib.u8p))
+            C.byref(cfg), C.byref(r), _p(mu, _lib.dp), _p(a, _lib.dp), _p(cls, _lib.u8p),
+            _p(alpha, _lib.dp), _p(pm, _lib.dp))
     if d_bed is None:
         check(L.saige_fit_null(device, _p(body, _lib.u8p), *args))
     else:
         check(L.saige_fit_null_device(device, C.c_void_p(d_bed), *args))
-    return _fit_null_dict(r, mu, a, cls)
+    return _fit_null_dict(r, mu, a, cls, alpha, pm)
 
 
-def _fit_null_dict(r, mu, a, cls):
+def _fit_null_dict(r, mu, a, cls, alpha, pm):
     rr = r.reml
     return dict(tau=np.array(rr.tau[:]), iterations=rr.iterations, converged=bool(rr.converged),
                 log_likelihood=rr.log_likelihood, grm_products=rr.grm_products,
                 pcg_iterations=rr.pcg_iterations, pcg_not_converged=rr.pcg_not_converged,
                 vr=r.variance_ratio, vr_markers_used=r.vr_markers_used,
+                per_marker_vr=pm[:r.vr_markers_used].copy(), alpha=alpha,
                 n_grm_markers=r.n_grm_markers, n_vr_markers=r.n_vr_markers, mu=mu,
                 xvx_inv_xv=a, marker_class=cls, seconds_qc=r.seconds_qc,
                 seconds_reml=r.seconds_reml, seconds_vr=r.seconds_vr,
@@ -267,14 +271,16 @@ def fit_null_multi(bed_body, bytes_per_marker, n_total, m_total, Y, x_cm, traits
     mu = np.empty((T, n))
     a = np.empty((T, p * n))
     cls = np.empty(m_total, np.uint8)
+    alpha = np.empty((T, p))
+    pm = np.empty((T, max(m_total, 1)))
     args = (bytes_per_marker, n_total, m_total, sp, n, _p(Y, _lib.dp), T, _p(x, _lib.dp), p,
             tt.ctypes.data_as(C.POINTER(C.c_int)), C.byref(cfg), res, _p(mu, _lib.dp),
-            _p(a, _lib.dp), _p(cls, _lib.u8p))
+            _p(a, _lib.dp), _p(cls, _lib.u8p), _p(alpha, _lib.dp), _p(pm, _lib.dp))
     if d_bed is None:
         check(L.saige_fit_null_multi(device, _p(body, _lib.u8p), *args))
     else:
         check(L.saige_fit_null_multi_device(device, C.c_void_p(d_bed), *args))
-    return [_fit_null_dict(res[t], mu[t], a[t], cls) for t in range(T)]
+    return [_fit_null_dict(res[t], mu[t], a[t], cls, alpha[t], pm[t]) for t in range(T)]
 
 
 def results_to_dict(arr):

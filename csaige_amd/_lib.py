@@ -118,16 +118,16 @@ SIGNATURES = [
                                                 C.POINTER(VrConfig), dp, u32p, dp]),
     ("saige_fit_null", C.c_int, [C.c_int, u8p, sz, sz, sz, u32p, sz, dp, dp, sz, C.c_int,
                                  C.POINTER(FitNullConfig), C.POINTER(FitNullResult), dp, dp,
-                                 u8p]),
+                                 u8p, dp, dp]),
     ("saige_fit_null_device", C.c_int, [C.c_int, vp, sz, sz, sz, u32p, sz, dp, dp, sz, C.c_int,
                                         C.POINTER(FitNullConfig), C.POINTER(FitNullResult), dp,
-                                        dp, u8p]),
+                                        dp, u8p, dp, dp]),
     ("saige_fit_null_multi", C.c_int, [C.c_int, u8p, sz, sz, sz, u32p, sz, dp, sz, dp, sz,
                                        C.POINTER(C.c_int), C.POINTER(FitNullConfig),
-                                       C.POINTER(FitNullResult), dp, dp, u8p]),
+                                       C.POINTER(FitNullResult), dp, dp, u8p, dp, dp]),
     ("saige_fit_null_multi_device", C.c_int, [C.c_int, vp, sz, sz, sz, u32p, sz, dp, sz, dp, sz,
                                               C.POINTER(C.c_int), C.POINTER(FitNullConfig),
-                                              C.POINTER(FitNullResult), dp, dp, u8p]),
+                                              C.POINTER(FitNullResult), dp, dp, u8p, dp, dp]),
     ("saige_engine_create", C.c_int, [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, sz,
                                       sz, C.c_double, dp, dp, sz, C.c_int, C.c_int, C.c_double,
                                       C.c_double, dp, C.POINTER(vp)]),

@@ -904,7 +904,7 @@ static int qc_split(int device, const uint8_t* bed_body, bool on_device, size_t 
 static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const double* x, size_t p,
                      int trait_type, const saige_fit_null_config* cfg, const MarkerSplit& ms,
                      const uint8_t* vr_rows, saige_fit_null_result* result, double* mu_out,
-                     double* xvx_inv_xv) {
+                     double* xvx_inv_xv, double* alpha_out, double* per_marker_vr) {
   using clk = std::chrono::steady_clock;
   const size_t n = h->g.n;
   hipStream_t s = h->stream;
@@ -917,6 +917,7 @@ static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const dou
   if (mu_out) SAIGE_HIP(hipMemcpyAsync(mu_out, st.mu.p, n * 8, hipMemcpyDeviceToHost, s));
   if (xvx_inv_xv)
     SAIGE_HIP(hipMemcpyAsync(xvx_inv_xv, st.a_flat.p, n * p * 8, hipMemcpyDeviceToHost, s));
+  if (alpha_out) SAIGE_HIP(hipMemcpyAsync(alpha_out, st.coef.p, p * 8, hipMemcpyDeviceToHost, s));
   double vr = 1.0;
   uint32_t used = 0;
   if (!ms.vr_ids.empty()) {
@@ -927,7 +928,7 @@ static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const dou
     SAIGE_TRY(estimate_vr_dev(h, sink, src, ms.vr_mac.data(), ms.vr_ids.size(),
                               st.mu.as<double>(), st.w.as<double>(), result->reml.tau, trait_type,
                               st.x.as<double>(), ld, st.at.as<double>(), p, &vc, &vr, &used,
-                              nullptr, st));
+                              per_marker_vr, st));
   }
   auto t3 = clk::now();
   result->variance_ratio = vr;
@@ -965,7 +966,8 @@ static int fit_null_impl(int device, const uint8_t* bed_body, bool on_device, si
                          size_t n_total, size_t m_total, const uint32_t* sample_idx, size_t n,
                          const double* y, const double* x, size_t p, int trait_type,
                          const saige_fit_null_config* cfg, saige_fit_null_result* result,
-                         double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
+                         double* mu_out, double* xvx_inv_xv, uint8_t* marker_class,
+                         double* alpha_out, double* per_marker_vr) {
   SAIGE_ARG(bed_body && y && x && cfg && result, "null pointer");
   SAIGE_ARG(p >= 1 && n > 0, "dimensions");
   using clk = std::chrono::steady_clock;
@@ -987,7 +989,7 @@ static int fit_null_impl(int device, const uint8_t* bed_body, bool on_device, si
     DirectSink sink(h);
     if (rc == SAIGE_OK)
       rc = fit_trait(h, &sink, y, x, p, trait_type, cfg, ms, rows.as<uint8_t>(), result, mu_out,
-                     xvx_inv_xv);
+                     xvx_inv_xv, alpha_out, per_marker_vr);
   }
   saige_grm_destroy(h);
   if (rc == SAIGE_OK) {
@@ -1130,7 +1132,8 @@ static int fit_null_multi_impl(int device, const uint8_t* bed_body, bool on_devi
                                size_t n, const double* y, size_t n_traits, const double* x,
                                size_t p, const int* trait_types, const saige_fit_null_config* cfg,
                                saige_fit_null_result* results, double* mu_out,
-                               double* xvx_inv_xv, uint8_t* marker_class) {
+                               double* xvx_inv_xv, uint8_t* marker_class, double* alpha_out,
+                               double* per_marker_vr) {
   SAIGE_ARG(bed_body && y && x && cfg && results && trait_types, "null pointer");
   SAIGE_ARG(p >= 1 && n > 0 && n_traits >= 1 && n_traits <= 64, "dimensions");
   for (size_t t = 0; t < n_traits; ++t)
@@ -1163,7 +1166,9 @@ static int fit_null_multi_impl(int device, const uint8_t* bed_body, bool on_devi
           if (r == SAIGE_OK)
             r = fit_trait(h, &sink, y + t * n, x, p, trait_types[t], cfg, ms, rows.as<uint8_t>(),
                           &results[t], mu_out ? mu_out + t * n : nullptr,
-                          xvx_inv_xv ? xvx_inv_xv + t * p * n : nullptr);
+                          xvx_inv_xv ? xvx_inv_xv + t * p * n : nullptr,
+                          alpha_out ? alpha_out + t * p : nullptr,
+                          per_marker_vr ? per_marker_vr + t * m_total : nullptr);
           if (r != SAIGE_OK) errs[t] = saige_last_error();
           rcs[t] = r;
           batcher.leave();
@@ -1196,20 +1201,23 @@ int saige_fit_null(int device, const uint8_t* bed_body, size_t bpm, size_t n_tot
                    size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
                    const double* x, size_t p, int trait_type, const saige_fit_null_config* cfg,
                    saige_fit_null_result* result, double* mu_out, double* xvx_inv_xv,
-                   uint8_t* marker_class) {
+                   uint8_t* marker_class, double* alpha, double* per_marker_vr) {
   clear_error();
   return fit_null_impl(device, bed_body, false, bpm, n_total, m_total, sample_idx, n, y, x, p,
-                       trait_type, cfg, result, mu_out, xvx_inv_xv, marker_class);
+                       trait_type, cfg, result, mu_out, xvx_inv_xv, marker_class, alpha,
+                       per_marker_vr);
 }
 
 int saige_fit_null_device(int device, const uint8_t* d_bed_body, size_t bpm, size_t n_total,
                           size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
                           const double* x, size_t p, int trait_type,
                           const saige_fit_null_config* cfg, saige_fit_null_result* result,
-                          double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
+                          double* mu_out, double* xvx_inv_xv, uint8_t* marker_class,
+                          double* alpha, double* per_marker_vr) {
   clear_error();
   return fit_null_impl(device, d_bed_body, true, bpm, n_total, m_total, sample_idx, n, y, x, p,
-                       trait_type, cfg, result, mu_out, xvx_inv_xv, marker_class);
+                       trait_type, cfg, result, mu_out, xvx_inv_xv, marker_class, alpha,
+                       per_marker_vr);
 }
 
 
@@ -1217,11 +1225,12 @@ int saige_fit_null_multi(int device, const uint8_t* bed_body, size_t bpm, size_t
                          size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
                          size_t n_traits, const double* x, size_t p, const int* trait_types,
                          const saige_fit_null_config* cfg, saige_fit_null_result* results,
-                         double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
+                         double* mu_out, double* xvx_inv_xv, uint8_t* marker_class,
+                         double* alpha, double* per_marker_vr) {
   clear_error();
   return fit_null_multi_impl(device, bed_body, false, bpm, n_total, m_total, sample_idx, n, y,
                              n_traits, x, p, trait_types, cfg, results, mu_out, xvx_inv_xv,
-                             marker_class);
+                             marker_class, alpha, per_marker_vr);
 }
 
 int saige_fit_null_multi_device(int device, const uint8_t* d_bed_body, size_t bpm,
@@ -1229,11 +1238,12 @@ int saige_fit_null_multi_device(int device, const uint8_t* d_bed_body, size_t bp
                                 size_t n, const double* y, size_t n_traits, const double* x,
                                 size_t p, const int* trait_types,
                                 const saige_fit_null_config* cfg, saige_fit_null_result* results,
-                                double* mu_out, double* xvx_inv_xv, uint8_t* marker_class) {
+                                double* mu_out, double* xvx_inv_xv, uint8_t* marker_class,
+                                double* alpha, double* per_marker_vr) {
   clear_error();
   return fit_null_multi_impl(device, d_bed_body, true, bpm, n_total, m_total, sample_idx, n, y,
                              n_traits, x, p, trait_types, cfg, results, mu_out, xvx_inv_xv,
-                             marker_class);
+                             marker_class, alpha, per_marker_vr);
 }
 
 }  // extern "C"

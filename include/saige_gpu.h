@@ -202,7 +202,11 @@ int saige_estimate_variance_ratio(saige_grm* h, const double* dosages, const dou
  * Whole step 1: saige-cli fit_null::run (saige-cli/src/commands/fit_null.rs:199-394)
  * after phenotype parsing: marker QC (min_maf, max_missing), the ChaCha8 VR/GRM split
  * (:230-259), GRM build, AI-REML, XVX^{-1}X'V and the variance ratio.  y and x are in
- * model sample order (the sample_idx order).
+ * model sample order (the sample_idx order).  Outputs (each may be NULL): mu (n),
+ * xvx_inv_xv (p*n, NullModel::xvx_inv_xv_flat), marker_class (m_total: 0 failed QC, 1 GRM,
+ * 2 VR pool), alpha (p fixed effects, AiRemlResult::alpha) and per_marker_vr (room for
+ * m_total values; the first result->vr_markers_used are VarianceRatioResult::per_marker_vr)
+ * -- with tau and the variance ratio these fill NullModel (null_model.rs:17-60).
  * ---------------------------------------------------------------------------- */
 typedef struct {
   saige_ai_reml_config reml;
@@ -225,14 +229,15 @@ int saige_fit_null(int device, const uint8_t* bed_body, size_t bytes_per_marker,
                    size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,
                    const double* x, size_t p, int trait_type, const saige_fit_null_config* cfg,
                    saige_fit_null_result* result, double* mu, double* xvx_inv_xv,
-                   uint8_t* marker_class);
+                   uint8_t* marker_class, double* alpha, double* per_marker_vr);
 
 /* Same, with the .bed body already resident in device memory. */
 int saige_fit_null_device(int device, const uint8_t* d_bed_body, size_t bytes_per_marker,
                           size_t n_total, size_t m_total, const uint32_t* sample_idx, size_t n,
                           const double* y, const double* x, size_t p, int trait_type,
                           const saige_fit_null_config* cfg, saige_fit_null_result* result,
-                          double* mu, double* xvx_inv_xv, uint8_t* marker_class);
+                          double* mu, double* xvx_inv_xv, uint8_t* marker_class, double* alpha,
+                          double* per_marker_vr);
 
 /* Several phenotypes over the same samples, covariates and GRM: n_traits runs of
  * fit_null::run (fit_null.rs:199-394, one per trait, each with its own W, tau, PCG
@@ -241,20 +246,22 @@ int saige_fit_null_device(int device, const uint8_t* d_bed_body, size_t bytes_pe
  * into one multi-column product (its own Sigma epilogue per trait), so each trait follows
  * its single-trait iteration while the genotypes are streamed once per joined product.
  * y: n x n_traits col-major; trait_types[n_traits]; results[n_traits];
- * mu: n x n_traits; xvx_inv_xv: (p*n) x n_traits; marker_class: m_total (shared).
+ * mu: n x n_traits; xvx_inv_xv: (p*n) x n_traits; marker_class: m_total (shared);
+ * alpha: p x n_traits; per_marker_vr: m_total x n_traits.
  * No counterpart in saige-rs beyond calling fit_null once per phenotype. */
 int saige_fit_null_multi(int device, const uint8_t* bed_body, size_t bytes_per_marker,
                          size_t n_total, size_t m_total, const uint32_t* sample_idx, size_t n,
                          const double* y, size_t n_traits, const double* x, size_t p,
                          const int* trait_types, const saige_fit_null_config* cfg,
                          saige_fit_null_result* results, double* mu, double* xvx_inv_xv,
-                         uint8_t* marker_class);
+                         uint8_t* marker_class, double* alpha, double* per_marker_vr);
 int saige_fit_null_multi_device(int device, const uint8_t* d_bed_body, size_t bytes_per_marker,
                                 size_t n_total, size_t m_total, const uint32_t* sample_idx,
                                 size_t n, const double* y, size_t n_traits, const double* x,
                                 size_t p, const int* trait_types,
                                 const saige_fit_null_config* cfg, saige_fit_null_result* results,
-                                double* mu, double* xvx_inv_xv, uint8_t* marker_class);
+                                double* mu, double* xvx_inv_xv, uint8_t* marker_class,
+                                double* alpha, double* per_marker_vr);
 
 /* ------------------------------------------------------------------------------
  * Step 2: score test engine.

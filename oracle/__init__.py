@@ -130,7 +130,7 @@ def _declare(L):
     L.orc_fit_null.argtypes = [_u8p, C.c_size_t, C.c_size_t, C.c_size_t, _u32p, C.c_size_t, _dp,
                                _dp, C.c_size_t, C.c_int, C.POINTER(RemlConfig), C.c_double,
                                C.c_double, C.c_int, C.c_int, C.POINTER(FitNullOut), _dp, _dp,
-                               _u8p, _dp]
+                               _u8p, _dp, _dp, _dp]
     for nm in ("orc_k0_binom", "orc_k2_binom"):
         getattr(L, nm).argtypes = [C.c_double, _dp, _dp, C.c_size_t]
         getattr(L, nm).restype = C.c_double
@@ -358,17 +358,20 @@ def fit_null(bed_body, bytes_per_marker, n_total, m_total, sample_idx, y, x_cm,
     a = np.empty(p * n)
     cls = np.empty(m_total, np.uint8)
     trace = np.full(2 * max_iter, np.nan)
+    alpha = np.empty(p)
+    per = np.empty(max(m_total, 1))
     rc = lib().orc_fit_null(_p(body, _u8p), bytes_per_marker, n_total, m_total, _p(si, _u32p), n,
                             _p(y, _dp), _p(x, _dp), p, trait, C.byref(cfg), min_maf, max_missing,
                             n_markers_vr, grm_mode, C.byref(out), _p(mu, _dp), _p(a, _dp),
-                            _p(cls, _u8p), _p(trace, _dp))
+                            _p(cls, _u8p), _p(trace, _dp), _p(alpha, _dp), _p(per, _dp))
     if rc != 0:
         raise RuntimeError(f"orc_fit_null failed: {rc}")
     return dict(tau=np.array(out.tau[:]), iterations=out.iterations, converged=bool(out.converged),
                 vr=out.vr, vr_markers_used=out.vr_markers_used, n_grm_markers=out.n_grm_markers,
                 n_vr_markers=out.n_vr_markers, grm_products=out.grm_products,
                 pcg_iterations=out.pcg_iterations, log_likelihood=out.log_likelihood, mu=mu,
-                xvx_inv_xv=a, marker_class=cls,
+                xvx_inv_xv=a, marker_class=cls, alpha=alpha,
+                per_marker_vr=per[:out.vr_markers_used].copy(),
                 tau_trace=trace[:2 * out.iterations].reshape(-1, 2))
 
 

@@ -807,7 +807,7 @@ int orc_fit_null(const uint8_t* bed, size_t bytes_per_marker, size_t n_total, si
                  size_t p, int trait, const OrcRemlConfig* cfg, double min_maf,
                  double max_missing, int n_markers_vr, int grm_mode, OrcFitNullOut* out,
                  double* mu_out, double* xvx_inv_xv_out, uint8_t* is_grm_marker,
-                 double* tau_trace) {
+                 double* tau_trace, double* alpha_out, double* per_marker_vr) {
   std::vector<double> d(n);
   std::vector<size_t> passing;
   std::vector<double> pass_af, pass_mac;
@@ -846,7 +846,7 @@ int orc_fit_null(const uint8_t* bed, size_t bytes_per_marker, size_t n_total, si
   OrcRemlResult rr;
   std::vector<double> mu(n), w(n);
   int rc = orc_fit_ai_reml(grm, y, x, n, p, trait, cfg, &rr, mu.data(), w.data(), nullptr, nullptr,
-                           nullptr, tau_trace);
+                           alpha_out, tau_trace);
   if (rc != 0) { delete grm; return rc; }
   std::vector<double> a(p * n);
   if (orc_xvx_inv_xv(x, n, p, w.data(), a.data()) != 0) { delete grm; return -5; }
@@ -854,7 +854,7 @@ int orc_fit_null(const uint8_t* bed, size_t bytes_per_marker, size_t n_total, si
   if (nv > 0) {
     rc = orc_estimate_vr(grm, vr_dos.data(), vr_mac.data(), nv, mu.data(), w.data(), rr.tau, trait,
                          x, a.data(), n, p, n_markers_vr, 20.0, 0.001, cfg->seed, &vr, &used,
-                         nullptr, &vprod);
+                         per_marker_vr, &vprod);
     if (rc != 0) { delete grm; return rc; }
   }
   delete grm;

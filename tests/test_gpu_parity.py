@@ -306,6 +306,41 @@ def test_pipeline_fit_null_then_score():
     assert err.max() < 1e-5
 
 
+def test_model_file_step1_to_step2(tmp_path):
+    """fit-null on the GPU written as a .saige.model (with alpha and the per-marker variance
+    ratios against the oracle), read back, and step 2 from the loaded model equal to step 2
+    from the fit itself; the SAIGE result lines carry the same numbers."""
+    from csaige_amd import model as M
+    c = synth.unrelated(n=800, m_grm=2000, m_test=120, seed=21, h2=0.4, prevalence=0.1)
+    idx = np.arange(c.n, dtype=np.uint32)
+    fg = sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], c.y, c.x, sample_idx=idx)
+    fo = oracle.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], idx, c.y, c.x)
+    np.testing.assert_allclose(fg["alpha"], fo["alpha"], rtol=1e-6, atol=1e-9)
+    assert len(fg["per_marker_vr"]) == fg["vr_markers_used"] == len(fo["per_marker_vr"])
+    np.testing.assert_allclose(fg["per_marker_vr"], fo["per_marker_vr"], rtol=1e-6)
+    ids = [f"sample{i}" for i in range(c.n)]
+    m = M.NullModel.from_fit_null(fg, c.y, c.x, ids)
+    M.save_model(m, tmp_path / "gpu.saige.model")
+    m2 = M.load_model(tmp_path / "gpu.saige.model")
+    assert m2.sample_ids == ids and m2.tau == tuple(fg["tau"])
+    assert abs(m2.variance_ratio.variance_ratio - fg["vr"]) == 0.0
+    e1 = sg.ScoreTestEngine(fg["mu"], c.y - fg["mu"], fg["tau"], c.x, fg["xvx_inv_xv"], fg["vr"],
+                            y=c.y)
+    e2 = M.engine_from_model(m2)
+    got1 = e1.test_markers_plink(c.bed_test, c.bpm, c.n, c.bed_test.shape[0])
+    got2 = e2.test_markers_plink(c.bed_test, c.bpm, c.n, c.bed_test.shape[0])
+    for k in ("ac", "score", "var_t", "pvalue", "pvalue_noadj", "is_spa", "beta", "se_beta"):
+        np.testing.assert_array_equal(got1[k], got2[k], err_msg=k)
+    nv = len(got2["pvalue"])
+    M.write_results(tmp_path / "out.txt", got2, ["1"] * nv, np.arange(nv) + 1,
+                    [f"rs{v}" for v in range(nv)], ["A"] * nv, ["G"] * nv)
+    lines = (tmp_path / "out.txt").read_text().splitlines()
+    assert lines[0].split() == M.HEADER.split() and len(lines) == nv + 1
+    cols = M.HEADER.split()
+    back = np.array([float(ln.split()[cols.index("p.value")]) for ln in lines[1:]])
+    np.testing.assert_array_equal(back, got2["pvalue"])  # shortest round-trip digits
+
+
 def test_unrelated_2000_workload_end_to_end():
     """The 2,000-sample CPU-runnable workload in full: 2,000 unrelated samples, 5,000 GRM
     markers, 2,000 test variants, 2 covariates, liability h2 = 0.3, top 10% cases; fit-null
