@@ -283,6 +283,17 @@ def fit_null_multi(bed_body, bytes_per_marker, n_total, m_total, Y, x_cm, traits
     return [_fit_null_dict(res[t], mu[t], a[t], cls, alpha[t], pm[t]) for t in range(T)]
 
 
+def bgen_dosages(batch, sample_idx=None, device=0):
+    """Dosages (nvar x n) of a ``bgen.BgenBatch`` decoded on the device (bgen.rs:373-430)."""
+    si, sp = _idx(sample_idx)
+    n = batch.n_file if si is None else len(si)
+    out = np.empty((batch.n_variants, n))
+    check(load().saige_bgen_dosages(device, _p(batch.blocks, _lib.u8p), batch.ld,
+                                    batch.n_variants, _p(batch.info, _lib.u32p), batch.n_file,
+                                    sp, n, _p(out, _lib.dp)))
+    return out
+
+
 def results_to_dict(arr):
     """Columns of a ctypes array of saige_variant_result as numpy arrays."""
     raw = np.frombuffer(arr, dtype=_RESULT_DTYPE)
@@ -358,6 +369,16 @@ class ScoreTestEngine:
         out = (_lib.VariantResult * nvar)()
         check(load().saige_test_markers_plink(self.h, _p(body, _lib.u8p), bytes_per_marker,
                                               n_total, sp, nvar, out))
+        return results_to_dict(out)
+
+    def test_markers_bgen(self, batch, sample_idx=None):
+        """Variants of a ``bgen.BgenBatch`` (decompressed layout-2 blocks), decoded on the
+        device inside the score test; ``sample_idx`` maps model samples to file samples."""
+        si, sp = _idx(sample_idx)
+        nvar = batch.n_variants
+        out = (_lib.VariantResult * nvar)()
+        check(load().saige_test_markers_bgen(self.h, _p(batch.blocks, _lib.u8p), batch.ld, nvar,
+                                             _p(batch.info, _lib.u32p), batch.n_file, sp, out))
         return results_to_dict(out)
 
     def set_root_proposal(self, mlp):

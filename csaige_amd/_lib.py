@@ -138,6 +138,9 @@ SIGNATURES = [
     ("saige_test_markers_u8_device", C.c_int, [vp, vp, sz, sz, C.c_double, vp]),
     ("saige_test_markers_plink", C.c_int, [vp, u8p, sz, sz, u32p, sz,
                                            C.POINTER(VariantResult)]),
+    ("saige_bgen_dosages", C.c_int, [C.c_int, u8p, sz, sz, u32p, C.c_uint32, u32p, sz, dp]),
+    ("saige_test_markers_bgen", C.c_int, [vp, u8p, sz, sz, u32p, C.c_uint32, u32p,
+                                          C.POINTER(VariantResult)]),
     ("saige_mlp_create", C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
     ("saige_mlp_destroy", None, [vp]),
     ("saige_mlp_set_params", C.c_int, [vp, fp, fp]),

@@ -155,6 +155,56 @@ struct LoadF64 {
     for (int k = 0; k < 16; ++k) out[k] = g[v * ld + i + k];
   }
 };
+// BGEN v1.2 layout-2 probability data (decompressed), one block per variant at a stride of
+// ld bytes, zero-filled past each block (read_bits reads 0 past the data, bgen.rs:439-452).
+// info[v]: bits per probability (1..32) | 0x100 when the variant is diploid, biallelic and
+// unphased, the case parse_layout2_probabilities decodes (bgen.rs:373-430):
+//   dosage = P(AB) + 2 (1 - P(AA) - P(AB)),  P = value / (2^bits - 1)  (second allele)
+// Other variants read 0 for every sample that is not missing; a missing sample (bit 7 of its
+// ploidy byte) reads NaN.  Probabilities are packed LSB first, sample s at bit 2 s bits.
+__device__ __forceinline__ uint64_t bgen_bits(const uint8_t* p, size_t bit, uint32_t nbits) {
+  uint64_t r = 0;
+  for (uint32_t k = 0; k < nbits; ++k) {
+    const size_t t = bit + k;
+    r |= (uint64_t)((p[t >> 3] >> (t & 7)) & 1u) << k;
+  }
+  return r;
+}
+
+struct LoadBgen {
+  static constexpr bool kMayMiss = true;
+  const uint8_t* blk;
+  size_t ld;
+  const uint32_t* info;
+  const uint32_t* idx;  // model sample -> sample of the file, or null
+  uint32_t n_file;      // samples per block (the block's N)
+  int vec;              // always 0: one sample per call
+  __device__ __forceinline__ double operator()(size_t v, size_t i) const {
+    const uint8_t* b = blk + v * ld;
+    const size_t s = idx ? idx[i] : i;
+    if (b[8 + s] & 0x80u) return dnan();
+    const uint32_t inf = info[v];
+    if (!(inf & 0x100u)) return 0.0;
+    const uint32_t bits = inf & 0xffu;
+    const double maxv = (double)((1ull << bits) - 1ull);
+    const uint8_t* pr = b + 10 + n_file;
+    uint64_t a, c;
+    if (bits == 8) {
+      a = pr[2 * s];
+      c = pr[2 * s + 1];
+    } else {
+      a = bgen_bits(pr, 2 * s * bits, bits);
+      c = bgen_bits(pr, (2 * s + 1) * bits, bits);
+    }
+    const double p_aa = (double)a / maxv, p_ab = (double)c / maxv;
+    return p_ab + 2.0 * (1.0 - p_aa - p_ab);
+  }
+  __device__ __forceinline__ void load16(size_t v, size_t i, double* g) const {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) g[k] = (*this)(v, i + k);
+  }
+};
+
 struct LoadPacked {  // plink.rs:170-194, 237-247
   static constexpr bool kMayMiss = true;
   const uint8_t* bed;
@@ -1365,7 +1415,18 @@ __global__ void sums_reduce_kernel(const double* __restrict__ part, size_t split
 }
 
 template <class L> constexpr int loader_id() {
-  return std::is_same<L, LoadU8>::value ? 1 : (std::is_same<L, LoadF64>::value ? 2 : 3);
+  return std::is_same<L, LoadU8>::value ? 1
+       : std::is_same<L, LoadF64>::value ? 2
+       : std::is_same<L, LoadPacked>::value ? 3 : 4;
+}
+
+// dosages[v * n + i] of a batch of BGEN blocks (the decode of bgen.rs:373-430)
+__global__ void bgen_decode_kernel(LoadBgen load, size_t n, size_t nvar,
+                                   double* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t v = blockIdx.y;
+  if (i >= n || v >= nvar) return;
+  out[v * n + i] = load(v, i);
 }
 
 // SPA of a batch's gated variants with S work-groups per variant and pass (about 1,024
@@ -1771,6 +1832,101 @@ int saige_test_markers_plink(saige_engine* e, const uint8_t* bed_body, size_t bp
     SAIGE_HIP(hipMemcpyAsync(db, bed_body + v0 * bpm, nb * bpm, hipMemcpyHostToDevice, s));
     SAIGE_TRY(run_batch(e, LoadPacked{db, bpm, sample_idx ? e->idx.as<uint32_t>() : nullptr,
                                        (int)(!sample_idx && bpm % 4 == 0)}, nb, dr));
+    SAIGE_HIP(hipMemcpyAsync(out + v0, dr, nb * sizeof(saige_variant_result), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipStreamSynchronize(s));
+  }
+  return SAIGE_OK;
+}
+
+// Host-side checks shared by the BGEN entry points.
+static int bgen_check(const uint8_t* blocks, size_t ld, size_t nvar, const uint32_t* info,
+                      uint32_t n_file, const uint32_t* sample_idx, size_t n) {
+  SAIGE_ARG(nvar == 0 || (blocks && info), "null pointer");
+  SAIGE_ARG(ld >= 10 + (size_t)n_file, "ld shorter than a layout-2 header");
+  if (!sample_idx) SAIGE_ARG(n == n_file, "n must equal the block's N without a subset");
+  for (size_t k = 0; sample_idx && k < n; ++k)
+    SAIGE_ARG(sample_idx[k] < n_file, "sample_idx out of range");
+  for (size_t v = 0; v < nvar; ++v) {
+    const uint32_t bits = info[v] & 0xffu;
+    if (bits == 0) {  // bgen.rs:396-398
+      set_error("BGEN: 0 bits per probability not supported");
+      return SAIGE_ERR_ARG;
+    }
+    SAIGE_ARG(bits <= 32, "bits per probability > 32");
+    uint32_t nb;
+    std::memcpy(&nb, blocks + v * ld, 4);
+    SAIGE_ARG(nb == n_file, "a block's N differs from n_file");
+    const size_t need = 10 + (size_t)n_file + ((2 * (size_t)n_file * bits + 7) >> 3);
+    SAIGE_ARG(!(info[v] & 0x100u) || need <= ld, "ld shorter than a block's probabilities");
+  }
+  return SAIGE_OK;
+}
+
+int saige_bgen_dosages(int device, const uint8_t* blocks, size_t ld, size_t nvar,
+                       const uint32_t* info, uint32_t n_file, const uint32_t* sample_idx,
+                       size_t n, double* dosages) {
+  clear_error();
+  SAIGE_ARG(nvar == 0 || n == 0 || dosages, "null pointer");
+  SAIGE_TRY(bgen_check(blocks, ld, nvar, info, n_file, sample_idx, n));
+  if (nvar == 0 || n == 0) return SAIGE_OK;
+  SAIGE_TRY(use_device(device));
+  hipStream_t s;
+  SAIGE_HIP(hipStreamCreate(&s));
+  int rc = SAIGE_OK;
+  {
+    DevBuf d_blk, d_info, d_idx, d_out;
+    hipError_t err = hipSuccess;
+    if ((rc = d_blk.ensure(nvar * ld)) == SAIGE_OK && (rc = d_info.ensure(nvar * 4)) == SAIGE_OK &&
+        (rc = d_idx.ensure(std::max<size_t>(n, 1) * 4)) == SAIGE_OK &&
+        (rc = d_out.ensure(nvar * n * 8)) == SAIGE_OK) {
+      err = hipMemcpyAsync(d_blk.p, blocks, nvar * ld, hipMemcpyHostToDevice, s);
+      if (err == hipSuccess) err = hipMemcpyAsync(d_info.p, info, nvar * 4, hipMemcpyHostToDevice, s);
+      if (err == hipSuccess && sample_idx)
+        err = hipMemcpyAsync(d_idx.p, sample_idx, n * 4, hipMemcpyHostToDevice, s);
+      if (err == hipSuccess) {
+        LoadBgen load{d_blk.as<uint8_t>(), ld, d_info.as<uint32_t>(),
+                      sample_idx ? d_idx.as<uint32_t>() : nullptr, n_file, 0};
+        bgen_decode_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)nvar), 256, 0, s>>>(
+            load, n, nvar, d_out.as<double>());
+        err = hipGetLastError();
+      }
+      if (err == hipSuccess) err = hipMemcpyAsync(dosages, d_out.p, nvar * n * 8, hipMemcpyDeviceToHost, s);
+      if (err == hipSuccess) err = hipStreamSynchronize(s);
+      if (err != hipSuccess) {
+        set_error("bgen decode: %s", hipGetErrorString(err));
+        rc = SAIGE_ERR_DEVICE;
+      }
+    }
+  }
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+int saige_test_markers_bgen(saige_engine* e, const uint8_t* blocks, size_t ld, size_t nvar,
+                            const uint32_t* info, uint32_t n_file, const uint32_t* sample_idx,
+                            saige_variant_result* out) {
+  clear_error();
+  SAIGE_ARG(e && (nvar == 0 || out), "null pointer");
+  SAIGE_TRY(bgen_check(blocks, ld, nvar, info, n_file, sample_idx, e->n));
+  std::lock_guard<std::mutex> lk(e->mu);
+  SAIGE_HIP(hipSetDevice(e->device));
+  hipStream_t s = e->stream;
+  if (sample_idx) {
+    SAIGE_TRY(e->idx.ensure(e->n * 4));
+    SAIGE_HIP(hipMemcpyAsync(e->idx.p, sample_idx, e->n * 4, hipMemcpyHostToDevice, s));
+  }
+  for (size_t v0 = 0; v0 < nvar; v0 += kHostBatch) {
+    const size_t nb = std::min(kHostBatch, nvar - v0);
+    const size_t blk_bytes = round_up(nb * ld, 256), info_bytes = round_up(nb * 4, 256);
+    SAIGE_TRY(e->in.ensure(blk_bytes + info_bytes + nb * sizeof(saige_variant_result)));
+    uint8_t* db = e->in.as<uint8_t>();
+    uint32_t* di = reinterpret_cast<uint32_t*>(db + blk_bytes);
+    saige_variant_result* dr = reinterpret_cast<saige_variant_result*>(db + blk_bytes + info_bytes);
+    SAIGE_HIP(hipMemcpyAsync(db, blocks + v0 * ld, nb * ld, hipMemcpyHostToDevice, s));
+    SAIGE_HIP(hipMemcpyAsync(di, info + v0, nb * 4, hipMemcpyHostToDevice, s));
+    SAIGE_TRY(run_batch(e, LoadBgen{db, ld, di, sample_idx ? e->idx.as<uint32_t>() : nullptr,
+                                    n_file, 0},
+                        nb, dr));
     SAIGE_HIP(hipMemcpyAsync(out + v0, dr, nb * sizeof(saige_variant_result), hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
   }

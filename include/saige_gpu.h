@@ -311,6 +311,24 @@ int saige_test_markers_plink(saige_engine* e, const uint8_t* bed_body, size_t by
                              size_t n_total, const uint32_t* sample_idx, size_t nvar,
                              saige_variant_result* out);
 
+/* BGEN v1.2 layout-2 genotype data (BgenReader::read_marker, saige-geno/src/bgen.rs:314-437
+ * with the subset of :466-480), decoded on the device inside the score test.
+ *   blocks: nvar decompressed layout-2 probability blocks, one every ld bytes, zero-filled
+ *           past each block's data (read_bits reads 0 there, bgen.rs:439-452); every
+ *           block's N (its first u32) must equal n_file.
+ *   info[v]: bits per probability (1..32; 0 is the reference's error) | 0x100 when the
+ *           variant is diploid, biallelic and unphased -- the case whose dosage
+ *           P(AB) + 2 (1 - P(AA) - P(AB)) is decoded; other variants read 0, and a missing
+ *           sample (bit 7 of its ploidy byte) NaN, as bgen.rs:408-430.
+ *   sample_idx: model sample -> sample of the file (NULL: all n_file samples, in order).
+ * saige_bgen_dosages writes the decoded dosages (n x nvar, column per variant). */
+int saige_bgen_dosages(int device, const uint8_t* blocks, size_t ld, size_t nvar,
+                       const uint32_t* info, uint32_t n_file, const uint32_t* sample_idx,
+                       size_t n, double* dosages);
+int saige_test_markers_bgen(saige_engine* e, const uint8_t* blocks, size_t ld, size_t nvar,
+                            const uint32_t* info, uint32_t n_file, const uint32_t* sample_idx,
+                            saige_variant_result* out);
+
 /* ------------------------------------------------------------------------------
  * Learned saddle-point root proposal (no counterpart in saige-rs).
  *   A 6 -> 64 -> 64 -> 1 GELU network proposes the Newton starting points of the two SPA

@@ -101,6 +101,8 @@ class VariantResult(C.Structure):
 
 def _declare(L):
     L.orc_decode_marker.argtypes = [_u8p, C.c_size_t, _u32p, C.c_size_t, _dp]
+    L.orc_bgen_layout2_dosages.argtypes = [_u8p, C.c_size_t, _dp, C.c_size_t,
+                                           C.POINTER(C.c_size_t)]
     L.orc_compute_af.argtypes = [_dp, C.c_size_t, _dp, _dp, _dp, _u64p]
     L.orc_chacha_block.argtypes = [_u32p, C.c_uint64, C.c_uint64, C.c_int, _u32p]
     L.orc_seed_from_u64.argtypes = [C.c_uint64, _u32p]
@@ -340,6 +342,21 @@ def estimate_vr(grm, dosages, macs, mu, w, tau, x_cm, a_flat, trait=TRAIT_BINARY
     if rc != 0:
         raise RuntimeError(f"orc_estimate_vr failed: {rc}")
     return vr.value, used.value, per[:used.value]
+
+
+def bgen_layout2_dosages(block):
+    """parse_layout2_probabilities (bgen.rs:373-430) of one decompressed block."""
+    b = np.frombuffer(bytes(block), dtype=np.uint8)
+    cap = int(np.frombuffer(b[:4].tobytes(), "<u4")[0]) if len(b) >= 4 else 0
+    out = np.empty(max(cap, 1))
+    n = C.c_size_t(0)
+    rc = lib().orc_bgen_layout2_dosages(_p(np.ascontiguousarray(b), _u8p), len(b), _p(out, _dp),
+                                        cap, C.byref(n))
+    if rc == -1:
+        raise ValueError("BGEN: 0 bits per probability not supported")
+    if rc != 0:
+        raise ValueError("BGEN: malformed layout-2 block")
+    return out[:n.value].copy()
 
 
 def fit_null(bed_body, bytes_per_marker, n_total, m_total, sample_idx, y, x_cm,

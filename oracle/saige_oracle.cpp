@@ -32,6 +32,8 @@
 //   SPA                  : KATs binary.rs:246-307; K', K'' against finite differences of K
 //   chi2 / normal tails  : statrs 0.18.0 is third-party; restated as erf/erfc closed forms
 //                          (parity unpinned at the last bit)
+//   BGEN layout-2 decode : the reference's .bgen fixtures against the genotype calls of the
+//                          same markers in their .vcf files (tests/golden/bgen_*)
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -866,6 +868,59 @@ int orc_fit_null(const uint8_t* bed, size_t bytes_per_marker, size_t n_total, si
   out->log_likelihood = rr.log_likelihood;
   if (mu_out) for (size_t i = 0; i < n; ++i) mu_out[i] = mu[i];
   if (xvx_inv_xv_out) for (size_t i = 0; i < p * n; ++i) xvx_inv_xv_out[i] = a[i];
+  return 0;
+}
+
+// --- BGEN layout 2  saige-geno/src/bgen.rs:373-452 (parse_layout2_probabilities, read_bits)
+static uint64_t orc_read_bits(const uint8_t* data, size_t len, size_t byte_start,
+                              size_t bit_offset, size_t n_bits) {  // bgen.rs:439-452
+  uint64_t result = 0;
+  for (size_t bit = 0; bit < n_bits; ++bit) {
+    size_t total_bit = bit_offset + bit;
+    size_t byte_idx = byte_start + total_bit / 8;
+    size_t bit_idx = total_bit % 8;
+    if (byte_idx < len) {
+      uint64_t bit_val = (data[byte_idx] >> bit_idx) & 1u;
+      result |= bit_val << bit;
+    }
+  }
+  return result;
+}
+
+// Dosages of one decompressed block (out needs room for the block's N); returns 0, -1 for
+// 0 bits per probability (bgen.rs:396-398), -2 for a block shorter than its header.
+int orc_bgen_layout2_dosages(const uint8_t* data, size_t len, double* out, size_t cap,
+                             size_t* n_out) {
+  if (len < 8) return -2;
+  uint32_t n_samples; uint16_t n_alleles;
+  std::memcpy(&n_samples, data, 4);
+  std::memcpy(&n_alleles, data + 4, 2);
+  uint8_t min_ploidy = data[6], max_ploidy = data[7];
+  if (len < 10 + (size_t)n_samples || n_samples > cap) return -2;
+  std::vector<bool> missing(n_samples);
+  for (size_t i = 0; i < n_samples; ++i) missing[i] = (data[8 + i] & 0x80) != 0;
+  bool phased = data[8 + n_samples] != 0;
+  size_t bits_per_prob = data[9 + n_samples];
+  if (bits_per_prob == 0) return -1;
+  uint64_t max_val = (bits_per_prob >= 64) ? ~0ull : ((1ull << bits_per_prob) - 1);
+  for (size_t i = 0; i < n_samples; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
+  if (n_alleles == 2 && min_ploidy == 2 && max_ploidy == 2 && !phased) {
+    size_t prob_start = 10 + n_samples, bit_offset = 0;
+    for (size_t i = 0; i < n_samples; ++i) {
+      if (missing[i]) { bit_offset += 2 * bits_per_prob; continue; }
+      double p_aa = (double)orc_read_bits(data, len, prob_start, bit_offset, bits_per_prob) /
+                    (double)max_val;
+      bit_offset += bits_per_prob;
+      double p_ab = (double)orc_read_bits(data, len, prob_start, bit_offset, bits_per_prob) /
+                    (double)max_val;
+      bit_offset += bits_per_prob;
+      out[i] = p_ab + 2.0 * (1.0 - p_aa - p_ab);
+    }
+  } else {
+    for (size_t i = 0; i < n_samples; ++i)
+      out[i] = missing[i] ? std::numeric_limits<double>::quiet_NaN() : 0.0;
+  }
+  *n_out = n_samples;
   return 0;
 }
 
