@@ -117,6 +117,45 @@ class OnTheFlyGrm:
                                        self.n))
         return out
 
+    def set_marker_chromosomes(self, chrom_codes):
+        """Chromosome code (0..C-1) of every GRM marker, for the LOCO products and fits."""
+        c = np.ascontiguousarray(chrom_codes, dtype=np.int32)
+        check(load().saige_grm_set_marker_chromosomes(self.h, c.ctypes.data_as(
+            C.POINTER(C.c_int32))))
+        self._n_chrom = int(c.max()) + 1 if c.size else 0
+
+    def mat_mat_loco(self, V, exclude_chrom):
+        """GRM product over the markers not on chromosome code ``exclude_chrom``;
+        V: (k, n) array of k vectors; returns (k, n)."""
+        V = _f64(np.atleast_2d(V))
+        k = V.shape[0]
+        out = np.empty((k, self.n))
+        check(load().saige_grm_mat_mat_loco(self.h, _p(V, _lib.dp), self.n, k,
+                                            int(exclude_chrom), _p(out, _lib.dp), self.n))
+        return out
+
+    def fit_ai_reml_loco(self, y, x_cm, trait=TRAIT_BINARY, max_iter=30, tol=0.02,
+                         pcg_tol=1e-5, pcg_max_iter=500, n_random_vectors=30, seed=12345):
+        """run_loco (loco.rs:54-114): one fit_ai_reml per chromosome code, on the GRM
+        without that chromosome; the refits share every GRM product.  One dict per code."""
+        L = load()
+        y = _f64(y)
+        x = _f64(x_cm)
+        p, n = x.shape
+        nc = self._n_chrom
+        cfg = _lib.AiRemlConfig(max_iter, tol, pcg_tol, pcg_max_iter, n_random_vectors, seed)
+        res = (_lib.AiRemlResult * nc)()
+        mu, w, r = np.empty((nc, n)), np.empty((nc, n)), np.empty((nc, n))
+        alpha = np.empty((nc, p))
+        check(L.saige_fit_ai_reml_loco(self.h, _p(y, _lib.dp), _p(x, _lib.dp), n, p, trait,
+                                       C.byref(cfg), nc, res, _p(mu, _lib.dp), _p(w, _lib.dp),
+                                       _p(r, _lib.dp), _p(alpha, _lib.dp)))
+        return [dict(tau=np.array(res[c].tau[:]), iterations=res[c].iterations,
+                     converged=bool(res[c].converged), log_likelihood=res[c].log_likelihood,
+                     grm_products=res[c].grm_products, pcg_iterations=res[c].pcg_iterations,
+                     mu=mu[c], working_weights=w[c], residuals=r[c], alpha=alpha[c])
+                for c in range(nc)]
+
     def mat_mat_device(self, d_v, ldv, k, d_out, ldo):
         check(load().saige_grm_mat_mat_device(self.h, d_v, ldv, k, d_out, ldo))
 

@@ -113,6 +113,10 @@ SIGNATURES = [
                                         C.POINTER(PcgInfo)]),
     ("saige_fit_ai_reml", C.c_int, [vp, dp, dp, sz, sz, C.c_int, C.POINTER(AiRemlConfig),
                                     C.POINTER(AiRemlResult), dp, dp, dp, dp, dp]),
+    ("saige_grm_set_marker_chromosomes", C.c_int, [vp, C.POINTER(C.c_int32)]),
+    ("saige_grm_mat_mat_loco", C.c_int, [vp, dp, sz, sz, C.c_int32, dp, sz]),
+    ("saige_fit_ai_reml_loco", C.c_int, [vp, dp, dp, sz, sz, C.c_int, C.POINTER(AiRemlConfig),
+                                         sz, C.POINTER(AiRemlResult), dp, dp, dp, dp]),
     ("saige_xvx_inv_xv", C.c_int, [dp, sz, sz, dp, dp]),
     ("saige_estimate_variance_ratio", C.c_int, [vp, dp, dp, sz, dp, dp, dp, C.c_int, dp, dp, sz,
                                                 C.POINTER(VrConfig), dp, u32p, dp]),

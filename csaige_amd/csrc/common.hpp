@@ -198,9 +198,28 @@ struct saige_grm {
   uint64_t products = 0;   // column-products performed (diagnostic)
   saige::KernelTimer timer; // GEMM kernel launch timing (diagnostic)
   void* mixed = nullptr;   // mixed-precision state (grm_mixed.hip)
+  // Leave-one-chromosome-out: chromosome code per marker (handle order) and M / M_c per
+  // code (0 when a chromosome holds every marker); loco_req, set around one product call,
+  // gives each column the chromosome its GRM leaves out (-1: none).
+  saige::DevBuf loco_chrom, loco_scale;
+  int loco_n_chrom = 0;
+  const int* loco_req = nullptr;
 };
 
 namespace saige {
+
+// Columns of one product launch and the chromosome each leaves out of its GRM (-1: none).
+constexpr int kLocoMaxCols = 128;
+struct LocoCodes {
+  int code[kLocoMaxCols];
+};
+// Fills lc from h->loco_req[c0 .. c0 + kc); false when the product has no LOCO columns.
+bool loco_codes(const saige_grm* h, size_t c0, size_t kc, LocoCodes& lc);
+// The stage-1 result T (markers x columns, element (row, c) at t[row * rs + c * cs]) for
+// LOCO: rows of the left-out chromosome become 0, the others are scaled by M / M_c, so the
+// stage-2 product with the 1/M factor is the GRM of the kept markers.
+void loco_mask(const saige_grm* h, double* t, size_t rs, size_t cs, size_t m, size_t kc,
+               const LocoCodes& lc, hipStream_t s);
 
 // GRM product on device: out[:, c] = scale_out * (Z Z^T v[:, c]) * (1/M) for c < k;
 // v and out have leading dimensions ldv / ldo (>= n).  Rows >= n of out are not written.
@@ -216,14 +235,21 @@ int col_dots(hipStream_t s, const double* a, size_t lda, const double* b, size_t
 int gram(hipStream_t s, const double* A, size_t lda, size_t na, const double* B, size_t ldb,
          size_t nb, size_t n, double* h_out, DevBuf& part, DevBuf& res);
 
-// The handle itself as a product sink.
+// The handle itself as a product sink; with loco >= 0 its products leave that chromosome
+// out of the GRM.
 struct DirectSink : ProductSink {
   saige_grm* h;
+  int loco = -1;
   explicit DirectSink(saige_grm* hh) : h(hh) {}
   int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
               const double* tau) override {
     columns += k;
-    return grm_product(h, v, ldv, k, out, ldo, w, tau);
+    if (loco < 0) return grm_product(h, v, ldv, k, out, ldo, w, tau);
+    std::vector<int> codes(k, loco);
+    h->loco_req = codes.data();
+    const int rc = grm_product(h, v, ldv, k, out, ldo, w, tau);
+    h->loco_req = nullptr;
+    return rc;
   }
 };
 

@@ -157,9 +157,37 @@ __global__ void grm_s2_reduce(const double* __restrict__ ypart, size_t splits, s
   out[i + c * ldo] = g;
 }
 
+__global__ void loco_mask_kernel(double* __restrict__ t, size_t rs, size_t cs, size_t m,
+                                 size_t kc, const int* __restrict__ chrom,
+                                 const double* __restrict__ scale, LocoCodes lc) {
+  const size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (row >= m || c >= kc) return;
+  const int cc = lc.code[c];
+  if (cc < 0) return;
+  double& x = t[row * rs + c * cs];
+  x = chrom[row] == cc ? 0.0 : x * scale[cc];
+}
+
+bool loco_codes(const saige_grm* h, size_t c0, size_t kc, LocoCodes& lc) {
+  if (!h->loco_req || h->loco_n_chrom == 0) return false;
+  bool any = false;
+  for (size_t c = 0; c < kc; ++c) {
+    lc.code[c] = h->loco_req[c0 + c];
+    any = any || lc.code[c] >= 0;
+  }
+  return any;
+}
+
+void loco_mask(const saige_grm* h, double* t, size_t rs, size_t cs, size_t m, size_t kc,
+               const LocoCodes& lc, hipStream_t s) {
+  loco_mask_kernel<<<dim3((unsigned)ceil_div(m, 256), (unsigned)kc), 256, 0, s>>>(
+      t, rs, cs, m, kc, h->loco_chrom.as<int>(), h->loco_scale.as<double>(), lc);
+}
+
 template <int K>
 static int product_k(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
-                     size_t ldo, const double* d_w, const double* tau) {
+                     size_t ldo, const double* d_w, const double* tau, size_t c0) {
   GrmDevice& g = h->g;
   hipStream_t s = h->stream;
   // stage 1 split over samples, stage 2 split over markers, aiming at >= 2048 blocks
@@ -185,6 +213,8 @@ static int product_k(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
                          h->ws_tpart.as<double>(), g.m_pad));
   grm_s1_reduce<K><<<(unsigned)ceil_div(g.m_pad * K, 256), 256, 0, s>>>(
       h->ws_tpart.as<double>(), splits1, g.m_pad, h->ws_t.as<double>());
+  LocoCodes lc;
+  if (loco_codes(h, c0, k, lc)) loco_mask(h, h->ws_t.as<double>(), K, 1, g.m, k, lc, s);
   SAIGE_TRY(timed_launch(h->timer, grm_s2_f64<K>, dim3((unsigned)nblk_n, (unsigned)splits2),
                          dim3(256), s, g.geno, g.row_bytes, g.lut, h->ws_t.as<double>(), mps,
                          g.m, h->ws_ypart.as<double>(), g.n_pad));
@@ -227,16 +257,16 @@ int grm_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d
     const double* v = d_v + c0 * ldv;
     double* o = d_out + c0 * ldo;
     int rc;
-    if (kc <= 1) rc = product_k<1>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 2) rc = product_k<2>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 4) rc = product_k<4>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 8) rc = product_k<8>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 16) rc = product_k<16>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 24) rc = product_k<24>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 32) rc = product_k<32>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 40) rc = product_k<40>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 48) rc = product_k<48>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else rc = product_k<64>(h, v, ldv, kc, o, ldo, d_w, tau);
+    if (kc <= 1) rc = product_k<1>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 2) rc = product_k<2>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 4) rc = product_k<4>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 8) rc = product_k<8>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 16) rc = product_k<16>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 24) rc = product_k<24>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 32) rc = product_k<32>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 40) rc = product_k<40>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 48) rc = product_k<48>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else rc = product_k<64>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
     if (rc != SAIGE_OK) return rc;
   }
   return SAIGE_OK;
@@ -495,6 +525,50 @@ int saige_grm_mat_mat(saige_grm* h, const double* v, size_t ldv, size_t k, doubl
 
 int saige_grm_mat_vec(saige_grm* h, const double* v, double* out) {
   return saige_grm_mat_mat(h, v, h ? h->g.n : 0, 1, out, h ? h->g.n : 0);
+}
+
+int saige_grm_set_marker_chromosomes(saige_grm* h, const int32_t* chrom) {
+  clear_error();
+  SAIGE_ARG(h, "null pointer");
+  std::lock_guard<std::recursive_mutex> lk(h->mu);
+  SAIGE_HIP(hipSetDevice(h->device));
+  const size_t m = h->g.m;
+  if (!chrom) {
+    h->loco_n_chrom = 0;
+    return SAIGE_OK;
+  }
+  int nc = 0;
+  for (size_t j = 0; j < m; ++j) {
+    SAIGE_ARG(chrom[j] >= 0 && chrom[j] < 4096, "chromosome codes in [0, 4096)");
+    nc = std::max(nc, chrom[j] + 1);
+  }
+  std::vector<size_t> count(nc, 0);
+  for (size_t j = 0; j < m; ++j) ++count[chrom[j]];
+  std::vector<double> scale(std::max(nc, 1), 1.0);
+  for (int c = 0; c < nc; ++c)  // M / M_c; a GRM without markers is the zero product
+    scale[c] = count[c] == m ? 0.0 : (double)m / (double)(m - count[c]);
+  SAIGE_TRY(h->loco_chrom.ensure(std::max<size_t>(m, 1) * 4));
+  SAIGE_TRY(h->loco_scale.ensure(scale.size() * 8));
+  SAIGE_HIP(hipMemcpyAsync(h->loco_chrom.p, chrom, m * 4, hipMemcpyHostToDevice, h->stream));
+  SAIGE_HIP(hipMemcpyAsync(h->loco_scale.p, scale.data(), scale.size() * 8,
+                           hipMemcpyHostToDevice, h->stream));
+  SAIGE_HIP(hipStreamSynchronize(h->stream));
+  h->loco_n_chrom = nc;
+  return SAIGE_OK;
+}
+
+int saige_grm_mat_mat_loco(saige_grm* h, const double* v, size_t ldv, size_t k,
+                           int32_t exclude_chrom, double* out, size_t ldo) {
+  clear_error();
+  SAIGE_ARG(h, "null pointer");
+  std::lock_guard<std::recursive_mutex> lk(h->mu);
+  SAIGE_ARG(h->loco_n_chrom > 0, "no marker chromosomes set (saige_grm_set_marker_chromosomes)");
+  SAIGE_ARG(exclude_chrom >= 0 && exclude_chrom < h->loco_n_chrom, "chromosome code");
+  std::vector<int> codes(k, exclude_chrom);
+  h->loco_req = codes.data();
+  const int rc = saige_grm_mat_mat(h, v, ldv, k, out, ldo);
+  h->loco_req = nullptr;
+  return rc;
 }
 
 }  // extern "C"

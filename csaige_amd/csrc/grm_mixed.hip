@@ -391,7 +391,7 @@ int grm_mixed_prepare(saige_grm* h) {
 
 template <int KT>
 static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
-                            size_t ldo, const double* d_w, const double* tau) {
+                            size_t ldo, const double* d_w, const double* tau, size_t c0) {
   constexpr int KP = 16 * KT;
   GrmDevice& g = h->g;
   MixedState* ms = mstate(h);
@@ -439,6 +439,8 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   stage1_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(m, 256), (unsigned)k), 256, 0, s>>>(
       ms->part1.as<float>(), splits1, m_pad, m, k, ms->sc.as<double>(), ms->p2.as<double>(),
       ms->zs.as<double>(), cs, mean, ms->w.as<double>(), m);
+  LocoCodes lc;
+  if (loco_codes(h, c0, k, lc)) loco_mask(h, ms->w.as<double>(), 1, m, m, k, lc, s);
 
   // ---- W: mean over real markers, centre + split, S2p = sum_m 2p_m W'_split ----
   col_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(ms->w.as<double>(), m,
@@ -479,12 +481,12 @@ int grm_mixed_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, dou
     const double* v = d_v + c0 * ldv;
     double* o = d_out + c0 * ldo;
     int rc;
-    if (kc <= 16) rc = mixed_product_kt<1>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 32) rc = mixed_product_kt<2>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 48) rc = mixed_product_kt<3>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 64) rc = mixed_product_kt<4>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else if (kc <= 96) rc = mixed_product_kt<6>(h, v, ldv, kc, o, ldo, d_w, tau);
-    else rc = mixed_product_kt<8>(h, v, ldv, kc, o, ldo, d_w, tau);
+    if (kc <= 16) rc = mixed_product_kt<1>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 32) rc = mixed_product_kt<2>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 48) rc = mixed_product_kt<3>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 64) rc = mixed_product_kt<4>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else if (kc <= 96) rc = mixed_product_kt<6>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
+    else rc = mixed_product_kt<8>(h, v, ldv, kc, o, ldo, d_w, tau, c0);
     if (rc != SAIGE_OK) return rc;
   }
   return SAIGE_OK;
@@ -867,7 +869,7 @@ static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const
 
 template <int KT>
 static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
-                           size_t ldo, const double* d_w, const double* tau) {
+                           size_t ldo, const double* d_w, const double* tau, size_t c0) {
   constexpr int KP = 16 * KT;
   constexpr int RT = 4;
   constexpr int NW = KT <= 2 ? 4 : 8;
@@ -912,6 +914,8 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
   i8_stage1_epilogue_kernel<KP><<<(unsigned)(m_pad / 64), 256, 0, s>>>(
       ms->ipart1.as<int>(), splits1, m_pad, m, k, psum, pmax, pdot, 1.0 / (double)n, nch_n,
       ms->sc.as<double>(), ms->p2.as<double>(), ms->zs.as<double>(), ms->w.as<double>(), m);
+  LocoCodes lc;
+  if (loco_codes(h, c0, k, lc)) loco_mask(h, ms->w.as<double>(), 1, m, m, k, lc, s);
 
   // ---- W: column sums over real markers, maxima, slices + S2p partials ----
   col_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(ms->w.as<double>(), m,
@@ -946,13 +950,13 @@ int grm_int8_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
     double* o = d_out + c0 * ldo;
     int rc;
     switch (ceil_div(3 * kc, 16)) {
-      case 1: rc = int8_product_kt<1>(h, v, ldv, kc, o, ldo, d_w, tau); break;
-      case 2: rc = int8_product_kt<2>(h, v, ldv, kc, o, ldo, d_w, tau); break;
-      case 3: rc = int8_product_kt<3>(h, v, ldv, kc, o, ldo, d_w, tau); break;
-      case 4: rc = int8_product_kt<4>(h, v, ldv, kc, o, ldo, d_w, tau); break;
-      case 5: rc = int8_product_kt<5>(h, v, ldv, kc, o, ldo, d_w, tau); break;
-      case 6: rc = int8_product_kt<6>(h, v, ldv, kc, o, ldo, d_w, tau); break;
-      default: rc = int8_product_kt<7>(h, v, ldv, kc, o, ldo, d_w, tau); break;
+      case 1: rc = int8_product_kt<1>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 2: rc = int8_product_kt<2>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 3: rc = int8_product_kt<3>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 4: rc = int8_product_kt<4>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 5: rc = int8_product_kt<5>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 6: rc = int8_product_kt<6>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      default: rc = int8_product_kt<7>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
     }
     if (rc != SAIGE_OK) return rc;
   }

@@ -1034,6 +1034,7 @@ class ProductBatcher {
     size_t ldo;
     const double* w;
     double tau[2];
+    int loco = -1;  // chromosome left out of this request's GRM (-1: none)
     int rc = SAIGE_OK;
     bool done = false;
   };
@@ -1068,6 +1069,19 @@ class ProductBatcher {
   }
   int run(const std::vector<Request*>& reqs) {
     ++joined_products;
+    // per-column left-out chromosomes of the joined product
+    std::vector<int> codes;
+    bool loco = false;
+    for (const Request* q : reqs) {
+      codes.insert(codes.end(), q->k, q->loco);
+      loco = loco || q->loco >= 0;
+    }
+    h_->loco_req = loco ? codes.data() : nullptr;
+    const int rc = run_joined(reqs);
+    h_->loco_req = nullptr;
+    return rc;
+  }
+  int run_joined(const std::vector<Request*>& reqs) {
     if (reqs.size() == 1) {
       const Request& q = *reqs[0];
       joined_columns += q.k;
@@ -1109,6 +1123,7 @@ class ProductBatcher {
 
 struct BatchedSink : ProductSink {
   ProductBatcher* b;
+  int loco = -1;
   explicit BatchedSink(ProductBatcher* bb) : b(bb) {}
   int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
               const double* tau) override {
@@ -1123,6 +1138,7 @@ struct BatchedSink : ProductSink {
     r.w = w;
     r.tau[0] = tau ? tau[0] : 0.0;
     r.tau[1] = tau ? tau[1] : 0.0;
+    r.loco = loco;
     return b->submit(r);
   }
 };
@@ -1193,9 +1209,78 @@ static int fit_null_multi_impl(int device, const uint8_t* bed_body, bool on_devi
   return rc;
 }
 
+// Leave-one-chromosome-out refits (glmm/loco.rs:54-114): one AI-REML per chromosome code
+// 0..n_chrom-1, each on the GRM without that chromosome's markers.  The refits run on host
+// threads and their products meet in a ProductBatcher: one joined product streams the
+// genotypes once for every refit still iterating, each column masked for its chromosome.
+static int fit_loco_impl(saige_grm* h, const double* y, const double* x, size_t n, size_t p,
+                         int trait_type, const saige_ai_reml_config* cfg, size_t n_chrom,
+                         saige_ai_reml_result* results, double* mu, double* w,
+                         double* residuals, double* alpha) {
+  ProductBatcher batcher(h, n_chrom);
+  std::vector<int> rcs(n_chrom, SAIGE_OK);
+  std::vector<std::string> errs(n_chrom);
+  std::vector<std::thread> threads;
+  for (size_t c = 0; c < n_chrom; ++c) {
+    threads.emplace_back([&, c]() {
+      BatchedSink sink(&batcher);
+      sink.loco = (int)c;
+      int r = hipSetDevice(h->device) == hipSuccess ? SAIGE_OK : SAIGE_ERR_DEVICE;
+      RemlState st;
+      if (r == SAIGE_OK)
+        r = fit_ai_reml_dev(h, &sink, y, x, n, p, trait_type, cfg, &results[c], st);
+      batcher.leave();
+      if (r == SAIGE_OK) {
+        hipStream_t s = h->stream;
+        std::vector<double> hm(n);
+        hipError_t e = hipMemcpyAsync(hm.data(), st.mu.p, n * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && w) e = hipMemcpyAsync(w + c * n, st.w.p, n * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && alpha)
+          e = hipMemcpyAsync(alpha + c * p, st.coef.p, p * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+          set_error("LOCO outputs: %s", hipGetErrorString(e));
+          r = SAIGE_ERR_DEVICE;
+        } else {
+          if (mu) std::copy(hm.begin(), hm.end(), mu + c * n);
+          if (residuals)
+            for (size_t i = 0; i < n; ++i) residuals[c * n + i] = y[i] - hm[i];
+        }
+      }
+      if (r != SAIGE_OK) errs[c] = saige_last_error();
+      rcs[c] = r;
+    });
+  }
+  for (auto& th : threads) th.join();
+  for (size_t c = 0; c < n_chrom; ++c)
+    if (rcs[c] != SAIGE_OK) {
+      set_error("chromosome code %zu: %s", c, errs[c].c_str());
+      return rcs[c];
+    }
+  return SAIGE_OK;
+}
+
 }  // namespace saige
 
 extern "C" {
+
+int saige_fit_ai_reml_loco(saige_grm* h, const double* y, const double* x, size_t n, size_t p,
+                           int trait_type, const saige_ai_reml_config* cfg, size_t n_chrom,
+                           saige_ai_reml_result* results, double* mu, double* w,
+                           double* residuals, double* alpha) {
+  clear_error();
+  SAIGE_ARG(h && y && x && cfg && results, "null pointer");
+  SAIGE_ARG(n == h->g.n, "y length != GRM samples");
+  SAIGE_ARG(p >= 1, "p >= 1");
+  SAIGE_ARG(trait_type >= 0 && trait_type <= 2, "trait_type");
+  SAIGE_ARG(n_chrom >= 1 && n_chrom <= 64, "1 <= n_chrom <= 64");
+  std::lock_guard<std::recursive_mutex> lk(h->mu);
+  SAIGE_ARG(h->loco_n_chrom > 0 && (size_t)h->loco_n_chrom == n_chrom,
+            "n_chrom must equal the number of chromosome codes set on the handle");
+  SAIGE_HIP(hipSetDevice(h->device));
+  return fit_loco_impl(h, y, x, n, p, trait_type, cfg, n_chrom, results, mu, w, residuals,
+                       alpha);
+}
 
 int saige_fit_null(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,
                    size_t m_total, const uint32_t* sample_idx, size_t n, const double* y,

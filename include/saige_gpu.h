@@ -170,6 +170,24 @@ int saige_fit_ai_reml(saige_grm* h, const double* y, const double* x, size_t n, 
                       saige_ai_reml_result* result, double* mu, double* w, double* residuals,
                       double* eta, double* alpha);
 
+/* Leave-one-chromosome-out (saige-core/src/glmm/loco.rs:54-114).
+ *   saige_grm_set_marker_chromosomes: chromosome code per GRM marker (handle order), codes
+ *     0..C-1 each used by at least one marker (NULL clears).  The host maps chromosome names
+ *     to codes in sorted order, the order run_loco fits them (loco.rs:64-71).
+ *   saige_grm_mat_mat_loco: the GRM product over the markers not on chromosome
+ *     exclude_chrom (OnTheFlyGrm::new on the kept markers, loco.rs:78-96): the same resident
+ *     genotypes with that chromosome's rows of Z^T v set to 0 and 1/M_c for 1/M.
+ *   saige_fit_ai_reml_loco: fit_ai_reml once per chromosome code on the GRM without it
+ *     (loco.rs:98-111); the C refits run together and share every product.  Outputs per
+ *     code c: results[c], and (each may be NULL) mu, w, residuals (n x C) and alpha (p x C). */
+int saige_grm_set_marker_chromosomes(saige_grm* h, const int32_t* chrom);
+int saige_grm_mat_mat_loco(saige_grm* h, const double* v, size_t ldv, size_t k,
+                           int32_t exclude_chrom, double* out, size_t ldo);
+int saige_fit_ai_reml_loco(saige_grm* h, const double* y, const double* x, size_t n, size_t p,
+                           int trait_type, const saige_ai_reml_config* cfg, size_t n_chrom,
+                           saige_ai_reml_result* results, double* mu, double* w,
+                           double* residuals, double* alpha);
+
 /* ------------------------------------------------------------------------------
  * (X'WX)^{-1} X'W as produced by fit_null.rs:289-307 (a p*n flat buffer filled
  * element (j, i) -> [j*n + i], which every consumer reads as col-major p x n,

@@ -344,6 +344,22 @@ def estimate_vr(grm, dosages, macs, mu, w, tau, x_cm, a_flat, trait=TRAIT_BINARY
     return vr.value, used.value, per[:used.value]
 
 
+def run_loco(y, x_cm, marker_chroms, dosages, afs, trait=TRAIT_BINARY, grm_mode=GRM_F64,
+             **reml):
+    """run_loco (saige-core/src/glmm/loco.rs:54-114): for every chromosome, in sorted order
+    of the names, fit_ai_reml on the OnTheFlyGrm of the markers on the other chromosomes.
+    Returns {chromosome: fit_ai_reml result}."""
+    chroms = sorted(set(marker_chroms))
+    d = _f64(dosages)
+    a = _f64(afs)
+    out = {}
+    for c in chroms:
+        keep = np.array([mc != c for mc in marker_chroms])
+        g = Grm(d[keep], a[keep], grm_mode)
+        out[c] = fit_ai_reml(g, y, x_cm, trait=trait, **reml)
+    return out
+
+
 def bgen_layout2_dosages(block):
     """parse_layout2_probabilities (bgen.rs:373-430) of one decompressed block."""
     b = np.frombuffer(bytes(block), dtype=np.uint8)
