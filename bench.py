@@ -69,6 +69,63 @@ def time_product(sg, hip, d_bed, bpm, n, markers, af, precision, k, reps):
     return ms, kt.value / max(kl.value, 1), kl.value
 
 
+def run_loco(args):
+    """LOCO (glmm/loco.rs:54-114) on the families cohort: one AI-REML refit per chromosome
+    (22 chromosomes of equal marker count), the refits' GRM products joined; the int8 MFMA
+    path timed against the plain FP64 path and against one full-GRM fit; one JSON line."""
+    import csaige_amd as sg
+    from csaige_amd import synth
+    t0 = time.time()
+    c = synth.families(n_fam=args.n_fam, kids=3, m=args.markers, seed=2, h2=0.4, prevalence=0.05)
+    n, m = c.n, c.bed.shape[0]
+    n_chrom = 22
+    chrom = (np.arange(m) * n_chrom // m).astype(np.int32)
+    ac, nv, af, mac = sg.marker_stats(c.bed, c.bpm, n, m)
+    log(f"loco: cohort n={n} m={m}, {n_chrom} chromosomes ({time.time() - t0:.1f}s)")
+    out = {}
+    for prec in [sg.PREC_INT8] + ([] if args.no_fp64 else [sg.PREC_FP64]):
+        g = sg.OnTheFlyGrm(c.bed, c.bpm, n, af, precision=prec)
+        g.set_marker_chromosomes(chrom)
+        sg.fit_ai_reml(g, c.y, c.x, tol=1e-5)  # warm-up: kernels, device memory cache
+        t = time.perf_counter()
+        full = sg.fit_ai_reml(g, c.y, c.x, tol=1e-5)
+        dt_full = time.perf_counter() - t
+        t = time.perf_counter()
+        r = g.fit_ai_reml_loco(c.y, c.x, tol=1e-5)
+        dt = time.perf_counter() - t
+        tag = "int8" if prec == sg.PREC_INT8 else "fp64"
+        log(f"loco {tag}: {n_chrom} refits in {dt:.2f}s ({dt / n_chrom:.3f}s per refit); one "
+            f"full-GRM fit {dt_full:.3f}s; tau_g range {min(x['tau'][1] for x in r):.6g}.."
+            f"{max(x['tau'][1] for x in r):.6g}, full {full['tau'][1]:.6g}")
+        out[tag] = (dt, dt_full, r)
+        g.close()
+    dt8, full8, r8 = out["int8"]
+    acc = None
+    if "fp64" in out:
+        rf = out["fp64"][2]
+        acc = {"max_tau_rel_diff": float(max(abs(a["tau"][1] - b["tau"][1]) / b["tau"][1]
+                                             for a, b in zip(r8, rf))),
+               "iterations_equal": bool(all(a["iterations"] == b["iterations"]
+                                            for a, b in zip(r8, rf)))}
+        log(f"loco accuracy int8 vs fp64: {acc}")
+    line = {
+        "metric": "LOCO refits wall-s, 1 GPU: int8 mixed vs FP64",
+        "value": dt8, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": dt8 * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int8 MFMA / int32 accumulate / fp64 solver",
+        "data": "synthetic (seeded families cohort)",
+        "config": {"workload": f"LOCO AI-REML refits, {args.n_fam} nuclear families x 5 = {n} "
+                               f"samples, {m} markers in {n_chrom} chromosomes, binary 5%",
+                   "per_refit_s": dt8 / n_chrom, "full_grm_fit_s": full8,
+                   "fp64_wall_s": out["fp64"][0] if "fp64" in out else None,
+                   "speedup_vs_fp64": out["fp64"][0] / dt8 if "fp64" in out else None,
+                   "tau_g": [float(x["tau"][1]) for x in r8],
+                   "iterations": [int(x["iterations"]) for x in r8],
+                   "accuracy": acc},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_biobank(args):
     """Several phenotypes at biobank scale: 400,000 samples (15% in half-sibling pairs),
     150,000 markers resident in HBM, 4 binary traits at 0.5/1/5/20% prevalence fitted
@@ -274,9 +331,11 @@ def main():
                     help="HBM bytes per GEMM launch measured with rocprofv3 counters")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (counter calibration)")
-    ap.add_argument("--workload", default="families", choices=["families", "biobank", "learned"],
+    ap.add_argument("--workload", default="families",
+                    choices=["families", "biobank", "learned", "loco"],
                     help="families: the default fit-null benchmark; biobank: several "
-                         "phenotypes at 400,000 x 150,000; learned: the SPA root proposal")
+                         "phenotypes at 400,000 x 150,000; learned: the SPA root proposal; "
+                         "loco: leave-one-chromosome-out refits")
     ap.add_argument("--ls-n", type=int, default=50000)
     ap.add_argument("--ls-variants", type=int, default=524288,
                     help="learned: variants per prevalence for training (2 roots each)")
@@ -304,6 +363,9 @@ def main():
         return
     if args.workload == "learned":
         run_learned(args)
+        return
+    if args.workload == "loco":
+        run_loco(args)
         return
 
     import csaige_amd as sg
