@@ -420,6 +420,34 @@ class ScoreTestEngine:
                                              _p(batch.info, _lib.u32p), batch.n_file, sp, out))
         return results_to_dict(out)
 
+    def test_regions(self, G, region_sizes, weights, rho=None):
+        """BURDEN / SKAT / SKAT-O (region.rs:76-345) for regions of consecutive rows of
+        ``G`` ((nvar, n) dosages, NaN = missing) with ``region_sizes`` rows each; returns
+        a dict of arrays (pvalue = SKAT-O, burden_pvalue, skat_pvalue, rho, n_markers)."""
+        G = _f64(np.atleast_2d(G))
+        nvar = G.shape[0]
+        sizes = np.asarray(region_sizes, dtype=np.int64)
+        starts = np.zeros(len(sizes) + 1, np.uint64)
+        starts[1:] = np.cumsum(sizes)
+        if int(starts[-1]) != nvar:
+            raise ValueError("region sizes must add up to the number of rows of G")
+        w = _f64(weights)
+        rr = None if rho is None else _f64(rho)
+        out = (_lib.RegionResult * max(len(sizes), 1))()
+        check(load().saige_region_tests(self.h, _p(G, _lib.dp), self.n, nvar,
+                                        starts.ctypes.data_as(C.POINTER(C.c_uint64)), len(sizes),
+                                        _p(w, _lib.dp), None if rr is None else _p(rr, _lib.dp),
+                                        0 if rr is None else len(rr), out))
+        raw = np.frombuffer(out, dtype=np.dtype({"names": ["pvalue", "burden_pvalue",
+                                                           "skat_pvalue", "rho", "n_markers"],
+                                                 "formats": ["f8", "f8", "f8", "f8", "u4"],
+                                                 "offsets": [getattr(_lib.RegionResult, f).offset
+                                                             for f in ("pvalue", "burden_pvalue",
+                                                                       "skat_pvalue", "rho",
+                                                                       "n_markers")],
+                                                 "itemsize": C.sizeof(_lib.RegionResult)}))
+        return {k: raw[k][:len(sizes)].copy() for k in raw.dtype.names}
+
     def set_root_proposal(self, mlp):
         """Start the SPA Newton iterations from the learned proposal (None: from 0)."""
         self._mlp = mlp

@@ -82,6 +82,11 @@ class VariantResult(C.Structure):
                 ("spa_features", C.c_float * 6)]
 
 
+class RegionResult(C.Structure):
+    _fields_ = [("pvalue", C.c_double), ("burden_pvalue", C.c_double),
+                ("skat_pvalue", C.c_double), ("rho", C.c_double), ("n_markers", C.c_uint32)]
+
+
 class SpaResult(C.Structure):
     _fields_ = [("pvalue", C.c_double), ("converged", C.c_int32), ("is_spa", C.c_int32),
                 ("iters1", C.c_int32), ("iters2", C.c_int32), ("root1", C.c_double),
@@ -145,6 +150,9 @@ SIGNATURES = [
     ("saige_bgen_dosages", C.c_int, [C.c_int, u8p, sz, sz, u32p, C.c_uint32, u32p, sz, dp]),
     ("saige_test_markers_bgen", C.c_int, [vp, u8p, sz, sz, u32p, C.c_uint32, u32p,
                                           C.POINTER(VariantResult)]),
+    ("saige_region_tests", C.c_int, [vp, dp, sz, sz, C.POINTER(C.c_uint64), sz, dp, dp, sz,
+                                     C.POINTER(RegionResult)]),
+    ("saige_chi2_sf", C.c_double, [C.c_double, C.c_double]),
     ("saige_mlp_create", C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
     ("saige_mlp_destroy", None, [vp]),
     ("saige_mlp_set_params", C.c_int, [vp, fp, fp]),

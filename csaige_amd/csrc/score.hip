@@ -1989,3 +1989,340 @@ int saige_spa_binary(int device, const double* mu, const double* g, size_t n, do
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Region tests: BURDEN, SKAT and SKAT-O (saige-core/src/score_test/region.rs:76-345).
+// ---------------------------------------------------------------------------------
+// For a region's weighted dosages gw_j = w_j g_j (missing -> 0, region.rs:86-94) and
+// g~_j = gw_j - X (A gw_j), the scores S_j = g~_j' res and Phi = tau_e g~' diag(mu2) g~
+// (:75-135) come from per-variant sums and one Gram per region:
+//   S_j    = w_j (g_j'res - (A g_j)'(X'res))
+//   Phi_jk = tau_e w_j w_k (g_j'M2 g_k - (A g_j)'(X'M2 g_k) - (A g_k)'(X'M2 g_j)
+//                           + (A g_j)'(X'M2X)(A g_k))
+// The per-variant sums are the score-test sums kernel's; the Gram g' diag(mu2) g is
+// computed in 64 x 64 tiles (region_gram_kernel), its sample range split over the grid and
+// the partials added in split order.
+namespace saige {
+
+constexpr int kRegTile = 64;
+constexpr int kRegChunk = 32;
+
+struct RegionTile {
+  uint32_t rj, rk;  // first variant rows of the tile's j and k blocks
+  uint32_t mj, mk;  // their sizes (<= 64)
+};
+
+// part[(tile * splits + split) * 4096 + j * 64 + k] = sum over the split's samples of
+// g_j mu2 g_k (missing dosages as 0)
+__global__ __launch_bounds__(256) void region_gram_kernel(const double* __restrict__ g, size_t ldg,
+                                                          const double* __restrict__ feat, int nf,
+                                                          int mu2_col, size_t n, size_t sps,
+                                                          const RegionTile* __restrict__ tiles,
+                                                          double* __restrict__ part) {
+  __shared__ double a[kRegTile][kRegChunk + 1], b[kRegTile][kRegChunk + 1];
+  const RegionTile t = tiles[blockIdx.x];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const size_t i0 = (size_t)blockIdx.y * sps, i1 = min(i0 + sps, n);
+  double acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+  for (size_t c0 = i0; c0 < i1; c0 += kRegChunk) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kRegTile * kRegChunk; e += 256) {
+      const int row = e / kRegChunk, col = e % kRegChunk;
+      const size_t i = c0 + col;
+      double va = 0.0, vb = 0.0;
+      if (i < i1) {
+        if ((uint32_t)row < t.mj) {
+          const double x = g[(t.rj + row) * ldg + i];
+          va = isnan(x) ? 0.0 : x * feat[i * nf + mu2_col];
+        }
+        if ((uint32_t)row < t.mk) {
+          const double x = g[(t.rk + row) * ldg + i];
+          vb = isnan(x) ? 0.0 : x;
+        }
+      }
+      a[row][col] = va;
+      b[row][col] = vb;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int s = 0; s < kRegChunk; ++s) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) av[r] = a[4 * ty + r][s];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bv[c] = b[4 * tx + c][s];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = fma(av[r], bv[c], acc[r][c]);
+    }
+  }
+  double* o = part + ((size_t)blockIdx.x * gridDim.y + blockIdx.y) * (kRegTile * kRegTile);
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[(4 * ty + r) * kRegTile + 4 * tx + c] = acc[r][c];
+}
+
+__global__ void region_gram_reduce(const double* __restrict__ part, size_t splits, size_t ntile,
+                                   double* __restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t tile = blockIdx.y;
+  if (e >= (size_t)kRegTile * kRegTile || tile >= ntile) return;
+  double s = 0.0;
+  for (size_t q = 0; q < splits; ++q) s += part[(tile * splits + q) * kRegTile * kRegTile + e];
+  out[tile * kRegTile * kRegTile + e] = s;
+}
+
+// 1 - P(a, x) with P the regularized lower incomplete gamma function: the power series for
+// x < a + 1 and the Legendre continued fraction (modified Lentz) otherwise, as in Press et
+// al., Numerical Recipes, section 6.2.  This is 1 - ChiSquared::new(2a).cdf(2x) for the
+// statrs call of region.rs:174, 230, 323.
+static double gamma_p(double a, double x) {
+  if (!(x > 0.0)) return 0.0;
+  if (std::isinf(x)) return 1.0;
+  const double lg = std::lgamma(a);
+  if (x < a + 1.0) {
+    double ap = a, del = 1.0 / a, sum = del;
+    for (int k = 0; k < 100000; ++k) {
+      ap += 1.0;
+      del *= x / ap;
+      sum += del;
+      if (std::fabs(del) < std::fabs(sum) * 1e-17) break;
+    }
+    return sum * std::exp(-x + a * std::log(x) - lg);
+  }
+  const double tiny = 1e-300;
+  double b = x + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
+  for (int k = 1; k < 100000; ++k) {
+    const double an = -k * (k - a);
+    b += 2.0;
+    d = an * d + b;
+    if (std::fabs(d) < tiny) d = tiny;
+    c = b + an / c;
+    if (std::fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (std::fabs(del - 1.0) < 1e-17) break;
+  }
+  return 1.0 - std::exp(-x + a * std::log(x) - lg) * h;
+}
+static double chi2_sf(double df, double x) { return 1.0 - gamma_p(0.5 * df, 0.5 * x); }
+
+// burden_test, skat_test and skat_o_test (region.rs:143-339) from the scores and Phi
+static void region_pvalues(const std::vector<double>& S, const std::vector<double>& phi, size_t m,
+                           const double* rho, size_t n_rho, saige_region_result* r) {
+  r->pvalue = r->burden_pvalue = r->skat_pvalue = 1.0;
+  r->rho = 0.0;
+  r->n_markers = (uint32_t)m;
+  if (m == 0) return;
+  auto P = [&](size_t j, size_t k) { return phi[j * m + k]; };
+  double sum_score = 0.0, q_skat = 0.0, sum_phi = 0.0, trace_phi = 0.0, phi_sq = 0.0;
+  for (size_t j = 0; j < m; ++j) sum_score += S[j];
+  for (size_t j = 0; j < m; ++j) q_skat += S[j] * S[j];
+  for (size_t j = 0; j < m; ++j)
+    for (size_t k = 0; k < m; ++k) sum_phi += P(j, k);
+  for (size_t j = 0; j < m; ++j) trace_phi += P(j, j);
+  for (size_t j = 0; j < m; ++j)
+    for (size_t k = 0; k < m; ++k) phi_sq += P(j, k) * P(j, k);
+  // burden (:157-175)
+  if (!(sum_phi < 1e-30)) r->burden_pvalue = chi2_sf(1.0, sum_score * sum_score / sum_phi);
+  // SKAT (:191-233)
+  if (!(trace_phi < 1e-30)) {
+    const double var_q = 2.0 * phi_sq;
+    if (!(var_q < 1e-30)) {
+      const double kappa = var_q / (2.0 * trace_phi);
+      const double df = 2.0 * trace_phi * trace_phi / var_q;
+      if (df > 0.0 && kappa > 0.0) r->skat_pvalue = chi2_sf(df, q_skat / kappa);
+    }
+  }
+  // SKAT-O (:246-339)
+  const double q_burden = sum_score * sum_score;
+  std::vector<double> row_sums(m, 0.0);
+  for (size_t j = 0; j < m; ++j)
+    for (size_t k = 0; k < m; ++k) row_sums[j] += P(j, k);
+  double best = 1.0, best_rho = 0.0;
+  for (size_t q = 0; q < n_rho; ++q) {
+    const double rh = rho[q];
+    const double q_rho = (1.0 - rh) * q_skat + rh * q_burden;
+    const double mean_q = (1.0 - rh) * trace_phi + rh * sum_phi;
+    if (mean_q < 1e-30) continue;
+    double var_q = 0.0;
+    for (size_t j = 0; j < m; ++j)
+      for (size_t k = 0; k < m; ++k) {
+        const double a = (1.0 - rh) * P(j, k) + rh * row_sums[k];
+        const double b = (1.0 - rh) * P(k, j) + rh * row_sums[j];
+        var_q += a * b;
+      }
+    var_q *= 2.0;
+    if (var_q < 1e-30) continue;
+    const double kappa = var_q / (2.0 * mean_q);
+    const double df = 2.0 * mean_q * mean_q / var_q;
+    if (df > 0.0 && kappa > 0.0) {
+      const double pv = chi2_sf(df, q_rho / kappa);
+      if (pv < best) {
+        best = pv;
+        best_rho = rh;
+      }
+    }
+  }
+  r->pvalue = best;
+  r->rho = best_rho;
+}
+
+template <int P>
+static int region_batch_p(saige_engine* e, const double* dg, size_t nvar,
+                          const std::vector<size_t>& starts, const double* weights,
+                          const double* rho, size_t n_rho, saige_region_result* out) {
+  hipStream_t s = e->stream;
+  const size_t n = e->n;
+  constexpr int NS = Layout<P>::NS, NF = Layout<P>::NF;
+  const size_t nreg = starts.size() - 1;
+  // per-variant sums: g'A (p), g'res, g'M2 x (p)
+  const size_t nvar_pad = round_up(nvar, 128);
+  const size_t splits = std::max<size_t>(1, std::min<size_t>(ceil_div(2048, ceil_div(nvar, 64)),
+                                                              ceil_div(n, 512)));
+  const size_t sps = round_up(ceil_div(n, splits), 16);
+  const size_t sp = ceil_div(n, sps);
+  SAIGE_TRY(e->part.ensure((sp + 1) * NS * nvar_pad * 8));
+  score_sums_kernel<P, LoadF64><<<dim3((unsigned)ceil_div(nvar, 64), (unsigned)sp), 64, 0, s>>>(
+      LoadF64{dg, n, 1}, e->feat.as<double>(), n, nvar, sps, e->part.as<double>(), nvar_pad);
+  double* sums = e->part.as<double>() + sp * NS * nvar_pad;
+  sums_reduce_kernel<<<dim3((unsigned)ceil_div(nvar_pad, 256), (unsigned)NS), 256, 0, s>>>(
+      e->part.as<double>(), sp, NS, nvar_pad, sums);
+  // Gram tiles (k block >= j block) of every region
+  std::vector<RegionTile> tiles;
+  std::vector<size_t> tile0(nreg + 1, 0);
+  for (size_t r = 0; r < nreg; ++r) {
+    tile0[r] = tiles.size();
+    const size_t a = starts[r], m = starts[r + 1] - starts[r];
+    for (size_t j = 0; j < m; j += kRegTile)
+      for (size_t k = j; k < m; k += kRegTile)
+        tiles.push_back(RegionTile{(uint32_t)(a + j), (uint32_t)(a + k),
+                                   (uint32_t)std::min<size_t>(kRegTile, m - j),
+                                   (uint32_t)std::min<size_t>(kRegTile, m - k)});
+  }
+  tile0[nreg] = tiles.size();
+  const size_t nt = tiles.size();
+  std::vector<double> hs(NS * nvar_pad), hg(nt * kRegTile * kRegTile);
+  if (nt > 0) {
+    const size_t gsplits = std::max<size_t>(1, std::min<size_t>(ceil_div(2048, nt),
+                                                                ceil_div(n, 4096)));
+    const size_t gsps = round_up(ceil_div(n, gsplits), kRegChunk);
+    const size_t gsp = ceil_div(n, gsps);
+    const size_t tile_bytes = round_up(nt * sizeof(RegionTile), 256);
+    SAIGE_TRY(e->res.ensure(tile_bytes + (gsp + 1) * nt * kRegTile * kRegTile * 8));
+    RegionTile* dt = reinterpret_cast<RegionTile*>(e->res.as<uint8_t>());
+    double* gpart = reinterpret_cast<double*>(e->res.as<uint8_t>() + tile_bytes);
+    double* gram = gpart + gsp * nt * kRegTile * kRegTile;
+    SAIGE_HIP(hipMemcpyAsync(dt, tiles.data(), nt * sizeof(RegionTile), hipMemcpyHostToDevice, s));
+    region_gram_kernel<<<dim3((unsigned)nt, (unsigned)gsp), 256, 0, s>>>(
+        dg, n, e->feat.as<double>(), NF, P + 1, n, gsps, dt, gpart);
+    region_gram_reduce<<<dim3((unsigned)ceil_div(kRegTile * kRegTile, 256), (unsigned)nt), 256, 0,
+                         s>>>(gpart, gsp, nt, gram);
+    SAIGE_HIP(hipGetLastError());
+    SAIGE_HIP(hipMemcpyAsync(hg.data(), gram, hg.size() * 8, hipMemcpyDeviceToHost, s));
+  }
+  SAIGE_HIP(hipMemcpyAsync(hs.data(), sums, hs.size() * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  const EngineConst& ec = e->ec;
+  for (size_t r = 0; r < nreg; ++r) {
+    const size_t a = starts[r], m = starts[r + 1] - a;
+    std::vector<double> S(m), phi(m * m), ag(m * P), xg(m * P), gram(m * m);
+    for (size_t t = tile0[r]; t < tile0[r + 1]; ++t) {
+      const RegionTile& T = tiles[t];
+      for (uint32_t j = 0; j < T.mj; ++j)
+        for (uint32_t k = 0; k < T.mk; ++k) {
+          const double v = hg[t * kRegTile * kRegTile + j * kRegTile + k];
+          const size_t jj = T.rj - a + j, kk = T.rk - a + k;
+          gram[jj * m + kk] = v;
+          gram[kk * m + jj] = v;  // the tile below the diagonal mirrors this one
+        }
+    }
+    for (size_t j = 0; j < m; ++j) {
+      const size_t v = a + j;
+      double gres = hs[(2 + P) * nvar_pad + v];
+      for (int q = 0; q < P; ++q) {
+        ag[j * P + q] = hs[(2 + q) * nvar_pad + v];
+        xg[j * P + q] = hs[(4 + P + q) * nvar_pad + v];
+        gres -= ag[j * P + q] * ec.xtr[q];
+      }
+      S[j] = weights[v] * gres;
+    }
+    for (size_t j = 0; j < m; ++j)
+      for (size_t k = 0; k < m; ++k) {
+        double v = gram[j * m + k];
+        for (int q = 0; q < P; ++q) {
+          v -= ag[j * P + q] * xg[k * P + q] + ag[k * P + q] * xg[j * P + q];
+          double t = 0.0;
+          for (int q2 = 0; q2 < P; ++q2) t += ec.xm2x[q * P + q2] * ag[k * P + q2];
+          v += ag[j * P + q] * t;
+        }
+        phi[j * m + k] = ec.tau_e * weights[a + j] * weights[a + k] * v;
+      }
+    region_pvalues(S, phi, m, rho, n_rho, &out[r]);
+  }
+  return SAIGE_OK;
+}
+
+}  // namespace saige
+
+extern "C" double saige_chi2_sf(double df, double x) { return saige::chi2_sf(df, x); }
+
+extern "C" int saige_region_tests(saige_engine* e, const double* g, size_t ldg, size_t nvar,
+                                  const uint64_t* region_start, size_t n_regions,
+                                  const double* weights, const double* rho, size_t n_rho,
+                                  saige_region_result* out) {
+  clear_error();
+  SAIGE_ARG(e && region_start && (n_regions == 0 || out), "null pointer");
+  SAIGE_ARG(nvar == 0 || (g && weights), "null pointer");
+  SAIGE_ARG(ldg >= e->n, "ldg < n");
+  SAIGE_ARG(region_start[0] == 0 && region_start[n_regions] == nvar, "region_start bounds");
+  for (size_t r = 0; r < n_regions; ++r)
+    SAIGE_ARG(region_start[r + 1] >= region_start[r], "region_start must not decrease");
+  static const double kRho[11] = {0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0};
+  if (!rho) {  // default_rho_values (region.rs:368-370)
+    rho = kRho;
+    n_rho = 11;
+  }
+  std::lock_guard<std::mutex> lk(e->mu);
+  SAIGE_HIP(hipSetDevice(e->device));
+  hipStream_t s = e->stream;
+  const size_t n = e->n;
+  // groups of whole regions with at most kHostBatch variants (a larger region alone)
+  size_t r0 = 0;
+  while (r0 < n_regions) {
+    size_t r1 = r0 + 1;
+    while (r1 < n_regions && region_start[r1 + 1] - region_start[r0] <= kHostBatch) ++r1;
+    const size_t v0 = region_start[r0], nb = region_start[r1] - v0;
+    std::vector<size_t> starts(r1 - r0 + 1);
+    for (size_t r = r0; r <= r1; ++r) starts[r - r0] = region_start[r] - v0;
+    if (nb == 0) {
+      for (size_t r = r0; r < r1; ++r) region_pvalues({}, {}, 0, rho, n_rho, &out[r]);
+    } else {
+      SAIGE_TRY(e->in.ensure(nb * n * 8));
+      double* dg = e->in.as<double>();
+      SAIGE_HIP(hipMemcpy2DAsync(dg, n * 8, g + v0 * ldg, ldg * 8, n * 8, nb,
+                                 hipMemcpyHostToDevice, s));
+      int rc = SAIGE_ERR_ARG;
+      switch (e->p) {
+        case 1: rc = region_batch_p<1>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
+        case 2: rc = region_batch_p<2>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
+        case 3: rc = region_batch_p<3>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
+        case 4: rc = region_batch_p<4>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
+        case 5: rc = region_batch_p<5>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
+        case 6: rc = region_batch_p<6>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
+        case 7: rc = region_batch_p<7>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
+        case 8: rc = region_batch_p<8>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
+      }
+      if (rc != SAIGE_OK) return rc;
+    }
+    r0 = r1;
+  }
+  return SAIGE_OK;
+}

@@ -307,6 +307,21 @@ typedef struct {
   float spa_features[6];           /* inputs of the learned root proposal (diagnostic)  */
 } saige_variant_result;            /* numeric fields of SingleVariantResult (:23-74) */
 
+/* Region tests (saige-core/src/score_test/region.rs:76-345): BURDEN, SKAT and SKAT-O with
+ * the Satterthwaite moment match of the reference, for regions of consecutive dosage rows.
+ *   g: nvar x ldg (row per variant, NaN = missing, which region tests read as 0,
+ *      region.rs:86-94); region_start[n_regions + 1]: first row of each region, the last
+ *      entry = nvar; weights[nvar]: per-variant weights (beta_weights(MAF, 1, 25) in
+ *      saige test, assoc_test.rs:299); rho[n_rho]: the SKAT-O grid (NULL: the reference's
+ *      default 0, 0.1, ..., 1).  pvalue is SKAT-O's (the smallest over rho, whose rho is
+ *      reported).  p-values are 1 - ChiSquared(df).cdf(x) as the reference computes them,
+ *      the CDF through the regularized incomplete gamma function. */
+typedef struct {
+  double pvalue, burden_pvalue, skat_pvalue;
+  double rho;
+  uint32_t n_markers;
+} saige_region_result;
+
 int saige_engine_create(int device, const double* mu, const double* mu2,
                         const double* residuals, double tau_e, double tau_g,
                         const double* xvx_inv_xv, const double* x, size_t n, size_t p,
@@ -346,6 +361,11 @@ int saige_bgen_dosages(int device, const uint8_t* blocks, size_t ld, size_t nvar
 int saige_test_markers_bgen(saige_engine* e, const uint8_t* blocks, size_t ld, size_t nvar,
                             const uint32_t* info, uint32_t n_file, const uint32_t* sample_idx,
                             saige_variant_result* out);
+int saige_region_tests(saige_engine* e, const double* g, size_t ldg, size_t nvar,
+                       const uint64_t* region_start, size_t n_regions, const double* weights,
+                       const double* rho, size_t n_rho, saige_region_result* out);
+/* 1 - ChiSquared::new(df).cdf(x) as the region tests evaluate it (host arithmetic). */
+double saige_chi2_sf(double df, double x);
 
 /* ------------------------------------------------------------------------------
  * Learned saddle-point root proposal (no counterpart in saige-rs).
