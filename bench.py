@@ -126,6 +126,58 @@ def run_loco(args):
     print(json.dumps(line), flush=True)
 
 
+def run_region(args):
+    """Region tests (score_test/region.rs): BURDEN, SKAT and SKAT-O for 400 genes of 25
+    variants at n = 50,000 (intercept-only null, 5% cases), dosages handed over as host f64
+    rows as the reference's region loop holds them; the oracle's restatement on one core for
+    a sample of the genes as the CPU baseline; one JSON line."""
+    import csaige_amd as sg
+    import oracle
+    from csaige_amd import region as R, synth
+    from oracle import region as OR
+    n, n_genes, per_gene = 50000, 400, 25
+    nf = synth.null_fit(n=n, p=1, eta_mean=float(np.log(0.05 / 0.95)), eta_sd=0.5, seed=7)
+    a = sg.xvx_inv_xv(nf.x, nf.mu * (1 - nf.mu))
+    nv = n_genes * per_gene
+    q = np.empty((nv, n), np.uint8)
+    synth.dosage_u8(n, 0, nv, seed=8, maf=(5e-4, 0.05), out=q)
+    G = q.astype(np.float64) / 127.0
+    del q
+    af = G.mean(axis=1) / 2
+    w = R.beta_weights(af)
+    e = sg.ScoreTestEngine(nf.mu, nf.y - nf.mu, (1.0, 0.3), nf.x, a, 1.0, y=nf.y)
+    e.test_regions(G[:per_gene], [per_gene], w[:per_gene])  # warm-up
+    t = time.perf_counter()
+    res = e.test_regions(G, [per_gene] * n_genes, w)
+    dt = time.perf_counter() - t
+    log(f"region: {n_genes} genes x {per_gene} variants at n={n}: {dt:.3f}s "
+        f"({n_genes / dt:.0f} genes/s, host f64 dosages uploaded inside the call); "
+        f"min SKAT-O p {res['pvalue'].min():.3g}")
+    k_cpu = 8
+    t = time.perf_counter()
+    cpu = [OR.run_region_test(G[g * per_gene:(g + 1) * per_gene], w[g * per_gene:(g + 1) * per_gene],
+                              nf.mu * (1 - nf.mu), nf.y - nf.mu, nf.x, a, 1.0,
+                              R.default_rho_values()) for g in range(k_cpu)]
+    dt_cpu = (time.perf_counter() - t) / k_cpu
+    dlp = max(abs(np.log10(res["pvalue"][g]) - np.log10(cpu[g][0])) for g in range(k_cpu))
+    log(f"region cpu: {dt_cpu:.3f}s per gene on 1 thread (oracle restatement); GPU vs oracle "
+        f"max |d log10 p| {dlp:.2e} over {k_cpu} genes")
+    line = {
+        "metric": "region tests genes/s, 1 GPU (BURDEN + SKAT + SKAT-O)",
+        "value": n_genes / dt, "unit": "genes/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded null fit, uint8-quantized "
+                                                     "dosages as f64)",
+        "config": {"workload": f"{n_genes} genes x {per_gene} variants, n={n}, "
+                               f"Beta(MAF;1,25) weights, 11 rho values",
+                   "includes_host_upload": True, "max_abs_dlog10p_vs_oracle": float(dlp)},
+        "cpu_baseline": {"value": 1.0 / dt_cpu, "unit": "genes/s", "cores": 1, "kind": "port",
+                         "sample": f"oracle region restatement (numpy, explicit g~) on "
+                                   f"{k_cpu} genes"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_biobank(args):
     """Several phenotypes at biobank scale: 400,000 samples (15% in half-sibling pairs),
     150,000 markers resident in HBM, 4 binary traits at 0.5/1/5/20% prevalence fitted
@@ -332,7 +384,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (counter calibration)")
     ap.add_argument("--workload", default="families",
-                    choices=["families", "biobank", "learned", "loco"],
+                    choices=["families", "biobank", "learned", "loco", "region"],
                     help="families: the default fit-null benchmark; biobank: several "
                          "phenotypes at 400,000 x 150,000; learned: the SPA root proposal; "
                          "loco: leave-one-chromosome-out refits")
@@ -366,6 +418,9 @@ def main():
         return
     if args.workload == "loco":
         run_loco(args)
+        return
+    if args.workload == "region":
+        run_region(args)
         return
 
     import csaige_amd as sg
