@@ -80,6 +80,33 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// Pinned host memory (hipHostMalloc) for the small device-to-host reads of the solver loops:
+// a copy into it is one DMA transfer, with no staging through pageable memory.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes) return SAIGE_OK;
+    release();
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return SAIGE_ERR_NOMEM;
+    }
+    bytes = want;
+    return SAIGE_OK;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~HostBuf() { release(); }
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+};
+
 int use_device(int device);
 
 // Optional per-handle timing of a handle's GEMM-class kernel launches.  Timed launches go
@@ -178,6 +205,7 @@ struct ProductSink {
 // Workspace of one batched PCG solve (one per concurrent fit).
 struct PcgWork {
   DevBuf vec, scal, part;
+  HostBuf host;  // active-column list and column states read back each iteration
 };
 
 }  // namespace saige

@@ -237,40 +237,184 @@ __global__ void decode_kernel(const uint8_t* __restrict__ bed, size_t bpm,
   out[j * ldo + k] = c == 0u ? 2.0 : (c == 2u ? 1.0 : (c == 3u ? 0.0 : __longlong_as_double(0x7ff8000000000000ll)));
 }
 
-// Repack selected markers into padded SNP-major rows of the model-ordered samples.
-// Output byte b of a row holds model samples 4b..4b+3; padding samples get code 01.
-__global__ void repack_kernel(const uint8_t* __restrict__ src, size_t src_bpm,
-                              const uint32_t* __restrict__ idx, size_t n, size_t m_chunk,
-                              uint8_t* __restrict__ dst, size_t row_bytes) {
-  size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t j = blockIdx.y;
-  if (b >= row_bytes || j >= m_chunk) return;
-  const uint8_t* row = src + j * src_bpm;
-  uint32_t out = 0;
-  for (int t = 0; t < 4; ++t) {
-    size_t k = 4 * b + t;
-    uint32_t c = (k < n) ? code_at(row, idx, k) : 1u;
-    out |= c << (2 * t);
-  }
-  dst[j * row_bytes + b] = (uint8_t)out;
+// Repacking selected markers into padded SNP-major rows of the model-ordered samples:
+// output byte b of a row holds model samples 4b..4b+3; samples >= n get code 01.  Source
+// row j is midx[j] (or j) of a buffer with src_bpm bytes per row; sample k of the model
+// order is source sample idx[k] (or k).  Every read stays inside [floor4(row start),
+// row start + src_bpm).
+
+// 16 samples of an output dword at or past sample n get code 01.
+__device__ __forceinline__ uint32_t pad_codes(uint32_t w, long long valid) {
+  if (valid >= 16) return w;
+  const uint32_t keep = valid <= 0 ? 0u : ((1u << (2 * valid)) - 1u);
+  return (w & keep) | (0x55555555u & ~keep);
 }
 
-// Same, reading the source rows from device memory through a device marker index list.
-__global__ void repack_gather_kernel(const uint8_t* __restrict__ src, size_t src_bpm,
-                                     const uint32_t* __restrict__ midx,
-                                     const uint32_t* __restrict__ idx, size_t n, size_t m,
-                                     uint8_t* __restrict__ dst, size_t row_bytes) {
-  size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t j = blockIdx.y;
-  if (b >= row_bytes || j >= m) return;
+// Identity sample order: output dword o of a row is source bytes 4o..4o+3, a funnel shift
+// of two aligned source dwords; each thread writes 16 bytes.
+__global__ __launch_bounds__(256) void repack_copy_kernel(
+    const uint8_t* __restrict__ src, size_t src_bpm, const uint32_t* __restrict__ midx,
+    size_t n, size_t m, uint8_t* __restrict__ dst, size_t row_bytes) {
+  const size_t j = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m || 16 * p >= row_bytes) return;
   const uint8_t* row = src + (size_t)(midx ? midx[j] : j) * src_bpm;
-  uint32_t out = 0;
-  for (int t = 0; t < 4; ++t) {
-    size_t k = 4 * b + t;
-    uint32_t c = (k < n) ? code_at(row, idx, k) : 1u;
-    out |= c << (2 * t);
+  const size_t off = (size_t)row & 3;
+  const uint32_t* base = reinterpret_cast<const uint32_t*>(row - off);
+  const size_t nbytes = (n + 3) / 4;
+  uint32_t out[4];
+  if (16 * p + 20 <= src_bpm) {  // base[4p .. 4p + 4] lie inside the row
+    uint32_t d[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) d[e] = base[4 * p + e];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      out[e] = off ? __builtin_amdgcn_alignbyte(d[e + 1], d[e], (uint32_t)off) : d[e];
+  } else {  // the row's last bytes, one at a time
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      uint32_t w = 0;
+      for (int b = 0; b < 4; ++b) {
+        const size_t q = 16 * p + 4 * e + b;
+        w |= (uint32_t)(q < nbytes ? row[q] : 0x55u) << (8 * b);
+      }
+      out[e] = w;
+    }
   }
-  dst[j * row_bytes + b] = (uint8_t)out;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) out[e] = pad_codes(out[e], (long long)n - (long long)(64 * p + 16 * e));
+  *reinterpret_cast<uint4*>(dst + j * row_bytes + 16 * p) = make_uint4(out[0], out[1], out[2], out[3]);
+}
+
+constexpr int kRepackRows = 8;  // source rows staged per work-group at most
+
+// A sample list: rows_per_group source rows staged in LDS (`stage` bytes apart, with
+// aligned dword loads), then every output dword gathers its 16 codes from them, reading
+// each list entry once for all the rows.
+__global__ __launch_bounds__(256) void repack_stage_kernel(
+    const uint8_t* __restrict__ src, size_t src_bpm, const uint32_t* __restrict__ midx,
+    const uint32_t* __restrict__ idx, size_t n, size_t m, uint8_t* __restrict__ dst,
+    size_t row_bytes, size_t stage, int rows_per_group) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+  const size_t j0 = (size_t)blockIdx.x * rows_per_group;
+  const int rows = (int)min((size_t)rows_per_group, m - j0);
+  uint32_t offs[kRepackRows];
+#pragma unroll
+  for (int r = 0; r < kRepackRows; ++r) {
+    offs[r] = 0;
+    if (r < rows) {
+      const uint8_t* row = src + (size_t)(midx ? midx[j0 + r] : j0 + r) * src_bpm;
+      const size_t off = (size_t)row & 3;
+      offs[r] = (uint32_t)off;
+      const uint32_t* base = reinterpret_cast<const uint32_t*>(row - off);
+      const size_t nd = (off + src_bpm + 3) / 4;  // dwords covering the row
+      const size_t nfull = (off + src_bpm) / 4;   // those that end inside it
+      uint32_t* l = lds + r * (stage / 4);
+      for (size_t d = threadIdx.x; d < nd; d += 256) {
+        uint32_t w = 0;
+        if (d < nfull) {
+          w = base[d];
+        } else {
+          for (int b = 0; b < 4; ++b) {
+            const size_t q = 4 * d + b;
+            if (q >= off && q - off < src_bpm) w |= (uint32_t)row[q - off] << (8 * b);
+          }
+        }
+        l[d] = w;
+      }
+    }
+  }
+  __syncthreads();
+  const uint8_t* lb = reinterpret_cast<const uint8_t*>(lds);
+  for (size_t o = threadIdx.x; o < row_bytes / 4; o += 256) {
+    uint32_t w[kRepackRows];
+#pragma unroll
+    for (int r = 0; r < kRepackRows; ++r) w[r] = 0;
+    uint32_t sl[16];  // the 16 list entries of this output dword
+    if (16 * o + 16 <= n) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint4 q = reinterpret_cast<const uint4*>(idx + 16 * o)[v];
+        sl[4 * v] = q.x; sl[4 * v + 1] = q.y; sl[4 * v + 2] = q.z; sl[4 * v + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 16; ++f) sl[f] = (16 * o + f < n) ? idx[16 * o + f] : 0u;
+    }
+#pragma unroll
+    for (int f = 0; f < 16; ++f) {
+      const size_t k = 16 * o + f;
+      if (k < n) {
+        const uint32_t sidx = sl[f];
+#pragma unroll
+        for (int r = 0; r < kRepackRows; ++r)
+          if (r < rows)
+            w[r] |= ((uint32_t)(lb[r * stage + offs[r] + (sidx >> 2)] >> (2 * (sidx & 3))) & 3u)
+                    << (2 * f);
+      } else {
+#pragma unroll
+        for (int r = 0; r < kRepackRows; ++r) w[r] |= 1u << (2 * f);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kRepackRows; ++r)
+      if (r < rows) *reinterpret_cast<uint32_t*>(dst + (j0 + r) * row_bytes + 4 * o) = w[r];
+  }
+}
+
+// A sample list with rows too long to stage: each output dword gathers its 16 codes from
+// global memory.
+__global__ __launch_bounds__(256) void repack_gather_kernel(
+    const uint8_t* __restrict__ src, size_t src_bpm, const uint32_t* __restrict__ midx,
+    const uint32_t* __restrict__ idx, size_t n, size_t m, uint8_t* __restrict__ dst,
+    size_t row_bytes) {
+  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t j = blockIdx.y;
+  if (j >= m || 4 * o >= row_bytes) return;
+  const uint8_t* row = src + (size_t)(midx ? midx[j] : j) * src_bpm;
+  uint32_t w = 0;
+  for (int f = 0; f < 16; ++f) {
+    const size_t k = 16 * o + f;
+    uint32_t c = 1u;
+    if (k < n) {
+      const uint32_t sidx = idx[k];
+      c = (row[sidx >> 2] >> (2 * (sidx & 3))) & 3u;
+    }
+    w |= c << (2 * f);
+  }
+  *reinterpret_cast<uint32_t*>(dst + j * row_bytes + 4 * o) = w;
+}
+
+// Launches the repack of m rows (row_bytes a multiple of 16).
+static int repack_rows(hipStream_t s, const uint8_t* src, size_t bpm, const uint32_t* d_midx,
+                       const uint32_t* d_idx, size_t n, size_t m, uint8_t* dst,
+                       size_t row_bytes) {
+  const size_t ychunk = 65535;
+  if (!d_idx) {
+    for (size_t j0 = 0; j0 < m; j0 += ychunk) {
+      const size_t mc = std::min(ychunk, m - j0);
+      repack_copy_kernel<<<dim3((unsigned)ceil_div(row_bytes / 16, 256), (unsigned)mc), 256, 0, s>>>(
+          d_midx ? src : src + j0 * bpm, bpm, d_midx ? d_midx + j0 : nullptr, n, mc,
+          dst + j0 * row_bytes, row_bytes);
+    }
+    SAIGE_HIP(hipGetLastError());
+    return SAIGE_OK;
+  }
+  const size_t stage = round_up(bpm + 8, 16);  // covers floor4 offset + row, in dwords
+  const size_t rows = std::min<size_t>(kRepackRows, 65536 / stage);
+  if (rows >= 1) {
+    repack_stage_kernel<<<(unsigned)ceil_div(m, rows), 256, rows * stage, s>>>(
+        src, bpm, d_midx, d_idx, n, m, dst, row_bytes, stage, (int)rows);
+  } else {
+    for (size_t j0 = 0; j0 < m; j0 += ychunk) {
+      const size_t mc = std::min(ychunk, m - j0);
+      repack_gather_kernel<<<dim3((unsigned)ceil_div(row_bytes / 4, 256), (unsigned)mc), 256, 0, s>>>(
+          d_midx ? src : src + j0 * bpm, bpm, d_midx ? d_midx + j0 : nullptr, d_idx, n, mc,
+          dst + j0 * row_bytes, row_bytes);
+    }
+  }
+  SAIGE_HIP(hipGetLastError());
+  return SAIGE_OK;
 }
 
 int repack_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const uint32_t* d_idx,
@@ -282,15 +426,8 @@ int repack_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const ui
     SAIGE_TRY(d_mi.ensure(m * 4));
     SAIGE_HIP(hipMemcpyAsync(d_mi.p, marker_idx, m * 4, hipMemcpyHostToDevice, s));
   }
-  const size_t chunk = 65535;
-  for (size_t j0 = 0; j0 < m; j0 += chunk) {
-    size_t mc = std::min(chunk, m - j0);
-    dim3 grid((unsigned)ceil_div(row_bytes, 256), (unsigned)mc);
-    repack_gather_kernel<<<grid, 256, 0, s>>>(
-        marker_idx ? d_bed : d_bed + j0 * bpm, bpm, marker_idx ? d_mi.as<uint32_t>() + j0 : nullptr,
-        d_idx, n, mc, d_dst + j0 * row_bytes, row_bytes);
-    SAIGE_HIP(hipGetLastError());
-  }
+  SAIGE_TRY(repack_rows(s, d_bed, bpm, marker_idx ? d_mi.as<uint32_t>() : nullptr, d_idx, n, m,
+                        d_dst, row_bytes));
   SAIGE_HIP(hipStreamSynchronize(s));
   return SAIGE_OK;
 }
@@ -349,10 +486,8 @@ int upload_repack(hipStream_t s, const uint8_t* bed_body, size_t bpm, size_t n_t
       srcp = host.data();
     }
     SAIGE_HIP(hipMemcpyAsync(staging.p, srcp, mc * bpm, hipMemcpyHostToDevice, s));
-    dim3 grid((unsigned)ceil_div(row_bytes, 256), (unsigned)mc);
-    repack_kernel<<<grid, 256, 0, s>>>(staging.as<uint8_t>(), bpm, d_idx, n, mc,
-                                       d_dst + j0 * row_bytes, row_bytes);
-    SAIGE_HIP(hipGetLastError());
+    SAIGE_TRY(repack_rows(s, staging.as<uint8_t>(), bpm, nullptr, d_idx, n, mc,
+                          d_dst + j0 * row_bytes, row_bytes));
     SAIGE_HIP(hipStreamSynchronize(s));
   }
   return SAIGE_OK;

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace saige {
 
@@ -306,6 +307,25 @@ __global__ void finalize_sums_kernel(const double* __restrict__ part, size_t nch
   res[q] = s;
 }
 
+// Pinned staging for small results read back on the calling thread (grown on demand and
+// kept for the thread's lifetime): the copy is one DMA transfer into host memory.
+static void* pinned_staging(size_t bytes) {
+  static thread_local void* p = nullptr;
+  static thread_local size_t have = 0;
+  if (bytes > have) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    have = 0;
+    const size_t want = std::max<size_t>(bytes, 64 * 1024);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return nullptr;
+    }
+    have = want;
+  }
+  return p;
+}
+
 static int dot_pairs(hipStream_t s, const double* A, size_t lda, size_t na, const double* B,
                      size_t ldb, size_t nq, int diag, size_t n, double* h_out, DevBuf& part,
                      DevBuf& res) {
@@ -318,8 +338,14 @@ static int dot_pairs(hipStream_t s, const double* A, size_t lda, size_t na, cons
   finalize_sums_kernel<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(part.as<double>(), nchunks,
                                                                      nq, res.as<double>());
   SAIGE_HIP(hipGetLastError());
-  SAIGE_HIP(hipMemcpyAsync(h_out, res.p, nq * 8, hipMemcpyDeviceToHost, s));
+  void* pin = pinned_staging(nq * 8);
+  if (!pin) {
+    set_error("hipHostMalloc failed");
+    return SAIGE_ERR_NOMEM;
+  }
+  SAIGE_HIP(hipMemcpyAsync(pin, res.p, nq * 8, hipMemcpyDeviceToHost, s));
   SAIGE_HIP(hipStreamSynchronize(s));
+  memcpy(h_out, pin, nq * 8);
   return SAIGE_OK;
 }
 

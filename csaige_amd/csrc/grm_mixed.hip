@@ -43,41 +43,114 @@ constexpr int kMixRows = 4 * 16 * kMixRT;      // rows per work-group (4 waves)
 // ---------------------------------------------------------------------------------
 // Prepare: sample-major copy, per-sample row sums, missing check.
 // ---------------------------------------------------------------------------------
+// A 4 x 4 matrix of 2-bit fields (byte r = row, field c = column) transposed in place: the
+// fields of the 2 x 2 sub-blocks swap first, then the two off-diagonal sub-blocks.
+__device__ __forceinline__ uint32_t transpose_2bit_4x4(uint32_t w) {
+  uint32_t t = (w ^ (w >> 6)) & 0x00CC00CCu;
+  w ^= t ^ (t << 6);
+  t = (w ^ (w >> 12)) & 0x0000F0F0u;
+  w ^= t ^ (t << 12);
+  return w;
+}
+
+// o[e] = byte e of d0 | byte e of d1 << 8 | byte e of d2 << 16 | byte e of d3 << 24
+__device__ __forceinline__ void transpose_bytes_4x4(uint32_t d0, uint32_t d1, uint32_t d2,
+                                                    uint32_t d3, uint32_t* o) {
+  const uint32_t a = __builtin_amdgcn_perm(d1, d0, 0x05010400u);
+  const uint32_t b = __builtin_amdgcn_perm(d1, d0, 0x07030602u);
+  const uint32_t c = __builtin_amdgcn_perm(d3, d2, 0x05010400u);
+  const uint32_t d = __builtin_amdgcn_perm(d3, d2, 0x07030602u);
+  o[0] = __builtin_amdgcn_perm(c, a, 0x05040100u);
+  o[1] = __builtin_amdgcn_perm(c, a, 0x07060302u);
+  o[2] = __builtin_amdgcn_perm(d, b, 0x05040100u);
+  o[3] = __builtin_amdgcn_perm(d, b, 0x07060302u);
+}
+
+constexpr int kTrStride = 68;  // bytes per staged marker row (64 used)
+
 // geno: m_pad rows of rb bytes (n_pad/4); geno_t: n_pad rows of rbt bytes (m_pad/4).
-// One work-group transposes a 64-marker x 256-sample tile through LDS: 64 coalesced row
-// segments of 64 bytes in, 256 sample rows of 16 bytes out.  The same pass accumulates,
-// per sample, the dosage sum over real markers and the count of missing calls (integer
-// atomics, so the totals do not depend on the order).
+// A work-group covers 256 samples and loops over `tiles_per_group` tiles of 256 markers:
+// the tile's 64-byte row segments go to LDS, then each thread turns 16 markers x 4 samples
+// into 4 output words (four 4 x 4 bit transposes and one byte transpose) and stores them,
+// a wave writing 64 contiguous bytes of 4 sample rows per store.  The dosage sum of each
+// sample over real markers (code 00 -> 2, 10 -> 1: 2 #{lo = 0} - #{lo = 0, hi = 1}) and the
+// number of missing calls (code 01) come from popcounts of the output words; each sample's
+// sum reaches global memory once per work-group (integer atomics, order-free).
 __global__ __launch_bounds__(256) void transpose_codes_kernel(
     const uint8_t* __restrict__ geno, size_t rb, uint8_t* __restrict__ geno_t, size_t rbt,
-    size_t n, size_t m, unsigned* __restrict__ rsum, unsigned* __restrict__ n_miss) {
-  __shared__ __attribute__((aligned(16))) uint8_t tile[64][64 + 4];
+    size_t n, size_t m, size_t m_pad, size_t tiles_per_group, unsigned* __restrict__ rsum,
+    unsigned* __restrict__ n_miss) {
+  __shared__ __attribute__((aligned(16))) uint8_t tile[256 * kTrStride];
+  __shared__ unsigned red[256];
   const int t = threadIdx.x;
-  const size_t m0 = (size_t)blockIdx.x * 64, s0 = (size_t)blockIdx.y * 256;
-  {
-    const int row = t >> 2, seg = t & 3;
-    const uint4 v = *reinterpret_cast<const uint4*>(geno + (m0 + row) * rb + (s0 >> 2) + 16 * seg);
-    uint32_t* d = reinterpret_cast<uint32_t*>(&tile[row][16 * seg]);
-    d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+  const size_t s0 = (size_t)blockIdx.y * 256;
+  const int mb = t & 15, sbl = t >> 4;  // markers 16 mb..16 mb + 15, sample bytes sbl + 16 j
+  unsigned sum[16], miss = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) sum[i] = 0;
+  const size_t tile0 = (size_t)blockIdx.x * tiles_per_group;
+  const size_t tile1 = min(tile0 + tiles_per_group, m_pad / 256);
+  for (size_t tl = tile0; tl < tile1; ++tl) {
+    const size_t m0 = tl * 256;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {  // row (t >> 2) + 64 u, 16 bytes at 16 (t & 3)
+      const int row = (t >> 2) + 64 * u, seg = t & 3;
+      const uint4 v =
+          *reinterpret_cast<const uint4*>(geno + (m0 + row) * rb + (s0 >> 2) + 16 * seg);
+      uint32_t* d = reinterpret_cast<uint32_t*>(tile + row * kTrStride + 16 * seg);
+      d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+    __syncthreads();
+    const long long real = (long long)m - (long long)(m0 + 16 * mb);
+    const uint32_t rmask =
+        real >= 16 ? 0xFFFFFFFFu : (real <= 0 ? 0u : ((1u << (2 * real)) - 1u));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int sb = sbl + 16 * j;
+      uint32_t q[4];  // markers 4 i + r (byte r) of 4 samples (fields) -> byte = sample
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          w |= (uint32_t)tile[(16 * mb + 4 * i + r) * kTrStride + sb] << (8 * r);
+        q[i] = transpose_2bit_4x4(w);
+      }
+      uint32_t o[4];  // sample 4 sb + u: its 16 markers in order
+      transpose_bytes_4x4(q[0], q[1], q[2], q[3], o);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const size_t i = s0 + 4 * sb + u;
+        *reinterpret_cast<uint32_t*>(geno_t + i * rbt + (m0 >> 2) + 4 * mb) = o[u];
+        const uint32_t lo = o[u] & 0x55555555u & rmask, hi = (o[u] >> 1) & 0x55555555u & rmask;
+        const uint32_t lo0 = ~o[u] & 0x55555555u & rmask;
+        sum[4 * j + u] += 2u * __popc(lo0) - __popc(lo0 & hi);
+        if (i < n) miss += __popc(lo & ~hi);
+      }
+    }
+    __syncthreads();  // the tile is refilled next
   }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      red[t] = sum[4 * j + u];
+      __syncthreads();
+      if (mb == 0) {
+        unsigned acc = 0;
+        for (int b = 0; b < 16; ++b) acc += red[t + b];
+        const size_t i = s0 + 4 * (sbl + 16 * j) + u;
+        if (i < n && acc) atomicAdd(&rsum[i], acc);
+      }
+      __syncthreads();
+    }
+  red[t] = miss;
   __syncthreads();
-  const size_t i = s0 + t;
-  const int byte = t >> 2, sh = 2 * (t & 3);
-  uint32_t out[4] = {0u, 0u, 0u, 0u};
-  uint32_t sum = 0, miss = 0;
-#pragma unroll 8
-  for (int j = 0; j < 64; ++j) {
-    const uint32_t c = (tile[j][byte] >> sh) & 3u;
-    out[j >> 4] |= c << (2 * (j & 15));
-    const bool real = (m0 + j) < m;
-    sum += real ? ((c == 0u) ? 2u : (c == 2u ? 1u : 0u)) : 0u;
-    miss += (real && c == 1u) ? 1u : 0u;
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
   }
-  *reinterpret_cast<uint4*>(geno_t + i * rbt + (m0 >> 2)) = make_uint4(out[0], out[1], out[2], out[3]);
-  if (i < n) {
-    if (sum) atomicAdd(&rsum[i], sum);
-    if (miss) atomicAdd(n_miss, miss);
-  }
+  if (t == 0 && red[0]) atomicAdd(n_miss, red[0]);
 }
 
 __global__ void u32_to_f64_kernel(const unsigned* __restrict__ a, size_t n, double* __restrict__ out) {
@@ -345,9 +418,14 @@ int grm_mixed_prepare(saige_grm* h) {
   SAIGE_TRY(ms->nmiss.ensure(64 + g.n_pad * 4));
   unsigned* d_rsum_u = reinterpret_cast<unsigned*>(ms->nmiss.as<char>() + 64);
   SAIGE_HIP(hipMemsetAsync(ms->nmiss.p, 0, 64 + g.n_pad * 4, s));
-  transpose_codes_kernel<<<dim3((unsigned)(g.m_pad / 64), (unsigned)(g.n_pad / 256)), 256, 0, s>>>(
-      g.geno, g.row_bytes, ms->geno_t.as<uint8_t>(), ms->rbt, g.n, g.m, d_rsum_u,
-      ms->nmiss.as<unsigned>());
+  {
+    // about 2,048 work-groups: sample blocks x marker groups
+    const size_t tiles = g.m_pad / 256, sblocks = g.n_pad / 256;
+    const size_t tpg = ceil_div(tiles, std::max<size_t>(1, std::min(tiles, ceil_div(2048, sblocks))));
+    transpose_codes_kernel<<<dim3((unsigned)ceil_div(tiles, tpg), (unsigned)sblocks), 256, 0, s>>>(
+        g.geno, g.row_bytes, ms->geno_t.as<uint8_t>(), ms->rbt, g.n, g.m, g.m_pad, tpg, d_rsum_u,
+        ms->nmiss.as<unsigned>());
+  }
   u32_to_f64_kernel<<<(unsigned)ceil_div(g.n_pad, 256), 256, 0, s>>>(d_rsum_u, g.n_pad,
                                                                      ms->rsum.as<double>());
   SAIGE_HIP(hipGetLastError());
@@ -740,15 +818,28 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
         part[(split * rows_pad + rowb + 16 * g + 4 * q + j) * KP + 16 * t + r] = acc[g][t][j];
 }
 
-// Sum the integer partials of 64 rows over the splits into an LDS tile (reads follow the
-// partial layout); the caller's epilogue then writes column-major outputs.
+// Sum the integer partials of 64 rows over the splits into an LDS tile (16-byte loads along
+// the partial layout, four splits in flight per step); the caller's epilogue then writes
+// column-major outputs.
 template <int KP>
 __device__ __forceinline__ void i8_tile_sum(const int* __restrict__ part, size_t splits,
                                             size_t rows_pad, size_t row0, int (*tile)[KP + 1]) {
-  for (int e = threadIdx.x; e < 64 * KP; e += 256) {
-    int acc = 0;
-    for (size_t s = 0; s < splits; ++s) acc += part[(s * rows_pad + row0) * KP + e];
-    tile[e / KP][e % KP] = acc;
+  const i32x4* base = reinterpret_cast<const i32x4*>(part + row0 * KP);
+  const size_t stride = rows_pad * KP / 4;  // one split, in 16-byte groups
+  for (int e4 = threadIdx.x; e4 < 16 * KP; e4 += 256) {
+    i32x4 acc = {0, 0, 0, 0};
+    size_t sp = 0;
+    for (; sp + 4 <= splits; sp += 4) {
+      const i32x4 a = base[sp * stride + e4], b = base[(sp + 1) * stride + e4];
+      const i32x4 c = base[(sp + 2) * stride + e4], d = base[(sp + 3) * stride + e4];
+      acc += (a + b) + (c + d);
+    }
+    for (; sp < splits; ++sp) acc += base[sp * stride + e4];
+    const int e = 4 * e4;  // KP is a multiple of 16: the four values share a row
+    tile[e / KP][e % KP] = acc.x;
+    tile[e / KP][e % KP + 1] = acc.y;
+    tile[e / KP][e % KP + 2] = acc.z;
+    tile[e / KP][e % KP + 3] = acc.w;
   }
   __syncthreads();
 }

@@ -262,16 +262,19 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   pcg_init_fin<<<kb, 256, 0, s>>>(part, nchunks, k, tol, st);
   SAIGE_HIP(hipGetLastError());
   int n_active = 0;
-  std::vector<int> list(k + 1);
+  SAIGE_TRY(ws.host.ensure(list_bytes + k * sizeof(ColState)));
+  int* list = ws.host.as<int>();  // pinned: [count, active column indices]
+  ColState* hs = reinterpret_cast<ColState*>(ws.host.as<char>() + list_bytes);
   {
-    std::vector<ColState> hs(k);
-    SAIGE_HIP(hipMemcpyAsync(hs.data(), st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipMemcpyAsync(hs, st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
     for (size_t c = 0; c < k; ++c)
       if (hs[c].active) list[1 + n_active++] = (int)c;
     list[0] = n_active;
-    if (n_active > 0 && (size_t)n_active < k)
-      SAIGE_HIP(hipMemcpyAsync(d_nact, list.data(), (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    if (n_active > 0 && (size_t)n_active < k) {
+      SAIGE_HIP(hipMemcpyAsync(d_nact, list, (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+      SAIGE_HIP(hipStreamSynchronize(s));  // the pinned list is rewritten below
+    }
   }
   for (uint32_t iter = 0; iter < max_iter && n_active > 0; ++iter) {
     if ((size_t)n_active == k) {
@@ -290,15 +293,14 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
     pcg_beta_fin<<<1, 1024, 0, s>>>(part, nchunks, k, (int)iter, st, d_nact);
     pcg_p_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(P, Z, ld, n, st);
     SAIGE_HIP(hipGetLastError());
-    SAIGE_HIP(hipMemcpyAsync(list.data(), d_nact, (k + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipMemcpyAsync(list, d_nact, sizeof(int), hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
     n_active = list[0];
   }
   pcg_final<<<kb, 256, 0, s>>>(k, max_iter, st);
   SAIGE_HIP(hipGetLastError());
   if (info) {
-    std::vector<ColState> hs(k);
-    SAIGE_HIP(hipMemcpyAsync(hs.data(), st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipMemcpyAsync(hs, st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
     for (size_t c = 0; c < k; ++c) {
       info[c].iterations = (uint32_t)hs[c].iters;
