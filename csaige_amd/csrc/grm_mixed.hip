@@ -153,6 +153,35 @@ __global__ __launch_bounds__(256) void transpose_codes_kernel(
   if (t == 0 && red[0]) atomicAdd(n_miss, red[0]);
 }
 
+// The int8 GEMM reads the code matrices in tiles of 16 rows x 128 bytes (512 reduction
+// indices), each tile 2 KB contiguous: tile (row / 16, byte / 128) starts at
+// ((row / 16) * (row_bytes / 128) + byte / 128) * 2048, row r % 16 at 128 (r % 16) in it.  A
+// wave's load of 16 rows then touches one 2 KB block instead of 16 rows 12.5 KB apart.
+__global__ void retile_codes_kernel(const uint8_t* __restrict__ src, size_t row_bytes,
+                                    size_t rows, uint8_t* __restrict__ dst) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // 16-byte unit
+  const size_t per_row = row_bytes / 16;
+  if (v >= rows * per_row) return;
+  const size_t row = v / per_row, b = 16 * (v % per_row);
+  const uint4 x = *reinterpret_cast<const uint4*>(src + row * row_bytes + b);
+  *reinterpret_cast<uint4*>(dst + ((row / 16) * (row_bytes / 128) + b / 128) * 2048 +
+                            (row % 16) * 128 + b % 128) = x;
+}
+
+// buf (rows x row_bytes, row-major) replaced by its tiled form
+static int retile_codes(hipStream_t s, DevBuf& buf, size_t rows, size_t row_bytes) {
+  DevBuf tmp;
+  SAIGE_TRY(tmp.ensure(rows * row_bytes));
+  retile_codes_kernel<<<(unsigned)ceil_div(rows * (row_bytes / 16), 256), 256, 0, s>>>(
+      buf.as<uint8_t>(), row_bytes, rows, tmp.as<uint8_t>());
+  SAIGE_HIP(hipGetLastError());
+  SAIGE_HIP(hipStreamSynchronize(s));  // the row-major buffer goes back to the cache
+  std::swap(buf.p, tmp.p);
+  std::swap(buf.bytes, tmp.bytes);
+  std::swap(buf.device, tmp.device);
+  return SAIGE_OK;
+}
+
 __global__ void u32_to_f64_kernel(const unsigned* __restrict__ a, size_t n, double* __restrict__ out) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (double)a[i];
@@ -464,6 +493,11 @@ int grm_mixed_prepare(saige_grm* h) {
   SAIGE_HIP(hipMemcpy(ms->sc.p, sc.data(), g.m_pad * 8, hipMemcpyHostToDevice));
   SAIGE_HIP(hipMemcpy(ms->p2.p, p2.data(), g.m_pad * 8, hipMemcpyHostToDevice));
   SAIGE_HIP(hipMemcpy(ms->zs.p, zs.data(), g.m_pad * 8, hipMemcpyHostToDevice));
+  if (h->precision == SAIGE_PREC_INT8) {  // nothing reads the row-major forms from here on
+    SAIGE_TRY(retile_codes(s, h->geno, g.m_pad, g.row_bytes));
+    g.geno = h->geno.as<uint8_t>();
+    SAIGE_TRY(retile_codes(s, ms->geno_t, g.n_pad, ms->rbt));
+  }
   return SAIGE_OK;
 }
 
@@ -609,46 +643,61 @@ constexpr size_t kI8MaxRed = 8000000;
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-// Column scalars rebuilt from fixed-order partial sums (one value per kDotChunk rows, added
-// in chunk order): the mean of column c, and the three slice scales from the partial maxima
-// of |x - mean| (s0 = max / 127, s1 = s0 / 254, s2 = s1 / 254; 1 for an all-zero column).
+// Column scalars rebuilt from fixed-order partials (one value per kDotChunk rows, combined in
+// chunk order): the mean of column c, and the three slice scales from the largest
+// |x - mean|, which is max(xmax - mean, mean - xmin) over the chunks' maxima and minima
+// (rounding is monotone and odd, so this is the largest rounded |x_i - mean| itself);
+// s0 = max / 127, s1 = s0 / 254, s2 = s1 / 254, and 1 for an all-zero column.
 __device__ __forceinline__ double chunk_sum(const double* __restrict__ part, size_t c,
                                             size_t nchunks) {
   double s = 0.0;
   for (size_t q = 0; q < nchunks; ++q) s += part[c * nchunks + q];
   return s;
 }
-__device__ __forceinline__ void slice_scales(const double* __restrict__ pmax, size_t c,
-                                             size_t nchunks, double* sc3) {
+__device__ __forceinline__ void slice_scales(const double* __restrict__ pmax,
+                                             const double* __restrict__ pmin, double mean,
+                                             size_t c, size_t nchunks, double* sc3) {
   double m = 0.0;
-  for (size_t q = 0; q < nchunks; ++q) m = fmax(m, pmax[c * nchunks + q]);
+  for (size_t q = 0; q < nchunks; ++q)
+    m = fmax(m, fmax(pmax[c * nchunks + q] - mean, mean - pmin[c * nchunks + q]));
   const double s0 = m > 0.0 ? m / 127.0 : 1.0;
   sc3[0] = s0;
   sc3[1] = s0 / 254.0;
   sc3[2] = sc3[1] / 254.0;
 }
 
-// per-column partial max of |x - mean| over chunks of kDotChunk rows, the mean taken from
-// the column's partial sums (psum) times inv_len
-__global__ __launch_bounds__(kDotBlock) void col_absmax_partial_kernel(
-    const double* __restrict__ x, size_t ld, size_t len, const double* __restrict__ psum,
-    double inv_len, size_t nchunks, double* __restrict__ part) {
+// Per column and chunk of kDotChunk rows: the sum (the same order as col_partial_kernel),
+// the maximum and the minimum.
+__global__ __launch_bounds__(kDotBlock) void col_stats_kernel(
+    const double* __restrict__ x, size_t ld, size_t len, size_t nchunks,
+    double* __restrict__ psum, double* __restrict__ pmax, double* __restrict__ pmin) {
   const size_t c = blockIdx.y;
-  __shared__ double mu_s;
-  if (threadIdx.x == 0) mu_s = chunk_sum(psum, c, nchunks) * inv_len;
-  __syncthreads();
-  const double mu = mu_s;
   const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, len);
-  double m = 0.0;
-  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) m = fmax(m, fabs(x[i + c * ld] - mu));
-  __shared__ double sh[kDotBlock];
-  sh[threadIdx.x] = m;
+  double s = 0.0, hi = -INFINITY, lo = INFINITY;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) {
+    const double v = x[i + c * ld];
+    s += v;
+    hi = fmax(hi, v);
+    lo = fmin(lo, v);
+  }
+  __shared__ double sh[3][kDotBlock];
+  sh[0][threadIdx.x] = s;
+  sh[1][threadIdx.x] = hi;
+  sh[2][threadIdx.x] = lo;
   __syncthreads();
   for (int w = kDotBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + w]);
+      sh[2][threadIdx.x] = fmin(sh[2][threadIdx.x], sh[2][threadIdx.x + w]);
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[c * nchunks + blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) {
+    psum[c * nchunks + blockIdx.x] = sh[0][0];
+    pmax[c * nchunks + blockIdx.x] = sh[1][0];
+    pmin[c * nchunks + blockIdx.x] = sh[2][0];
+  }
 }
 
 __device__ __forceinline__ size_t perm16(size_t i) {
@@ -662,8 +711,9 @@ __device__ __forceinline__ double q127(double t) { return fmin(fmax(rint(t), -12
 // grid.y = k + (KP - 3k): the last rows of the grid clear the padding columns.
 __global__ __launch_bounds__(kDotBlock) void quantize3_kernel(
     const double* __restrict__ x, size_t ldx, size_t len, size_t k,
-    const double* __restrict__ psum, const double* __restrict__ pmax, double inv_len,
-    size_t nchunks, size_t pad_len, int8_t* __restrict__ q, double* __restrict__ xs, size_t ldxs,
+    const double* __restrict__ psum, const double* __restrict__ pmax,
+    const double* __restrict__ pmin, double inv_len, size_t nchunks, size_t pad_len,
+    int8_t* __restrict__ q, double* __restrict__ xs, size_t ldxs,
     const double* __restrict__ wt, double* __restrict__ pdot) {
   const size_t c = blockIdx.y;
   const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, pad_len);
@@ -674,7 +724,7 @@ __global__ __launch_bounds__(kDotBlock) void quantize3_kernel(
   __shared__ double cs_s[4];
   if (threadIdx.x == 0) {
     cs_s[0] = chunk_sum(psum, c, nchunks) * inv_len;
-    slice_scales(pmax, c, nchunks, cs_s + 1);
+    slice_scales(pmax, pmin, cs_s[0], c, nchunks, cs_s + 1);
   }
   __syncthreads();
   const double mu = cs_s[0], s0 = cs_s[1], s1 = cs_s[2], s2 = cs_s[3];
@@ -709,9 +759,10 @@ __global__ __launch_bounds__(kDotBlock) void quantize3_kernel(
 
 // part[split][row][16*KT] = sum over the split's reduction range of g(row, red) * q[col][red].
 // NW waves; each wave owns 16*RT rows and reads their packed codes straight from global
-// memory (KCH/16 bytes = KCH/4 genotypes per lane and chunk of KCH reduction indices, the
-// next chunk prefetched in registers; lane q takes the q-th quarter of a row's chunk, so a
-// row's chunk is one 64-byte or 128-byte segment).  The KP x KCH operand chunk is staged in
+// memory in the tiled layout of retile_codes_kernel (code_stride = bytes per row; KCH/16
+// bytes = KCH/4 genotypes per lane and chunk of KCH reduction indices, the next chunk
+// prefetched in registers; lane q takes the q-th quarter of a row's chunk, so one load of
+// a 16-row group stays inside one 2 KB tile).  The KP x KCH operand chunk is staged in
 // LDS, double-buffered so that one
 // barrier per chunk separates the reads of one buffer from the refill of the other (column
 // stride 272 bytes: the 16-byte fragment reads of 8 consecutive columns fall in distinct
@@ -739,7 +790,9 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
   const size_t k1 = min(k0 + red_per_split, red_len);
   if (k0 >= k1) return;  // uniform over the work-group
   const int bcol0 = tid / SEGS, bseg = tid % SEGS;
-  const uint8_t* abase = codes + (rowb + r) * code_stride + (KCH / 16) * q;
+  static_assert(KCH <= 512, "a row's chunk lies inside one 128-byte tile segment");
+  const size_t ntile = code_stride / 128;  // tiles per 16-row group
+  const uint8_t* abase = codes + (rowb / 16) * ntile * 2048 + r * 128 + (KCH / 16) * q;
   const int8_t* bbase = bq + (size_t)bcol0 * b_stride + 16 * bseg;
   i32x4 acc[RT][KT];
 #pragma unroll
@@ -751,7 +804,8 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
   for (int g = 0; g < RT; ++g)
 #pragma unroll
     for (int a = 0; a < AV; ++a)
-      pa[g][a] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + k0 / 4 + 16 * a);
+      pa[g][a] = *reinterpret_cast<const u32x4*>(abase + (g * ntile + k0 / 512) * 2048 +
+                                                 (k0 / 4) % 128 + 16 * a);
 #pragma unroll
   for (int j = 0; j < B_LOADS; ++j)
     if (bcol0 + BCSTEP * j < KP)
@@ -775,7 +829,8 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
       for (int g = 0; g < RT; ++g)
 #pragma unroll
         for (int a = 0; a < AV; ++a)
-          pa[g][a] = *reinterpret_cast<const u32x4*>(abase + 16 * g * code_stride + kn / 4 + 16 * a);
+          pa[g][a] = *reinterpret_cast<const u32x4*>(abase + (g * ntile + kn / 512) * 2048 +
+                                                     (kn / 4) % 128 + 16 * a);
 #pragma unroll
       for (int j = 0; j < B_LOADS; ++j)
         if (bcol0 + BCSTEP * j < KP)
@@ -846,19 +901,20 @@ __device__ __forceinline__ void i8_tile_sum(const int* __restrict__ part, size_t
 
 // Stage-1 epilogue: U'_mc = sum_j s_jc I_jmc; T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c;
 // W_mc = s_m T_mc (column-major, ldw).  mean_c, the slice scales and cs_c are rebuilt from
-// the partials of the operand pass (column sums psum, maxima pmax, sliced sums pcs).
+// the partials of the operand pass (column sums psum, maxima pmax, minima pmin, sliced sums
+// pcs).
 template <int KP>
 __global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
     const int* __restrict__ part, size_t splits, size_t m_pad, size_t m, size_t k,
     const double* __restrict__ psum, const double* __restrict__ pmax,
-    const double* __restrict__ pcs, double inv_n, size_t nch, const double* __restrict__ sc,
+    const double* __restrict__ pmin, const double* __restrict__ pcs, double inv_n, size_t nch, const double* __restrict__ sc,
     const double* __restrict__ p2, const double* __restrict__ zs, double* __restrict__ w,
     size_t ldw) {
   __shared__ int tile[64][KP + 1];
   __shared__ double col[KP / 3 + 1][5];  // mean, s0, s1, s2, cs
   for (size_t c = threadIdx.x; c < k; c += 256) {
     col[c][0] = chunk_sum(psum, c, nch) * inv_n;
-    slice_scales(pmax, c, nch, &col[c][1]);
+    slice_scales(pmax, pmin, col[c][0], c, nch, &col[c][1]);
     col[c][4] = chunk_sum(pcs, c, nch);
   }
   const size_t row0 = (size_t)blockIdx.x * 64;
@@ -881,14 +937,14 @@ template <int KP>
 __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
     const int* __restrict__ part, size_t splits, size_t n_pad, size_t n, size_t k,
     const double* __restrict__ psum, const double* __restrict__ pmax,
-    const double* __restrict__ ps2p, double inv_m, size_t nch, const double* __restrict__ rsum,
+    const double* __restrict__ pmin, const double* __restrict__ ps2p, double inv_m, size_t nch, const double* __restrict__ rsum,
     double P2, double* __restrict__ out, size_t ldo, const double* __restrict__ v, size_t ldv,
     const double* __restrict__ wts, double tau0, double tau1) {
   __shared__ int tile[64][KP + 1];
   __shared__ double col[KP / 3 + 1][5];  // nu, s0, s1, s2, S2p
   for (size_t c = threadIdx.x; c < k; c += 256) {
     col[c][0] = chunk_sum(psum, c, nch) * inv_m;
-    slice_scales(pmax, c, nch, &col[c][1]);
+    slice_scales(pmax, pmin, col[c][0], c, nch, &col[c][1]);
     col[c][4] = chunk_sum(ps2p, c, nch);
   }
   const size_t row0 = (size_t)blockIdx.x * 64;
@@ -979,41 +1035,39 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
   SAIGE_TRY(ms->vs.ensure(n * k * 8));
   SAIGE_TRY(ms->ws_.ensure(m * k * 8));
   SAIGE_TRY(ms->w.ensure(m * k * 8));
-  SAIGE_TRY(ms->red.ensure(3 * std::max(nch_n, nch_m) * k * 8));
+  SAIGE_TRY(ms->red.ensure(4 * std::max(nch_n, nch_m) * k * 8));
 
-  // Ten launches per product.  Column scalars (means, slice scales, sums of the sliced
+  // Eight launches per product.  Column scalars (means, slice scales, sums of the sliced
   // values) stay as fixed-order partials per kDotChunk rows and are finished by the kernel
   // that needs them.
   double* psum = ms->red.as<double>();                // column sums of V, then of W
   double* pmax = psum + std::max(nch_n, nch_m) * k;   // partial maxima
-  double* pdot = pmax + std::max(nch_n, nch_m) * k;   // sums of the sliced values
+  double* pmin = pmax + std::max(nch_n, nch_m) * k;   // partial minima
+  double* pdot = pmin + std::max(nch_n, nch_m) * k;   // sums of the sliced values
   const dim3 gq_n((unsigned)ceil_div(n_pad, kDotChunk), (unsigned)k + pad_cols);
   const dim3 gq_m((unsigned)ceil_div(m_pad, kDotChunk), (unsigned)k + pad_cols);
 
-  // ---- V: column sums, maxima of |V - mean|, slices + sums of the sliced values ----
-  col_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n, psum);
-  col_absmax_partial_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(
-      d_v, ldv, n, psum, 1.0 / (double)n, nch_n, pmax);
-  quantize3_kernel<<<gq_n, kDotBlock, 0, s>>>(d_v, ldv, n, k, psum, pmax, 1.0 / (double)n, nch_n,
-                                              n_pad, ms->vq.as<int8_t>(), ms->vs.as<double>(), n,
-                                              nullptr, pdot);
+  // ---- V: column sums, maxima and minima, slices + sums of the sliced values ----
+  col_stats_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n,
+                                                                             psum, pmax, pmin);
+  quantize3_kernel<<<gq_n, kDotBlock, 0, s>>>(d_v, ldv, n, k, psum, pmax, pmin, 1.0 / (double)n,
+                                              nch_n, n_pad, ms->vq.as<int8_t>(),
+                                              ms->vs.as<double>(), n, nullptr, pdot);
 
   // ---- stage 1: I = G^T Q_v (rows = markers, reduction over samples) ----
   size_t splits1 = 0;
   SAIGE_TRY((i8_gemm<RT, KT, NW, KCH>(h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
                                   n_pad, ms->ipart1, &splits1)));
   i8_stage1_epilogue_kernel<KP><<<(unsigned)(m_pad / 64), 256, 0, s>>>(
-      ms->ipart1.as<int>(), splits1, m_pad, m, k, psum, pmax, pdot, 1.0 / (double)n, nch_n,
+      ms->ipart1.as<int>(), splits1, m_pad, m, k, psum, pmax, pmin, pdot, 1.0 / (double)n, nch_n,
       ms->sc.as<double>(), ms->p2.as<double>(), ms->zs.as<double>(), ms->w.as<double>(), m);
   LocoCodes lc;
   if (loco_codes(h, c0, k, lc)) loco_mask(h, ms->w.as<double>(), 1, m, m, k, lc, s);
 
-  // ---- W: column sums over real markers, maxima, slices + S2p partials ----
-  col_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(ms->w.as<double>(), m,
-                                                                               m, nch_m, psum);
-  col_absmax_partial_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
-      ms->w.as<double>(), m, m, psum, 1.0 / (double)m, nch_m, pmax);
-  quantize3_kernel<<<gq_m, kDotBlock, 0, s>>>(ms->w.as<double>(), m, m, k, psum, pmax,
+  // ---- W: column sums over real markers, maxima and minima, slices + S2p partials ----
+  col_stats_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
+      ms->w.as<double>(), m, m, nch_m, psum, pmax, pmin);
+  quantize3_kernel<<<gq_m, kDotBlock, 0, s>>>(ms->w.as<double>(), m, m, k, psum, pmax, pmin,
                                               1.0 / (double)m, nch_m, m_pad, ms->wq.as<int8_t>(),
                                               ms->ws_.as<double>(), m, ms->p2.as<double>(), pdot);
 
@@ -1022,7 +1076,7 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
   SAIGE_TRY((i8_gemm<RT, KT, NW, KCH>(h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
                                   m_pad, n_pad, m_pad, ms->ipart2, &splits2)));
   i8_stage2_epilogue_kernel<KP><<<(unsigned)(n_pad / 64), 256, 0, s>>>(
-      ms->ipart2.as<int>(), splits2, n_pad, n, k, psum, pmax, pdot, 1.0 / (double)m, nch_m,
+      ms->ipart2.as<int>(), splits2, n_pad, n, k, psum, pmax, pmin, pdot, 1.0 / (double)m, nch_m,
       ms->rsum.as<double>(), ms->P2, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0,
       tau ? tau[1] : 0.0);
   SAIGE_HIP(hipGetLastError());
