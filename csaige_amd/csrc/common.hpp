@@ -217,6 +217,8 @@ struct saige_grm {
   std::recursive_mutex mu;
   saige::GrmDevice g;
   saige::DevBuf geno, lut, scale, shift, scale64, shift64, geno_t, miss_idx, miss_ptr;
+  // reduced precision: per-marker 1/sd and 2p/sd (m_pad entries, 0 past m) on the host
+  std::vector<double> host_scale, host_shift;
   // workspaces for products and solves
   saige::DevBuf ws_vt, ws_tpart, ws_t, ws_ypart, ws_in, ws_out;
   saige::DevBuf ws_vhi, ws_vlo, ws_tmp;

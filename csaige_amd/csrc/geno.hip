@@ -432,6 +432,26 @@ int repack_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const ui
   return SAIGE_OK;
 }
 
+// Allele counts of m markers of device-resident packed rows, into device memory.
+int marker_ac_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, size_t n, size_t m,
+                          double* d_ac) {
+  if (m == 0) return SAIGE_OK;
+  DevBuf d_nv, d_af, d_mac;
+  SAIGE_TRY(d_nv.ensure(m * 4));
+  SAIGE_TRY(d_af.ensure(m * 8));
+  SAIGE_TRY(d_mac.ensure(m * 8));
+  const size_t chunk = 1u << 30;
+  for (size_t j0 = 0; j0 < m; j0 += chunk) {
+    const size_t mc = std::min(chunk, m - j0);
+    marker_stats_packed_kernel<<<(unsigned)mc, 256, 0, s>>>(
+        d_bed + j0 * bpm, bpm, n, mc, d_ac + j0, d_nv.as<uint32_t>() + j0,
+        d_af.as<double>() + j0, d_mac.as<double>() + j0);
+  }
+  SAIGE_HIP(hipGetLastError());
+  SAIGE_HIP(hipStreamSynchronize(s));  // the scratch outputs are released on return
+  return SAIGE_OK;
+}
+
 // Allele counts of m markers of a device-resident .bed body (host outputs).
 int marker_stats_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const uint32_t* d_idx,
                              size_t n, size_t m, double* ac, uint32_t* n_valid, double* af,

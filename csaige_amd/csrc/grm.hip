@@ -437,23 +437,18 @@ int grm_create_impl(int device, const uint8_t* bed_body, bool on_device, size_t 
   }
   std::vector<double> lut, sc, sh;
   build_lut(af, m, g.m_pad, lut, sc, sh);
-  if ((rc = h->lut.ensure(lut.size() * 8)) != SAIGE_OK) return fail(rc);
-  g.lut = h->lut.as<double>();
-  if (hipMemcpyAsync(g.lut, lut.data(), lut.size() * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-      hipStreamSynchronize(h->stream) != hipSuccess) {
-    set_error("lut upload failed");
-    return fail(SAIGE_ERR_DEVICE);
-  }
-  if (precision != SAIGE_PREC_FP64) {
-    if ((rc = h->scale64.ensure(g.m_pad * 8)) != SAIGE_OK) return fail(rc);
-    if ((rc = h->shift64.ensure(g.m_pad * 8)) != SAIGE_OK) return fail(rc);
-    g.scale64 = h->scale64.as<double>();
-    g.shift64 = h->shift64.as<double>();
-    if (hipMemcpy(g.scale64, sc.data(), g.m_pad * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g.shift64, sh.data(), g.m_pad * 8, hipMemcpyHostToDevice) != hipSuccess) {
-      set_error("scale upload failed");
+  if (precision == SAIGE_PREC_FP64) {  // the table is read by the FP64 kernels only
+    if ((rc = h->lut.ensure(lut.size() * 8)) != SAIGE_OK) return fail(rc);
+    g.lut = h->lut.as<double>();
+    if (hipMemcpyAsync(g.lut, lut.data(), lut.size() * 8, hipMemcpyHostToDevice, h->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+      set_error("lut upload failed");
       return fail(SAIGE_ERR_DEVICE);
     }
+  } else {
+    h->host_scale.swap(sc);
+    h->host_shift.swap(sh);
     if ((rc = grm_mixed_prepare(h)) != SAIGE_OK) return fail(rc);
   }
   *out = h;

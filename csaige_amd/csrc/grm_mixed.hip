@@ -36,6 +36,19 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 int marker_stats_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, const uint32_t* d_idx,
                              size_t n, size_t m, double* ac, uint32_t* n_valid, double* af,
                              double* mac);
+int marker_ac_from_device(hipStream_t s, const uint8_t* d_bed, size_t bpm, size_t n, size_t m,
+                          double* d_ac);
+
+// zs_m = s_m (ac_m - 2 p_m n) for real markers, 0 for the padding (rounded as written: no
+// contraction into a fused multiply-add)
+__global__ void marker_zs_kernel(const double* __restrict__ sc, const double* __restrict__ p2,
+                                 const double* __restrict__ ac, double n, size_t m, size_t m_pad,
+                                 double* __restrict__ zs) {
+#pragma clang fp contract(off)
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m_pad) return;
+  zs[j] = j < m ? sc[j] * (ac[j] - p2[j] * n) : 0.0;
+}
 
 constexpr int kMixRT = 8;                     // 16-row MFMA tiles per wave
 constexpr int kMixRows = 4 * 16 * kMixRT;      // rows per work-group (4 waves)
@@ -466,33 +479,31 @@ int grm_mixed_prepare(saige_grm* h) {
               "factored MFMA paths require complete calls (use SAIGE_PREC_FP64)", nmiss);
     return SAIGE_ERR_ARG;
   }
-  // per-marker s_m, 2 p_m, zs_m = s_m (ac_m - 2 p_m n) with ac from the stored codes
-  std::vector<double> sc(g.m_pad, 0.0), p2(g.m_pad, 0.0), zs(g.m_pad, 0.0), lut(g.m_pad * 4);
-  std::vector<double> sc64(g.m_pad), sh64(g.m_pad);
-  SAIGE_HIP(hipMemcpy(sc64.data(), g.scale64, g.m_pad * 8, hipMemcpyDeviceToHost));
-  SAIGE_HIP(hipMemcpy(sh64.data(), g.shift64, g.m_pad * 8, hipMemcpyDeviceToHost));
-  std::vector<uint8_t> row(g.row_bytes);
+  // per-marker s_m, 2 p_m (from the host's 1/sd and 2p/sd) and, on the device,
+  // zs_m = s_m (ac_m - 2 p_m n) with ac from the stored codes
+  std::vector<double> sc(g.m_pad, 0.0), p2(g.m_pad, 0.0);
   double P2 = 0.0;
   for (size_t j = 0; j < g.m; ++j) {
-    sc[j] = sc64[j];
-    p2[j] = sc64[j] > 0.0 ? sh64[j] / sc64[j] : 0.0;  // shift = 2p * s
+    sc[j] = h->host_scale[j];
+    p2[j] = sc[j] > 0.0 ? h->host_shift[j] / sc[j] : 0.0;  // shift = 2p * s
     P2 += p2[j];
-  }
-  // zs_m needs the allele count of each marker: one pass over the SNP-major rows
-  {
-    std::vector<double> ac(g.m), af(g.m), mac(g.m);
-    std::vector<uint32_t> nv(g.m);
-    SAIGE_TRY(marker_stats_from_device(s, g.geno, g.row_bytes, nullptr, g.n, g.m, ac.data(),
-                                       nv.data(), af.data(), mac.data()));
-    for (size_t j = 0; j < g.m; ++j) zs[j] = sc[j] * (ac[j] - p2[j] * (double)g.n);
   }
   ms->P2 = P2;
   SAIGE_TRY(ms->sc.ensure(g.m_pad * 8));
   SAIGE_TRY(ms->p2.ensure(g.m_pad * 8));
   SAIGE_TRY(ms->zs.ensure(g.m_pad * 8));
-  SAIGE_HIP(hipMemcpy(ms->sc.p, sc.data(), g.m_pad * 8, hipMemcpyHostToDevice));
-  SAIGE_HIP(hipMemcpy(ms->p2.p, p2.data(), g.m_pad * 8, hipMemcpyHostToDevice));
-  SAIGE_HIP(hipMemcpy(ms->zs.p, zs.data(), g.m_pad * 8, hipMemcpyHostToDevice));
+  SAIGE_HIP(hipMemcpyAsync(ms->sc.p, sc.data(), g.m_pad * 8, hipMemcpyHostToDevice, s));
+  SAIGE_HIP(hipMemcpyAsync(ms->p2.p, p2.data(), g.m_pad * 8, hipMemcpyHostToDevice, s));
+  {
+    DevBuf d_ac;
+    SAIGE_TRY(d_ac.ensure(std::max<size_t>(g.m, 1) * 8));
+    SAIGE_TRY(marker_ac_from_device(s, g.geno, g.row_bytes, g.n, g.m, d_ac.as<double>()));
+    marker_zs_kernel<<<(unsigned)ceil_div(g.m_pad, 256), 256, 0, s>>>(
+        ms->sc.as<double>(), ms->p2.as<double>(), d_ac.as<double>(), (double)g.n, g.m, g.m_pad,
+        ms->zs.as<double>());
+    SAIGE_HIP(hipGetLastError());
+    SAIGE_HIP(hipStreamSynchronize(s));  // the host vectors and d_ac end here
+  }
   if (h->precision == SAIGE_PREC_INT8) {  // nothing reads the row-major forms from here on
     SAIGE_TRY(retile_codes(s, h->geno, g.m_pad, g.row_bytes));
     g.geno = h->geno.as<uint8_t>();
