@@ -264,6 +264,11 @@ int col_dots(hipStream_t s, const double* a, size_t lda, const double* b, size_t
 // Gram block: out[a_i + na*b_j] = dot(A[:, a_i], B[:, b_j]) (host result, na x nb col-major).
 int gram(hipStream_t s, const double* A, size_t lda, size_t na, const double* B, size_t ldb,
          size_t nb, size_t n, double* h_out, DevBuf& part, DevBuf& res);
+// Column dots into device memory (d_out[c], same arithmetic as col_dots), no host wait; with
+// read_doubles, several dot products share one device-to-host read.
+int col_dots_dev(hipStream_t s, const double* a, size_t lda, const double* b, size_t ldb,
+                 size_t n, size_t k, double* d_out, DevBuf& part);
+int read_doubles(hipStream_t s, const double* d, size_t nq, double* h_out);
 
 // The handle itself as a product sink; with loco >= 0 its products leave that chromosome
 // out of the GRM.

@@ -359,6 +359,32 @@ int gram(hipStream_t s, const double* A, size_t lda, size_t na, const double* B,
   return dot_pairs(s, A, lda, na, B, ldb, na * nb, 0, n, h_out, part, res);
 }
 
+int col_dots_dev(hipStream_t s, const double* a, size_t lda, const double* b, size_t ldb,
+                 size_t n, size_t k, double* d_out, DevBuf& part) {
+  if (k == 0) return SAIGE_OK;
+  const size_t nchunks = std::max<size_t>(1, ceil_div(n, kDotChunk));
+  SAIGE_TRY(part.ensure(k * nchunks * 8));
+  dot_pairs_kernel<<<dim3((unsigned)nchunks, (unsigned)k), kDotBlock, 0, s>>>(
+      a, lda, b, ldb, k, 1, n, nchunks, part.as<double>());
+  finalize_sums_kernel<<<(unsigned)ceil_div(k, 256), 256, 0, s>>>(part.as<double>(), nchunks, k,
+                                                                    d_out);
+  SAIGE_HIP(hipGetLastError());
+  return SAIGE_OK;
+}
+
+int read_doubles(hipStream_t s, const double* d, size_t nq, double* h_out) {
+  if (nq == 0) return SAIGE_OK;
+  void* pin = pinned_staging(nq * 8);
+  if (!pin) {
+    set_error("hipHostMalloc failed");
+    return SAIGE_ERR_NOMEM;
+  }
+  SAIGE_HIP(hipMemcpyAsync(pin, d, nq * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  memcpy(h_out, pin, nq * 8);
+  return SAIGE_OK;
+}
+
 // Per-marker table of standardized values: pcg.rs:106-115 in fp64.
 static void build_lut(const double* af, size_t m, size_t m_pad, std::vector<double>& lut,
                       std::vector<double>& sc, std::vector<double>& sh) {

@@ -74,32 +74,48 @@ __global__ __launch_bounds__(kDotBlock) void pcg_init_kernel(
   }
 }
 
-__global__ void pcg_init_fin(const double* __restrict__ part, size_t nchunks, size_t k,
-                             double tol, ColState* __restrict__ st) {
-  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= k) return;
-  double rz = 0.0, bb = 0.0, rr = 0.0;
-  for (size_t q = 0; q < nchunks; ++q) {
-    rz += part[(0 * k + c) * nchunks + q];
-    bb += part[(1 * k + c) * nchunks + q];
-    rr += part[(2 * k + c) * nchunks + q];
+// Initial column states (one block of 1024 threads covers all columns) and the active list:
+// n_active[0] is the number of active columns, n_active[1..] their indices in order.
+__global__ __launch_bounds__(1024) void pcg_init_fin(const double* __restrict__ part,
+                                                     size_t nchunks, size_t k, double tol,
+                                                     ColState* __restrict__ st,
+                                                     int* __restrict__ n_active) {
+  __shared__ int flag[1024];
+  const size_t c = threadIdx.x;
+  int act = 0;
+  if (c < k) {
+    double rz = 0.0, bb = 0.0, rr = 0.0;
+    for (size_t q = 0; q < nchunks; ++q) {
+      rz += part[(0 * k + c) * nchunks + q];
+      bb += part[(1 * k + c) * nchunks + q];
+      rr += part[(2 * k + c) * nchunks + q];
+    }
+    ColState s;
+    s.rz = rz;
+    s.tol_abs = tol * fmax(sqrt(bb), 1.0);
+    s.alpha = 0.0;
+    s.beta = 0.0;
+    s.rnorm = sqrt(rr);
+    s.iters = 0;
+    s.pad = 0;
+    if (s.rnorm < s.tol_abs) {
+      s.active = 0;
+      s.converged = 1;
+    } else {
+      s.active = 1;
+      s.converged = 0;
+    }
+    st[c] = s;
+    act = s.active;
   }
-  ColState s;
-  s.rz = rz;
-  s.tol_abs = tol * fmax(sqrt(bb), 1.0);
-  s.alpha = 0.0;
-  s.beta = 0.0;
-  s.rnorm = sqrt(rr);
-  s.iters = 0;
-  s.pad = 0;
-  if (s.rnorm < s.tol_abs) {
-    s.active = 0;
-    s.converged = 1;
-  } else {
-    s.active = 1;
-    s.converged = 0;
+  flag[threadIdx.x] = act;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int cnt = 0;
+    for (size_t q = 0; q < k; ++q)
+      if (flag[q]) n_active[1 + cnt++] = (int)q;
+    n_active[0] = cnt;
   }
-  st[c] = s;
 }
 
 // partial p'Ap for active columns
@@ -259,23 +275,16 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
 
   pcg_init_kernel<<<gchunk, kDotBlock, 0, s>>>(d_b, ldb, d_x, ldx, R, Z, P, ld, d_w, tau0, tau1, n,
                                                nchunks, part);
-  pcg_init_fin<<<kb, 256, 0, s>>>(part, nchunks, k, tol, st);
+  pcg_init_fin<<<1, 1024, 0, s>>>(part, nchunks, k, tol, st, d_nact);
   SAIGE_HIP(hipGetLastError());
-  int n_active = 0;
-  SAIGE_TRY(ws.host.ensure(list_bytes + k * sizeof(ColState)));
-  int* list = ws.host.as<int>();  // pinned: [count, active column indices]
+  // [count, active list | column states] read back in one pinned copy per iteration
+  const size_t state_bytes = list_bytes + k * sizeof(ColState);
+  SAIGE_TRY(ws.host.ensure(state_bytes));
+  int* list = ws.host.as<int>();
   ColState* hs = reinterpret_cast<ColState*>(ws.host.as<char>() + list_bytes);
-  {
-    SAIGE_HIP(hipMemcpyAsync(hs, st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
-    SAIGE_HIP(hipStreamSynchronize(s));
-    for (size_t c = 0; c < k; ++c)
-      if (hs[c].active) list[1 + n_active++] = (int)c;
-    list[0] = n_active;
-    if (n_active > 0 && (size_t)n_active < k) {
-      SAIGE_HIP(hipMemcpyAsync(d_nact, list, (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-      SAIGE_HIP(hipStreamSynchronize(s));  // the pinned list is rewritten below
-    }
-  }
+  SAIGE_HIP(hipMemcpyAsync(list, d_nact, state_bytes, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  int n_active = list[0];
   for (uint32_t iter = 0; iter < max_iter && n_active > 0; ++iter) {
     if ((size_t)n_active == k) {
       SAIGE_TRY(sink->product(P, ld, k, AP, ld, d_w, tau));
@@ -293,15 +302,19 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
     pcg_beta_fin<<<1, 1024, 0, s>>>(part, nchunks, k, (int)iter, st, d_nact);
     pcg_p_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(P, Z, ld, n, st);
     SAIGE_HIP(hipGetLastError());
-    SAIGE_HIP(hipMemcpyAsync(list, d_nact, sizeof(int), hipMemcpyDeviceToHost, s));
+    SAIGE_HIP(hipMemcpyAsync(list, d_nact, state_bytes, hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
     n_active = list[0];
   }
-  pcg_final<<<kb, 256, 0, s>>>(k, max_iter, st);
-  SAIGE_HIP(hipGetLastError());
+  if (n_active > 0) {  // stopped at max_iter: the remaining columns' final states
+    pcg_final<<<kb, 256, 0, s>>>(k, max_iter, st);
+    SAIGE_HIP(hipGetLastError());
+    if (info) {
+      SAIGE_HIP(hipMemcpyAsync(hs, st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
+      SAIGE_HIP(hipStreamSynchronize(s));
+    }
+  }
   if (info) {
-    SAIGE_HIP(hipMemcpyAsync(hs, st, k * sizeof(ColState), hipMemcpyDeviceToHost, s));
-    SAIGE_HIP(hipStreamSynchronize(s));
     for (size_t c = 0; c < k; ++c) {
       info[c].iterations = (uint32_t)hs[c].iters;
       info[c].residual = hs[c].rnorm;

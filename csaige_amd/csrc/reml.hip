@@ -230,6 +230,7 @@ __global__ void rademacher_kernel(ChaChaKey key, size_t n, size_t R, double* __r
 
 struct RemlState {
   DevBuf x, y, eta, mu, w, rhs1, sol1, gu, py, gpy, pu, rhs2, sol2, tmp, coef, part, res, ones;
+  DevBuf dres;  // scalars of one AI-REML step, read back together
   DevBuf glm, xinv, a_flat, at;  // IRLS work columns; (X'WX)^-1 and X'W-side products
   PcgWork pcg;
 };
@@ -310,7 +311,8 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
   uint32_t pcg_fail = 0;
   std::vector<saige_pcg_info> info(k1);
   std::vector<double> G((1 + p) * (p + R)), xsx(p * p), xsy(p), l, tcol(p), corr(p * std::max<size_t>(R, 2));
-  std::vector<double> tr(std::max<size_t>(R, 1)), misc(4);
+  std::vector<double> tr(std::max<size_t>(R, 1)), misc(4), sc1(R + 4);
+  SAIGE_TRY(st.dres.ensure((R + 4) * 8));
 
   for (uint32_t iter = 0; iter < cfg->max_iter; ++iter) {
     n_iter = iter + 1;
@@ -348,16 +350,21 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
     project_kernel<<<grid2(n, 1), 256, 0, s>>>(SIY, ld, SIX, ld, p, COEF, n, 1, PY, ld);
     SAIGE_HIP(hipGetLastError());
     SAIGE_TRY(sink->product(PY, ld, 1, GPY, ld, nullptr, nullptr));
-    double ypapy;
-    SAIGE_TRY(col_dots(s, PY, ld, GPY, ld, n, 1, &ypapy, st.part, st.res));
-    double trace = 0.0;
+    // y'P GRM P y and the trace terms (GRM u)'P u in one read (dres[0], dres[1..R])
+    double* DRES = st.dres.as<double>();
+    SAIGE_TRY(col_dots_dev(s, PY, ld, GPY, ld, n, 1, DRES, st.part));
     if (R > 0) {
-      // P u_r = Sigma^-1 u_r - Sigma^-1 X corr_r (ai_reml.rs:242-251); trace terms (GRM u)'Pu
+      // P u_r = Sigma^-1 u_r - Sigma^-1 X corr_r (ai_reml.rs:242-251)
       SAIGE_HIP(hipMemcpyAsync(COEF, corr.data(), p * R * 8, hipMemcpyHostToDevice, s));
       project_kernel<<<grid2(n, R), 256, 0, s>>>(SIU, ld, SIX, ld, p, COEF, n, R, PU, ld);
       SAIGE_HIP(hipGetLastError());
-      SAIGE_TRY(col_dots(s, GU, ld, PU, ld, n, R, tr.data(), st.part, st.res));
-      for (size_t r = 0; r < R; ++r) trace += tr[r];
+      SAIGE_TRY(col_dots_dev(s, GU, ld, PU, ld, n, R, DRES + 1, st.part));
+    }
+    SAIGE_TRY(read_doubles(s, DRES, 1 + R, sc1.data()));
+    const double ypapy = sc1[0];
+    double trace = 0.0;
+    if (R > 0) {
+      for (size_t r = 0; r < R; ++r) trace += sc1[1 + r];
       trace /= (double)R;
     }
     const double score_g = ypapy - trace;
@@ -382,25 +389,33 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
     SAIGE_HIP(hipMemcpyAsync(COEF, corr.data(), p * k2 * 8, hipMemcpyHostToDevice, s));
     project_kernel<<<grid2(n, k2), 256, 0, s>>>(SOL2, ld, SIX, ld, p, COEF, n, k2, TMP, ld);
     SAIGE_HIP(hipGetLastError());
-    double ai11;
-    SAIGE_TRY(col_dots(s, GPY, ld, TMP, ld, n, 1, &ai11, st.part, st.res));
+    // ai11 = dres[0]; without a fixed tau_e also ypa0py, ai00, ai01 = dres[1..3] and the
+    // trace terms u'W^-1 P u = dres[4..4+R), all in one read
+    SAIGE_TRY(col_dots_dev(s, GPY, ld, TMP, ld, n, 1, DRES, st.part));
     double score_e = 0.0, ai00 = 0.0, ai01 = 0.0;
     if (!fix_tau_e) {
       // TMP = [P GRM P y | P (P y / W) | U / W]; trace(P diag(1/W)) reuses P u
       // (ai_reml.rs:281-317 recomputes the same solves)
-      double trw = 0.0;
+      SAIGE_TRY(col_dots_dev(s, PY, ld, RHS2 + ld, ld, n, 1, DRES + 1, st.part));
+      SAIGE_TRY(col_dots_dev(s, RHS2 + ld, ld, TMP + ld, ld, n, 1, DRES + 2, st.part));
+      SAIGE_TRY(col_dots_dev(s, RHS2 + ld, ld, TMP, ld, n, 1, DRES + 3, st.part));
       if (R > 0) {
         div_w_kernel<<<grid2(n, R), 256, 0, s>>>(U, ld, dW, n, R, TMP + 2 * ld, ld);
         SAIGE_HIP(hipGetLastError());
-        SAIGE_TRY(col_dots(s, TMP + 2 * ld, ld, PU, ld, n, R, tr.data(), st.part, st.res));
-        for (size_t r = 0; r < R; ++r) trw += tr[r];
+        SAIGE_TRY(col_dots_dev(s, TMP + 2 * ld, ld, PU, ld, n, R, DRES + 4, st.part));
+      }
+    }
+    SAIGE_TRY(read_doubles(s, DRES, fix_tau_e ? 1 : 4 + R, sc1.data()));
+    const double ai11 = sc1[0];
+    if (!fix_tau_e) {
+      double trw = 0.0;
+      if (R > 0) {
+        for (size_t r = 0; r < R; ++r) trw += sc1[4 + r];
         trw /= (double)R;
       }
-      double ypa0py;
-      SAIGE_TRY(col_dots(s, PY, ld, RHS2 + ld, ld, n, 1, &ypa0py, st.part, st.res));
-      score_e = ypa0py - trw;
-      SAIGE_TRY(col_dots(s, RHS2 + ld, ld, TMP + ld, ld, n, 1, &ai00, st.part, st.res));
-      SAIGE_TRY(col_dots(s, RHS2 + ld, ld, TMP, ld, n, 1, &ai01, st.part, st.res));
+      score_e = sc1[1] - trw;
+      ai00 = sc1[2];
+      ai01 = sc1[3];
     }
     // tau update with step halving (ai_reml.rs:329-369)
     double tn[2], max_change;
