@@ -514,6 +514,11 @@ def main():
     # 2 * n * M * k useful flop (exact genotypes times the k-column operand)
     launch_flops = 2.0 * n * mg * k
     mixed_tf = launch_flops / (kern["int8"] * 1e-3) / 1e12
+    # algorithmic bytes of a launch: one pass over the packed genotypes, ceil(n/4) * M; at
+    # 2 * k = 68 useful ops per genotype (272 per byte, under the int8 ridge of 625) the
+    # launch is bound by HBM on the algorithmic roofline
+    launch_bytes = float((n + 3) // 4) * mg
+    mixed_gbs = launch_bytes / (kern["int8"] * 1e-3) / 1e9
 
     # ---- step 2: streamed score test + SPA (n = 400,000, 4,096-variant uint8 batches) ----
     step2 = None
@@ -642,14 +647,18 @@ def main():
                    "grm_column_products": int(rm["grm_products"]),
                    "pcg_iterations": int(rm["pcg_iterations"]), "accuracy": acc,
                    "product_ms": prod, "gemm_launch_ms": kern},
-        "roofline": {"bound": "mfma", "achieved": mixed_tf, "peak": I8_PEAK_TOPS,
-                     "unit": "TOP/s", "frac": mixed_tf / I8_PEAK_TOPS,
+        "roofline": {"bound": "hbm", "achieved": mixed_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": mixed_gbs / HBM_PEAK_GBS,
                      "traffic": (args.traffic_bytes if args.traffic_bytes is not None else
                                  MEASURED_TRAFFIC.get((args.n_fam, args.markers))),
                      "kernel": f"i8_gemm_kernel (int8 MFMA genotype GEMM), k={k}: "
                                f"{kern['int8']:.3f} ms/launch (HIP events on the handle stream), "
-                               f"algorithmic 2*n*M*k = {launch_flops:.3g} op/launch; the three "
-                               f"operand slices execute 3x that (padded to 16-column tiles); "
+                               f"algorithmic bytes ceil(n/4)*M = {launch_bytes:.4g} B/launch "
+                               f"and 2*n*M*k = {launch_flops:.3g} op/launch "
+                               f"({mixed_tf:.0f} TOP/s useful = {mixed_tf / I8_PEAK_TOPS:.3f} of "
+                               f"{I8_PEAK_TOPS:.0f}); the three operand slices execute 3x the "
+                               f"useful ops (padded to 16-column tiles), "
+                               f"{mixed_tf * 16 * -(-3 * k // 16) / k:.0f} TOP/s executed; "
                                f"whole product {prod['int8']:.3f} ms; bf16 GEMM "
                                f"{kern['bf16']:.3f} ms/launch; fp64 GEMM {kern['fp64']:.3f} "
                                f"ms/launch = {launch_flops / (kern['fp64'] * 1e-3) / 1e12:.2f} "
