@@ -965,6 +965,25 @@ __device__ __forceinline__ double recip_nr(double d) {
   return isinf(d) ? 0.0 : r;
 }
 
+// Running product for a sum of logs: sum_i ln x_i = ln(P) + E ln 2 with P in [0.5, 1) and E
+// an integer, each factor split by frexp (exact) so that P neither overflows nor underflows.
+// One rounding of relative size 2^-53 per factor replaces a logarithm per factor: at
+// n = 400,000 the sum moves by at most n 2^-53 ~ 4e-11 (about 1e-13 for random rounding).
+struct LogProd {
+  double p = 1.0;
+  int e = 0;
+  __device__ __forceinline__ void mul(double x) {
+    int ex;
+    p *= frexp(x, &ex);
+    e += ex;
+    p = frexp(p, &ex);
+    e += ex;
+  }
+  __device__ __forceinline__ double log_value() const {
+    return log(p) + (double)e * 0.69314718055994530942;  // ln 2
+  }
+};
+
 // KIND 0: g_pos, g_neg, the NA moments, m1 = sum mu g~ (single_variant.rs:189-192) and the
 // carrier moments sum mu, sum mu(1-mu), sum mu^2(1-mu) over samples with dosage > 0 (the
 // learned proposal's inputs); 1: K'(t) and K''(t) sums at both pending points; 2: K(zeta)
@@ -997,6 +1016,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
   double a[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) a[k] = 0.0;
+  LogProd lk0, lk1;  // KIND 2: K at the roots as the logs of running products
   for (size_t i = i0 + threadIdx.x; i < i1; i += kSpaPassBlock) {
     const double m = mu[i];
     if (KIND == 0) {
@@ -1027,12 +1047,16 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
       a[0] = fma(mg, r0, a[0]);
       a[2] = fma(mg, r1, a[2]);
     } else {
-      // K at the roots is ln(1 - m + m e^{g zeta}) (binary.rs:148-151)
-      a[0] += log(fma(m, exp(gi * t0), om));
-      a[2] += log(fma(m, exp(gi * t1), om));
+      // K at the roots is the sum of ln(1 - m + m e^{g zeta}) (binary.rs:148-151)
+      lk0.mul(fma(m, exp(gi * t0), om));
+      lk1.mul(fma(m, exp(gi * t1), om));
     }
     a[1] = fma(w2 * e0, r0 * r0, a[1]);
     a[3] = fma(w2 * e1, r1 * r1, a[3]);
+  }
+  if (KIND == 2) {
+    a[0] = lk0.log_value();
+    a[2] = lk1.log_value();
   }
   block_sum_t<kSpaPassBlock, K>(a, sh);
   if (threadIdx.x == 0) {
@@ -1102,6 +1126,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
   double a[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) a[k] = 0.0;
+  LogProd lk0, lk1;  // KIND 2: K at the roots as the logs of running products
   uint32_t bmin = 255u, bmax = 0u;
   auto body = [&](uint32_t by, double m) {
     const double* e = tab + by * TW;
@@ -1138,9 +1163,10 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
         a[0] = fma(mg, r0, a[0]);
         a[2] = fma(mg, r1, a[2]);
       } else {
-        // K at the roots is ln(1 - m + m e^{g zeta}) (binary.rs:148-151)
-        a[0] = fma(e[5], log(fma(m, e[3], om)), a[0]);
-        a[2] = fma(e[5], log(fma(m, e[4], om)), a[2]);
+        // K at the roots is the sum of ln(1 - m + m e^{g zeta}) over the included bytes
+        // (binary.rs:148-151); an excluded byte's factor is 1
+        lk0.mul(e[5] != 0.0 ? fma(m, e[3], om) : 1.0);
+        lk1.mul(e[5] != 0.0 ? fma(m, e[4], om) : 1.0);
       }
       a[1] = fma(w2 * e0, r0 * r0, a[1]);
       a[3] = fma(w2 * e1, r1 * r1, a[3]);
@@ -1176,6 +1202,10 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
     tail = i4;
   }
   for (size_t j = tail + threadIdx.x; j < i1; j += kSpaPassBlock) body(row[j], mu[j]);
+  if (KIND == 2) {
+    a[0] = lk0.log_value();
+    a[2] = lk1.log_value();
+  }
   __shared__ uint32_t brange[2];
   if (KIND == 0) {
     if (threadIdx.x == 0) {
