@@ -420,12 +420,17 @@ class ScoreTestEngine:
                                              _p(batch.info, _lib.u32p), batch.n_file, sp, out))
         return results_to_dict(out)
 
-    def test_regions(self, G, region_sizes, weights, rho=None):
+    def test_regions(self, G, region_sizes, weights, rho=None, device_rows=None):
         """BURDEN / SKAT / SKAT-O (region.rs:76-345) for regions of consecutive rows of
         ``G`` ((nvar, n) dosages, NaN = missing) with ``region_sizes`` rows each; returns
-        a dict of arrays (pvalue = SKAT-O, burden_pvalue, skat_pvalue, rho, n_markers)."""
-        G = _f64(np.atleast_2d(G))
-        nvar = G.shape[0]
+        a dict of arrays (pvalue = SKAT-O, burden_pvalue, skat_pvalue, rho, n_markers).
+        With ``device_rows=(d_ptr, ld, nvar)`` the rows are read in place from device memory
+        (``G`` is then ignored)."""
+        if device_rows is None:
+            G = _f64(np.atleast_2d(G))
+            nvar = G.shape[0]
+        else:
+            d_ptr, ldg, nvar = device_rows
         sizes = np.asarray(region_sizes, dtype=np.int64)
         starts = np.zeros(len(sizes) + 1, np.uint64)
         starts[1:] = np.cumsum(sizes)
@@ -434,10 +439,12 @@ class ScoreTestEngine:
         w = _f64(weights)
         rr = None if rho is None else _f64(rho)
         out = (_lib.RegionResult * max(len(sizes), 1))()
-        check(load().saige_region_tests(self.h, _p(G, _lib.dp), self.n, nvar,
-                                        starts.ctypes.data_as(C.POINTER(C.c_uint64)), len(sizes),
-                                        _p(w, _lib.dp), None if rr is None else _p(rr, _lib.dp),
-                                        0 if rr is None else len(rr), out))
+        args = (starts.ctypes.data_as(C.POINTER(C.c_uint64)), len(sizes), _p(w, _lib.dp),
+                None if rr is None else _p(rr, _lib.dp), 0 if rr is None else len(rr), out)
+        if device_rows is None:
+            check(load().saige_region_tests(self.h, _p(G, _lib.dp), self.n, nvar, *args))
+        else:
+            check(load().saige_region_tests_device(self.h, d_ptr, ldg, nvar, *args))
         raw = np.frombuffer(out, dtype=np.dtype({"names": ["pvalue", "burden_pvalue",
                                                            "skat_pvalue", "rho", "n_markers"],
                                                  "formats": ["f8", "f8", "f8", "f8", "u4"],

@@ -152,6 +152,8 @@ SIGNATURES = [
                                           C.POINTER(VariantResult)]),
     ("saige_region_tests", C.c_int, [vp, dp, sz, sz, C.POINTER(C.c_uint64), sz, dp, dp, sz,
                                      C.POINTER(RegionResult)]),
+    ("saige_region_tests_device", C.c_int, [vp, vp, sz, sz, C.POINTER(C.c_uint64), sz, dp, dp,
+                                            sz, C.POINTER(RegionResult)]),
     ("saige_chi2_sf", C.c_double, [C.c_double, C.c_double]),
     ("saige_mlp_create", C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
     ("saige_mlp_destroy", None, [vp]),

@@ -965,10 +965,13 @@ __device__ __forceinline__ double recip_nr(double d) {
   return isinf(d) ? 0.0 : r;
 }
 
-// Running product for a sum of logs: sum_i ln x_i = ln(P) + E ln 2 with P in [0.5, 1) and E
-// an integer, each factor split by frexp (exact) so that P neither overflows nor underflows.
-// One rounding of relative size 2^-53 per factor replaces a logarithm per factor: at
-// n = 400,000 the sum moves by at most n 2^-53 ~ 4e-11 (about 1e-13 for random rounding).
+// Running product for a sum of logs: sum_i ln x_i = ln(P) + E ln 2 with P a product of
+// mantissas and E an integer.  mul() splits each factor x > 0 by frexp (exact) into a
+// mantissa in [0.5, 1) and an exponent, norm() brings P back to [0.5, 1) and is called at
+// least once every 8 factors, so P stays in [2^-9, 1).  One rounding of relative size 2^-53
+// per factor replaces a logarithm per factor: at n = 400,000 the sum moves by at most
+// n 2^-53 ~ 4e-11 (about 1e-13 for random rounding).  An infinite or NaN factor makes P
+// infinite or NaN, as the sum of logs would be.
 struct LogProd {
   double p = 1.0;
   int e = 0;
@@ -976,6 +979,9 @@ struct LogProd {
     int ex;
     p *= frexp(x, &ex);
     e += ex;
+  }
+  __device__ __forceinline__ void norm() {
+    int ex;
     p = frexp(p, &ex);
     e += ex;
   }
@@ -1050,6 +1056,8 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
       // K at the roots is the sum of ln(1 - m + m e^{g zeta}) (binary.rs:148-151)
       lk0.mul(fma(m, exp(gi * t0), om));
       lk1.mul(fma(m, exp(gi * t1), om));
+      lk0.norm();
+      lk1.norm();
     }
     a[1] = fma(w2 * e0, r0 * r0, a[1]);
     a[3] = fma(w2 * e1, r1 * r1, a[3]);
@@ -1198,10 +1206,20 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
       body((w >> 8) & 0xffu, ca.y);
       body((w >> 16) & 0xffu, cb.x);
       body(w >> 24, cb.y);
+      if (KIND == 2) {
+        lk0.norm();
+        lk1.norm();
+      }
     }
     tail = i4;
   }
-  for (size_t j = tail + threadIdx.x; j < i1; j += kSpaPassBlock) body(row[j], mu[j]);
+  for (size_t j = tail + threadIdx.x; j < i1; j += kSpaPassBlock) {
+    body(row[j], mu[j]);
+    if (KIND == 2) {
+      lk0.norm();
+      lk1.norm();
+    }
+  }
   if (KIND == 2) {
     a[0] = lk0.log_value();
     a[2] = lk1.log_value();
@@ -2206,7 +2224,7 @@ static void region_pvalues(const std::vector<double>& S, const std::vector<doubl
 }
 
 template <int P>
-static int region_batch_p(saige_engine* e, const double* dg, size_t nvar,
+static int region_batch_p(saige_engine* e, const double* dg, size_t ldg, size_t nvar,
                           const std::vector<size_t>& starts, const double* weights,
                           const double* rho, size_t n_rho, saige_region_result* out) {
   hipStream_t s = e->stream;
@@ -2221,7 +2239,7 @@ static int region_batch_p(saige_engine* e, const double* dg, size_t nvar,
   const size_t sp = ceil_div(n, sps);
   SAIGE_TRY(e->part.ensure((sp + 1) * NS * nvar_pad * 8));
   score_sums_kernel<P, LoadF64><<<dim3((unsigned)ceil_div(nvar, 64), (unsigned)sp), 64, 0, s>>>(
-      LoadF64{dg, n, 1}, e->feat.as<double>(), n, nvar, sps, e->part.as<double>(), nvar_pad);
+      LoadF64{dg, ldg, 1}, e->feat.as<double>(), n, nvar, sps, e->part.as<double>(), nvar_pad);
   double* sums = e->part.as<double>() + sp * NS * nvar_pad;
   sums_reduce_kernel<<<dim3((unsigned)ceil_div(nvar_pad, 256), (unsigned)NS), 256, 0, s>>>(
       e->part.as<double>(), sp, NS, nvar_pad, sums);
@@ -2252,7 +2270,7 @@ static int region_batch_p(saige_engine* e, const double* dg, size_t nvar,
     double* gram = gpart + gsp * nt * kRegTile * kRegTile;
     SAIGE_HIP(hipMemcpyAsync(dt, tiles.data(), nt * sizeof(RegionTile), hipMemcpyHostToDevice, s));
     region_gram_kernel<<<dim3((unsigned)nt, (unsigned)gsp), 256, 0, s>>>(
-        dg, n, e->feat.as<double>(), NF, P + 1, n, gsps, dt, gpart);
+        dg, ldg, e->feat.as<double>(), NF, P + 1, n, gsps, dt, gpart);
     region_gram_reduce<<<dim3((unsigned)ceil_div(kRegTile * kRegTile, 256), (unsigned)nt), 256, 0,
                          s>>>(gpart, gsp, nt, gram);
     SAIGE_HIP(hipGetLastError());
@@ -2304,11 +2322,12 @@ static int region_batch_p(saige_engine* e, const double* dg, size_t nvar,
 
 extern "C" double saige_chi2_sf(double df, double x) { return saige::chi2_sf(df, x); }
 
-extern "C" int saige_region_tests(saige_engine* e, const double* g, size_t ldg, size_t nvar,
-                                  const uint64_t* region_start, size_t n_regions,
-                                  const double* weights, const double* rho, size_t n_rho,
-                                  saige_region_result* out) {
-  clear_error();
+// Groups of whole regions with at most kHostBatch variants (a larger region alone); host
+// rows are copied to the engine's workspace per group, device rows are read in place.
+static int region_tests_impl(saige_engine* e, const double* g, size_t ldg, bool on_device,
+                             size_t nvar, const uint64_t* region_start, size_t n_regions,
+                             const double* weights, const double* rho, size_t n_rho,
+                             saige_region_result* out) {
   SAIGE_ARG(e && region_start && (n_regions == 0 || out), "null pointer");
   SAIGE_ARG(nvar == 0 || (g && weights), "null pointer");
   SAIGE_ARG(ldg >= e->n, "ldg < n");
@@ -2324,7 +2343,6 @@ extern "C" int saige_region_tests(saige_engine* e, const double* g, size_t ldg, 
   SAIGE_HIP(hipSetDevice(e->device));
   hipStream_t s = e->stream;
   const size_t n = e->n;
-  // groups of whole regions with at most kHostBatch variants (a larger region alone)
   size_t r0 = 0;
   while (r0 < n_regions) {
     size_t r1 = r0 + 1;
@@ -2335,24 +2353,49 @@ extern "C" int saige_region_tests(saige_engine* e, const double* g, size_t ldg, 
     if (nb == 0) {
       for (size_t r = r0; r < r1; ++r) region_pvalues({}, {}, 0, rho, n_rho, &out[r]);
     } else {
-      SAIGE_TRY(e->in.ensure(nb * n * 8));
-      double* dg = e->in.as<double>();
-      SAIGE_HIP(hipMemcpy2DAsync(dg, n * 8, g + v0 * ldg, ldg * 8, n * 8, nb,
-                                 hipMemcpyHostToDevice, s));
+      const double* dg = g + v0 * ldg;
+      size_t ld = ldg;
+      if (!on_device) {
+        SAIGE_TRY(e->in.ensure(nb * n * 8));
+        SAIGE_HIP(hipMemcpy2DAsync(e->in.p, n * 8, g + v0 * ldg, ldg * 8, n * 8, nb,
+                                   hipMemcpyHostToDevice, s));
+        dg = e->in.as<double>();
+        ld = n;
+      }
       int rc = SAIGE_ERR_ARG;
+      const double* w = weights + v0;
       switch (e->p) {
-        case 1: rc = region_batch_p<1>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
-        case 2: rc = region_batch_p<2>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
-        case 3: rc = region_batch_p<3>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
-        case 4: rc = region_batch_p<4>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
-        case 5: rc = region_batch_p<5>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
-        case 6: rc = region_batch_p<6>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
-        case 7: rc = region_batch_p<7>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
-        case 8: rc = region_batch_p<8>(e, dg, nb, starts, weights + v0, rho, n_rho, out + r0); break;
+        case 1: rc = region_batch_p<1>(e, dg, ld, nb, starts, w, rho, n_rho, out + r0); break;
+        case 2: rc = region_batch_p<2>(e, dg, ld, nb, starts, w, rho, n_rho, out + r0); break;
+        case 3: rc = region_batch_p<3>(e, dg, ld, nb, starts, w, rho, n_rho, out + r0); break;
+        case 4: rc = region_batch_p<4>(e, dg, ld, nb, starts, w, rho, n_rho, out + r0); break;
+        case 5: rc = region_batch_p<5>(e, dg, ld, nb, starts, w, rho, n_rho, out + r0); break;
+        case 6: rc = region_batch_p<6>(e, dg, ld, nb, starts, w, rho, n_rho, out + r0); break;
+        case 7: rc = region_batch_p<7>(e, dg, ld, nb, starts, w, rho, n_rho, out + r0); break;
+        case 8: rc = region_batch_p<8>(e, dg, ld, nb, starts, w, rho, n_rho, out + r0); break;
       }
       if (rc != SAIGE_OK) return rc;
     }
     r0 = r1;
   }
   return SAIGE_OK;
+}
+
+extern "C" int saige_region_tests(saige_engine* e, const double* g, size_t ldg, size_t nvar,
+                                  const uint64_t* region_start, size_t n_regions,
+                                  const double* weights, const double* rho, size_t n_rho,
+                                  saige_region_result* out) {
+  clear_error();
+  return region_tests_impl(e, g, ldg, false, nvar, region_start, n_regions, weights, rho, n_rho,
+                           out);
+}
+
+extern "C" int saige_region_tests_device(saige_engine* e, const double* d_g, size_t ldg,
+                                         size_t nvar, const uint64_t* region_start,
+                                         size_t n_regions, const double* weights,
+                                         const double* rho, size_t n_rho,
+                                         saige_region_result* out) {
+  clear_error();
+  return region_tests_impl(e, d_g, ldg, true, nvar, region_start, n_regions, weights, rho, n_rho,
+                           out);
 }

@@ -364,6 +364,12 @@ int saige_test_markers_bgen(saige_engine* e, const uint8_t* blocks, size_t ld, s
 int saige_region_tests(saige_engine* e, const double* g, size_t ldg, size_t nvar,
                        const uint64_t* region_start, size_t n_regions, const double* weights,
                        const double* rho, size_t n_rho, saige_region_result* out);
+/* The same with the dosage rows resident on the engine's device (d_g: nvar rows of ldg
+ * doubles in device memory, read in place; weights, region_start and out stay on the host). */
+int saige_region_tests_device(saige_engine* e, const double* d_g, size_t ldg, size_t nvar,
+                              const uint64_t* region_start, size_t n_regions,
+                              const double* weights, const double* rho, size_t n_rho,
+                              saige_region_result* out);
 /* 1 - ChiSquared::new(df).cdf(x) as the region tests evaluate it (host arithmetic). */
 double saige_chi2_sf(double df, double x);
 

@@ -148,3 +148,27 @@ def test_gpu_region_driver():
     assert abs(out[0]["pvalue"] - want[0]) <= 1e-13 or \
         abs(math.log10(out[0]["pvalue"]) - math.log10(want[0])) < 1e-6
     assert sum(out[1]["mac_category_counts"]) == 25
+
+
+@pytest.mark.gpu
+def test_gpu_region_tests_device_rows_match_host_rows():
+    """saige_region_tests_device reads resident rows (with a leading dimension above n) and
+    gives the host-row call's results bit for bit."""
+    from csaige_amd import hip
+    X, mu, y, a, rng = _engine_case(p=2, seed=9)
+    n = X.shape[1]
+    sizes = [4, 0, 33, 70, 1]
+    nvar = sum(sizes)
+    G = rng.binomial(2, rng.uniform(0.002, 0.05, nvar)[:, None], (nvar, n)).astype(float)
+    G[3, :25] = np.nan
+    w = R.beta_weights(np.nanmean(G, axis=1) / 2)
+    e = sg.ScoreTestEngine(mu, y - mu, (1.0, 0.4), X, a, 1.0, y=y)
+    want = e.test_regions(G, sizes, w)
+    ld = n + 13
+    padded = np.full((nvar, ld), 7.0)
+    padded[:, :n] = G
+    d = hip.DeviceBuffer(padded.nbytes)
+    d.upload(np.ascontiguousarray(padded))
+    got = e.test_regions(None, sizes, w, device_rows=(d.value, ld, nvar))
+    for k in want:
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
