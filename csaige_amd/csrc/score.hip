@@ -438,6 +438,8 @@ __global__ __launch_bounds__(256) void score_sums_u8t_kernel(LoadU8 load,
     }
 #pragma unroll
     for (int u = 0; u < VB; ++u) {
+      // 256 samples of dosage 0 add nothing: the wave skips them (rare variants)
+      if (__ballot(w[u] != 0u) == 0) continue;
       sq[u] = __builtin_amdgcn_udot4(w[u], 0x01010101u, sq[u], false);
       sc[u] = __builtin_amdgcn_udot4(w[u], cm, sc[u], false);
 #pragma unroll
@@ -1117,10 +1119,11 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
       e[3] = 0.0;
       if (KIND == 1) {
         // an overflowing e^{-g t} makes that sample's K' term 0 and its K'' term NaN
-        // (m g / inf, inf / inf): the capped value gives K' and K'' terms below 1e-300,
-        // and the NaN is added once per variant below
-        e[1] = fmin(e[1], DBL_MAX);
-        e[2] = fmin(e[2], DBL_MAX);
+        // (m g / inf, inf / inf): the capped value (with odds at most 1e12, 1 + odds e stays
+        // finite) gives K' and K'' terms below 1e-283 relative to g, and the NaN is added
+        // once per variant below
+        e[1] = fmin(e[1], 1e295);
+        e[2] = fmin(e[2], 1e295);
       }
       if (KIND == 2) {
         e[3] = incl ? exp(gb * t0) : 1.0;
@@ -1156,21 +1159,24 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
       }
       bmin = min(bmin, by);
       bmax = max(bmax, by);
+    } else if (KIND == 1) {
+      // here m holds the odds rho = (1 - mu) / mu: with u = mu / ((1 - mu) e + mu) =
+      // 1 / (1 + rho e), the K' term mu g / ((1 - mu) e + mu) is g u and the K'' term
+      // (1 - mu) mu g^2 e / ((1 - mu) e + mu)^2 is g u (g - g u)
+      const double gk = e[0];
+      const double u0 = recip_nr_finite(fma(m, e[1], 1.0));
+      const double u1 = recip_nr_finite(fma(m, e[2], 1.0));
+      const double g0 = gk * u0, g1 = gk * u1;
+      a[0] += g0;
+      a[2] += g1;
+      a[1] = fma(g0, gk - g0, a[1]);
+      a[3] = fma(g1, gk - g1, a[3]);
     } else {
       const double gk = e[0], e0 = e[1], e1 = e[2];
       const double mg = m * gk, w2 = om * mg * gk;
-      double r0, r1;
-      if (KIND == 1) {
-        r0 = recip_nr_finite(fma(om, e0, m));
-        r1 = recip_nr_finite(fma(om, e1, m));
-      } else {
-        r0 = recip_nr(fma(om, e0, m));
-        r1 = recip_nr(fma(om, e1, m));
-      }
-      if (KIND == 1) {
-        a[0] = fma(mg, r0, a[0]);
-        a[2] = fma(mg, r1, a[2]);
-      } else {
+      const double r0 = recip_nr(fma(om, e0, m));
+      const double r1 = recip_nr(fma(om, e1, m));
+      {
         // K at the roots is the sum of ln(1 - m + m e^{g zeta}) over the included bytes
         // (binary.rs:148-151); an excluded byte's factor is 1
         lk0.mul(e[5] != 0.0 ? fma(m, e[3], om) : 1.0);
@@ -1432,6 +1438,7 @@ struct saige_engine {
   bool intercept_only = false;
   const saige_mlp* mlp = nullptr;
   DevBuf feat, cmask, mu_d, x_d, idx, part, res, gate, ngate, in;
+  DevBuf odds_d;  // (1 - mu) / mu per sample, at most 1e12 (the Newton passes of the SPA)
   DevBuf spa_vars, spa_part, spa_active;  // batched SPA state
   // sums-kernel shape (variants per lane, sample splits) chosen by timing, per (loader,
   // covariates, variants)
@@ -1501,15 +1508,16 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out) {
   GateInfo* gate = e->gate.as<GateInfo>();
   const uint32_t* ngp = e->ngate.as<uint32_t>();
   const double* mu = e->mu_d.as<double>();
+  const double* odds = e->odds_d.as<double>();
   const MlpView mv = e->mlp_view();
   const dim3 grid(ng, (unsigned)S);
   const unsigned sb = (unsigned)ceil_div(ng, 256);
   const int fast = e->ec.fast, iS = (int)S;
   auto launch_pass = [&](auto kind) {
     constexpr int KIND = decltype(kind)::value;
-    if constexpr (TAB)
-      spa_pass_tab_kernel<KIND><<<grid, kSpaPassBlock, 0, s>>>(gt.load, mu, n, gate, ngp, fast,
-                                                               slice, vars, part);
+    if constexpr (TAB)  // the Newton passes take the odds (1 - mu) / mu in place of mu
+      spa_pass_tab_kernel<KIND><<<grid, kSpaPassBlock, 0, s>>>(
+          gt.load, KIND == 1 ? odds : mu, n, gate, ngp, fast, slice, vars, part);
     else
       spa_pass_kernel<G, KIND><<<grid, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
                                                               vars, part);
@@ -1773,11 +1781,15 @@ int saige_engine_create(int device, const double* mu, const double* mu2, const d
   };
   if ((rc = e->feat.ensure(feat.size() * 8)) != SAIGE_OK) return fail(rc);
   if ((rc = e->mu_d.ensure(n * 8)) != SAIGE_OK) return fail(rc);
+  if ((rc = e->odds_d.ensure(n * 8)) != SAIGE_OK) return fail(rc);
+  std::vector<double> odds(n);
+  for (size_t i = 0; i < n; ++i) odds[i] = std::min((1.0 - mu[i]) / std::max(mu[i], 1e-300), 1e12);
   if ((rc = e->x_d.ensure(n * p * 8)) != SAIGE_OK) return fail(rc);
   if ((rc = e->cmask.ensure(cmask.size())) != SAIGE_OK) return fail(rc);
   if (hipMemcpy(e->feat.p, feat.data(), feat.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->cmask.p, cmask.data(), cmask.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->mu_d.p, mu, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->odds_d.p, odds.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->x_d.p, x, n * p * 8, hipMemcpyHostToDevice) != hipSuccess) {
     set_error("engine upload failed");
     return fail(SAIGE_ERR_DEVICE);
