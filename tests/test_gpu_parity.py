@@ -92,6 +92,33 @@ def test_grm_subset_and_marker_selection():
     np.testing.assert_allclose(g.mat_vec(v), want, atol=1e-12 * np.abs(want).max(), rtol=0)
 
 
+@pytest.mark.parametrize("n_total,n,prec", [(270000, 900, None), (270000, 900, "int8"),
+                                            (5003, 5003, "int8"), (1021, 700, "int8")],
+                         ids=["long-rows-gather", "long-rows-gather-int8", "identity-int8",
+                              "staged-list-int8"])
+def test_grm_repack_paths(n_total, n, prec):
+    """The three repack forms: identity sample order (funnel-shift copy), a sample list with
+    rows staged in LDS, and a list over rows too long to stage (270,000 samples, 67.5 KB per
+    marker: direct gather); unaligned row starts (odd bytes per marker), markers selected and
+    reordered.  FP64 and the int8 path (tiled codes, bit-transposed sample-major copy)."""
+    m_total = 37
+    body, bpm = random_bed(n_total, m_total, seed=n_total % 97, p_missing=0.0)
+    rng = np.random.default_rng(n)
+    idx = None if n == n_total else rng.choice(n_total, n, replace=False).astype(np.uint32)
+    markers = rng.permutation(m_total)[:29].astype(np.uint32)
+    d = np.stack([oracle.decode_marker(body[j], n_total, idx) for j in markers])
+    af = np.array([oracle.compute_af(r)[1] for r in d])
+    kw = {} if prec is None else {"precision": sg.PREC_INT8}
+    g = sg.OnTheFlyGrm(body, bpm, n_total, af, sample_idx=idx, marker_idx=markers, **kw)
+    V = rng.standard_normal((3, n))
+    got = g.mat_mat(V)
+    ref = oracle.Grm(d, af, oracle.GRM_F64)
+    for c in range(3):
+        want = ref.mat_vec(V[c])
+        tol = 1e-12 if prec is None else 1e-6
+        assert np.linalg.norm(got[c] - want) <= tol * np.linalg.norm(want), (c, prec)
+
+
 def test_grm_symmetric():
     n = 200
     body, bpm, d, af = _grm_pair(n, 300, seed=9)
