@@ -43,6 +43,22 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host the CPU baseline ran on: CPU model, logical CPUs, and the CPUs this process
+    may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
 def time_product(sg, hip, d_bed, bpm, n, markers, af, precision, k, reps):
     """Average device time of one k-column GRM product, HIP events on the handle's stream."""
     L = sg.load()
@@ -619,7 +635,7 @@ def main():
         p1, reps_1 = per_product(1, 4.0)
         cpu_fit = pt * ref["grm_products"]
         cpu = {"value": cpu_fit, "unit": "s (fit-null, extrapolated)", "cores": args.cpu_threads,
-               "kind": "port",
+               "kind": "port", "host": host_cpu(),
                "sample": f"f32 OnTheFlyGrm products (saige-rs pcg.rs:128-180 restated, rayon-"
                          f"shaped chunks) at n={n} x {mc} markers: {reps_t} products on "
                          f"{args.cpu_threads} threads, {reps_1} on 1; scaled by {mg}/{mc} markers "
