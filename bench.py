@@ -35,8 +35,9 @@ BF16_PEAK_TFLOPS = 2500.0
 I8_PEAK_TOPS = 5000.0
 # HBM bytes per int8 GEMM launch at k = 34, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (one counter per run; FETCH_SIZE scaled by 2.0 from a 1 GiB copy) for the default families
-# workload: mean of the marker-side and sample-side launches.  Keyed by (families, markers).
-MEASURED_TRAFFIC = {(10000, 100000): 1.51e9}
+# workload: mean of the marker-side and sample-side launches.  Keyed by (families, markers,
+# int8 form).
+MEASURED_TRAFFIC = {(10000, 100000, "int8"): 1.51e9, (10000, 100000, "int8x2"): 1.44e9}
 
 
 def log(msg):
@@ -407,6 +408,9 @@ def main():
                     help="host threads of the CPU baseline (the box's CPU share)")
     ap.add_argument("--no-step2", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--precision", default="int8x2", choices=["int8", "int8x2"],
+                    help="int8 form timed for the headline value: three operand slices (int8) "
+                         "or two (int8x2); the other form is timed beside it")
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="HBM bytes per GEMM launch measured with rocprofv3 counters")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -477,20 +481,27 @@ def main():
             f"products={r['grm_products']} pcg_iters={r['pcg_iterations']} "
             f"(qc {r['seconds_qc']:.3f} reml {r['seconds_reml']:.3f} vr {r['seconds_vr']:.3f})")
 
+    int8_forms = {"int8": sg.PREC_INT8, "int8x2": sg.PREC_INT8X2}
+    head = args.precision
+    other = "int8x2" if head == "int8" else "int8"
+    nsl = 3 if head == "int8" else 2  # int8 operand slices of the headline form
     # untimed warm-up fits load the kernels and fill the device memory cache, for each
     # precision that is timed below
     for w in range(args.warmup):
         log(f"warmup {w}")
         one_fit(sg.PREC_INT8)
+        one_fit(sg.PREC_INT8X2)
         one_fit(sg.PREC_MIXED)
         if not args.no_fp64:
             one_fit(sg.PREC_FP64)
     times, rm = [], None
     for s in range(args.steps):
-        dt, rm = one_fit(sg.PREC_INT8)
+        dt, rm = one_fit(int8_forms[head])
         times.append(dt)
-        describe(f"step {s} mixed int8", dt, rm)
+        describe(f"step {s} mixed {head}", dt, rm)
     mixed_s = sum(times) / len(times)
+    other_s, ro = one_fit(int8_forms[other])
+    describe(f"mixed {other}", other_s, ro)
     bf16_s, rb = one_fit(sg.PREC_MIXED)
     describe("mixed bf16", bf16_s, rb)
     fp64_s, rf = None, None
@@ -513,7 +524,7 @@ def main():
 
         pf = pvals(rf)
         mask = pf >= 1e-10
-        for tag, r in (("int8", rm), ("bf16", rb)):
+        for tag, r in ((head, rm), (other, ro), ("bf16", rb)):
             pm = pvals(r)
             acc[tag] = {
                 "tau_rel_diff": float(abs(r["tau"][1] - rf["tau"][1]) / rf["tau"][1]),
@@ -530,7 +541,8 @@ def main():
     k = 34
     flops = 4.0 * n * mg * k
     prod, kern = {}, {}
-    for tag, prec in (("int8", sg.PREC_INT8), ("bf16", sg.PREC_MIXED), ("fp64", sg.PREC_FP64)):
+    for tag, prec in (("int8", sg.PREC_INT8), ("int8x2", sg.PREC_INT8X2), ("bf16", sg.PREC_MIXED),
+                      ("fp64", sg.PREC_FP64)):
         ms, kms, kl = time_product(sg, hip, d_bed, c.bpm, n, grm_markers, af[grm_markers], prec,
                                    k, args.product_reps)
         prod[tag] = ms
@@ -541,12 +553,12 @@ def main():
     # the dominant kernel: one genotype GEMM launch = one of the two passes of a product,
     # 2 * n * M * k useful flop (exact genotypes times the k-column operand)
     launch_flops = 2.0 * n * mg * k
-    mixed_tf = launch_flops / (kern["int8"] * 1e-3) / 1e12
+    mixed_tf = launch_flops / (kern[head] * 1e-3) / 1e12
     # algorithmic bytes of a launch: one pass over the packed genotypes, ceil(n/4) * M; at
     # 2 * k = 68 useful ops per genotype (272 per byte, under the int8 ridge of 625) the
     # launch is bound by HBM on the algorithmic roofline
     launch_bytes = float((n + 3) // 4) * mg
-    mixed_gbs = launch_bytes / (kern["int8"] * 1e-3) / 1e9
+    mixed_gbs = launch_bytes / (kern[head] * 1e-3) / 1e9
 
     # ---- step 2: streamed score test + SPA (n = 400,000, 4,096-variant uint8 batches) ----
     step2 = None
@@ -663,12 +675,13 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8 MFMA (exact genotypes x 3 int8 operand slices) / int32 accumulate / "
+        "dtype": f"int8 MFMA (exact genotypes x {nsl} int8 operand slices) / int32 accumulate / "
                  "fp64 solver",
         "data": "synthetic (seeded families cohort; packed genotypes resident in HBM)",
         "config": {"workload": f"fit-null binary, {args.n_fam} nuclear families x 5 = {n} samples, "
                                f"{m} markers ({mg} GRM after the VR split), 30 probes",
-                   "precision": "int8", "bf16_wall_s": bf16_s, "fp64_wall_s": fp64_s,
+                   "precision": head, f"{other}_wall_s": other_s, "bf16_wall_s": bf16_s,
+                   "fp64_wall_s": fp64_s,
                    "speedup_vs_fp64": (fp64_s / mixed_s) if fp64_s else None,
                    "tau_g": float(rm["tau"][1]), "ai_reml_iterations": int(rm["iterations"]),
                    "variance_ratio": float(rm["vr"]),
@@ -678,16 +691,16 @@ def main():
         "roofline": {"bound": "hbm", "achieved": mixed_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": mixed_gbs / HBM_PEAK_GBS,
                      "traffic": (args.traffic_bytes if args.traffic_bytes is not None else
-                                 MEASURED_TRAFFIC.get((args.n_fam, args.markers))),
-                     "kernel": f"i8_gemm_kernel (int8 MFMA genotype GEMM), k={k}: "
-                               f"{kern['int8']:.3f} ms/launch (HIP events on the handle stream), "
+                                 MEASURED_TRAFFIC.get((args.n_fam, args.markers, head))),
+                     "kernel": f"i8_gemm_kernel (int8 MFMA genotype GEMM, {head}), k={k}: "
+                               f"{kern[head]:.3f} ms/launch (HIP events on the handle stream), "
                                f"algorithmic bytes ceil(n/4)*M = {launch_bytes:.4g} B/launch "
                                f"and 2*n*M*k = {launch_flops:.3g} op/launch "
                                f"({mixed_tf:.0f} TOP/s useful = {mixed_tf / I8_PEAK_TOPS:.3f} of "
-                               f"{I8_PEAK_TOPS:.0f}); the three operand slices execute 3x the "
-                               f"useful ops (padded to 16-column tiles), "
-                               f"{mixed_tf * 16 * -(-3 * k // 16) / k:.0f} TOP/s executed; "
-                               f"whole product {prod['int8']:.3f} ms; bf16 GEMM "
+                               f"{I8_PEAK_TOPS:.0f}); the {nsl} operand slices execute {nsl}x "
+                               f"the useful ops (padded to 16-column tiles), "
+                               f"{mixed_tf * 16 * -(-nsl * k // 16) / k:.0f} TOP/s executed; "
+                               f"whole product {prod[head]:.3f} ms; bf16 GEMM "
                                f"{kern['bf16']:.3f} ms/launch; fp64 GEMM {kern['fp64']:.3f} "
                                f"ms/launch = {launch_flops / (kern['fp64'] * 1e-3) / 1e12:.2f} "
                                f"TFLOP/s of {FP64_PEAK_TFLOPS}"},

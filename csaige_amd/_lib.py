@@ -16,6 +16,7 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_NOMEM", 4: "ERR_NOT_PD
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_INT8 = 2
+PREC_INT8X2 = 3
 TRAIT_BINARY = 0
 TRAIT_QUANTITATIVE = 1
 TRAIT_SURVIVAL = 2

@@ -422,9 +422,11 @@ int grm_create_impl(int device, const uint8_t* bed_body, bool on_device, size_t 
   SAIGE_ARG(m == 0 || (bed_body && af), "bed_body/af");
   SAIGE_ARG(bpm >= (n_total + 3) / 4, "bytes_per_marker < ceil(n_total/4)");
   SAIGE_ARG(precision == SAIGE_PREC_FP64 || precision == SAIGE_PREC_MIXED ||
-                precision == SAIGE_PREC_INT8, "precision");
-  SAIGE_ARG(precision != SAIGE_PREC_INT8 || (n <= 8000000 && m <= 8000000),
-            "SAIGE_PREC_INT8 needs n and M <= 8,000,000 (exact int32 accumulation)");
+                precision == SAIGE_PREC_INT8 || precision == SAIGE_PREC_INT8X2, "precision");
+  SAIGE_ARG((precision != SAIGE_PREC_INT8 && precision != SAIGE_PREC_INT8X2) ||
+                (n <= 8000000 && m <= 8000000),
+            "SAIGE_PREC_INT8 and SAIGE_PREC_INT8X2 need n and M <= 8,000,000 (exact int32 "
+            "accumulation)");
   if (!sample_idx) SAIGE_ARG(n == n_total, "n must equal n_total without a subset");
   SAIGE_TRY(use_device(device));
   saige_grm* h = new saige_grm();

@@ -504,7 +504,8 @@ int grm_mixed_prepare(saige_grm* h) {
     SAIGE_HIP(hipGetLastError());
     SAIGE_HIP(hipStreamSynchronize(s));  // the host vectors and d_ac end here
   }
-  if (h->precision == SAIGE_PREC_INT8) {  // nothing reads the row-major forms from here on
+  if (h->precision == SAIGE_PREC_INT8 || h->precision == SAIGE_PREC_INT8X2) {
+    // nothing reads the row-major forms from here on
     SAIGE_TRY(retile_codes(s, h->geno, g.m_pad, g.row_bytes));
     g.geno = h->geno.as<uint8_t>();
     SAIGE_TRY(retile_codes(s, ms->geno_t, g.n_pad, ms->rbt));
@@ -598,7 +599,8 @@ int grm_int8_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
 
 int grm_mixed_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
                       size_t ldo, const double* d_w, const double* tau) {
-  if (h->precision == SAIGE_PREC_INT8) return grm_int8_product(h, d_v, ldv, k, d_out, ldo, d_w, tau);
+  if (h->precision == SAIGE_PREC_INT8 || h->precision == SAIGE_PREC_INT8X2)
+    return grm_int8_product(h, d_v, ldv, k, d_out, ldo, d_w, tau);
   for (size_t c0 = 0; c0 < k; c0 += 128) {
     const size_t kc = std::min<size_t>(128, k - c0);
     const double* v = d_v + c0 * ldv;
@@ -630,25 +632,28 @@ void grm_mixed_release(saige_grm* h) {
 
 
 // =================================================================================
-// Int8 path (SAIGE_PREC_INT8): exact genotypes times int8 operand slices on integer MFMA.
+// Int8 paths (SAIGE_PREC_INT8, SAIGE_PREC_INT8X2): exact genotypes times int8 operand
+// slices on integer MFMA.
 // =================================================================================
-// The centred operand x' of each column is split into three int8 slices with per-column
-// scales s0 = max|x'| / 127, s1 = s0 / 254, s2 = s1 / 254:
+// The centred operand x' of each column is split into S int8 slices (S = 3 for
+// SAIGE_PREC_INT8, 2 for SAIGE_PREC_INT8X2) with per-column scales s0 = max|x'| / 127,
+// s1 = s0 / 254, s2 = s1 / 254:
 //   q0 = rint(x'/s0),  q1 = rint((x' - s0 q0)/s1),  q2 = rint((x' - s0 q0 - s1 q1)/s2),
-// each in [-127, 127], so |x' - sum_j s_j q_j| <= s2/2 = max|x'| / (2 * 127 * 254^2), about
-// 6.1e-8 of the column's largest entry.  G^T Q_j is computed exactly with
+// each in [-127, 127], so |x' - sum_j s_j q_j| <= s_{S-1}/2: max|x'| / (2 * 127 * 254^2),
+// about 6.1e-8 of the column's largest entry, with three slices; max|x'| / (2 * 127 * 254),
+// about 1.55e-5, with two.  G^T Q_j is computed exactly with
 // v_mfma_i32_16x16x64_i8 (int32 accumulation: |sum| <= 2 * 127 * n < 2^31 for n <= 8e6),
 // the slices are recombined in fp64, and x'_split = sum_j s_j q_j enters the rank-one
 // terms of the factored form exactly as in the bf16 path.  Integer partial sums make the
 // result independent of how the reduction is split.
 //
 // Operand layout: slice j of vector c is column j*k + c of a KP-column int8 matrix (columns
-// from 3k up to KP are zero), each column pad_len long in reduction order.  Within every
+// from S*k up to KP are zero), each column pad_len long in reduction order.  Within every
 // group of 16 reduction indices the bytes are stored 4x4-transposed (index 4b+u at byte
 // 4u+b): a lane expands one 32-bit word of packed codes (code 4b+u at bits 8b+2u) with
 // v_perm into four words, word u holding codes {u, 4+u, 8+u, 12+u} one per byte, and the
 // operand's bytes then pair with the same reduction indices.
-constexpr int kI8MaxKT = 7;       // at most 112 slice columns (37 vectors) per launch
+constexpr int kI8MaxKT = 7;  // at most 112 slice columns (37 vectors of 3 slices, 56 of 2)
 constexpr size_t kI8MaxRed = 8000000;
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
@@ -716,20 +721,21 @@ __device__ __forceinline__ size_t perm16(size_t i) {
 }
 __device__ __forceinline__ double q127(double t) { return fmin(fmax(rint(t), -127.0), 127.0); }
 
-// x' = x - mean -> three int8 slices (stored in the permuted reduction order, zero for rows
-// >= len), the fp64 value they represent (xs), and per chunk of kDotChunk rows the partial
-// sum of wt * xs (wt = 1 when null) for the rank-one terms.  One block per (chunk, column);
-// grid.y = k + (KP - 3k): the last rows of the grid clear the padding columns.
-__global__ __launch_bounds__(kDotBlock) void quantize3_kernel(
-    const double* __restrict__ x, size_t ldx, size_t len, size_t k,
+// x' = x - mean -> nsl (2 or 3) int8 slices (stored in the permuted reduction order, zero
+// for rows >= len), the fp64 value they represent (xs), and per chunk of kDotChunk rows the
+// partial sum of wt * xs (wt = 1 when null) for the rank-one terms.  One block per (chunk,
+// column); grid.y = k + (KP - nsl*k): the last rows of the grid clear the padding columns.
+__global__ __launch_bounds__(kDotBlock) void quantize_slices_kernel(
+    const double* __restrict__ x, size_t ldx, size_t len, size_t k, int nsl,
     const double* __restrict__ psum, const double* __restrict__ pmax,
     const double* __restrict__ pmin, double inv_len, size_t nchunks, size_t pad_len,
     int8_t* __restrict__ q, double* __restrict__ xs, size_t ldxs,
     const double* __restrict__ wt, double* __restrict__ pdot) {
   const size_t c = blockIdx.y;
   const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, pad_len);
-  if (c >= k) {  // padding column 3k + (c - k)
-    for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) q[(2 * k + c) * pad_len + perm16(i)] = 0;
+  if (c >= k) {  // padding column nsl*k + (c - k)
+    const size_t pc = (size_t)(nsl - 1) * k + c;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) q[pc * pad_len + perm16(i)] = 0;
     return;
   }
   __shared__ double cs_s[4];
@@ -747,16 +753,21 @@ __global__ __launch_bounds__(kDotBlock) void quantize3_kernel(
       t0 = q127(xv / s0);
       double r = fma(-s0, t0, xv);
       t1 = q127(r / s1);
-      r = fma(-s1, t1, r);
-      t2 = q127(r / s2);
-      const double v = s0 * t0 + s1 * t1 + s2 * t2;
+      double v;
+      if (nsl == 3) {
+        r = fma(-s1, t1, r);
+        t2 = q127(r / s2);
+        v = s0 * t0 + s1 * t1 + s2 * t2;
+      } else {
+        v = s0 * t0 + s1 * t1;
+      }
       xs[i + c * ldxs] = v;
       acc = wt ? fma(wt[i], v, acc) : acc + v;
     }
     const size_t pos = perm16(i);
     q[c * pad_len + pos] = (int8_t)t0;
     q[(k + c) * pad_len + pos] = (int8_t)t1;
-    q[(2 * k + c) * pad_len + pos] = (int8_t)t2;
+    if (nsl == 3) q[(2 * k + c) * pad_len + pos] = (int8_t)t2;
   }
   __shared__ double sh[kDotBlock];
   sh[threadIdx.x] = acc;
@@ -910,19 +921,19 @@ __device__ __forceinline__ void i8_tile_sum(const int* __restrict__ part, size_t
   __syncthreads();
 }
 
-// Stage-1 epilogue: U'_mc = sum_j s_jc I_jmc; T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c;
+// Stage-1 epilogue: U'_mc = sum_j s_jc I_jmc over the nsl slices; T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c;
 // W_mc = s_m T_mc (column-major, ldw).  mean_c, the slice scales and cs_c are rebuilt from
 // the partials of the operand pass (column sums psum, maxima pmax, minima pmin, sliced sums
 // pcs).
 template <int KP>
 __global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
-    const int* __restrict__ part, size_t splits, size_t m_pad, size_t m, size_t k,
+    const int* __restrict__ part, size_t splits, size_t m_pad, size_t m, size_t k, int nsl,
     const double* __restrict__ psum, const double* __restrict__ pmax,
     const double* __restrict__ pmin, const double* __restrict__ pcs, double inv_n, size_t nch, const double* __restrict__ sc,
     const double* __restrict__ p2, const double* __restrict__ zs, double* __restrict__ w,
     size_t ldw) {
   __shared__ int tile[64][KP + 1];
-  __shared__ double col[KP / 3 + 1][5];  // mean, s0, s1, s2, cs
+  __shared__ double col[KP / 2 + 1][5];  // mean, s0, s1, s2, cs
   for (size_t c = threadIdx.x; c < k; c += 256) {
     col[c][0] = chunk_sum(psum, c, nch) * inv_n;
     slice_scales(pmax, pmin, col[c][0], c, nch, &col[c][1]);
@@ -935,8 +946,10 @@ __global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
   if (mm >= m) return;
   const double s = sc[mm], pp = p2[mm], z = zs[mm];
   for (size_t c = threadIdx.x >> 6; c < k; c += 4) {
-    const double u = col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c] +
-                     col[c][3] * (double)tile[r][2 * k + c];
+    const double u = nsl == 3 ? col[c][1] * (double)tile[r][c] +
+                                    col[c][2] * (double)tile[r][k + c] +
+                                    col[c][3] * (double)tile[r][2 * k + c]
+                              : col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c];
     const double t = s * (u - pp * col[c][4]) + z * col[c][0];
     w[mm + c * ldw] = s * t;
   }
@@ -946,13 +959,13 @@ __global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
 // nu_c, the slice scales and S2p_c are rebuilt from the partials of the W pass.
 template <int KP>
 __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
-    const int* __restrict__ part, size_t splits, size_t n_pad, size_t n, size_t k,
+    const int* __restrict__ part, size_t splits, size_t n_pad, size_t n, size_t k, int nsl,
     const double* __restrict__ psum, const double* __restrict__ pmax,
     const double* __restrict__ pmin, const double* __restrict__ ps2p, double inv_m, size_t nch, const double* __restrict__ rsum,
     double P2, double* __restrict__ out, size_t ldo, const double* __restrict__ v, size_t ldv,
     const double* __restrict__ wts, double tau0, double tau1) {
   __shared__ int tile[64][KP + 1];
-  __shared__ double col[KP / 3 + 1][5];  // nu, s0, s1, s2, S2p
+  __shared__ double col[KP / 2 + 1][5];  // nu, s0, s1, s2, S2p
   for (size_t c = threadIdx.x; c < k; c += 256) {
     col[c][0] = chunk_sum(psum, c, nch) * inv_m;
     slice_scales(pmax, pmin, col[c][0], c, nch, &col[c][1]);
@@ -965,8 +978,9 @@ __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
   if (i >= n) return;
   const double ri = rsum[i];
   for (size_t c = threadIdx.x >> 6; c < k; c += 4) {
-    double y = col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c] +
-               col[c][3] * (double)tile[r][2 * k + c];
+    double y = nsl == 3 ? col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c] +
+                              col[c][3] * (double)tile[r][2 * k + c]
+                        : col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c];
     y = (y + col[c][0] * (ri - P2) - col[c][4]) * inv_m;
     if (wts) y = tau0 * v[i + c * ldv] / fmax(wts[i], 1e-30) + tau1 * y;
     out[i + c * ldo] = y;
@@ -1026,8 +1040,9 @@ static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const
 }
 
 template <int KT>
-static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
-                           size_t ldo, const double* d_w, const double* tau, size_t c0) {
+static int int8_product_kt(saige_grm* h, int nsl, const double* d_v, size_t ldv, size_t k,
+                           double* d_out, size_t ldo, const double* d_w, const double* tau,
+                           size_t c0) {
   constexpr int KP = 16 * KT;
   constexpr int RT = 4;
   constexpr int NW = KT <= 2 ? 4 : 8;
@@ -1040,7 +1055,7 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
   const size_t n = g.n, m = g.m, n_pad = g.n_pad, m_pad = g.m_pad;
   const size_t nch_n = std::max<size_t>(1, ceil_div(n, kDotChunk));
   const size_t nch_m = std::max<size_t>(1, ceil_div(m, kDotChunk));
-  const unsigned pad_cols = (unsigned)(KP - 3 * k);
+  const unsigned pad_cols = (unsigned)(KP - nsl * k);
   SAIGE_TRY(ms->vq.ensure(KP * n_pad));
   SAIGE_TRY(ms->wq.ensure(KP * m_pad));
   SAIGE_TRY(ms->vs.ensure(n * k * 8));
@@ -1061,16 +1076,17 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
   // ---- V: column sums, maxima and minima, slices + sums of the sliced values ----
   col_stats_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n,
                                                                              psum, pmax, pmin);
-  quantize3_kernel<<<gq_n, kDotBlock, 0, s>>>(d_v, ldv, n, k, psum, pmax, pmin, 1.0 / (double)n,
-                                              nch_n, n_pad, ms->vq.as<int8_t>(),
-                                              ms->vs.as<double>(), n, nullptr, pdot);
+  quantize_slices_kernel<<<gq_n, kDotBlock, 0, s>>>(d_v, ldv, n, k, nsl, psum, pmax, pmin,
+                                                    1.0 / (double)n, nch_n, n_pad,
+                                                    ms->vq.as<int8_t>(), ms->vs.as<double>(), n,
+                                                    nullptr, pdot);
 
   // ---- stage 1: I = G^T Q_v (rows = markers, reduction over samples) ----
   size_t splits1 = 0;
   SAIGE_TRY((i8_gemm<RT, KT, NW, KCH>(h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
                                   n_pad, ms->ipart1, &splits1)));
   i8_stage1_epilogue_kernel<KP><<<(unsigned)(m_pad / 64), 256, 0, s>>>(
-      ms->ipart1.as<int>(), splits1, m_pad, m, k, psum, pmax, pmin, pdot, 1.0 / (double)n, nch_n,
+      ms->ipart1.as<int>(), splits1, m_pad, m, k, nsl, psum, pmax, pmin, pdot, 1.0 / (double)n, nch_n,
       ms->sc.as<double>(), ms->p2.as<double>(), ms->zs.as<double>(), ms->w.as<double>(), m);
   LocoCodes lc;
   if (loco_codes(h, c0, k, lc)) loco_mask(h, ms->w.as<double>(), 1, m, m, k, lc, s);
@@ -1078,16 +1094,17 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
   // ---- W: column sums over real markers, maxima and minima, slices + S2p partials ----
   col_stats_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
       ms->w.as<double>(), m, m, nch_m, psum, pmax, pmin);
-  quantize3_kernel<<<gq_m, kDotBlock, 0, s>>>(ms->w.as<double>(), m, m, k, psum, pmax, pmin,
-                                              1.0 / (double)m, nch_m, m_pad, ms->wq.as<int8_t>(),
-                                              ms->ws_.as<double>(), m, ms->p2.as<double>(), pdot);
+  quantize_slices_kernel<<<gq_m, kDotBlock, 0, s>>>(ms->w.as<double>(), m, m, k, nsl, psum, pmax,
+                                                    pmin, 1.0 / (double)m, nch_m, m_pad,
+                                                    ms->wq.as<int8_t>(), ms->ws_.as<double>(), m,
+                                                    ms->p2.as<double>(), pdot);
 
   // ---- stage 2: I = G Q_w (rows = samples, reduction over markers) ----
   size_t splits2 = 0;
   SAIGE_TRY((i8_gemm<RT, KT, NW, KCH>(h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
                                   m_pad, n_pad, m_pad, ms->ipart2, &splits2)));
   i8_stage2_epilogue_kernel<KP><<<(unsigned)(n_pad / 64), 256, 0, s>>>(
-      ms->ipart2.as<int>(), splits2, n_pad, n, k, psum, pmax, pmin, pdot, 1.0 / (double)m, nch_m,
+      ms->ipart2.as<int>(), splits2, n_pad, n, k, nsl, psum, pmax, pmin, pdot, 1.0 / (double)m, nch_m,
       ms->rsum.as<double>(), ms->P2, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0,
       tau ? tau[1] : 0.0);
   SAIGE_HIP(hipGetLastError());
@@ -1098,21 +1115,23 @@ static int int8_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t k
 int grm_int8_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
                      size_t ldo, const double* d_w, const double* tau) {
   SAIGE_ARG(h->g.n_pad <= kI8MaxRed && h->g.m_pad <= kI8MaxRed,
-            "SAIGE_PREC_INT8 needs n and M <= 8,000,000 (exact int32 accumulation)");
-  constexpr size_t kMaxVec = 16 * kI8MaxKT / 3;  // 37 vectors per launch
-  for (size_t c0 = 0; c0 < k; c0 += kMaxVec) {
-    const size_t kc = std::min(kMaxVec, k - c0);
+            "SAIGE_PREC_INT8 and SAIGE_PREC_INT8X2 need n and M <= 8,000,000 (exact int32 "
+            "accumulation)");
+  const int nsl = h->precision == SAIGE_PREC_INT8X2 ? 2 : 3;
+  const size_t max_vec = 16 * kI8MaxKT / nsl;  // 37 vectors per launch with 3 slices, 56 with 2
+  for (size_t c0 = 0; c0 < k; c0 += max_vec) {
+    const size_t kc = std::min(max_vec, k - c0);
     const double* v = d_v + c0 * ldv;
     double* o = d_out + c0 * ldo;
     int rc;
-    switch (ceil_div(3 * kc, 16)) {
-      case 1: rc = int8_product_kt<1>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 2: rc = int8_product_kt<2>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 3: rc = int8_product_kt<3>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 4: rc = int8_product_kt<4>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 5: rc = int8_product_kt<5>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 6: rc = int8_product_kt<6>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      default: rc = int8_product_kt<7>(h, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+    switch (ceil_div(nsl * kc, 16)) {
+      case 1: rc = int8_product_kt<1>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 2: rc = int8_product_kt<2>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 3: rc = int8_product_kt<3>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 4: rc = int8_product_kt<4>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 5: rc = int8_product_kt<5>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      case 6: rc = int8_product_kt<6>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+      default: rc = int8_product_kt<7>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
     }
     if (rc != SAIGE_OK) return rc;
   }

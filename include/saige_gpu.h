@@ -38,7 +38,7 @@ typedef enum {
   SAIGE_ERR_NO_MARKERS = 6      /* no marker passes QC */
 } saige_status;
 
-enum { SAIGE_PREC_FP64 = 0, SAIGE_PREC_MIXED = 1, SAIGE_PREC_INT8 = 2 };
+enum { SAIGE_PREC_FP64 = 0, SAIGE_PREC_MIXED = 1, SAIGE_PREC_INT8 = 2, SAIGE_PREC_INT8X2 = 3 };
 enum { SAIGE_TRAIT_BINARY = 0, SAIGE_TRAIT_QUANTITATIVE = 1, SAIGE_TRAIT_SURVIVAL = 2 };
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -82,7 +82,9 @@ int saige_plink_decode(int device, const uint8_t* bed_body, size_t bytes_per_mar
  *   precision SAIGE_PREC_INT8: exact genotypes on int8 MFMA with the standardization
  *   factored out, the centred v split into three int8 slices with per-column scales
  *   (|v - split| <= 6.1e-8 max|v - mean|), exact int32 accumulation; n, M <= 8,000,000.
- *   Both reduced-precision modes refuse markers with missing calls.
+ *   precision SAIGE_PREC_INT8X2: as SAIGE_PREC_INT8 with two int8 slices
+ *   (|v - split| <= 1.55e-5 max|v - mean|): 2/3 of the MFMA work.
+ *   The reduced-precision modes refuse markers with missing calls.
  * ---------------------------------------------------------------------------- */
 typedef struct saige_grm saige_grm;
 
@@ -231,7 +233,7 @@ typedef struct {
   double min_maf;            /* 0.01 */
   double max_missing_rate;   /* 0.15 */
   uint32_t n_markers_vr;     /* 30 */
-  int precision;             /* SAIGE_PREC_FP64 | SAIGE_PREC_MIXED | SAIGE_PREC_INT8 */
+  int precision;             /* SAIGE_PREC_FP64 | _MIXED | _INT8 | _INT8X2 */
 } saige_fit_null_config;
 
 typedef struct {

@@ -402,15 +402,20 @@ def test_unrelated_2000_workload_end_to_end():
 # --------------------------------------------------------------------------- mixed precision
 # Stated error bounds of the reduced-precision products against the fp64 oracle, relative
 # to the column norm: bf16 hi+lo with fp32 accumulation 2e-5; three int8 slices with exact
-# int32 accumulation 1e-6 (per-entry operand error <= 6.1e-8 of the column's largest entry).
-_BOUND = {sg.PREC_MIXED: 2e-5, sg.PREC_INT8: 1e-6}
+# int32 accumulation 1e-6 (per-entry operand error <= 6.1e-8 of the column's largest entry);
+# two int8 slices 1e-4 (per-entry operand error <= 1.55e-5 of the column's largest entry, in
+# each of the two operand passes).
+_BOUND = {sg.PREC_MIXED: 2e-5, sg.PREC_INT8: 1e-6, sg.PREC_INT8X2: 1e-4}
+_REDUCED = [sg.PREC_MIXED, sg.PREC_INT8, sg.PREC_INT8X2]
+_REDUCED_IDS = ["bf16", "int8", "int8x2"]
 
 
-@pytest.mark.parametrize("prec", [sg.PREC_MIXED, sg.PREC_INT8], ids=["bf16", "int8"])
+@pytest.mark.parametrize("prec", _REDUCED, ids=_REDUCED_IDS)
 @pytest.mark.parametrize("n,m,k", [(1000, 1300, 1), (2048, 3000, 34), (700, 513, 100)])
 def test_grm_mixed_error_bound(n, m, k, prec):
     """Exact genotypes on MFMA with the standardization factored out: stated error against
-    the fp64 oracle (k = 100 runs the int8 path as three column groups)."""
+    the fp64 oracle (k = 100 runs the three-slice int8 path as three column groups and the
+    two-slice one as two)."""
     _say(f"mixed grm prec={prec} n={n} m={m} k={k}")
     body, bpm, d, af = _grm_pair(n, m, seed=3 * n + m, p_missing=0.0, mono=True)
     g = sg.OnTheFlyGrm(body, bpm, n, af, precision=prec)
@@ -427,20 +432,20 @@ def test_grm_mixed_error_bound(n, m, k, prec):
     # one column alone gives the same values as inside the block (int8: bit for bit,
     # the integer partial sums do not depend on how the work is split)
     one = g.mat_mat(V[k // 2:k // 2 + 1])[0]
-    if prec == sg.PREC_INT8:
+    if prec in (sg.PREC_INT8, sg.PREC_INT8X2):
         np.testing.assert_array_equal(one, got[k // 2])
     else:
         assert np.linalg.norm(one - got[k // 2]) < 1e-5 * np.linalg.norm(one)
 
 
-@pytest.mark.parametrize("prec", [sg.PREC_MIXED, sg.PREC_INT8], ids=["bf16", "int8"])
+@pytest.mark.parametrize("prec", _REDUCED, ids=_REDUCED_IDS)
 def test_grm_mixed_refuses_missing_calls(prec):
     body, bpm, d, af = _grm_pair(300, 200, seed=1, p_missing=0.01)
     with pytest.raises(sg.SaigeError):
         sg.OnTheFlyGrm(body, bpm, 300, af, precision=prec)
 
 
-@pytest.mark.parametrize("prec", [sg.PREC_MIXED, sg.PREC_INT8], ids=["bf16", "int8"])
+@pytest.mark.parametrize("prec", _REDUCED, ids=_REDUCED_IDS)
 def test_fit_null_mixed_vs_fp64_oracle(prec):
     c = synth.unrelated(n=1500, m_grm=3000, m_test=0, seed=12, h2=0.5, prevalence=0.2)
     idx = np.arange(c.n, dtype=np.uint32)

@@ -54,7 +54,8 @@ def _cohort(n=600, m=1800, seed=7):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec,bound", [(sg.PREC_FP64, 1e-12), (sg.PREC_INT8, 1e-6),
-                                        (sg.PREC_MIXED, 2e-5)], ids=["fp64", "int8", "bf16"])
+                                        (sg.PREC_INT8X2, 1e-4), (sg.PREC_MIXED, 2e-5)],
+                         ids=["fp64", "int8", "int8x2", "bf16"])
 def test_gpu_loco_products_vs_oracle(prec, bound):
     c, d, af, chrom = _cohort()
     g = sg.OnTheFlyGrm(c.bed, c.bpm, c.n, af, precision=prec)
