@@ -89,7 +89,8 @@ def time_product(sg, hip, d_bed, bpm, n, markers, af, precision, k, reps):
 def run_loco(args):
     """LOCO (glmm/loco.rs:54-114) on the families cohort: one AI-REML refit per chromosome
     (22 chromosomes of equal marker count), the refits' GRM products joined; the int8 MFMA
-    path timed against the plain FP64 path and against one full-GRM fit; one JSON line."""
+    path (both operand-slice forms) timed against the plain FP64 path and against one full-GRM
+    fit; one JSON line."""
     import csaige_amd as sg
     from csaige_amd import synth
     t0 = time.time()
@@ -99,8 +100,12 @@ def run_loco(args):
     chrom = (np.arange(m) * n_chrom // m).astype(np.int32)
     ac, nv, af, mac = sg.marker_stats(c.bed, c.bpm, n, m)
     log(f"loco: cohort n={n} m={m}, {n_chrom} chromosomes ({time.time() - t0:.1f}s)")
+    forms = {"int8": sg.PREC_INT8, "int8x2": sg.PREC_INT8X2, "fp64": sg.PREC_FP64}
+    head = args.precision
+    other = "int8x2" if head == "int8" else "int8"
     out = {}
-    for prec in [sg.PREC_INT8] + ([] if args.no_fp64 else [sg.PREC_FP64]):
+    for tag in [head, other] + ([] if args.no_fp64 else ["fp64"]):
+        prec = forms[tag]
         g = sg.OnTheFlyGrm(c.bed, c.bpm, n, af, precision=prec)
         g.set_marker_chromosomes(chrom)
         sg.fit_ai_reml(g, c.y, c.x, tol=1e-5)  # warm-up: kernels, device memory cache
@@ -110,29 +115,34 @@ def run_loco(args):
         t = time.perf_counter()
         r = g.fit_ai_reml_loco(c.y, c.x, tol=1e-5)
         dt = time.perf_counter() - t
-        tag = "int8" if prec == sg.PREC_INT8 else "fp64"
         log(f"loco {tag}: {n_chrom} refits in {dt:.2f}s ({dt / n_chrom:.3f}s per refit); one "
             f"full-GRM fit {dt_full:.3f}s; tau_g range {min(x['tau'][1] for x in r):.6g}.."
             f"{max(x['tau'][1] for x in r):.6g}, full {full['tau'][1]:.6g}")
         out[tag] = (dt, dt_full, r)
         g.close()
-    dt8, full8, r8 = out["int8"]
+    dt8, full8, r8 = out[head]
     acc = None
     if "fp64" in out:
         rf = out["fp64"][2]
-        acc = {"max_tau_rel_diff": float(max(abs(a["tau"][1] - b["tau"][1]) / b["tau"][1]
-                                             for a, b in zip(r8, rf))),
-               "iterations_equal": bool(all(a["iterations"] == b["iterations"]
-                                            for a, b in zip(r8, rf)))}
-        log(f"loco accuracy int8 vs fp64: {acc}")
+        acc = {}
+        for tag in (head, other):
+            ri = out[tag][2]
+            acc[tag] = {"max_tau_rel_diff": float(max(abs(a["tau"][1] - b["tau"][1]) / b["tau"][1]
+                                                      for a, b in zip(ri, rf))),
+                        "iterations_equal": bool(all(a["iterations"] == b["iterations"]
+                                                     for a, b in zip(ri, rf)))}
+        log(f"loco accuracy vs fp64: {acc}")
     line = {
         "metric": "LOCO refits wall-s, 1 GPU: int8 mixed vs FP64",
         "value": dt8, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": dt8 * 1e3, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int8 MFMA / int32 accumulate / fp64 solver",
+        "vs_baseline": None,
+        "dtype": f"int8 MFMA ({'two' if head == 'int8x2' else 'three'} operand slices) / int32 "
+                 "accumulate / fp64 solver",
         "data": "synthetic (seeded families cohort)",
         "config": {"workload": f"LOCO AI-REML refits, {args.n_fam} nuclear families x 5 = {n} "
                                f"samples, {m} markers in {n_chrom} chromosomes, binary 5%",
+                   "precision": head, f"{other}_wall_s": out[other][0],
                    "per_refit_s": dt8 / n_chrom, "full_grm_fit_s": full8,
                    "fp64_wall_s": out["fp64"][0] if "fp64" in out else None,
                    "speedup_vs_fp64": out["fp64"][0] / dt8 if "fp64" in out else None,
@@ -210,8 +220,8 @@ def run_region(args):
 def run_biobank(args):
     """Several phenotypes at biobank scale: 400,000 samples (15% in half-sibling pairs),
     150,000 markers resident in HBM, 4 binary traits at 0.5/1/5/20% prevalence fitted
-    together (joined multi-column GRM products, per-trait W), int8 MFMA path timed against
-    the plain FP64 path; one JSON line."""
+    together (joined multi-column GRM products, per-trait W), int8 MFMA path (both
+    operand-slice forms) timed against the plain FP64 path; one JSON line."""
     import csaige_amd as sg
     from csaige_amd import hip, synth
     t0 = time.time()
@@ -235,49 +245,63 @@ def run_biobank(args):
         hip.device_synchronize()
         return time.perf_counter() - t, r
 
-    dt8, r8 = fit(sg.PREC_INT8)
+    forms = {"int8": sg.PREC_INT8, "int8x2": sg.PREC_INT8X2}
+    head = args.precision
+    other = "int8x2" if head == "int8" else "int8"
+    dt8, r8 = fit(forms[head])
     for t, r in enumerate(r8):
-        log(f"int8 joined: trait {t} tau={r['tau'][1]:.8g} it={r['iterations']} vr={r['vr']:.6g} "
-            f"pcg={r['pcg_iterations']} products={r['grm_products']}")
-    log(f"int8 joined fit of {len(prev)} traits: {dt8:.2f}s")
+        log(f"{head} joined: trait {t} tau={r['tau'][1]:.8g} it={r['iterations']} "
+            f"vr={r['vr']:.6g} pcg={r['pcg_iterations']} products={r['grm_products']}")
+    log(f"{head} joined fit of {len(prev)} traits: {dt8:.2f}s")
+    dto, ro = fit(forms[other])
+    log(f"{other} joined fit of {len(prev)} traits: {dto:.2f}s")
     sep = None
     if args.bb_separate:
         sep = 0.0
         for t in range(len(prev)):
-            dt, _ = fit(sg.PREC_INT8, t)
+            dt, _ = fit(forms[head], t)
             sep += dt
-        log(f"int8 one trait at a time: {sep:.2f}s")
+        log(f"{head} one trait at a time: {sep:.2f}s")
     dtf, rf = (None, None)
     acc = None
     if not args.no_fp64:
         dtf, rf = fit(sg.PREC_FP64)
         log(f"fp64 joined fit: {dtf:.2f}s")
-        acc = []
+        acc = {head: [], other: []}
         ntest = 2048
+
+        def pvals(r, t):
+            e = sg.ScoreTestEngine(r["mu"], c.y[t] - r["mu"], r["tau"], c.x, r["xvx_inv_xv"],
+                                   r["vr"], y=c.y[t])
+            p = e.test_markers_plink(c.bed[:ntest], c.bpm, n, ntest)["pvalue"]
+            e.close()
+            return p
+
         for t in range(len(prev)):
-            ps = []
-            for r in (rf[t], r8[t]):
-                e = sg.ScoreTestEngine(r["mu"], c.y[t] - r["mu"], r["tau"], c.x, r["xvx_inv_xv"],
-                                       r["vr"], y=c.y[t])
-                ps.append(e.test_markers_plink(c.bed[:ntest], c.bpm, n, ntest)["pvalue"])
-                e.close()
-            mask = ps[0] >= 1e-10
-            acc.append({"tau_rel_diff": float(abs(r8[t]["tau"][1] - rf[t]["tau"][1]) / rf[t]["tau"][1]),
-                        "vr_rel_diff": float(abs(r8[t]["vr"] - rf[t]["vr"]) / rf[t]["vr"]),
-                        "max_abs_dlog10p": float(np.max(np.abs(np.log10(ps[0][mask]) -
-                                                               np.log10(ps[1][mask]))))})
-        log(f"accuracy int8 vs fp64: {acc}")
+            pf = pvals(rf[t], t)
+            mask = pf >= 1e-10
+            for tag, rr in ((head, r8), (other, ro)):
+                pm = pvals(rr[t], t)
+                acc[tag].append({
+                    "tau_rel_diff": float(abs(rr[t]["tau"][1] - rf[t]["tau"][1]) / rf[t]["tau"][1]),
+                    "vr_rel_diff": float(abs(rr[t]["vr"] - rf[t]["vr"]) / rf[t]["vr"]),
+                    "max_abs_dlog10p": float(np.max(np.abs(np.log10(pf[mask]) -
+                                                           np.log10(pm[mask]))))})
+        log(f"accuracy vs fp64: {acc}")
     line = {
         "metric": "fit-null wall-s, several phenotypes, 1 GPU: int8 mixed vs FP64",
         "value": dt8, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": dt8 * 1e3, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int8 MFMA / int32 accumulate / fp64 solver",
+        "vs_baseline": None,
+        "dtype": f"int8 MFMA ({'two' if head == 'int8x2' else 'three'} operand slices) / int32 "
+                 "accumulate / fp64 solver",
         "data": "synthetic (seeded biobank cohort; packed genotypes resident in HBM)",
         "config": {"workload": f"fit-null of {len(prev)} binary traits (prevalence "
                                f"{'/'.join(str(p) for p in prev)}), {n} samples (15% in half-sib "
                                f"pairs), {m} markers, 30 probes per trait, joined products",
+                   "precision": head, f"{other}_wall_s": dto,
                    "fp64_wall_s": dtf, "speedup_vs_fp64": (dtf / dt8) if dtf else None,
-                   "int8_one_trait_at_a_time_s": sep,
+                   f"{head}_one_trait_at_a_time_s": sep,
                    "tau_g": [float(r["tau"][1]) for r in r8],
                    "ai_reml_iterations": [int(r["iterations"]) for r in r8],
                    "pcg_iterations": [int(r["pcg_iterations"]) for r in r8],
