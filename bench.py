@@ -518,12 +518,25 @@ def main():
         one_fit(sg.PREC_MIXED)
         if not args.no_fp64:
             one_fit(sg.PREC_FP64)
+    # the dominant kernel inside the timed region: every genotype GEMM launch of the timed
+    # fits is timed with HIP events on the stream it runs on (the library's launch log); the
+    # 34-column block solves (y, 3 covariates, 30 probes) run at 16 * ceil(nsl * 34 / 16)
+    # operand columns
+    L = sg.load()
+    fit_width = 16 * -(-nsl * 34 // 16)
+    sg.check(L.saige_launch_log_enable(1, fit_width))
     times, rm = [], None
     for s in range(args.steps):
         dt, rm = one_fit(int8_forms[head])
         times.append(dt)
         describe(f"step {s} mixed {head}", dt, rm)
     mixed_s = sum(times) / len(times)
+    lt, ln = C.c_double(), C.c_uint64()
+    sg.check(L.saige_launch_log_read(fit_width, C.byref(lt), C.byref(ln)))
+    sg.check(L.saige_launch_log_enable(0, 0))
+    fit_launch_ms, fit_launches = lt.value / max(ln.value, 1), int(ln.value)
+    log(f"{head} GEMM launches at {fit_width} operand columns in the timed fits: "
+        f"{fit_launches}, {fit_launch_ms:.4f} ms each")
     other_s, ro = one_fit(int8_forms[other])
     describe(f"mixed {other}", other_s, ro)
     bf16_s, rb = one_fit(sg.PREC_MIXED)
@@ -575,14 +588,15 @@ def main():
             f"(useful), {2.0 * ((n + 3) // 4) * mg / (ms * 1e-3) / 1e9:.1f} GB/s packed G; "
             f"GEMM kernel {kms:.3f} ms/launch over {kl} launches")
     # the dominant kernel: one genotype GEMM launch = one of the two passes of a product,
-    # 2 * n * M * k useful flop (exact genotypes times the k-column operand)
+    # 2 * n * M * k useful flop (exact genotypes times the k-column operand), timed over the
+    # launches of the timed fits
     launch_flops = 2.0 * n * mg * k
-    mixed_tf = launch_flops / (kern[head] * 1e-3) / 1e12
+    mixed_tf = launch_flops / (fit_launch_ms * 1e-3) / 1e12
     # algorithmic bytes of a launch: one pass over the packed genotypes, ceil(n/4) * M; at
     # 2 * k = 68 useful ops per genotype (272 per byte, under the int8 ridge of 625) the
     # launch is bound by HBM on the algorithmic roofline
     launch_bytes = float((n + 3) // 4) * mg
-    mixed_gbs = launch_bytes / (kern[head] * 1e-3) / 1e9
+    mixed_gbs = launch_bytes / (fit_launch_ms * 1e-3) / 1e9
 
     # ---- step 2: streamed score test + SPA (n = 400,000, 4,096-variant uint8 batches) ----
     step2 = None
@@ -711,13 +725,17 @@ def main():
                    "variance_ratio": float(rm["vr"]),
                    "grm_column_products": int(rm["grm_products"]),
                    "pcg_iterations": int(rm["pcg_iterations"]), "accuracy": acc,
-                   "product_ms": prod, "gemm_launch_ms": kern},
+                   "product_ms": prod, "gemm_launch_ms": kern,
+                   "fit_gemm_launch_ms": fit_launch_ms, "fit_gemm_launches": fit_launches},
         "roofline": {"bound": "hbm", "achieved": mixed_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": mixed_gbs / HBM_PEAK_GBS,
                      "traffic": (args.traffic_bytes if args.traffic_bytes is not None else
                                  MEASURED_TRAFFIC.get((args.n_fam, args.markers, head))),
                      "kernel": f"i8_gemm_kernel (int8 MFMA genotype GEMM, {head}), k={k}: "
-                               f"{kern[head]:.3f} ms/launch (HIP events on the handle stream), "
+                               f"{fit_launch_ms:.4f} ms/launch over the {fit_launches} launches "
+                               f"at {fit_width} operand columns in the timed fits (HIP events "
+                               f"on the stream of the fit's GRM handle; "
+                               f"{kern[head]:.3f} ms/launch on dense random operands), "
                                f"algorithmic bytes ceil(n/4)*M = {launch_bytes:.4g} B/launch "
                                f"and 2*n*M*k = {launch_flops:.3g} op/launch "
                                f"({mixed_tf:.0f} TOP/s useful = {mixed_tf / I8_PEAK_TOPS:.3f} of "

@@ -150,11 +150,23 @@ struct KernelTimer {
   }
 };
 
-// Launch a kernel on stream s, timed by t when its timing is on.
+// Process-wide launch log (diagnostic): while enabled, every GEMM-class launch of every
+// handle (also the handles that the fit entry points create internally) gets a
+// dispatch-stamped event pair and is filed under its operand width, the padded number of
+// dense columns it multiplies.  Thread-safe; at most kMaxLaunches records between reads.
+bool launch_log_on();
+// Event pair for a launch of the given width on the current device (null when the log is
+// off or full).
+int launch_log_pair(uint32_t width, hipEvent_t* a, hipEvent_t* b);
+
+// Launch a kernel on stream s, timed by t when its timing is on, else by the launch log
+// when that is on.  width is the launch's operand width for the launch log.
 template <class F, class... Args>
-int timed_launch(KernelTimer& t, F kernel, dim3 grid, dim3 block, hipStream_t s, Args... args) {
+int timed_launch(KernelTimer& t, uint32_t width, F kernel, dim3 grid, dim3 block,
+                 hipStream_t s, Args... args) {
   hipEvent_t a, b;
   SAIGE_TRY(t.pair(&a, &b));
+  if (!a && launch_log_on()) SAIGE_TRY(launch_log_pair(width, &a, &b));
   hipExtLaunchKernelGGL(kernel, grid, block, 0, s, a, b, 0, args...);
   SAIGE_HIP(hipGetLastError());
   return SAIGE_OK;

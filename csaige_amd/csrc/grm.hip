@@ -18,10 +18,101 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace saige {
+
+// ---------------------------------------------------------------------------------
+// Launch log (see launch_log_pair in common.hpp).
+// ---------------------------------------------------------------------------------
+namespace {
+struct LaunchLog {
+  static constexpr size_t kMaxLaunches = 65536;
+  struct Rec {
+    hipEvent_t a, b;
+    uint32_t width;
+    int device;
+  };
+  struct FreeEvent {
+    hipEvent_t e;
+    int device;
+  };
+  std::mutex mu;
+  std::atomic<bool> on{false};
+  uint32_t only_width = 0;      // record only launches of this width (0: every width)
+  std::vector<Rec> recs;        // launches recorded since the last read
+  std::vector<FreeEvent> pool;  // events free for reuse, per device
+};
+LaunchLog& launch_log() {
+  static LaunchLog* l = new LaunchLog;  // never destroyed: events outlive static teardown
+  return *l;
+}
+int take_event(LaunchLog& L, int dev, hipEvent_t* e) {
+  for (size_t i = 0; i < L.pool.size(); ++i)
+    if (L.pool[i].device == dev) {
+      *e = L.pool[i].e;
+      L.pool.erase(L.pool.begin() + (ptrdiff_t)i);
+      return SAIGE_OK;
+    }
+  SAIGE_HIP(hipEventCreate(e));
+  return SAIGE_OK;
+}
+// Waits for the recorded launches, sums the durations of those of one width (0: all) when
+// total_ms is given, and returns every event to the pool.
+int drain_launch_log(LaunchLog& L, uint32_t width, double* total_ms, uint64_t* launches) {
+  double t = 0.0;
+  uint64_t cnt = 0;
+  int cur = 0, rc = SAIGE_OK;
+  SAIGE_HIP(hipGetDevice(&cur));
+  for (const LaunchLog::Rec& r : L.recs) {
+    // the pair is reused only after its launch has ended
+    const bool done = hipSetDevice(r.device) == hipSuccess && hipEventSynchronize(r.b) == hipSuccess;
+    if (total_ms && rc == SAIGE_OK && (width == 0 || r.width == width)) {
+      float ms = 0.f;
+      if (!done || hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) {
+        set_error("launch log: a recorded launch could not be timed");
+        rc = SAIGE_ERR_DEVICE;
+      }
+      t += ms;
+      ++cnt;
+    }
+    L.pool.push_back({r.a, r.device});
+    L.pool.push_back({r.b, r.device});
+  }
+  L.recs.clear();
+  (void)hipSetDevice(cur);
+  if (total_ms) *total_ms = t;
+  if (launches) *launches = cnt;
+  return rc;
+}
+}  // namespace
+
+bool launch_log_on() { return launch_log().on.load(std::memory_order_relaxed); }
+
+int launch_log_pair(uint32_t width, hipEvent_t* a, hipEvent_t* b) {
+  *a = *b = nullptr;
+  LaunchLog& L = launch_log();
+  std::lock_guard<std::mutex> lk(L.mu);
+  if (!L.on.load(std::memory_order_relaxed) || L.recs.size() >= LaunchLog::kMaxLaunches ||
+      (L.only_width != 0 && width != L.only_width))
+    return SAIGE_OK;
+  int dev = 0;
+  SAIGE_HIP(hipGetDevice(&dev));
+  hipEvent_t e0, e1;
+  SAIGE_TRY(take_event(L, dev, &e0));
+  const int rc = take_event(L, dev, &e1);
+  if (rc != SAIGE_OK) {
+    L.pool.push_back({e0, dev});
+    return rc;
+  }
+  L.recs.push_back({e0, e1, width, dev});
+  *a = e0;
+  *b = e1;
+  return SAIGE_OK;
+}
 
 int upload_repack(hipStream_t s, const uint8_t* bed_body, size_t bpm, size_t n_total,
                   const uint32_t* d_idx, size_t n, const uint32_t* marker_idx, size_t m,
@@ -209,14 +300,14 @@ static int product_k(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
 
   transpose_v_kernel<K><<<(unsigned)ceil_div(g.n_pad, 256), 256, 0, s>>>(
       d_v, ldv, g.n, k, h->ws_vt.as<double>(), g.n_pad);
-  SAIGE_TRY(timed_launch(h->timer, grm_s1_f64<K>, dim3((unsigned)nblk_m, (unsigned)splits1),
+  SAIGE_TRY(timed_launch(h->timer, (uint32_t)K, grm_s1_f64<K>, dim3((unsigned)nblk_m, (unsigned)splits1),
                          dim3(256), s, g.geno, g.row_bytes, g.lut, h->ws_vt.as<double>(), bps,
                          h->ws_tpart.as<double>(), g.m_pad));
   grm_s1_reduce<K><<<(unsigned)ceil_div(g.m_pad * K, 256), 256, 0, s>>>(
       h->ws_tpart.as<double>(), splits1, g.m_pad, h->ws_t.as<double>());
   LocoCodes lc;
   if (loco_codes(h, c0, k, lc)) loco_mask(h, h->ws_t.as<double>(), K, 1, g.m, k, lc, s);
-  SAIGE_TRY(timed_launch(h->timer, grm_s2_f64<K>, dim3((unsigned)nblk_n, (unsigned)splits2),
+  SAIGE_TRY(timed_launch(h->timer, (uint32_t)K, grm_s2_f64<K>, dim3((unsigned)nblk_n, (unsigned)splits2),
                          dim3(256), s, g.geno, g.row_bytes, g.lut, h->ws_t.as<double>(), mps,
                          g.m, h->ws_ypart.as<double>(), g.n_pad));
   grm_s2_reduce<K><<<dim3((unsigned)ceil_div(g.n, 256), (unsigned)k), 256, 0, s>>>(
@@ -486,6 +577,26 @@ int grm_create_impl(int device, const uint8_t* bed_body, bool on_device, size_t 
 }  // namespace saige
 
 extern "C" {
+
+int saige_launch_log_enable(int enable, uint32_t width) {
+  using namespace saige;
+  clear_error();
+  LaunchLog& L = launch_log();
+  std::lock_guard<std::mutex> lk(L.mu);
+  SAIGE_TRY(drain_launch_log(L, 0, nullptr, nullptr));
+  L.only_width = width;
+  L.on.store(enable != 0, std::memory_order_relaxed);
+  return SAIGE_OK;
+}
+
+int saige_launch_log_read(uint32_t width, double* total_ms, uint64_t* launches) {
+  using namespace saige;
+  clear_error();
+  SAIGE_ARG(total_ms && launches, "total_ms and launches");
+  LaunchLog& L = launch_log();
+  std::lock_guard<std::mutex> lk(L.mu);
+  return drain_launch_log(L, width, total_ms, launches);
+}
 
 int saige_grm_create(int device, const uint8_t* bed_body, size_t bpm, size_t n_total,
                      const uint32_t* sample_idx, size_t n, const uint32_t* marker_idx, size_t m,

@@ -557,7 +557,7 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   size_t rps1 = round_up(ceil_div(n_pad, splits1), kKC);
   splits1 = ceil_div(n_pad, rps1);
   SAIGE_TRY(ms->part1.ensure(splits1 * m_pad * KP * 4));
-  SAIGE_TRY(timed_launch(h->timer, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk1, (unsigned)splits1),
+  SAIGE_TRY(timed_launch(h->timer, (uint32_t)KP, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk1, (unsigned)splits1),
                          dim3(256), s, g.geno, g.row_bytes, ms->vhi.as<__bf16>(),
                          ms->vlo.as<__bf16>(), n_pad, rps1, n_pad, ms->part1.as<float>(), m_pad));
   stage1_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(m, 256), (unsigned)k), 256, 0, s>>>(
@@ -583,7 +583,7 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   size_t rps2 = round_up(ceil_div(m_pad, splits2), kKC);
   splits2 = ceil_div(m_pad, rps2);
   SAIGE_TRY(ms->part2.ensure(splits2 * n_pad * KP * 4));
-  SAIGE_TRY(timed_launch(h->timer, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk2, (unsigned)splits2),
+  SAIGE_TRY(timed_launch(h->timer, (uint32_t)KP, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk2, (unsigned)splits2),
                          dim3(256), s, ms->geno_t.as<uint8_t>(), ms->rbt, ms->whi.as<__bf16>(),
                          ms->wlo.as<__bf16>(), m_pad, rps2, m_pad, ms->part2.as<float>(), n_pad));
   stage2_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(
@@ -1032,7 +1032,7 @@ static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const
   const size_t rps = ceil_div(nch, splits) * KCH;
   const size_t sp = ceil_div(red_len, rps);
   SAIGE_TRY(part.ensure(sp * rows_pad * KP * 4));
-  SAIGE_TRY(timed_launch(h->timer, i8_gemm_kernel<RT, KT, NW, KCH>, dim3((unsigned)rblk, (unsigned)sp),
+  SAIGE_TRY(timed_launch(h->timer, (uint32_t)KP, i8_gemm_kernel<RT, KT, NW, KCH>, dim3((unsigned)rblk, (unsigned)sp),
                          dim3(NW * 64), s, codes, code_stride, bq, b_stride, rps, red_len,
                          part.as<int>(), rows_pad));
   *splits_out = sp;

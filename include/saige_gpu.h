@@ -119,6 +119,21 @@ int saige_grm_synchronize(saige_grm* h);
  * At most 4096 launches are recorded between reads. */
 int saige_grm_set_kernel_timing(saige_grm* h, int enable);
 int saige_grm_kernel_timing(saige_grm* h, double* total_ms, uint64_t* launches);
+/* Launch log (diagnostic, no counterpart in saige-rs): while enabled, every genotype GEMM
+ * launch of every GRM handle in the process, including the handles that the fit-null, LOCO
+ * and several-phenotype entry points create internally, is timed with HIP events on its
+ * stream and filed under its operand width: the padded number of dense columns it
+ * multiplies (int8: 16 * ceil(S * k / 16) slice columns for S operand slices; bf16:
+ * 16 * ceil(k / 16); fp64: the column block of the launch).  A handle whose own kernel
+ * timing is on records there instead.  saige_launch_log_enable clears the log and records
+ * only the launches of the given width from then on (0: every width; only recorded launches
+ * pay for the event pair).
+ * saige_launch_log_read waits for the recorded launches, returns the summed time and the
+ * count of the launches of one width (0: every width) and clears the log; call it when no
+ * launch is being issued on another thread.  At most 65536 launches are kept between
+ * reads. */
+int saige_launch_log_enable(int enable, uint32_t width);
+int saige_launch_log_read(uint32_t width, double* total_ms, uint64_t* launches);
 
 /* ------------------------------------------------------------------------------
  * Preconditioned conjugate gradient on Sigma = tau_e * diag(1/W) + tau_g * GRM.

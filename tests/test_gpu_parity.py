@@ -490,3 +490,42 @@ def test_fit_null_multi_matches_single_trait(prec):
         want = oracle.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], idx, Y[0], c.x)
         assert rel(multi[0]["tau"], want["tau"]).max() < 1e-6
         assert abs(multi[0]["vr"] - want["vr"]) < 1e-6 * want["vr"]
+
+
+# --------------------------------------------------------------------------- launch log
+def test_launch_log_files_launches_by_operand_width():
+    """The process-wide launch log times the GEMM launches of a product under their operand
+    width (k = 34 with two int8 slices: 80 columns; two launches per product) and also sees
+    the launches of the GRM handle that a fit creates internally; a width filter keeps only
+    the launches of that width."""
+    import ctypes as C
+    L = sg.load()
+    body, bpm, d, af = _grm_pair(1200, 700, seed=5, p_missing=0.0)
+    g = sg.OnTheFlyGrm(body, bpm, 1200, af, precision=sg.PREC_INT8X2)
+    V = np.random.default_rng(2).standard_normal((34, 1200))
+    g.mat_mat(V)  # warm-up outside the log
+    ms, cnt = C.c_double(), C.c_uint64()
+    sg.check(L.saige_launch_log_enable(1, 0))
+    try:
+        g.mat_mat(V)
+        g.mat_mat(V[:3])  # 6 slice columns: width 16
+        sg.check(L.saige_launch_log_read(80, C.byref(ms), C.byref(cnt)))
+        assert cnt.value == 2 and ms.value > 0.0
+        sg.check(L.saige_launch_log_read(0, C.byref(ms), C.byref(cnt)))
+        assert cnt.value == 0  # the read cleared the log, including the other width
+        c = synth.unrelated(n=600, m_grm=800, m_test=0, seed=3, h2=0.5, prevalence=0.2)
+        idx = np.arange(c.n, dtype=np.uint32)
+        r = sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], c.y, c.x, sample_idx=idx,
+                        precision=sg.PREC_INT8X2)
+        sg.check(L.saige_launch_log_read(0, C.byref(ms), C.byref(cnt)))
+        _say(f"launch log: fit-null {cnt.value} GEMM launches, {ms.value:.2f} ms, "
+             f"{r['grm_products']} column-products")
+        assert cnt.value >= 2 and ms.value > 0.0
+        sg.check(L.saige_launch_log_enable(1, 16))  # narrow launches only
+        g.mat_mat(V)
+        g.mat_mat(V[:3])
+        sg.check(L.saige_launch_log_read(0, C.byref(ms), C.byref(cnt)))
+        assert cnt.value == 2
+    finally:
+        sg.check(L.saige_launch_log_enable(0, 0))
+        g.close()
