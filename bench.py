@@ -591,6 +591,10 @@ def main():
     # 2 * n * M * k useful flop (exact genotypes times the k-column operand), timed over the
     # launches of the timed fits
     launch_flops = 2.0 * n * mg * k
+    if fit_launches == 0:  # no block solve at that width (another covariate count)
+        fit_launch_ms = kern[head]
+        log(f"no {fit_width}-column launches in the timed fits: the roofline uses the "
+            f"product loop's {kern[head]:.4f} ms/launch")
     mixed_tf = launch_flops / (fit_launch_ms * 1e-3) / 1e12
     # algorithmic bytes of a launch: one pass over the packed genotypes, ceil(n/4) * M; at
     # 2 * k = 68 useful ops per genotype (272 per byte, under the int8 ridge of 625) the
@@ -735,7 +739,8 @@ def main():
                                f"{fit_launch_ms:.4f} ms/launch over the {fit_launches} launches "
                                f"at {fit_width} operand columns in the timed fits (HIP events "
                                f"on the stream of the fit's GRM handle; "
-                               f"{kern[head]:.3f} ms/launch on dense random operands), "
+                               f"{kern[head]:.3f} ms/launch on dense random operands; with no "
+                               f"such launch, the random-operand figure), "
                                f"algorithmic bytes ceil(n/4)*M = {launch_bytes:.4g} B/launch "
                                f"and 2*n*M*k = {launch_flops:.3g} op/launch "
                                f"({mixed_tf:.0f} TOP/s useful = {mixed_tf / I8_PEAK_TOPS:.3f} of "
