@@ -460,7 +460,8 @@ def test_fit_null_mixed_vs_fp64_oracle(prec):
 
 
 # --------------------------------------------------------------------------- several phenotypes
-@pytest.mark.parametrize("prec", [sg.PREC_FP64, sg.PREC_INT8], ids=["fp64", "int8"])
+@pytest.mark.parametrize("prec", [sg.PREC_FP64, sg.PREC_INT8, sg.PREC_INT8X2],
+                         ids=["fp64", "int8", "int8x2"])
 def test_fit_null_multi_matches_single_trait(prec):
     """Three phenotypes fitted together (joined GRM products) follow their single-trait
     fits: same AI-REML and PCG iteration counts, same tau / VR / mu to 1e-10."""
