@@ -21,12 +21,12 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (PREC_FP64, PREC_MIXED, PREC_INT8, PREC_INT8X2, TRAIT_BINARY, TRAIT_QUANTITATIVE, TRAIT_SURVIVAL,
+from ._lib import (PREC_FP64, PREC_MIXED, PREC_INT8, PREC_INT8X2, PREC_F32, TRAIT_BINARY, TRAIT_QUANTITATIVE, TRAIT_SURVIVAL,
                    SaigeError, check, load)
 
 __all__ = ["OnTheFlyGrm", "fit_ai_reml", "solve_sigma", "xvx_inv_xv", "estimate_variance_ratio",
            "fit_null", "fit_null_multi", "ScoreTestEngine", "spa_binary_fast", "marker_stats", "decode",
-           "PREC_FP64", "PREC_MIXED", "PREC_INT8", "PREC_INT8X2", "TRAIT_BINARY", "TRAIT_QUANTITATIVE", "TRAIT_SURVIVAL",
+           "PREC_FP64", "PREC_MIXED", "PREC_INT8", "PREC_INT8X2", "PREC_F32", "TRAIT_BINARY", "TRAIT_QUANTITATIVE", "TRAIT_SURVIVAL",
            "SaigeError", "load", "results_to_dict", "SpaRootMlp", "spa_training_set"]
 
 

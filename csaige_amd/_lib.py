@@ -17,6 +17,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_INT8 = 2
 PREC_INT8X2 = 3
+PREC_F32 = 4
 TRAIT_BINARY = 0
 TRAIT_QUANTITATIVE = 1
 TRAIT_SURVIVAL = 2

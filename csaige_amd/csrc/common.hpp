@@ -207,10 +207,13 @@ namespace saige {
 // the products of several concurrent fits (phenotypes sharing the GRM) into one
 // multi-column product.  out = GRM v, or Sigma v = tau0 v / max(w, 1e-30) + tau1 GRM v when
 // w is non-null.  `columns` counts the column-products this sink has served.
+// `slices` asks the int8 forms for that many operand slices on these columns (3: the
+// three-slice operand even on a SAIGE_PREC_INT8X2 handle; 0: the handle's own form); the
+// other precisions ignore it.  A column's result depends only on its own slice count.
 struct ProductSink {
   uint64_t columns = 0;
   virtual int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo,
-                      const double* w, const double* tau) = 0;
+                      const double* w, const double* tau, int slices = 0) = 0;
   virtual ~ProductSink() = default;
 };
 
@@ -228,7 +231,7 @@ struct saige_grm {
   hipStream_t stream = nullptr;
   std::recursive_mutex mu;
   saige::GrmDevice g;
-  saige::DevBuf geno, lut, scale, shift, scale64, shift64, geno_t, miss_idx, miss_ptr;
+  saige::DevBuf geno, lut, lut32, scale, shift, scale64, shift64, geno_t, miss_idx, miss_ptr;
   // reduced precision: per-marker 1/sd and 2p/sd (m_pad entries, 0 past m) on the host
   std::vector<double> host_scale, host_shift;
   // workspaces for products and solves
@@ -246,14 +249,17 @@ struct saige_grm {
   saige::DevBuf loco_chrom, loco_scale;
   int loco_n_chrom = 0;
   const int* loco_req = nullptr;
+  // Per column of the product call in progress: operand slices asked for (0: the handle's
+  // own form); null for none.  Set around one product call, like loco_req.
+  const uint8_t* slice_req = nullptr;
 };
 
 namespace saige {
 
 // Columns of one product launch and the chromosome each leaves out of its GRM (-1: none).
-constexpr int kLocoMaxCols = 128;
+constexpr int kLocoMaxCols = 256;
 struct LocoCodes {
-  int code[kLocoMaxCols];
+  int16_t code[kLocoMaxCols];
 };
 // Fills lc from h->loco_req[c0 .. c0 + kc); false when the product has no LOCO columns.
 bool loco_codes(const saige_grm* h, size_t c0, size_t kc, LocoCodes& lc);
@@ -289,13 +295,15 @@ struct DirectSink : ProductSink {
   int loco = -1;
   explicit DirectSink(saige_grm* hh) : h(hh) {}
   int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
-              const double* tau) override {
+              const double* tau, int slices = 0) override {
     columns += k;
-    if (loco < 0) return grm_product(h, v, ldv, k, out, ldo, w, tau);
-    std::vector<int> codes(k, loco);
-    h->loco_req = codes.data();
+    std::vector<int> codes(loco < 0 ? 0 : k, loco);
+    std::vector<uint8_t> sl(slices ? k : 0, (uint8_t)slices);
+    h->loco_req = loco < 0 ? nullptr : codes.data();
+    h->slice_req = slices ? sl.data() : nullptr;
     const int rc = grm_product(h, v, ldv, k, out, ldo, w, tau);
     h->loco_req = nullptr;
+    h->slice_req = nullptr;
     return rc;
   }
 };
@@ -304,6 +312,7 @@ struct DirectSink : ProductSink {
 // through `sink`, vectors in `ws`.
 int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double* d_w,
                      const double tau[2], const double* d_b, size_t ldb, size_t k, double tol,
-                     uint32_t max_iter, double* d_x, size_t ldx, saige_pcg_info* info);
+                     uint32_t max_iter, double* d_x, size_t ldx, saige_pcg_info* info,
+                     int slices = 0);
 
 }  // namespace saige

@@ -265,7 +265,7 @@ bool loco_codes(const saige_grm* h, size_t c0, size_t kc, LocoCodes& lc) {
   if (!h->loco_req || h->loco_n_chrom == 0) return false;
   bool any = false;
   for (size_t c = 0; c < kc; ++c) {
-    lc.code[c] = h->loco_req[c0 + c];
+    lc.code[c] = (int16_t)h->loco_req[c0 + c];
     any = any || lc.code[c] >= 0;
   }
   return any;
@@ -318,6 +318,109 @@ static int product_k(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
   return SAIGE_OK;
 }
 
+// ---------------------------------------------------------------------------------
+// SAIGE_PREC_F32: saige-rs's own f32 arithmetic (pcg.rs:128-180 at one rayon thread, where
+// the whole marker range is one chunk and the order of every sum is fixed).  v is cast to
+// f32; per marker, dot_m = sum_i z_im v_i in sample order starting from 0; per sample,
+// acc_i += z_im dot_m in marker order; out = (acc_i * (1/M as f32)) as f64.  Every step is
+// one f32 multiply and one f32 add, rounded separately (no contraction), with z_im the f32
+// rounding of the fp64 standardized value (pcg.rs:111), so each product is bit-identical
+// to the reference's.
+// ---------------------------------------------------------------------------------
+__global__ void to_f32_kernel(const double* __restrict__ v, size_t ldv, size_t n, size_t k,
+                              float* __restrict__ vf, size_t ldf) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (i < n && c < k) vf[i + c * ldf] = (float)v[i + c * ldv];
+}
+
+// dot[m + c * ldd] = sum over samples in order of z_im * vf_ic; one lane per (marker, column)
+__global__ __launch_bounds__(256) void grm_f32_dot_kernel(const uint8_t* __restrict__ geno,
+                                                          size_t row_bytes,
+                                                          const float* __restrict__ lut32,
+                                                          const float* __restrict__ vf, size_t ldf,
+                                                          size_t n, size_t m,
+                                                          float* __restrict__ dot, size_t ldd) {
+#pragma clang fp contract(off)
+  const size_t mk = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (mk >= m) return;
+  const float z0 = lut32[4 * mk + 0], z1 = lut32[4 * mk + 1], z2 = lut32[4 * mk + 2],
+              z3 = lut32[4 * mk + 3];
+  const uint8_t* row = geno + mk * row_bytes;
+  const float* x = vf + c * ldf;
+  float d = 0.0f;
+  size_t i = 0;
+  for (; i + 16 <= n; i += 16) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(row + i / 4);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const uint32_t code = (w >> (2 * t)) & 3u;
+      const float z = code == 0u ? z0 : (code == 1u ? z1 : (code == 2u ? z2 : z3));
+      const float prod = z * x[i + t];
+      d = d + prod;
+    }
+  }
+  for (; i < n; ++i) {
+    const uint32_t code = (row[i / 4] >> (2 * (i % 4))) & 3u;
+    const float z = code == 0u ? z0 : (code == 1u ? z1 : (code == 2u ? z2 : z3));
+    const float prod = z * x[i];
+    d = d + prod;
+  }
+  dot[mk + c * ldd] = d;
+}
+
+// acc_i = sum over markers in order of z_im * dot_mc; out = (acc_i * scale) as f64, or the
+// Sigma form tau0 v / max(w, 1e-30) + tau1 * that.  One lane per (sample, column).
+__global__ __launch_bounds__(256) void grm_f32_acc_kernel(
+    const uint8_t* __restrict__ geno, size_t row_bytes, const float* __restrict__ lut32,
+    const float* __restrict__ dot, size_t ldd, size_t n, size_t m, float scale,
+    double* __restrict__ out, size_t ldo, const double* __restrict__ v, size_t ldv,
+    const double* __restrict__ w, double tau0, double tau1) {
+#pragma clang fp contract(off)
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (i >= n) return;
+  const size_t byte = i >> 2;
+  const uint32_t sh = 2u * (uint32_t)(i & 3);
+  const float* dc = dot + c * ldd;
+  float acc = 0.0f;
+  for (size_t mk = 0; mk < m; ++mk) {
+    const uint32_t code = (geno[mk * row_bytes + byte] >> sh) & 3u;
+    const float z = lut32[4 * mk + code];
+    const float prod = z * dc[mk];
+    acc = acc + prod;
+  }
+  const float r = acc * scale;
+  double g = (double)r;
+  if (w) g = tau0 * v[i + c * ldv] / fmax(w[i], 1e-30) + tau1 * g;
+  out[i + c * ldo] = g;
+}
+
+static int product_f32(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d_out,
+                       size_t ldo, const double* d_w, const double* tau) {
+  GrmDevice& g = h->g;
+  hipStream_t s = h->stream;
+  SAIGE_ARG(!h->loco_req, "LOCO products are not built for SAIGE_PREC_F32");
+  const size_t ldf = round_up(g.n, 64);
+  SAIGE_TRY(h->ws_vhi.ensure(ldf * k * 4));
+  SAIGE_TRY(h->ws_vlo.ensure(g.m * k * 4));
+  float* vf = h->ws_vhi.as<float>();
+  float* dot = h->ws_vlo.as<float>();
+  to_f32_kernel<<<dim3((unsigned)ceil_div(g.n, 256), (unsigned)k), 256, 0, s>>>(d_v, ldv, g.n, k,
+                                                                                 vf, ldf);
+  SAIGE_TRY(timed_launch(h->timer, (uint32_t)k, grm_f32_dot_kernel,
+                         dim3((unsigned)ceil_div(g.m, 256), (unsigned)k), dim3(256), s, g.geno,
+                         g.row_bytes, h->lut32.as<float>(), vf, ldf, g.n, g.m, dot, g.m));
+  const float scale = 1.0f / (float)g.m;
+  SAIGE_TRY(timed_launch(h->timer, (uint32_t)k, grm_f32_acc_kernel,
+                         dim3((unsigned)ceil_div(g.n, 256), (unsigned)k), dim3(256), s, g.geno,
+                         g.row_bytes, h->lut32.as<float>(), dot, g.m, g.n, g.m, scale, d_out,
+                         ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0, tau ? tau[1] : 0.0));
+  h->products += k;
+  return SAIGE_OK;
+}
+
 __global__ void zero_cols_kernel(double* out, size_t ldo, size_t n, size_t k) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t c = blockIdx.y;
@@ -343,6 +446,7 @@ int grm_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, double* d
     SAIGE_HIP(hipGetLastError());
     return SAIGE_OK;
   }
+  if (h->precision == SAIGE_PREC_F32) return product_f32(h, d_v, ldv, k, d_out, ldo, d_w, tau);
   if (h->precision != SAIGE_PREC_FP64) return grm_mixed_product(h, d_v, ldv, k, d_out, ldo, d_w, tau);
   for (size_t c0 = 0; c0 < k; c0 += 64) {
     size_t kc = std::min<size_t>(64, k - c0);
@@ -513,7 +617,8 @@ int grm_create_impl(int device, const uint8_t* bed_body, bool on_device, size_t 
   SAIGE_ARG(m == 0 || (bed_body && af), "bed_body/af");
   SAIGE_ARG(bpm >= (n_total + 3) / 4, "bytes_per_marker < ceil(n_total/4)");
   SAIGE_ARG(precision == SAIGE_PREC_FP64 || precision == SAIGE_PREC_MIXED ||
-                precision == SAIGE_PREC_INT8 || precision == SAIGE_PREC_INT8X2, "precision");
+                precision == SAIGE_PREC_INT8 || precision == SAIGE_PREC_INT8X2 ||
+                precision == SAIGE_PREC_F32, "precision");
   SAIGE_ARG((precision != SAIGE_PREC_INT8 && precision != SAIGE_PREC_INT8X2) ||
                 (n <= 8000000 && m <= 8000000),
             "SAIGE_PREC_INT8 and SAIGE_PREC_INT8X2 need n and M <= 8,000,000 (exact int32 "
@@ -561,6 +666,16 @@ int grm_create_impl(int device, const uint8_t* bed_body, bool on_device, size_t 
     g.lut = h->lut.as<double>();
     if (hipMemcpyAsync(g.lut, lut.data(), lut.size() * 8, hipMemcpyHostToDevice, h->stream) !=
             hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+      set_error("lut upload failed");
+      return fail(SAIGE_ERR_DEVICE);
+    }
+  } else if (precision == SAIGE_PREC_F32) {  // f32 rounding of each standardized value
+    std::vector<float> l32(lut.size());
+    for (size_t e = 0; e < lut.size(); ++e) l32[e] = (float)lut[e];
+    if ((rc = h->lut32.ensure(l32.size() * 4)) != SAIGE_OK) return fail(rc);
+    if (hipMemcpyAsync(h->lut32.p, l32.data(), l32.size() * 4, hipMemcpyHostToDevice,
+                       h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess) {
       set_error("lut upload failed");
       return fail(SAIGE_ERR_DEVICE);

@@ -16,8 +16,8 @@
 //   W_mc   = s_m T_mc;  W' = W - nu_c (nu_c the mean over real markers)
 //   M Y_ic = (G W'_split)_ic - sum_m 2 p_m W'_split_mc + nu_c (r_i - P2)
 //            with r_i = sum_m g_im and P2 = sum_m 2 p_m.
-// Missing genotypes (code 01) decode to 0 in the MFMA operand; handles whose markers have
-// missing calls are refused for this precision (the sparse correction is not built).
+// Missing genotypes (code 01) decode to 0 in the MFMA operand; sparse per-marker and
+// per-sample corrections restore saige-rs's mean imputation (see "Missing calls" below).
 //
 // Layouts: G is resident twice, SNP-major (stage 1 reads a marker's samples) and
 // sample-major (stage 2 reads a sample's markers), 2 bits per genotype in both.  One
@@ -435,6 +435,173 @@ __global__ void stage2_epilogue_kernel(const float* __restrict__ part, size_t sp
 }
 
 // ---------------------------------------------------------------------------------
+// Missing calls in the factored forms.
+// ---------------------------------------------------------------------------------
+// A missing call (code 01) decodes to 0 in the MFMA operand, while saige-rs imputes it to
+// the mean 2p_m, so that its standardized value is 0 (pcg.rs:108-113).  With per-marker
+// lists of the missing samples and per-sample lists of the missing markers, two sparse
+// corrections make the factored product exact again:
+//   stage 1:  T_mc += s_m 2p_m sum_{i missing in m} (x'_split_ic + mean_c)
+//             (so W_mc = s_m T_mc gains s_m^2 2p_m times that sum);
+//   stage 2:  M Y_ic += sum_{m missing in i} 2p_m (W'_split_mc + nu_c).
+// (r_i and ac_m already count only the calls present.)  Each correction is a gather of
+// k-long rows of a row-major copy of the split operand, summed in list order.
+
+// Missing calls among the first `len` codes of each row (rows of rb bytes): count per row.
+__global__ __launch_bounds__(256) void miss_count_kernel(const uint8_t* __restrict__ codes,
+                                                         size_t rb, size_t len,
+                                                         unsigned* __restrict__ cnt) {
+  const size_t row = blockIdx.x;
+  const size_t nw = (len + 15) / 16;
+  unsigned c = 0;
+  for (size_t w = threadIdx.x; w < nw; w += 256) {
+    const uint32_t word = *reinterpret_cast<const uint32_t*>(codes + row * rb + 4 * w);
+    const size_t valid = len - 16 * w;
+    const uint32_t mask = valid >= 16 ? 0x55555555u : ((1u << (2 * valid)) - 1u) & 0x55555555u;
+    c += __popc(word & ~(word >> 1) & mask);
+  }
+  __shared__ unsigned sh[256];
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cnt[row] = sh[0];
+}
+
+// The indices of those missing calls, in increasing order: row rows[b] writes from ptr[b].
+__global__ __launch_bounds__(256) void miss_fill_kernel(const uint8_t* __restrict__ codes,
+                                                        size_t rb, size_t len,
+                                                        const uint32_t* __restrict__ rows,
+                                                        const uint64_t* __restrict__ ptr,
+                                                        uint32_t* __restrict__ out) {
+  const size_t row = rows[blockIdx.x];
+  const size_t nw = (len + 15) / 16;
+  __shared__ unsigned sc[256];
+  uint64_t base = ptr[blockIdx.x];
+  for (size_t w0 = 0; w0 < nw; w0 += 256) {
+    const size_t w = w0 + threadIdx.x;
+    uint32_t bits = 0;
+    if (w < nw) {
+      const uint32_t word = *reinterpret_cast<const uint32_t*>(codes + row * rb + 4 * w);
+      const size_t valid = len - 16 * w;
+      const uint32_t mask = valid >= 16 ? 0x55555555u : ((1u << (2 * valid)) - 1u) & 0x55555555u;
+      bits = word & ~(word >> 1) & mask;
+    }
+    const unsigned c = __popc(bits);
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 1; s < 256; s <<= 1) {  // inclusive scan
+      const unsigned add = (int)threadIdx.x >= s ? sc[threadIdx.x - s] : 0u;
+      __syncthreads();
+      sc[threadIdx.x] += add;
+      __syncthreads();
+    }
+    uint64_t pos = base + sc[threadIdx.x] - c;
+    while (bits) {
+      const int b = __builtin_ctz(bits);
+      out[pos++] = (uint32_t)(16 * w + (size_t)(b >> 1));
+      bits &= bits - 1u;
+    }
+    base += sc[255];
+    __syncthreads();
+  }
+}
+
+// Sparse lists of one side: the rows with missing calls, their offsets and the indices.
+struct MissList {
+  DevBuf ids, ptr, list;
+  size_t rows = 0;
+};
+
+static int build_miss_list(hipStream_t s, const uint8_t* codes, size_t rb, size_t rows,
+                           size_t len, MissList& ml) {
+  DevBuf cnt;
+  SAIGE_TRY(cnt.ensure(rows * 4));
+  miss_count_kernel<<<(unsigned)rows, 256, 0, s>>>(codes, rb, len, cnt.as<unsigned>());
+  SAIGE_HIP(hipGetLastError());
+  std::vector<unsigned> hc(rows);
+  SAIGE_HIP(hipMemcpyAsync(hc.data(), cnt.p, rows * 4, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> ids;
+  std::vector<uint64_t> ptr(1, 0);
+  for (size_t r = 0; r < rows; ++r)
+    if (hc[r]) {
+      ids.push_back((uint32_t)r);
+      ptr.push_back(ptr.back() + hc[r]);
+    }
+  ml.rows = ids.size();
+  if (ml.rows == 0) return SAIGE_OK;
+  SAIGE_TRY(ml.ids.ensure(ids.size() * 4));
+  SAIGE_TRY(ml.ptr.ensure(ptr.size() * 8));
+  SAIGE_TRY(ml.list.ensure(ptr.back() * 4));
+  SAIGE_HIP(hipMemcpyAsync(ml.ids.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, s));
+  SAIGE_HIP(hipMemcpyAsync(ml.ptr.p, ptr.data(), ptr.size() * 8, hipMemcpyHostToDevice, s));
+  miss_fill_kernel<<<(unsigned)ml.rows, 256, 0, s>>>(codes, rb, len, ml.ids.as<uint32_t>(),
+                                                     ml.ptr.as<uint64_t>(), ml.list.as<uint32_t>());
+  SAIGE_HIP(hipGetLastError());
+  SAIGE_HIP(hipStreamSynchronize(s));  // the host vectors end here
+  return SAIGE_OK;
+}
+
+// means[c] = (sum of the column's chunk partials) * inv_len
+__global__ void chunk_means_kernel(const double* __restrict__ psum, size_t nch, size_t k,
+                                   double inv_len, double* __restrict__ means) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double s = 0.0;
+  for (size_t q = 0; q < nch; ++q) s += psum[c * nch + q];
+  means[c] = s * inv_len;
+}
+
+// rows[i * k + c] = xs[i + c * ldxs] + mean[c]: the split operand as rows, for the gathers
+__global__ void rows_plus_mean_kernel(const double* __restrict__ xs, size_t ldxs, size_t len,
+                                      size_t k, const double* __restrict__ mean,
+                                      double* __restrict__ rows) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len * k) return;
+  const size_t i = e / k, c = e % k;
+  rows[e] = xs[i + c * ldxs] + mean[c];
+}
+
+// w[m + c * ldw] += sc_m^2 2p_m sum_{i in list(m)} rows[i * k + c] for the listed markers
+__global__ void miss_stage1_kernel(const uint32_t* __restrict__ ids,
+                                   const uint64_t* __restrict__ ptr,
+                                   const uint32_t* __restrict__ list,
+                                   const double* __restrict__ rows, size_t k,
+                                   const double* __restrict__ sc, const double* __restrict__ p2,
+                                   double* __restrict__ w, size_t ldw) {
+  const size_t mk = ids[blockIdx.x];
+  const uint64_t e0 = ptr[blockIdx.x], e1 = ptr[blockIdx.x + 1];
+  const double f = sc[mk] * sc[mk] * p2[mk];
+  for (size_t c = threadIdx.x; c < k; c += blockDim.x) {
+    double acc = 0.0;
+    for (uint64_t e = e0; e < e1; ++e) acc += rows[(size_t)list[e] * k + c];
+    w[mk + c * ldw] += f * acc;
+  }
+}
+
+// out[i + c * ldo] += scale * sum_{m in list(i)} 2p_m rows[m * k + c] for the listed samples
+__global__ void miss_stage2_kernel(const uint32_t* __restrict__ ids,
+                                   const uint64_t* __restrict__ ptr,
+                                   const uint32_t* __restrict__ list,
+                                   const double* __restrict__ rows, size_t k,
+                                   const double* __restrict__ p2, double scale,
+                                   double* __restrict__ out, size_t ldo) {
+  const size_t i = ids[blockIdx.x];
+  const uint64_t e0 = ptr[blockIdx.x], e1 = ptr[blockIdx.x + 1];
+  for (size_t c = threadIdx.x; c < k; c += blockDim.x) {
+    double acc = 0.0;
+    for (uint64_t e = e0; e < e1; ++e) {
+      const size_t mk = list[e];
+      acc += p2[mk] * rows[mk * k + c];
+    }
+    out[i + c * ldo] += scale * acc;
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------------
 struct MixedState {
@@ -445,10 +612,65 @@ struct MixedState {
   DevBuf vhi, vlo, vs, whi, wlo, ws_, w, part1, part2, red, cs, mean, nu, s2p;
   // int8 path: sliced operands, per-column slice scales, integer partials
   DevBuf vq, wq, ipart1, ipart2;
+  // missing calls: per-marker sample lists, per-sample marker lists, gather rows
+  MissList miss_mk, miss_sm;
+  DevBuf mrows, mmean;
 };
 
 static MixedState* mstate(saige_grm* h);
 MixedState* mixed_state(saige_grm* h) { return mstate(h); }
+
+// Stage-1 missing-call correction of W (m x k, ldw) from the split V' (xs: n x k, ldxs) and
+// the column means of V (`mean`, or the chunk partials psum over nch chunks).
+static int missing_stage1(saige_grm* h, const double* xs, size_t ldxs, size_t k,
+                          const double* mean, const double* psum, size_t nch, double* w,
+                          size_t ldw) {
+  MixedState* ms = mstate(h);
+  if (ms->miss_mk.rows == 0 || k == 0) return SAIGE_OK;
+  hipStream_t s = h->stream;
+  const size_t n = h->g.n;
+  SAIGE_TRY(ms->mrows.ensure(n * k * 8));
+  if (!mean) {
+    SAIGE_TRY(ms->mmean.ensure(k * 8));
+    chunk_means_kernel<<<(unsigned)ceil_div(k, 256), 256, 0, s>>>(psum, nch, k, 1.0 / (double)n,
+                                                                   ms->mmean.as<double>());
+    mean = ms->mmean.as<double>();
+  }
+  rows_plus_mean_kernel<<<(unsigned)ceil_div(n * k, 256), 256, 0, s>>>(xs, ldxs, n, k, mean,
+                                                                        ms->mrows.as<double>());
+  miss_stage1_kernel<<<(unsigned)ms->miss_mk.rows, 64, 0, s>>>(
+      ms->miss_mk.ids.as<uint32_t>(), ms->miss_mk.ptr.as<uint64_t>(),
+      ms->miss_mk.list.as<uint32_t>(), ms->mrows.as<double>(), k, ms->sc.as<double>(),
+      ms->p2.as<double>(), w, ldw);
+  SAIGE_HIP(hipGetLastError());
+  return SAIGE_OK;
+}
+
+// Stage-2 missing-call correction of the output (scale = tau1 / M, or 1 / M) from the split
+// W' (ws: m x k, ldws) and the column means of W (`nu`, or the chunk partials psum).
+static int missing_stage2(saige_grm* h, const double* ws, size_t ldws, size_t k,
+                          const double* nu, const double* psum, size_t nch, double* out,
+                          size_t ldo, double scale) {
+  MixedState* ms = mstate(h);
+  if (ms->miss_sm.rows == 0 || k == 0) return SAIGE_OK;
+  hipStream_t s = h->stream;
+  const size_t m = h->g.m;
+  SAIGE_TRY(ms->mrows.ensure(m * k * 8));
+  if (!nu) {
+    SAIGE_TRY(ms->mmean.ensure(k * 8));
+    chunk_means_kernel<<<(unsigned)ceil_div(k, 256), 256, 0, s>>>(psum, nch, k, 1.0 / (double)m,
+                                                                   ms->mmean.as<double>());
+    nu = ms->mmean.as<double>();
+  }
+  rows_plus_mean_kernel<<<(unsigned)ceil_div(m * k, 256), 256, 0, s>>>(ws, ldws, m, k, nu,
+                                                                        ms->mrows.as<double>());
+  miss_stage2_kernel<<<(unsigned)ms->miss_sm.rows, 64, 0, s>>>(
+      ms->miss_sm.ids.as<uint32_t>(), ms->miss_sm.ptr.as<uint64_t>(),
+      ms->miss_sm.list.as<uint32_t>(), ms->mrows.as<double>(), k, ms->p2.as<double>(), scale,
+      out, ldo);
+  SAIGE_HIP(hipGetLastError());
+  return SAIGE_OK;
+}
 
 int grm_mixed_prepare(saige_grm* h) {
   GrmDevice& g = h->g;
@@ -475,9 +697,10 @@ int grm_mixed_prepare(saige_grm* h) {
   SAIGE_HIP(hipMemcpyAsync(&nmiss, ms->nmiss.p, 4, hipMemcpyDeviceToHost, s));
   SAIGE_HIP(hipStreamSynchronize(s));
   if (nmiss) {
-    set_error("reduced precision: %u missing genotype calls among the GRM markers; the "
-              "factored MFMA paths require complete calls (use SAIGE_PREC_FP64)", nmiss);
-    return SAIGE_ERR_ARG;
+    // per-marker lists from the SNP-major rows (real samples), per-sample lists from the
+    // sample-major copy (real markers); both before the int8 forms retile the code matrices
+    SAIGE_TRY(build_miss_list(s, g.geno, g.row_bytes, g.m, g.n, ms->miss_mk));
+    SAIGE_TRY(build_miss_list(s, ms->geno_t.as<uint8_t>(), ms->rbt, g.n, g.m, ms->miss_sm));
   }
   // per-marker s_m, 2 p_m (from the host's 1/sd and 2p/sd) and, on the device,
   // zs_m = s_m (ac_m - 2 p_m n) with ac from the stored codes
@@ -563,6 +786,7 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   stage1_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(m, 256), (unsigned)k), 256, 0, s>>>(
       ms->part1.as<float>(), splits1, m_pad, m, k, ms->sc.as<double>(), ms->p2.as<double>(),
       ms->zs.as<double>(), cs, mean, ms->w.as<double>(), m);
+  SAIGE_TRY(missing_stage1(h, ms->vs.as<double>(), n, k, mean, nullptr, 0, ms->w.as<double>(), m));
   LocoCodes lc;
   if (loco_codes(h, c0, k, lc)) loco_mask(h, ms->w.as<double>(), 1, m, m, k, lc, s);
 
@@ -590,6 +814,8 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
       ms->part2.as<float>(), splits2, n_pad, n, k, ms->rsum.as<double>(), ms->P2, nu, s2p,
       1.0 / (double)m, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0, tau ? tau[1] : 0.0);
   SAIGE_HIP(hipGetLastError());
+  SAIGE_TRY(missing_stage2(h, ms->ws_.as<double>(), m, k, nu, nullptr, 0, d_out, ldo,
+                           (tau ? tau[1] : 1.0) / (double)m));
   h->products += k;
   return SAIGE_OK;
 }
@@ -635,29 +861,53 @@ void grm_mixed_release(saige_grm* h) {
 // Int8 paths (SAIGE_PREC_INT8, SAIGE_PREC_INT8X2): exact genotypes times int8 operand
 // slices on integer MFMA.
 // =================================================================================
-// The centred operand x' of each column is split into S int8 slices (S = 3 for
-// SAIGE_PREC_INT8, 2 for SAIGE_PREC_INT8X2) with per-column scales s0 = max|x'| / 127,
-// s1 = s0 / 254, s2 = s1 / 254:
+// The centred operand x' of each column is split into 2 or 3 int8 slices with per-column
+// scales s0 = max|x'| / 127, s1 = s0 / 254, s2 = s1 / 254:
 //   q0 = rint(x'/s0),  q1 = rint((x' - s0 q0)/s1),  q2 = rint((x' - s0 q0 - s1 q1)/s2),
 // each in [-127, 127], so |x' - sum_j s_j q_j| <= s_{S-1}/2: max|x'| / (2 * 127 * 254^2),
 // about 6.1e-8 of the column's largest entry, with three slices; max|x'| / (2 * 127 * 254),
-// about 1.55e-5, with two.  G^T Q_j is computed exactly with
+// about 1.55e-5, with two.  SAIGE_PREC_INT8 handles always take three slices;
+// SAIGE_PREC_INT8X2 handles take two unless a product asks for three on some of its
+// columns (saige_grm::slice_req).  G^T Q_j is computed exactly with
 // v_mfma_i32_16x16x64_i8 (int32 accumulation: |sum| <= 2 * 127 * n < 2^31 for n <= 8e6),
 // the slices are recombined in fp64, and x'_split = sum_j s_j q_j enters the rank-one
 // terms of the factored form exactly as in the bf16 path.  Integer partial sums make the
-// result independent of how the reduction is split.
+// result independent of how the reduction is split, and a column's result depends only on
+// its own slices, not on the other columns of the launch.
 //
-// Operand layout: slice j of vector c is column j*k + c of a KP-column int8 matrix (columns
-// from S*k up to KP are zero), each column pad_len long in reduction order.  Within every
-// group of 16 reduction indices the bytes are stored 4x4-transposed (index 4b+u at byte
-// 4u+b): a lane expands one 32-bit word of packed codes (code 4b+u at bits 8b+2u) with
-// v_perm into four words, word u holding codes {u, 4+u, 8+u, 12+u} one per byte, and the
-// operand's bytes then pair with the same reduction indices.
-constexpr int kI8MaxKT = 7;  // at most 112 slice columns (37 vectors of 3 slices, 56 of 2)
+// Operand layout of one launch with k vectors, k3 of them with a third slice: slice 0 of
+// vector c is column c, slice 1 column k + c, slice 2 column 2k + (the rank of c among the
+// three-slice vectors); columns from 2k + k3 up to the padded width KPT are zero.  Each
+// column is pad_len long in reduction order.  Within every group of 16 reduction indices the
+// bytes are stored 4x4-transposed (index 4b+u at byte 4u+b): a lane expands one 32-bit word
+// of packed codes (code 4b+u at bits 8b+2u) with v_perm into four words, word u holding
+// codes {u, 4+u, 8+u, 12+u} one per byte, and the operand's bytes then pair with the same
+// reduction indices.
+//
+// Column groups: KPT = G * 16 * KT with KT <= 7 tiles per group.  One launch covers all G
+// groups; work-groups that share a row block and a reduction split but differ in the group
+// are issued back to back on the same XCD (XCD = linear work-group id mod 8), so the
+// genotype bytes that one of them streams from HBM are read by the others from that XCD's
+// L2: the genotypes stream once per pass, whatever the number of columns.
+constexpr int kI8MaxVec = 256;    // vectors per launch
+constexpr int kI8MaxCols = 512;   // slice columns per launch
 constexpr size_t kI8MaxRed = 8000000;
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Bit c set: vector c of the launch has a third operand slice.
+struct SliceMap {
+  uint32_t bits[kI8MaxVec / 32];
+};
+// Operand column of the third slice of vector c, or -1 without one.
+__host__ __device__ __forceinline__ int third_col(const SliceMap& sm, size_t k, size_t c) {
+  const uint32_t w = sm.bits[c >> 5], b = 1u << (c & 31);
+  if (!(w & b)) return -1;
+  int r = __builtin_popcount(w & (b - 1));
+  for (size_t j = 0; j < (c >> 5); ++j) r += __builtin_popcount(sm.bits[j]);
+  return (int)(2 * k) + r;
+}
 
 // Column scalars rebuilt from fixed-order partials (one value per kDotChunk rows, combined in
 // chunk order): the mean of column c, and the three slice scales from the largest
@@ -721,23 +971,25 @@ __device__ __forceinline__ size_t perm16(size_t i) {
 }
 __device__ __forceinline__ double q127(double t) { return fmin(fmax(rint(t), -127.0), 127.0); }
 
-// x' = x - mean -> nsl (2 or 3) int8 slices (stored in the permuted reduction order, zero
-// for rows >= len), the fp64 value they represent (xs), and per chunk of kDotChunk rows the
-// partial sum of wt * xs (wt = 1 when null) for the rank-one terms.  One block per (chunk,
-// column); grid.y = k + (KP - nsl*k): the last rows of the grid clear the padding columns.
+// x' = x - mean -> 2 or 3 int8 slices per column (stored in the permuted reduction order,
+// zero for rows >= len), the fp64 value they represent (xs), and per chunk of kDotChunk rows
+// the partial sum of wt * xs (wt = 1 when null) for the rank-one terms.  One block per
+// (chunk, column); grid.y = k + (KPT - 2k - k3): the last rows of the grid clear the padding
+// columns.
 __global__ __launch_bounds__(kDotBlock) void quantize_slices_kernel(
-    const double* __restrict__ x, size_t ldx, size_t len, size_t k, int nsl,
-    const double* __restrict__ psum, const double* __restrict__ pmax,
+    const double* __restrict__ x, size_t ldx, size_t len, size_t k, const SliceMap sm,
+    size_t k3, const double* __restrict__ psum, const double* __restrict__ pmax,
     const double* __restrict__ pmin, double inv_len, size_t nchunks, size_t pad_len,
     int8_t* __restrict__ q, double* __restrict__ xs, size_t ldxs,
     const double* __restrict__ wt, double* __restrict__ pdot) {
   const size_t c = blockIdx.y;
   const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, pad_len);
-  if (c >= k) {  // padding column nsl*k + (c - k)
-    const size_t pc = (size_t)(nsl - 1) * k + c;
+  if (c >= k) {  // padding column 2k + k3 + (c - k)
+    const size_t pc = 2 * k + k3 + (c - k);
     for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) q[pc * pad_len + perm16(i)] = 0;
     return;
   }
+  const int c3 = third_col(sm, k, c);
   __shared__ double cs_s[4];
   if (threadIdx.x == 0) {
     cs_s[0] = chunk_sum(psum, c, nchunks) * inv_len;
@@ -754,7 +1006,7 @@ __global__ __launch_bounds__(kDotBlock) void quantize_slices_kernel(
       double r = fma(-s0, t0, xv);
       t1 = q127(r / s1);
       double v;
-      if (nsl == 3) {
+      if (c3 >= 0) {
         r = fma(-s1, t1, r);
         t2 = q127(r / s2);
         v = s0 * t0 + s1 * t1 + s2 * t2;
@@ -767,7 +1019,7 @@ __global__ __launch_bounds__(kDotBlock) void quantize_slices_kernel(
     const size_t pos = perm16(i);
     q[c * pad_len + pos] = (int8_t)t0;
     q[(k + c) * pad_len + pos] = (int8_t)t1;
-    if (nsl == 3) q[(2 * k + c) * pad_len + pos] = (int8_t)t2;
+    if (c3 >= 0) q[(size_t)c3 * pad_len + pos] = (int8_t)t2;
   }
   __shared__ double sh[kDotBlock];
   sh[threadIdx.x] = acc;
@@ -779,22 +1031,26 @@ __global__ __launch_bounds__(kDotBlock) void quantize_slices_kernel(
   if (threadIdx.x == 0 && i0 < len) pdot[c * nchunks + blockIdx.x] = sh[0];
 }
 
-// part[split][row][16*KT] = sum over the split's reduction range of g(row, red) * q[col][red].
+// part[split][row][part_ld] (columns grp*16*KT .. of the group) = sum over the split's
+// reduction range of g(row, red) * q[col][red].
+// Work-group order: linear id L runs over round_up(nrb * nsp, 8) * ngroups work-groups;
+// j = L / 8 and x = L % 8 give the tile (j / ngroups) * 8 + x (row block tile % nrb, split
+// tile / nrb) and the column group j % ngroups, so the groups of one tile are consecutive
+// work-groups of one XCD.  Tiles past nrb * nsp exit at once (a whole work-group).
 // NW waves; each wave owns 16*RT rows and reads their packed codes straight from global
 // memory in the tiled layout of retile_codes_kernel (code_stride = bytes per row; KCH/16
 // bytes = KCH/4 genotypes per lane and chunk of KCH reduction indices, the next chunk
 // prefetched in registers; lane q takes the q-th quarter of a row's chunk, so one load of
-// a 16-row group stays inside one 2 KB tile).  The KP x KCH operand chunk is staged in
-// LDS, double-buffered so that one
-// barrier per chunk separates the reads of one buffer from the refill of the other (column
-// stride 272 bytes: the 16-byte fragment reads of 8 consecutive columns fall in distinct
-// banks).  Eight waves of RT = 4 keep the 112 accumulators of k = 34 in VGPRs at two waves
-// per SIMD; four waves of RT = 8 spill them into AGPR shuffles.
+// a 16-row group stays inside one 2 KB tile).  The group's 16*KT x KCH operand chunk is
+// staged in LDS, double-buffered so that one barrier per chunk separates the reads of one
+// buffer from the refill of the other (column stride 272 bytes: the 16-byte fragment reads
+// of 8 consecutive columns fall in distinct banks).  Eight waves of RT = 4 keep the 112
+// accumulators of KT = 7 in VGPRs at two waves per SIMD.
 template <int RT, int KT, int NW, int KCH>
 __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
     const uint8_t* __restrict__ codes, size_t code_stride, const int8_t* __restrict__ bq,
     size_t b_stride, size_t red_per_split, size_t red_len, int* __restrict__ part,
-    size_t rows_pad) {
+    size_t rows_pad, size_t part_ld, size_t nrb, size_t nsp, size_t ngroups) {
   constexpr int KP = 16 * KT;
   constexpr int ROWS = NW * 16 * RT;
   constexpr int NT = NW * 64;
@@ -805,9 +1061,12 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
   constexpr int BCSTEP = NT / SEGS;
   constexpr int BUF = KP * SB;
   __shared__ __attribute__((aligned(16))) int8_t sB[2 * BUF];
+  const size_t L = blockIdx.x, jx = L >> 3;
+  const size_t tile = (jx / ngroups) * 8 + (L & 7), grp = jx % ngroups;
+  if (tile >= nrb * nsp) return;  // uniform over the work-group
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 15, q = lane >> 4;
-  const size_t rowb = (size_t)blockIdx.x * ROWS + (size_t)wave * 16 * RT;
-  const size_t split = blockIdx.y;
+  const size_t rowb = (tile % nrb) * ROWS + (size_t)wave * 16 * RT;
+  const size_t split = tile / nrb;
   const size_t k0 = split * red_per_split;
   const size_t k1 = min(k0 + red_per_split, red_len);
   if (k0 >= k1) return;  // uniform over the work-group
@@ -815,7 +1074,7 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
   static_assert(KCH <= 512, "a row's chunk lies inside one 128-byte tile segment");
   const size_t ntile = code_stride / 128;  // tiles per 16-row group
   const uint8_t* abase = codes + (rowb / 16) * ntile * 2048 + r * 128 + (KCH / 16) * q;
-  const int8_t* bbase = bq + (size_t)bcol0 * b_stride + 16 * bseg;
+  const int8_t* bbase = bq + (grp * KP + (size_t)bcol0) * b_stride + 16 * bseg;
   i32x4 acc[RT][KT];
 #pragma unroll
   for (int g = 0; g < RT; ++g)
@@ -886,24 +1145,29 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
     __syncthreads();
     buf = 1 - buf;
   }
+  int* pbase = part + (split * rows_pad + rowb) * part_ld + grp * KP;
 #pragma unroll
   for (int g = 0; g < RT; ++g)
 #pragma unroll
     for (int t = 0; t < KT; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        part[(split * rows_pad + rowb + 16 * g + 4 * q + j) * KP + 16 * t + r] = acc[g][t][j];
+        pbase[(16 * g + 4 * q + j) * part_ld + 16 * t + r] = acc[g][t][j];
 }
 
-// Sum the integer partials of 64 rows over the splits into an LDS tile (16-byte loads along
-// the partial layout, four splits in flight per step); the caller's epilogue then writes
-// column-major outputs.
-template <int KP>
+// Rows of one epilogue work-group: the LDS tile of ER rows x (kp + 1) ints stays under 33 KB.
+__host__ __device__ inline int epi_rows(size_t kp) { return kp <= 128 ? 64 : (kp <= 256 ? 32 : 16); }
+
+// Sum the integer partials of ER rows over the splits into an LDS tile (ER x (kp + 1); 16-byte
+// loads along the partial layout, four splits in flight per step); the caller's epilogue
+// then writes column-major outputs.
 __device__ __forceinline__ void i8_tile_sum(const int* __restrict__ part, size_t splits,
-                                            size_t rows_pad, size_t row0, int (*tile)[KP + 1]) {
-  const i32x4* base = reinterpret_cast<const i32x4*>(part + row0 * KP);
-  const size_t stride = rows_pad * KP / 4;  // one split, in 16-byte groups
-  for (int e4 = threadIdx.x; e4 < 16 * KP; e4 += 256) {
+                                            size_t rows_pad, size_t row0, int er, size_t kp,
+                                            int* __restrict__ tile) {
+  const i32x4* base = reinterpret_cast<const i32x4*>(part + row0 * kp);
+  const size_t stride = rows_pad * kp / 4;  // one split, in 16-byte groups
+  const size_t n4 = (size_t)er * kp / 4;
+  for (size_t e4 = threadIdx.x; e4 < n4; e4 += 256) {
     i32x4 acc = {0, 0, 0, 0};
     size_t sp = 0;
     for (; sp + 4 <= splits; sp += 4) {
@@ -912,76 +1176,90 @@ __device__ __forceinline__ void i8_tile_sum(const int* __restrict__ part, size_t
       acc += (a + b) + (c + d);
     }
     for (; sp < splits; ++sp) acc += base[sp * stride + e4];
-    const int e = 4 * e4;  // KP is a multiple of 16: the four values share a row
-    tile[e / KP][e % KP] = acc.x;
-    tile[e / KP][e % KP + 1] = acc.y;
-    tile[e / KP][e % KP + 2] = acc.z;
-    tile[e / KP][e % KP + 3] = acc.w;
+    const size_t e = 4 * e4;  // kp is a multiple of 16: the four values share a row
+    int* t = tile + (e / kp) * (kp + 1) + e % kp;
+    t[0] = acc.x;
+    t[1] = acc.y;
+    t[2] = acc.z;
+    t[3] = acc.w;
   }
   __syncthreads();
 }
 
-// Stage-1 epilogue: U'_mc = sum_j s_jc I_jmc over the nsl slices; T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c;
-// W_mc = s_m T_mc (column-major, ldw).  mean_c, the slice scales and cs_c are rebuilt from
-// the partials of the operand pass (column sums psum, maxima pmax, minima pmin, sliced sums
-// pcs).
-template <int KP>
-__global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
-    const int* __restrict__ part, size_t splits, size_t m_pad, size_t m, size_t k, int nsl,
-    const double* __restrict__ psum, const double* __restrict__ pmax,
-    const double* __restrict__ pmin, const double* __restrict__ pcs, double inv_n, size_t nch, const double* __restrict__ sc,
-    const double* __restrict__ p2, const double* __restrict__ zs, double* __restrict__ w,
-    size_t ldw) {
-  __shared__ int tile[64][KP + 1];
-  __shared__ double col[KP / 2 + 1][5];  // mean, s0, s1, s2, cs
+// Column scalars of the epilogues: [mean or nu, s0, s1, s2, cs or S2p] and the operand
+// column of the third slice (-1 without one), rebuilt from the operand pass's partials.
+struct I8Col {
+  double v[5];
+  int c3;
+};
+__device__ __forceinline__ void i8_col_scalars(I8Col* col, size_t k, const SliceMap& sm,
+                                               const double* __restrict__ psum,
+                                               const double* __restrict__ pmax,
+                                               const double* __restrict__ pmin,
+                                               const double* __restrict__ pdot, double inv_len,
+                                               size_t nch) {
   for (size_t c = threadIdx.x; c < k; c += 256) {
-    col[c][0] = chunk_sum(psum, c, nch) * inv_n;
-    slice_scales(pmax, pmin, col[c][0], c, nch, &col[c][1]);
-    col[c][4] = chunk_sum(pcs, c, nch);
+    col[c].v[0] = chunk_sum(psum, c, nch) * inv_len;
+    slice_scales(pmax, pmin, col[c].v[0], c, nch, &col[c].v[1]);
+    col[c].v[4] = chunk_sum(pdot, c, nch);
+    col[c].c3 = third_col(sm, k, c);
   }
-  const size_t row0 = (size_t)blockIdx.x * 64;
-  i8_tile_sum<KP>(part, splits, m_pad, row0, tile);
-  const int r = threadIdx.x & 63;
+}
+__device__ __forceinline__ double i8_recombine(const int* __restrict__ trow, const I8Col& cc,
+                                               size_t k, size_t c) {
+  double u = cc.v[1] * (double)trow[c] + cc.v[2] * (double)trow[k + c];
+  if (cc.c3 >= 0) u += cc.v[3] * (double)trow[cc.c3];
+  return u;
+}
+
+// Stage-1 epilogue: U'_mc = sum_j s_jc I_jmc over the column's slices;
+// T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c; W_mc = s_m T_mc (column-major, ldw).
+__global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
+    const int* __restrict__ part, size_t splits, size_t m_pad, size_t m, size_t k, size_t kp,
+    const SliceMap sm, const double* __restrict__ psum, const double* __restrict__ pmax,
+    const double* __restrict__ pmin, const double* __restrict__ pcs, double inv_n, size_t nch,
+    const double* __restrict__ sc, const double* __restrict__ p2, const double* __restrict__ zs,
+    double* __restrict__ w, size_t ldw) {
+  extern __shared__ int tile[];
+  __shared__ I8Col col[kI8MaxVec];
+  const int er = epi_rows(kp);
+  i8_col_scalars(col, k, sm, psum, pmax, pmin, pcs, inv_n, nch);
+  const size_t row0 = (size_t)blockIdx.x * er;
+  i8_tile_sum(part, splits, m_pad, row0, er, kp, tile);
+  const int r = threadIdx.x % er;
   const size_t mm = row0 + r;
   if (mm >= m) return;
+  const int* trow = tile + r * (kp + 1);
   const double s = sc[mm], pp = p2[mm], z = zs[mm];
-  for (size_t c = threadIdx.x >> 6; c < k; c += 4) {
-    const double u = nsl == 3 ? col[c][1] * (double)tile[r][c] +
-                                    col[c][2] * (double)tile[r][k + c] +
-                                    col[c][3] * (double)tile[r][2 * k + c]
-                              : col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c];
-    const double t = s * (u - pp * col[c][4]) + z * col[c][0];
+  for (size_t c = threadIdx.x / er; c < k; c += 256 / er) {
+    const double u = i8_recombine(trow, col[c], k, c);
+    const double t = s * (u - pp * col[c].v[4]) + z * col[c].v[0];
     w[mm + c * ldw] = s * t;
   }
 }
 
 // Stage-2 epilogue: M Y_ic = sum_j s_jc I_jic + nu_c (r_i - P2) - S2p_c; optional Sigma form.
-// nu_c, the slice scales and S2p_c are rebuilt from the partials of the W pass.
-template <int KP>
 __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
-    const int* __restrict__ part, size_t splits, size_t n_pad, size_t n, size_t k, int nsl,
-    const double* __restrict__ psum, const double* __restrict__ pmax,
-    const double* __restrict__ pmin, const double* __restrict__ ps2p, double inv_m, size_t nch, const double* __restrict__ rsum,
-    double P2, double* __restrict__ out, size_t ldo, const double* __restrict__ v, size_t ldv,
-    const double* __restrict__ wts, double tau0, double tau1) {
-  __shared__ int tile[64][KP + 1];
-  __shared__ double col[KP / 2 + 1][5];  // nu, s0, s1, s2, S2p
-  for (size_t c = threadIdx.x; c < k; c += 256) {
-    col[c][0] = chunk_sum(psum, c, nch) * inv_m;
-    slice_scales(pmax, pmin, col[c][0], c, nch, &col[c][1]);
-    col[c][4] = chunk_sum(ps2p, c, nch);
-  }
-  const size_t row0 = (size_t)blockIdx.x * 64;
-  i8_tile_sum<KP>(part, splits, n_pad, row0, tile);
-  const int r = threadIdx.x & 63;
+    const int* __restrict__ part, size_t splits, size_t n_pad, size_t n, size_t k, size_t kp,
+    const SliceMap sm, const double* __restrict__ psum, const double* __restrict__ pmax,
+    const double* __restrict__ pmin, const double* __restrict__ ps2p, double inv_m, size_t nch,
+    const double* __restrict__ rsum, double P2, double* __restrict__ out, size_t ldo,
+    const double* __restrict__ v, size_t ldv, const double* __restrict__ wts, double tau0,
+    double tau1) {
+  extern __shared__ int tile[];
+  __shared__ I8Col col[kI8MaxVec];
+  const int er = epi_rows(kp);
+  i8_col_scalars(col, k, sm, psum, pmax, pmin, ps2p, inv_m, nch);
+  const size_t row0 = (size_t)blockIdx.x * er;
+  i8_tile_sum(part, splits, n_pad, row0, er, kp, tile);
+  const int r = threadIdx.x % er;
   const size_t i = row0 + r;
   if (i >= n) return;
+  const int* trow = tile + r * (kp + 1);
   const double ri = rsum[i];
-  for (size_t c = threadIdx.x >> 6; c < k; c += 4) {
-    double y = nsl == 3 ? col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c] +
-                              col[c][3] * (double)tile[r][2 * k + c]
-                        : col[c][1] * (double)tile[r][c] + col[c][2] * (double)tile[r][k + c];
-    y = (y + col[c][0] * (ri - P2) - col[c][4]) * inv_m;
+  for (size_t c = threadIdx.x / er; c < k; c += 256 / er) {
+    double y = i8_recombine(trow, col[c], k, c);
+    y = (y + col[c].v[0] * (ri - P2) - col[c].v[4]) * inv_m;
     if (wts) y = tau0 * v[i + c * ldv] / fmax(wts[i], 1e-30) + tau1 * y;
     out[i + c * ldo] = y;
   }
@@ -1004,60 +1282,110 @@ static size_t resident_slots() {
 }
 
 // Reduction splits: the fewest rounds of resident work-groups times chunks per split, with
-// a 2% charge per extra split for the partial-sum traffic.
-static size_t pick_splits(size_t rblk, size_t nchunks, size_t slots) {
+// a 2% charge per extra split for the partial-sum traffic.  blocks = work-groups per split.
+static size_t pick_splits(size_t blocks, size_t nchunks, size_t slots) {
   size_t best = 1;
   double best_cost = 1e300;
   for (size_t sp = 1; sp <= std::min<size_t>(nchunks, 32); ++sp) {
     const size_t per = ceil_div(nchunks, sp);
     const size_t eff = ceil_div(nchunks, per);
-    const double rounds = (double)ceil_div(rblk * eff, slots);
+    const double rounds = (double)ceil_div(blocks * eff, slots);
     const double cost = rounds * (double)per * (1.0 + 0.02 * (double)(eff - 1));
     if (cost < best_cost - 1e-9) { best_cost = cost; best = eff; }
   }
   return best;
 }
 
-template <int RT, int KT, int NW, int KCH>
+template <int KT>
+struct I8Shape {
+  static constexpr int RT = 4;
+  static constexpr int NW = KT <= 2 ? 4 : 8;
+  // few columns: the pass is bound by the genotype stream, so each row's chunk is a whole
+  // 128-byte line; more columns: 64-byte chunks keep the prefetch registers in budget
+  static constexpr int KCH = KT <= 2 ? 512 : 256;
+};
+
+template <int KT>
 static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const int8_t* bq,
-                   size_t b_stride, size_t rows_pad, size_t red_len, DevBuf& part,
+                   size_t b_stride, size_t rows_pad, size_t red_len, size_t groups, DevBuf& part,
                    size_t* splits_out) {
-  constexpr int KP = 16 * KT;
-  constexpr int ROWS = NW * 16 * RT;
+  using S = I8Shape<KT>;
+  constexpr int ROWS = S::NW * 16 * S::RT;
   static_assert(kMarkerPad % ROWS == 0 && kSamplePad % ROWS == 0, "row tiles must divide padding");
   hipStream_t s = h->stream;
+  const size_t kpt = (size_t)16 * KT * groups;
   const size_t rblk = rows_pad / ROWS;
-  const size_t nch = red_len / KCH;
-  const size_t splits = pick_splits(rblk, nch, resident_slots<RT, KT, NW, KCH>());
-  const size_t rps = ceil_div(nch, splits) * KCH;
+  const size_t nch = red_len / S::KCH;
+  const size_t splits =
+      pick_splits(rblk * groups, nch, resident_slots<S::RT, KT, S::NW, S::KCH>());
+  const size_t rps = ceil_div(nch, splits) * S::KCH;
   const size_t sp = ceil_div(red_len, rps);
-  SAIGE_TRY(part.ensure(sp * rows_pad * KP * 4));
-  SAIGE_TRY(timed_launch(h->timer, (uint32_t)KP, i8_gemm_kernel<RT, KT, NW, KCH>, dim3((unsigned)rblk, (unsigned)sp),
-                         dim3(NW * 64), s, codes, code_stride, bq, b_stride, rps, red_len,
-                         part.as<int>(), rows_pad));
+  SAIGE_TRY(part.ensure(sp * rows_pad * kpt * 4));
+  const size_t blocks = round_up(rblk * sp, 8) * groups;
+  SAIGE_ARG(blocks < (1u << 31), "int8 GEMM grid too large");
+  SAIGE_TRY(timed_launch(h->timer, (uint32_t)kpt, i8_gemm_kernel<S::RT, KT, S::NW, S::KCH>,
+                         dim3((unsigned)blocks), dim3(S::NW * 64), s, codes, code_stride, bq,
+                         b_stride, rps, red_len, part.as<int>(), rows_pad, kpt, rblk, sp,
+                         groups));
   *splits_out = sp;
   return SAIGE_OK;
 }
 
-template <int KT>
-static int int8_product_kt(saige_grm* h, int nsl, const double* d_v, size_t ldv, size_t k,
-                           double* d_out, size_t ldo, const double* d_w, const double* tau,
-                           size_t c0) {
-  constexpr int KP = 16 * KT;
-  constexpr int RT = 4;
-  constexpr int NW = KT <= 2 ? 4 : 8;
-  // few columns: the pass is bound by the genotype stream, so each row's chunk is a whole
-  // 128-byte line; more columns: 64-byte chunks keep the prefetch registers in budget
-  constexpr int KCH = KT <= 2 ? 512 : 256;
+static int i8_gemm_dispatch(int kt, saige_grm* h, const uint8_t* codes, size_t code_stride,
+                            const int8_t* bq, size_t b_stride, size_t rows_pad, size_t red_len,
+                            size_t groups, DevBuf& part, size_t* splits) {
+  switch (kt) {
+    case 1: return i8_gemm<1>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
+    case 2: return i8_gemm<2>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
+    case 3: return i8_gemm<3>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
+    case 4: return i8_gemm<4>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
+    case 5: return i8_gemm<5>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
+    case 6: return i8_gemm<6>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
+    default: return i8_gemm<7>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
+  }
+}
+
+// Column groups for `tiles` 16-column tiles: one group of all of them up to 7; beyond, the
+// group width (4 to 7 tiles) with the fewest executed tiles, then the fewest groups.
+static void i8_groups(size_t tiles, int* kt, size_t* groups) {
+  if (tiles <= 7) {
+    *kt = (int)tiles;
+    *groups = 1;
+    return;
+  }
+  size_t best_cost = SIZE_MAX, best_g = 0;
+  int best_kt = 7;
+  for (int t = 7; t >= 4; --t) {
+    const size_t g = ceil_div(tiles, (size_t)t);
+    const size_t cost = g * (size_t)t;
+    if (cost < best_cost || (cost == best_cost && g < best_g)) {
+      best_cost = cost;
+      best_g = g;
+      best_kt = t;
+    }
+  }
+  *kt = best_kt;
+  *groups = best_g;
+}
+
+// One launch set: k vectors, those in sm with a third slice (k3 of them).
+static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, const double* d_v,
+                               size_t ldv, size_t k, double* d_out, size_t ldo,
+                               const double* d_w, const double* tau, size_t c0) {
   GrmDevice& g = h->g;
   MixedState* ms = mstate(h);
   hipStream_t s = h->stream;
   const size_t n = g.n, m = g.m, n_pad = g.n_pad, m_pad = g.m_pad;
   const size_t nch_n = std::max<size_t>(1, ceil_div(n, kDotChunk));
   const size_t nch_m = std::max<size_t>(1, ceil_div(m, kDotChunk));
-  const unsigned pad_cols = (unsigned)(KP - nsl * k);
-  SAIGE_TRY(ms->vq.ensure(KP * n_pad));
-  SAIGE_TRY(ms->wq.ensure(KP * m_pad));
+  const size_t cols = 2 * k + k3;
+  int kt = 0;
+  size_t groups = 0;
+  i8_groups(ceil_div(cols, 16), &kt, &groups);
+  const size_t kpt = (size_t)16 * kt * groups;
+  const unsigned pad_cols = (unsigned)(kpt - cols);
+  SAIGE_TRY(ms->vq.ensure(kpt * n_pad));
+  SAIGE_TRY(ms->wq.ensure(kpt * m_pad));
   SAIGE_TRY(ms->vs.ensure(n * k * 8));
   SAIGE_TRY(ms->ws_.ensure(m * k * 8));
   SAIGE_TRY(ms->w.ensure(m * k * 8));
@@ -1072,42 +1400,48 @@ static int int8_product_kt(saige_grm* h, int nsl, const double* d_v, size_t ldv,
   double* pdot = pmin + std::max(nch_n, nch_m) * k;   // sums of the sliced values
   const dim3 gq_n((unsigned)ceil_div(n_pad, kDotChunk), (unsigned)k + pad_cols);
   const dim3 gq_m((unsigned)ceil_div(m_pad, kDotChunk), (unsigned)k + pad_cols);
+  const int er = epi_rows(kpt);
+  const size_t epi_lds = (size_t)er * (kpt + 1) * 4;
 
   // ---- V: column sums, maxima and minima, slices + sums of the sliced values ----
   col_stats_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n,
                                                                              psum, pmax, pmin);
-  quantize_slices_kernel<<<gq_n, kDotBlock, 0, s>>>(d_v, ldv, n, k, nsl, psum, pmax, pmin,
+  quantize_slices_kernel<<<gq_n, kDotBlock, 0, s>>>(d_v, ldv, n, k, sm, k3, psum, pmax, pmin,
                                                     1.0 / (double)n, nch_n, n_pad,
                                                     ms->vq.as<int8_t>(), ms->vs.as<double>(), n,
                                                     nullptr, pdot);
 
   // ---- stage 1: I = G^T Q_v (rows = markers, reduction over samples) ----
   size_t splits1 = 0;
-  SAIGE_TRY((i8_gemm<RT, KT, NW, KCH>(h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
-                                  n_pad, ms->ipart1, &splits1)));
-  i8_stage1_epilogue_kernel<KP><<<(unsigned)(m_pad / 64), 256, 0, s>>>(
-      ms->ipart1.as<int>(), splits1, m_pad, m, k, nsl, psum, pmax, pmin, pdot, 1.0 / (double)n, nch_n,
-      ms->sc.as<double>(), ms->p2.as<double>(), ms->zs.as<double>(), ms->w.as<double>(), m);
+  SAIGE_TRY(i8_gemm_dispatch(kt, h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
+                             n_pad, groups, ms->ipart1, &splits1));
+  i8_stage1_epilogue_kernel<<<(unsigned)(m_pad / er), 256, epi_lds, s>>>(
+      ms->ipart1.as<int>(), splits1, m_pad, m, k, kpt, sm, psum, pmax, pmin, pdot,
+      1.0 / (double)n, nch_n, ms->sc.as<double>(), ms->p2.as<double>(), ms->zs.as<double>(),
+      ms->w.as<double>(), m);
+  SAIGE_TRY(missing_stage1(h, ms->vs.as<double>(), n, k, nullptr, psum, nch_n, ms->w.as<double>(), m));
   LocoCodes lc;
   if (loco_codes(h, c0, k, lc)) loco_mask(h, ms->w.as<double>(), 1, m, m, k, lc, s);
 
   // ---- W: column sums over real markers, maxima and minima, slices + S2p partials ----
   col_stats_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
       ms->w.as<double>(), m, m, nch_m, psum, pmax, pmin);
-  quantize_slices_kernel<<<gq_m, kDotBlock, 0, s>>>(ms->w.as<double>(), m, m, k, nsl, psum, pmax,
-                                                    pmin, 1.0 / (double)m, nch_m, m_pad,
+  quantize_slices_kernel<<<gq_m, kDotBlock, 0, s>>>(ms->w.as<double>(), m, m, k, sm, k3, psum,
+                                                    pmax, pmin, 1.0 / (double)m, nch_m, m_pad,
                                                     ms->wq.as<int8_t>(), ms->ws_.as<double>(), m,
                                                     ms->p2.as<double>(), pdot);
 
   // ---- stage 2: I = G Q_w (rows = samples, reduction over markers) ----
   size_t splits2 = 0;
-  SAIGE_TRY((i8_gemm<RT, KT, NW, KCH>(h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
-                                  m_pad, n_pad, m_pad, ms->ipart2, &splits2)));
-  i8_stage2_epilogue_kernel<KP><<<(unsigned)(n_pad / 64), 256, 0, s>>>(
-      ms->ipart2.as<int>(), splits2, n_pad, n, k, nsl, psum, pmax, pmin, pdot, 1.0 / (double)m, nch_m,
-      ms->rsum.as<double>(), ms->P2, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0,
-      tau ? tau[1] : 0.0);
+  SAIGE_TRY(i8_gemm_dispatch(kt, h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
+                             m_pad, n_pad, m_pad, groups, ms->ipart2, &splits2));
+  i8_stage2_epilogue_kernel<<<(unsigned)(n_pad / er), 256, epi_lds, s>>>(
+      ms->ipart2.as<int>(), splits2, n_pad, n, k, kpt, sm, psum, pmax, pmin, pdot,
+      1.0 / (double)m, nch_m, ms->rsum.as<double>(), ms->P2, d_out, ldo, d_v, ldv, d_w,
+      tau ? tau[0] : 0.0, tau ? tau[1] : 0.0);
   SAIGE_HIP(hipGetLastError());
+  SAIGE_TRY(missing_stage2(h, ms->ws_.as<double>(), m, k, nullptr, psum, nch_m, d_out, ldo,
+                           (tau ? tau[1] : 1.0) / (double)m));
   h->products += k;
   return SAIGE_OK;
 }
@@ -1117,23 +1451,27 @@ int grm_int8_product(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
   SAIGE_ARG(h->g.n_pad <= kI8MaxRed && h->g.m_pad <= kI8MaxRed,
             "SAIGE_PREC_INT8 and SAIGE_PREC_INT8X2 need n and M <= 8,000,000 (exact int32 "
             "accumulation)");
-  const int nsl = h->precision == SAIGE_PREC_INT8X2 ? 2 : 3;
-  const size_t max_vec = 16 * kI8MaxKT / nsl;  // 37 vectors per launch with 3 slices, 56 with 2
-  for (size_t c0 = 0; c0 < k; c0 += max_vec) {
-    const size_t kc = std::min(max_vec, k - c0);
-    const double* v = d_v + c0 * ldv;
-    double* o = d_out + c0 * ldo;
-    int rc;
-    switch (ceil_div(nsl * kc, 16)) {
-      case 1: rc = int8_product_kt<1>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 2: rc = int8_product_kt<2>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 3: rc = int8_product_kt<3>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 4: rc = int8_product_kt<4>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 5: rc = int8_product_kt<5>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      case 6: rc = int8_product_kt<6>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
-      default: rc = int8_product_kt<7>(h, nsl, v, ldv, kc, o, ldo, d_w, tau, c0); break;
+  const int base = h->precision == SAIGE_PREC_INT8X2 ? 2 : 3;
+  auto slices = [&](size_t c) {
+    const int want = h->slice_req ? (int)h->slice_req[c] : 0;
+    return std::max(base, want >= 3 ? 3 : 2);
+  };
+  // launches of at most kI8MaxVec vectors and kI8MaxCols slice columns
+  for (size_t c0 = 0; c0 < k;) {
+    SliceMap sm{};
+    size_t kc = 0, k3 = 0;
+    while (c0 + kc < k && kc < (size_t)kI8MaxVec) {
+      const int sl = slices(c0 + kc);
+      if (2 * (kc + 1) + k3 + (sl == 3) > (size_t)kI8MaxCols) break;
+      if (sl == 3) {
+        sm.bits[kc >> 5] |= 1u << (kc & 31);
+        ++k3;
+      }
+      ++kc;
     }
-    if (rc != SAIGE_OK) return rc;
+    SAIGE_TRY(int8_product_launch(h, sm, k3, d_v + c0 * ldv, ldv, kc, d_out + c0 * ldo, ldo, d_w,
+                                  tau, c0));
+    c0 += kc;
   }
   return SAIGE_OK;
 }

@@ -248,7 +248,8 @@ __global__ void pcg_final(size_t k, uint32_t max_iter, ColState* __restrict__ st
 
 int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double* d_w,
                      const double tau[2], const double* d_b, size_t ldb, size_t k, double tol,
-                     uint32_t max_iter, double* d_x, size_t ldx, saige_pcg_info* info) {
+                     uint32_t max_iter, double* d_x, size_t ldx, saige_pcg_info* info,
+                     int slices) {
   if (k == 0) return SAIGE_OK;
   SAIGE_ARG(k <= 1024, "k <= 1024 right-hand sides per solve");
   const size_t n = h->g.n;
@@ -287,13 +288,13 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   int n_active = list[0];
   for (uint32_t iter = 0; iter < max_iter && n_active > 0; ++iter) {
     if ((size_t)n_active == k) {
-      SAIGE_TRY(sink->product(P, ld, k, AP, ld, d_w, tau));
+      SAIGE_TRY(sink->product(P, ld, k, AP, ld, d_w, tau, slices));
     } else {
       // only the columns still iterating: their results do not depend on which other
       // columns share the product
       const dim3 gg((unsigned)ceil_div(n, 256), (unsigned)n_active);
       pcg_gather_kernel<<<gg, 256, 0, s>>>(P, ld, n, d_nact + 1, 0, PC);
-      SAIGE_TRY(sink->product(PC, ld, (size_t)n_active, APC, ld, d_w, tau));
+      SAIGE_TRY(sink->product(PC, ld, (size_t)n_active, APC, ld, d_w, tau, slices));
       pcg_gather_kernel<<<gg, 256, 0, s>>>(APC, ld, n, d_nact + 1, 1, AP);
     }
     pcg_pap_kernel<<<gchunk, kDotBlock, 0, s>>>(P, AP, ld, n, nchunks, st, pap_part);

@@ -297,7 +297,14 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
     SAIGE_HIP(hipGetLastError());
   }
   const double* U = RHS1 + (1 + p) * ld;
-  if (R > 0) SAIGE_TRY(sink->product(U, ld, R, GU, ld, nullptr, nullptr));  // constant GRM * u
+  // operand slices of the int8x2 form (the other forms ignore the request): three for every
+  // product whose result enters the tau update directly or is computed once (GRM u, GRM P y,
+  // the second-stage and variance-ratio solves); two for the main block solves while the
+  // relative tau change is above kFineSwitch, three from then on
+  constexpr int kFine = 3;
+  constexpr double kFineSwitch = 1e-2;
+  bool fine = h->precision != SAIGE_PREC_INT8X2;
+  if (R > 0) SAIGE_TRY(sink->product(U, ld, R, GU, ld, nullptr, nullptr, kFine));  // constant GRM * u
 
   // eta, mu from the GLM fit
   SAIGE_HIP(hipMemcpyAsync(COEF, alpha.data(), p * 8, hipMemcpyHostToDevice, s));
@@ -318,8 +325,9 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
     n_iter = iter + 1;
     reml_prep_kernel<<<grid1(n), 256, 0, s>>>(dY, dEta, dMu, trait, n, dW, RHS1);
     SAIGE_HIP(hipGetLastError());
+    const bool fine_iter = fine;
     SAIGE_TRY(pcg_sigma_device(h, sink, st.pcg, dW, tau, RHS1, ld, k1, cfg->pcg_tol,
-                               cfg->pcg_max_iter, SOL1, ld, info.data()));
+                               cfg->pcg_max_iter, SOL1, ld, info.data(), fine_iter ? kFine : 0));
     for (size_t c = 0; c < k1; ++c) {
       pcg_iters += info[c].iterations;
       pcg_fail += info[c].converged ? 0 : 1;
@@ -349,7 +357,7 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
     // P y = Sigma^-1 y - Sigma^-1 X alpha (ai_reml.rs:221-226)
     project_kernel<<<grid2(n, 1), 256, 0, s>>>(SIY, ld, SIX, ld, p, COEF, n, 1, PY, ld);
     SAIGE_HIP(hipGetLastError());
-    SAIGE_TRY(sink->product(PY, ld, 1, GPY, ld, nullptr, nullptr));
+    SAIGE_TRY(sink->product(PY, ld, 1, GPY, ld, nullptr, nullptr, kFine));
     // y'P GRM P y and the trace terms (GRM u)'P u in one read (dres[0], dres[1..R])
     double* DRES = st.dres.as<double>();
     SAIGE_TRY(col_dots_dev(s, PY, ld, GPY, ld, n, 1, DRES, st.part));
@@ -377,7 +385,7 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
       SAIGE_HIP(hipGetLastError());
     }
     SAIGE_TRY(pcg_sigma_device(h, sink, st.pcg, dW, tau, RHS2, ld, k2, cfg->pcg_tol,
-                               cfg->pcg_max_iter, SOL2, ld, info.data()));
+                               cfg->pcg_max_iter, SOL2, ld, info.data(), kFine));
     for (size_t c = 0; c < k2; ++c) {
       pcg_iters += info[c].iterations;
       pcg_fail += info[c].converged ? 0 : 1;
@@ -454,7 +462,9 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
     }
     tau[0] = tn[0];
     tau[1] = tn[1];
-    if (max_change < cfg->tol) {
+    if (max_change < kFineSwitch) fine = true;
+    // a stop reached on two-slice block solves is confirmed by one more iteration on three
+    if (max_change < cfg->tol && fine_iter) {
       converged = true;
       break;
     }
@@ -728,7 +738,7 @@ static int estimate_vr_dev(saige_grm* h, ProductSink* sink, const VrSource& src,
     SAIGE_HIP(hipGetLastError());
     std::vector<saige_pcg_info> info(k);
     SAIGE_TRY(pcg_sigma_device(h, sink, st.pcg, d_w, tau, GT.as<double>(), ld, k, 1e-5, 500,
-                               SG.as<double>(), ld, info.data()));
+                               SG.as<double>(), ld, info.data(), 3));
     SAIGE_TRY(col_dots(s, GT.as<double>(), ld, SG.as<double>(), ld, n, k, v1.data(), st.part, st.res));
     SAIGE_HIP(hipMemcpyAsync(acb.p, acs.data(), k * 8, hipMemcpyHostToDevice, s));
     var2_weight_kernel<<<grid2(n, k), 256, 0, s>>>(GT.as<double>(), ld, d_mu, trait, acb.as<double>(),
@@ -1050,6 +1060,7 @@ class ProductBatcher {
     const double* w;
     double tau[2];
     int loco = -1;  // chromosome left out of this request's GRM (-1: none)
+    int slices = 0; // int8 operand slices asked for (0: the handle's own form)
     int rc = SAIGE_OK;
     bool done = false;
   };
@@ -1084,16 +1095,21 @@ class ProductBatcher {
   }
   int run(const std::vector<Request*>& reqs) {
     ++joined_products;
-    // per-column left-out chromosomes of the joined product
+    // per-column left-out chromosomes and operand slices of the joined product
     std::vector<int> codes;
-    bool loco = false;
+    std::vector<uint8_t> sl;
+    bool loco = false, slices = false;
     for (const Request* q : reqs) {
       codes.insert(codes.end(), q->k, q->loco);
+      sl.insert(sl.end(), q->k, (uint8_t)q->slices);
       loco = loco || q->loco >= 0;
+      slices = slices || q->slices != 0;
     }
     h_->loco_req = loco ? codes.data() : nullptr;
+    h_->slice_req = slices ? sl.data() : nullptr;
     const int rc = run_joined(reqs);
     h_->loco_req = nullptr;
+    h_->slice_req = nullptr;
     return rc;
   }
   int run_joined(const std::vector<Request*>& reqs) {
@@ -1141,10 +1157,11 @@ struct BatchedSink : ProductSink {
   int loco = -1;
   explicit BatchedSink(ProductBatcher* bb) : b(bb) {}
   int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
-              const double* tau) override {
+              const double* tau, int slices = 0) override {
     if (k == 0) return SAIGE_OK;
     columns += k;
     ProductBatcher::Request r;
+    r.slices = slices;
     r.v = v;
     r.ldv = ldv;
     r.k = k;

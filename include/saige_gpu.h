@@ -38,7 +38,8 @@ typedef enum {
   SAIGE_ERR_NO_MARKERS = 6      /* no marker passes QC */
 } saige_status;
 
-enum { SAIGE_PREC_FP64 = 0, SAIGE_PREC_MIXED = 1, SAIGE_PREC_INT8 = 2, SAIGE_PREC_INT8X2 = 3 };
+enum { SAIGE_PREC_FP64 = 0, SAIGE_PREC_MIXED = 1, SAIGE_PREC_INT8 = 2, SAIGE_PREC_INT8X2 = 3,
+       SAIGE_PREC_F32 = 4 };
 enum { SAIGE_TRAIT_BINARY = 0, SAIGE_TRAIT_QUANTITATIVE = 1, SAIGE_TRAIT_SURVIVAL = 2 };
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -83,8 +84,15 @@ int saige_plink_decode(int device, const uint8_t* bed_body, size_t bytes_per_mar
  *   factored out, the centred v split into three int8 slices with per-column scales
  *   (|v - split| <= 6.1e-8 max|v - mean|), exact int32 accumulation; n, M <= 8,000,000.
  *   precision SAIGE_PREC_INT8X2: as SAIGE_PREC_INT8 with two int8 slices
- *   (|v - split| <= 1.55e-5 max|v - mean|): 2/3 of the MFMA work.
- *   The reduced-precision modes refuse markers with missing calls.
+ *   (|v - split| <= 1.55e-5 max|v - mean|): 2/3 of the MFMA work.  The fits take three
+ *   slices for the narrow solves and, once the relative tau change is below 1e-2, for the
+ *   main block solves too.
+ *   The factored modes correct markers with missing calls by sparse per-marker and
+ *   per-sample lists (mean imputation as pcg.rs:108-113).
+ *   precision SAIGE_PREC_F32: saige-rs's own arithmetic (pcg.rs:128-180 at one rayon
+ *   thread): f32 standardized values, f32 sums in sample order per marker and in marker
+ *   order per sample, no contraction; each product bit-identical to the reference's.
+ *   No LOCO products in this mode.
  * ---------------------------------------------------------------------------- */
 typedef struct saige_grm saige_grm;
 

@@ -123,6 +123,9 @@ def _declare(L):
     L.orc_fit_ai_reml.argtypes = [C.c_void_p, _dp, _dp, C.c_size_t, C.c_size_t, C.c_int,
                                   C.POINTER(RemlConfig), C.POINTER(RemlResult), _dp, _dp, _dp,
                                   _dp, _dp, _dp]
+    L.orc_fit_ai_reml_probes.argtypes = [C.c_void_p, _dp, _dp, C.c_size_t, C.c_size_t, C.c_int,
+                                         C.POINTER(RemlConfig), C.POINTER(RemlResult), _dp, _dp,
+                                         _dp, _dp, _dp, _dp, _dp]
     L.orc_xvx_inv_xv.argtypes = [_dp, C.c_size_t, C.c_size_t, _dp, _dp]
     L.orc_cal_cv.argtypes = [_dp, C.c_size_t]
     L.orc_cal_cv.restype = C.c_double
@@ -287,19 +290,27 @@ def pcg_solve_dense(a, b, tol=1e-5, max_iter=500):
 
 
 def fit_ai_reml(grm, y, x_cm, trait=TRAIT_BINARY, max_iter=30, tol=1e-5, pcg_tol=1e-5,
-                pcg_max_iter=500, n_random_vectors=30, seed=12345):
-    """ai_reml.rs:86-427.  ``x_cm`` is (p, n): row j = covariate column j."""
+                pcg_max_iter=500, n_random_vectors=30, seed=12345, probes=None):
+    """ai_reml.rs:86-427.  ``x_cm`` is (p, n): row j = covariate column j.  ``probes``
+    ((R, n), test use only) replaces the ChaCha8 Rademacher probes, e.g. sqrt(n) times the
+    unit vectors for the exact trace."""
     y = _f64(y)
     x = _f64(x_cm)
     p, n = x.shape
+    if probes is not None:
+        probes = _f64(probes)
+        n_random_vectors = probes.shape[0]
     cfg = RemlConfig(max_iter, tol, pcg_tol, pcg_max_iter, n_random_vectors, seed)
     res = RemlResult()
     mu, w, r, eta = (np.empty(n) for _ in range(4))
     alpha = np.empty(p)
     trace = np.full(2 * max_iter, np.nan)
-    rc = lib().orc_fit_ai_reml(grm.h, _p(y, _dp), _p(x, _dp), n, p, trait, C.byref(cfg),
-                               C.byref(res), _p(mu, _dp), _p(w, _dp), _p(r, _dp), _p(eta, _dp),
-                               _p(alpha, _dp), _p(trace, _dp))
+    args = (grm.h, _p(y, _dp), _p(x, _dp), n, p, trait, C.byref(cfg), C.byref(res), _p(mu, _dp),
+            _p(w, _dp), _p(r, _dp), _p(eta, _dp), _p(alpha, _dp), _p(trace, _dp))
+    if probes is None:
+        rc = lib().orc_fit_ai_reml(*args)
+    else:
+        rc = lib().orc_fit_ai_reml_probes(*args, _p(probes, _dp))
     if rc != 0:
         raise RuntimeError(f"orc_fit_ai_reml failed: {rc}")
     return dict(tau=np.array(res.tau[:]), iterations=res.iterations, converged=bool(res.converged),

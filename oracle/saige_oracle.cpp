@@ -541,9 +541,13 @@ struct OrcRemlResult {
   double tau[2]; int iterations; int converged; double log_likelihood;
   long long grm_products; long long pcg_iterations;
 };
-int orc_fit_ai_reml(void* grm_h, const double* y, const double* x, size_t n, size_t p, int trait,
-                    const OrcRemlConfig* cfg, OrcRemlResult* res, double* mu_out, double* w_out,
-                    double* res_out, double* eta_out, double* alpha_out, double* tau_trace) {
+// probes: R x n probe vectors (probe r at [r*n]) in place of the ChaCha8 Rademacher draws, or
+// null for the reference's draws.  Test infrastructure only: sqrt(n) e_1 .. sqrt(n) e_n make the
+// Hutchinson mean the exact trace, which pins the iteration against the REML optimum.
+static int fit_ai_reml_impl(void* grm_h, const double* y, const double* x, size_t n, size_t p,
+                            int trait, const OrcRemlConfig* cfg, OrcRemlResult* res,
+                            double* mu_out, double* w_out, double* res_out, double* eta_out,
+                            double* alpha_out, double* tau_trace, const double* probes) {
   const Grm* grm = (const Grm*)grm_h;
   bool fix_tau_e = (trait == 0);
   double tau[2] = {1.0, 0.0};
@@ -554,7 +558,8 @@ int orc_fit_ai_reml(void* grm_h, const double* y, const double* x, size_t n, siz
 
   size_t R = (size_t)cfg->n_random_vectors;
   std::vector<double> rv(R * n);
-  orc_rademacher(cfg->seed, n, R, rv.data());
+  if (probes) std::copy(probes, probes + R * n, rv.begin());
+  else orc_rademacher(cfg->seed, n, R, rv.data());
 
   long long products = 0, pcg_iters = 0;
   bool converged = false;
@@ -699,6 +704,20 @@ int orc_fit_ai_reml(void* grm_h, const double* y, const double* x, size_t n, siz
   }
   if (alpha_out) for (size_t j = 0; j < p; ++j) alpha_out[j] = alpha[j];
   return 0;
+}
+
+int orc_fit_ai_reml(void* grm_h, const double* y, const double* x, size_t n, size_t p, int trait,
+                    const OrcRemlConfig* cfg, OrcRemlResult* res, double* mu_out, double* w_out,
+                    double* res_out, double* eta_out, double* alpha_out, double* tau_trace) {
+  return fit_ai_reml_impl(grm_h, y, x, n, p, trait, cfg, res, mu_out, w_out, res_out, eta_out,
+                          alpha_out, tau_trace, nullptr);
+}
+int orc_fit_ai_reml_probes(void* grm_h, const double* y, const double* x, size_t n, size_t p,
+                           int trait, const OrcRemlConfig* cfg, OrcRemlResult* res,
+                           double* mu_out, double* w_out, double* res_out, double* eta_out,
+                           double* alpha_out, double* tau_trace, const double* probes) {
+  return fit_ai_reml_impl(grm_h, y, x, n, p, trait, cfg, res, mu_out, w_out, res_out, eta_out,
+                          alpha_out, tau_trace, probes);
 }
 
 // --- XVX^{-1} X'V  saige-cli/src/commands/fit_null.rs:289-307.

@@ -284,6 +284,58 @@ def test_ai_reml_trajectory_vs_dense(trait):
     np.testing.assert_allclose(r["tau_trace"][:k], traj[:k], rtol=1e-7, atol=1e-10)
 
 
+def _reml_neg_loglik(tau, K, y, X):
+    """Minus the restricted log-likelihood of y ~ N(X b, tau_e I + tau_g K), up to a constant:
+    (log|S| + log|X'S^-1 X| + y'P y) / 2 with P = S^-1 - S^-1 X (X'S^-1 X)^-1 X'S^-1 (Patterson
+    and Thompson 1971), evaluated through Cholesky factors."""
+    te, tg = tau
+    if te <= 0 or tg < 0:
+        return np.inf
+    n = len(y)
+    S = te * np.eye(n) + tg * K
+    L = np.linalg.cholesky(S)
+    Si_X = np.linalg.solve(S, X)
+    Si_y = np.linalg.solve(S, y)
+    XSX = X.T @ Si_X
+    Lx = np.linalg.cholesky(XSX)
+    b = np.linalg.solve(XSX, X.T @ Si_y)
+    ypy = y @ Si_y - (X.T @ Si_y) @ b
+    return np.sum(np.log(np.diag(L))) + np.sum(np.log(np.diag(Lx))) + 0.5 * ypy
+
+
+def test_ai_reml_converges_to_reml_optimum_exact_trace():
+    """Pins the quantitative AI-REML iteration (ai_reml.rs:135-384: the score terms y'PKPy and
+    y'PPy, the Hutchinson traces, the 2 x 2 average-information step, the projections) by the
+    mathematics it solves: with probes sqrt(n) e_1 .. sqrt(n) e_n the Hutchinson means are the
+    exact traces tr(PK) and tr(P), so the converged tau must be the maximiser of the restricted
+    likelihood, which scipy finds here from the likelihood values alone (no score formula)."""
+    from scipy.optimize import minimize
+    n, m = 150, 400
+    d, af = _grm_case(n=n, m=m, seed=41, p_missing=0.0)
+    K = _dense_grm(d, af)
+    rng = np.random.default_rng(5)
+    X = np.column_stack([np.ones(n), rng.standard_normal(n)])
+    g = np.linalg.cholesky(K + 1e-9 * np.eye(n)) @ rng.standard_normal(n)
+    y = X @ np.array([0.5, -0.3]) + 0.8 * g + 0.6 * rng.standard_normal(n)
+    grm = oracle.Grm(d, af, oracle.GRM_F64)
+    r = oracle.fit_ai_reml(grm, y, X.T, trait=oracle.TRAIT_QUANTITATIVE, max_iter=200,
+                           tol=1e-12, pcg_tol=1e-13, pcg_max_iter=5000,
+                           probes=np.sqrt(n) * np.eye(n))
+    assert r["converged"]
+    opt = minimize(_reml_neg_loglik, x0=np.array([0.5, 0.5]), args=(K, y, X),
+                   method="Nelder-Mead",
+                   options=dict(xatol=1e-12, fatol=1e-15, maxiter=20000, maxfev=40000))
+    assert opt.success
+    np.testing.assert_allclose(r["tau"], opt.x, rtol=1e-6)
+    # the optimum is interior and not the start: both components moved
+    assert opt.x[1] > 0.05 and abs(opt.x[0] - 1.0) > 0.05
+    # a plausible mistake moves the fixed point: with the Rademacher probes (an estimated
+    # trace) the converged tau is off the optimum by far more than the 1e-6 bar
+    r30 = oracle.fit_ai_reml(grm, y, X.T, trait=oracle.TRAIT_QUANTITATIVE, max_iter=200,
+                             tol=1e-12, pcg_tol=1e-13, pcg_max_iter=5000)
+    assert np.abs(r30["tau"] - opt.x).max() / opt.x.max() > 1e-4
+
+
 def test_ai_reml_identity_grm_kat():
     # ai_reml.rs:541-591: quantitative, identity GRM, tau > 0, finite negative loglik
     n = 20
