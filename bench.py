@@ -414,7 +414,7 @@ def run_learned(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-fam", type=int, default=10000)
     ap.add_argument("--markers", type=int, default=100000)
@@ -439,11 +439,18 @@ def main():
                     help="HBM bytes per GEMM launch measured with rocprofv3 counters")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (counter calibration)")
-    ap.add_argument("--workload", default="families",
-                    choices=["families", "biobank", "learned", "loco", "region"],
-                    help="families: the default fit-null benchmark; biobank: several "
-                         "phenotypes at 400,000 x 150,000; learned: the SPA root proposal; "
-                         "loco: leave-one-chromosome-out refits")
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "families", "biobank", "learned", "loco", "region"],
+                    help="headline (default): 4 traits at 400,000 x 150,000 fitted together "
+                         "plus step 2 over 1,000,000 variants; families: one trait at 50,000 x "
+                         "100,000; biobank: the 4-trait fit alone; learned: the SPA root "
+                         "proposal; loco: leave-one-chromosome-out refits; region: region tests")
+    ap.add_argument("--step2-variants", type=int, default=1000000,
+                    help="headline: step-2 variants streamed (generated on the device)")
+    ap.add_argument("--cpu-fit-markers", type=int, default=500,
+                    help="headline: markers of the CPU baseline's sampled f32 GRM product")
+    ap.add_argument("--no-families", action="store_true",
+                    help="headline: skip the families extra keys")
     ap.add_argument("--ls-n", type=int, default=50000)
     ap.add_argument("--ls-variants", type=int, default=524288,
                     help="learned: variants per prevalence for training (2 roots each)")
@@ -478,7 +485,15 @@ def main():
     if args.workload == "region":
         run_region(args)
         return
+    if args.workload == "families":
+        run_families(args)
+        return
+    run_headline(args)
 
+
+def run_families(args):
+    """Fit-null of the families cohort (50,000 samples x 100,000 markers, one binary trait):
+    int8 MFMA forms, bf16 and FP64 timed, one JSON line."""
     import csaige_amd as sg
     from csaige_amd import hip, synth
 
@@ -755,6 +770,367 @@ def main():
         "step2": step2,
     }
     print(json.dumps(line), flush=True)
+
+
+# HBM bytes per launch of the dominant genotype GEMM at the 4-trait 400,000 x 150,000 fit,
+# measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (one counter per run; FETCH_SIZE scaled
+# by the factor a 1 GiB device copy gives, 2.0): keyed by (n, markers, form, operand width).
+MEASURED_TRAFFIC_BIOBANK = {}
+
+
+def step2_stream(args, sg, hip, synth):
+    """Step 2 over all of the workload's variants: n = 400,000, intercept-only null with
+    mu = logistic(N(-4.6, 0.5)), VR 1.05, SPA where |score/sd| > 2, 4,096-variant uint8
+    batches generated on the device (the seeded generator's device twin) and scored as they
+    come.  Returns the step2 dict and the CPU baseline of step 2."""
+    n2, V, B = args.step2_n, args.step2_variants, 4096
+    nf = synth.null_fit(n=n2, p=1, eta_mean=-4.6, eta_sd=0.5, seed=3)
+    a = sg.xvx_inv_xv(nf.x, nf.mu * (1 - nf.mu))
+    cutoff = 0.04550026389635842  # 2 * (1 - Phi(2)): SPA where |score/sd| > 2
+    eng = sg.ScoreTestEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
+                             spa_pval_cutoff=cutoff)
+    mafs = synth.dosage_mafs(0, V, seed=4)
+    d_m = hip.DeviceBuffer(mafs.nbytes)
+    d_m.upload(mafs)
+    d_q = [hip.DeviceBuffer(B * n2), hip.DeviceBuffer(B * n2)]
+    rsize = C.sizeof(sg._lib.VariantResult)
+    d_r = hip.DeviceBuffer(V * rsize)
+    d_w = hip.DeviceBuffer(B * rsize)
+    es = eng.stream
+    nb = -(-V // B)
+    sizes = sorted({min(B, V - b * B) for b in range(nb)})
+    for nv in sizes:  # first use of each batch size tunes the sums kernel's shape: untimed
+        synth.dosage_u8_device(n2, 0, nv, d_m.value, d_q[0].value, seed=4, stream=es)
+        eng.test_markers_u8_device(d_q[0].value, n2, nv, 1.0 / 127, d_w.value)
+    hip.device_synchronize()
+    log(f"step2: {V} variants at n={n2} in {nb} batches of {B}, generated on the device")
+    ev = [(hip.Event(), hip.Event(), hip.Event()) for _ in range(nb)]
+    t = time.perf_counter()
+    for b in range(nb):
+        first = b * B
+        nv = min(B, V - first)
+        g0, g1, s1 = ev[b]
+        g0.record(es)
+        synth.dosage_u8_device(n2, first, nv, d_m.value + 8 * first, d_q[b % 2].value, seed=4,
+                               stream=es)
+        g1.record(es)
+        eng.test_markers_u8_device(d_q[b % 2].value, n2, nv, 1.0 / 127,
+                                   d_r.value + first * rsize)
+        s1.record(es)
+    hip.device_synchronize()
+    wall = time.perf_counter() - t
+    gen_ms = sum(g0.elapsed_ms(g1) for g0, g1, _ in ev)
+    score_ms = sum(g1.elapsed_ms(s1) for _, g1, s1 in ev)
+    host = np.empty(V * rsize, np.uint8)
+    d_r.download(host)
+    out = (sg._lib.VariantResult * V).from_buffer(host)
+    r = sg.results_to_dict(out)
+    vps = V / (score_ms * 1e-3)
+    step2 = dict(n=n2, variants=V, batches=nb, batch=B, ms=score_ms, variants_per_s=vps,
+                 generation_ms=gen_ms, wall_s_with_generation=wall,
+                 spa_fraction=float(r["is_spa"].mean()),
+                 dosage_gbs=V * n2 / (score_ms * 1e-3) / 1e9,
+                 timing="HIP events on the engine stream around each batch's score-test "
+                        "launches (generation timed apart)")
+    log(f"step2: {V} variants scored in {score_ms:.1f} ms -> {vps:.3g} variants/s "
+        f"({step2['dosage_gbs']:.0f} GB/s of dosages); generation {gen_ms:.1f} ms; SPA'd "
+        f"{r['is_spa'].mean() * 100:.2f}%")
+    cpu2 = None
+    if not args.no_cpu:
+        # the oracle's score test + fast SPA (saige-rs single_variant.rs / fast.rs restated) on
+        # one core, as saige-rs runs step 2 (assoc_test.rs:165), on the first variants; its
+        # p-values also check the GPU's
+        import oracle
+        nv = args.cpu_step2_variants
+        q_cpu, _ = synth.dosage_u8(n2, 0, nv, seed=4)
+        eo = oracle.ScoreEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
+                                spa_cutoff=cutoff)
+        t = time.perf_counter()
+        want = eo.test_batch_u8(q_cpu, 1.0 / 127)
+        dt = time.perf_counter() - t
+        got_p = r["pvalue"][:nv]
+        finite = want["pvalue"] >= 1e-300
+        dlp = float(np.max(np.abs(np.log10(got_p[finite]) - np.log10(want["pvalue"][finite]))))
+        step2["max_abs_dlog10p_vs_oracle"] = dlp
+        step2["is_spa_mismatches_vs_oracle"] = int(np.sum(r["is_spa"][:nv] != want["is_spa"]))
+        step2["cpu_variants_per_s"] = nv / dt
+        cpu2 = {"value": nv / dt, "unit": "variants/s", "cores": 1, "kind": "port",
+                "sample": f"oracle score test + fast SPA of the first {nv} of the {V} variants "
+                          f"(n={n2}), single-threaded as saige-rs runs step 2 "
+                          f"(assoc_test.rs:165); {dt:.1f} s"}
+        log(f"step2 cpu: {nv} variants in {dt:.2f} s on 1 thread -> {nv / dt:.1f} variants/s; "
+            f"GPU vs oracle max |d log10 p| = {dlp:.2e}")
+    eng.close()
+    return step2, cpu2
+
+
+def run_headline(args):
+    """The default line.  Fit-null of 4 binary traits (prevalence 0.5/1/5/20%) over 400,000
+    samples (15% in half-sibling pairs) x 150,000 markers resident in HBM, the traits fitted
+    together with joined GRM products: the int8 MFMA form with two operand slices (three for
+    the narrow solves and near convergence) is `value`; the three-slice form and the plain
+    FP64 path are timed beside it on the same cohort, and each form's tau, VR and step-2
+    p-values are compared with FP64's.  Then step 2 over 1,000,000 variants, the roofline of
+    the dominant GEMM launch and the CPU baselines."""
+    import csaige_amd as sg
+    from csaige_amd import hip, synth
+    t0 = time.time()
+    prev = (0.005, 0.01, 0.05, 0.20)
+    log(f"generating biobank cohort n={args.bb_n} m={args.bb_m}")
+    c = synth.biobank(n=args.bb_n, m=args.bb_m, frac_related=0.15, prevalences=prev, seed=5)
+    n, m = c.n, c.bed.shape[0]
+    log(f"cohort generated in {time.time() - t0:.1f}s: {c.bed.nbytes / 1e9:.2f} GB packed, "
+        f"cases {c.y.sum(axis=1).astype(int).tolist()}")
+    idx = np.arange(n, dtype=np.uint32)
+    d_bed = hip.DeviceBuffer(c.bed.nbytes)
+    d_bed.upload(c.bed)
+    hip.device_synchronize()
+
+    def fit(precision):
+        hip.device_synchronize()
+        t = time.perf_counter()
+        r = sg.fit_null_multi(None, c.bpm, n, m, c.y, c.x, sample_idx=idx, precision=precision,
+                              d_bed=d_bed.value)
+        hip.device_synchronize()
+        return time.perf_counter() - t, r
+
+    def describe(tag, dt, rr):
+        log(f"{tag}: {dt:.3f}s " + "; ".join(
+            f"t{t} tau={r['tau'][1]:.7g} it={r['iterations']} vr={r['vr']:.6g} "
+            f"pcg={r['pcg_iterations']}" for t, r in enumerate(rr)))
+
+    forms = {"int8": sg.PREC_INT8, "int8x2": sg.PREC_INT8X2}
+    head = args.precision
+    other = "int8x2" if head == "int8" else "int8"
+    for w in range(args.warmup):
+        dt, rr = fit(forms[head])
+        describe(f"warmup {w} {head}", dt, rr)
+    times = []
+    rm = None
+    for s in range(args.steps):
+        dt, rm = fit(forms[head])
+        times.append(dt)
+        describe(f"step {s} {head}", dt, rm)
+    head_s = float(np.mean(times))
+    # the dominant kernel, in an untimed repetition: every genotype GEMM launch of one fit is
+    # timed with HIP events on the stream it runs on (the library's launch log)
+    sg.launch_log_enable(True, 0)
+    dt_log, _ = fit(forms[head])
+    logsum = sg.launch_log_summary()
+    sg.launch_log_enable(False)
+    log(f"launch log ({dt_log:.3f}s fit): " + ", ".join(
+        f"w{w}: {v[1]} x {v[0] / max(v[1], 1):.3f} ms" for w, v in logsum.items()))
+    other_s, ro = fit(forms[other])
+    describe(f"{other}", other_s, ro)
+    fp64_s, rf = None, None
+    if not args.no_fp64:
+        fp64_s, rf = fit(sg.PREC_FP64)
+        describe("fp64", fp64_s, rf)
+
+    # ---- accuracy against FP64: tau, VR, step-2 -log10 p of 2,048 variants per trait ----
+    acc, valid = None, {head: True, other: True}
+    if rf is not None:
+        acc = {head: [], other: []}
+        ntest = 2048
+
+        def pvals(r, t):
+            e = sg.ScoreTestEngine(r["mu"], c.y[t] - r["mu"], r["tau"], c.x, r["xvx_inv_xv"],
+                                   r["vr"], y=c.y[t])
+            p = e.test_markers_plink(c.bed[:ntest], c.bpm, n, ntest)["pvalue"]
+            e.close()
+            return p
+
+        for t in range(len(prev)):
+            pf = pvals(rf[t], t)
+            mask = pf >= 1e-10
+            for tag, rr in ((head, rm), (other, ro)):
+                pm = pvals(rr[t], t)
+                a = {"tau_rel_diff": float(abs(rr[t]["tau"][1] - rf[t]["tau"][1]) / rf[t]["tau"][1]),
+                     "vr_rel_diff": float(abs(rr[t]["vr"] - rf[t]["vr"]) / rf[t]["vr"]),
+                     "max_abs_dlog10p": float(np.max(np.abs(np.log10(pf[mask]) -
+                                                            np.log10(pm[mask])))),
+                     "iterations": [int(rr[t]["iterations"]), int(rf[t]["iterations"])],
+                     "converged": bool(rr[t]["converged"])}
+                acc[tag].append(a)
+                valid[tag] = valid[tag] and (a["converged"] and a["tau_rel_diff"] <= 1e-4 and
+                                             a["max_abs_dlog10p"] <= 1e-3)
+        log(f"accuracy vs fp64 (targets tau 1e-4, -log10 p 1e-3): {acc}")
+    # the headline form must meet the mixed-precision targets; otherwise the three-slice form
+    # (or, failing that, nothing) carries the line
+    value, value_form, note = head_s, head, None
+    if not valid[head]:
+        if valid[other]:
+            value, value_form = other_s, other
+            note = f"{head} missed the accuracy targets in this run; {other} carries the value"
+        else:
+            note = "no int8 form met the accuracy targets in this run: value is not valid"
+        log("WARNING: " + note)
+
+    # ---- roofline of the dominant kernel: the genotype GEMM width with the most time ----
+    dom_w = max(logsum, key=lambda w: logsum[w][0]) if logsum else None
+    roof, kern = None, None
+    if dom_w is not None:
+        tot_ms, cnt, vecs = logsum[dom_w]
+        launch_ms = tot_ms / cnt
+        mg = int(rm[0]["n_grm_markers"])
+        launch_bytes = float((n + 3) // 4) * mg          # one pass over the packed genotypes
+        useful_ops = 2.0 * n * mg * vecs / cnt           # 2 n M k over the launches' real k
+        exec_ops = 2.0 * n * mg * dom_w                  # the padded slice columns it executes
+        gbs = launch_bytes / (launch_ms * 1e-3) / 1e9
+        all_ms = sum(v[0] for v in logsum.values())
+        kern = {"width": dom_w, "launches": cnt, "ms_per_launch": launch_ms,
+                "mean_vectors": vecs / cnt, "share_of_gemm_time": tot_ms / all_ms,
+                "useful_tops": useful_ops / (launch_ms * 1e-3) / 1e12,
+                "executed_tops": exec_ops / (launch_ms * 1e-3) / 1e12,
+                "per_width": {str(w): {"launches": v[1], "ms_per_launch": v[0] / max(v[1], 1),
+                                       "mean_vectors": v[2] / max(v[1], 1)}
+                              for w, v in logsum.items()}}
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_PEAK_GBS,
+                "traffic": MEASURED_TRAFFIC_BIOBANK.get((n, m, value_form, dom_w)),
+                "kernel": f"i8_gemm_kernel ({head}) at {dom_w} operand columns (mean "
+                          f"{vecs / cnt:.1f} vectors), the GEMM width with the most time in one "
+                          f"{head} fit ({tot_ms / all_ms * 100:.0f}% of its GEMM time): "
+                          f"{launch_ms:.3f} ms/launch over {cnt} launches (HIP events on the "
+                          f"fit's GRM stream, untimed repetition); algorithmic bytes "
+                          f"ceil(n/4)*M = {launch_bytes:.4g} B/launch; "
+                          f"{kern_useful(useful_ops, launch_ms)} useful, "
+                          f"{exec_ops / (launch_ms * 1e-3) / 1e12:.0f} TOP/s executed of "
+                          f"{I8_PEAK_TOPS:.0f}"}
+        log(f"dominant GEMM: width {dom_w}, {launch_ms:.3f} ms/launch, {gbs:.0f} GB/s "
+            f"({gbs / HBM_PEAK_GBS:.3f} of HBM)")
+
+    # ---- FP64 roofline floor: the FP64 fit's column-products at 78.6 TFLOP/s ----
+    fp64_floor = None
+    if rf is not None:
+        mg = int(rf[0]["n_grm_markers"])
+        cols = sum(int(r["grm_products"]) for r in rf)
+        fp64_floor = 4.0 * n * mg * cols / (FP64_PEAK_TFLOPS * 1e12)
+
+    # ---- CPU baseline of the fit: sampled f32 OnTheFlyGrm products, extrapolated ----
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        ref = rf if rf is not None else rm
+        mg = int(ref[0]["n_grm_markers"])
+        grm_markers = np.flatnonzero(ref[0]["marker_class"] == 1)[:args.cpu_fit_markers]
+        mc = len(grm_markers)
+        ac, nv, af, mac = sg.marker_stats(c.bed[grm_markers], c.bpm, n, mc)
+        dos = sg.decode(c.bed[grm_markers], c.bpm, n, mc)
+        g32 = oracle.Grm(dos, af, oracle.GRM_F32)
+        del dos
+        v = np.random.default_rng(1).standard_normal(n)
+
+        def per_product(threads, seconds):
+            t = time.perf_counter()
+            reps = 0
+            while time.perf_counter() - t < seconds or reps < 1:
+                g32.mat_vec_threads(v, threads)
+                reps += 1
+            return (time.perf_counter() - t) / reps * (mg / mc), reps
+
+        pt, reps_t = per_product(args.cpu_threads, 8.0)
+        p1, reps_1 = per_product(1, 4.0)
+        cols = sum(int(r["grm_products"]) for r in ref)
+        cpu = {"value": pt * cols, "unit": "s (fit-null of the 4 traits, extrapolated)",
+               "cores": args.cpu_threads, "kind": "port", "host": host_cpu(),
+               "sample": f"f32 OnTheFlyGrm products (saige-rs pcg.rs:128-180 restated, "
+                         f"rayon-shaped chunks) at n={n} x {mc} markers: {reps_t} products on "
+                         f"{args.cpu_threads} threads, {reps_1} on 1; scaled by {mg}/{mc} "
+                         f"markers and by the {cols} column-products the GPU fits of the 4 "
+                         f"traits performed (a lower bound: saige-rs also recomputes GRM u "
+                         f"every iteration and needs ~720 GB for its f32 copy of Z at this "
+                         f"size); one full product {pt:.2f} s on {args.cpu_threads} threads, "
+                         f"{p1:.1f} s on 1"}
+        log(f"cpu baseline: {pt:.3f} s/product on {args.cpu_threads} threads -> "
+            f"{pt * cols:.0f} s fit-null (extrapolated)")
+
+    # ---- step 2 over all variants ----
+    step2, cpu2 = (None, None)
+    if not args.no_step2:
+        d_bed.free()
+        step2, cpu2 = step2_stream(args, sg, hip, synth)
+        if cpu is not None and cpu2 is not None:
+            cpu["step2"] = cpu2
+
+    # ---- the one-trait families workload, as extra keys ----
+    fam = None
+    if not args.no_families:
+        fam = families_extra(args, sg, hip, synth, forms[head])
+
+    line = {
+        "metric": METRIC,
+        "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"int8 MFMA (exact genotypes x {'2-3' if value_form == 'int8x2' else '3'} int8 "
+                 "operand slices) / int32 accumulate / fp64 solver",
+        "data": "synthetic (seeded biobank cohort, packed genotypes resident in HBM; step-2 "
+                "dosages generated on the device)",
+        "config": {"workload": f"fit-null of {len(prev)} binary traits (prevalence "
+                               f"{'/'.join(str(p) for p in prev)}) fitted together, {n} samples "
+                               f"(15% in half-sib pairs), {m} markers, 30 probes per trait, "
+                               f"joined GRM products; step 2 over {args.step2_variants} "
+                               f"variants at n={args.step2_n}",
+                   "precision": value_form, "note": note,
+                   f"{head}_wall_s": head_s, f"{other}_wall_s": other_s,
+                   "fp64_wall_s": fp64_s,
+                   "speedup_vs_fp64": (fp64_s / value) if fp64_s else None,
+                   "fp64_roofline_floor_s": fp64_floor,
+                   "speedup_vs_fp64_roofline_floor": (fp64_floor / value) if fp64_floor else None,
+                   "accuracy_targets_met": valid if rf is not None else None,
+                   "tau_g": [float(r["tau"][1]) for r in rm],
+                   "ai_reml_iterations": [int(r["iterations"]) for r in rm],
+                   "pcg_iterations": [int(r["pcg_iterations"]) for r in rm],
+                   "grm_column_products": [int(r["grm_products"]) for r in rm],
+                   "variance_ratio": [float(r["vr"]) for r in rm],
+                   "accuracy_vs_fp64": acc, "step_times_s": times,
+                   "dominant_gemm": kern, "families": fam},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "step2": step2,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def kern_useful(ops, ms):
+    return f"{ops / (ms * 1e-3) / 1e12:.0f} TOP/s"
+
+
+def families_extra(args, sg, hip, synth, prec):
+    """The one-trait families workload (50,000 samples x 100,000 markers): the int8 form and
+    FP64, for comparison with earlier lines."""
+    c = synth.families(n_fam=args.n_fam, kids=3, m=args.markers, seed=2, h2=0.4, prevalence=0.05)
+    n, m = c.n, c.bed.shape[0]
+    idx = np.arange(n, dtype=np.uint32)
+    d_bed = hip.DeviceBuffer(c.bed.nbytes)
+    d_bed.upload(c.bed)
+
+    def one(p):
+        hip.device_synchronize()
+        t = time.perf_counter()
+        r = sg.fit_null(None, c.bpm, n, m, c.y, c.x, sample_idx=idx, precision=p,
+                        d_bed=d_bed.value)
+        hip.device_synchronize()
+        return time.perf_counter() - t, r
+
+    one(prec)
+    ts = []
+    for _ in range(3):
+        dt, r = one(prec)
+        ts.append(dt)
+    out = {"workload": f"fit-null binary, {args.n_fam} nuclear families x 5 = {n} samples, "
+                       f"{m} markers", "wall_s": float(np.mean(ts)),
+           "tau_g": float(r["tau"][1]), "ai_reml_iterations": int(r["iterations"])}
+    if not args.no_fp64:
+        one(sg.PREC_FP64)
+        dtf, rf = one(sg.PREC_FP64)
+        out.update(fp64_wall_s=dtf, speedup_vs_fp64=dtf / out["wall_s"],
+                   tau_rel_diff=float(abs(r["tau"][1] - rf["tau"][1]) / rf["tau"][1]),
+                   fp64_ai_reml_iterations=int(rf["iterations"]))
+    log(f"families: {out}")
+    d_bed.free()
+    return out
 
 
 if __name__ == "__main__":

@@ -27,7 +27,8 @@ from ._lib import (PREC_FP64, PREC_MIXED, PREC_INT8, PREC_INT8X2, PREC_F32, TRAI
 __all__ = ["OnTheFlyGrm", "fit_ai_reml", "solve_sigma", "xvx_inv_xv", "estimate_variance_ratio",
            "fit_null", "fit_null_multi", "ScoreTestEngine", "spa_binary_fast", "marker_stats", "decode",
            "PREC_FP64", "PREC_MIXED", "PREC_INT8", "PREC_INT8X2", "PREC_F32", "TRAIT_BINARY", "TRAIT_QUANTITATIVE", "TRAIT_SURVIVAL",
-           "SaigeError", "load", "results_to_dict", "SpaRootMlp", "spa_training_set"]
+           "SaigeError", "load", "results_to_dict", "SpaRootMlp", "spa_training_set",
+           "launch_log_enable", "launch_log_summary"]
 
 
 def _f64(a):
@@ -320,6 +321,27 @@ def fit_null_multi(bed_body, bytes_per_marker, n_total, m_total, Y, x_cm, traits
     else:
         check(L.saige_fit_null_multi_device(device, C.c_void_p(d_bed), *args))
     return [_fit_null_dict(res[t], mu[t], a[t], cls, alpha[t], pm[t]) for t in range(T)]
+
+
+def launch_log_enable(on=True, width=0):
+    """Start (clearing it) or stop the process-wide launch log of the genotype GEMM launches;
+    with width != 0 only launches of that operand width are recorded (diagnostic)."""
+    check(load().saige_launch_log_enable(int(bool(on)), int(width)))
+
+
+def launch_log_summary(cap=64):
+    """Launches recorded since the last read, per operand width: {width: (total_ms, launches,
+    vectors)}; clears the log (diagnostic)."""
+    w = np.zeros(cap, np.uint32)
+    ms = np.zeros(cap)
+    cnt = np.zeros(cap, np.uint64)
+    vec = np.zeros(cap, np.uint64)
+    nw = C.c_size_t()
+    check(load().saige_launch_log_summary(_p(w, _lib.u32p), _p(ms, _lib.dp),
+                                          cnt.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          vec.ctypes.data_as(C.POINTER(C.c_uint64)), cap,
+                                          C.byref(nw)))
+    return {int(w[i]): (float(ms[i]), int(cnt[i]), int(vec[i])) for i in range(min(nw.value, cap))}
 
 
 def bgen_dosages(batch, sample_idx=None, device=0):

@@ -2,6 +2,7 @@
 gfx950).  Produces, in-tree:
   csaige_amd/libsaige_gpu.so          the product: HIP kernels + the C ABI of include/saige_gpu.h
   csaige_amd/synth/libsaige_synth.so  seeded synthetic cohorts (harness)
+  csaige_amd/synth/libsaige_synth_gpu.so  device twin of the step-2 dosage generator (harness)
 """
 from __future__ import annotations
 
@@ -27,6 +28,7 @@ HIP_FLAGS = [
 ]
 LIB = os.path.join(HERE, "libsaige_gpu.so")
 SYNTH = os.path.join(HERE, "synth", "libsaige_synth.so")
+SYNTH_GPU = os.path.join(HERE, "synth", "libsaige_synth_gpu.so")
 
 
 def _run(cmd):
@@ -51,11 +53,14 @@ def build(verbose: bool = False) -> None:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
     tmp = SYNTH + ".tmp"
-    _run(["g++", "-O2", "-fPIC", "-shared", "-fopenmp", "-std=c++17",
+    _run(["g++", "-O2", "-fPIC", "-shared", "-fopenmp", "-std=c++17", "-ffp-contract=off",
           os.path.join(HERE, "synth", "cohort.cpp"), "-o", tmp])
     os.replace(tmp, SYNTH)
+    tmp = SYNTH_GPU + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(HERE, "synth", "dosage_gpu.hip"), "-o", tmp])
+    os.replace(tmp, SYNTH_GPU)
     if verbose:
-        print("built", LIB, SYNTH)
+        print("built", LIB, SYNTH, SYNTH_GPU)
 
 
 if __name__ == "__main__":

@@ -155,18 +155,33 @@ struct KernelTimer {
 // dispatch-stamped event pair and is filed under its operand width, the padded number of
 // dense columns it multiplies.  Thread-safe; at most kMaxLaunches records between reads.
 bool launch_log_on();
-// Event pair for a launch of the given width on the current device (null when the log is
-// off or full).
-int launch_log_pair(uint32_t width, hipEvent_t* a, hipEvent_t* b);
+// The log's lock: held from handing out a launch's event pair until the launch is issued,
+// so that a concurrent read or enable (which return every recorded pair to the pool) cannot
+// recycle a pair whose launch has not been issued yet.
+std::mutex& launch_log_mutex();
+// What the launch log files a launch under: its operand width (the padded number of dense
+// columns it multiplies) and the number of vectors those columns carry.
+struct LaunchTag {
+  uint32_t width, vectors;
+};
+// Event pair for a launch on the current device (null when the log is off or full); the
+// caller holds launch_log_mutex().
+int launch_log_pair_locked(LaunchTag tag, hipEvent_t* a, hipEvent_t* b);
 
 // Launch a kernel on stream s, timed by t when its timing is on, else by the launch log
-// when that is on.  width is the launch's operand width for the launch log.
+// when that is on.
 template <class F, class... Args>
-int timed_launch(KernelTimer& t, uint32_t width, F kernel, dim3 grid, dim3 block,
+int timed_launch(KernelTimer& t, LaunchTag tag, F kernel, dim3 grid, dim3 block,
                  hipStream_t s, Args... args) {
   hipEvent_t a, b;
   SAIGE_TRY(t.pair(&a, &b));
-  if (!a && launch_log_on()) SAIGE_TRY(launch_log_pair(width, &a, &b));
+  if (!a && launch_log_on()) {
+    std::lock_guard<std::mutex> lk(launch_log_mutex());
+    SAIGE_TRY(launch_log_pair_locked(tag, &a, &b));
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, a, b, 0, args...);
+    SAIGE_HIP(hipGetLastError());
+    return SAIGE_OK;
+  }
   hipExtLaunchKernelGGL(kernel, grid, block, 0, s, a, b, 0, args...);
   SAIGE_HIP(hipGetLastError());
   return SAIGE_OK;

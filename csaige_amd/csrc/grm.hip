@@ -33,7 +33,7 @@ struct LaunchLog {
   static constexpr size_t kMaxLaunches = 65536;
   struct Rec {
     hipEvent_t a, b;
-    uint32_t width;
+    uint32_t width, vectors;
     int device;
   };
   struct FreeEvent {
@@ -92,10 +92,12 @@ int drain_launch_log(LaunchLog& L, uint32_t width, double* total_ms, uint64_t* l
 
 bool launch_log_on() { return launch_log().on.load(std::memory_order_relaxed); }
 
-int launch_log_pair(uint32_t width, hipEvent_t* a, hipEvent_t* b) {
+std::mutex& launch_log_mutex() { return launch_log().mu; }
+
+int launch_log_pair_locked(LaunchTag tag, hipEvent_t* a, hipEvent_t* b) {
+  const uint32_t width = tag.width;
   *a = *b = nullptr;
   LaunchLog& L = launch_log();
-  std::lock_guard<std::mutex> lk(L.mu);
   if (!L.on.load(std::memory_order_relaxed) || L.recs.size() >= LaunchLog::kMaxLaunches ||
       (L.only_width != 0 && width != L.only_width))
     return SAIGE_OK;
@@ -108,7 +110,7 @@ int launch_log_pair(uint32_t width, hipEvent_t* a, hipEvent_t* b) {
     L.pool.push_back({e0, dev});
     return rc;
   }
-  L.recs.push_back({e0, e1, width, dev});
+  L.recs.push_back({e0, e1, width, tag.vectors, dev});
   *a = e0;
   *b = e1;
   return SAIGE_OK;
@@ -300,14 +302,14 @@ static int product_k(saige_grm* h, const double* d_v, size_t ldv, size_t k, doub
 
   transpose_v_kernel<K><<<(unsigned)ceil_div(g.n_pad, 256), 256, 0, s>>>(
       d_v, ldv, g.n, k, h->ws_vt.as<double>(), g.n_pad);
-  SAIGE_TRY(timed_launch(h->timer, (uint32_t)K, grm_s1_f64<K>, dim3((unsigned)nblk_m, (unsigned)splits1),
+  SAIGE_TRY(timed_launch(h->timer, LaunchTag{(uint32_t)K, (uint32_t)k}, grm_s1_f64<K>, dim3((unsigned)nblk_m, (unsigned)splits1),
                          dim3(256), s, g.geno, g.row_bytes, g.lut, h->ws_vt.as<double>(), bps,
                          h->ws_tpart.as<double>(), g.m_pad));
   grm_s1_reduce<K><<<(unsigned)ceil_div(g.m_pad * K, 256), 256, 0, s>>>(
       h->ws_tpart.as<double>(), splits1, g.m_pad, h->ws_t.as<double>());
   LocoCodes lc;
   if (loco_codes(h, c0, k, lc)) loco_mask(h, h->ws_t.as<double>(), K, 1, g.m, k, lc, s);
-  SAIGE_TRY(timed_launch(h->timer, (uint32_t)K, grm_s2_f64<K>, dim3((unsigned)nblk_n, (unsigned)splits2),
+  SAIGE_TRY(timed_launch(h->timer, LaunchTag{(uint32_t)K, (uint32_t)k}, grm_s2_f64<K>, dim3((unsigned)nblk_n, (unsigned)splits2),
                          dim3(256), s, g.geno, g.row_bytes, g.lut, h->ws_t.as<double>(), mps,
                          g.m, h->ws_ypart.as<double>(), g.n_pad));
   grm_s2_reduce<K><<<dim3((unsigned)ceil_div(g.n, 256), (unsigned)k), 256, 0, s>>>(
@@ -409,11 +411,11 @@ static int product_f32(saige_grm* h, const double* d_v, size_t ldv, size_t k, do
   float* dot = h->ws_vlo.as<float>();
   to_f32_kernel<<<dim3((unsigned)ceil_div(g.n, 256), (unsigned)k), 256, 0, s>>>(d_v, ldv, g.n, k,
                                                                                  vf, ldf);
-  SAIGE_TRY(timed_launch(h->timer, (uint32_t)k, grm_f32_dot_kernel,
+  SAIGE_TRY(timed_launch(h->timer, LaunchTag{(uint32_t)k, (uint32_t)k}, grm_f32_dot_kernel,
                          dim3((unsigned)ceil_div(g.m, 256), (unsigned)k), dim3(256), s, g.geno,
                          g.row_bytes, h->lut32.as<float>(), vf, ldf, g.n, g.m, dot, g.m));
   const float scale = 1.0f / (float)g.m;
-  SAIGE_TRY(timed_launch(h->timer, (uint32_t)k, grm_f32_acc_kernel,
+  SAIGE_TRY(timed_launch(h->timer, LaunchTag{(uint32_t)k, (uint32_t)k}, grm_f32_acc_kernel,
                          dim3((unsigned)ceil_div(g.n, 256), (unsigned)k), dim3(256), s, g.geno,
                          g.row_bytes, h->lut32.as<float>(), dot, g.m, g.n, g.m, scale, d_out,
                          ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0, tau ? tau[1] : 0.0));
@@ -704,6 +706,50 @@ int saige_launch_log_enable(int enable, uint32_t width) {
   return SAIGE_OK;
 }
 
+int saige_launch_log_summary(uint32_t* widths, double* total_ms, uint64_t* launches,
+                             uint64_t* vectors, size_t cap, size_t* n_widths) {
+  using namespace saige;
+  clear_error();
+  SAIGE_ARG(n_widths && (cap == 0 || (widths && total_ms && launches && vectors)),
+            "null pointer");
+  LaunchLog& L = launch_log();
+  std::lock_guard<std::mutex> lk(L.mu);
+  struct Agg {
+    double ms = 0.0;
+    uint64_t count = 0, vec = 0;
+  };
+  std::vector<std::pair<uint32_t, Agg>> agg;  // in increasing width
+  int cur = 0, rc = SAIGE_OK;
+  SAIGE_HIP(hipGetDevice(&cur));
+  for (const LaunchLog::Rec& r : L.recs) {
+    float ms = 0.f;
+    const bool ok = hipSetDevice(r.device) == hipSuccess && hipEventSynchronize(r.b) == hipSuccess &&
+                    hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess;
+    if (!ok && rc == SAIGE_OK) {
+      set_error("launch log: a recorded launch could not be timed");
+      rc = SAIGE_ERR_DEVICE;
+    }
+    auto it = std::lower_bound(agg.begin(), agg.end(), r.width,
+                               [](const std::pair<uint32_t, Agg>& x, uint32_t w) { return x.first < w; });
+    if (it == agg.end() || it->first != r.width) it = agg.insert(it, {r.width, Agg{}});
+    it->second.ms += ms;
+    it->second.count += 1;
+    it->second.vec += r.vectors;
+    L.pool.push_back({r.a, r.device});
+    L.pool.push_back({r.b, r.device});
+  }
+  L.recs.clear();
+  (void)hipSetDevice(cur);
+  *n_widths = agg.size();
+  for (size_t i = 0; i < agg.size() && i < cap; ++i) {
+    widths[i] = agg[i].first;
+    total_ms[i] = agg[i].second.ms;
+    launches[i] = agg[i].second.count;
+    vectors[i] = agg[i].second.vec;
+  }
+  return rc;
+}
+
 int saige_launch_log_read(uint32_t width, double* total_ms, uint64_t* launches) {
   using namespace saige;
   clear_error();
@@ -819,6 +865,14 @@ int saige_grm_set_marker_chromosomes(saige_grm* h, const int32_t* chrom) {
   }
   std::vector<size_t> count(nc, 0);
   for (size_t j = 0; j < m; ++j) ++count[chrom[j]];
+  // dense codes: run_loco fits only the chromosomes present (loco.rs:62-95), so a code
+  // without markers would be a refit of a chromosome that does not exist
+  for (int c = 0; c < nc; ++c)
+    if (count[c] == 0) {
+      set_error("invalid argument: chromosome code %d has no marker (codes must be 0..C-1, "
+                "each used)", c);
+      return SAIGE_ERR_ARG;
+    }
   std::vector<double> scale(std::max(nc, 1), 1.0);
   for (int c = 0; c < nc; ++c)  // M / M_c; a GRM without markers is the zero product
     scale[c] = count[c] == m ? 0.0 : (double)m / (double)(m - count[c]);

@@ -780,7 +780,7 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   size_t rps1 = round_up(ceil_div(n_pad, splits1), kKC);
   splits1 = ceil_div(n_pad, rps1);
   SAIGE_TRY(ms->part1.ensure(splits1 * m_pad * KP * 4));
-  SAIGE_TRY(timed_launch(h->timer, (uint32_t)KP, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk1, (unsigned)splits1),
+  SAIGE_TRY(timed_launch(h->timer, LaunchTag{(uint32_t)KP, (uint32_t)k}, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk1, (unsigned)splits1),
                          dim3(256), s, g.geno, g.row_bytes, ms->vhi.as<__bf16>(),
                          ms->vlo.as<__bf16>(), n_pad, rps1, n_pad, ms->part1.as<float>(), m_pad));
   stage1_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(m, 256), (unsigned)k), 256, 0, s>>>(
@@ -807,7 +807,7 @@ static int mixed_product_kt(saige_grm* h, const double* d_v, size_t ldv, size_t 
   size_t rps2 = round_up(ceil_div(m_pad, splits2), kKC);
   splits2 = ceil_div(m_pad, rps2);
   SAIGE_TRY(ms->part2.ensure(splits2 * n_pad * KP * 4));
-  SAIGE_TRY(timed_launch(h->timer, (uint32_t)KP, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk2, (unsigned)splits2),
+  SAIGE_TRY(timed_launch(h->timer, LaunchTag{(uint32_t)KP, (uint32_t)k}, mix_gemm_lds_kernel<KT>, dim3((unsigned)rblk2, (unsigned)splits2),
                          dim3(256), s, ms->geno_t.as<uint8_t>(), ms->rbt, ms->whi.as<__bf16>(),
                          ms->wlo.as<__bf16>(), m_pad, rps2, m_pad, ms->part2.as<float>(), n_pad));
   stage2_epilogue_kernel<KP><<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(
@@ -1308,7 +1308,7 @@ struct I8Shape {
 template <int KT>
 static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const int8_t* bq,
                    size_t b_stride, size_t rows_pad, size_t red_len, size_t groups, DevBuf& part,
-                   size_t* splits_out) {
+                   size_t* splits_out, size_t vectors) {
   using S = I8Shape<KT>;
   constexpr int ROWS = S::NW * 16 * S::RT;
   static_assert(kMarkerPad % ROWS == 0 && kSamplePad % ROWS == 0, "row tiles must divide padding");
@@ -1323,7 +1323,7 @@ static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const
   SAIGE_TRY(part.ensure(sp * rows_pad * kpt * 4));
   const size_t blocks = round_up(rblk * sp, 8) * groups;
   SAIGE_ARG(blocks < (1u << 31), "int8 GEMM grid too large");
-  SAIGE_TRY(timed_launch(h->timer, (uint32_t)kpt, i8_gemm_kernel<S::RT, KT, S::NW, S::KCH>,
+  SAIGE_TRY(timed_launch(h->timer, LaunchTag{(uint32_t)kpt, (uint32_t)vectors}, i8_gemm_kernel<S::RT, KT, S::NW, S::KCH>,
                          dim3((unsigned)blocks), dim3(S::NW * 64), s, codes, code_stride, bq,
                          b_stride, rps, red_len, part.as<int>(), rows_pad, kpt, rblk, sp,
                          groups));
@@ -1333,15 +1333,15 @@ static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const
 
 static int i8_gemm_dispatch(int kt, saige_grm* h, const uint8_t* codes, size_t code_stride,
                             const int8_t* bq, size_t b_stride, size_t rows_pad, size_t red_len,
-                            size_t groups, DevBuf& part, size_t* splits) {
+                            size_t groups, DevBuf& part, size_t* splits, size_t vectors) {
   switch (kt) {
-    case 1: return i8_gemm<1>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
-    case 2: return i8_gemm<2>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
-    case 3: return i8_gemm<3>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
-    case 4: return i8_gemm<4>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
-    case 5: return i8_gemm<5>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
-    case 6: return i8_gemm<6>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
-    default: return i8_gemm<7>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits);
+    case 1: return i8_gemm<1>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 2: return i8_gemm<2>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 3: return i8_gemm<3>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 4: return i8_gemm<4>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 5: return i8_gemm<5>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 6: return i8_gemm<6>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    default: return i8_gemm<7>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
   }
 }
 
@@ -1414,7 +1414,7 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
   // ---- stage 1: I = G^T Q_v (rows = markers, reduction over samples) ----
   size_t splits1 = 0;
   SAIGE_TRY(i8_gemm_dispatch(kt, h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
-                             n_pad, groups, ms->ipart1, &splits1));
+                             n_pad, groups, ms->ipart1, &splits1, k));
   i8_stage1_epilogue_kernel<<<(unsigned)(m_pad / er), 256, epi_lds, s>>>(
       ms->ipart1.as<int>(), splits1, m_pad, m, k, kpt, sm, psum, pmax, pmin, pdot,
       1.0 / (double)n, nch_n, ms->sc.as<double>(), ms->p2.as<double>(), ms->zs.as<double>(),
@@ -1434,7 +1434,7 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
   // ---- stage 2: I = G Q_w (rows = samples, reduction over markers) ----
   size_t splits2 = 0;
   SAIGE_TRY(i8_gemm_dispatch(kt, h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
-                             m_pad, n_pad, m_pad, groups, ms->ipart2, &splits2));
+                             m_pad, n_pad, m_pad, groups, ms->ipart2, &splits2, k));
   i8_stage2_epilogue_kernel<<<(unsigned)(n_pad / er), 256, epi_lds, s>>>(
       ms->ipart2.as<int>(), splits2, n_pad, n, k, kpt, sm, psum, pmax, pmin, pdot,
       1.0 / (double)m, nch_m, ms->rsum.as<double>(), ms->P2, d_out, ldo, d_v, ldv, d_w,

@@ -34,8 +34,45 @@ def _load():
                                     dp, C.c_double, C.c_size_t, C.c_double, C.c_double,
                                     C.c_size_t, C.c_double, u8p, dp, dp, dp]
         L.synth_biobank.restype = C.c_int
+        L.synth_dosage_mafs.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, C.c_double,
+                                        C.c_double, dp]
         _lib = L
     return _lib
+
+
+_gpu = None
+
+
+def _load_gpu():
+    global _gpu
+    if _gpu is None:
+        so = os.path.join(_HERE, "libsaige_synth_gpu.so")
+        if not os.path.exists(so):
+            raise ImportError(f"{so} missing: run python csaige_amd/build.py")
+        L = C.CDLL(so)
+        L.synth_dosage_u8_device.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t,
+                                             C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.synth_dosage_u8_device.restype = C.c_int
+        _gpu = L
+    return _gpu
+
+
+def dosage_mafs(first, nvar, seed=4, maf=(1e-4, 0.5)):
+    """MAFs of step-2 variants [first, first + nvar) (those dosage_u8 draws from)."""
+    out = np.empty(nvar)
+    _load().synth_dosage_mafs(seed, first, nvar, maf[0], maf[1], _p(out, dp))
+    return out
+
+
+def dosage_u8_device(n, first, nvar, d_mafs, d_q, ld=None, seed=4, stream=None):
+    """dosage_u8 generated on the device: rows of variants [first, first + nvar) written to the
+    device pointer d_q (nvar x ld bytes) from their MAFs at the device pointer d_mafs
+    (dosage_mafs), on `stream` (a hipStream_t; None: the null stream).  Same bytes as
+    dosage_u8."""
+    ld = n if ld is None else ld
+    rc = _load_gpu().synth_dosage_u8_device(seed, n, first, nvar, ld, d_mafs, d_q, stream)
+    if rc != 0:
+        raise ValueError("synth_dosage_u8_device: bad arguments or launch failure")
 
 
 def _p(a, t):

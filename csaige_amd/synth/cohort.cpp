@@ -31,6 +31,13 @@ inline double normal(uint64_t seed, uint64_t stream, uint64_t idx) {
 }
 inline uint8_t code_of(int c) { return c == 2 ? 0u : (c == 1 ? 2u : 3u); }
 
+// Polygenic scores are summed in 2^-40 fixed point: integer sums do not depend on the order
+// in which the threads' partial sums meet, so a cohort is the same bit for bit on every run
+// and thread count (each term is rounded once, to 9e-13).
+constexpr double kPgsScale = 1099511627776.0;  // 2^40
+inline int64_t pgs_fix(double v) { return (int64_t)std::llround(v * kPgsScale); }
+inline double pgs_value(int64_t v) { return (double)v / kPgsScale; }
+
 enum Stream : uint64_t {
   S_MAF = 1, S_GENO = 2, S_BETA = 3, S_COV = 4, S_NOISE = 5, S_TRANSMIT = 6, S_MAF_TEST = 7,
   S_GENO_TEST = 8, S_ETA = 9, S_Y = 10, S_DOSAGE = 11, S_DOSAGE_MAF = 12, S_DOSAGE_NOISE = 13,
@@ -62,10 +69,10 @@ int synth_unrelated(uint64_t seed, size_t n, size_t m_grm, size_t m_test, double
                     uint8_t* bed_grm, uint8_t* bed_test, double* maf_grm, double* maf_test,
                     double* y, double* x, double* liability_out) {
   size_t bpm = (n + 3) / 4;
-  std::vector<double> pgs(n, 0.0);
+  std::vector<int64_t> pgs(n, 0);
 #pragma omp parallel
   {
-    std::vector<double> local(n, 0.0);
+    std::vector<int64_t> local(n, 0);
 #pragma omp for schedule(static)
     for (long long jj = 0; jj < (long long)m_grm; ++jj) {
       size_t j = (size_t)jj;
@@ -81,7 +88,7 @@ int synth_unrelated(uint64_t seed, size_t n, size_t m_grm, size_t m_test, double
         double u2 = (double)(uint32_t)(h >> 32) * (1.0 / 4294967296.0);
         int c = (u1 < maf) + (u2 < maf);
         row[i / 4] |= (uint8_t)(code_of(c) << (2 * (i % 4)));
-        local[i] += beta * ((double)c - mean) / sd;
+        local[i] += pgs_fix(beta * ((double)c - mean) / sd);
       }
     }
 #pragma omp critical
@@ -108,7 +115,7 @@ int synth_unrelated(uint64_t seed, size_t n, size_t m_grm, size_t m_test, double
   std::vector<double> liab(n);
   for (size_t i = 0; i < n; ++i) {
     x[i] = 1.0;
-    double l = pgs[i];
+    double l = pgs_value(pgs[i]);
     for (size_t c = 0; c < n_cov; ++c) {
       double v = normal(seed, S_COV + 100 * (c + 1), i);
       x[(c + 1) * n + i] = v;
@@ -131,10 +138,10 @@ int synth_families(uint64_t seed, size_t n_fam, size_t kids, size_t m, double ma
   size_t fs = 2 + kids;
   size_t n = n_fam * fs;
   size_t bpm = (n + 3) / 4;
-  std::vector<double> pgs(n, 0.0);
+  std::vector<int64_t> pgs(n, 0);
 #pragma omp parallel
   {
-    std::vector<double> local(n, 0.0);
+    std::vector<int64_t> local(n, 0);
     std::vector<int> dose(fs);
 #pragma omp for schedule(static)
     for (long long jj = 0; jj < (long long)m; ++jj) {
@@ -162,7 +169,7 @@ int synth_families(uint64_t seed, size_t n_fam, size_t kids, size_t m, double ma
         for (size_t s = 0; s < fs; ++s) {
           size_t i = f * fs + s;
           row[i / 4] |= (uint8_t)(code_of(dose[s]) << (2 * (i % 4)));
-          local[i] += beta * ((double)dose[s] - mean) / sd;
+          local[i] += pgs_fix(beta * ((double)dose[s] - mean) / sd);
         }
       }
     }
@@ -172,7 +179,7 @@ int synth_families(uint64_t seed, size_t n_fam, size_t kids, size_t m, double ma
   std::vector<double> liab(n);
   for (size_t i = 0; i < n; ++i) {
     x[i] = 1.0;
-    double l = pgs[i];
+    double l = pgs_value(pgs[i]);
     for (size_t c = 0; c < n_cov; ++c) {
       double v = normal(seed, S_COV + 100 * (c + 1), i);
       x[(c + 1) * n + i] = v;
@@ -200,10 +207,10 @@ int synth_biobank(uint64_t seed, size_t n, size_t m, size_t n_pairs, size_t n_tr
   const size_t nrel = 2 * n_pairs;
   const double pc = std::min(1.0, (double)n_causal / (double)std::max<size_t>(m, 1));
   const double bsd = std::sqrt(h2 / (double)std::max<size_t>(n_causal, 1));
-  std::vector<double> pgs(n * n_traits, 0.0);
+  std::vector<int64_t> pgs(n * n_traits, 0);
 #pragma omp parallel
   {
-    std::vector<double> local;
+    std::vector<int64_t> local;
     std::vector<uint8_t> dose(n);
 #pragma omp for schedule(dynamic, 64)
     for (long long jj = 0; jj < (long long)m; ++jj) {
@@ -231,12 +238,12 @@ int synth_biobank(uint64_t seed, size_t n, size_t m, size_t n_pairs, size_t n_tr
       std::memset(row, 0, bpm);
       for (size_t i = 0; i < n; ++i) row[i / 4] |= (uint8_t)(code_of(dose[i]) << (2 * (i % 4)));
       if (u01(h3(seed, S_CAUSAL, j)) < pc) {
-        if (local.empty()) local.assign(n * n_traits, 0.0);
+        if (local.empty()) local.assign(n * n_traits, 0);
         const double mean = 2.0 * maf, sd = std::sqrt(2.0 * maf * (1.0 - maf));
         for (size_t t = 0; t < n_traits; ++t) {
           const double beta = bsd * normal(seed, S_BETA_T + 100 * t, j);
-          double* lt = local.data() + t * n;
-          for (size_t i = 0; i < n; ++i) lt[i] += beta * ((double)dose[i] - mean) / sd;
+          int64_t* lt = local.data() + t * n;
+          for (size_t i = 0; i < n; ++i) lt[i] += pgs_fix(beta * ((double)dose[i] - mean) / sd);
         }
       }
     }
@@ -256,7 +263,8 @@ int synth_biobank(uint64_t seed, size_t n, size_t m, size_t n_pairs, size_t n_tr
   std::vector<double> liab(n);
   for (size_t t = 0; t < n_traits; ++t) {
     for (size_t i = 0; i < n; ++i)
-      liab[i] = pgs[t * n + i] + cov[i] + std::sqrt(1.0 - h2) * normal(seed, S_NOISE_T + 100 * t, i);
+      liab[i] = pgs_value(pgs[t * n + i]) + cov[i] +
+                std::sqrt(1.0 - h2) * normal(seed, S_NOISE_T + 100 * t, i);
     threshold_cases(liab, prevalence[t], y + t * n);
   }
   return 0;
@@ -278,17 +286,30 @@ int synth_null_fit(uint64_t seed, size_t n, size_t p, double eta_mean, double et
   return 0;
 }
 
+// MAF of step-2 variants [first, first + nvar): log-uniform(lo, hi).
+int synth_dosage_mafs(uint64_t seed, size_t first, size_t nvar, double maf_lo, double maf_hi,
+                      double* maf_out) {
+  const double llo = std::log(maf_lo), lhi = std::log(maf_hi);
+  for (size_t vv = 0; vv < nvar; ++vv)
+    maf_out[vv] = std::exp(llo + (lhi - llo) * u01(h3(seed, S_DOSAGE_MAF, first + vv)));
+  return 0;
+}
+
 // uint8-quantized dosages (dosage = q / 127, so 0, 1 and 2 are exact) for variants
 // [first, first + nvar), variant-major rows of `ld` bytes.  MAF ~ log-uniform(lo, hi);
 // hard calls under HWE; every variant with index % 4 == 3 carries imputation-like noise
-// (q = clamp(round(127 g + 12 N(0,1)), 0, 254)).
+// q = clamp(round(127 g + 12 e), 0, 254) with e = sqrt(3) (u1 + u2 + u3 + u4 - 2), the four
+// u the 16-bit fields of one hash over 65536 (Irwin-Hall, mean 0, variance 1).  Every step is
+// integer or one correctly rounded fp64 operation, so the device generator
+// (dosage_gpu.hip) writes the same bytes.
 int synth_dosage_u8(uint64_t seed, size_t n, size_t first, size_t nvar, size_t ld, double maf_lo,
                     double maf_hi, uint8_t* q, double* maf_out) {
-  double llo = std::log(maf_lo), lhi = std::log(maf_hi);
+  std::vector<double> mafs(nvar);
+  synth_dosage_mafs(seed, first, nvar, maf_lo, maf_hi, mafs.data());
 #pragma omp parallel for schedule(static)
   for (long long vv = 0; vv < (long long)nvar; ++vv) {
     size_t v = first + (size_t)vv;
-    double maf = std::exp(llo + (lhi - llo) * u01(h3(seed, S_DOSAGE_MAF, v)));
+    const double maf = mafs[vv];
     if (maf_out) maf_out[vv] = maf;
     uint8_t* row = q + (size_t)vv * ld;
     bool noisy = (v % 4) == 3;
@@ -299,8 +320,13 @@ int synth_dosage_u8(uint64_t seed, size_t n, size_t first, size_t nvar, size_t l
       int c = (u1 < maf) + (u2 < maf);
       int qq = 127 * c;
       if (noisy) {
-        double e = 12.0 * normal(seed, S_DOSAGE_NOISE, v * n + i);
-        qq = (int)std::lround(127.0 * c + e);
+        const uint64_t g = h3(seed, S_DOSAGE_NOISE, v * n + i);
+        const uint32_t s4 = (uint32_t)(g & 0xFFFFu) + (uint32_t)((g >> 16) & 0xFFFFu) +
+                            (uint32_t)((g >> 32) & 0xFFFFu) + (uint32_t)(g >> 48);
+        const double t = (double)s4 * (1.0 / 65536.0) - 2.0;   // exact
+        const double e = (t * 1.7320508075688772) * 12.0;
+        const double y = 127.0 * (double)c + e;
+        qq = (int)std::lround(y);
         qq = std::min(254, std::max(0, qq));
       }
       row[i] = (uint8_t)qq;

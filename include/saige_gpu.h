@@ -142,6 +142,13 @@ int saige_grm_kernel_timing(saige_grm* h, double* total_ms, uint64_t* launches);
  * reads. */
 int saige_launch_log_enable(int enable, uint32_t width);
 int saige_launch_log_read(uint32_t width, double* total_ms, uint64_t* launches);
+/* Every width at once (diagnostic, no counterpart in saige-rs): for each operand width
+ * recorded since the last read, in increasing order, the summed launch time, the launch
+ * count and the summed number of vectors (the columns of the product before slicing and
+ * padding) of its launches; at most cap widths are written, *n_widths is the number recorded.
+ * Clears the log like saige_launch_log_read. */
+int saige_launch_log_summary(uint32_t* widths, double* total_ms, uint64_t* launches,
+                             uint64_t* vectors, size_t cap, size_t* n_widths);
 
 /* ------------------------------------------------------------------------------
  * Preconditioned conjugate gradient on Sigma = tau_e * diag(1/W) + tau_g * GRM.

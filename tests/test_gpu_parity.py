@@ -556,6 +556,7 @@ def test_fit_null_multi_matches_single_trait(prec):
         _say(f"prec {prec} trait {t}: tau multi={multi[t]['tau']} single={one['tau']} "
              f"it {multi[t]['iterations']}/{one['iterations']} pcg {multi[t]['pcg_iterations']}"
              f"/{one['pcg_iterations']}")
+        assert multi[t]["converged"] and one["converged"]
         assert multi[t]["iterations"] == one["iterations"]
         assert multi[t]["pcg_iterations"] == one["pcg_iterations"]
         assert multi[t]["grm_products"] == one["grm_products"]
@@ -605,4 +606,29 @@ def test_launch_log_files_launches_by_operand_width():
         assert cnt.value == 2
     finally:
         sg.check(L.saige_launch_log_enable(0, 0))
+        g.close()
+
+
+def test_launch_log_summary_per_width():
+    """The per-width summary files each GEMM launch under its operand width with the number of
+    vectors it carried: k = 34 with two int8 slices is one launch pair at 80 columns carrying
+    34 vectors each; 136 vectors run as one launch pair of three column groups (288 columns);
+    the summary drains the log."""
+    body, bpm, d, af = _grm_pair(1200, 700, seed=5, p_missing=0.0)
+    g = sg.OnTheFlyGrm(body, bpm, 1200, af, precision=sg.PREC_INT8X2)
+    V = np.random.default_rng(2).standard_normal((136, 1200))
+    g.mat_mat(V)  # warm-up outside the log
+    sg.launch_log_enable(True, 0)
+    try:
+        g.mat_mat(V[:34])
+        g.mat_mat(V)
+        s = sg.launch_log_summary()
+        _say(f"launch log summary: {s}")
+        assert set(s) == {80, 288}
+        assert s[80][1] == 2 and s[80][2] == 68
+        assert s[288][1] == 2 and s[288][2] == 272
+        assert all(v[0] > 0.0 for v in s.values())
+        assert sg.launch_log_summary() == {}
+    finally:
+        sg.launch_log_enable(False)
         g.close()

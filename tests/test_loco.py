@@ -100,3 +100,19 @@ def test_gpu_loco_refits_vs_oracle(trait):
     # a refit is not the full-GRM fit
     full = sg.fit_ai_reml(g, y, c.x, trait=trait, tol=1e-5)
     assert rel(got[0]["tau"], full["tau"]).max() > 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_loco_rejects_chromosome_codes_without_markers():
+    """Codes must be dense (0..C-1, each used): run_loco fits only the chromosomes present
+    (loco.rs:62-95), so a gap in the codes (chromosomes coded 1..22, say) is refused instead of
+    refitting a chromosome that does not exist."""
+    c, d, af, chrom = _cohort(n=300, m=400)
+    g = sg.OnTheFlyGrm(c.bed, c.bpm, c.n, af)
+    with pytest.raises(sg.SaigeError):
+        g.set_marker_chromosomes(chrom + 1)  # code 0 unused
+    gap = chrom.copy()
+    gap[gap == 1] = 3  # codes 0, 2, 3: code 1 unused
+    with pytest.raises(sg.SaigeError):
+        g.set_marker_chromosomes(gap)
+    g.set_marker_chromosomes(chrom)  # dense codes are accepted
