@@ -485,6 +485,11 @@ class ScoreTestEngine:
     def test_markers_u8_device(self, d_q, ldq, nvar, scale, d_out):
         check(load().saige_test_markers_u8_device(self.h, d_q, ldq, nvar, scale, d_out))
 
+    def test_markers_u8_device_noalloc(self, d_q, ldq, nvar, scale, d_out):
+        """test_markers_u8_device with no device allocation (at most 4,096 variants per call;
+        work areas in the library's per-GPU scratch array)."""
+        check(load().saige_test_markers_u8_device_noalloc(self.h, d_q, ldq, nvar, scale, d_out))
+
     @property
     def stream(self):
         return load().saige_engine_stream(self.h)

@@ -152,6 +152,7 @@ SIGNATURES = [
     ("saige_test_markers_f64", C.c_int, [vp, dp, sz, sz, C.POINTER(VariantResult)]),
     ("saige_test_markers_u8", C.c_int, [vp, u8p, sz, sz, C.c_double, C.POINTER(VariantResult)]),
     ("saige_test_markers_u8_device", C.c_int, [vp, vp, sz, sz, C.c_double, vp]),
+    ("saige_test_markers_u8_device_noalloc", C.c_int, [vp, vp, sz, sz, C.c_double, vp]),
     ("saige_test_markers_plink", C.c_int, [vp, u8p, sz, sz, u32p, sz,
                                            C.POINTER(VariantResult)]),
     ("saige_bgen_dosages", C.c_int, [C.c_int, u8p, sz, sz, u32p, C.c_uint32, u32p, sz, dp]),

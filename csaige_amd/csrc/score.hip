@@ -1455,6 +1455,24 @@ struct saige_engine {
 
 namespace saige {
 
+// Device work areas of one batch: the sums partials, the gated-variant list and its count,
+// and the batched SPA state.  The allocating entry points take them from the engine's
+// buffers (grown on demand); with `fixed` they are given (the scratch array of
+// saige_test_markers_u8_device_noalloc) and neither grow nor trigger the shape search.
+struct BatchArea {
+  bool fixed = false;
+  double* part = nullptr;
+  size_t part_bytes = 0;
+  GateInfo* gate = nullptr;
+  size_t gate_cap = 0;  // variants
+  uint32_t* ngate = nullptr;
+  SpaVar* spa_vars = nullptr;
+  size_t spa_vars_cap = 0;  // variants
+  double* spa_part = nullptr;
+  size_t spa_part_cap = 0;  // doubles
+  uint32_t* spa_active = nullptr;
+};
+
 __global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
 
 // Sum the per-split partials of the sums kernel in split order (one thread per variant and
@@ -1488,25 +1506,36 @@ __global__ void bgen_decode_kernel(LoadBgen load, size_t n, size_t nvar,
 // work-groups per pass, slices of at least 2,048 samples); the host reads the number of
 // variants still searching after every fourth pass.
 template <class G, bool TAB>
-static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out) {
+static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const BatchArea& ar) {
   hipStream_t s = e->stream;
   const size_t n = e->n;
   uint32_t ng = 0;
-  SAIGE_HIP(hipMemcpyAsync(&ng, e->ngate.p, 4, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipMemcpyAsync(&ng, ar.ngate, 4, hipMemcpyDeviceToHost, s));
   SAIGE_HIP(hipStreamSynchronize(s));
   if (ng == 0) return SAIGE_OK;
   size_t S = std::max<size_t>(1, std::min<size_t>(ceil_div(1024, ng), ceil_div(n, 2048)));
   S = std::min<size_t>(S, 64);
   const size_t slice = round_up(ceil_div(n, S), 256);
   S = ceil_div(n, slice);
-  SAIGE_TRY(e->spa_vars.ensure(ng * sizeof(SpaVar)));
-  SAIGE_TRY(e->spa_part.ensure(ng * S * kSpaPart * 8));
-  SAIGE_TRY(e->spa_active.ensure(64));
-  uint32_t* active = e->spa_active.as<uint32_t>();
-  SpaVar* vars = e->spa_vars.as<SpaVar>();
-  double* part = e->spa_part.as<double>();
-  GateInfo* gate = e->gate.as<GateInfo>();
-  const uint32_t* ngp = e->ngate.as<uint32_t>();
+  uint32_t* active;
+  SpaVar* vars;
+  double* part;
+  if (ar.fixed) {  // ng <= the batch's variants, ng * S <= 1024 + ng: inside the area
+    SAIGE_ARG(ng <= ar.spa_vars_cap && ng * S * kSpaPart <= ar.spa_part_cap,
+              "batched SPA state exceeds the scratch area");
+    active = ar.spa_active;
+    vars = ar.spa_vars;
+    part = ar.spa_part;
+  } else {
+    SAIGE_TRY(e->spa_vars.ensure(ng * sizeof(SpaVar)));
+    SAIGE_TRY(e->spa_part.ensure(ng * S * kSpaPart * 8));
+    SAIGE_TRY(e->spa_active.ensure(64));
+    active = e->spa_active.as<uint32_t>();
+    vars = e->spa_vars.as<SpaVar>();
+    part = e->spa_part.as<double>();
+  }
+  GateInfo* gate = ar.gate;
+  const uint32_t* ngp = ar.ngate;
   const double* mu = e->mu_d.as<double>();
   const double* odds = e->odds_d.as<double>();
   const MlpView mv = e->mlp_view();
@@ -1592,7 +1621,7 @@ static size_t split_len(size_t n, size_t splits, int form) {
 // than 64 * 1024 samples use the default shape.
 template <int P, class L>
 static int sums_shape(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int* form_out,
-                      size_t* splits_out) {
+                      size_t* splits_out, size_t max_splits = 0) {
   constexpr int NS = Layout<P>::NS;
   const size_t n = e->n;
   const bool u8 = std::is_same<L, LoadU8>::value && load.vec;
@@ -1603,13 +1632,18 @@ static int sums_shape(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int
                                                                ceil_div(n, 512)));
   const auto key = std::make_tuple(loader_id<L>() * 2 + (int)load.vec, P, nvar);
   auto it = e->tuned_shape.find(key);
-  if (it != e->tuned_shape.end()) {
+  if (it != e->tuned_shape.end() && (max_splits == 0 || it->second.second <= max_splits)) {
     *form_out = it->second.first;
     *splits_out = it->second.second;
     return SAIGE_OK;
   }
   int best_form = form0;
   size_t best = norm(form0 == 0 ? 16 : dflt_sp, form0);
+  if (max_splits) {  // no search: the default shape within the split bound
+    *form_out = best_form;
+    *splits_out = std::min(best, max_splits);
+    return SAIGE_OK;
+  }
   if (n >= 65536) {
     hipStream_t s = e->stream;
     float best_ms = 1e30f;
@@ -1646,59 +1680,133 @@ static int sums_shape(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int
   return SAIGE_OK;
 }
 
+// Sample splits of the sums kernel at most, in a fixed work area.
+constexpr size_t kFixedSplits = 64;
+
 template <int P, class L>
-static int run_batch_p(saige_engine* e, L load, size_t nvar, saige_variant_result* d_out) {
+static int run_batch_p(saige_engine* e, L load, size_t nvar, saige_variant_result* d_out,
+                       const BatchArea* fixed) {
   hipStream_t s = e->stream;
   const size_t n = e->n;
   constexpr int NS = Layout<P>::NS;
   const size_t nvar_pad = round_up(nvar, 128);
   size_t splits = 1;
   int form = 1;
-  SAIGE_TRY((sums_shape<P, L>(e, load, nvar, nvar_pad, &form, &splits)));
+  SAIGE_TRY((sums_shape<P, L>(e, load, nvar, nvar_pad, &form, &splits,
+                              fixed ? kFixedSplits : 0)));
   const size_t sps = split_len(n, splits, form);
-  SAIGE_TRY(e->part.ensure((splits + 1) * NS * nvar_pad * 8));
-  SAIGE_TRY(e->gate.ensure(nvar * sizeof(GateInfo)));
-  SAIGE_TRY(e->ngate.ensure(64));
-  zero_u32_kernel<<<1, 1, 0, s>>>(e->ngate.as<uint32_t>());  // a kernel, not a memset node
-  launch_sums<P, L>(e, load, nvar, nvar_pad, form, splits, sps, e->part.as<double>());
-  double* sums = e->part.as<double>() + splits * NS * nvar_pad;
+  const size_t part_bytes = (splits + 1) * NS * nvar_pad * 8;
+  BatchArea ar;
+  if (fixed) {
+    ar = *fixed;
+    SAIGE_ARG(part_bytes <= ar.part_bytes && nvar <= ar.gate_cap,
+              "batch exceeds the scratch area");
+  } else {
+    SAIGE_TRY(e->part.ensure(part_bytes));
+    SAIGE_TRY(e->gate.ensure(nvar * sizeof(GateInfo)));
+    SAIGE_TRY(e->ngate.ensure(64));
+    ar.part = e->part.as<double>();
+    ar.gate = e->gate.as<GateInfo>();
+    ar.ngate = e->ngate.as<uint32_t>();
+  }
+  zero_u32_kernel<<<1, 1, 0, s>>>(ar.ngate);  // a kernel, not a memset node
+  launch_sums<P, L>(e, load, nvar, nvar_pad, form, splits, sps, ar.part);
+  double* sums = ar.part + splits * NS * nvar_pad;
   sums_reduce_kernel<<<dim3((unsigned)ceil_div(nvar_pad, 256), (unsigned)NS), 256, 0, s>>>(
-      e->part.as<double>(), splits, NS, nvar_pad, sums);
+      ar.part, splits, NS, nvar_pad, sums);
   score_finalize_kernel<P><<<(unsigned)ceil_div(nvar, 128), 128, 0, s>>>(
-      sums, 1, nvar, nvar_pad, n, e->ec, L::kMayMiss ? 1 : 0, d_out,
-      e->gate.as<GateInfo>(), e->ngate.as<uint32_t>());
+      sums, 1, nvar, nvar_pad, n, e->ec, L::kMayMiss ? 1 : 0, d_out, ar.gate, ar.ngate);
   SAIGE_HIP(hipGetLastError());
   if (e->ec.use_spa) {
     bool tab = false;
     if constexpr (std::is_same<L, LoadU8>::value) tab = (P == 1 && e->intercept_only);
     if (tab) {
       if constexpr (std::is_same<L, LoadU8>::value)
-        SAIGE_TRY((spa_batched<GTildeU8Tab, true>(e, GTildeU8Tab{load}, d_out)));
+        SAIGE_TRY((spa_batched<GTildeU8Tab, true>(e, GTildeU8Tab{load}, d_out, ar)));
     } else {
       SAIGE_TRY((spa_batched<GTilde<L, P>, false>(e, GTilde<L, P>{load, e->x_d.as<double>(), n},
-                                                   d_out)));
+                                                   d_out, ar)));
     }
   }
   return SAIGE_OK;
 }
 
 template <class L>
-static int run_batch(saige_engine* e, L load, size_t nvar, saige_variant_result* d_out) {
+static int run_batch(saige_engine* e, L load, size_t nvar, saige_variant_result* d_out,
+                     const BatchArea* fixed = nullptr) {
   switch (e->p) {
-    case 1: return run_batch_p<1>(e, load, nvar, d_out);
-    case 2: return run_batch_p<2>(e, load, nvar, d_out);
-    case 3: return run_batch_p<3>(e, load, nvar, d_out);
-    case 4: return run_batch_p<4>(e, load, nvar, d_out);
-    case 5: return run_batch_p<5>(e, load, nvar, d_out);
-    case 6: return run_batch_p<6>(e, load, nvar, d_out);
-    case 7: return run_batch_p<7>(e, load, nvar, d_out);
-    case 8: return run_batch_p<8>(e, load, nvar, d_out);
+    case 1: return run_batch_p<1>(e, load, nvar, d_out, fixed);
+    case 2: return run_batch_p<2>(e, load, nvar, d_out, fixed);
+    case 3: return run_batch_p<3>(e, load, nvar, d_out, fixed);
+    case 4: return run_batch_p<4>(e, load, nvar, d_out, fixed);
+    case 5: return run_batch_p<5>(e, load, nvar, d_out, fixed);
+    case 6: return run_batch_p<6>(e, load, nvar, d_out, fixed);
+    case 7: return run_batch_p<7>(e, load, nvar, d_out, fixed);
+    case 8: return run_batch_p<8>(e, load, nvar, d_out, fixed);
   }
   set_error("p = %zu covariates not supported (1..8)", e->p);
   return SAIGE_ERR_ARG;
 }
 
 constexpr size_t kHostBatch = 4096;
+
+// ---------------------------------------------------------------------------------
+// Work area of saige_test_markers_u8_device_noalloc: one file-scope device array (each GPU
+// the library runs on holds its own copy until the process ends), carved into one batch's
+// work areas, so that the call allocates nothing.  Sized for kScratchVariants variants per
+// call at any covariate count (p <= 8) with at most kFixedSplits sample splits of the sums
+// kernel.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU (two
+// engines on two host threads) race; calls on one engine are serialized by its lock.
+// ---------------------------------------------------------------------------------
+constexpr size_t kScratchVariants = 4096;
+constexpr size_t cround(size_t x, size_t m) { return (x + m - 1) / m * m; }
+constexpr size_t kScratchPart = (kFixedSplits + 1) * (2 * kMaxP + 6) * kScratchVariants * 8;
+constexpr size_t kScratchGate = cround(kScratchVariants * sizeof(GateInfo), 256);
+constexpr size_t kScratchVars = cround(kScratchVariants * sizeof(SpaVar), 256);
+// the SPA partials: ng * S <= 1024 + ng variant-slices (S <= ceil(1024 / ng))
+constexpr size_t kScratchSpaPart = cround((1024 + kScratchVariants) * kSpaPart * 8, 256);
+constexpr size_t kScratchBytes = cround(kScratchPart, 256) + kScratchGate + 256 + kScratchVars +
+                                 kScratchSpaPart + 256;
+static_assert(kScratchBytes <= (48u << 20), "scratch stays within 48 MiB");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+
+// The batch's work areas in this GPU's scratch array; its address is looked up once per
+// device, with that device current, and kept.
+static int scratch_area(int device, BatchArea* ar) {
+  static std::mutex mu;
+  static unsigned char* base_of[kScratchDevices] = {};
+  SAIGE_ARG(device >= 0 && device < kScratchDevices, "device index < 64");
+  unsigned char* base;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!base_of[device]) {
+      void* ptr = nullptr;
+      SAIGE_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      base_of[device] = static_cast<unsigned char*>(ptr);
+    }
+    base = base_of[device];
+  }
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += cround(bytes, 256);
+    return p;
+  };
+  ar->fixed = true;
+  ar->part = reinterpret_cast<double*>(take(kScratchPart));
+  ar->part_bytes = kScratchPart;
+  ar->gate = reinterpret_cast<GateInfo*>(take(kScratchGate));
+  ar->gate_cap = kScratchVariants;
+  ar->ngate = reinterpret_cast<uint32_t*>(take(256));
+  ar->spa_vars = reinterpret_cast<SpaVar*>(take(kScratchVars));
+  ar->spa_vars_cap = kScratchVariants;
+  ar->spa_part = reinterpret_cast<double*>(take(kScratchSpaPart));
+  ar->spa_part_cap = kScratchSpaPart / 8;
+  ar->spa_active = reinterpret_cast<uint32_t*>(take(256));
+  return off <= kScratchBytes ? SAIGE_OK : SAIGE_ERR_ARG;
+}
 
 }  // namespace saige
 
@@ -1820,6 +1928,27 @@ int saige_test_markers_u8_device(saige_engine* e, const uint8_t* d_q, size_t ldq
   SAIGE_HIP(hipSetDevice(e->device));
   const int vec = (ldq % 16 == 0) && ((uintptr_t)d_q % 16 == 0);
   return run_batch(e, LoadU8{d_q, ldq, scale, vec}, nvar, d_out);
+}
+
+int saige_test_markers_u8_device_noalloc(saige_engine* e, const uint8_t* d_q, size_t ldq,
+                                         size_t nvar, double scale,
+                                         saige_variant_result* d_out) {
+  clear_error();
+  SAIGE_ARG(e && (nvar == 0 || (d_q && d_out)), "null pointer");
+  SAIGE_ARG(ldq >= e->n, "ldq < n");
+  if (nvar > kScratchVariants) {
+    set_error("invalid argument: saige_test_markers_u8_device_noalloc takes at most %zu "
+              "variants per call (the size of its scratch array), got %zu",
+              kScratchVariants, nvar);
+    return SAIGE_ERR_ARG;
+  }
+  if (nvar == 0) return SAIGE_OK;
+  std::lock_guard<std::mutex> lk(e->mu);
+  SAIGE_HIP(hipSetDevice(e->device));
+  BatchArea ar;
+  SAIGE_TRY(scratch_area(e->device, &ar));
+  const int vec = (ldq % 16 == 0) && ((uintptr_t)d_q % 16 == 0);
+  return run_batch(e, LoadU8{d_q, ldq, scale, vec}, nvar, d_out, &ar);
 }
 
 int saige_test_markers_u8(saige_engine* e, const uint8_t* q, size_t ldq, size_t nvar, double scale,

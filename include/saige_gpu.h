@@ -371,6 +371,18 @@ int saige_test_markers_u8(saige_engine* e, const uint8_t* q, size_t ldq, size_t 
                           double scale, saige_variant_result* out);
 int saige_test_markers_u8_device(saige_engine* e, const uint8_t* d_q, size_t ldq, size_t nvar,
                                  double scale, saige_variant_result* d_out);
+/* saige_test_markers_u8_device without any device allocation.  No counterpart in saige-rs:
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The batch's work areas (sums partials,
+ * gated-variant list, batched SPA state) come from a file-scope device array of 48 MiB,
+ * one per GPU, held until the process ends; at most 4,096 variants per call (more is
+ * SAIGE_ERR_ARG), the sums kernel's shape is the one the engine has tuned for that batch
+ * size when it has at most 64 sample splits, else the default (no shape search).  Calls on
+ * one GPU share its array and keep nothing between calls; calls that overlap on one GPU
+ * (engines used from several host threads) race. */
+int saige_test_markers_u8_device_noalloc(saige_engine* e, const uint8_t* d_q, size_t ldq,
+                                         size_t nvar, double scale,
+                                         saige_variant_result* d_out);
 /* Hard calls straight from a .bed body (PlinkReader::read_marker + test_marker). */
 int saige_test_markers_plink(saige_engine* e, const uint8_t* bed_body, size_t bytes_per_marker,
                              size_t n_total, const uint32_t* sample_idx, size_t nvar,
