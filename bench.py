@@ -432,7 +432,7 @@ def main():
                     help="host threads of the CPU baseline (the box's CPU share)")
     ap.add_argument("--no-step2", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--precision", default="int8x2", choices=["int8", "int8x2"],
+    ap.add_argument("--precision", default="int8x2", choices=["int8", "int8x2", "bf16", "fp64"],
                     help="int8 form timed for the headline value: three operand slices (int8) "
                          "or two (int8x2); the other form is timed beside it")
     ap.add_argument("--traffic-bytes", type=float, default=None,
@@ -440,7 +440,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (counter calibration)")
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "families", "biobank", "learned", "loco", "region"],
+                    choices=["headline", "families", "biobank", "learned", "loco", "region",
+                             "product"],
                     help="headline (default): 4 traits at 400,000 x 150,000 fitted together "
                          "plus step 2 over 1,000,000 variants; families: one trait at 50,000 x "
                          "100,000; biobank: the 4-trait fit alone; learned: the SPA root "
@@ -449,6 +450,12 @@ def main():
                     help="headline: step-2 variants streamed (generated on the device)")
     ap.add_argument("--cpu-fit-markers", type=int, default=500,
                     help="headline: markers of the CPU baseline's sampled f32 GRM product")
+    ap.add_argument("--prod-k", type=int, default=34,
+                    help="product: dense columns per product")
+    ap.add_argument("--prod-biobank", action="store_true",
+                    help="product: the 400,000 x 150,000 biobank cohort")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="run the PCG iterations without HIP graph capture")
     ap.add_argument("--no-families", action="store_true",
                     help="headline: skip the families extra keys")
     ap.add_argument("--ls-n", type=int, default=50000)
@@ -469,6 +476,9 @@ def main():
         hip.device_synchronize()
         log("copied 3 x 1 GiB device to device")
         return
+    if args.no_graphs:
+        import csaige_amd as sg
+        sg.pcg_graphs(False)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the multi-GPU "
               "sharding of the sample axis is a design, not built code", file=sys.stderr)
@@ -488,7 +498,41 @@ def main():
     if args.workload == "families":
         run_families(args)
         return
+    if args.workload == "product":
+        run_product(args)
+        return
     run_headline(args)
+
+
+def run_product(args):
+    """GRM products only (for counter profiles): the families cohort (or, with --prod-biobank,
+    the 400,000 x 150,000 biobank cohort), one handle of the chosen int8 form, --product-reps
+    products of --prod-k dense random columns; one JSON line with the launch log per width."""
+    import csaige_amd as sg
+    from csaige_amd import hip, synth
+    if args.prod_biobank:
+        c = synth.biobank(n=args.bb_n, m=args.bb_m, frac_related=0.15, seed=5)
+    else:
+        c = synth.families(n_fam=args.n_fam, kids=3, m=args.markers, seed=2, h2=0.4,
+                           prevalence=0.05)
+    n, m = c.n, c.bed.shape[0]
+    d_bed = hip.DeviceBuffer(c.bed.nbytes)
+    d_bed.upload(c.bed)
+    ac, nv, af, mac = sg.marker_stats(c.bed, c.bpm, n, m)
+    markers = np.arange(m, dtype=np.uint32)
+    prec = {"int8": sg.PREC_INT8, "int8x2": sg.PREC_INT8X2, "bf16": sg.PREC_MIXED,
+            "fp64": sg.PREC_FP64}[args.precision]
+    k = args.prod_k
+    sg.launch_log_enable(True, 0)
+    ms, kms, kl = time_product(sg, hip, d_bed, c.bpm, n, markers, af, prec, k, args.product_reps)
+    summ = sg.launch_log_summary()
+    sg.launch_log_enable(False)
+    line = {"workload": f"GRM product, n={n} x M={m}, k={k}, {args.precision}",
+            "product_ms": ms, "gemm_launch_ms": kms, "gemm_launches": kl,
+            "per_width": {str(w): {"launches": v[1], "ms_per_launch": v[0] / max(v[1], 1)}
+                          for w, v in summ.items()},
+            "hbm_gbs_per_launch": float((n + 3) // 4) * m / (kms * 1e-3) / 1e9}
+    print(json.dumps(line), flush=True)
 
 
 def run_families(args):
@@ -539,17 +583,20 @@ def run_families(args):
     # operand columns
     L = sg.load()
     fit_width = 16 * -(-nsl * 34 // 16)
-    sg.check(L.saige_launch_log_enable(1, fit_width))
     times, rm = [], None
     for s in range(args.steps):
         dt, rm = one_fit(int8_forms[head])
         times.append(dt)
         describe(f"step {s} mixed {head}", dt, rm)
     mixed_s = sum(times) / len(times)
+    # the launch log in an untimed repetition (its event pairs stay out of the timed fits)
+    sg.check(L.saige_launch_log_enable(1, fit_width))
+    one_fit(int8_forms[head])
     lt, ln = C.c_double(), C.c_uint64()
     sg.check(L.saige_launch_log_read(fit_width, C.byref(lt), C.byref(ln)))
     sg.check(L.saige_launch_log_enable(0, 0))
     fit_launch_ms, fit_launches = lt.value / max(ln.value, 1), int(ln.value)
+    log(f"PCG iterations captured / replayed / eager so far: {sg.pcg_graph_counts()}")
     log(f"{head} GEMM launches at {fit_width} operand columns in the timed fits: "
         f"{fit_launches}, {fit_launch_ms:.4f} ms each")
     other_s, ro = one_fit(int8_forms[other])
@@ -752,7 +799,7 @@ def run_families(args):
                                  MEASURED_TRAFFIC.get((args.n_fam, args.markers, head))),
                      "kernel": f"i8_gemm_kernel (int8 MFMA genotype GEMM, {head}), k={k}: "
                                f"{fit_launch_ms:.4f} ms/launch over the {fit_launches} launches "
-                               f"at {fit_width} operand columns in the timed fits (HIP events "
+                               f"at {fit_width} operand columns in one untimed fit (HIP events "
                                f"on the stream of the fit's GRM handle; "
                                f"{kern[head]:.3f} ms/launch on dense random operands; with no "
                                f"such launch, the random-operand figure), "
@@ -922,6 +969,7 @@ def run_headline(args):
         f"w{w}: {v[1]} x {v[0] / max(v[1], 1):.3f} ms" for w, v in logsum.items()))
     other_s, ro = fit(forms[other])
     describe(f"{other}", other_s, ro)
+    log(f"PCG iterations captured / replayed / eager so far: {sg.pcg_graph_counts()}")
     fp64_s, rf = None, None
     if not args.no_fp64:
         fp64_s, rf = fit(sg.PREC_FP64)
@@ -985,19 +1033,31 @@ def run_headline(args):
                 "per_width": {str(w): {"launches": v[1], "ms_per_launch": v[0] / max(v[1], 1),
                                        "mean_vectors": v[2] / max(v[1], 1)}
                               for w, v in logsum.items()}}
-        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": gbs / HBM_PEAK_GBS,
-                "traffic": MEASURED_TRAFFIC_BIOBANK.get((n, m, value_form, dom_w)),
-                "kernel": f"i8_gemm_kernel ({head}) at {dom_w} operand columns (mean "
-                          f"{vecs / cnt:.1f} vectors), the GEMM width with the most time in one "
-                          f"{head} fit ({tot_ms / all_ms * 100:.0f}% of its GEMM time): "
-                          f"{launch_ms:.3f} ms/launch over {cnt} launches (HIP events on the "
-                          f"fit's GRM stream, untimed repetition); algorithmic bytes "
-                          f"ceil(n/4)*M = {launch_bytes:.4g} B/launch; "
-                          f"{kern_useful(useful_ops, launch_ms)} useful, "
-                          f"{exec_ops / (launch_ms * 1e-3) / 1e12:.0f} TOP/s executed of "
-                          f"{I8_PEAK_TOPS:.0f}"}
-        log(f"dominant GEMM: width {dom_w}, {launch_ms:.3f} ms/launch, {gbs:.0f} GB/s "
+        # which roof bounds the launch on its algorithmic work: 2 n M k useful ops over
+        # ceil(n/4) M genotype bytes is 8 k ops per byte against the int8 ridge of
+        # 5.0 POP/s / 8 TB/s = 625
+        useful_tops = useful_ops / (launch_ms * 1e-3) / 1e12
+        intensity = useful_ops / launch_bytes
+        mfma_bound = intensity >= I8_PEAK_TOPS / HBM_PEAK_GBS * 1e3
+        traffic = MEASURED_TRAFFIC_BIOBANK.get((n, m, value_form, dom_w))
+        text = (f"i8_gemm_kernel ({head}) at {dom_w} operand columns (mean {vecs / cnt:.1f} "
+                f"vectors), the GEMM width with the most time in one {head} fit "
+                f"({tot_ms / all_ms * 100:.0f}% of its GEMM time): {launch_ms:.3f} ms/launch over "
+                f"{cnt} launches (HIP events on the fit's GRM stream, untimed repetition); "
+                f"algorithmic work per launch 2*n*M*k = {useful_ops:.4g} int8 op over "
+                f"ceil(n/4)*M = {launch_bytes:.4g} B ({intensity:.0f} op/B, int8 ridge 625): "
+                f"{useful_tops:.0f} TOP/s useful, {exec_ops / (launch_ms * 1e-3) / 1e12:.0f} TOP/s "
+                f"executed (operand slices and padding) of {I8_PEAK_TOPS:.0f}, {gbs:.0f} GB/s "
+                f"of packed genotypes ({gbs / HBM_PEAK_GBS:.3f} of HBM)")
+        if mfma_bound:
+            roof = {"bound": "mfma", "achieved": useful_tops, "peak": I8_PEAK_TOPS,
+                    "unit": "TFLOP/s", "frac": useful_tops / I8_PEAK_TOPS, "traffic": traffic,
+                    "kernel": text + "; int8 ops counted as TFLOP/s"}
+        else:
+            roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": text}
+        log(f"dominant GEMM: width {dom_w}, {launch_ms:.3f} ms/launch, {useful_tops:.0f} TOP/s "
+            f"useful ({useful_tops / I8_PEAK_TOPS:.3f} of int8 peak), {gbs:.0f} GB/s "
             f"({gbs / HBM_PEAK_GBS:.3f} of HBM)")
 
     # ---- FP64 roofline floor: the FP64 fit's column-products at 78.6 TFLOP/s ----
@@ -1091,10 +1151,6 @@ def run_headline(args):
         "step2": step2,
     }
     print(json.dumps(line), flush=True)
-
-
-def kern_useful(ops, ms):
-    return f"{ops / (ms * 1e-3) / 1e12:.0f} TOP/s"
 
 
 def families_extra(args, sg, hip, synth, prec):

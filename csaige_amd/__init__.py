@@ -28,7 +28,8 @@ __all__ = ["OnTheFlyGrm", "fit_ai_reml", "solve_sigma", "xvx_inv_xv", "estimate_
            "fit_null", "fit_null_multi", "ScoreTestEngine", "spa_binary_fast", "marker_stats", "decode",
            "PREC_FP64", "PREC_MIXED", "PREC_INT8", "PREC_INT8X2", "PREC_F32", "TRAIT_BINARY", "TRAIT_QUANTITATIVE", "TRAIT_SURVIVAL",
            "SaigeError", "load", "results_to_dict", "SpaRootMlp", "spa_training_set",
-           "launch_log_enable", "launch_log_summary"]
+           "launch_log_enable", "launch_log_summary", "pcg_graphs",
+           "pcg_graph_counts"]
 
 
 def _f64(a):
@@ -321,6 +322,19 @@ def fit_null_multi(bed_body, bytes_per_marker, n_total, m_total, Y, x_cm, traits
     else:
         check(L.saige_fit_null_multi_device(device, C.c_void_p(d_bed), *args))
     return [_fit_null_dict(res[t], mu[t], a[t], cls, alpha[t], pm[t]) for t in range(T)]
+
+
+def pcg_graphs(enable=None):
+    """Query (None) or switch the capture of PCG iterations in HIP graphs; returns the
+    previous state."""
+    return bool(load().saige_pcg_graphs(-1 if enable is None else int(bool(enable))))
+
+
+def pcg_graph_counts():
+    """(captured, replayed, eager) PCG iterations since the process started."""
+    a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    check(load().saige_pcg_graph_counts(C.byref(a), C.byref(b), C.byref(c)))
+    return a.value, b.value, c.value
 
 
 def launch_log_enable(on=True, width=0):

@@ -121,6 +121,9 @@ SIGNATURES = [
     ("saige_launch_log_summary", C.c_int, [C.POINTER(C.c_uint32), dp, C.POINTER(C.c_uint64),
                                           C.POINTER(C.c_uint64), C.c_size_t,
                                           C.POINTER(C.c_size_t)]),
+    ("saige_pcg_graphs", C.c_int, [C.c_int]),
+    ("saige_pcg_graph_counts", C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint64)]),
     ("saige_pcg_solve_sigma", C.c_int, [vp, dp, dp, dp, sz, sz, C.c_double, C.c_uint32, dp, sz,
                                         C.POINTER(PcgInfo)]),
     ("saige_fit_ai_reml", C.c_int, [vp, dp, dp, sz, sz, C.c_int, C.POINTER(AiRemlConfig),

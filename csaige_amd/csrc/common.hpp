@@ -52,6 +52,11 @@ void clear_error();
 // saige_release_cached_memory() empties the cache.
 void* cache_alloc(size_t bytes, size_t* rounded, int* device);
 void cache_free(void* p, size_t rounded, int device);
+// Counts every allocation and release of a DevBuf in the process.  A captured HIP graph
+// bakes in buffer addresses, so it is replayed only while this count is the one it was
+// captured at.
+uint64_t devbuf_generation();
+void devbuf_generation_bump();
 
 // Grow-only device buffer owned by a handle.
 struct DevBuf {
@@ -64,13 +69,17 @@ struct DevBuf {
     if (want == 0) return SAIGE_OK;
     size_t got = 0;
     p = cache_alloc(want, &got, &device);
+    devbuf_generation_bump();
     if (!p) return SAIGE_ERR_NOMEM;
     bytes = got;
     return SAIGE_OK;
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
   void release() {
-    if (p) cache_free(p, bytes, device);
+    if (p) {
+      cache_free(p, bytes, device);
+      devbuf_generation_bump();
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -229,14 +238,38 @@ struct ProductSink {
   uint64_t columns = 0;
   virtual int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo,
                       const double* w, const double* tau, int slices = 0) = 0;
+  // Whether this sink's products may be captured in a HIP graph by the calling thread
+  // (-1 no; otherwise a tag that distinguishes the product variants it issues).
+  virtual int graph_tag() const { return -1; }
   virtual ~ProductSink() = default;
+};
+
+// One captured PCG iteration: the launches of one iteration at one active width, for one
+// block solve's buffers; replayed while the DevBuf generation is the one it was captured at.
+struct PcgGraph {
+  uint64_t key[12];
+  uint64_t generation = 0;
+  hipGraphExec_t exec = nullptr;  // null: seen once, not captured yet
 };
 
 // Workspace of one batched PCG solve (one per concurrent fit).
 struct PcgWork {
   DevBuf vec, scal, part;
   HostBuf host;  // active-column list and column states read back each iteration
+  std::vector<PcgGraph> graphs;
+  void drop_graphs() {
+    for (PcgGraph& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+  ~PcgWork() { drop_graphs(); }
+  PcgWork() = default;
+  PcgWork(const PcgWork&) = delete;
+  PcgWork& operator=(const PcgWork&) = delete;
 };
+
+// Process-wide switch of the PCG iteration graphs (on by default).
+bool pcg_graphs_on();
 
 }  // namespace saige
 
@@ -309,6 +342,7 @@ struct DirectSink : ProductSink {
   saige_grm* h;
   int loco = -1;
   explicit DirectSink(saige_grm* hh) : h(hh) {}
+  int graph_tag() const override { return loco < 0 ? 0 : 1 + loco; }
   int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
               const double* tau, int slices = 0) override {
     columns += k;

@@ -4,6 +4,8 @@
 // set_sample_subset, read_marker) and traits.rs:42-54 (MarkerData::compute_af).
 #include "common.hpp"
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -54,6 +56,10 @@ void cache_release_locked(MemCache& c, int device) {
   (void)hipSetDevice(cur);
 }
 }  // namespace
+
+static std::atomic<uint64_t> g_devbuf_generation{0};
+uint64_t devbuf_generation() { return g_devbuf_generation.load(std::memory_order_acquire); }
+void devbuf_generation_bump() { g_devbuf_generation.fetch_add(1, std::memory_order_acq_rel); }
 
 void* cache_alloc(size_t bytes, size_t* rounded, int* device) {
   int dev = 0;

@@ -611,7 +611,7 @@ struct MixedState {
   // per-product workspaces
   DevBuf vhi, vlo, vs, whi, wlo, ws_, w, part1, part2, red, cs, mean, nu, s2p;
   // int8 path: sliced operands, per-column slice scales, integer partials
-  DevBuf vq, wq, ipart1, ipart2;
+  DevBuf vq, wq, ipart1, ipart2, cols;
   // missing calls: per-marker sample lists, per-sample marker lists, gather rows
   MissList miss_mk, miss_sm;
   DevBuf mrows, mmean;
@@ -976,10 +976,22 @@ __device__ __forceinline__ double q127(double t) { return fmin(fmax(rint(t), -12
 // the partial sum of wt * xs (wt = 1 when null) for the rank-one terms.  One block per
 // (chunk, column); grid.y = k + (KPT - 2k - k3): the last rows of the grid clear the padding
 // columns.
+// The column's mean and slice scales, finished once per operand pass from the fixed-order
+// partials: ms4[4c .. 4c + 3] = mean, s0, s1, s2.
+__global__ void col_mean_scales_kernel(size_t k, const double* __restrict__ psum,
+                                       const double* __restrict__ pmax,
+                                       const double* __restrict__ pmin, double inv_len,
+                                       size_t nchunks, double* __restrict__ ms4) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  const double mean = chunk_sum(psum, c, nchunks) * inv_len;
+  ms4[4 * c] = mean;
+  slice_scales(pmax, pmin, mean, c, nchunks, ms4 + 4 * c + 1);
+}
+
 __global__ __launch_bounds__(kDotBlock) void quantize_slices_kernel(
     const double* __restrict__ x, size_t ldx, size_t len, size_t k, const SliceMap sm,
-    size_t k3, const double* __restrict__ psum, const double* __restrict__ pmax,
-    const double* __restrict__ pmin, double inv_len, size_t nchunks, size_t pad_len,
+    size_t k3, const double* __restrict__ ms4, size_t nchunks, size_t pad_len,
     int8_t* __restrict__ q, double* __restrict__ xs, size_t ldxs,
     const double* __restrict__ wt, double* __restrict__ pdot) {
   const size_t c = blockIdx.y;
@@ -990,13 +1002,8 @@ __global__ __launch_bounds__(kDotBlock) void quantize_slices_kernel(
     return;
   }
   const int c3 = third_col(sm, k, c);
-  __shared__ double cs_s[4];
-  if (threadIdx.x == 0) {
-    cs_s[0] = chunk_sum(psum, c, nchunks) * inv_len;
-    slice_scales(pmax, pmin, cs_s[0], c, nchunks, cs_s + 1);
-  }
-  __syncthreads();
-  const double mu = cs_s[0], s0 = cs_s[1], s1 = cs_s[2], s2 = cs_s[3];
+  const double mu = ms4[4 * c], s0 = ms4[4 * c + 1], s1 = ms4[4 * c + 2], s2 = ms4[4 * c + 3];
+  (void)nchunks;
   double acc = 0.0;
   for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) {
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
@@ -1187,23 +1194,25 @@ __device__ __forceinline__ void i8_tile_sum(const int* __restrict__ part, size_t
 }
 
 // Column scalars of the epilogues: [mean or nu, s0, s1, s2, cs or S2p] and the operand
-// column of the third slice (-1 without one), rebuilt from the operand pass's partials.
+// column of the third slice (-1 without one), finished once per product from the operand
+// pass's fixed-order partials (i8_col_scalars_kernel) and read by every epilogue block.
 struct I8Col {
   double v[5];
   int c3;
 };
-__device__ __forceinline__ void i8_col_scalars(I8Col* col, size_t k, const SliceMap& sm,
-                                               const double* __restrict__ psum,
-                                               const double* __restrict__ pmax,
-                                               const double* __restrict__ pmin,
-                                               const double* __restrict__ pdot, double inv_len,
-                                               size_t nch) {
-  for (size_t c = threadIdx.x; c < k; c += 256) {
-    col[c].v[0] = chunk_sum(psum, c, nch) * inv_len;
-    slice_scales(pmax, pmin, col[c].v[0], c, nch, &col[c].v[1]);
-    col[c].v[4] = chunk_sum(pdot, c, nch);
-    col[c].c3 = third_col(sm, k, c);
-  }
+__global__ void i8_col_scalars_kernel(size_t k, const SliceMap sm, const double* __restrict__ ms4,
+                                      const double* __restrict__ pdot, size_t nch,
+                                      I8Col* __restrict__ col) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  I8Col x;
+  for (int j = 0; j < 4; ++j) x.v[j] = ms4[4 * c + j];
+  x.v[4] = chunk_sum(pdot, c, nch);
+  x.c3 = third_col(sm, k, c);
+  col[c] = x;
+}
+__device__ __forceinline__ void i8_col_load(I8Col* col, size_t k, const I8Col* __restrict__ src) {
+  for (size_t c = threadIdx.x; c < k; c += 256) col[c] = src[c];
 }
 __device__ __forceinline__ double i8_recombine(const int* __restrict__ trow, const I8Col& cc,
                                                size_t k, size_t c) {
@@ -1216,14 +1225,13 @@ __device__ __forceinline__ double i8_recombine(const int* __restrict__ trow, con
 // T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c; W_mc = s_m T_mc (column-major, ldw).
 __global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
     const int* __restrict__ part, size_t splits, size_t m_pad, size_t m, size_t k, size_t kp,
-    const SliceMap sm, const double* __restrict__ psum, const double* __restrict__ pmax,
-    const double* __restrict__ pmin, const double* __restrict__ pcs, double inv_n, size_t nch,
-    const double* __restrict__ sc, const double* __restrict__ p2, const double* __restrict__ zs,
-    double* __restrict__ w, size_t ldw) {
+    const I8Col* __restrict__ cols, const double* __restrict__ sc,
+    const double* __restrict__ p2, const double* __restrict__ zs, double* __restrict__ w,
+    size_t ldw) {
   extern __shared__ int tile[];
   __shared__ I8Col col[kI8MaxVec];
   const int er = epi_rows(kp);
-  i8_col_scalars(col, k, sm, psum, pmax, pmin, pcs, inv_n, nch);
+  i8_col_load(col, k, cols);
   const size_t row0 = (size_t)blockIdx.x * er;
   i8_tile_sum(part, splits, m_pad, row0, er, kp, tile);
   const int r = threadIdx.x % er;
@@ -1241,15 +1249,13 @@ __global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
 // Stage-2 epilogue: M Y_ic = sum_j s_jc I_jic + nu_c (r_i - P2) - S2p_c; optional Sigma form.
 __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
     const int* __restrict__ part, size_t splits, size_t n_pad, size_t n, size_t k, size_t kp,
-    const SliceMap sm, const double* __restrict__ psum, const double* __restrict__ pmax,
-    const double* __restrict__ pmin, const double* __restrict__ ps2p, double inv_m, size_t nch,
-    const double* __restrict__ rsum, double P2, double* __restrict__ out, size_t ldo,
-    const double* __restrict__ v, size_t ldv, const double* __restrict__ wts, double tau0,
-    double tau1) {
+    const I8Col* __restrict__ cols, double inv_m, const double* __restrict__ rsum, double P2,
+    double* __restrict__ out, size_t ldo, const double* __restrict__ v, size_t ldv,
+    const double* __restrict__ wts, double tau0, double tau1) {
   extern __shared__ int tile[];
   __shared__ I8Col col[kI8MaxVec];
   const int er = epi_rows(kp);
-  i8_col_scalars(col, k, sm, psum, pmax, pmin, ps2p, inv_m, nch);
+  i8_col_load(col, k, cols);
   const size_t row0 = (size_t)blockIdx.x * er;
   i8_tile_sum(part, splits, n_pad, row0, er, kp, tile);
   const int r = threadIdx.x % er;
@@ -1390,6 +1396,10 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
   SAIGE_TRY(ms->ws_.ensure(m * k * 8));
   SAIGE_TRY(ms->w.ensure(m * k * 8));
   SAIGE_TRY(ms->red.ensure(4 * std::max(nch_n, nch_m) * k * 8));
+  SAIGE_TRY(ms->cols.ensure(k * sizeof(I8Col) + k * 4 * 8));
+  I8Col* colsc = ms->cols.as<I8Col>();
+  double* ms4 = reinterpret_cast<double*>(ms->cols.as<char>() + k * sizeof(I8Col));
+  const unsigned cb = (unsigned)ceil_div(k, 64);
 
   // Eight launches per product.  Column scalars (means, slice scales, sums of the sliced
   // values) stay as fixed-order partials per kDotChunk rows and are finished by the kernel
@@ -1406,8 +1416,8 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
   // ---- V: column sums, maxima and minima, slices + sums of the sliced values ----
   col_stats_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n,
                                                                              psum, pmax, pmin);
-  quantize_slices_kernel<<<gq_n, kDotBlock, 0, s>>>(d_v, ldv, n, k, sm, k3, psum, pmax, pmin,
-                                                    1.0 / (double)n, nch_n, n_pad,
+  col_mean_scales_kernel<<<cb, 64, 0, s>>>(k, psum, pmax, pmin, 1.0 / (double)n, nch_n, ms4);
+  quantize_slices_kernel<<<gq_n, kDotBlock, 0, s>>>(d_v, ldv, n, k, sm, k3, ms4, nch_n, n_pad,
                                                     ms->vq.as<int8_t>(), ms->vs.as<double>(), n,
                                                     nullptr, pdot);
 
@@ -1415,10 +1425,10 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
   size_t splits1 = 0;
   SAIGE_TRY(i8_gemm_dispatch(kt, h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
                              n_pad, groups, ms->ipart1, &splits1, k));
+  i8_col_scalars_kernel<<<cb, 64, 0, s>>>(k, sm, ms4, pdot, nch_n, colsc);
   i8_stage1_epilogue_kernel<<<(unsigned)(m_pad / er), 256, epi_lds, s>>>(
-      ms->ipart1.as<int>(), splits1, m_pad, m, k, kpt, sm, psum, pmax, pmin, pdot,
-      1.0 / (double)n, nch_n, ms->sc.as<double>(), ms->p2.as<double>(), ms->zs.as<double>(),
-      ms->w.as<double>(), m);
+      ms->ipart1.as<int>(), splits1, m_pad, m, k, kpt, colsc, ms->sc.as<double>(),
+      ms->p2.as<double>(), ms->zs.as<double>(), ms->w.as<double>(), m);
   SAIGE_TRY(missing_stage1(h, ms->vs.as<double>(), n, k, nullptr, psum, nch_n, ms->w.as<double>(), m));
   LocoCodes lc;
   if (loco_codes(h, c0, k, lc)) loco_mask(h, ms->w.as<double>(), 1, m, m, k, lc, s);
@@ -1426,19 +1436,21 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
   // ---- W: column sums over real markers, maxima and minima, slices + S2p partials ----
   col_stats_kernel<<<dim3((unsigned)nch_m, (unsigned)k), kDotBlock, 0, s>>>(
       ms->w.as<double>(), m, m, nch_m, psum, pmax, pmin);
-  quantize_slices_kernel<<<gq_m, kDotBlock, 0, s>>>(ms->w.as<double>(), m, m, k, sm, k3, psum,
-                                                    pmax, pmin, 1.0 / (double)m, nch_m, m_pad,
-                                                    ms->wq.as<int8_t>(), ms->ws_.as<double>(), m,
-                                                    ms->p2.as<double>(), pdot);
+  col_mean_scales_kernel<<<cb, 64, 0, s>>>(k, psum, pmax, pmin, 1.0 / (double)m, nch_m, ms4);
+  quantize_slices_kernel<<<gq_m, kDotBlock, 0, s>>>(ms->w.as<double>(), m, m, k, sm, k3, ms4,
+                                                    nch_m, m_pad, ms->wq.as<int8_t>(),
+                                                    ms->ws_.as<double>(), m, ms->p2.as<double>(),
+                                                    pdot);
 
   // ---- stage 2: I = G Q_w (rows = samples, reduction over markers) ----
   size_t splits2 = 0;
   SAIGE_TRY(i8_gemm_dispatch(kt, h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
                              m_pad, n_pad, m_pad, groups, ms->ipart2, &splits2, k));
+  i8_col_scalars_kernel<<<cb, 64, 0, s>>>(k, sm, ms4, pdot, nch_m, colsc);
   i8_stage2_epilogue_kernel<<<(unsigned)(n_pad / er), 256, epi_lds, s>>>(
-      ms->ipart2.as<int>(), splits2, n_pad, n, k, kpt, sm, psum, pmax, pmin, pdot,
-      1.0 / (double)m, nch_m, ms->rsum.as<double>(), ms->P2, d_out, ldo, d_v, ldv, d_w,
-      tau ? tau[0] : 0.0, tau ? tau[1] : 0.0);
+      ms->ipart2.as<int>(), splits2, n_pad, n, k, kpt, colsc, 1.0 / (double)m,
+      ms->rsum.as<double>(), ms->P2, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0,
+      tau ? tau[1] : 0.0);
   SAIGE_HIP(hipGetLastError());
   SAIGE_TRY(missing_stage2(h, ms->ws_.as<double>(), m, k, nullptr, psum, nch_m, d_out, ldo,
                            (tau ? tau[1] : 1.0) / (double)m));

@@ -12,7 +12,9 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstring>
 
 namespace saige {
 
@@ -21,6 +23,8 @@ struct ColState {
   int active, iters, converged, pad;
 };
 
+// tau[0], tau[1] live in device memory (PcgWork scalars), so that a captured iteration reads
+// the current solve's values.
 __device__ __forceinline__ double precond_apply(double r, double w, double tau0, double tau1) {
   double diag = tau0 / fmax(w, 1e-30) + tau1;
   return fabs(diag) > 1e-30 ? r / diag : r;
@@ -49,8 +53,9 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
 __global__ __launch_bounds__(kDotBlock) void pcg_init_kernel(
     const double* __restrict__ b, size_t ldb, double* __restrict__ x, size_t ldx,
     double* __restrict__ r, double* __restrict__ z, double* __restrict__ p, size_t ld,
-    const double* __restrict__ w, double tau0, double tau1, size_t n, size_t nchunks,
+    const double* __restrict__ w, const double* __restrict__ tau, size_t n, size_t nchunks,
     double* __restrict__ part) {
+  const double tau0 = tau[0], tau1 = tau[1];
   const size_t c = blockIdx.y;
   const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, n);
   double s_rz = 0.0, s_bb = 0.0, s_rr = 0.0;
@@ -79,8 +84,10 @@ __global__ __launch_bounds__(kDotBlock) void pcg_init_kernel(
 __global__ __launch_bounds__(1024) void pcg_init_fin(const double* __restrict__ part,
                                                      size_t nchunks, size_t k, double tol,
                                                      ColState* __restrict__ st,
-                                                     int* __restrict__ n_active) {
+                                                     int* __restrict__ n_active,
+                                                     int* __restrict__ iter) {
   __shared__ int flag[1024];
+  if (threadIdx.x == 0) *iter = 0;
   const size_t c = threadIdx.x;
   int act = 0;
   if (c < k) {
@@ -140,11 +147,13 @@ __global__ __launch_bounds__(kDotBlock) void pcg_pap_kernel(const double* __rest
 __global__ __launch_bounds__(kDotBlock) void pcg_update_kernel(
     double* __restrict__ x, size_t ldx, double* __restrict__ r, double* __restrict__ z,
     const double* __restrict__ p, const double* __restrict__ ap, size_t ld,
-    const double* __restrict__ w, double tau0, double tau1, size_t n, size_t nchunks,
-    const double* __restrict__ pap_part, int iter, ColState* __restrict__ st,
-    double* __restrict__ part) {
+    const double* __restrict__ w, const double* __restrict__ tau, size_t n, size_t nchunks,
+    const double* __restrict__ pap_part, const int* __restrict__ iter_d,
+    ColState* __restrict__ st, double* __restrict__ part) {
   const size_t c = blockIdx.y;
   if (!st[c].active) return;
+  const double tau0 = tau[0], tau1 = tau[1];
+  const int iter = *iter_d;
   __shared__ double alpha_s;
   __shared__ int stop_s;
   if (threadIdx.x == 0) {
@@ -187,11 +196,13 @@ __global__ __launch_bounds__(kDotBlock) void pcg_update_kernel(
 // columns still active: n_active[0] is their number, n_active[1..] their indices in
 // increasing order (one block of 1024 threads covers all columns).
 __global__ __launch_bounds__(1024) void pcg_beta_fin(const double* __restrict__ part,
-                                                     size_t nchunks, size_t k, int iter,
+                                                     size_t nchunks, size_t k,
+                                                     int* __restrict__ iter_d,
                                                      ColState* __restrict__ st,
                                                      int* __restrict__ n_active) {
   __shared__ int flag[1024];
   const size_t c = threadIdx.x;
+  const int iter = *iter_d;
   if (c < k && st[c].active) {
     double rz = 0.0, rr = 0.0;
     for (size_t q = 0; q < nchunks; ++q) {
@@ -214,7 +225,24 @@ __global__ __launch_bounds__(1024) void pcg_beta_fin(const double* __restrict__ 
     for (size_t j = 0; j < k; ++j)
       if (flag[j]) n_active[1 + cnt++] = (int)j;
     n_active[0] = cnt;
+    *iter_d = iter + 1;  // every thread has read it before the barrier above
   }
+}
+
+// Sigma p = tau0 p / max(w, 1e-30) + tau1 GRM p for the active columns (ap holds GRM p on
+// entry); the same expression as the closure of ai_reml.rs:142-152.
+__global__ void pcg_sigma_kernel(const double* __restrict__ p, double* __restrict__ ap, size_t ld,
+                                 const double* __restrict__ w, const double* __restrict__ tau,
+                                 size_t n, const ColState* __restrict__ st) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t c = blockIdx.y;
+  if (i >= n || !st[c].active) return;
+  ap[i + c * ld] = tau[0] * p[i + c * ld] / fmax(w[i], 1e-30) + tau[1] * ap[i + c * ld];
+}
+
+__global__ void pcg_set_tau_kernel(double* __restrict__ tau, double t0, double t1) {
+  tau[0] = t0;
+  tau[1] = t1;
 }
 
 // Active columns gathered into a contiguous block, and the product of the block scattered
@@ -246,6 +274,10 @@ __global__ void pcg_final(size_t k, uint32_t max_iter, ColState* __restrict__ st
   st[c].active = 0;
 }
 
+static std::atomic<bool> g_pcg_graphs{true};
+static std::atomic<uint64_t> g_graph_captures{0}, g_graph_replays{0}, g_graph_eager{0};
+bool pcg_graphs_on() { return g_pcg_graphs.load(std::memory_order_relaxed); }
+
 int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double* d_w,
                      const double tau[2], const double* d_b, size_t ldb, size_t k, double tol,
                      uint32_t max_iter, double* d_x, size_t ldx, saige_pcg_info* info,
@@ -259,7 +291,7 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   SAIGE_TRY(ws.vec.ensure(6 * ld * k * 8));
   SAIGE_TRY(ws.part.ensure(4 * k * nchunks * 8));
   const size_t list_bytes = round_up((k + 1) * sizeof(int), 256);
-  SAIGE_TRY(ws.scal.ensure(list_bytes + k * sizeof(ColState)));
+  SAIGE_TRY(ws.scal.ensure(list_bytes + k * sizeof(ColState) + 256));
   double* R = ws.vec.as<double>();
   double* Z = R + ld * k;
   double* P = Z + ld * k;
@@ -268,15 +300,18 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   double* APC = PC + ld * k;  // their product
   ColState* st = reinterpret_cast<ColState*>(ws.scal.as<char>() + list_bytes);
   int* d_nact = ws.scal.as<int>();  // [count, active column indices]
+  // tau of this solve and the iteration counter, read by the iteration's kernels
+  double* d_tau = reinterpret_cast<double*>(ws.scal.as<char>() + list_bytes + k * sizeof(ColState));
+  int* d_iter = reinterpret_cast<int*>(d_tau + 2);
   double* part = ws.part.as<double>();
   double* pap_part = part + 3 * k * nchunks;  // p'Ap partials, apart from the update's
-  const double tau0 = tau[0], tau1 = tau[1];
   const unsigned kb = (unsigned)ceil_div(k, 256);
   dim3 gchunk((unsigned)nchunks, (unsigned)k);
 
-  pcg_init_kernel<<<gchunk, kDotBlock, 0, s>>>(d_b, ldb, d_x, ldx, R, Z, P, ld, d_w, tau0, tau1, n,
+  pcg_set_tau_kernel<<<1, 1, 0, s>>>(d_tau, tau[0], tau[1]);
+  pcg_init_kernel<<<gchunk, kDotBlock, 0, s>>>(d_b, ldb, d_x, ldx, R, Z, P, ld, d_w, d_tau, n,
                                                nchunks, part);
-  pcg_init_fin<<<1, 1024, 0, s>>>(part, nchunks, k, tol, st, d_nact);
+  pcg_init_fin<<<1, 1024, 0, s>>>(part, nchunks, k, tol, st, d_nact, d_iter);
   SAIGE_HIP(hipGetLastError());
   // [count, active list | column states] read back in one pinned copy per iteration
   const size_t state_bytes = list_bytes + k * sizeof(ColState);
@@ -286,23 +321,98 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
   SAIGE_HIP(hipMemcpyAsync(list, d_nact, state_bytes, hipMemcpyDeviceToHost, s));
   SAIGE_HIP(hipStreamSynchronize(s));
   int n_active = list[0];
-  for (uint32_t iter = 0; iter < max_iter && n_active > 0; ++iter) {
-    if ((size_t)n_active == k) {
-      SAIGE_TRY(sink->product(P, ld, k, AP, ld, d_w, tau, slices));
+
+  // One iteration's launches at the current width (n_active): the GRM product of the active
+  // columns (gathered when some have stopped), Sigma, p'Ap, the update and the next beta.
+  auto enqueue = [&](size_t width) -> int {
+    if (width == k) {
+      SAIGE_TRY(sink->product(P, ld, k, AP, ld, nullptr, nullptr, slices));
     } else {
       // only the columns still iterating: their results do not depend on which other
       // columns share the product
-      const dim3 gg((unsigned)ceil_div(n, 256), (unsigned)n_active);
+      const dim3 gg((unsigned)ceil_div(n, 256), (unsigned)width);
       pcg_gather_kernel<<<gg, 256, 0, s>>>(P, ld, n, d_nact + 1, 0, PC);
-      SAIGE_TRY(sink->product(PC, ld, (size_t)n_active, APC, ld, d_w, tau, slices));
+      SAIGE_TRY(sink->product(PC, ld, width, APC, ld, nullptr, nullptr, slices));
       pcg_gather_kernel<<<gg, 256, 0, s>>>(APC, ld, n, d_nact + 1, 1, AP);
     }
+    const dim3 gn((unsigned)ceil_div(n, 256), (unsigned)k);
+    pcg_sigma_kernel<<<gn, 256, 0, s>>>(P, AP, ld, d_w, d_tau, n, st);
     pcg_pap_kernel<<<gchunk, kDotBlock, 0, s>>>(P, AP, ld, n, nchunks, st, pap_part);
-    pcg_update_kernel<<<gchunk, kDotBlock, 0, s>>>(d_x, ldx, R, Z, P, AP, ld, d_w, tau0, tau1, n,
-                                                   nchunks, pap_part, (int)iter, st, part);
-    pcg_beta_fin<<<1, 1024, 0, s>>>(part, nchunks, k, (int)iter, st, d_nact);
-    pcg_p_kernel<<<dim3((unsigned)ceil_div(n, 256), (unsigned)k), 256, 0, s>>>(P, Z, ld, n, st);
+    pcg_update_kernel<<<gchunk, kDotBlock, 0, s>>>(d_x, ldx, R, Z, P, AP, ld, d_w, d_tau, n,
+                                                   nchunks, pap_part, d_iter, st, part);
+    pcg_beta_fin<<<1, 1024, 0, s>>>(part, nchunks, k, d_iter, st, d_nact);
+    pcg_p_kernel<<<gn, 256, 0, s>>>(P, Z, ld, n, st);
     SAIGE_HIP(hipGetLastError());
+    return SAIGE_OK;
+  };
+
+  // Captured iterations: a graph per (block, width, buffers), captured the second time that
+  // iteration shape is met with the DevBuf generation unchanged (the first run, eager, has
+  // sized every workspace the product needs, so the capture allocates nothing) and replayed
+  // while the generation stays.  Not with the kernel timer or the launch log on (their event
+  // records are not captured), nor for sinks that join products across threads.
+  const int tag = sink->graph_tag();
+  const bool graphs = pcg_graphs_on() && tag >= 0 && !h->timer.on && !launch_log_on();
+  for (uint32_t iter = 0; iter < max_iter && n_active > 0; ++iter) {
+    const size_t width = (size_t)n_active;
+    bool done = false;
+    PcgGraph* g = nullptr;
+    uint64_t key[12] = {};
+    if (graphs) {
+      const uint64_t gen = devbuf_generation();
+      const uint64_t kk[12] = {(uint64_t)(uintptr_t)h, (uint64_t)tag, (uint64_t)slices, k, width,
+                               (uint64_t)(uintptr_t)d_x, ldx, (uint64_t)(uintptr_t)d_w,
+                               (uint64_t)(uintptr_t)R, (uint64_t)(uintptr_t)part,
+                               (uint64_t)(uintptr_t)st, n};
+      std::memcpy(key, kk, sizeof(key));
+      for (PcgGraph& c : ws.graphs)
+        if (std::memcmp(c.key, key, sizeof(key)) == 0) g = &c;
+      if (g && g->generation != gen) {  // a workspace moved since: forget the graph
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        g->exec = nullptr;
+        g->generation = 0;
+      }
+      if (g && g->exec) {
+        SAIGE_HIP(hipGraphLaunch(g->exec, s));
+        sink->columns += width;
+        h->products += width;
+        g_graph_replays.fetch_add(1, std::memory_order_relaxed);
+        done = true;
+      } else if (g && g->generation == gen) {  // met before at this generation: capture
+        hipGraph_t graph = nullptr;
+        SAIGE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue(width);
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (rc == SAIGE_OK && ec == hipSuccess && graph && devbuf_generation() == gen) {
+          hipGraphExec_t exec = nullptr;
+          if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+            g->exec = exec;
+            SAIGE_HIP(hipGraphLaunch(exec, s));
+            g_graph_captures.fetch_add(1, std::memory_order_relaxed);
+            done = true;
+          }
+        }
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        if (!done) {  // run it eagerly below
+          sink->columns -= rc == SAIGE_OK ? width : 0;  // the capture pass counted it
+          h->products -= rc == SAIGE_OK ? width : 0;
+          g->generation = 0;
+        }
+      }
+    }
+    if (!done) {
+      SAIGE_TRY(enqueue(width));
+      g_graph_eager.fetch_add(1, std::memory_order_relaxed);
+      if (graphs) {  // remember the shape at the generation after its workspaces exist
+        if (!g) {
+          ws.graphs.emplace_back();
+          g = &ws.graphs.back();
+          std::memcpy(g->key, key, sizeof(key));
+        }
+        g->generation = devbuf_generation();
+      }
+    }
     SAIGE_HIP(hipMemcpyAsync(list, d_nact, state_bytes, hipMemcpyDeviceToHost, s));
     SAIGE_HIP(hipStreamSynchronize(s));
     n_active = list[0];
@@ -328,6 +438,19 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
 }  // namespace saige
 
 using namespace saige;
+
+extern "C" int saige_pcg_graphs(int enable) {
+  const int prev = saige::g_pcg_graphs.load() ? 1 : 0;
+  if (enable >= 0) saige::g_pcg_graphs.store(enable != 0);
+  return prev;
+}
+
+extern "C" int saige_pcg_graph_counts(uint64_t* captured, uint64_t* replayed, uint64_t* eager) {
+  if (captured) *captured = saige::g_graph_captures.load();
+  if (replayed) *replayed = saige::g_graph_replays.load();
+  if (eager) *eager = saige::g_graph_eager.load();
+  return SAIGE_OK;
+}
 
 extern "C" int saige_pcg_solve_sigma(saige_grm* h, const double* w, const double tau[2],
                                      const double* b, size_t ldb, size_t k, double tol,

@@ -168,6 +168,18 @@ typedef struct {
 int saige_pcg_solve_sigma(saige_grm* h, const double* w, const double tau[2], const double* b,
                           size_t ldb, size_t k, double tol, uint32_t max_iter, double* x,
                           size_t ldx, saige_pcg_info* info);
+/* PCG iteration graphs (no counterpart in saige-rs): the launches of one PCG iteration at one
+ * active width (GRM product of the active columns, Sigma, p'Ap, update, beta) are captured in
+ * a HIP graph the second time that iteration shape is met and replayed from then on, while
+ * no device workspace has moved; the host reads the active count between graph launches.
+ * Single-trait solves only (the joined products of several traits or LOCO refits run
+ * eagerly), and not while kernel timing or the launch log is on.  The results are those of
+ * the uncaptured launches bit for bit.  enable: 1 on (the default), 0 off, -1 query; returns
+ * the previous state. */
+int saige_pcg_graphs(int enable);
+/* PCG iterations run since the process started: captured into a new graph, replayed from
+ * one, and launched without a graph (diagnostic). */
+int saige_pcg_graph_counts(uint64_t* captured, uint64_t* replayed, uint64_t* eager);
 
 /* ------------------------------------------------------------------------------
  * AI-REML null-model fit.

@@ -632,3 +632,43 @@ def test_launch_log_summary_per_width():
     finally:
         sg.launch_log_enable(False)
         g.close()
+
+
+# --------------------------------------------------------------------------- PCG graphs
+@pytest.mark.parametrize("prec", [sg.PREC_FP64, sg.PREC_INT8X2], ids=["fp64", "int8x2"])
+def test_pcg_graphs_match_uncaptured_launches(prec):
+    """PCG iterations captured in HIP graphs and replayed give the uncaptured launches'
+    results bit for bit: repeated solves on one handle (the first eager, the second captured,
+    the later ones replayed, with eager work and host reads between them), and whole fit-null
+    runs with the graphs on, off and on again."""
+    n, m = 1100, 1500
+    body, bpm, d, af = _grm_pair(n, m, seed=31, p_missing=0.0)
+    g = sg.OnTheFlyGrm(body, bpm, n, af, precision=prec)
+    rng = np.random.default_rng(3)
+    w = rng.uniform(0.05, 0.25, n)
+    B = rng.standard_normal((34, n))
+    prev = sg.pcg_graphs(False)
+    try:
+        want = sg.solve_sigma(g, w, (1.0, 0.4), B)
+        sg.pcg_graphs(True)
+        for rep in range(4):
+            got = sg.solve_sigma(g, w, (1.0, 0.4), B)
+            np.testing.assert_array_equal(got[0], want[0], err_msg=f"solve {rep}")
+            np.testing.assert_array_equal(got[1], want[1])
+            g.mat_vec(B[0])  # eager work between the replays
+        c = synth.unrelated(n=1500, m_grm=3000, m_test=0, seed=12, h2=0.5, prevalence=0.2)
+        idx = np.arange(c.n, dtype=np.uint32)
+        runs = []
+        for on in (True, False, True):
+            sg.pcg_graphs(on)
+            runs.append(sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], c.y, c.x, sample_idx=idx,
+                                    precision=prec))
+        for r in (runs[0], runs[2]):
+            _say(f"graphs prec {prec}: tau {r['tau'][1]!r} vs {runs[1]['tau'][1]!r}")
+            for key in ("tau", "mu", "per_marker_vr"):
+                np.testing.assert_array_equal(r[key], runs[1][key], err_msg=key)
+            for key in ("vr", "iterations", "pcg_iterations", "grm_products"):
+                assert r[key] == runs[1][key], key
+    finally:
+        sg.pcg_graphs(prev)
+        g.close()
