@@ -247,7 +247,7 @@ def estimate_variance_ratio(grm, dosages, macs, mu, w, tau, x_cm, a_flat, trait=
 def fit_null(bed_body, bytes_per_marker, n_total, m_total, y, x_cm, sample_idx=None,
              trait=TRAIT_BINARY, precision=PREC_FP64, max_iter=30, tol=1e-5, seed=12345,
              n_random_vectors=30, min_maf=0.01, max_missing=0.15, n_markers_vr=30, device=0,
-             d_bed=None):
+             d_bed=None, use_categorical_vr=False):
     """fit_null::run after phenotype parsing (fit_null.rs:199-394).  With ``d_bed`` (a device
     pointer holding the .bed body) the genotypes are read from HBM instead of host memory."""
     L = load()
@@ -258,7 +258,8 @@ def fit_null(bed_body, bytes_per_marker, n_total, m_total, y, x_cm, sample_idx=N
     x = _f64(x_cm)
     p = x.shape[0]
     cfg = _lib.FitNullConfig(_lib.AiRemlConfig(max_iter, tol, 1e-5, 500, n_random_vectors, seed),
-                             min_maf, max_missing, n_markers_vr, precision)
+                             min_maf, max_missing, n_markers_vr, precision,
+                             int(bool(use_categorical_vr)))
     r = _lib.FitNullResult()
     mu = np.empty(n)
     a = np.empty(p * n)
@@ -285,13 +286,15 @@ def _fit_null_dict(r, mu, a, cls, alpha, pm):
                 n_grm_markers=r.n_grm_markers, n_vr_markers=r.n_vr_markers, mu=mu,
                 xvx_inv_xv=a, marker_class=cls, seconds_qc=r.seconds_qc,
                 seconds_reml=r.seconds_reml, seconds_vr=r.seconds_vr,
-                seconds_total=r.seconds_total)
+                seconds_total=r.seconds_total,
+                categorical_vr=[(r.categorical_vr_upper[b], r.categorical_vr[b])
+                                for b in range(r.n_categorical_vr)])
 
 
 def fit_null_multi(bed_body, bytes_per_marker, n_total, m_total, Y, x_cm, traits=None,
                    sample_idx=None, precision=PREC_FP64, max_iter=30, tol=1e-5, seed=12345,
                    n_random_vectors=30, min_maf=0.01, max_missing=0.15, n_markers_vr=30,
-                   device=0, d_bed=None):
+                   device=0, d_bed=None, use_categorical_vr=False):
     """Several phenotypes (rows of ``Y``, model sample order) fitted together over one GRM:
     each trait follows its own fit_null, the GRM products of all traits are joined into one
     multi-column product.  Returns one fit_null dict per trait."""
@@ -307,7 +310,8 @@ def fit_null_multi(bed_body, bytes_per_marker, n_total, m_total, Y, x_cm, traits
     p = x.shape[0]
     tt = np.ascontiguousarray([TRAIT_BINARY] * T if traits is None else traits, dtype=np.int32)
     cfg = _lib.FitNullConfig(_lib.AiRemlConfig(max_iter, tol, 1e-5, 500, n_random_vectors, seed),
-                             min_maf, max_missing, n_markers_vr, precision)
+                             min_maf, max_missing, n_markers_vr, precision,
+                             int(bool(use_categorical_vr)))
     res = (_lib.FitNullResult * T)()
     mu = np.empty((T, n))
     a = np.empty((T, p * n))

@@ -60,7 +60,7 @@ class VrConfig(C.Structure):
 class FitNullConfig(C.Structure):
     _fields_ = [("reml", AiRemlConfig), ("min_maf", C.c_double),
                 ("max_missing_rate", C.c_double), ("n_markers_vr", C.c_uint32),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("use_categorical_vr", C.c_uint32)]
 
 
 class FitNullResult(C.Structure):
@@ -68,7 +68,8 @@ class FitNullResult(C.Structure):
                 ("vr_markers_used", C.c_uint32), ("n_grm_markers", C.c_uint32),
                 ("n_vr_markers", C.c_uint32), ("seconds_qc", C.c_double),
                 ("seconds_reml", C.c_double), ("seconds_vr", C.c_double),
-                ("seconds_total", C.c_double)]
+                ("seconds_total", C.c_double), ("n_categorical_vr", C.c_uint32),
+                ("categorical_vr_upper", C.c_double * 8), ("categorical_vr", C.c_double * 8)]
 
 
 class VariantResult(C.Structure):

@@ -684,12 +684,32 @@ static int vr_fill(hipStream_t s, const VrSource& src, size_t n, const std::vect
   return SAIGE_OK;
 }
 
+// compute_categorical_vr (variance_ratio.rs:263-287) with the default MAC bins of
+// VarianceRatioConfig (:50-51): (upper bound, mean ratio) of every bin that holds a marker.
+static void categorical_vr(const std::vector<double>& vr, const std::vector<double>& mac,
+                           std::vector<std::pair<double, double>>& out) {
+  static const double lo[8] = {0.5, 1.5, 2.5, 3.5, 4.5, 5.5, 10.5, 20.5};
+  static const double hi[8] = {1.5, 2.5, 3.5, 4.5, 5.5, 10.5, 20.5, INFINITY};
+  out.clear();
+  for (int b = 0; b < 8; ++b) {
+    double sum = 0.0;
+    size_t cnt = 0;
+    for (size_t i = 0; i < vr.size(); ++i)
+      if (mac[i] > lo[b] && mac[i] <= hi[b]) {
+        sum += vr[i];
+        ++cnt;
+      }
+    if (cnt) out.emplace_back(hi[b], sum / (double)cnt);
+  }
+}
+
 static int estimate_vr_dev(saige_grm* h, ProductSink* sink, const VrSource& src,
                            const double* mac, size_t m,
                            const double* d_mu, const double* d_w, const double tau[2], int trait,
                            const double* d_x, size_t ldx, const double* d_at, size_t p,
                            const saige_vr_config* cfg, double* vr_out, uint32_t* n_used,
-                           double* per_marker, RemlState& st) {
+                           double* per_marker, RemlState& st,
+                           std::vector<std::pair<double, double>>* cat = nullptr) {
   hipStream_t s = h->stream;
   const size_t n = h->g.n;
   const size_t ld = round_up(n, 64);
@@ -702,7 +722,7 @@ static int estimate_vr_dev(saige_grm* h, ProductSink* sink, const VrSource& src,
   DevBuf G, GT, SG, WG, acb, agb, flipb, ids, onesb;
   SAIGE_TRY(onesb.ensure(ld * 8));
   fill_kernel<<<grid1(n), 256, 0, s>>>(onesb.as<double>(), n, 1.0);
-  std::vector<double> vr;
+  std::vector<double> vr, vr_mac;
   size_t target = cfg->n_markers, tested = 0, cursor = 0;
   bool done = false;
   while (!done && cursor < m) {
@@ -752,6 +772,7 @@ static int estimate_vr_dev(saige_grm* h, ProductSink* sink, const VrSource& src,
       double var2 = v2[c];
       if (var2 < 1e-30) continue;
       vr.push_back(var1 / var2);
+      vr_mac.push_back(mac[chunk[c]]);
       ++tested;
       if (tested >= target) {
         if (cal_cv(vr) <= cfg->ratio_cv_cutoff) done = true;
@@ -769,6 +790,7 @@ static int estimate_vr_dev(saige_grm* h, ProductSink* sink, const VrSource& src,
   *vr_out = mean;
   if (n_used) *n_used = (uint32_t)vr.size();
   if (per_marker) std::copy(vr.begin(), vr.end(), per_marker);
+  if (cat) categorical_vr(vr, vr_mac, *cat);
   (void)d_x;
   return SAIGE_OK;
 }
@@ -950,10 +972,16 @@ static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const dou
     src.d_rows = vr_rows;
     src.row_bytes = h->g.row_bytes;
     saige_vr_config vc{cfg->n_markers_vr, 20.0, 0.001, cfg->reml.seed};
+    std::vector<std::pair<double, double>> cat;
     SAIGE_TRY(estimate_vr_dev(h, sink, src, ms.vr_mac.data(), ms.vr_ids.size(),
                               st.mu.as<double>(), st.w.as<double>(), result->reml.tau, trait_type,
                               st.x.as<double>(), ld, st.at.as<double>(), p, &vc, &vr, &used,
-                              per_marker_vr, st));
+                              per_marker_vr, st, cfg->use_categorical_vr ? &cat : nullptr));
+    result->n_categorical_vr = (uint32_t)cat.size();
+    for (size_t b = 0; b < cat.size() && b < 8; ++b) {
+      result->categorical_vr_upper[b] = cat[b].first;
+      result->categorical_vr[b] = cat[b].second;
+    }
   }
   auto t3 = clk::now();
   result->variance_ratio = vr;

@@ -96,7 +96,9 @@ class NullModel:
         """The model fit_null.rs:382-394 saves, from a ``fit_null`` result: x_cm is the p x n
         covariate array (rows = covariates), i.e. X flat col-major n x p."""
         x = np.asarray(x_cm, dtype=np.float64)
-        vr = VarianceRatioResult(float(fit["vr"]), [], int(fit["vr_markers_used"]),
+        vr = VarianceRatioResult(float(fit["vr"]),
+                                 [(float(u), float(r)) for u, r in fit.get("categorical_vr", [])],
+                                 int(fit["vr_markers_used"]),
                                  np.asarray(fit["per_marker_vr"], dtype=np.float64).copy())
         return cls.new(trait_type, sample_ids, fit["tau"], fit["alpha"], fit["mu"], y,
                        x.ravel(), x.shape[0], fit["xvx_inv_xv"], vr)

@@ -275,7 +275,8 @@ typedef struct {
   double min_maf;            /* 0.01 */
   double max_missing_rate;   /* 0.15 */
   uint32_t n_markers_vr;     /* 30 */
-  int precision;             /* SAIGE_PREC_FP64 | _MIXED | _INT8 | _INT8X2 */
+  int precision;             /* SAIGE_PREC_FP64 | _MIXED | _INT8 | _INT8X2 | _F32 */
+  uint32_t use_categorical_vr; /* --use-categorical-vr (fit_null.rs:94-96, 348): MAC-binned ratios */
 } saige_fit_null_config;
 
 typedef struct {
@@ -285,6 +286,12 @@ typedef struct {
   uint32_t n_grm_markers;
   uint32_t n_vr_markers;
   double seconds_qc, seconds_reml, seconds_vr, seconds_total;
+  /* VarianceRatioResult::categorical_vr (variance_ratio.rs:221-230, 263-287): the bins of
+   * the reference's default MAC boundaries that hold a used marker, as (upper MAC bound,
+   * mean ratio) pairs in bin order; n_categorical_vr is 0 unless use_categorical_vr. */
+  uint32_t n_categorical_vr;
+  double categorical_vr_upper[8];
+  double categorical_vr[8];
 } saige_fit_null_result;
 
 int saige_fit_null(int device, const uint8_t* bed_body, size_t bytes_per_marker, size_t n_total,

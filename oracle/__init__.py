@@ -330,6 +330,23 @@ def xvx_inv_xv(x_cm, w):
     return out
 
 
+def categorical_vr(per_marker_vr, mac):
+    """compute_categorical_vr (saige-core/src/glmm/variance_ratio.rs:263-287) with the default
+    MAC bins of VarianceRatioConfig (:50-51): (upper bound, mean ratio) of each bin that holds
+    a marker, in bin order; a marker is in a bin when lower < MAC <= upper."""
+    lo = [0.5, 1.5, 2.5, 3.5, 4.5, 5.5, 10.5, 20.5]
+    hi = [1.5, 2.5, 3.5, 4.5, 5.5, 10.5, 20.5, float("inf")]
+    out = []
+    for a, b in zip(lo, hi):
+        vals = [v for v, m in zip(per_marker_vr, mac) if a < m <= b]
+        if vals:
+            s = 0.0
+            for v in vals:
+                s += v
+            out.append((b, s / len(vals)))
+    return out
+
+
 def cal_cv(v):
     v = _f64(v)
     return lib().orc_cal_cv(_p(v, _dp), len(v))

@@ -672,3 +672,23 @@ def test_pcg_graphs_match_uncaptured_launches(prec):
     finally:
         sg.pcg_graphs(prev)
         g.close()
+
+
+def test_fit_null_categorical_variance_ratio():
+    """--use-categorical-vr (fit_null.rs:94-96, 348): the MAC-binned ratios
+    (variance_ratio.rs:221-230, 263-287) of the markers the VR estimate used, in the order it
+    used them (the ChaCha8 shuffle of the VR pool); the overall ratio is unchanged."""
+    c = synth.unrelated(n=1500, m_grm=3000, m_test=0, seed=12, h2=0.5, prevalence=0.2)
+    idx = np.arange(c.n, dtype=np.uint32)
+    plain = sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], c.y, c.x, sample_idx=idx)
+    got = sg.fit_null(c.bed, c.bpm, c.n, c.bed.shape[0], c.y, c.x, sample_idx=idx,
+                      use_categorical_vr=True)
+    assert got["vr"] == plain["vr"] and plain["categorical_vr"] == []
+    pool = np.flatnonzero(got["marker_class"] == 2)
+    order = oracle.shuffle(12345, len(pool)).astype(np.int64)
+    used = pool[order[:got["vr_markers_used"]]]
+    mac = sg.marker_stats(c.bed, c.bpm, c.n, c.bed.shape[0])[3][used]
+    want = oracle.categorical_vr(got["per_marker_vr"], mac)
+    _say(f"categorical VR: {got['categorical_vr']} (want {want}) over {len(used)} markers")
+    assert len(want) >= 1
+    assert got["categorical_vr"] == want

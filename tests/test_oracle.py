@@ -550,3 +550,15 @@ def test_pvalue_closed_forms():
         assert abs(oracle.chisq1_sf(t) - special.chdtrc(1, t)) < 2e-16 + 1e-14 * special.chdtrc(1, t) * 1e2
     for z in (-5.0, -1.0, 0.0, 2.0, 6.0):
         assert abs(oracle.norm_cdf(z) - special.ndtr(z)) < 1e-16 + 1e-15 * special.ndtr(z)
+
+
+def test_categorical_vr_bins():
+    """compute_categorical_vr (variance_ratio.rs:263-287): a marker is in the bin whose lower
+    bound it exceeds and whose upper bound it does not; empty bins are left out; the ratio of
+    a bin is the mean over its markers, in the order they were used."""
+    vr = [1.0, 2.0, 3.0, 4.0, 5.0, 6.0]
+    mac = [1.0, 1.5, 2.5, 20.5, 20.6, 1000.0]
+    got = oracle.categorical_vr(vr, mac)
+    assert got == [(1.5, 1.5), (2.5, 3.0), (20.5, 4.0), (float("inf"), 5.5)]
+    assert oracle.categorical_vr([], []) == []
+    assert oracle.categorical_vr([0.9], [0.5]) == []  # MAC 0.5 is in no bin (lower bound excluded)
