@@ -54,3 +54,16 @@ def test_device_dosage_generator_matches_host():
     d_q.download(got)
     assert (want[(3 - first) % 4::4] % 127 != 0).any()  # the noisy variants are there
     np.testing.assert_array_equal(got, want)
+
+
+def test_config_fixture_cohorts_regenerate():
+    """The committed oracle fixtures of the families and biobank configurations were made on
+    the cohorts the generator makes today (same hash of genotypes, phenotypes, covariates)."""
+    import numpy as np
+
+    import tests.golden.make_config_vectors as mk
+    gold = os.path.join(ROOT, "tests", "golden")
+    fam = np.load(os.path.join(gold, "config_families.npz"))
+    bb = np.load(os.path.join(gold, "config_biobank.npz"))
+    assert mk.cohort_hash(mk.families()) == str(fam["cohort_sha256"])
+    assert mk.cohort_hash(mk.biobank()) == str(bb["cohort_sha256"])
