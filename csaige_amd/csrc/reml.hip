@@ -300,9 +300,12 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
   // operand slices of the int8x2 form (the other forms ignore the request): three for every
   // product whose result enters the tau update directly or is computed once (GRM u, GRM P y,
   // the second-stage and variance-ratio solves); two for the main block solves while the
-  // relative tau change is above kFineSwitch, three from then on
+  // relative tau change is above kFineSwitch, three from then on.  Two slices move tau by
+  // about 1e-5 relative; switching at 1e-3 keeps the two-slice iterations well above that
+  // floor (at 1e-4 the fits took up to two extra iterations, at 1e-2 a third more of the
+  // block products ran on three slices)
   constexpr int kFine = 3;
-  constexpr double kFineSwitch = 1e-2;
+  constexpr double kFineSwitch = 1e-3;
   bool fine = h->precision != SAIGE_PREC_INT8X2;
   if (R > 0) SAIGE_TRY(sink->product(U, ld, R, GU, ld, nullptr, nullptr, kFine));  // constant GRM * u
 
