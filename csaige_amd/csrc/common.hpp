@@ -236,8 +236,11 @@ namespace saige {
 // other precisions ignore it.  A column's result depends only on its own slice count.
 struct ProductSink {
   uint64_t columns = 0;
+  // slices: int8 operand slices asked for every column (0: the handle's own form);
+  // col_slices, when given, per column (k entries) instead
   virtual int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo,
-                      const double* w, const double* tau, int slices = 0) = 0;
+                      const double* w, const double* tau, int slices = 0,
+                      const uint8_t* col_slices = nullptr) = 0;
   // Whether this sink's products may be captured in a HIP graph by the calling thread
   // (-1 no; otherwise a tag that distinguishes the product variants it issues).
   virtual int graph_tag() const { return -1; }
@@ -344,12 +347,13 @@ struct DirectSink : ProductSink {
   explicit DirectSink(saige_grm* hh) : h(hh) {}
   int graph_tag() const override { return loco < 0 ? 0 : 1 + loco; }
   int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
-              const double* tau, int slices = 0) override {
+              const double* tau, int slices = 0, const uint8_t* col_slices = nullptr) override {
     columns += k;
     std::vector<int> codes(loco < 0 ? 0 : k, loco);
     std::vector<uint8_t> sl(slices ? k : 0, (uint8_t)slices);
+    if (col_slices) sl.assign(col_slices, col_slices + k);
     h->loco_req = loco < 0 ? nullptr : codes.data();
-    h->slice_req = slices ? sl.data() : nullptr;
+    h->slice_req = sl.empty() ? nullptr : sl.data();
     const int rc = grm_product(h, v, ldv, k, out, ldo, w, tau);
     h->loco_req = nullptr;
     h->slice_req = nullptr;
@@ -358,10 +362,11 @@ struct DirectSink : ProductSink {
 };
 
 // Batched PCG on Sigma (device pointers; B, X: n x k with leading dimension ld), products
-// through `sink`, vectors in `ws`.
+// through `sink`, vectors in `ws`; int8 operand slices for every column (slices) or per
+// column of the block (col_slices, k entries).
 int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double* d_w,
                      const double tau[2], const double* d_b, size_t ldb, size_t k, double tol,
                      uint32_t max_iter, double* d_x, size_t ldx, saige_pcg_info* info,
-                     int slices = 0);
+                     int slices = 0, const uint8_t* col_slices = nullptr);
 
 }  // namespace saige

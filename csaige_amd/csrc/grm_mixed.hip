@@ -1038,8 +1038,9 @@ __global__ __launch_bounds__(kDotBlock) void quantize_slices_kernel(
   if (threadIdx.x == 0 && i0 < len) pdot[c * nchunks + blockIdx.x] = sh[0];
 }
 
-// part[split][row][part_ld] (columns grp*16*KT .. of the group) = sum over the split's
-// reduction range of g(row, red) * q[col][red].
+// part[split][col][row] (split stride part_ld * rows_pad; columns grp*16*KT .. of the group)
+// = sum over the split's reduction range of g(row, red) * q[col][red]: column-major per split,
+// so that the epilogue reads each column's partials as contiguous rows.
 // Work-group order: linear id L runs over round_up(nrb * nsp, 8) * ngroups work-groups;
 // j = L / 8 and x = L % 8 give the tile (j / ngroups) * 8 + x (row block tile % nrb, split
 // tile / nrb) and the column group j % ngroups, so the groups of one tile are consecutive
@@ -1152,45 +1153,13 @@ __global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
     __syncthreads();
     buf = 1 - buf;
   }
-  int* pbase = part + (split * rows_pad + rowb) * part_ld + grp * KP;
+  // lane (r, q) holds rows 16g + 4q .. + 3 of column 16t + r: one 16-byte store each
+  int* pbase = part + (split * part_ld + grp * KP + r) * rows_pad + rowb + 4 * q;
 #pragma unroll
   for (int g = 0; g < RT; ++g)
 #pragma unroll
     for (int t = 0; t < KT; ++t)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        pbase[(16 * g + 4 * q + j) * part_ld + 16 * t + r] = acc[g][t][j];
-}
-
-// Rows of one epilogue work-group: the LDS tile of ER rows x (kp + 1) ints stays under 33 KB.
-__host__ __device__ inline int epi_rows(size_t kp) { return kp <= 128 ? 64 : (kp <= 256 ? 32 : 16); }
-
-// Sum the integer partials of ER rows over the splits into an LDS tile (ER x (kp + 1); 16-byte
-// loads along the partial layout, four splits in flight per step); the caller's epilogue
-// then writes column-major outputs.
-__device__ __forceinline__ void i8_tile_sum(const int* __restrict__ part, size_t splits,
-                                            size_t rows_pad, size_t row0, int er, size_t kp,
-                                            int* __restrict__ tile) {
-  const i32x4* base = reinterpret_cast<const i32x4*>(part + row0 * kp);
-  const size_t stride = rows_pad * kp / 4;  // one split, in 16-byte groups
-  const size_t n4 = (size_t)er * kp / 4;
-  for (size_t e4 = threadIdx.x; e4 < n4; e4 += 256) {
-    i32x4 acc = {0, 0, 0, 0};
-    size_t sp = 0;
-    for (; sp + 4 <= splits; sp += 4) {
-      const i32x4 a = base[sp * stride + e4], b = base[(sp + 1) * stride + e4];
-      const i32x4 c = base[(sp + 2) * stride + e4], d = base[(sp + 3) * stride + e4];
-      acc += (a + b) + (c + d);
-    }
-    for (; sp < splits; ++sp) acc += base[sp * stride + e4];
-    const size_t e = 4 * e4;  // kp is a multiple of 16: the four values share a row
-    int* t = tile + (e / kp) * (kp + 1) + e % kp;
-    t[0] = acc.x;
-    t[1] = acc.y;
-    t[2] = acc.z;
-    t[3] = acc.w;
-  }
-  __syncthreads();
+      *reinterpret_cast<i32x4*>(pbase + (size_t)16 * t * rows_pad + 16 * g) = acc[g][t];
 }
 
 // Column scalars of the epilogues: [mean or nu, s0, s1, s2, cs or S2p] and the operand
@@ -1211,62 +1180,74 @@ __global__ void i8_col_scalars_kernel(size_t k, const SliceMap sm, const double*
   x.c3 = third_col(sm, k, c);
   col[c] = x;
 }
-__device__ __forceinline__ void i8_col_load(I8Col* col, size_t k, const I8Col* __restrict__ src) {
-  for (size_t c = threadIdx.x; c < k; c += 256) col[c] = src[c];
-}
-__device__ __forceinline__ double i8_recombine(const int* __restrict__ trow, const I8Col& cc,
-                                               size_t k, size_t c) {
-  double u = cc.v[1] * (double)trow[c] + cc.v[2] * (double)trow[k + c];
-  if (cc.c3 >= 0) u += cc.v[3] * (double)trow[cc.c3];
+// Vectors per epilogue work-group (blockIdx.y): a thread keeps its row's per-row values
+// across them; the column scalars are uniform over the work-group.
+constexpr int kEpiVec = 8;
+
+// sum_j s_jc I_jc of one row: the integer partials of the column's slices (columns c, k + c
+// and the third-slice column) added over the splits, then scaled.  p points at the row in
+// column 0 of split 0; ld = rows per column, sstride = one split.
+__device__ __forceinline__ double i8_recombine(const int* __restrict__ p, size_t sstride,
+                                               size_t splits, size_t ld, size_t k, size_t c,
+                                               const I8Col& cc) {
+  const int* a = p + c * ld;
+  const int* b = p + (k + c) * ld;
+  int i0 = 0, i1 = 0, i2 = 0;
+  if (cc.c3 >= 0) {
+    const int* d = p + (size_t)cc.c3 * ld;
+    for (size_t sp = 0; sp < splits; ++sp) {
+      i0 += a[sp * sstride];
+      i1 += b[sp * sstride];
+      i2 += d[sp * sstride];
+    }
+  } else {
+    for (size_t sp = 0; sp < splits; ++sp) {
+      i0 += a[sp * sstride];
+      i1 += b[sp * sstride];
+    }
+  }
+  double u = cc.v[1] * (double)i0 + cc.v[2] * (double)i1;
+  if (cc.c3 >= 0) u += cc.v[3] * (double)i2;
   return u;
 }
 
 // Stage-1 epilogue: U'_mc = sum_j s_jc I_jmc over the column's slices;
 // T_mc = s_m (U'_mc - 2p_m cs_c) + zs_m mean_c; W_mc = s_m T_mc (column-major, ldw).
+// Grid (ceil(m / 256), ceil(k / kEpiVec)); a thread per marker row.
 __global__ __launch_bounds__(256) void i8_stage1_epilogue_kernel(
     const int* __restrict__ part, size_t splits, size_t m_pad, size_t m, size_t k, size_t kp,
     const I8Col* __restrict__ cols, const double* __restrict__ sc,
     const double* __restrict__ p2, const double* __restrict__ zs, double* __restrict__ w,
     size_t ldw) {
-  extern __shared__ int tile[];
-  __shared__ I8Col col[kI8MaxVec];
-  const int er = epi_rows(kp);
-  i8_col_load(col, k, cols);
-  const size_t row0 = (size_t)blockIdx.x * er;
-  i8_tile_sum(part, splits, m_pad, row0, er, kp, tile);
-  const int r = threadIdx.x % er;
-  const size_t mm = row0 + r;
+  const size_t mm = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (mm >= m) return;
-  const int* trow = tile + r * (kp + 1);
   const double s = sc[mm], pp = p2[mm], z = zs[mm];
-  for (size_t c = threadIdx.x / er; c < k; c += 256 / er) {
-    const double u = i8_recombine(trow, col[c], k, c);
-    const double t = s * (u - pp * col[c].v[4]) + z * col[c].v[0];
+  const size_t c0 = (size_t)blockIdx.y * kEpiVec, c1 = min(c0 + kEpiVec, k);
+  for (size_t c = c0; c < c1; ++c) {
+    const I8Col cc = cols[c];
+    const double u = i8_recombine(part + mm, kp * m_pad, splits, m_pad, k, c, cc);
+    const double t = s * (u - pp * cc.v[4]) + z * cc.v[0];
     w[mm + c * ldw] = s * t;
   }
 }
 
 // Stage-2 epilogue: M Y_ic = sum_j s_jc I_jic + nu_c (r_i - P2) - S2p_c; optional Sigma form.
+// Grid (ceil(n / 256), ceil(k / kEpiVec)); a thread per sample row.
 __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
     const int* __restrict__ part, size_t splits, size_t n_pad, size_t n, size_t k, size_t kp,
     const I8Col* __restrict__ cols, double inv_m, const double* __restrict__ rsum, double P2,
     double* __restrict__ out, size_t ldo, const double* __restrict__ v, size_t ldv,
     const double* __restrict__ wts, double tau0, double tau1) {
-  extern __shared__ int tile[];
-  __shared__ I8Col col[kI8MaxVec];
-  const int er = epi_rows(kp);
-  i8_col_load(col, k, cols);
-  const size_t row0 = (size_t)blockIdx.x * er;
-  i8_tile_sum(part, splits, n_pad, row0, er, kp, tile);
-  const int r = threadIdx.x % er;
-  const size_t i = row0 + r;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int* trow = tile + r * (kp + 1);
   const double ri = rsum[i];
-  for (size_t c = threadIdx.x / er; c < k; c += 256 / er) {
-    double y = i8_recombine(trow, col[c], k, c);
-    y = (y + col[c].v[0] * (ri - P2) - col[c].v[4]) * inv_m;
-    if (wts) y = tau0 * v[i + c * ldv] / fmax(wts[i], 1e-30) + tau1 * y;
+  const double wi = wts ? fmax(wts[i], 1e-30) : 1.0;
+  const size_t c0 = (size_t)blockIdx.y * kEpiVec, c1 = min(c0 + kEpiVec, k);
+  for (size_t c = c0; c < c1; ++c) {
+    const I8Col cc = cols[c];
+    double y = i8_recombine(part + i, kp * n_pad, splits, n_pad, k, c, cc);
+    y = (y + cc.v[0] * (ri - P2) - cc.v[4]) * inv_m;
+    if (wts) y = tau0 * v[i + c * ldv] / wi + tau1 * y;
     out[i + c * ldo] = y;
   }
 }
@@ -1410,8 +1391,8 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
   double* pdot = pmin + std::max(nch_n, nch_m) * k;   // sums of the sliced values
   const dim3 gq_n((unsigned)ceil_div(n_pad, kDotChunk), (unsigned)k + pad_cols);
   const dim3 gq_m((unsigned)ceil_div(m_pad, kDotChunk), (unsigned)k + pad_cols);
-  const int er = epi_rows(kpt);
-  const size_t epi_lds = (size_t)er * (kpt + 1) * 4;
+  const dim3 ge1((unsigned)ceil_div(m, 256), (unsigned)ceil_div(k, kEpiVec));
+  const dim3 ge2((unsigned)ceil_div(n, 256), (unsigned)ceil_div(k, kEpiVec));
 
   // ---- V: column sums, maxima and minima, slices + sums of the sliced values ----
   col_stats_kernel<<<dim3((unsigned)nch_n, (unsigned)k), kDotBlock, 0, s>>>(d_v, ldv, n, nch_n,
@@ -1426,7 +1407,7 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
   SAIGE_TRY(i8_gemm_dispatch(kt, h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
                              n_pad, groups, ms->ipart1, &splits1, k));
   i8_col_scalars_kernel<<<cb, 64, 0, s>>>(k, sm, ms4, pdot, nch_n, colsc);
-  i8_stage1_epilogue_kernel<<<(unsigned)(m_pad / er), 256, epi_lds, s>>>(
+  i8_stage1_epilogue_kernel<<<ge1, 256, 0, s>>>(
       ms->ipart1.as<int>(), splits1, m_pad, m, k, kpt, colsc, ms->sc.as<double>(),
       ms->p2.as<double>(), ms->zs.as<double>(), ms->w.as<double>(), m);
   SAIGE_TRY(missing_stage1(h, ms->vs.as<double>(), n, k, nullptr, psum, nch_n, ms->w.as<double>(), m));
@@ -1447,7 +1428,7 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
   SAIGE_TRY(i8_gemm_dispatch(kt, h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
                              m_pad, n_pad, m_pad, groups, ms->ipart2, &splits2, k));
   i8_col_scalars_kernel<<<cb, 64, 0, s>>>(k, sm, ms4, pdot, nch_m, colsc);
-  i8_stage2_epilogue_kernel<<<(unsigned)(n_pad / er), 256, epi_lds, s>>>(
+  i8_stage2_epilogue_kernel<<<ge2, 256, 0, s>>>(
       ms->ipart2.as<int>(), splits2, n_pad, n, k, kpt, colsc, 1.0 / (double)m,
       ms->rsum.as<double>(), ms->P2, d_out, ldo, d_v, ldv, d_w, tau ? tau[0] : 0.0,
       tau ? tau[1] : 0.0);

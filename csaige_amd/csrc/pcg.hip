@@ -281,7 +281,7 @@ bool pcg_graphs_on() { return g_pcg_graphs.load(std::memory_order_relaxed); }
 int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double* d_w,
                      const double tau[2], const double* d_b, size_t ldb, size_t k, double tol,
                      uint32_t max_iter, double* d_x, size_t ldx, saige_pcg_info* info,
-                     int slices) {
+                     int slices, const uint8_t* col_slices) {
   if (k == 0) return SAIGE_OK;
   SAIGE_ARG(k <= 1024, "k <= 1024 right-hand sides per solve");
   const size_t n = h->g.n;
@@ -324,15 +324,26 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
 
   // One iteration's launches at the current width (n_active): the GRM product of the active
   // columns (gathered when some have stopped), Sigma, p'Ap, the update and the next beta.
+  // per-column slice requests of the active columns, in the gathered order (the host copy
+  // of the active list is current whenever an iteration is enqueued)
+  std::vector<uint8_t> act_sl;
+  auto active_slices = [&](size_t width) -> const uint8_t* {
+    if (!col_slices) return nullptr;
+    act_sl.resize(width);
+    for (size_t j = 0; j < width; ++j)
+      act_sl[j] = col_slices[width == k ? j : (size_t)list[1 + j]];
+    return act_sl.data();
+  };
   auto enqueue = [&](size_t width) -> int {
+    const uint8_t* sl = active_slices(width);
     if (width == k) {
-      SAIGE_TRY(sink->product(P, ld, k, AP, ld, nullptr, nullptr, slices));
+      SAIGE_TRY(sink->product(P, ld, k, AP, ld, nullptr, nullptr, slices, sl));
     } else {
       // only the columns still iterating: their results do not depend on which other
       // columns share the product
       const dim3 gg((unsigned)ceil_div(n, 256), (unsigned)width);
       pcg_gather_kernel<<<gg, 256, 0, s>>>(P, ld, n, d_nact + 1, 0, PC);
-      SAIGE_TRY(sink->product(PC, ld, width, APC, ld, nullptr, nullptr, slices));
+      SAIGE_TRY(sink->product(PC, ld, width, APC, ld, nullptr, nullptr, slices, sl));
       pcg_gather_kernel<<<gg, 256, 0, s>>>(APC, ld, n, d_nact + 1, 1, AP);
     }
     const dim3 gn((unsigned)ceil_div(n, 256), (unsigned)k);
@@ -360,7 +371,13 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
     uint64_t key[12] = {};
     if (graphs) {
       const uint64_t gen = devbuf_generation();
-      const uint64_t kk[12] = {(uint64_t)(uintptr_t)h, (uint64_t)tag, (uint64_t)slices, k, width,
+      // the slice requests are part of the captured product: FNV-1a of the active columns'
+      uint64_t slkey = (uint64_t)slices;
+      if (const uint8_t* sl = active_slices(width)) {
+        slkey = 1469598103934665603ull;
+        for (size_t j = 0; j < width; ++j) slkey = (slkey ^ sl[j]) * 1099511628211ull;
+      }
+      const uint64_t kk[12] = {(uint64_t)(uintptr_t)h, (uint64_t)tag, slkey, k, width,
                                (uint64_t)(uintptr_t)d_x, ldx, (uint64_t)(uintptr_t)d_w,
                                (uint64_t)(uintptr_t)R, (uint64_t)(uintptr_t)part,
                                (uint64_t)(uintptr_t)st, n};

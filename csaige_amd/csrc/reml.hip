@@ -307,6 +307,7 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
   constexpr int kFine = 3;
   constexpr double kFineSwitch = 1e-3;
   bool fine = h->precision != SAIGE_PREC_INT8X2;
+  std::vector<uint8_t> blk_sl(k1, 0);  // per-column slice requests of the main block
   if (R > 0) SAIGE_TRY(sink->product(U, ld, R, GU, ld, nullptr, nullptr, kFine));  // constant GRM * u
 
   // eta, mu from the GLM fit
@@ -329,8 +330,10 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
     reml_prep_kernel<<<grid1(n), 256, 0, s>>>(dY, dEta, dMu, trait, n, dW, RHS1);
     SAIGE_HIP(hipGetLastError());
     const bool fine_iter = fine;
+    // [y~ | X] on three slices once fine; the probe columns stay on the handle's form
+    std::fill(blk_sl.begin(), blk_sl.begin() + 1 + p, (uint8_t)(fine_iter ? kFine : 0));
     SAIGE_TRY(pcg_sigma_device(h, sink, st.pcg, dW, tau, RHS1, ld, k1, cfg->pcg_tol,
-                               cfg->pcg_max_iter, SOL1, ld, info.data(), fine_iter ? kFine : 0));
+                               cfg->pcg_max_iter, SOL1, ld, info.data(), 0, blk_sl.data()));
     for (size_t c = 0; c < k1; ++c) {
       pcg_iters += info[c].iterations;
       pcg_fail += info[c].converged ? 0 : 1;
@@ -1092,6 +1095,7 @@ class ProductBatcher {
     double tau[2];
     int loco = -1;  // chromosome left out of this request's GRM (-1: none)
     int slices = 0; // int8 operand slices asked for (0: the handle's own form)
+    const uint8_t* col_slices = nullptr;  // or per column (k entries)
     int rc = SAIGE_OK;
     bool done = false;
   };
@@ -1132,9 +1136,14 @@ class ProductBatcher {
     bool loco = false, slices = false;
     for (const Request* q : reqs) {
       codes.insert(codes.end(), q->k, q->loco);
-      sl.insert(sl.end(), q->k, (uint8_t)q->slices);
+      if (q->col_slices) {
+        sl.insert(sl.end(), q->col_slices, q->col_slices + q->k);
+        slices = true;
+      } else {
+        sl.insert(sl.end(), q->k, (uint8_t)q->slices);
+        slices = slices || q->slices != 0;
+      }
       loco = loco || q->loco >= 0;
-      slices = slices || q->slices != 0;
     }
     h_->loco_req = loco ? codes.data() : nullptr;
     h_->slice_req = slices ? sl.data() : nullptr;
@@ -1188,11 +1197,12 @@ struct BatchedSink : ProductSink {
   int loco = -1;
   explicit BatchedSink(ProductBatcher* bb) : b(bb) {}
   int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
-              const double* tau, int slices = 0) override {
+              const double* tau, int slices = 0, const uint8_t* col_slices = nullptr) override {
     if (k == 0) return SAIGE_OK;
     columns += k;
     ProductBatcher::Request r;
     r.slices = slices;
+    r.col_slices = col_slices;
     r.v = v;
     r.ldv = ldv;
     r.k = k;
