@@ -299,11 +299,11 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
   const double* U = RHS1 + (1 + p) * ld;
   // operand slices of the int8x2 form (the other forms ignore the request): three for every
   // product whose result enters the tau update directly or is computed once (GRM u, GRM P y,
-  // the second-stage and variance-ratio solves); two for the main block solves while the
-  // relative tau change is above kFineSwitch, three from then on.  Two slices move tau by
-  // about 1e-5 relative; switching at 1e-3 keeps the two-slice iterations well above that
-  // floor (at 1e-4 the fits took up to two extra iterations, at 1e-2 a third more of the
-  // block products ran on three slices)
+  // the second-stage and variance-ratio solves); in the main block, the y~ and X columns take
+  // two while the relative tau change is above kFineSwitch and three from then on, the probe
+  // columns (whose solves only enter the Monte-Carlo trace estimate) two throughout.  Two
+  // slices move tau by about 1e-5 relative; switching at 1e-3 keeps the two-slice iterations
+  // well above that floor (at 1e-4 the fits took up to two extra iterations)
   constexpr int kFine = 3;
   constexpr double kFineSwitch = 1e-3;
   bool fine = h->precision != SAIGE_PREC_INT8X2;
