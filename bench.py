@@ -1062,10 +1062,13 @@ def run_headline(args):
             f"({gbs / HBM_PEAK_GBS:.3f} of HBM)")
 
     # ---- FP64 roofline floor: the FP64 fit's column-products at 78.6 TFLOP/s ----
+    # grm_products counts each trait's columns; the joined fit computes the probes' GRM u
+    # (30 columns, the same probes for every trait) once, so the other traits' copies are
+    # not products
     fp64_floor = None
     if rf is not None:
         mg = int(rf[0]["n_grm_markers"])
-        cols = sum(int(r["grm_products"]) for r in rf)
+        cols = sum(int(r["grm_products"]) for r in rf) - (len(rf) - 1) * 30
         fp64_floor = 4.0 * n * mg * cols / (FP64_PEAK_TFLOPS * 1e12)
 
     # ---- CPU baseline of the fit: sampled f32 OnTheFlyGrm products, extrapolated ----

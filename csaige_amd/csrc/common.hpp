@@ -244,6 +244,10 @@ struct ProductSink {
   // Whether this sink's products may be captured in a HIP graph by the calling thread
   // (-1 no; otherwise a tag that distinguishes the product variants it issues).
   virtual int graph_tag() const { return -1; }
+  // The next product's input equals that of any other product issued with the same nonzero
+  // key (same columns, slices and left-out chromosome), so a sink that joins products may
+  // compute it once.
+  virtual void share_next(uint64_t key) { (void)key; }
   virtual ~ProductSink() = default;
 };
 

@@ -308,7 +308,12 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
   constexpr double kFineSwitch = 1e-3;
   bool fine = h->precision != SAIGE_PREC_INT8X2;
   std::vector<uint8_t> blk_sl(k1, 0);  // per-column slice requests of the main block
-  if (R > 0) SAIGE_TRY(sink->product(U, ld, R, GU, ld, nullptr, nullptr, kFine));  // constant GRM * u
+  if (R > 0) {
+    // constant GRM * u; the probes depend only on (seed, n, R), so fits of several traits
+    // over the same samples share it
+    sink->share_next(((uint64_t)cfg->seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)n << 20) ^ R);
+    SAIGE_TRY(sink->product(U, ld, R, GU, ld, nullptr, nullptr, kFine));
+  }
 
   // eta, mu from the GLM fit
   SAIGE_HIP(hipMemcpyAsync(COEF, alpha.data(), p * 8, hipMemcpyHostToDevice, s));
@@ -1096,6 +1101,7 @@ class ProductBatcher {
     int loco = -1;  // chromosome left out of this request's GRM (-1: none)
     int slices = 0; // int8 operand slices asked for (0: the handle's own form)
     const uint8_t* col_slices = nullptr;  // or per column (k entries)
+    uint64_t share = 0;  // nonzero: same input as every other request with this key
     int rc = SAIGE_OK;
     bool done = false;
   };
@@ -1117,6 +1123,7 @@ class ProductBatcher {
   }
   uint64_t joined_products = 0;  // multi-column products run
   uint64_t joined_columns = 0;
+  uint64_t shared_columns = 0;  // columns copied from an identical request instead
 
  private:
   void run_locked() {
@@ -1128,8 +1135,23 @@ class ProductBatcher {
     pending_.clear();
     cv_.notify_all();
   }
-  int run(const std::vector<Request*>& reqs) {
+  int run(const std::vector<Request*>& all) {
     ++joined_products;
+    // requests with the same share key, slices and left-out chromosome are computed once
+    std::vector<Request*> reqs;
+    std::vector<std::pair<Request*, Request*>> dup;  // (duplicate, the request computing it)
+    for (Request* q : all) {
+      Request* first = nullptr;
+      if (q->share)
+        for (Request* u : reqs)
+          if (u->share == q->share && u->k == q->k && u->loco == q->loco &&
+              u->slices == q->slices && !u->col_slices && !q->col_slices && !u->w && !q->w) {
+            first = u;
+            break;
+          }
+      if (first) dup.emplace_back(q, first);
+      else reqs.push_back(q);
+    }
     // per-column left-out chromosomes and operand slices of the joined product
     std::vector<int> codes;
     std::vector<uint8_t> sl;
@@ -1150,7 +1172,15 @@ class ProductBatcher {
     const int rc = run_joined(reqs);
     h_->loco_req = nullptr;
     h_->slice_req = nullptr;
-    return rc;
+    if (rc != SAIGE_OK) return rc;
+    for (const auto& d : dup) {  // plain products (no Sigma form): copy the shared result
+      const size_t n = h_->g.n;
+      SAIGE_HIP(hipMemcpy2DAsync(d.first->out, d.first->ldo * 8, d.second->out,
+                                 d.second->ldo * 8, n * 8, d.first->k, hipMemcpyDeviceToDevice,
+                                 h_->stream));
+      shared_columns += d.first->k;
+    }
+    return SAIGE_OK;
   }
   int run_joined(const std::vector<Request*>& reqs) {
     if (reqs.size() == 1) {
@@ -1195,12 +1225,17 @@ class ProductBatcher {
 struct BatchedSink : ProductSink {
   ProductBatcher* b;
   int loco = -1;
+  uint64_t share = 0;  // key of the next product (share_next), then cleared
   explicit BatchedSink(ProductBatcher* bb) : b(bb) {}
+  void share_next(uint64_t key) override { share = key; }
   int product(const double* v, size_t ldv, size_t k, double* out, size_t ldo, const double* w,
               const double* tau, int slices = 0, const uint8_t* col_slices = nullptr) override {
+    const uint64_t key = share;
+    share = 0;
     if (k == 0) return SAIGE_OK;
     columns += k;
     ProductBatcher::Request r;
+    r.share = key;
     r.slices = slices;
     r.col_slices = col_slices;
     r.v = v;
