@@ -85,8 +85,8 @@ int saige_plink_decode(int device, const uint8_t* bed_body, size_t bytes_per_mar
  *   (|v - split| <= 6.1e-8 max|v - mean|), exact int32 accumulation; n, M <= 8,000,000.
  *   precision SAIGE_PREC_INT8X2: as SAIGE_PREC_INT8 with two int8 slices
  *   (|v - split| <= 1.55e-5 max|v - mean|): 2/3 of the MFMA work.  The fits take three
- *   slices for the narrow solves and, once the relative tau change is below 1e-2, for the
- *   main block solves too.
+ *   slices for the narrow solves and, once the relative tau change is below 1e-3, for the
+ *   y~ and X columns of the main block solves (the probe columns keep two).
  *   The factored modes correct markers with missing calls by sparse per-marker and
  *   per-sample lists (mean imputation as pcg.rs:108-113).
  *   precision SAIGE_PREC_F32: saige-rs's own arithmetic (pcg.rs:128-180 at one rayon
