@@ -236,6 +236,34 @@ struct RemlState {
 };
 
 // fit_ai_reml with X (n x p, host col-major) and y (host); outputs stay on device in st.
+// Per-chunk sums of the final log-likelihood terms (ai_reml.rs:393-414): binary
+// y log(mu) + (1 - y) log(1 - mu) with mu clamped to [1e-15, 1 - 1e-15], quantitative
+// (y - mu)^2; kDotChunk samples per block, a fixed reduction tree.
+__global__ __launch_bounds__(kDotBlock) void loglik_terms_kernel(const double* __restrict__ y,
+                                                                 const double* __restrict__ mu,
+                                                                 size_t n, int trait,
+                                                                 double* __restrict__ part) {
+  const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, n);
+  double acc = 0.0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) {
+    if (trait != SAIGE_TRAIT_QUANTITATIVE) {
+      const double mi = fmin(fmax(mu[i], 1e-15), 1.0 - 1e-15);
+      acc += y[i] * log(mi) + (1.0 - y[i]) * log(1.0 - mi);
+    } else {
+      const double d = y[i] - mu[i];
+      acc += d * d;
+    }
+  }
+  __shared__ double sh[kDotBlock];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kDotBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, const double* x,
                            size_t n, size_t p, int trait, const saige_ai_reml_config* cfg,
                            saige_ai_reml_result* res, RemlState& st) {
@@ -480,20 +508,24 @@ static int fit_ai_reml_dev(saige_grm* h, ProductSink* sink, const double* y, con
       break;
     }
   }
-  // final W and log-likelihood (ai_reml.rs:393-414)
-  std::vector<double> mu(n);
-  SAIGE_HIP(hipMemcpyAsync(mu.data(), dMu, n * 8, hipMemcpyDeviceToHost, s));
-  SAIGE_HIP(hipStreamSynchronize(s));
+  // final W and log-likelihood (ai_reml.rs:393-414): the per-sample terms summed on the
+  // device in fixed-order chunks (the host adds the chunk sums in order), so a trait thread
+  // of a joined fit does not hold the other traits' products back with an O(n) host loop
   double ll = 0.0;
-  if (trait != SAIGE_TRAIT_QUANTITATIVE) {
-    for (size_t i = 0; i < n; ++i) {
-      double mi = std::min(std::max(mu[i], 1e-15), 1.0 - 1e-15);
-      ll += y[i] * std::log(mi) + (1.0 - y[i]) * std::log(1.0 - mi);
-    }
-  } else {
-    double ss = 0.0;
-    for (size_t i = 0; i < n; ++i) ss += (y[i] - mu[i]) * (y[i] - mu[i]);
-    ll = -0.5 * (double)n * std::log(2.0 * M_PI * tau[0]) - 0.5 * ss / tau[0];
+  {
+    const size_t nch = std::max<size_t>(1, ceil_div(n, kDotChunk));
+    SAIGE_TRY(st.part.ensure(nch * 8));
+    loglik_terms_kernel<<<(unsigned)nch, kDotBlock, 0, s>>>(dY, dMu, n, trait,
+                                                             st.part.as<double>());
+    SAIGE_HIP(hipGetLastError());
+    std::vector<double> cs(nch);
+    SAIGE_TRY(read_doubles(s, st.part.as<double>(), nch, cs.data()));
+    double sum = 0.0;
+    for (double c : cs) sum += c;
+    if (trait != SAIGE_TRAIT_QUANTITATIVE)
+      ll = sum;
+    else
+      ll = -0.5 * (double)n * std::log(2.0 * M_PI * tau[0]) - 0.5 * sum / tau[0];
   }
   // final working weights on device (family.rs:55-66)
   reml_prep_kernel<<<grid1(n), 256, 0, s>>>(dY, dEta, dMu, trait, n, dW, RHS1);
