@@ -993,21 +993,16 @@ static int qc_split(int device, const uint8_t* bed_body, bool on_device, size_t 
 // packed rows in model order (shared, read-only).
 static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const double* x, size_t p,
                      int trait_type, const saige_fit_null_config* cfg, const MarkerSplit& ms,
-                     const uint8_t* vr_rows, saige_fit_null_result* result, double* mu_out,
-                     double* xvx_inv_xv, double* alpha_out, double* per_marker_vr) {
+                     const uint8_t* vr_rows, saige_fit_null_result* result,
+                     double* per_marker_vr, RemlState& st) {
   using clk = std::chrono::steady_clock;
   const size_t n = h->g.n;
   hipStream_t s = h->stream;
   const size_t ld = round_up(n, 64);
-  RemlState st;
   auto t1 = clk::now();
   SAIGE_TRY(fit_ai_reml_dev(h, sink, y, x, n, p, trait_type, &cfg->reml, &result->reml, st));
   auto t2 = clk::now();
   SAIGE_TRY(xvx_inv_xv_dev(s, st.x.as<double>(), ld, st.w.as<double>(), n, p, st));
-  if (mu_out) SAIGE_HIP(hipMemcpyAsync(mu_out, st.mu.p, n * 8, hipMemcpyDeviceToHost, s));
-  if (xvx_inv_xv)
-    SAIGE_HIP(hipMemcpyAsync(xvx_inv_xv, st.a_flat.p, n * p * 8, hipMemcpyDeviceToHost, s));
-  if (alpha_out) SAIGE_HIP(hipMemcpyAsync(alpha_out, st.coef.p, p * 8, hipMemcpyDeviceToHost, s));
   double vr = 1.0;
   uint32_t used = 0;
   if (!ms.vr_ids.empty()) {
@@ -1034,7 +1029,21 @@ static int fit_trait(saige_grm* h, ProductSink* sink, const double* y, const dou
   result->seconds_reml = std::chrono::duration<double>(t2 - t1).count();
   result->seconds_vr = std::chrono::duration<double>(t3 - t2).count();
   result->reml.grm_products = sink->columns;
-  SAIGE_HIP(hipStreamSynchronize(s));  // mu_out and xvx_inv_xv copies
+  return SAIGE_OK;
+}
+
+// mu, XVX^-1X'V (p x n, a_flat) and alpha of a fitted trait to the caller's host arrays.  A
+// joined fit calls this after the trait has left the batcher: the copies into pageable memory
+// hold the calling thread, which the other traits would otherwise wait for.
+static int copy_trait_outputs(saige_grm* h, const RemlState& st, size_t p, double* mu_out,
+                              double* xvx_inv_xv, double* alpha_out) {
+  const size_t n = h->g.n;
+  hipStream_t s = h->stream;
+  if (mu_out) SAIGE_HIP(hipMemcpyAsync(mu_out, st.mu.p, n * 8, hipMemcpyDeviceToHost, s));
+  if (xvx_inv_xv)
+    SAIGE_HIP(hipMemcpyAsync(xvx_inv_xv, st.a_flat.p, n * p * 8, hipMemcpyDeviceToHost, s));
+  if (alpha_out) SAIGE_HIP(hipMemcpyAsync(alpha_out, st.coef.p, p * 8, hipMemcpyDeviceToHost, s));
+  SAIGE_HIP(hipStreamSynchronize(s));
   return SAIGE_OK;
 }
 
@@ -1083,9 +1092,11 @@ static int fit_null_impl(int device, const uint8_t* bed_body, bool on_device, si
     DevBuf rows;
     rc = vr_rows_upload(h, bed_body, on_device, bpm, n_total, sample_idx, n, ms, rows);
     DirectSink sink(h);
+    RemlState st;
     if (rc == SAIGE_OK)
-      rc = fit_trait(h, &sink, y, x, p, trait_type, cfg, ms, rows.as<uint8_t>(), result, mu_out,
-                     xvx_inv_xv, alpha_out, per_marker_vr);
+      rc = fit_trait(h, &sink, y, x, p, trait_type, cfg, ms, rows.as<uint8_t>(), result,
+                     per_marker_vr, st);
+    if (rc == SAIGE_OK) rc = copy_trait_outputs(h, st, p, mu_out, xvx_inv_xv, alpha_out);
   }
   saige_grm_destroy(h);
   if (rc == SAIGE_OK) {
@@ -1318,16 +1329,18 @@ static int fit_null_multi_impl(int device, const uint8_t* bed_body, bool on_devi
       for (size_t t = 0; t < n_traits; ++t) {
         threads.emplace_back([&, t]() {
           BatchedSink sink(&batcher);
+          RemlState st;
           int r = hipSetDevice(h->device) == hipSuccess ? SAIGE_OK : SAIGE_ERR_DEVICE;
           if (r == SAIGE_OK)
             r = fit_trait(h, &sink, y + t * n, x, p, trait_types[t], cfg, ms, rows.as<uint8_t>(),
-                          &results[t], mu_out ? mu_out + t * n : nullptr,
-                          xvx_inv_xv ? xvx_inv_xv + t * p * n : nullptr,
-                          alpha_out ? alpha_out + t * p : nullptr,
-                          per_marker_vr ? per_marker_vr + t * m_total : nullptr);
+                          &results[t], per_marker_vr ? per_marker_vr + t * m_total : nullptr, st);
+          batcher.leave();
+          if (r == SAIGE_OK)
+            r = copy_trait_outputs(h, st, p, mu_out ? mu_out + t * n : nullptr,
+                                   xvx_inv_xv ? xvx_inv_xv + t * p * n : nullptr,
+                                   alpha_out ? alpha_out + t * p : nullptr);
           if (r != SAIGE_OK) errs[t] = saige_last_error();
           rcs[t] = r;
-          batcher.leave();
         });
       }
       for (auto& th : threads) th.join();
