@@ -46,7 +46,10 @@ def build(verbose: bool = False) -> None:
     for src in SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        # the int8 GEMM's LDS DMA asm lists M0 as clobbered on purpose (it sets M0 itself);
+        # clang warns about any reserved register in a clobber list
+        extra = ["-Wno-inline-asm"] if src == "grm_mixed.hip" else []
+        cmds.append([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"

@@ -166,26 +166,45 @@ __global__ __launch_bounds__(256) void transpose_codes_kernel(
   if (t == 0 && red[0]) atomicAdd(n_miss, red[0]);
 }
 
-// The int8 GEMM reads the code matrices in tiles of 16 rows x 128 bytes (512 reduction
-// indices), each tile 2 KB contiguous: tile (row / 16, byte / 128) starts at
-// ((row / 16) * (row_bytes / 128) + byte / 128) * 2048, row r % 16 at 128 (r % 16) in it.  A
-// wave's load of 16 rows then touches one 2 KB block instead of 16 rows 12.5 KB apart.
-__global__ void retile_codes_kernel(const uint8_t* __restrict__ src, size_t row_bytes,
-                                    size_t rows, uint8_t* __restrict__ dst) {
-  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // 16-byte unit
-  const size_t per_row = row_bytes / 16;
-  if (v >= rows * per_row) return;
-  const size_t row = v / per_row, b = 16 * (v % per_row);
-  const uint4 x = *reinterpret_cast<const uint4*>(src + row * row_bytes + b);
-  *reinterpret_cast<uint4*>(dst + ((row / 16) * (row_bytes / 128) + b / 128) * 2048 +
-                            (row % 16) * 128 + b % 128) = x;
+// The int8 GEMM streams the genotypes in blocks of 512 rows x kI8Chunk reduction indices,
+// each block 16 KB contiguous, so that a work-group's whole reduction range is one run of
+// HBM and every block lands in LDS unchanged by 16-byte DMA loads.  Block (row / 512,
+// red / kI8Chunk) starts at ((row / 512) * (red_len / kI8Chunk) + red / kI8Chunk) * 16384;
+// inside it the 32-bit words (16 reduction indices of one row) are ordered
+// [step s = (red % 128) / 64][row group rg = (row % 512) / 64][lane 16 q + r][tile g]
+// with r = row % 16, g = (row % 64) / 16, q = (red % 64) / 16: one 16-byte LDS read gives a
+// lane its words of four 16-row tiles for one 64-long MFMA step.  The words hold genotype
+// values, not PLINK codes: each 2-bit field is the allele count 0, 1 or 2 (codes 00 -> 2,
+// 10 -> 1, 11 -> 0, and the missing code 01 -> 0), so that (w >> 2u) & 0x03030303 is an
+// MFMA operand word.
+constexpr size_t kI8Chunk = 128;
+constexpr size_t kI8Block = 512 * kI8Chunk / 4;  // bytes per (row block, chunk)
+
+__device__ __forceinline__ uint32_t codes_to_values(uint32_t w) {
+  const uint32_t lo = w & 0x55555555u, hi = (w >> 1) & 0x55555555u;
+  const uint32_t nlo = ~lo & 0x55555555u;  // set for codes 00 and 10
+  return ((nlo & ~hi & 0x55555555u) << 1) | (nlo & hi);
 }
 
-// buf (rows x row_bytes, row-major) replaced by its tiled form
-static int retile_codes(hipStream_t s, DevBuf& buf, size_t rows, size_t row_bytes) {
+__global__ void block_codes_kernel(const uint8_t* __restrict__ src, size_t row_bytes,
+                                   size_t rows, uint8_t* __restrict__ dst) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // 32-bit word
+  const size_t per_row = row_bytes / 4;
+  if (v >= rows * per_row) return;
+  const size_t row = v / per_row, kd = v % per_row, red = 16 * kd;
+  const size_t rb = row / 512, rg = (row % 512) / 64, g = (row % 64) / 16, r = row % 16;
+  const size_t kc = red / kI8Chunk, s = (red % kI8Chunk) / 64, q = (red % 64) / 16;
+  const size_t nkc = row_bytes * 4 / kI8Chunk;
+  const size_t off = (rb * nkc + kc) * kI8Block + ((s * 8 + rg) * 64 + 16 * q + r) * 16 + g * 4;
+  *reinterpret_cast<uint32_t*>(dst + off) =
+      codes_to_values(*reinterpret_cast<const uint32_t*>(src + row * row_bytes + 4 * kd));
+}
+
+// buf (rows x row_bytes of PLINK codes, row-major) replaced by its blocked value form
+static int block_codes(hipStream_t s, DevBuf& buf, size_t rows, size_t row_bytes) {
   DevBuf tmp;
   SAIGE_TRY(tmp.ensure(rows * row_bytes));
-  retile_codes_kernel<<<(unsigned)ceil_div(rows * (row_bytes / 16), 256), 256, 0, s>>>(
+  block_codes_kernel<<<(unsigned)ceil_div(rows * (row_bytes / 4), 256), 256, 0, s>>>(
       buf.as<uint8_t>(), row_bytes, rows, tmp.as<uint8_t>());
   SAIGE_HIP(hipGetLastError());
   SAIGE_HIP(hipStreamSynchronize(s));  // the row-major buffer goes back to the cache
@@ -698,7 +717,7 @@ int grm_mixed_prepare(saige_grm* h) {
   SAIGE_HIP(hipStreamSynchronize(s));
   if (nmiss) {
     // per-marker lists from the SNP-major rows (real samples), per-sample lists from the
-    // sample-major copy (real markers); both before the int8 forms retile the code matrices
+    // sample-major copy (real markers); both before the int8 forms re-block the code matrices
     SAIGE_TRY(build_miss_list(s, g.geno, g.row_bytes, g.m, g.n, ms->miss_mk));
     SAIGE_TRY(build_miss_list(s, ms->geno_t.as<uint8_t>(), ms->rbt, g.n, g.m, ms->miss_sm));
   }
@@ -729,9 +748,9 @@ int grm_mixed_prepare(saige_grm* h) {
   }
   if (h->precision == SAIGE_PREC_INT8 || h->precision == SAIGE_PREC_INT8X2) {
     // nothing reads the row-major forms from here on
-    SAIGE_TRY(retile_codes(s, h->geno, g.m_pad, g.row_bytes));
+    SAIGE_TRY(block_codes(s, h->geno, g.m_pad, g.row_bytes));
     g.geno = h->geno.as<uint8_t>();
-    SAIGE_TRY(retile_codes(s, ms->geno_t, g.n_pad, ms->rbt));
+    SAIGE_TRY(block_codes(s, ms->geno_t, g.n_pad, ms->rbt));
   }
   return SAIGE_OK;
 }
@@ -880,11 +899,11 @@ void grm_mixed_release(saige_grm* h) {
 // three-slice vectors); columns from 2k + k3 up to the padded width KPT are zero.  Each
 // column is pad_len long in reduction order.  Within every group of 16 reduction indices the
 // bytes are stored 4x4-transposed (index 4b+u at byte 4u+b): a lane expands one 32-bit word
-// of packed codes (code 4b+u at bits 8b+2u) with v_perm into four words, word u holding
-// codes {u, 4+u, 8+u, 12+u} one per byte, and the operand's bytes then pair with the same
-// reduction indices.
+// of packed genotype values (value 4b+u at bits 8b+2u) by a shift and a mask into four
+// words, word u holding values {u, 4+u, 8+u, 12+u} one per byte, and the operand's bytes
+// then pair with the same reduction indices.
 //
-// Column groups: KPT = G * 16 * KT with KT <= 7 tiles per group.  One launch covers all G
+// Column groups: KPT = G * 16 * KT with KT <= 10 tiles per group.  One launch covers all G
 // groups; work-groups that share a row block and a reduction split but differ in the group
 // are issued back to back on the same XCD (XCD = linear work-group id mod 8), so the
 // genotype bytes that one of them streams from HBM are read by the others from that XCD's
@@ -1038,125 +1057,152 @@ __global__ __launch_bounds__(kDotBlock) void quantize_slices_kernel(
   if (threadIdx.x == 0 && i0 < len) pdot[c * nchunks + blockIdx.x] = sh[0];
 }
 
+// 16 bytes from global memory into LDS without passing through registers: lane i's 16 bytes
+// land at LDS byte address lds + 16 i (lds is wave-uniform and goes through M0).  The load
+// counts in vmcnt like any vector load; its data are in LDS once vmcnt has dropped past it.
+// Written in asm so that the compiler neither adds waits of its own between these loads and
+// the LDS reads (the ring below orders them with explicit vmcnt waits and barriers) nor
+// expects M0 to survive.
+__device__ __forceinline__ void lds_dma16(uint32_t lds, const void* g) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(g)
+               : "memory", "m0");
+}
+__device__ __forceinline__ uint32_t lds_address(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// LDS stages of the int8 GEMM ring: as many 16 KB + KT * 2 KB stages as fit in 160 KB, at
+// most five; four for KT <= 2 (two work-groups per CU for the genotype-bound narrow passes).
+template <int KT>
+struct I8Ring {
+  static constexpr int kStage = (int)kI8Block + 16 * KT * (int)kI8Chunk;
+  static constexpr int kFit = 163840 / kStage;
+  static constexpr int NS = KT <= 2 ? 4 : (kFit < 5 ? kFit : 5);
+};
+
 // part[split][col][row] (split stride part_ld * rows_pad; columns grp*16*KT .. of the group)
 // = sum over the split's reduction range of g(row, red) * q[col][red]: column-major per split,
 // so that the epilogue reads each column's partials as contiguous rows.
 // Work-group order: linear id L runs over round_up(nrb * nsp, 8) * ngroups work-groups;
 // j = L / 8 and x = L % 8 give the tile (j / ngroups) * 8 + x (row block tile % nrb, split
 // tile / nrb) and the column group j % ngroups, so the groups of one tile are consecutive
-// work-groups of one XCD.  Tiles past nrb * nsp exit at once (a whole work-group).
-// NW waves; each wave owns 16*RT rows and reads their packed codes straight from global
-// memory in the tiled layout of retile_codes_kernel (code_stride = bytes per row; KCH/16
-// bytes = KCH/4 genotypes per lane and chunk of KCH reduction indices, the next chunk
-// prefetched in registers; lane q takes the q-th quarter of a row's chunk, so one load of
-// a 16-row group stays inside one 2 KB tile).  The group's 16*KT x KCH operand chunk is
-// staged in LDS, double-buffered so that one barrier per chunk separates the reads of one
-// buffer from the refill of the other (column stride 272 bytes: the 16-byte fragment reads
-// of 8 consecutive columns fall in distinct banks).  Eight waves of RT = 4 keep the 112
-// accumulators of KT = 7 in VGPRs at two waves per SIMD.
-template <int RT, int KT, int NW, int KCH>
-__global__ __launch_bounds__(NW * 64) void i8_gemm_kernel(
-    const uint8_t* __restrict__ codes, size_t code_stride, const int8_t* __restrict__ bq,
-    size_t b_stride, size_t red_per_split, size_t red_len, int* __restrict__ part,
-    size_t rows_pad, size_t part_ld, size_t nrb, size_t nsp, size_t ngroups) {
-  constexpr int KP = 16 * KT;
-  constexpr int ROWS = NW * 16 * RT;
-  constexpr int NT = NW * 64;
-  constexpr int SB = KCH + 16;
-  constexpr int SEGS = KCH / 16;
-  constexpr int AV = KCH / 256;  // 16-byte code loads per row tile and chunk
-  constexpr int B_LOADS = (KP * SEGS + NT - 1) / NT;
-  constexpr int BCSTEP = NT / SEGS;
-  constexpr int BUF = KP * SB;
-  __shared__ __attribute__((aligned(16))) int8_t sB[2 * BUF];
+// work-groups of one XCD and share its L2 for the genotype blocks.  Tiles past nrb * nsp
+// exit at once (a whole work-group).
+// Eight waves; wave w owns rows 64 w .. 64 w + 63 of the 512-row block and all 16 KT
+// columns of the group (KT <= 10: 160 int32 accumulators per lane at two waves per SIMD).
+// Per chunk of 128 reduction indices a stage of the LDS ring holds the 16 KB genotype block
+// (layout of block_codes_kernel) and the group's operand chunk, 16 KT columns x 128 bytes,
+// its 16-byte units XOR-swizzled by column (unit u of column c at c * 128 + 16 (u ^ (c & 7)))
+// so that the 16-byte fragment reads of 8 consecutive columns fall in distinct banks.  Both
+// arrive by 16-byte DMA (D of them per wave and chunk; waves past the operand's last unit
+// repeat one, which rewrites identical bytes).  The ring runs NS - 2 chunks ahead: chunk
+// c + NS - 2 is loaded, spread over chunk c's MFMA stream, into the stage that chunk c - 2
+// used, which every wave has left by the barrier that opens chunk c - 1.  The barrier that
+// opens chunk c follows each wave's wait for its own loads of chunk c, so after it every
+// wave's loads of that chunk are in LDS.  Loads past the range repeat the last chunk into
+// stages no wave reads again; all loads are drained before the accumulators are stored.
+template <int KT, int NS>
+__global__ __launch_bounds__(512) void i8_gemm_kernel(
+    const uint8_t* __restrict__ codes, size_t nkc, const int8_t* __restrict__ bq,
+    size_t b_stride, size_t cps, int* __restrict__ part, size_t rows_pad, size_t part_ld,
+    size_t nrb, size_t nsp, size_t ngroups) {
+  constexpr int KCH = (int)kI8Chunk, NW = 8, ROWS = 512, KP = 16 * KT, UPC = KCH / 16;
+  constexpr int KS = KCH / 64;                    // MFMA steps per chunk
+  constexpr int ABYTES = (int)kI8Block, A_INS = ABYTES / 1024 / NW;
+  constexpr int BTOT = KP * KCH / 1024, B_INS = (BTOT + NW - 1) / NW;
+  constexpr int STAGE = ABYTES + KP * KCH;
+  constexpr int D = A_INS + B_INS;                // DMA loads per wave and chunk
+  constexpr int GB = 4 * KS;                      // row-tile blocks of KT MFMAs per chunk
+  constexpr int EVERY = GB / D > 0 ? GB / D : 1;  // one DMA load every EVERY blocks
+  static_assert(NS >= 3 && NS * STAGE <= 163840 && D * (NS - 3) <= 63, "LDS ring");
+  static_assert(ABYTES % (1024 * NW) == 0 && (KP * KCH) % 1024 == 0, "whole DMA loads");
+  __shared__ __attribute__((aligned(1024))) int8_t smem[NS * STAGE];
   const size_t L = blockIdx.x, jx = L >> 3;
   const size_t tile = (jx / ngroups) * 8 + (L & 7), grp = jx % ngroups;
   if (tile >= nrb * nsp) return;  // uniform over the work-group
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 15, q = lane >> 4;
-  const size_t rowb = (tile % nrb) * ROWS + (size_t)wave * 16 * RT;
-  const size_t split = tile / nrb;
-  const size_t k0 = split * red_per_split;
-  const size_t k1 = min(k0 + red_per_split, red_len);
-  if (k0 >= k1) return;  // uniform over the work-group
-  const int bcol0 = tid / SEGS, bseg = tid % SEGS;
-  static_assert(KCH <= 512, "a row's chunk lies inside one 128-byte tile segment");
-  const size_t ntile = code_stride / 128;  // tiles per 16-row group
-  const uint8_t* abase = codes + (rowb / 16) * ntile * 2048 + r * 128 + (KCH / 16) * q;
-  const int8_t* bbase = bq + (grp * KP + (size_t)bcol0) * b_stride + 16 * bseg;
-  i32x4 acc[RT][KT];
+  const int tid = threadIdx.x, lane = tid & 63, r = lane & 15, q = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const size_t rb = tile % nrb, split = tile / nrb;
+  const size_t c0 = split * cps, c1 = min(c0 + cps, nkc);
+  if (c0 >= c1) return;  // uniform over the work-group
+  const size_t nc = c1 - c0;
+  const uint8_t* asrc = codes + (rb * nkc + c0) * ABYTES + lane * 16;
+  const int8_t* bsrc[B_INS];
+  uint32_t bdst[B_INS];
 #pragma unroll
-  for (int g = 0; g < RT; ++g)
+  for (int i = 0; i < B_INS; ++i) {
+    int j = wave + NW * i;
+    if (j >= BTOT) j %= BTOT;
+    const int p = 64 * j + lane, c = p / UPC, s = p % UPC, u = s ^ (c & 7);
+    bsrc[i] = bq + (grp * KP + c) * b_stride + c0 * KCH + 16 * u;
+    bdst[i] = ABYTES + 1024 * j;
+  }
+  const uint32_t sbase = lds_address(smem);
+  auto load = [&](int d, size_t ci, uint32_t st) {
+    if (d < A_INS) {
+      const int j = wave + NW * d;
+      lds_dma16(st + j * 1024, asrc + ci * ABYTES + j * 1024);
+    } else {
+      lds_dma16(st + bdst[d - A_INS], bsrc[d - A_INS] + ci * KCH);
+    }
+  };
+#pragma unroll
+  for (int s = 0; s < NS - 2; ++s)
+#pragma unroll
+    for (int d = 0; d < D; ++d) load(d, min((size_t)s, nc - 1), sbase + s * STAGE);
+  i32x4 acc[4][KT];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
 #pragma unroll
     for (int t = 0; t < KT; ++t) acc[g][t] = i32x4{0, 0, 0, 0};
-  u32x4 pa[RT][AV], pb[B_LOADS], a_cur[RT][AV];
+  int cur = 0, nxt = NS - 2;
+  for (size_t c = 0; c < nc; ++c) {
+    asm volatile("s_waitcnt vmcnt(%0)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::"n"(D * (NS - 3))
+                 : "memory");
+    const size_t cn = min(c + NS - 2, nc - 1);
+    const uint32_t stn = sbase + nxt * STAGE;
+    const int8_t* sa = smem + cur * STAGE + wave * 1024 + lane * 16;
+    const int8_t* sb = smem + cur * STAGE + ABYTES + r * KCH;
 #pragma unroll
-  for (int g = 0; g < RT; ++g)
-#pragma unroll
-    for (int a = 0; a < AV; ++a)
-      pa[g][a] = *reinterpret_cast<const u32x4*>(abase + (g * ntile + k0 / 512) * 2048 +
-                                                 (k0 / 4) % 128 + 16 * a);
-#pragma unroll
-  for (int j = 0; j < B_LOADS; ++j)
-    if (bcol0 + BCSTEP * j < KP)
-      pb[j] = *reinterpret_cast<const u32x4*>(bbase + (size_t)(BCSTEP * j) * b_stride + k0);
-#pragma unroll
-  for (int j = 0; j < B_LOADS; ++j)
-    if (bcol0 + BCSTEP * j < KP)
-      *reinterpret_cast<u32x4*>(sB + (bcol0 + BCSTEP * j) * SB + 16 * bseg) = pb[j];
-  __syncthreads();
-  int buf = 0;
-  for (size_t kk = k0; kk < k1; kk += KCH) {
-#pragma unroll
-    for (int g = 0; g < RT; ++g)
-#pragma unroll
-      for (int a = 0; a < AV; ++a) a_cur[g][a] = pa[g][a];
-    {
-      // next chunk; the last pass reloads its own chunk (in bounds, unused) so that the
-      // prefetch stays in registers
-      const size_t kn = (kk + KCH < k1) ? kk + KCH : kk;
-#pragma unroll
-      for (int g = 0; g < RT; ++g)
-#pragma unroll
-        for (int a = 0; a < AV; ++a)
-          pa[g][a] = *reinterpret_cast<const u32x4*>(abase + (g * ntile + kn / 512) * 2048 +
-                                                     (kn / 4) % 128 + 16 * a);
-#pragma unroll
-      for (int j = 0; j < B_LOADS; ++j)
-        if (bcol0 + BCSTEP * j < KP)
-          pb[j] = *reinterpret_cast<const u32x4*>(bbase + (size_t)(BCSTEP * j) * b_stride + kn);
-    }
-    const int8_t* sb = sB + buf * BUF;
-#pragma unroll
-    for (int ks = 0; ks < KCH / 64; ++ks) {
+    for (int ks = 0; ks < KS; ++ks) {
+      const u32x4 aw = *reinterpret_cast<const u32x4*>(sa + ks * 8192);
       i32x4 b[KT];
 #pragma unroll
       for (int t = 0; t < KT; ++t)
-        b[t] = *reinterpret_cast<const i32x4*>(sb + (16 * t + r) * SB + (KCH / 4) * q + 16 * ks);
+        b[t] = *reinterpret_cast<const i32x4*>(sb + 16 * t * KCH + (((4 * ks + q) ^ (r & 7)) * 16));
 #pragma unroll
-      for (int g = 0; g < RT; ++g) {
-        const uint32_t w = a_cur[g][ks >> 2][ks & 3];
+      for (int g = 0; g < 4; ++g) {
+        const int gb = ks * 4 + g;
+        if (gb % EVERY == 0 && gb / EVERY < D) {
+          __builtin_amdgcn_sched_barrier(0);
+          load(gb / EVERY, cn, stn);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        // word u holds fields u of the four bytes: reduction indices 4b + u, b = 0..3
+        const uint32_t w = aw[g];
         i32x4 a;
-#pragma unroll
-        for (int u = 0; u < 4; ++u)  // codes 00, 01, 10, 11 -> 2, 0, 1, 0
-          a[u] = (int)__builtin_amdgcn_perm(0u, 0x00010002u, (w >> (2 * u)) & 0x03030303u);
+        a[0] = (int)(w & 0x03030303u);
+        a[1] = (int)((w >> 2) & 0x03030303u);
+        a[2] = (int)((w >> 4) & 0x03030303u);
+        a[3] = (int)((w >> 6) & 0x03030303u);
 #pragma unroll
         for (int t = 0; t < KT; ++t)
           acc[g][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[t], acc[g][t], 0, 0, 0);
       }
     }
-    // the other buffer was last read before the previous barrier
+    if (GB / EVERY < D) {
 #pragma unroll
-    for (int j = 0; j < B_LOADS; ++j)
-      if (bcol0 + BCSTEP * j < KP)
-        *reinterpret_cast<u32x4*>(sB + (1 - buf) * BUF + (bcol0 + BCSTEP * j) * SB + 16 * bseg) =
-            pb[j];
-    __syncthreads();
-    buf = 1 - buf;
+      for (int d = GB / EVERY; d < D; ++d) load(d, cn, stn);
+    }
+    cur = cur + 1 == NS ? 0 : cur + 1;
+    nxt = nxt + 1 == NS ? 0 : nxt + 1;
   }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   // lane (r, q) holds rows 16g + 4q .. + 3 of column 16t + r: one 16-byte store each
+  const size_t rowb = rb * ROWS + (size_t)wave * 64;
   int* pbase = part + (split * part_ld + grp * KP + r) * rows_pad + rowb + 4 * q;
 #pragma unroll
-  for (int g = 0; g < RT; ++g)
+  for (int g = 0; g < 4; ++g)
 #pragma unroll
     for (int t = 0; t < KT; ++t)
       *reinterpret_cast<i32x4*>(pbase + (size_t)16 * t * rows_pad + 16 * g) = acc[g][t];
@@ -1252,17 +1298,16 @@ __global__ __launch_bounds__(256) void i8_stage2_epilogue_kernel(
   }
 }
 
-// Work-groups of i8_gemm_kernel<RT, KT, NW, KCH> resident at once on the device (cached per
+// Work-groups of i8_gemm_kernel<KT, NS> resident at once on the device (cached per
 // instantiation).
-template <int RT, int KT, int NW, int KCH>
+template <int KT, int NS>
 static size_t resident_slots() {
   static size_t slots = 0;
   if (!slots) {
     int dev = 0, cus = 0, occ = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, i8_gemm_kernel<RT, KT, NW, KCH>, NW * 64,
-                                                       0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, i8_gemm_kernel<KT, NS>, 512, 0);
     slots = (size_t)std::max(1, cus) * (size_t)std::max(1, occ);
   }
   return slots;
@@ -1284,65 +1329,59 @@ static size_t pick_splits(size_t blocks, size_t nchunks, size_t slots) {
 }
 
 template <int KT>
-struct I8Shape {
-  static constexpr int RT = 4;
-  static constexpr int NW = KT <= 2 ? 4 : 8;
-  // few columns: the pass is bound by the genotype stream, so each row's chunk is a whole
-  // 128-byte line; more columns: 64-byte chunks keep the prefetch registers in budget
-  static constexpr int KCH = KT <= 2 ? 512 : 256;
-};
-
-template <int KT>
-static int i8_gemm(saige_grm* h, const uint8_t* codes, size_t code_stride, const int8_t* bq,
-                   size_t b_stride, size_t rows_pad, size_t red_len, size_t groups, DevBuf& part,
+static int i8_gemm(saige_grm* h, const uint8_t* codes, const int8_t* bq, size_t b_stride,
+                   size_t rows_pad, size_t red_len, size_t groups, DevBuf& part,
                    size_t* splits_out, size_t vectors) {
-  using S = I8Shape<KT>;
-  constexpr int ROWS = S::NW * 16 * S::RT;
-  static_assert(kMarkerPad % ROWS == 0 && kSamplePad % ROWS == 0, "row tiles must divide padding");
+  constexpr int NS = I8Ring<KT>::NS;
+  SAIGE_ARG(rows_pad % 512 == 0 && red_len % kI8Chunk == 0 && b_stride >= red_len,
+            "int8 GEMM shape does not match the blocked genotype layout");
   hipStream_t s = h->stream;
   const size_t kpt = (size_t)16 * KT * groups;
-  const size_t rblk = rows_pad / ROWS;
-  const size_t nch = red_len / S::KCH;
-  const size_t splits =
-      pick_splits(rblk * groups, nch, resident_slots<S::RT, KT, S::NW, S::KCH>());
-  const size_t rps = ceil_div(nch, splits) * S::KCH;
-  const size_t sp = ceil_div(red_len, rps);
+  const size_t rblk = rows_pad / 512;
+  const size_t nkc = red_len / kI8Chunk;
+  const size_t splits = pick_splits(rblk * groups, nkc, resident_slots<KT, NS>());
+  const size_t cps = ceil_div(nkc, splits);
+  const size_t sp = ceil_div(nkc, cps);
   SAIGE_TRY(part.ensure(sp * rows_pad * kpt * 4));
   const size_t blocks = round_up(rblk * sp, 8) * groups;
   SAIGE_ARG(blocks < (1u << 31), "int8 GEMM grid too large");
-  SAIGE_TRY(timed_launch(h->timer, LaunchTag{(uint32_t)kpt, (uint32_t)vectors}, i8_gemm_kernel<S::RT, KT, S::NW, S::KCH>,
-                         dim3((unsigned)blocks), dim3(S::NW * 64), s, codes, code_stride, bq,
-                         b_stride, rps, red_len, part.as<int>(), rows_pad, kpt, rblk, sp,
-                         groups));
+  SAIGE_TRY(timed_launch(h->timer, LaunchTag{(uint32_t)kpt, (uint32_t)vectors},
+                         i8_gemm_kernel<KT, NS>, dim3((unsigned)blocks), dim3(512), s, codes, nkc,
+                         bq, b_stride, cps, part.as<int>(), rows_pad, kpt, rblk, sp, groups));
   *splits_out = sp;
   return SAIGE_OK;
 }
 
-static int i8_gemm_dispatch(int kt, saige_grm* h, const uint8_t* codes, size_t code_stride,
-                            const int8_t* bq, size_t b_stride, size_t rows_pad, size_t red_len,
-                            size_t groups, DevBuf& part, size_t* splits, size_t vectors) {
+static int i8_gemm_dispatch(int kt, saige_grm* h, const uint8_t* codes, const int8_t* bq,
+                            size_t b_stride, size_t rows_pad, size_t red_len, size_t groups,
+                            DevBuf& part, size_t* splits, size_t vectors) {
   switch (kt) {
-    case 1: return i8_gemm<1>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
-    case 2: return i8_gemm<2>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
-    case 3: return i8_gemm<3>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
-    case 4: return i8_gemm<4>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
-    case 5: return i8_gemm<5>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
-    case 6: return i8_gemm<6>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
-    default: return i8_gemm<7>(h, codes, code_stride, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 1: return i8_gemm<1>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 2: return i8_gemm<2>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 3: return i8_gemm<3>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 4: return i8_gemm<4>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 5: return i8_gemm<5>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 6: return i8_gemm<6>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 7: return i8_gemm<7>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 8: return i8_gemm<8>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    case 9: return i8_gemm<9>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
+    default: return i8_gemm<10>(h, codes, bq, b_stride, rows_pad, red_len, groups, part, splits, vectors);
   }
 }
 
-// Column groups for `tiles` 16-column tiles: one group of all of them up to 7; beyond, the
-// group width (4 to 7 tiles) with the fewest executed tiles, then the fewest groups.
+// Column groups for `tiles` 16-column tiles: one group of all of them up to kI8MaxKT; beyond,
+// the group width (5 to kI8MaxKT tiles) with the fewest executed tiles, then the fewest
+// groups.
+constexpr int kI8MaxKT = 10;
 static void i8_groups(size_t tiles, int* kt, size_t* groups) {
-  if (tiles <= 7) {
+  if (tiles <= (size_t)kI8MaxKT) {
     *kt = (int)tiles;
     *groups = 1;
     return;
   }
   size_t best_cost = SIZE_MAX, best_g = 0;
-  int best_kt = 7;
-  for (int t = 7; t >= 4; --t) {
+  int best_kt = kI8MaxKT;
+  for (int t = kI8MaxKT; t >= 5; --t) {
     const size_t g = ceil_div(tiles, (size_t)t);
     const size_t cost = g * (size_t)t;
     if (cost < best_cost || (cost == best_cost && g < best_g)) {
@@ -1404,8 +1443,8 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
 
   // ---- stage 1: I = G^T Q_v (rows = markers, reduction over samples) ----
   size_t splits1 = 0;
-  SAIGE_TRY(i8_gemm_dispatch(kt, h, g.geno, g.row_bytes, ms->vq.as<int8_t>(), n_pad, m_pad,
-                             n_pad, groups, ms->ipart1, &splits1, k));
+  SAIGE_TRY(i8_gemm_dispatch(kt, h, g.geno, ms->vq.as<int8_t>(), n_pad, m_pad, n_pad, groups,
+                             ms->ipart1, &splits1, k));
   i8_col_scalars_kernel<<<cb, 64, 0, s>>>(k, sm, ms4, pdot, nch_n, colsc);
   i8_stage1_epilogue_kernel<<<ge1, 256, 0, s>>>(
       ms->ipart1.as<int>(), splits1, m_pad, m, k, kpt, colsc, ms->sc.as<double>(),
@@ -1425,8 +1464,8 @@ static int int8_product_launch(saige_grm* h, const SliceMap& sm, size_t k3, cons
 
   // ---- stage 2: I = G Q_w (rows = samples, reduction over markers) ----
   size_t splits2 = 0;
-  SAIGE_TRY(i8_gemm_dispatch(kt, h, ms->geno_t.as<uint8_t>(), ms->rbt, ms->wq.as<int8_t>(),
-                             m_pad, n_pad, m_pad, groups, ms->ipart2, &splits2, k));
+  SAIGE_TRY(i8_gemm_dispatch(kt, h, ms->geno_t.as<uint8_t>(), ms->wq.as<int8_t>(), m_pad, n_pad,
+                             m_pad, groups, ms->ipart2, &splits2, k));
   i8_col_scalars_kernel<<<cb, 64, 0, s>>>(k, sm, ms4, pdot, nch_m, colsc);
   i8_stage2_epilogue_kernel<<<ge2, 256, 0, s>>>(
       ms->ipart2.as<int>(), splits2, n_pad, n, k, kpt, colsc, 1.0 / (double)m,

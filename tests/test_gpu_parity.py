@@ -475,6 +475,32 @@ def test_grm_mixed_error_bound(n, m, k, p_missing, prec):
         assert np.linalg.norm(one - got[k // 2]) < 1e-5 * np.linalg.norm(one)
 
 
+@pytest.mark.parametrize("k", [1, 6, 15, 20, 25, 30, 35, 40, 45, 50, 60])
+def test_grm_int8_every_tile_width(k):
+    """The int8 GEMM is instantiated per column-tile count KT = 1..10 of a group (three
+    slices: 3k columns in ceil(3k / 16) tiles, one group up to 10 tiles; k = 60 is two groups
+    of 6 tiles).  Every width meets the three-slice bound against the fp64 oracle, and a
+    column computed inside the wide launch equals, bit for bit, the same column computed
+    alone (KT = 1): the int32 sums are exact whatever the tile shape, the split of the
+    reduction or the group."""
+    n, m = 900, 1100
+    _say(f"int8 tile width k={k} ({-(-3 * k // 16)} tiles)")
+    body, bpm, d, af = _grm_pair(n, m, seed=11 * k + 5, p_missing=0.02, mono=True)
+    g = sg.OnTheFlyGrm(body, bpm, n, af, precision=sg.PREC_INT8)
+    ref = oracle.Grm(d, af, oracle.GRM_F64)
+    rng = np.random.default_rng(100 + k)
+    V = rng.standard_normal((k, n)) + 1.5
+    got = g.mat_mat(V)
+    worst = 0.0
+    for c in range(k):
+        want = ref.mat_vec(V[c])
+        worst = max(worst, np.linalg.norm(got[c] - want) / np.linalg.norm(want))
+    _say(f"int8 k={k} worst relative error {worst:.2e}")
+    assert worst < _BOUND[sg.PREC_INT8]
+    for c in sorted({0, k // 2, k - 1}):
+        np.testing.assert_array_equal(g.mat_mat(V[c:c + 1])[0], got[c])
+
+
 @pytest.mark.parametrize("prec", _REDUCED, ids=_REDUCED_IDS)
 @pytest.mark.parametrize("p_missing", [0.0, 0.03], ids=["complete", "missing3pct"])
 def test_fit_null_mixed_vs_fp64_oracle(prec, p_missing):
