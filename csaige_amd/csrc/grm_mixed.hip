@@ -186,26 +186,47 @@ __device__ __forceinline__ uint32_t codes_to_values(uint32_t w) {
   return ((nlo & ~hi & 0x55555555u) << 1) | (nlo & hi);
 }
 
-__global__ void block_codes_kernel(const uint8_t* __restrict__ src, size_t row_bytes,
-                                   size_t rows, uint8_t* __restrict__ dst) {
-  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // 32-bit word
-  const size_t per_row = row_bytes / 4;
-  if (v >= rows * per_row) return;
-  const size_t row = v / per_row, kd = v % per_row, red = 16 * kd;
-  const size_t rb = row / 512, rg = (row % 512) / 64, g = (row % 64) / 16, r = row % 16;
-  const size_t kc = red / kI8Chunk, s = (red % kI8Chunk) / 64, q = (red % 64) / 16;
+// One work-group per (row block rb, four chunks kq): the 512 rows' 128-byte runs of the four
+// chunks are read as whole lines (eight threads per row), converted to values and placed in
+// LDS in the blocked order, then written out as the four contiguous 16 KB blocks.
+__global__ __launch_bounds__(256) void block_codes_kernel(const uint8_t* __restrict__ src,
+                                                          size_t row_bytes,
+                                                          uint8_t* __restrict__ dst) {
+  constexpr int kUnits = 4 * (int)kI8Block / 16;  // 16-byte units in four blocks
+  __shared__ __attribute__((aligned(16))) uint32_t sh[4 * kI8Block / 4];
+  const size_t kq = blockIdx.x, rb = blockIdx.y;
   const size_t nkc = row_bytes * 4 / kI8Chunk;
-  const size_t off = (rb * nkc + kc) * kI8Block + ((s * 8 + rg) * 64 + 16 * q + r) * 16 + g * 4;
-  *reinterpret_cast<uint32_t*>(dst + off) =
-      codes_to_values(*reinterpret_cast<const uint32_t*>(src + row * row_bytes + 4 * kd));
+  for (int i = 0; i < kUnits / 256; ++i) {
+    const int u = (int)threadIdx.x + 256 * i;  // row u / 8, 16-byte unit u % 8 of its 128 bytes
+    const int row = u >> 3, j = u & 7;
+    const uint4 x = *reinterpret_cast<const uint4*>(src + (rb * 512 + row) * row_bytes +
+                                                    kq * 128 + 16 * j);
+    const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+    const int rg = row >> 6, g = (row >> 4) & 3, r = row & 15;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const int kd = 4 * j + d;  // 16-index word of the 512: chunk kd / 8, step, quarter
+      const int kcl = kd >> 3, s = (kd >> 2) & 1, q = kd & 3;
+      sh[(kcl * (int)kI8Block + ((s * 8 + rg) * 64 + 16 * q + r) * 16 + g * 4) / 4] =
+          codes_to_values(w[d]);
+    }
+  }
+  __syncthreads();
+  uint4* out = reinterpret_cast<uint4*>(dst + (rb * nkc + 4 * kq) * kI8Block);
+  for (int i = 0; i < kUnits / 256; ++i) {
+    const int u = (int)threadIdx.x + 256 * i;
+    out[u] = reinterpret_cast<const uint4*>(sh)[u];
+  }
 }
 
 // buf (rows x row_bytes of PLINK codes, row-major) replaced by its blocked value form
 static int block_codes(hipStream_t s, DevBuf& buf, size_t rows, size_t row_bytes) {
+  SAIGE_ARG(rows % 512 == 0 && row_bytes % 128 == 0, "blocked genotype layout needs 512-row "
+            "blocks and 512-index reduction groups");
   DevBuf tmp;
   SAIGE_TRY(tmp.ensure(rows * row_bytes));
-  block_codes_kernel<<<(unsigned)ceil_div(rows * (row_bytes / 4), 256), 256, 0, s>>>(
-      buf.as<uint8_t>(), row_bytes, rows, tmp.as<uint8_t>());
+  block_codes_kernel<<<dim3((unsigned)(row_bytes / 128), (unsigned)(rows / 512)), 256, 0, s>>>(
+      buf.as<uint8_t>(), row_bytes, tmp.as<uint8_t>());
   SAIGE_HIP(hipGetLastError());
   SAIGE_HIP(hipStreamSynchronize(s));  // the row-major buffer goes back to the cache
   std::swap(buf.p, tmp.p);
