@@ -37,7 +37,7 @@ I8_PEAK_TOPS = 5000.0
 # (one counter per run; FETCH_SIZE scaled by 2.0 from a 1 GiB copy) for the default families
 # workload: mean of the marker-side and sample-side launches.  Keyed by (families, markers,
 # int8 form).
-MEASURED_TRAFFIC = {(10000, 100000, "int8"): 1.51e9, (10000, 100000, "int8x2"): 1.46e9}
+MEASURED_TRAFFIC = {(10000, 100000, "int8x2"): 1.45e9}
 
 
 def log(msg):
@@ -823,7 +823,7 @@ def run_families(args):
 # measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (one counter per run; FETCH_SIZE scaled
 # by the factor a 1 GiB device copy gives, 2.0): keyed by (n, markers, form, operand width).
 MEASURED_TRAFFIC_BIOBANK = {(400000, 150000, "int8x2", 224): 1.78e10,
-                            (400000, 150000, "int8x2", 288): 2.07e10}
+                            (400000, 150000, "int8x2", 288): 1.87e10}
 
 
 def step2_stream(args, sg, hip, synth):

@@ -127,7 +127,8 @@ def test_grm_repack_paths(n_total, n, prec):
     """The three repack forms: identity sample order (funnel-shift copy), a sample list with
     rows staged in LDS, and a list over rows too long to stage (270,000 samples, 67.5 KB per
     marker: direct gather); unaligned row starts (odd bytes per marker), markers selected and
-    reordered.  FP64 and the int8 path (tiled codes, bit-transposed sample-major copy)."""
+    reordered.  FP64 and the int8 path (blocked genotype values, bit-transposed sample-major
+    copy)."""
     m_total = 37
     body, bpm = random_bed(n_total, m_total, seed=n_total % 97, p_missing=0.0)
     rng = np.random.default_rng(n)
