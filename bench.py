@@ -385,11 +385,16 @@ def run_learned(args):
         it1 = float((r1["spa_iters1"][ok] + r1["spa_iters2"][ok]).mean())
         d = float(np.abs(np.log10(r0["pvalue"][ok]) - np.log10(r1["pvalue"][ok])).max())
         lost = gated & (r0["spa_converged"] == 0)
+        # the plain method's outcome on every gated variant: the same convergence and SPA
+        # flags, and the same (unadjusted) p where the plain search diverges
+        diff = gated & ((r0["spa_converged"] != r1["spa_converged"]) |
+                        (r0["is_spa"] != r1["is_spa"]))
+        diff |= lost & (r0["pvalue"] != r1["pvalue"])
         res = {"workload": tag, "variants": int(nbat * B), "spa_variants": int(gated.sum()),
                "mean_newton_iterations_plain": it0, "mean_newton_iterations_mlp": it1,
                "max_abs_dlog10p_vs_plain": d,
                "plain_diverged": int(lost.sum()),
-               "rescued_by_proposal": int((lost & (r1["spa_converged"] == 1)).sum()),
+               "outcome_differences_vs_plain": int(diff.sum()),
                "step2_device_s_plain": out["plain"][1], "step2_device_s_mlp": out["mlp"][1]}
         log(f"{tag}: {res}")
         return res

@@ -78,6 +78,22 @@ struct MlpView {
 };
 void mlp_view(const saige_mlp* h, const float** w, float* mean, float* inv_std, float* y_scale);
 
+// The learned proposal only replaces the path of find_root_k1_fast (fast.rs:53-132) where
+// that search, started at 0, is known to reach the same root; everywhere else the search
+// runs from 0 and its outcome, a root or the divergence that makes the reference report
+// the unadjusted p, is the reference's own.  The search from 0 fails only when an iterate
+// reaches |t| where exp(-g t) overflows (|t| > 709 / max|g~|, at least ~354 for dosages in
+// [0, 2]) or K''(t) underflows to 0, which takes |t| of the same size; on 2 x 10^4 gated
+// roots of seeded cohorts (n = 20,000 and 50,000, prevalence 0.5% to 10%) every failure had
+// |root| >= 259, and on roots |root| <= 50 the path of the search from 0 never left
+// 4.8 |root|.  So a proposal is used only as a start within kProposalMaxRoot of 0, and a
+// search from it that fails or ends at a root beyond kProposalMaxRoot is run again from 0.
+constexpr double kProposalMaxRoot = 20.0;
+
+__host__ __device__ __forceinline__ double proposal_start(double t) {
+  return (isfinite(t) && fabs(t) <= kProposalMaxRoot) ? t : 0.0;
+}
+
 // Both roots' starting points from the learned proposal; all threads of the block call it
 // and all receive the result.  128 threads: thread (r, j) owns hidden unit j of root r.
 __device__ void mlp_propose2(const MlpView& mv, const GateInfo& g, float* sh, double* tinit) {
@@ -106,7 +122,7 @@ __device__ void mlp_propose2(const MlpView& mv, const GateInfo& g, float* sh, do
     float o = w[kMlpB3];
     for (int k = 0; k < kMlpH; ++k) o += sh[tid * kMlpH + k];
     const double t = sinh((double)o * (double)mv.y_scale) / g.sd;
-    sh[2 * kMlpH + tid] = isfinite(t) ? (float)t : 0.0f;
+    sh[2 * kMlpH + tid] = (float)proposal_start(t);
   }
   __syncthreads();
   tinit[0] = (double)sh[2 * kMlpH];
@@ -656,6 +672,13 @@ struct RootState {
   int spent;  // iterations of an abandoned start (learned proposal that did not converge)
 };
 
+// A search begun at a learned proposal (tinit != 0) that failed, or that ended at a root
+// beyond kProposalMaxRoot, is run again from 0, once.
+__device__ __forceinline__ bool proposal_abandoned(const RootState& s, double tinit) {
+  return s.phase == 3 && tinit != 0.0 && s.spent == 0 &&
+         (!s.converged || !(fabs(s.root) <= kProposalMaxRoot));
+}
+
 // Begin (or restart) the search at t0 with the state find_root_k1_fast starts from.
 __device__ __forceinline__ void root_start(RootState& s, double t0) {
   s.t = t0;
@@ -793,8 +816,8 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
   }
   const int max_iter = 1000;
   // Each pass evaluates K'(t) - q and K''(t) at both roots' pending points.  A search from
-  // a learned proposal that fails is restarted once from 0, the plain method's start, so
-  // the proposal can only add roots, never lose one the plain method finds.
+  // a learned proposal that fails or ends beyond kProposalMaxRoot is restarted once from 0,
+  // the plain method's start, so that the outcome is the plain method's.
   for (int guard = 0; guard < 8 * max_iter + 16; ++guard) {
     if (st[0].phase == 3 && st[1].phase == 3) break;
     const double t0 = st[0].pend, t1 = st[1].pend;
@@ -837,7 +860,7 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
       const double k1 = b[2 * r] + na_mu + na_sigma * tt - q[r];
       const double k2 = b[2 * r + 1] + na_sigma;
       root_after_eval(st[r], k1, k2, tol, max_iter);
-      if (st[r].phase == 3 && !st[r].converged && tinit[r] != 0.0 && st[r].spent == 0) {
+      if (proposal_abandoned(st[r], tinit[r])) {
         st[r].spent = st[r].it;
         root_start(st[r], 0.0);
       }
@@ -924,7 +947,7 @@ __global__ __launch_bounds__(kSpaBlock) void spa_kernel(G gt, const double* __re
 // Batched SPA: the passes of spa_kernel with each gated variant's samples split over S
 // work-groups.  A pass kernel writes per-slice partial sums; a step kernel (one thread per
 // variant) adds them in slice order and advances both root searches with the same state
-// machine (root_after_eval, restart from 0 after a failed proposal).  A batch's few
+// machine (root_after_eval, restart from 0 after an abandoned proposal).  A batch's few
 // hundred gated variants then fill the GPU instead of occupying one work-group each.
 // ---------------------------------------------------------------------------------
 struct SpaVar {
@@ -1013,6 +1036,8 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
   const size_t v = g.var;
   const double thr = fast ? 1e-10 : -1.0;  // fast: |g~| > 1e-10 enters K (fast.rs:191-200)
   double t0 = 0.0, t1 = 0.0;
+  const bool act0 = KIND != 1 || vars[slot].st[0].phase != 3;
+  const bool act1 = KIND != 1 || vars[slot].st[1].phase != 3;
   if (KIND == 1) {
     t0 = vars[slot].st[0].pend;
     t1 = vars[slot].st[1].pend;
@@ -1047,14 +1072,24 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
     }
     const double gi = gt(v, i, g.ag);
     if (!(fabs(gi) > thr)) continue;
-    const double e0 = exp(-gi * t0), e1 = exp(-gi * t1);
     const double om = 1.0 - m;
     const double mg = m * gi, w2 = om * mg * gi;
+    if (KIND == 1) {  // a finished root's terms are skipped (uniform over the block)
+      if (act0) {
+        const double e0 = exp(-gi * t0), r0 = recip_nr(fma(om, e0, m));
+        a[0] = fma(mg, r0, a[0]);
+        a[1] = fma(w2 * e0, r0 * r0, a[1]);
+      }
+      if (act1) {
+        const double e1 = exp(-gi * t1), r1 = recip_nr(fma(om, e1, m));
+        a[2] = fma(mg, r1, a[2]);
+        a[3] = fma(w2 * e1, r1 * r1, a[3]);
+      }
+      continue;
+    }
+    const double e0 = exp(-gi * t0), e1 = exp(-gi * t1);
     const double r0 = recip_nr(fma(om, e0, m)), r1 = recip_nr(fma(om, e1, m));
-    if (KIND == 1) {
-      a[0] = fma(mg, r0, a[0]);
-      a[2] = fma(mg, r1, a[2]);
-    } else {
+    {
       // K at the roots is the sum of ln(1 - m + m e^{g zeta}) (binary.rs:148-151)
       lk0.mul(fma(m, exp(gi * t0), om));
       lk1.mul(fma(m, exp(gi * t1), om));
@@ -1139,7 +1174,9 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
   for (int k = 0; k < K; ++k) a[k] = 0.0;
   LogProd lk0, lk1;  // KIND 2: K at the roots as the logs of running products
   uint32_t bmin = 255u, bmax = 0u;
-  auto body = [&](uint32_t by, double m) {
+  // act: bit r set while root r still searches (KIND 1 skips the finished root's terms)
+  auto body = [&](uint32_t by, double m, auto act) {
+    constexpr int ACT = decltype(act)::value;
     const double* e = tab + by * TW;
     const double om = 1.0 - m;
     if (KIND == 0) {
@@ -1164,13 +1201,16 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
       // 1 / (1 + rho e), the K' term mu g / ((1 - mu) e + mu) is g u and the K'' term
       // (1 - mu) mu g^2 e / ((1 - mu) e + mu)^2 is g u (g - g u)
       const double gk = e[0];
-      const double u0 = recip_nr_finite(fma(m, e[1], 1.0));
-      const double u1 = recip_nr_finite(fma(m, e[2], 1.0));
-      const double g0 = gk * u0, g1 = gk * u1;
-      a[0] += g0;
-      a[2] += g1;
-      a[1] = fma(g0, gk - g0, a[1]);
-      a[3] = fma(g1, gk - g1, a[3]);
+      if constexpr ((ACT & 1) != 0) {
+        const double g0 = gk * recip_nr_finite(fma(m, e[1], 1.0));
+        a[0] += g0;
+        a[1] = fma(g0, gk - g0, a[1]);
+      }
+      if constexpr ((ACT & 2) != 0) {
+        const double g1 = gk * recip_nr_finite(fma(m, e[2], 1.0));
+        a[2] += g1;
+        a[3] = fma(g1, gk - g1, a[3]);
+      }
     } else {
       const double gk = e[0], e0 = e[1], e1 = e[2];
       const double mg = m * gk, w2 = om * mg * gk;
@@ -1190,42 +1230,49 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
   const size_t i0 = (size_t)blockIdx.y * slice, i1 = min(i0 + slice, n);
   const size_t i4 = i0 + ((i1 - i0) & ~size_t(3));
   constexpr size_t kStep = 4 * kSpaPassBlock;
-  size_t tail = i0;
-  if (ld8.vec) {  // 16-byte aligned rows; i0 is a multiple of 256
-    size_t i = i0 + 4 * threadIdx.x;
-    uint32_t wn = 0u;
-    f64x2 ma = {0.0, 0.0}, mb = {0.0, 0.0};
-    if (i < i4) {
-      wn = *reinterpret_cast<const uint32_t*>(row + i);
-      ma = *reinterpret_cast<const f64x2*>(mu + i);
-      mb = *reinterpret_cast<const f64x2*>(mu + i + 2);
-    }
-    for (; i < i4; i += kStep) {
-      const uint32_t w = wn;
-      const f64x2 ca = ma, cb = mb;
-      if (i + kStep < i4) {
-        wn = *reinterpret_cast<const uint32_t*>(row + i + kStep);
-        ma = *reinterpret_cast<const f64x2*>(mu + i + kStep);
-        mb = *reinterpret_cast<const f64x2*>(mu + i + kStep + 2);
+  auto sweep = [&](auto act) {
+    size_t tail = i0;
+    if (ld8.vec) {  // 16-byte aligned rows; i0 is a multiple of 256
+      size_t i = i0 + 4 * threadIdx.x;
+      uint32_t wn = 0u;
+      f64x2 ma = {0.0, 0.0}, mb = {0.0, 0.0};
+      if (i < i4) {
+        wn = *reinterpret_cast<const uint32_t*>(row + i);
+        ma = *reinterpret_cast<const f64x2*>(mu + i);
+        mb = *reinterpret_cast<const f64x2*>(mu + i + 2);
       }
-      body(w & 0xffu, ca.x);
-      body((w >> 8) & 0xffu, ca.y);
-      body((w >> 16) & 0xffu, cb.x);
-      body(w >> 24, cb.y);
+      for (; i < i4; i += kStep) {
+        const uint32_t w = wn;
+        const f64x2 ca = ma, cb = mb;
+        if (i + kStep < i4) {
+          wn = *reinterpret_cast<const uint32_t*>(row + i + kStep);
+          ma = *reinterpret_cast<const f64x2*>(mu + i + kStep);
+          mb = *reinterpret_cast<const f64x2*>(mu + i + kStep + 2);
+        }
+        body(w & 0xffu, ca.x, act);
+        body((w >> 8) & 0xffu, ca.y, act);
+        body((w >> 16) & 0xffu, cb.x, act);
+        body(w >> 24, cb.y, act);
+        if (KIND == 2) {
+          lk0.norm();
+          lk1.norm();
+        }
+      }
+      tail = i4;
+    }
+    for (size_t j = tail + threadIdx.x; j < i1; j += kSpaPassBlock) {
+      body(row[j], mu[j], act);
       if (KIND == 2) {
         lk0.norm();
         lk1.norm();
       }
     }
-    tail = i4;
-  }
-  for (size_t j = tail + threadIdx.x; j < i1; j += kSpaPassBlock) {
-    body(row[j], mu[j]);
-    if (KIND == 2) {
-      lk0.norm();
-      lk1.norm();
-    }
-  }
+  };
+  const bool act0 = KIND != 1 || vars[slot].st[0].phase != 3;
+  const bool act1 = KIND != 1 || vars[slot].st[1].phase != 3;
+  if (act0 && act1) sweep(std::integral_constant<int, 3>{});
+  else if (act0) sweep(std::integral_constant<int, 1>{});
+  else sweep(std::integral_constant<int, 2>{});
   if (KIND == 2) {
     a[0] = lk0.log_value();
     a[2] = lk1.log_value();
@@ -1361,7 +1408,7 @@ __global__ void spa_eval_step(const GateInfo* __restrict__ gate, const uint32_t*
     const double k1 = slice_sum(part, slot, S, 2 * r) + V.na_mu + V.na_sigma * tt - q[r];
     const double k2 = slice_sum(part, slot, S, 2 * r + 1) + V.na_sigma;
     root_after_eval(V.st[r], k1, k2, tol, max_iter);
-    if (V.st[r].phase == 3 && !V.st[r].converged && V.tinit[r] != 0.0 && V.st[r].spent == 0) {
+    if (proposal_abandoned(V.st[r], V.tinit[r])) {
       V.st[r].spent = V.st[r].it;
       root_start(V.st[r], 0.0);
     }

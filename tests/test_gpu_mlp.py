@@ -5,7 +5,6 @@ import pytest
 
 import csaige_amd as sg
 from csaige_amd import synth
-import oracle
 from oracle import mlp as om
 from tests.helpers import neglog10
 
@@ -58,7 +57,11 @@ def _spa_cohort(n, nvar, prevalence, seed):
 
 def test_trained_proposal_cuts_newton_iterations():
     """Train on roots from 1% and 10% prevalence runs, then compare on held-out variants:
-    same p-values (Newton refines both to tol 1e-10), fewer iterations."""
+    fewer Newton iterations and the plain method's outcome on every gated variant.  The
+    proposal starts only searches whose root lies within 20 of 0; there the search from 0
+    reaches the same unique root, and everywhere else the search runs from 0
+    (find_root_k1_fast, fast.rs:53-132), so a root the plain search misses (divergence, then
+    the unadjusted p of single_variant.rs) is missed the same way."""
     xs, ys = [], []
     for prev, seed in ((0.01, 11), (0.10, 21)):
         nf, a, q = _spa_cohort(20000, 6000, prev, seed)
@@ -81,36 +84,17 @@ def test_trained_proposal_cuts_newton_iterations():
     r1 = e.test_markers_u8(q)
     gated = r0["pvalue_noadj"] < 0.05
     plain_ok = gated & (r0["spa_converged"] == 1)
-    assert plain_ok.sum() > 50
+    lost = gated & (r0["spa_converged"] == 0)
+    assert plain_ok.sum() > 50 and lost.sum() > 0
     it0 = (r0["spa_iters1"][plain_ok] + r0["spa_iters2"][plain_ok]).mean()
     it1 = (r1["spa_iters1"][plain_ok] + r1["spa_iters2"][plain_ok]).mean()
     d = np.abs(neglog10(r0["pvalue"][plain_ok]) - neglog10(r1["pvalue"][plain_ok])).max()
     print(f"SPA'd {plain_ok.sum()}: mean Newton iterations {it0:.2f} -> {it1:.2f}; "
-          f"max |d -log10 p| {d:.2e}", flush=True)
-    # where the plain search from 0 converges, the proposal changes nothing but the path
-    assert it1 < 0.8 * it0
+          f"max |d -log10 p| {d:.2e}; plain search diverged on {lost.sum()}", flush=True)
+    assert it1 < 0.85 * it0
+    # the plain method's outcome on every gated variant
+    np.testing.assert_array_equal(r1["spa_converged"][gated], r0["spa_converged"][gated])
+    np.testing.assert_array_equal(r1["is_spa"][gated], r0["is_spa"][gated])
     assert d < 1e-6
-    np.testing.assert_array_equal(r0["is_spa"][plain_ok], r1["is_spa"][plain_ok])
-    np.testing.assert_array_equal(r1["spa_converged"][plain_ok], 1)
-    # where it diverges (K'' underflow sends the step to infinity), the proposal either
-    # falls back to the same unadjusted p or finds a root; such a root is checked with the
-    # oracle's search started there, which must stay there with the same p-value
-    lost = np.where(gated & (r0["spa_converged"] == 0))[0]
-    rescued = [v for v in lost if r1["spa_converged"][v] == 1]
-    print(f"plain search diverged on {len(lost)}; proposal found the roots of {len(rescued)}",
-          flush=True)
-    for v in lost:
-        if v not in rescued:
-            assert r1["pvalue"][v] == r0["pvalue"][v]
-    x = np.asarray(nf.x, np.float64).reshape(-1, len(nf.mu))
-    for v in rescued:
-        g = q[v].astype(np.float64) / 127.0
-        gt = g - x[0] * float(np.dot(a, g))
-        m1 = float(np.dot(nf.mu, gt))
-        qs = r1["score"][v] + m1
-        ref = oracle.spa_binary(nf.mu, gt, qs, 2.0 * m1 - qs, r1["pvalue_noadj"][v], tol=1e-10,
-                                init1=r1["spa_root1"][v], init2=r1["spa_root2"][v])
-        assert ref["converged"]
-        np.testing.assert_allclose([ref["root1"], ref["root2"]],
-                                   [r1["spa_root1"][v], r1["spa_root2"][v]], rtol=1e-8)
-        assert abs(neglog10(ref["pvalue"]) - neglog10(r1["pvalue"][v])) < 1e-6
+    np.testing.assert_array_equal(r1["pvalue"][lost], r0["pvalue"][lost])
+    np.testing.assert_array_equal(r1["pvalue"][lost], r1["pvalue_noadj"][lost])
