@@ -1015,6 +1015,69 @@ struct LogProd {
   }
 };
 
+// The Newton passes (KIND 1) work on the list of variants still searching: the step kernel
+// of each pass appends the variants that go on to the other half of a double list, and a
+// pass splits each listed variant into S slices with S chosen from the list's length, so
+// that a pass has about kSpaPassTarget work-groups however many variants remain (a pass
+// with few variants left takes as long as its share of the work, not as long as one slice
+// of a full batch).  A variant-slice's partials sit at its position in the pass; the
+// variant's sum over its slices is taken in slice order, so a variant's result depends only
+// on the list's length, never on its place in the list.
+constexpr uint32_t kSpaPassTarget = 1024;
+struct SpaList {
+  uint32_t* cnt;   // [2] variants in the list of pass parity 0 / 1
+  uint32_t* list;  // [2][ng] gate slots
+  uint32_t ng;
+  int pass;
+};
+__host__ __device__ __forceinline__ void spa_pass_dims(uint32_t active, size_t n, uint32_t* S,
+                                                       size_t* slice) {
+  size_t s = active == 0 ? 1 : kSpaPassTarget / active;
+  s = s < 1 ? 1 : (s > 64 ? 64 : s);
+  const size_t smax = (n + 2047) / 2048;  // slices of at least 2,048 samples
+  s = s > smax ? smax : s;
+  if (s < 1) s = 1;
+  const size_t sl = ((n + s - 1) / s + 255) / 256 * 256;
+  *slice = sl;
+  *S = (uint32_t)((n + sl - 1) / sl);
+}
+// What a pass work-group computes: the gate slot, its sample range and the position of its
+// partials.  KIND 0 and 2: grid (gated variants, S slices of `slice`); KIND 1: the list.
+struct SpaWork {
+  uint32_t slot;
+  size_t i0, i1, pos;
+  bool first;  // the variant's first slice
+  bool live;
+};
+template <int KIND>
+__device__ __forceinline__ SpaWork spa_work(const uint32_t* __restrict__ n_gate, size_t n,
+                                            size_t slice, const SpaList& sl) {
+  SpaWork w;
+  if constexpr (KIND != 1) {
+    w.slot = blockIdx.x;
+    w.live = w.slot < *n_gate;
+    w.i0 = (size_t)blockIdx.y * slice;
+    w.pos = (size_t)blockIdx.x * gridDim.y + blockIdx.y;
+    w.first = blockIdx.y == 0;
+  } else {
+    const int h = sl.pass & 1;
+    const uint32_t active = sl.cnt[h];
+    uint32_t S;
+    size_t slc;
+    spa_pass_dims(active, n, &S, &slc);
+    const uint32_t b = blockIdx.x;
+    w.live = b < active * S;
+    const uint32_t j = w.live ? b / S : 0u, q = w.live ? b % S : 0u;
+    w.slot = w.live ? sl.list[(size_t)h * sl.ng + j] : 0u;
+    w.i0 = (size_t)q * slc;
+    slice = slc;
+    w.pos = b;
+    w.first = q == 0;
+  }
+  w.i1 = min(w.i0 + slice, n);
+  return w;
+}
+
 // KIND 0: g_pos, g_neg, the NA moments, m1 = sum mu g~ (single_variant.rs:189-192) and the
 // carrier moments sum mu, sum mu(1-mu), sum mu^2(1-mu) over samples with dosage > 0 (the
 // learned proposal's inputs); 1: K'(t) and K''(t) sums at both pending points; 2: K(zeta)
@@ -1023,11 +1086,12 @@ template <class G, int KIND>
 __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
     G gt, const double* __restrict__ mu, size_t n, const GateInfo* __restrict__ gate,
     const uint32_t* __restrict__ n_gate, int fast, size_t slice,
-    const SpaVar* __restrict__ vars, double* __restrict__ part) {
+    const SpaVar* __restrict__ vars, double* __restrict__ part, SpaList sl) {
   constexpr int K = KIND == 0 ? 8 : 4;
-  const uint32_t slot = blockIdx.x;  // grid: (gated variants, slices)
-  if (slot >= *n_gate) return;
-  if (KIND == 1 && vars[slot].done) return;
+  if (KIND == 1 && blockIdx.x == 0 && threadIdx.x == 0) sl.cnt[(sl.pass + 1) & 1] = 0u;
+  const SpaWork wk = spa_work<KIND>(n_gate, n, slice, sl);
+  if (!wk.live) return;
+  const uint32_t slot = wk.slot;
   if (KIND == 2 && !(vars[slot].st[0].converged && vars[slot].st[1].converged)) return;
   __shared__ double sh[K * kSpaPassBlock];
   __shared__ GateInfo g;
@@ -1045,7 +1109,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
     t0 = vars[slot].st[0].root;
     t1 = vars[slot].st[1].root;
   }
-  const size_t i0 = (size_t)blockIdx.y * slice, i1 = min(i0 + slice, n);
+  const size_t i0 = wk.i0, i1 = wk.i1;
   double a[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) a[k] = 0.0;
@@ -1105,7 +1169,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
   }
   block_sum_t<kSpaPassBlock, K>(a, sh);
   if (threadIdx.x == 0) {
-    double* o = part + ((size_t)slot * gridDim.y + blockIdx.y) * kSpaPart;
+    double* o = part + wk.pos * kSpaPart;
     for (int k = 0; k < K; ++k) o[k] = a[k];
     if (KIND == 0) {
       o[8] = 0.0;
@@ -1123,12 +1187,13 @@ template <int KIND>
 __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
     LoadU8 ld8, const double* __restrict__ mu, size_t n, const GateInfo* __restrict__ gate,
     const uint32_t* __restrict__ n_gate, int fast, size_t slice,
-    const SpaVar* __restrict__ vars, double* __restrict__ part) {
+    const SpaVar* __restrict__ vars, double* __restrict__ part, SpaList sl) {
   constexpr int K = KIND == 0 ? 8 : 4;
   constexpr int TW = KIND == 0 ? 2 : (KIND == 1 ? 4 : 8);  // doubles per byte entry
-  const uint32_t slot = blockIdx.x;  // grid: (gated variants, slices)
-  if (slot >= *n_gate) return;
-  if (KIND == 1 && vars[slot].done) return;
+  if (KIND == 1 && blockIdx.x == 0 && threadIdx.x == 0) sl.cnt[(sl.pass + 1) & 1] = 0u;
+  const SpaWork wk = spa_work<KIND>(n_gate, n, slice, sl);
+  if (!wk.live) return;
+  const uint32_t slot = wk.slot;
   if (KIND == 2 && !(vars[slot].st[0].converged && vars[slot].st[1].converged)) return;
   __shared__ double sh[K * kSpaPassBlock];
   __shared__ __attribute__((aligned(16))) double tab[256 * TW];
@@ -1227,7 +1292,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
     }
   };
   const uint8_t* row = ld8.q + (size_t)g.var * ld8.ld;
-  const size_t i0 = (size_t)blockIdx.y * slice, i1 = min(i0 + slice, n);
+  const size_t i0 = wk.i0, i1 = wk.i1;
   const size_t i4 = i0 + ((i1 - i0) & ~size_t(3));
   constexpr size_t kStep = 4 * kSpaPassBlock;
   auto sweep = [&](auto act) {
@@ -1289,8 +1354,8 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
   }
   block_sum_t<kSpaPassBlock, K>(a, sh);
   if (threadIdx.x == 0) {
-    double* o = part + ((size_t)slot * gridDim.y + blockIdx.y) * kSpaPart;
-    if (KIND == 1 && blockIdx.y == 0) {
+    double* o = part + wk.pos * kSpaPart;
+    if (KIND == 1 && wk.first) {
       // g~ is increasing in the byte, so e^{-g t} is largest at the smallest or the largest
       // byte of the variant: K'' is NaN when either overflows (an included byte)
       const SpaVar& V = vars[slot];
@@ -1372,7 +1437,8 @@ __global__ void spa_setup_step(GateInfo* __restrict__ gate, const uint32_t* __re
 // Begin both searches at 0 (fast.rs:217-218) or at the learned proposal; a q outside the
 // support (fast.rs:67-73) has its root at infinity.
 __global__ void spa_start_step(const GateInfo* __restrict__ gate,
-                               const uint32_t* __restrict__ n_gate, SpaVar* __restrict__ vars) {
+                               const uint32_t* __restrict__ n_gate, SpaVar* __restrict__ vars,
+                               SpaList sl) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= *n_gate) return;
   SpaVar V = vars[slot];
@@ -1388,25 +1454,44 @@ __global__ void spa_start_step(const GateInfo* __restrict__ gate,
   }
   V.done = (V.st[0].phase == 3 && V.st[1].phase == 3) ? 1 : 0;
   vars[slot] = V;
+  if (!V.done) sl.list[atomicAdd(&sl.cnt[0], 1u)] = slot;  // the list of pass 0
 }
 
-// active[pass & 1] counts the variants still searching after this pass; the other counter
-// is cleared for the next pass.
-__global__ void spa_eval_step(const GateInfo* __restrict__ gate, const uint32_t* __restrict__ n_gate,
-                              int S, const double* __restrict__ part, double tol, int pass,
-                              SpaVar* __restrict__ vars, uint32_t* __restrict__ active) {
-  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot == 0) active[(pass + 1) & 1] = 0u;
-  if (slot >= *n_gate) return;
+// Step of Newton pass `pass` (sl.pass): one wave per listed variant. Lane q loads slice q's
+// partials (S <= 64) and the wave adds them in a fixed butterfly order (lane 0's totals are
+// used, so a variant's sums depend only on S); lane 0 then advances both searches and lists
+// the variant for the next pass if it goes on.
+__global__ __launch_bounds__(256) void spa_eval_step(const GateInfo* __restrict__ gate, size_t n,
+                                                     const double* __restrict__ part, double tol,
+                                                     SpaVar* __restrict__ vars, SpaList sl) {
+  const uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int h = sl.pass & 1;
+  const uint32_t active = sl.cnt[h];
+  if (j >= active) return;  // uniform over the wave
+  uint32_t S;
+  size_t slc;
+  spa_pass_dims(active, n, &S, &slc);
+  double sum[4] = {0.0, 0.0, 0.0, 0.0};
+  if ((uint32_t)lane < S) {
+    const double* pp = part + ((size_t)j * S + lane) * kSpaPart;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sum[k] = pp[k];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sum[k] += __shfl_xor(sum[k], off);
+  if (lane != 0) return;
+  const uint32_t slot = sl.list[(size_t)h * sl.ng + j];
   SpaVar V = vars[slot];
-  if (V.done) return;
   const double q[2] = {gate[slot].q, gate[slot].qinv};
   const int max_iter = 1000;
   for (int r = 0; r < 2; ++r) {
     if (V.st[r].phase == 3) continue;
     const double tt = V.st[r].pend;
-    const double k1 = slice_sum(part, slot, S, 2 * r) + V.na_mu + V.na_sigma * tt - q[r];
-    const double k2 = slice_sum(part, slot, S, 2 * r + 1) + V.na_sigma;
+    const double k1 = sum[2 * r] + V.na_mu + V.na_sigma * tt - q[r];
+    const double k2 = sum[2 * r + 1] + V.na_sigma;
     root_after_eval(V.st[r], k1, k2, tol, max_iter);
     if (proposal_abandoned(V.st[r], V.tinit[r])) {
       V.st[r].spent = V.st[r].it;
@@ -1415,7 +1500,7 @@ __global__ void spa_eval_step(const GateInfo* __restrict__ gate, const uint32_t*
   }
   V.done = (V.st[0].phase == 3 && V.st[1].phase == 3) ? 1 : 0;
   vars[slot] = V;
-  if (!V.done) atomicAdd(&active[pass & 1], 1u);
+  if (!V.done) sl.list[(size_t)(h ^ 1) * sl.ng + atomicAdd(&sl.cnt[h ^ 1], 1u)] = slot;
 }
 
 // saddle-point probabilities (fast.rs:135-180 and :220-241) and the outputs
@@ -1517,7 +1602,8 @@ struct BatchArea {
   size_t spa_vars_cap = 0;  // variants
   double* spa_part = nullptr;
   size_t spa_part_cap = 0;  // doubles
-  uint32_t* spa_active = nullptr;
+  uint32_t* spa_active = nullptr;  // two counts (256 bytes), then the two variant lists
+  size_t spa_list_cap = 0;         // variants per list
 };
 
 __global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
@@ -1567,16 +1653,20 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const
   uint32_t* active;
   SpaVar* vars;
   double* part;
-  if (ar.fixed) {  // ng <= the batch's variants, ng * S <= 1024 + ng: inside the area
-    SAIGE_ARG(ng <= ar.spa_vars_cap && ng * S * kSpaPart <= ar.spa_part_cap,
+  // partials: ng * S variant-slices for the setup and final passes, at most
+  // max(kSpaPassTarget, ng) for a Newton pass; both within kSpaPassTarget + ng
+  const size_t nslices = std::max<size_t>(ng * S, kSpaPassTarget + ng);
+  if (ar.fixed) {  // ng <= the batch's variants: inside the area
+    SAIGE_ARG(ng <= ar.spa_vars_cap && nslices * kSpaPart <= ar.spa_part_cap &&
+                  ng <= ar.spa_list_cap,
               "batched SPA state exceeds the scratch area");
     active = ar.spa_active;
     vars = ar.spa_vars;
     part = ar.spa_part;
   } else {
     SAIGE_TRY(e->spa_vars.ensure(ng * sizeof(SpaVar)));
-    SAIGE_TRY(e->spa_part.ensure(ng * S * kSpaPart * 8));
-    SAIGE_TRY(e->spa_active.ensure(64));
+    SAIGE_TRY(e->spa_part.ensure(nslices * kSpaPart * 8));
+    SAIGE_TRY(e->spa_active.ensure(256 + 2 * (size_t)ng * 4));
     active = e->spa_active.as<uint32_t>();
     vars = e->spa_vars.as<SpaVar>();
     part = e->spa_part.as<double>();
@@ -1589,27 +1679,34 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const
   const dim3 grid(ng, (unsigned)S);
   const unsigned sb = (unsigned)ceil_div(ng, 256);
   const int fast = e->ec.fast, iS = (int)S;
+  SpaList sl{active, active + 64, ng, 0};  // counts, then the two halves of the list
+  const dim3 grid1((unsigned)std::max<size_t>(kSpaPassTarget, ng));
   auto launch_pass = [&](auto kind) {
     constexpr int KIND = decltype(kind)::value;
+    const dim3 gr = KIND == 1 ? grid1 : grid;
     if constexpr (TAB)  // the Newton passes take the odds (1 - mu) / mu in place of mu
-      spa_pass_tab_kernel<KIND><<<grid, kSpaPassBlock, 0, s>>>(
-          gt.load, KIND == 1 ? odds : mu, n, gate, ngp, fast, slice, vars, part);
+      spa_pass_tab_kernel<KIND><<<gr, kSpaPassBlock, 0, s>>>(
+          gt.load, KIND == 1 ? odds : mu, n, gate, ngp, fast, slice, vars, part, sl);
     else
-      spa_pass_kernel<G, KIND><<<grid, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
-                                                              vars, part);
+      spa_pass_kernel<G, KIND><<<gr, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
+                                                            vars, part, sl);
   };
   zero_u32_kernel<<<1, 1, 0, s>>>(active);
   zero_u32_kernel<<<1, 1, 0, s>>>(active + 1);
   launch_pass(std::integral_constant<int, 0>{});
   spa_setup_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, vars, d_out);
   if (mv.enabled) spa_propose_kernel<<<ng, 128, 0, s>>>(gate, ngp, mv, vars);
-  spa_start_step<<<sb, 256, 0, s>>>(gate, ngp, vars);
+  spa_start_step<<<sb, 256, 0, s>>>(gate, ngp, vars, sl);
+  // A pass with no variant left costs a few microseconds, a read of the count one host
+  // round trip: the count is first read after pass 11, then after every fourth pass.
+  const unsigned sbw = (unsigned)ceil_div((size_t)ng * 64, 256);
   for (int pass = 0; pass < kSpaMaxPasses; ++pass) {
+    sl.pass = pass;
     launch_pass(std::integral_constant<int, 1>{});
-    spa_eval_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, e->ec.spa_tol, pass, vars, active);
-    if (pass % 4 == 3) {
+    spa_eval_step<<<sbw, 256, 0, s>>>(gate, n, part, e->ec.spa_tol, vars, sl);
+    if (pass >= 11 && pass % 4 == 3) {
       uint32_t cnt = 0;
-      SAIGE_HIP(hipMemcpyAsync(&cnt, active + (pass & 1), 4, hipMemcpyDeviceToHost, s));
+      SAIGE_HIP(hipMemcpyAsync(&cnt, active + ((pass + 1) & 1), 4, hipMemcpyDeviceToHost, s));
       SAIGE_HIP(hipStreamSynchronize(s));
       if (cnt == 0) break;
     }
@@ -1810,10 +1907,12 @@ constexpr size_t cround(size_t x, size_t m) { return (x + m - 1) / m * m; }
 constexpr size_t kScratchPart = (kFixedSplits + 1) * (2 * kMaxP + 6) * kScratchVariants * 8;
 constexpr size_t kScratchGate = cround(kScratchVariants * sizeof(GateInfo), 256);
 constexpr size_t kScratchVars = cround(kScratchVariants * sizeof(SpaVar), 256);
-// the SPA partials: ng * S <= 1024 + ng variant-slices (S <= ceil(1024 / ng))
+// the SPA partials: ng * S <= 1024 + ng variant-slices (S <= ceil(1024 / ng)) in the setup
+// and final passes, at most max(kSpaPassTarget, ng) in a Newton pass
 constexpr size_t kScratchSpaPart = cround((1024 + kScratchVariants) * kSpaPart * 8, 256);
+constexpr size_t kScratchSpaActive = cround(256 + 2 * kScratchVariants * 4, 256);
 constexpr size_t kScratchBytes = cround(kScratchPart, 256) + kScratchGate + 256 + kScratchVars +
-                                 kScratchSpaPart + 256;
+                                 kScratchSpaPart + kScratchSpaActive;
 static_assert(kScratchBytes <= (48u << 20), "scratch stays within 48 MiB");
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -1851,7 +1950,8 @@ static int scratch_area(int device, BatchArea* ar) {
   ar->spa_vars_cap = kScratchVariants;
   ar->spa_part = reinterpret_cast<double*>(take(kScratchSpaPart));
   ar->spa_part_cap = kScratchSpaPart / 8;
-  ar->spa_active = reinterpret_cast<uint32_t*>(take(256));
+  ar->spa_active = reinterpret_cast<uint32_t*>(take(kScratchSpaActive));
+  ar->spa_list_cap = kScratchVariants;
   return off <= kScratchBytes ? SAIGE_OK : SAIGE_ERR_ARG;
 }
 
