@@ -496,7 +496,10 @@ class ScoreTestEngine:
         return {k: raw[k][:len(sizes)].copy() for k in raw.dtype.names}
 
     def set_root_proposal(self, mlp):
-        """Start the SPA Newton iterations from the learned proposal (None: from 0)."""
+        """Start the SPA Newton iterations from the learned proposal (None: from 0).  Only
+        searches the proposal places within 20 of 0 start there, and a search from it that
+        fails or ends beyond 20 runs again from 0: every variant keeps the outcome of the
+        search from 0 (fast.rs:53-132)."""
         self._mlp = mlp
         check(load().saige_engine_set_root_proposal(self.h, None if mlp is None else mlp.h))
 

@@ -462,7 +462,11 @@ int saige_mlp_loss_grad(saige_mlp* h, const float* x, const float* y, size_t n, 
 int saige_mlp_train(saige_mlp* h, const float* x, const float* y, size_t n, uint32_t epochs,
                     uint32_t batch, float lr, uint64_t seed, float* loss_history);
 /* Use the proposal as the SPA starting point of an engine (NULL: start at 0 as
- * fast.rs:217-218 does).  The model must outlive its use by the engine. */
+ * fast.rs:217-218 does).  A proposal only starts searches it places within 20 of 0, and a
+ * search from it that fails or ends beyond 20 runs again from 0, so every variant keeps the
+ * outcome of the search from 0 (its root, or the unadjusted p where that search diverges);
+ * the proposal changes the path, not the result.  The model must outlive its use by the
+ * engine. */
 int saige_engine_set_root_proposal(saige_engine* e, const saige_mlp* mlp);
 
 /* ------------------------------------------------------------------------------
