@@ -444,6 +444,8 @@ def main():
                     help="HBM bytes per GEMM launch measured with rocprofv3 counters")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (counter calibration)")
+    ap.add_argument("--library-int8", action="store_true",
+                    help="only time the vendor int8 GEMM (torch._int_mm) at the product's shapes")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "families", "biobank", "learned", "loco", "region",
                              "product"],
@@ -481,6 +483,9 @@ def main():
         hip.device_synchronize()
         log("copied 3 x 1 GiB device to device")
         return
+    if args.library_int8:
+        library_int8_calibration()
+        return
     if args.no_graphs:
         import csaige_amd as sg
         sg.pcg_graphs(False)
@@ -507,6 +512,33 @@ def main():
         run_product(args)
         return
     run_headline(args)
+
+
+def library_int8_calibration():
+    """The vendor int8 GEMM (hipBLASLt through torch._int_mm) at the genotype product's
+    shapes, as a point of comparison for i8_gemm_kernel: 148,992 rows (the biobank's GRM
+    markers) by 65,536 reduction indices (a slice of the sample axis; the rate does not fall
+    with longer reductions) by 80 to 512 columns, and two square shapes."""
+    import torch
+    shapes = [(8192, 8192, 8192), (16384, 16384, 16384)] + [
+        (148992, 65536, n) for n in (80, 112, 224, 288, 512)]
+    for m, k, n in shapes:
+        a = torch.randint(-3, 3, (m, k), dtype=torch.int8, device="cuda")
+        b = torch.randint(-100, 100, (n, k), dtype=torch.int8, device="cuda").t()
+        torch._int_mm(a, b)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 10
+        e0.record()
+        for _ in range(reps):
+            torch._int_mm(a, b)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tops = 2.0 * m * k * n / ms / 1e9
+        log(f"torch._int_mm m={m} k={k} n={n}: {ms:.3f} ms, {tops:.0f} TOP/s "
+            f"({tops / I8_PEAK_TOPS:.3f} of int8 peak)")
+        del a, b
 
 
 def run_product(args):

@@ -503,6 +503,12 @@ class ScoreTestEngine:
         self._mlp = mlp
         check(load().saige_engine_set_root_proposal(self.h, None if mlp is None else mlp.h))
 
+    def set_sums_form(self, form):
+        """Kernel of the per-variant sums for uint8 rows: -1 timed choice (default), 0 lanes
+        over samples, 1 a lane per variant, 2 int8 matrix cores (exact integer sums against
+        a fixed-point expansion of each feature)."""
+        check(load().saige_engine_set_sums_form(self.h, int(form)))
+
     def test_markers_u8_device(self, d_q, ldq, nvar, scale, d_out):
         check(load().saige_test_markers_u8_device(self.h, d_q, ldq, nvar, scale, d_out))
 

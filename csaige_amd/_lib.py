@@ -176,6 +176,7 @@ SIGNATURES = [
     ("saige_mlp_train", C.c_int, [vp, fp, fp, sz, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
                                   fp]),
     ("saige_engine_set_root_proposal", C.c_int, [vp, vp]),
+    ("saige_engine_set_sums_form", C.c_int, [vp, C.c_int]),
     ("saige_spa_binary", C.c_int, [C.c_int, dp, dp, sz, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_int, C.POINTER(SpaResult)]),
 ]

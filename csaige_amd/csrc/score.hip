@@ -509,6 +509,223 @@ __global__ __launch_bounds__(256) void score_sums_u8t_kernel(LoadU8 load,
   }
 }
 
+// uint8 dosages on the int8 matrix cores.  The linear sums (dosage times A(p), the
+// residuals and mu2 x(p); the allele count; dosage times the case flag) are exact integer
+// dot products: every linear feature f is written once per engine as NDIG signed base-128
+// digits of f 2^-E_f (|f| 2^-E_f < 1/2, digits in [-64, 64], the expansion exact to
+// 2^(E_f - 7 NDIG)); the dosage byte q enters as q - 128 (its top bit flipped) and
+// v_mfma_i32_16x16x64_i8 adds byte times digit over 64 samples into int32.  The +128 of
+// q = (q - 128) + 128 is 128 times the digit column sums, which a row of ones through the
+// same instruction gives.  The digit columns of a feature are then scaled by 2^(E_f - 7 l)
+// and added in double, so a sum differs from the exact sum of the products only by the
+// expansion's rounding at 2^(E_f - 70) per sample and the last additions in double.  The
+// quadratic sum (q^2 mu2) stays in double on the vector units, skipped for dosage words that
+// are zero in every lane.
+// Digit table `limbs`: [256-sample block][column][256 samples], zero past n; column 0 ones,
+// 1 case flag, 2 + NDIG f + l digit l + 1 of linear feature f (A(p), res, mu2 x(p)).
+// A wave owns RT tiles of 16 variants; lane (r, qd) loads bytes 64 qd .. 64 qd + 63 of each
+// 256-sample block of variant 16 t + r, 16 per instruction, and the digit table's lanes
+// (column r of a tile, the same samples) match them.  Per-split sums go to part as from the
+// other sums kernels.
+template <int P> struct SumsM {
+  static constexpr int NL = 2 * P + 1;      // linear features
+  static constexpr int NDIG = 10;           // base-128 digits per feature
+  static constexpr int NC = 2 + NDIG * NL;  // ones, case, digits
+  static constexpr int NT = (NC + 15) / 16;
+  static constexpr int NCP = 16 * NT;       // columns of the table
+};
+// Samples of one split at most: |(q - 128) d| <= 8192, so int32 holds 262,143 of them.
+constexpr size_t kSumsMaxRange = 262144;
+
+struct SumsMExp {
+  int e[2 * kMaxP + 1];  // E_f of each linear feature
+};
+
+typedef int i32x4s __attribute__((ext_vector_type(4)));
+
+template <int P, int RT>
+__global__ __launch_bounds__(256) void score_sums_mfma_kernel(
+    LoadU8 load, const int8_t* __restrict__ limbs, const double* __restrict__ mu2p, SumsMExp fe,
+    size_t npad, size_t nvar, size_t sps, double* __restrict__ part, size_t nvar_pad) {
+  constexpr int NT = SumsM<P>::NT, NL = SumsM<P>::NL, NC = SumsM<P>::NC, NCP = SumsM<P>::NCP;
+  constexpr int ND = SumsM<P>::NDIG, NS = Layout<P>::NS;
+  // The four waves share the sample range: each 256-sample block's table columns and mu2
+  // are staged once per work-group in LDS (double-buffered, the next block's loaded into
+  // registers during the current block's products).  Column stride 272 bytes: the 16
+  // columns of a tile read by one instruction fall in distinct banks.
+  constexpr int BSTR = 272, UNITS = NCP * 16, UPT = (UNITS + 255) / 256;
+  __shared__ __attribute__((aligned(16))) int8_t sb[2][NCP * BSTR];
+  __shared__ __attribute__((aligned(16))) double sm[2][256];
+  const int tid = threadIdx.x, lane = tid & 63, r = lane & 15, qd = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const size_t vb = ((size_t)blockIdx.x * 4 + wave) * (16 * RT);
+  const bool live = vb < nvar;  // wave-uniform; every wave takes part in the staging
+  const size_t i0 = (size_t)blockIdx.y * sps, i1 = min(i0 + sps, npad);
+  const uint8_t* rowp[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) rowp[t] = load.q + min(vb + 16 * t + r, nvar - 1) * load.ld;
+  i32x4s acc[RT][NT], on[NT];
+  double q2[RT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    on[nt] = i32x4s{0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[t][nt] = i32x4s{0, 0, 0, 0};
+  }
+#pragma unroll
+  for (int t = 0; t < RT; ++t) q2[t] = 0.0;
+  const i32x4s ones = {0x01010101, 0x01010101, 0x01010101, 0x01010101};
+  // staging loads of block i: table units (16 bytes) u = tid + 256 k, one mu2 value
+  static_assert(UNITS % 256 == 0, "whole staging rounds");
+  i32x4s su[UPT];
+  double smv = 0.0;
+  auto fetch_stage = [&](size_t i) {
+    const int8_t* src = limbs + (i / 256) * (size_t)(NCP * 256);
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) su[k] = *reinterpret_cast<const i32x4s*>(src + 16 * (tid + 256 * k));
+    smv = mu2p[i + tid];
+  };
+  auto store_stage = [&](int buf) {
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+      const int u = tid + 256 * k;
+      *reinterpret_cast<i32x4s*>(&sb[buf][(u >> 4) * BSTR + 16 * (u & 15)]) = su[k];
+    }
+    sm[buf][tid] = smv;
+  };
+  // dosage bytes of block i; a 16-byte unit past the row's ld is read from the row's start
+  // instead (its samples are past n, where the table and mu2 are zero)
+  auto fetch_rows = [&](size_t i, u32x4w (&dst)[RT][4]) {
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const size_t cs = i + 64 * qd + 16 * j;
+        dst[t][j] = *reinterpret_cast<const u32x4w*>(rowp[t] + (cs < load.ld ? cs : 0));
+      }
+  };
+  auto compute = [&](int buf, const u32x4w (&w)[RT][4]) {
+    const int8_t* lb = &sb[buf][r * BSTR + 64 * qd];
+    const double* mp = &sm[buf][64 * qd];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      i32x4s b[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        b[nt] = *reinterpret_cast<const i32x4s*>(lb + nt * 16 * BSTR + 16 * j);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        on[nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ones, b[nt], on[nt], 0, 0, 0);
+      uint32_t nz = 0u;
+#pragma unroll
+      for (int t = 0; t < RT; ++t) {
+        const u32x4w x = w[t][j];
+        nz |= x[0] | x[1] | x[2] | x[3];
+        const i32x4s a = {(int)(x[0] ^ 0x80808080u), (int)(x[1] ^ 0x80808080u),
+                          (int)(x[2] ^ 0x80808080u), (int)(x[3] ^ 0x80808080u)};
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          acc[t][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[t][nt], 0, 0, 0);
+      }
+      if (__ballot(nz != 0u) != 0) {  // wave-uniform: skipped with no dosage in the step
+        double m[16];
+#pragma unroll
+        for (int k = 0; k < 16; k += 2) {
+          const f64x2 v2 = *reinterpret_cast<const f64x2*>(mp + 16 * j + k);
+          m[k] = v2.x;
+          m[k + 1] = v2.y;
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+          for (int k = 0; k < 16; ++k) {
+            const uint32_t qb = (w[t][j][k >> 2] >> (8 * (k & 3))) & 0xffu;
+            q2[t] = fma((double)(qb * qb), m[k], q2[t]);
+          }
+      }
+    }
+  };
+  // Two blocks per trip, the row registers alternating; the next block's loads (the last
+  // block reloads itself) are in flight during the current block's products, and its stage
+  // is written to the other LDS buffer after them.
+  u32x4w wa[RT][4], wb[RT][4];
+  auto block = [&](size_t i, int buf, const u32x4w (&cur)[RT][4], u32x4w (&nxt)[RT][4]) {
+    __syncthreads();  // block i staged; the other buffer no longer read
+    const size_t in = i + 256 < i1 ? i + 256 : i;
+    fetch_stage(in);
+    fetch_rows(in, nxt);
+    if (live) compute(buf, cur);
+    store_stage(buf ^ 1);
+  };
+  if (i0 < i1) {
+    fetch_stage(i0);
+    fetch_rows(i0, wa);
+    store_stage(0);
+    for (size_t i = i0; i < i1; i += 512) {
+      block(i, 0, wa, wb);
+      if (i + 256 < i1) block(i + 256, 1, wb, wa);
+    }
+  }
+  if (!live) return;
+  const size_t split = blockIdx.y;
+  const double sc1 = load.scale, sc2 = load.scale * load.scale;
+  // q^2 mu2: the four lanes (r, qd) hold quarters of variant vb + 16 t + r
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    double v = q2[t];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    const size_t var = vb + 16 * t + lane;
+    if (lane < 16 && var < nvar) part[(split * NS + 3 + P) * nvar_pad + var] = v * sc2;
+  }
+  // C layout: lane (c, g) holds rows 4 g .. 4 g + 3 of column c of each tile; a column's
+  // scale 2^(E_f - 7 l), its feature, and the ones-row sum (equal in the four rows)
+  const int c = lane & 15, g = lane >> 4;
+  double csc[NT];
+  int cf[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const int col = 16 * nt + c;
+    cf[nt] = (col >= 2 && col < NC) ? (col - 2) / ND : (col < 2 ? -1 - col : -3);
+    csc[nt] = cf[nt] >= 0 ? ldexp(1.0, fe.e[cf[nt]] - 7 * ((col - 2) % ND + 1)) : 1.0;
+  }
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double val[NL], cnt = 0.0, cse = 0.0;
+#pragma unroll
+      for (int f = 0; f < NL; ++f) val[f] = 0.0;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const double x = (double)acc[t][nt][i] + 128.0 * (double)on[nt][0];
+        if (cf[nt] == -1) cnt = x;
+        if (cf[nt] == -2) cse = x;
+#pragma unroll
+        for (int f = 0; f < NL; ++f) val[f] += cf[nt] == f ? x * csc[nt] : 0.0;
+      }
+#pragma unroll
+      for (int m2 = 1; m2 < 16; m2 <<= 1) {
+        cnt += __shfl_xor(cnt, m2);
+        cse += __shfl_xor(cse, m2);
+#pragma unroll
+        for (int f = 0; f < NL; ++f) val[f] += __shfl_xor(val[f], m2);
+      }
+      const size_t var = vb + 16 * t + 4 * g + i;
+      if (c == 0 && var < nvar) {
+        double* o = part + split * NS * nvar_pad + var;
+        o[0] = cnt * sc1;
+        o[nvar_pad] = 0.0;
+#pragma unroll
+        for (int f = 0; f <= P; ++f) o[(2 + f) * nvar_pad] = val[f] * sc1;
+#pragma unroll
+        for (int j = 0; j < P; ++j) o[(4 + P + j) * nvar_pad] = val[P + 1 + j] * sc1;
+        o[(4 + 2 * P) * nvar_pad] = cse * sc1;
+        o[(5 + 2 * P) * nvar_pad] = 0.0;
+      }
+    }
+}
+
 // Without possible missing values the counts n_valid and n_cases are the engine's
 // constants and are filled in by the finalize kernel.
 template <int P, class L>
@@ -1571,6 +1788,9 @@ struct saige_engine {
   const saige_mlp* mlp = nullptr;
   DevBuf feat, cmask, mu_d, x_d, idx, part, res, gate, ngate, in;
   DevBuf odds_d;  // (1 - mu) / mu per sample, at most 1e12 (the Newton passes of the SPA)
+  DevBuf limbs, mu2p;  // digit table and zero-padded mu2 of score_sums_mfma_kernel
+  SumsMExp fexp{};
+  int sums_form = -1;  // uint8 sums kernel: -1 timed choice, else form 0, 1 or 2
   DevBuf spa_vars, spa_part, spa_active;  // batched SPA state
   // sums-kernel shape (variants per lane, sample splits) chosen by timing, per (loader,
   // covariates, variants)
@@ -1718,11 +1938,12 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const
 }
 
 
-// One launch of the sums kernel.  uint8 dosages with 16-byte rows take a byte kernel: form 0
-// has lanes over samples (up to 3 covariates), form 1 a lane per variant; the other loaders
-// take the generic kernel.
+// One launch of the sums kernel.  uint8 dosages with 16-byte rows take a byte kernel: form 2
+// the int8 matrix cores (score_sums_mfma_kernel), form 0 lanes over samples (up to 3
+// covariates), form 1 a lane per variant; the other loaders take the generic kernel.
 template <int P>
 constexpr bool has_u8t() { return P <= 3; }
+constexpr int kSumsMfmaRT = 2;  // 16-variant tiles per wave of score_sums_mfma_kernel
 
 template <int P, class L>
 static void launch_sums(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int form,
@@ -1731,6 +1952,14 @@ static void launch_sums(saige_engine* e, L load, size_t nvar, size_t nvar_pad, i
   const size_t n = e->n;
   if constexpr (std::is_same<L, LoadU8>::value) {
     if (load.vec) {
+      if (form == 2) {
+        constexpr int RT = kSumsMfmaRT;
+        score_sums_mfma_kernel<P, RT>
+            <<<dim3((unsigned)ceil_div(nvar, 64 * RT), (unsigned)splits), 256, 0, s>>>(
+                load, e->limbs.as<int8_t>(), e->mu2p.as<double>(), e->fexp, round_up(n, 256),
+                nvar, sps, part, nvar_pad);
+        return;
+      }
       const uint32_t* cm = e->cmask.as<uint32_t>();
       if constexpr (has_u8t<P>()) {
         if (form == 0) {
@@ -1751,11 +1980,12 @@ static void launch_sums(saige_engine* e, L load, size_t nvar, size_t nvar_pad, i
       load, e->feat.as<double>(), n, nvar, sps, part, nvar_pad);
 }
 
-// Sample-range length of a split: whole 256-sample blocks for form 0, 16-sample segments
-// otherwise.
+// Sample-range length of a split: whole 256-sample blocks for forms 0 and 2 (at most
+// kSumsMaxRange for form 2), 16-sample segments otherwise.
 static size_t split_len(size_t n, size_t splits, int form) {
-  const size_t g = form == 0 ? 256 : 16;
-  return round_up(ceil_div(n, std::max<size_t>(splits, 1)), g);
+  const size_t g = form == 1 ? 16 : 256;
+  const size_t len = round_up(ceil_div(n, std::max<size_t>(splits, 1)), g);
+  return form == 2 ? std::min(len, kSumsMaxRange) : len;
 }
 
 // Tile-shape autotuner of the sums kernel: the first batch of a shape times the kernel and
@@ -1769,12 +1999,21 @@ static int sums_shape(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int
   constexpr int NS = Layout<P>::NS;
   const size_t n = e->n;
   const bool u8 = std::is_same<L, LoadU8>::value && load.vec;
-  const int form0 = (u8 && has_u8t<P>()) ? 0 : 1;
+  const bool mf = u8 && e->limbs.p != nullptr;
+  const int form0 = mf ? 2 : (u8 && has_u8t<P>()) ? 0 : 1;
   auto norm = [&](size_t sp, int form) { return ceil_div(n, split_len(n, sp, form)); };
   const size_t vblocks = ceil_div(nvar, 64);
   const size_t dflt_sp = std::max<size_t>(1, std::min<size_t>(ceil_div(2048, vblocks),
                                                                ceil_div(n, 512)));
   const auto key = std::make_tuple(loader_id<L>() * 2 + (int)load.vec, P, nvar);
+  if (u8 && e->sums_form >= 0) {  // a fixed form at its default splits
+    const int f = e->sums_form;
+    size_t sp = norm(f != 1 ? 16 : dflt_sp, f);
+    if (max_splits) sp = std::min(sp, max_splits);
+    *form_out = f;
+    *splits_out = sp;
+    return SAIGE_OK;
+  }
   auto it = e->tuned_shape.find(key);
   if (it != e->tuned_shape.end() && (max_splits == 0 || it->second.second <= max_splits)) {
     *form_out = it->second.first;
@@ -1782,7 +2021,7 @@ static int sums_shape(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int
     return SAIGE_OK;
   }
   int best_form = form0;
-  size_t best = norm(form0 == 0 ? 16 : dflt_sp, form0);
+  size_t best = norm(form0 != 1 ? 16 : dflt_sp, form0);
   if (max_splits) {  // no search: the default shape within the split bound
     *form_out = best_form;
     *splits_out = std::min(best, max_splits);
@@ -1794,7 +2033,11 @@ static int sums_shape(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int
     hipEvent_t a, b;
     SAIGE_HIP(hipEventCreate(&a));
     SAIGE_HIP(hipEventCreate(&b));
-    for (int form = form0; form <= 1; ++form) {
+    std::vector<int> forms;
+    if (mf) forms.push_back(2);
+    if (u8 && has_u8t<P>()) forms.push_back(0);
+    forms.push_back(1);
+    for (int form : forms) {
       for (size_t cand : {size_t(16), size_t(32), size_t(64), size_t(128), size_t(256)}) {
         const size_t sp = norm(cand, form);
         const size_t sps = split_len(n, sp, form);
@@ -1839,6 +2082,7 @@ static int run_batch_p(saige_engine* e, L load, size_t nvar, saige_variant_resul
   SAIGE_TRY((sums_shape<P, L>(e, load, nvar, nvar_pad, &form, &splits,
                               fixed ? kFixedSplits : 0)));
   const size_t sps = split_len(n, splits, form);
+  SAIGE_ARG(splits * sps >= n, "sample splits cover the samples");
   const size_t part_bytes = (splits + 1) * NS * nvar_pad * 8;
   BatchArea ar;
   if (fixed) {
@@ -1957,6 +2201,49 @@ static int scratch_area(int device, BatchArea* ar) {
 
 }  // namespace saige
 
+// The digit table and zero-padded mu2 of score_sums_mfma_kernel: the linear features (A(p),
+// res, mu2 x(p)) as NDIG balanced base-128 digits of f 2^-E_f, E_f = ilogb(max |f|) + 2, so
+// that |f| 2^-E_f < 1/2 and every digit lies in [-64, 64]; each step takes the nearest
+// integer of 128 x and keeps the exact remainder.
+static int build_sums_tables(saige_engine* e, const std::vector<double>& feat, size_t nf) {
+  const size_t n = e->n, p = e->p, nl = 2 * p + 1;
+  constexpr int ND = 10;
+  static_assert(ND == SumsM<1>::NDIG, "digits per feature");
+  const size_t ncp = round_up(2 + ND * nl, 16), npad = round_up(n, 256);
+  auto fcol = [&](size_t f) { return f <= p ? f : f + 1; };  // skips mu2 (column p + 1)
+  std::vector<int8_t> tab(npad * ncp, 0);
+  std::vector<double> m2(npad, 0.0);
+  for (size_t f = 0; f < nl; ++f) {
+    double mx = 0.0;
+    for (size_t i = 0; i < n; ++i) mx = std::max(mx, std::fabs(feat[i * nf + fcol(f)]));
+    SAIGE_ARG(std::isfinite(mx), "finite engine inputs");
+    e->fexp.e[f] = mx > 0.0 ? std::ilogb(mx) + 2 : 0;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    int8_t* b = tab.data() + (i / 256) * ncp * 256 + (i % 256);
+    b[0] = 1;
+    b[256] = feat[i * nf + 2 * p + 2] != 0.0 ? 1 : 0;
+    for (size_t f = 0; f < nl; ++f) {
+      double x = std::ldexp(feat[i * nf + fcol(f)], -e->fexp.e[f]);
+      for (int l = 0; l < ND; ++l) {
+        const double y = x * 128.0;
+        const double d = std::nearbyint(y);
+        x = y - d;
+        b[(2 + ND * f + l) * 256] = (int8_t)d;
+      }
+    }
+    m2[i] = feat[i * nf + p + 1];
+  }
+  SAIGE_TRY(e->limbs.ensure(tab.size()));
+  SAIGE_TRY(e->mu2p.ensure(npad * 8));
+  if (hipMemcpy(e->limbs.p, tab.data(), tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->mu2p.p, m2.data(), npad * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("engine upload failed");
+    return SAIGE_ERR_DEVICE;
+  }
+  return SAIGE_OK;
+}
+
 extern "C" {
 
 int saige_engine_create(int device, const double* mu, const double* mu2, const double* residuals,
@@ -2049,6 +2336,7 @@ int saige_engine_create(int device, const double* mu, const double* mu2, const d
     set_error("engine upload failed");
     return fail(SAIGE_ERR_DEVICE);
   }
+  if ((rc = build_sums_tables(e, feat, nf)) != SAIGE_OK) return fail(rc);
   *out = e;
   return SAIGE_OK;
 }
@@ -2274,6 +2562,16 @@ int saige_engine_set_root_proposal(saige_engine* e, const saige_mlp* mlp) {
   SAIGE_ARG(e, "engine");
   std::lock_guard<std::mutex> lk(e->mu);
   e->mlp = mlp;
+  return SAIGE_OK;
+}
+
+int saige_engine_set_sums_form(saige_engine* e, int form) {
+  clear_error();
+  SAIGE_ARG(e, "engine");
+  SAIGE_ARG(form >= -1 && form <= 2, "form -1, 0, 1 or 2");
+  SAIGE_ARG(form != 0 || e->p <= 3, "form 0 takes at most 3 covariates");
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->sums_form = form;
   return SAIGE_OK;
 }
 

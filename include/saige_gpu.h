@@ -468,6 +468,13 @@ int saige_mlp_train(saige_mlp* h, const float* x, const float* y, size_t n, uint
  * the proposal changes the path, not the result.  The model must outlive its use by the
  * engine. */
 int saige_engine_set_root_proposal(saige_engine* e, const saige_mlp* mlp);
+/* Kernel of the per-variant sums for uint8 dosage rows (no counterpart in the reference: a
+ * tuning and test control).  -1 (the default): chosen by timing the first batch of each
+ * shape (at least 65,536 samples), else the int8 matrix-core form; 0: lanes over samples
+ * (at most 3 covariates); 1: a lane per variant; 2: int8 matrix cores (exact integer sums of
+ * the dosage bytes against a 70-bit fixed-point expansion of each feature).  Every form
+ * computes the same sums; forms 0 and 1 add the double products in their own orders. */
+int saige_engine_set_sums_form(saige_engine* e, int form);
 
 /* ------------------------------------------------------------------------------
  * SPA on one covariate-adjusted genotype vector.

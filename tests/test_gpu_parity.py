@@ -265,6 +265,47 @@ def test_score_u8_vs_oracle(p):
     _compare(got, want, integer_dosages=False)
 
 
+@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("p", [1, 3])
+def test_score_u8_each_sums_form_vs_oracle(p, form):
+    """Each kernel of the uint8 sums (0 lanes over samples, 1 a lane per variant, 2 the int8
+    matrix cores on a fixed-point expansion of the features) against the oracle, on a sample
+    count that leaves a partial 256-sample block, hard calls and noisy dosages."""
+    n, nvar = 5000 + 77, 700
+    X, mu, y, a = _null(n, p, seed=60 + p)
+    q, _ = synth.dosage_u8(n, 0, nvar, seed=21 + p, maf=(1e-3, 0.5))
+    rng = np.random.default_rng(p)
+    for v in range(0, nvar, 9):
+        q[v, (y > 0.5) & (rng.random(n) < 0.3)] = 254
+    e = sg.ScoreTestEngine(mu, y - mu, (1.0, 0.3), X.T, a, 1.05, y=y, spa_pval_cutoff=0.05)
+    e.set_sums_form(form)
+    got = e.test_markers_u8(q, 1.0 / 127.0)
+    eo = oracle.ScoreEngine(mu, y - mu, (1.0, 0.3), X.T, a, 1.05, y=y, spa_cutoff=0.05)
+    want = eo.test_batch_u8(q, 1.0 / 127.0)
+    _say(f"p={p} form {form}: spa'd {int(want['is_spa'].sum())} of {nvar}")
+    assert want["is_spa"].sum() > 10
+    _compare(got, want, integer_dosages=False)
+
+
+def test_score_u8_timed_shape_matches_fixed_forms():
+    """At 70,000 samples the first batch times every form and split count (each computes the
+    full sums) and keeps the fastest; its results equal the fixed forms' to rounding, and the
+    allele counts of the int8 matrix-core form are the exact byte sums (scaled per split)."""
+    n, nvar = 70000, 512
+    X, mu, y, a = _null(n, 1, seed=77)
+    q, _ = synth.dosage_u8(n, 0, nvar, seed=78, maf=(1e-4, 0.5))
+    res = {}
+    for form in (-1, 0, 2):
+        e = sg.ScoreTestEngine(mu, y - mu, (1.0, 0.3), X.T, a, 1.0, y=y, spa_pval_cutoff=0.05)
+        e.set_sums_form(form)
+        res[form] = e.test_markers_u8(q, 1.0 / 127.0)
+        res[form] = e.test_markers_u8(q, 1.0 / 127.0)  # the batch after the shape search
+    ac = q.astype(np.int64).sum(axis=1) / 127.0
+    np.testing.assert_allclose(res[2]["ac"], ac, rtol=1e-14, atol=0)
+    for form in (-1, 0):
+        _compare(res[form], res[2], integer_dosages=False)
+
+
 @pytest.mark.parametrize("p", [1, 2])
 def test_score_u8_every_variant_spa_vs_oracle(p):
     """SPA on every variant (cutoff 1): a batch of 2,500 gated variants, more than the 1,024
