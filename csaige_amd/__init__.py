@@ -509,6 +509,14 @@ class ScoreTestEngine:
         a fixed-point expansion of each feature)."""
         check(load().saige_engine_set_sums_form(self.h, int(form)))
 
+    def set_spa_split(self, enable):
+        """Carrier-only SPA passes for uint8 variants with few carriers (intercept-only
+        engines; True by default).  Returns whether the engine has the zero-byte table they
+        need."""
+        has = C.c_int(0)
+        check(load().saige_engine_set_spa_split(self.h, 1 if enable else 0, C.byref(has)))
+        return bool(has.value)
+
     def test_markers_u8_device(self, d_q, ldq, nvar, scale, d_out):
         check(load().saige_test_markers_u8_device(self.h, d_q, ldq, nvar, scale, d_out))
 

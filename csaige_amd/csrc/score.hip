@@ -1173,22 +1173,113 @@ struct SpaVar {
   double tinit[2];
   int done;          // both searches finished
   int bmin, bmax;    // smallest and largest dosage byte of the variant (byte-table passes)
-  int pad;
+  int split;         // byte-table passes over the carriers only (SpaZeroTab)
 };
 constexpr int kSpaPassBlock = 256;
-constexpr int kSpaPart = 10;                  // partials per (variant, slice): 8 sums, byte range
-constexpr int kSpaMaxPasses = 8 * 1000 + 16;  // the guard of spa_kernel's pass loop
+constexpr int kSpaPart = 11;  // partials per (variant, slice): 8 sums, byte range, carriers
 
-template <int NT, int K>
-__device__ __forceinline__ void block_sum_t(double* v, double* sh) {
-  for (int k = 0; k < K; ++k) sh[k * NT + threadIdx.x] = v[k];
+// Sums over every sample of an intercept-only model, as functions of s, for the byte-table
+// passes of variants with few carriers: with x_i = rho_i e^s (rho_i = (1 - mu_i) / mu_i),
+//   A(s) = sum 1 / (1 + x_i),  B(s) = sum x_i / (1 + x_i)^2,  D(s) = sum ln(1 - mu_i + mu_i e^-s).
+// The samples with dosage byte 0 share g~_0, so their K'(t), K''(t) and K(zeta) terms are
+// g~_0 A, g~_0^2 B and D at s = -g~_0 t, less the same terms of the carriers (bytes > 0),
+// which the passes compute sample by sample with their own terms.  A, B and D / s are
+// analytic in |Im s| < pi (their singularities sit at s = -ln rho_i +- i pi; D(0) = 0); on
+// [s_lo, s_hi] each unit interval holds Chebyshev series of kZtN terms fitted at its
+// Chebyshev points to the exact sums (series error below 1e-18 relative for a Bernstein
+// ellipse of semi-minor axis 2, so the fit carries the rounding of the node sums).  D is
+// kept as D / s, so that near s = 0, where D ~ -s sum mu and K(zeta) - zeta q is small,
+// its error scales with s.  Outside, where every x_i is below e^-40 or above e^40, the first
+// terms of the expansions in x_i (or 1 / x_i) are exact to double precision.
+constexpr int kZtN = 21;
+constexpr double kZtMargin = 40.0;
+struct SpaZeroTab {
+  const double* coef;  // [piece][function A, B, D][kZtN]
+  double s_lo, s_hi;
+  int npiece, ok;
+  double n, rho_sum, rho_inv_sum, lnmu_sum, ln1mmu_sum;
+};
+__device__ __forceinline__ double zt_eval(const SpaZeroTab& T, double s, int f) {
+  if (s < T.s_lo) {
+    const double x = exp(s);
+    if (f == 0) return T.n - x * T.rho_sum;
+    if (f == 1) return x * T.rho_sum;
+    return fma(x, T.rho_sum, T.lnmu_sum - T.n * s);
+  }
+  if (!(s < T.s_hi)) {
+    const double x = exp(-s);
+    return f == 2 ? fma(x, T.rho_inv_sum, T.ln1mmu_sum) : x * T.rho_inv_sum;
+  }
+  const double u = s - T.s_lo;
+  int p = (int)u;
+  p = p >= T.npiece ? T.npiece - 1 : p;
+  const double y = 2.0 * (u - (double)p) - 1.0;
+  const double* c = T.coef + ((size_t)p * 3 + f) * kZtN;
+  double b1 = 0.0, b2 = 0.0;
+  for (int j = kZtN - 1; j >= 1; --j) {
+    const double b0 = fma(2.0 * y, b1, c[j] - b2);
+    b2 = b1;
+    b1 = b0;
+  }
+  const double v = fma(y, b1, 0.5 * c[0] - b2);
+  return f == 2 ? v * s : v;
+}
+// Exact A, B and D / s at the Chebyshev points of the table (one work-group per point;
+// ln(1 - mu + mu e^-s) as log1p(mu expm1(-s)), accurate near s = 0).
+__global__ __launch_bounds__(256) void spa_zero_nodes_kernel(const double* __restrict__ odds,
+                                                             const double* __restrict__ mu,
+                                                             size_t n,
+                                                             const double* __restrict__ nodes,
+                                                             double* __restrict__ out) {
+  const double s = nodes[blockIdx.x], es = exp(s), em1 = expm1(-s);
+  double v[3] = {0.0, 0.0, 0.0};
+  for (size_t i = threadIdx.x; i < n; i += 256) {
+    const double x = odds[i] * es;
+    const double u = 1.0 / (1.0 + x);
+    v[0] += u;
+    v[1] = fma(x * u, u, v[1]);
+    v[2] += log1p(mu[i] * em1);
+  }
+  __shared__ double sh[3][256];
+  for (int f = 0; f < 3; ++f) sh[f][threadIdx.x] = v[f];
   __syncthreads();
-  for (int w = NT / 2; w > 0; w >>= 1) {
+  for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
-      for (int k = 0; k < K; ++k) sh[k * NT + threadIdx.x] += sh[k * NT + threadIdx.x + w];
+      for (int f = 0; f < 3; ++f) sh[f][threadIdx.x] += sh[f][threadIdx.x + w];
     __syncthreads();
   }
-  for (int k = 0; k < K; ++k) v[k] = sh[k * NT];
+  if (threadIdx.x < 3)
+    out[3 * (size_t)blockIdx.x + threadIdx.x] = sh[threadIdx.x][0] / (threadIdx.x == 2 ? s : 1.0);
+}
+// Carriers queued per wave in the split passes (half a step queues at most 128 behind at
+// most 63).
+constexpr uint32_t kCarrierQ = 256;
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+constexpr int kSpaMaxPasses = 8 * 1000 + 16;  // the guard of spa_kernel's pass loop
+
+// Sum of v[0..K) over the NT threads of the block, returned in every thread: a butterfly
+// within each wave, then the waves' totals in wave order (a fixed order).  sh holds K * NT
+// / 64 doubles.
+template <int NT, int K>
+__device__ __forceinline__ void block_sum_t(double* v, double* sh) {
+  constexpr int NWV = NT / 64;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v[k] += __shfl_xor(v[k], m);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < K; ++k) sh[k * NWV + wv] = v[k];
+  __syncthreads();
+  for (int k = 0; k < K; ++k) {
+    double t = sh[k * NWV];
+    for (int w = 1; w < NWV; ++w) t += sh[k * NWV + w];
+    v[k] = t;
+  }
   __syncthreads();
 }
 
@@ -1240,7 +1331,7 @@ struct LogProd {
 // of a full batch).  A variant-slice's partials sit at its position in the pass; the
 // variant's sum over its slices is taken in slice order, so a variant's result depends only
 // on the list's length, never on its place in the list.
-constexpr uint32_t kSpaPassTarget = 1024;
+constexpr uint32_t kSpaPassTarget = 4096;
 struct SpaList {
   uint32_t* cnt;   // [2] variants in the list of pass parity 0 / 1
   uint32_t* list;  // [2][ng] gate slots
@@ -1310,7 +1401,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
   if (!wk.live) return;
   const uint32_t slot = wk.slot;
   if (KIND == 2 && !(vars[slot].st[0].converged && vars[slot].st[1].converged)) return;
-  __shared__ double sh[K * kSpaPassBlock];
+  __shared__ double sh[K * kSpaPassBlock / 64];
   __shared__ GateInfo g;
   if (threadIdx.x == 0) g = gate[slot];
   __syncthreads();
@@ -1391,6 +1482,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_kernel(
     if (KIND == 0) {
       o[8] = 0.0;
       o[9] = 255.0;
+      o[10] = 0.0;
     }
   }
 }
@@ -1404,15 +1496,15 @@ template <int KIND>
 __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
     LoadU8 ld8, const double* __restrict__ mu, size_t n, const GateInfo* __restrict__ gate,
     const uint32_t* __restrict__ n_gate, int fast, size_t slice,
-    const SpaVar* __restrict__ vars, double* __restrict__ part, SpaList sl) {
+    const SpaVar* __restrict__ vars, double* __restrict__ part, SpaList sl, SpaZeroTab zt) {
   constexpr int K = KIND == 0 ? 8 : 4;
-  constexpr int TW = KIND == 0 ? 2 : (KIND == 1 ? 4 : 8);  // doubles per byte entry
+  constexpr int TW = KIND == 0 ? 2 : (KIND == 1 ? 3 : 6);  // doubles per byte entry
   if (KIND == 1 && blockIdx.x == 0 && threadIdx.x == 0) sl.cnt[(sl.pass + 1) & 1] = 0u;
   const SpaWork wk = spa_work<KIND>(n_gate, n, slice, sl);
   if (!wk.live) return;
   const uint32_t slot = wk.slot;
   if (KIND == 2 && !(vars[slot].st[0].converged && vars[slot].st[1].converged)) return;
-  __shared__ double sh[K * kSpaPassBlock];
+  __shared__ double sh[K * kSpaPassBlock / 64];
   __shared__ __attribute__((aligned(16))) double tab[256 * TW];
   __shared__ GateInfo g;
   if (threadIdx.x == 0) g = gate[slot];
@@ -1433,7 +1525,6 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
       e[0] = incl ? gb : 0.0;
       e[1] = incl ? exp(-gb * t0) : 1.0;
       e[2] = incl ? exp(-gb * t1) : 1.0;
-      e[3] = 0.0;
       if (KIND == 1) {
         // an overflowing e^{-g t} makes that sample's K' term 0 and its K'' term NaN
         // (m g / inf, inf / inf): the capped value (with odds at most 1e12, 1 + odds e stays
@@ -1446,7 +1537,6 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
         e[3] = incl ? exp(gb * t0) : 1.0;
         e[4] = incl ? exp(gb * t1) : 1.0;
         e[5] = incl ? 1.0 : 0.0;
-        e[6] = e[7] = 0.0;
       }
     }
   }
@@ -1455,7 +1545,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
 #pragma unroll
   for (int k = 0; k < K; ++k) a[k] = 0.0;
   LogProd lk0, lk1;  // KIND 2: K at the roots as the logs of running products
-  uint32_t bmin = 255u, bmax = 0u;
+  uint32_t bmin = 255u, bmax = 0u, ncar = 0u;
   // act: bit r set while root r still searches (KIND 1 skips the finished root's terms)
   auto body = [&](uint32_t by, double m, auto act) {
     constexpr int ACT = decltype(act)::value;
@@ -1477,6 +1567,7 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
         a[7] = fma(m, m2, a[7]);
       }
       bmin = min(bmin, by);
+      ncar += by != 0u ? 1u : 0u;
       bmax = max(bmax, by);
     } else if (KIND == 1) {
       // here m holds the odds rho = (1 - mu) / mu: with u = mu / ((1 - mu) e + mu) =
@@ -1552,22 +1643,184 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
   };
   const bool act0 = KIND != 1 || vars[slot].st[0].phase != 3;
   const bool act1 = KIND != 1 || vars[slot].st[1].phase != 3;
-  if (act0 && act1) sweep(std::integral_constant<int, 3>{});
-  else if (act0) sweep(std::integral_constant<int, 1>{});
-  else sweep(std::integral_constant<int, 2>{});
-  if (KIND == 2) {
-    a[0] = lk0.log_value();
-    a[2] = lk1.log_value();
+  bool split = false;
+  if constexpr (KIND != 0) split = vars[slot].split != 0;  // uniform over the work-group
+  if (!split) {
+    if (act0 && act1) sweep(std::integral_constant<int, 3>{});
+    else if (act0) sweep(std::integral_constant<int, 1>{});
+    else sweep(std::integral_constant<int, 2>{});
+    if (KIND == 2) {
+      a[0] = lk0.log_value();
+      a[2] = lk1.log_value();
+    }
+  } else if constexpr (KIND != 0) {
+    // Carriers only (dosage byte > 0): each wave queues its carriers' bytes and mu (odds)
+    // in LDS and takes them 64 at a time, one per lane, adding the carrier's own terms and
+    // subtracting the terms it would have with byte 0's g~_0; byte 0's terms over all
+    // samples come from the table (SpaZeroTab), once per variant.
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const double* ez = tab;  // byte 0's entry
+    const bool zin = KIND == 1 ? ez[0] != 0.0 : ez[5] != 0.0;
+    double c[4] = {0.0, 0.0, 0.0, 0.0};
+    LogProd lz0, lz1;
+    auto carrier = [&](uint32_t by, double m, auto act) {
+      constexpr int ACT = decltype(act)::value;
+      body(by, m, act);
+      if (KIND == 2) {
+        lk0.norm();
+        lk1.norm();
+      }
+      if (!zin) return;
+      const double gz = ez[0];
+      if constexpr (KIND == 1) {
+        if constexpr ((ACT & 1) != 0) {
+          const double z0 = gz * recip_nr_finite(fma(m, ez[1], 1.0));
+          c[0] += z0;
+          c[1] = fma(z0, gz - z0, c[1]);
+        }
+        if constexpr ((ACT & 2) != 0) {
+          const double z1 = gz * recip_nr_finite(fma(m, ez[2], 1.0));
+          c[2] += z1;
+          c[3] = fma(z1, gz - z1, c[3]);
+        }
+      } else {
+        const double om = 1.0 - m, mg = m * gz, w2 = om * mg * gz;
+        const double r0 = recip_nr(fma(om, ez[1], m)), r1 = recip_nr(fma(om, ez[2], m));
+        lz0.mul(fma(m, ez[3], om));
+        lz1.mul(fma(m, ez[4], om));
+        lz0.norm();
+        lz1.norm();
+        c[1] = fma(w2 * ez[1], r0 * r0, c[1]);
+        c[3] = fma(w2 * ez[2], r1 * r1, c[3]);
+      }
+    };
+    // carrier queue of each wave: the sample's mu (odds in the Newton passes) and byte
+    __shared__ double cqm[kSpaPassBlock / 64][kCarrierQ];
+    __shared__ uint8_t cqb[kSpaPassBlock / 64][kCarrierQ];
+    auto split_sweep = [&](auto act) {
+      uint32_t head = 0u, tail = 0u;  // wave-uniform
+      auto drain = [&](uint32_t cnt) {
+        wave_sync();
+        if ((uint32_t)ln < cnt) {
+          const uint32_t e = (head + (uint32_t)ln) & (kCarrierQ - 1);
+          carrier(cqb[wv][e], cqm[wv][e], act);
+        }
+        head += cnt;
+        wave_sync();
+      };
+      auto enqueue = [&](uint32_t w, const double* m4) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          uint32_t t = tail;
+#pragma unroll
+          for (int k = 2 * h; k < 2 * h + 2; ++k) {
+            const uint32_t by = (w >> (8 * k)) & 0xffu;
+            const uint64_t mk = __ballot(by != 0u);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+            if (by != 0u) {
+              const uint32_t e = (t + below) & (kCarrierQ - 1);
+              cqm[wv][e] = m4[k];
+              cqb[wv][e] = (uint8_t)by;
+            }
+            t += (uint32_t)__builtin_popcountll(mk);
+          }
+          tail = t;
+          while (tail - head >= 64u) drain(64u);
+        }
+      };
+      // wave wv takes samples base .. base + 255 of every 1,024 (4 per lane); the next
+      // step's bytes and mu are in flight while a step is queued and its carriers taken
+      size_t base = i0 + 256 * (size_t)wv;
+      if (ld8.vec) {
+        const size_t i4 = i0 + ((i1 - i0) & ~size_t(3));
+        auto ld = [&](size_t i, uint32_t& w, f64x2& ma, f64x2& mb) {
+          if (i < i4) {
+            w = *reinterpret_cast<const uint32_t*>(row + i);
+            ma = *reinterpret_cast<const f64x2*>(mu + i);
+            mb = *reinterpret_cast<const f64x2*>(mu + i + 2);
+          } else {
+            w = 0u;
+          }
+        };
+        uint32_t wn = 0u;
+        f64x2 ma = {0.0, 0.0}, mb = {0.0, 0.0};
+        if (base < i1) ld(base + 4 * (size_t)ln, wn, ma, mb);
+        for (; base < i1; base += kStep) {
+          const uint32_t w = wn;
+          const double m4[4] = {ma.x, ma.y, mb.x, mb.y};
+          if (base + kStep < i1) ld(base + kStep + 4 * (size_t)ln, wn, ma, mb);
+          enqueue(w, m4);
+        }
+        // samples past the last whole group of four, one per lane
+        for (size_t j = i4 + (size_t)threadIdx.x; j < i1; j += kSpaPassBlock)
+          if (row[j] != 0u) carrier(row[j], mu[j], act);
+      } else {
+        for (; base < i1; base += kStep) {
+          const size_t i = base + 4 * (size_t)ln;
+          uint32_t w = 0u;
+          double m4[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int k = 0; k < 4; ++k)
+            if (i + k < i1) {
+              w |= (uint32_t)row[i + k] << (8 * k);
+              m4[k] = mu[i + k];
+            }
+          enqueue(w, m4);
+        }
+      }
+      drain(tail - head);
+    };
+    if (act0 && act1) split_sweep(std::integral_constant<int, 3>{});
+    else if (act0) split_sweep(std::integral_constant<int, 1>{});
+    else split_sweep(std::integral_constant<int, 2>{});
+    const bool lead = wk.first && threadIdx.x == 0;
+    const double gz = ez[0];
+    if constexpr (KIND == 1) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a[k] -= c[k];
+      if (lead && zin) {
+        if (act0) {
+          const double s0 = -gz * vars[slot].st[0].pend;
+          a[0] = fma(gz, zt_eval(zt, s0, 0), a[0]);
+          a[1] = fma(gz * gz, zt_eval(zt, s0, 1), a[1]);
+        }
+        if (act1) {
+          const double s1 = -gz * vars[slot].st[1].pend;
+          a[2] = fma(gz, zt_eval(zt, s1, 0), a[2]);
+          a[3] = fma(gz * gz, zt_eval(zt, s1, 1), a[3]);
+        }
+      }
+    } else {
+      a[0] = lk0.log_value() - lz0.log_value();
+      a[2] = lk1.log_value() - lz1.log_value();
+      a[1] -= c[1];
+      a[3] -= c[3];
+      if (zin) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const double zeta = vars[slot].st[r].root;
+          // an infinite factor 1 - mu + mu e^{g zeta} of a byte-0 sample makes K infinite,
+          // and an infinite e^{-g zeta} its K'' NaN (inf * 0), as in the per-sample sums
+          if (isinf(exp(gz * zeta))) a[2 * r] = lead ? __longlong_as_double(0x7ff0000000000000ll) : 0.0;
+          else if (lead) a[2 * r] += zt_eval(zt, -gz * zeta, 2);
+          if (lead)
+            a[2 * r + 1] += isinf(exp(-gz * zeta)) ? __longlong_as_double(0x7ff8000000000000ll)
+                                                    : gz * gz * zt_eval(zt, -gz * zeta, 1);
+        }
+      }
+    }
   }
-  __shared__ uint32_t brange[2];
+  __shared__ uint32_t brange[3];
   if (KIND == 0) {
     if (threadIdx.x == 0) {
       brange[0] = 255u;
       brange[1] = 0u;
+      brange[2] = 0u;
     }
     __syncthreads();
     atomicMin(&brange[0], bmin);
     atomicMax(&brange[1], bmax);
+    atomicAdd(&brange[2], ncar);
   }
   block_sum_t<kSpaPassBlock, K>(a, sh);
   if (threadIdx.x == 0) {
@@ -1590,15 +1843,9 @@ __global__ __launch_bounds__(kSpaPassBlock) void spa_pass_tab_kernel(
     if (KIND == 0) {
       o[8] = (double)brange[0];
       o[9] = (double)brange[1];
+      o[10] = (double)brange[2];
     }
   }
-}
-
-__device__ __forceinline__ double slice_sum(const double* __restrict__ part, uint32_t slot, int S,
-                                            int k) {
-  double s = 0.0;
-  for (int q = 0; q < S; ++q) s += part[((size_t)slot * S + q) * kSpaPart + k];
-  return s;
 }
 
 // learned starting points of both searches (one 128-thread group per gated variant)
@@ -1621,31 +1868,51 @@ __global__ __launch_bounds__(128) void spa_propose_kernel(const GateInfo* __rest
 
 // Totals of the setup pass: q = S / sqrt(VR) + m1, qinv = 2 m1 - q (single_variant.rs:
 // 189-196), the support bounds, the NA moments and the carrier-moment features.
-__global__ void spa_setup_step(GateInfo* __restrict__ gate, const uint32_t* __restrict__ n_gate,
-                               int S, const double* __restrict__ part, SpaVar* __restrict__ vars,
-                               saige_variant_result* __restrict__ out) {
-  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= *n_gate) return;
-  SpaVar V = vars[slot];
-  V.gpos = slice_sum(part, slot, S, 0);
-  V.gneg = slice_sum(part, slot, S, 1);
-  V.na_mu = slice_sum(part, slot, S, 2);
-  V.na_sigma = slice_sum(part, slot, S, 3);
-  V.tinit[0] = V.tinit[1] = 0.0;  // fast.rs:217-218
-  double bmn = 255.0, bmx = 0.0;
-  for (int q = 0; q < S; ++q) {
-    bmn = fmin(bmn, part[((size_t)slot * S + q) * kSpaPart + 8]);
-    bmx = fmax(bmx, part[((size_t)slot * S + q) * kSpaPart + 9]);
+// Variants whose carriers are at most kSplitCarriers of the samples take the carrier-only
+// byte-table passes (when the engine has the zero-byte table).
+constexpr double kSplitCarriers = 0.35;
+// One wave per variant: lane q holds slice q's partials (S <= 64), added in a fixed
+// butterfly order.
+__global__ __launch_bounds__(256) void spa_setup_step(GateInfo* __restrict__ gate,
+                                                      const uint32_t* __restrict__ n_gate, int S,
+                                                      const double* __restrict__ part,
+                                                      SpaVar* __restrict__ vars,
+                                                      saige_variant_result* __restrict__ out,
+                                                      size_t n, int split_ok) {
+  const uint32_t slot = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (slot >= *n_gate) return;  // uniform over the wave
+  double t[kSpaPart];
+  const bool have = lane < S;
+  const double* pp = part + ((size_t)slot * S + (have ? lane : 0)) * kSpaPart;
+#pragma unroll
+  for (int k = 0; k < kSpaPart; ++k) t[k] = have ? pp[k] : 0.0;
+  double bmn = have ? t[8] : 255.0, bmx = have ? t[9] : 0.0;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+    for (int k = 0; k < kSpaPart; ++k)
+      if (k != 8 && k != 9) t[k] += __shfl_xor(t[k], m);
+    bmn = fmin(bmn, __shfl_xor(bmn, m));
+    bmx = fmax(bmx, __shfl_xor(bmx, m));
   }
+  if (lane != 0) return;
+  SpaVar V = vars[slot];
+  V.split = (split_ok && t[10] <= kSplitCarriers * (double)n) ? 1 : 0;
+  V.gpos = t[0];
+  V.gneg = t[1];
+  V.na_mu = t[2];
+  V.na_sigma = t[3];
+  V.tinit[0] = V.tinit[1] = 0.0;  // fast.rs:217-218
   V.bmin = (int)bmn;
   V.bmax = (int)bmx;
   vars[slot] = V;
-  const double m1 = slice_sum(part, slot, S, 4);
+  const double m1 = t[4];
   GateInfo& g = gate[slot];
   g.q = g.q + m1;
   g.qinv = 2.0 * m1 - g.q;
   for (int j = 0; j < 3; ++j) {
-    const float f = (float)log(slice_sum(part, slot, S, 5 + j) + 1e-8);
+    const float f = (float)log(t[5 + j] + 1e-8);
     g.feat[3 + j] = (double)f;
     if (out) out[g.var].spa_features[3 + j] = f;
   }
@@ -1721,12 +1988,27 @@ __global__ __launch_bounds__(256) void spa_eval_step(const GateInfo* __restrict_
 }
 
 // saddle-point probabilities (fast.rs:135-180 and :220-241) and the outputs
-__global__ void spa_final_step(const GateInfo* __restrict__ gate, const uint32_t* __restrict__ n_gate,
-                               int S, const double* __restrict__ part,
-                               const SpaVar* __restrict__ vars,
-                               saige_variant_result* __restrict__ out) {
-  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= *n_gate) return;
+// One wave per variant: the slices' K and K'' sums added by a butterfly (S <= 64), then lane
+// 0 finishes the variant.
+__global__ __launch_bounds__(256) void spa_final_step(const GateInfo* __restrict__ gate,
+                                                      const uint32_t* __restrict__ n_gate, int S,
+                                                      const double* __restrict__ part,
+                                                      const SpaVar* __restrict__ vars,
+                                                      saige_variant_result* __restrict__ out) {
+  const uint32_t slot = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (slot >= *n_gate) return;  // uniform over the wave
+  double ks[4] = {0.0, 0.0, 0.0, 0.0};
+  if (lane < S) {
+    const double* pp = part + ((size_t)slot * S + lane) * kSpaPart;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ks[k] = pp[k];
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ks[k] += __shfl_xor(ks[k], m);
+  if (lane != 0) return;
   const SpaVar V = vars[slot];
   const GateInfo g = gate[slot];
   const double q[2] = {g.q, g.qinv};
@@ -1736,8 +2018,8 @@ __global__ void spa_final_step(const GateInfo* __restrict__ gate, const uint32_t
   if (both) {
     for (int r = 0; r < 2; ++r) {
       const double zeta = V.st[r].root;
-      const double k0 = slice_sum(part, slot, S, 2 * r) + V.na_mu * zeta + 0.5 * V.na_sigma * zeta * zeta;
-      const double k2 = slice_sum(part, slot, S, 2 * r + 1) + V.na_sigma;
+      const double k0 = ks[2 * r] + V.na_mu * zeta + 0.5 * V.na_sigma * zeta * zeta;
+      const double k2 = ks[2 * r + 1] + V.na_sigma;
       const double temp1 = zeta * q[r] - k0;
       if (isfinite(k0) && isfinite(k2) && !(temp1 < 0.0) && !(k2 < 0.0)) {
         const double sg = zeta > 0.0 ? 1.0 : (zeta < 0.0 ? -1.0 : (signbit(zeta) ? -1.0 : 1.0));
@@ -1791,6 +2073,9 @@ struct saige_engine {
   DevBuf limbs, mu2p;  // digit table and zero-padded mu2 of score_sums_mfma_kernel
   SumsMExp fexp{};
   int sums_form = -1;  // uint8 sums kernel: -1 timed choice, else form 0, 1 or 2
+  DevBuf zt_coef;      // Chebyshev coefficients of the zero-byte table
+  SpaZeroTab zt{};     // intercept-only models (zt.ok): carrier-only byte-table SPA passes
+  int spa_split = 1;   // use them for variants with few carriers
   DevBuf spa_vars, spa_part, spa_active;  // batched SPA state
   // sums-kernel shape (variants per lane, sample splits) chosen by timing, per (loader,
   // covariates, variants)
@@ -1855,8 +2140,10 @@ __global__ void bgen_decode_kernel(LoadBgen load, size_t n, size_t nvar,
   out[v * n + i] = load(v, i);
 }
 
-// SPA of a batch's gated variants with S work-groups per variant and pass (about 1,024
-// work-groups per pass, slices of at least 2,048 samples); the host reads the number of
+// SPA of a batch's gated variants with S work-groups per variant and pass (about
+// kSpaPassTarget work-groups per pass, slices of at least 2,048 samples: several times the
+// work-groups resident at once, so that slices of unequal cost, such as the carrier-only
+// slices beside full ones, spread evenly over the CUs); the host reads the number of
 // variants still searching after every fourth pass.
 template <class G, bool TAB>
 static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const BatchArea& ar) {
@@ -1866,7 +2153,7 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const
   SAIGE_HIP(hipMemcpyAsync(&ng, ar.ngate, 4, hipMemcpyDeviceToHost, s));
   SAIGE_HIP(hipStreamSynchronize(s));
   if (ng == 0) return SAIGE_OK;
-  size_t S = std::max<size_t>(1, std::min<size_t>(ceil_div(1024, ng), ceil_div(n, 2048)));
+  size_t S = std::max<size_t>(1, std::min<size_t>(ceil_div(kSpaPassTarget, ng), ceil_div(n, 2048)));
   S = std::min<size_t>(S, 64);
   const size_t slice = round_up(ceil_div(n, S), 256);
   S = ceil_div(n, slice);
@@ -1906,7 +2193,7 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const
     const dim3 gr = KIND == 1 ? grid1 : grid;
     if constexpr (TAB)  // the Newton passes take the odds (1 - mu) / mu in place of mu
       spa_pass_tab_kernel<KIND><<<gr, kSpaPassBlock, 0, s>>>(
-          gt.load, KIND == 1 ? odds : mu, n, gate, ngp, fast, slice, vars, part, sl);
+          gt.load, KIND == 1 ? odds : mu, n, gate, ngp, fast, slice, vars, part, sl, e->zt);
     else
       spa_pass_kernel<G, KIND><<<gr, kSpaPassBlock, 0, s>>>(gt, mu, n, gate, ngp, fast, slice,
                                                             vars, part, sl);
@@ -1914,7 +2201,9 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const
   zero_u32_kernel<<<1, 1, 0, s>>>(active);
   zero_u32_kernel<<<1, 1, 0, s>>>(active + 1);
   launch_pass(std::integral_constant<int, 0>{});
-  spa_setup_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, vars, d_out);
+  const int split_ok = (TAB && e->zt.ok && e->spa_split) ? 1 : 0;
+  spa_setup_step<<<(unsigned)ceil_div((size_t)ng * 64, 256), 256, 0, s>>>(gate, ngp, iS, part,
+                                                                         vars, d_out, n, split_ok);
   if (mv.enabled) spa_propose_kernel<<<ng, 128, 0, s>>>(gate, ngp, mv, vars);
   spa_start_step<<<sb, 256, 0, s>>>(gate, ngp, vars, sl);
   // A pass with no variant left costs a few microseconds, a read of the count one host
@@ -1932,7 +2221,8 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const
     }
   }
   launch_pass(std::integral_constant<int, 2>{});
-  spa_final_step<<<sb, 256, 0, s>>>(gate, ngp, iS, part, vars, d_out);
+  spa_final_step<<<(unsigned)ceil_div((size_t)ng * 64, 256), 256, 0, s>>>(gate, ngp, iS, part,
+                                                                         vars, d_out);
   SAIGE_HIP(hipGetLastError());
   return SAIGE_OK;
 }
@@ -2153,7 +2443,7 @@ constexpr size_t kScratchGate = cround(kScratchVariants * sizeof(GateInfo), 256)
 constexpr size_t kScratchVars = cround(kScratchVariants * sizeof(SpaVar), 256);
 // the SPA partials: ng * S <= 1024 + ng variant-slices (S <= ceil(1024 / ng)) in the setup
 // and final passes, at most max(kSpaPassTarget, ng) in a Newton pass
-constexpr size_t kScratchSpaPart = cround((1024 + kScratchVariants) * kSpaPart * 8, 256);
+constexpr size_t kScratchSpaPart = cround((kSpaPassTarget + kScratchVariants) * kSpaPart * 8, 256);
 constexpr size_t kScratchSpaActive = cround(256 + 2 * kScratchVariants * 4, 256);
 constexpr size_t kScratchBytes = cround(kScratchPart, 256) + kScratchGate + 256 + kScratchVars +
                                  kScratchSpaPart + kScratchSpaActive;
@@ -2241,6 +2531,65 @@ static int build_sums_tables(saige_engine* e, const std::vector<double>& feat, s
     set_error("engine upload failed");
     return SAIGE_ERR_DEVICE;
   }
+  return SAIGE_OK;
+}
+
+// The zero-byte table (SpaZeroTab) of an intercept-only engine: the exact sums at the
+// Chebyshev points of unit intervals covering s in [-ln max rho - 40, -ln min rho + 40], the
+// series coefficients by the discrete cosine transform of those values.  Left unbuilt (the
+// passes then take every sample) where some mu is within 1e-12 of 0 or 1, where the range
+// exceeds 1,024 intervals, or for 2^24 samples or more (the carrier queue's index width).
+static int build_spa_zero_table(saige_engine* e, const double* mu) {
+  const size_t n = e->n;
+  e->zt = SpaZeroTab{};
+  if (!e->intercept_only || n >= (size_t(1) << 24)) return SAIGE_OK;
+  double rmin = HUGE_VAL, rmax = 0.0;
+  SpaZeroTab T{};
+  T.n = (double)n;
+  for (size_t i = 0; i < n; ++i) {
+    if (!(mu[i] > 1e-12 && mu[i] < 1.0 - 1e-12)) return SAIGE_OK;
+    const double r = (1.0 - mu[i]) / mu[i];
+    rmin = std::min(rmin, r);
+    rmax = std::max(rmax, r);
+    T.rho_sum += r;
+    T.rho_inv_sum += 1.0 / r;
+    T.lnmu_sum += std::log(mu[i]);
+    T.ln1mmu_sum += std::log1p(-mu[i]);
+  }
+  T.s_lo = std::floor(-std::log(rmax) - kZtMargin);
+  const double hi = -std::log(rmin) + kZtMargin;
+  T.npiece = (int)std::ceil(hi - T.s_lo);
+  if (T.npiece < 1 || T.npiece > 1024 || T.s_lo < -700.0 || hi > 700.0) return SAIGE_OK;
+  T.s_hi = T.s_lo + T.npiece;
+  const size_t nn = (size_t)T.npiece * kZtN;
+  std::vector<double> nodes(nn), vals(3 * nn);
+  const double pi = 3.14159265358979323846;
+  for (int p = 0; p < T.npiece; ++p)
+    for (int k = 0; k < kZtN; ++k)
+      nodes[(size_t)p * kZtN + k] = T.s_lo + p + 0.5 + 0.5 * std::cos(pi * (k + 0.5) / kZtN);
+  DevBuf dn, dv;
+  SAIGE_TRY(dn.ensure(nn * 8));
+  SAIGE_TRY(dv.ensure(3 * nn * 8));
+  SAIGE_HIP(hipMemcpy(dn.p, nodes.data(), nn * 8, hipMemcpyHostToDevice));
+  spa_zero_nodes_kernel<<<(unsigned)nn, 256, 0, e->stream>>>(
+      e->odds_d.as<double>(), e->mu_d.as<double>(), n, dn.as<double>(), dv.as<double>());
+  SAIGE_HIP(hipGetLastError());
+  SAIGE_HIP(hipMemcpyAsync(vals.data(), dv.p, 3 * nn * 8, hipMemcpyDeviceToHost, e->stream));
+  SAIGE_HIP(hipStreamSynchronize(e->stream));
+  std::vector<double> coef((size_t)T.npiece * 3 * kZtN);
+  for (int p = 0; p < T.npiece; ++p)
+    for (int f = 0; f < 3; ++f)
+      for (int j = 0; j < kZtN; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < kZtN; ++k)
+          acc += vals[3 * ((size_t)p * kZtN + k) + f] * std::cos(pi * j * (k + 0.5) / kZtN);
+        coef[((size_t)p * 3 + f) * kZtN + j] = 2.0 * acc / kZtN;
+      }
+  SAIGE_TRY(e->zt_coef.ensure(coef.size() * 8));
+  SAIGE_HIP(hipMemcpy(e->zt_coef.p, coef.data(), coef.size() * 8, hipMemcpyHostToDevice));
+  T.coef = e->zt_coef.as<double>();
+  T.ok = 1;
+  e->zt = T;
   return SAIGE_OK;
 }
 
@@ -2337,6 +2686,7 @@ int saige_engine_create(int device, const double* mu, const double* mu2, const d
     return fail(SAIGE_ERR_DEVICE);
   }
   if ((rc = build_sums_tables(e, feat, nf)) != SAIGE_OK) return fail(rc);
+  if ((rc = build_spa_zero_table(e, mu)) != SAIGE_OK) return fail(rc);
   *out = e;
   return SAIGE_OK;
 }
@@ -2562,6 +2912,15 @@ int saige_engine_set_root_proposal(saige_engine* e, const saige_mlp* mlp) {
   SAIGE_ARG(e, "engine");
   std::lock_guard<std::mutex> lk(e->mu);
   e->mlp = mlp;
+  return SAIGE_OK;
+}
+
+int saige_engine_set_spa_split(saige_engine* e, int enable, int* has_table) {
+  clear_error();
+  SAIGE_ARG(e, "engine");
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->spa_split = enable ? 1 : 0;
+  if (has_table) *has_table = e->zt.ok;
   return SAIGE_OK;
 }
 

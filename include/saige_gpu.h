@@ -475,6 +475,14 @@ int saige_engine_set_root_proposal(saige_engine* e, const saige_mlp* mlp);
  * the dosage bytes against a 70-bit fixed-point expansion of each feature).  Every form
  * computes the same sums; forms 0 and 1 add the double products in their own orders. */
 int saige_engine_set_sums_form(saige_engine* e, int form);
+/* Carrier-only SPA passes (no counterpart in the reference: a tuning and test control).  An
+ * intercept-only engine holds a table of three sums over all samples as functions of one
+ * variable (Chebyshev series fitted to the exact sums, accurate to their rounding); with it,
+ * the K', K'' and K sums of a uint8 variant whose carriers (dosage > 0) are at most 35% of
+ * the samples are taken over the carriers only, the samples of dosage 0 entering through
+ * the table.  enable = 1 (the default) uses it, 0 sums over every sample; *has_table (may be
+ * NULL) tells whether the engine has the table. */
+int saige_engine_set_spa_split(saige_engine* e, int enable, int* has_table);
 
 /* ------------------------------------------------------------------------------
  * SPA on one covariate-adjusted genotype vector.
