@@ -523,10 +523,10 @@ __global__ __launch_bounds__(256) void score_sums_u8t_kernel(LoadU8 load,
 // are zero in every lane.
 // Digit table `limbs`: [256-sample block][column][256 samples], zero past n; column 0 ones,
 // 1 case flag, 2 + NDIG f + l digit l + 1 of linear feature f (A(p), res, mu2 x(p)).
-// A wave owns RT tiles of 16 variants; lane (r, qd) loads bytes 64 qd .. 64 qd + 63 of each
-// 256-sample block of variant 16 t + r, 16 per instruction, and the digit table's lanes
-// (column r of a tile, the same samples) match them.  Per-split sums go to part as from the
-// other sums kernels.
+// A wave owns RT tiles of 16 variants; in step j of a 256-sample block lane (r, qd) holds
+// bytes 64 j + 16 qd .. + 15 of variant 16 t + r (each load instruction reads 64 contiguous
+// bytes of 16 rows), and the digit table's lanes (column r of a tile, the same samples)
+// match them.  Per-split sums go to part as from the other sums kernels.
 template <int P> struct SumsM {
   static constexpr int NL = 2 * P + 1;      // linear features
   static constexpr int NDIG = 10;           // base-128 digits per feature
@@ -543,22 +543,22 @@ struct SumsMExp {
 
 typedef int i32x4s __attribute__((ext_vector_type(4)));
 
-template <int P, int RT>
-__global__ __launch_bounds__(256) void score_sums_mfma_kernel(
+template <int P, int RT, int NW>
+__global__ __launch_bounds__(64 * NW) void score_sums_mfma_kernel(
     LoadU8 load, const int8_t* __restrict__ limbs, const double* __restrict__ mu2p, SumsMExp fe,
     size_t npad, size_t nvar, size_t sps, double* __restrict__ part, size_t nvar_pad) {
   constexpr int NT = SumsM<P>::NT, NL = SumsM<P>::NL, NC = SumsM<P>::NC, NCP = SumsM<P>::NCP;
   constexpr int ND = SumsM<P>::NDIG, NS = Layout<P>::NS;
-  // The four waves share the sample range: each 256-sample block's table columns and mu2
+  // The NW waves share the sample range: each 256-sample block's table columns and mu2
   // are staged once per work-group in LDS (double-buffered, the next block's loaded into
   // registers during the current block's products).  Column stride 272 bytes: the 16
   // columns of a tile read by one instruction fall in distinct banks.
-  constexpr int BSTR = 272, UNITS = NCP * 16, UPT = (UNITS + 255) / 256;
+  constexpr int BSTR = 272, UNITS = NCP * 16, UPT = (UNITS + 64 * NW - 1) / (64 * NW);
   __shared__ __attribute__((aligned(16))) int8_t sb[2][NCP * BSTR];
   __shared__ __attribute__((aligned(16))) double sm[2][256];
   const int tid = threadIdx.x, lane = tid & 63, r = lane & 15, qd = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const size_t vb = ((size_t)blockIdx.x * 4 + wave) * (16 * RT);
+  const size_t vb = ((size_t)blockIdx.x * NW + wave) * (16 * RT);
   const bool live = vb < nvar;  // wave-uniform; every wave takes part in the staging
   const size_t i0 = (size_t)blockIdx.y * sps, i1 = min(i0 + sps, npad);
   const uint8_t* rowp[RT];
@@ -576,22 +576,26 @@ __global__ __launch_bounds__(256) void score_sums_mfma_kernel(
   for (int t = 0; t < RT; ++t) q2[t] = 0.0;
   const i32x4s ones = {0x01010101, 0x01010101, 0x01010101, 0x01010101};
   // staging loads of block i: table units (16 bytes) u = tid + 256 k, one mu2 value
-  static_assert(UNITS % 256 == 0, "whole staging rounds");
+  constexpr int NTH = 64 * NW;
+  static_assert(NTH >= 256, "a thread per staged mu2");
   i32x4s su[UPT];
   double smv = 0.0;
   auto fetch_stage = [&](size_t i) {
     const int8_t* src = limbs + (i / 256) * (size_t)(NCP * 256);
 #pragma unroll
-    for (int k = 0; k < UPT; ++k) su[k] = *reinterpret_cast<const i32x4s*>(src + 16 * (tid + 256 * k));
-    smv = mu2p[i + tid];
+    for (int k = 0; k < UPT; ++k)
+      if (UNITS % NTH == 0 || tid + NTH * k < UNITS)
+        su[k] = *reinterpret_cast<const i32x4s*>(src + 16 * (tid + NTH * k));
+    if (NTH == 256 || tid < 256) smv = mu2p[i + tid];
   };
   auto store_stage = [&](int buf) {
 #pragma unroll
     for (int k = 0; k < UPT; ++k) {
-      const int u = tid + 256 * k;
-      *reinterpret_cast<i32x4s*>(&sb[buf][(u >> 4) * BSTR + 16 * (u & 15)]) = su[k];
+      const int u = tid + NTH * k;
+      if (UNITS % NTH == 0 || u < UNITS)
+        *reinterpret_cast<i32x4s*>(&sb[buf][(u >> 4) * BSTR + 16 * (u & 15)]) = su[k];
     }
-    sm[buf][tid] = smv;
+    if (NTH == 256 || tid < 256) sm[buf][tid] = smv;
   };
   // dosage bytes of block i; a 16-byte unit past the row's ld is read from the row's start
   // instead (its samples are past n, where the table and mu2 are zero)
@@ -600,19 +604,19 @@ __global__ __launch_bounds__(256) void score_sums_mfma_kernel(
     for (int t = 0; t < RT; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const size_t cs = i + 64 * qd + 16 * j;
+        const size_t cs = i + 64 * j + 16 * qd;
         dst[t][j] = *reinterpret_cast<const u32x4w*>(rowp[t] + (cs < load.ld ? cs : 0));
       }
   };
   auto compute = [&](int buf, const u32x4w (&w)[RT][4]) {
-    const int8_t* lb = &sb[buf][r * BSTR + 64 * qd];
-    const double* mp = &sm[buf][64 * qd];
+    const int8_t* lb = &sb[buf][r * BSTR + 16 * qd];
+    const double* mp = &sm[buf][16 * qd];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       i32x4s b[NT];
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
-        b[nt] = *reinterpret_cast<const i32x4s*>(lb + nt * 16 * BSTR + 16 * j);
+        b[nt] = *reinterpret_cast<const i32x4s*>(lb + nt * 16 * BSTR + 64 * j);
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
         on[nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ones, b[nt], on[nt], 0, 0, 0);
@@ -631,7 +635,7 @@ __global__ __launch_bounds__(256) void score_sums_mfma_kernel(
         double m[16];
 #pragma unroll
         for (int k = 0; k < 16; k += 2) {
-          const f64x2 v2 = *reinterpret_cast<const f64x2*>(mp + 16 * j + k);
+          const f64x2 v2 = *reinterpret_cast<const f64x2*>(mp + 64 * j + k);
           m[k] = v2.x;
           m[k + 1] = v2.y;
         }
@@ -2234,6 +2238,7 @@ static int spa_batched(saige_engine* e, G gt, saige_variant_result* d_out, const
 template <int P>
 constexpr bool has_u8t() { return P <= 3; }
 constexpr int kSumsMfmaRT = 2;  // 16-variant tiles per wave of score_sums_mfma_kernel
+constexpr int kSumsMfmaNW = 8;  // its waves per work-group (sharing the staged table)
 
 template <int P, class L>
 static void launch_sums(saige_engine* e, L load, size_t nvar, size_t nvar_pad, int form,
@@ -2243,9 +2248,9 @@ static void launch_sums(saige_engine* e, L load, size_t nvar, size_t nvar_pad, i
   if constexpr (std::is_same<L, LoadU8>::value) {
     if (load.vec) {
       if (form == 2) {
-        constexpr int RT = kSumsMfmaRT;
-        score_sums_mfma_kernel<P, RT>
-            <<<dim3((unsigned)ceil_div(nvar, 64 * RT), (unsigned)splits), 256, 0, s>>>(
+        constexpr int RT = kSumsMfmaRT, NW = kSumsMfmaNW;
+        score_sums_mfma_kernel<P, RT, NW>
+            <<<dim3((unsigned)ceil_div(nvar, 16 * RT * NW), (unsigned)splits), 64 * NW, 0, s>>>(
                 load, e->limbs.as<int8_t>(), e->mu2p.as<double>(), e->fexp, round_up(n, 256),
                 nvar, sps, part, nvar_pad);
         return;
