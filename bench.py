@@ -446,6 +446,8 @@ def main():
                     help="only copy 1 GiB device to device 3 times (counter calibration)")
     ap.add_argument("--library-int8", action="store_true",
                     help="only time the vendor int8 GEMM (torch._int_mm) at the product's shapes")
+    ap.add_argument("--read-bandwidth", action="store_true",
+                    help="only time plain streaming reads and copies of device memory")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "families", "biobank", "learned", "loco", "region",
                              "product"],
@@ -483,6 +485,9 @@ def main():
         hip.device_synchronize()
         log("copied 3 x 1 GiB device to device")
         return
+    if args.read_bandwidth:
+        read_bandwidth_calibration()
+        return
     if args.library_int8:
         library_int8_calibration()
         return
@@ -512,6 +517,33 @@ def main():
         run_product(args)
         return
     run_headline(args)
+
+
+def read_bandwidth_calibration():
+    """What a plain streaming kernel reaches on this GPU: torch's sum over 15 GiB of int64
+    (a read of the size of the 400,000 x 150,000 packed genotypes) and a 1 GiB device copy,
+    as the practical ceiling beside the 8 TB/s peak."""
+    import torch
+    x = torch.ones(15 * 2**30 // 8, dtype=torch.int64, device="cuda")
+    y = torch.ones(2**30 // 8, dtype=torch.int64, device="cuda")
+    z = torch.empty_like(y)
+
+    def timed(f, reps):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms = timed(lambda: x.sum(), 5)
+    log(f"read 15 GiB (torch sum): {ms:.3f} ms, {15 * 2**30 / ms / 1e6:.0f} GB/s "
+        f"({15 * 2**30 / ms / 1e6 / HBM_PEAK_GBS:.3f} of {HBM_PEAK_GBS:.0f})")
+    ms = timed(lambda: z.copy_(y), 20)
+    log(f"copy 1 GiB: {ms:.3f} ms, {2 * 2**30 / ms / 1e6:.0f} GB/s read + write")
 
 
 def library_int8_calibration():

@@ -125,17 +125,24 @@ __global__ __launch_bounds__(1024) void pcg_init_fin(const double* __restrict__ 
   }
 }
 
-// partial p'Ap for active columns
-__global__ __launch_bounds__(kDotBlock) void pcg_pap_kernel(const double* __restrict__ p,
-                                                            const double* __restrict__ ap,
-                                                            size_t ld, size_t n, size_t nchunks,
-                                                            const ColState* __restrict__ st,
-                                                            double* __restrict__ part) {
+// Sigma p = tau0 p / max(w, 1e-30) + tau1 GRM p written over ap (ap holds GRM p on entry;
+// the closure of ai_reml.rs:142-152) and the chunk's partial p'(Sigma p), for the active
+// columns: one pass over p and ap.
+__global__ __launch_bounds__(kDotBlock) void pcg_sigma_pap_kernel(
+    const double* __restrict__ p, double* __restrict__ ap, size_t ld,
+    const double* __restrict__ w, const double* __restrict__ tau, size_t n, size_t nchunks,
+    const ColState* __restrict__ st, double* __restrict__ part) {
   const size_t c = blockIdx.y;
   if (!st[c].active) return;
+  const double t0 = tau[0], t1 = tau[1];
   const size_t i0 = (size_t)blockIdx.x * kDotChunk, i1 = min(i0 + (size_t)kDotChunk, n);
   double s = 0.0, d1 = 0.0, d2 = 0.0;
-  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) s = fma(p[i + c * ld], ap[i + c * ld], s);
+  for (size_t i = i0 + threadIdx.x; i < i1; i += kDotBlock) {
+    const double pi = p[i + c * ld];
+    const double v = t0 * pi / fmax(w[i], 1e-30) + t1 * ap[i + c * ld];
+    ap[i + c * ld] = v;
+    s = fma(pi, v, s);
+  }
   block_sum3(s, d1, d2);
   if (threadIdx.x == 0) part[c * nchunks + blockIdx.x] = s;
 }
@@ -227,17 +234,6 @@ __global__ __launch_bounds__(1024) void pcg_beta_fin(const double* __restrict__ 
     n_active[0] = cnt;
     *iter_d = iter + 1;  // every thread has read it before the barrier above
   }
-}
-
-// Sigma p = tau0 p / max(w, 1e-30) + tau1 GRM p for the active columns (ap holds GRM p on
-// entry); the same expression as the closure of ai_reml.rs:142-152.
-__global__ void pcg_sigma_kernel(const double* __restrict__ p, double* __restrict__ ap, size_t ld,
-                                 const double* __restrict__ w, const double* __restrict__ tau,
-                                 size_t n, const ColState* __restrict__ st) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t c = blockIdx.y;
-  if (i >= n || !st[c].active) return;
-  ap[i + c * ld] = tau[0] * p[i + c * ld] / fmax(w[i], 1e-30) + tau[1] * ap[i + c * ld];
 }
 
 __global__ void pcg_set_tau_kernel(double* __restrict__ tau, double t0, double t1) {
@@ -347,8 +343,8 @@ int pcg_sigma_device(saige_grm* h, ProductSink* sink, PcgWork& ws, const double*
       pcg_gather_kernel<<<gg, 256, 0, s>>>(APC, ld, n, d_nact + 1, 1, AP);
     }
     const dim3 gn((unsigned)ceil_div(n, 256), (unsigned)k);
-    pcg_sigma_kernel<<<gn, 256, 0, s>>>(P, AP, ld, d_w, d_tau, n, st);
-    pcg_pap_kernel<<<gchunk, kDotBlock, 0, s>>>(P, AP, ld, n, nchunks, st, pap_part);
+    pcg_sigma_pap_kernel<<<gchunk, kDotBlock, 0, s>>>(P, AP, ld, d_w, d_tau, n, nchunks, st,
+                                                      pap_part);
     pcg_update_kernel<<<gchunk, kDotBlock, 0, s>>>(d_x, ldx, R, Z, P, AP, ld, d_w, d_tau, n,
                                                    nchunks, pap_part, d_iter, st, part);
     pcg_beta_fin<<<1, 1024, 0, s>>>(part, nchunks, k, d_iter, st, d_nact);
