@@ -534,8 +534,9 @@ template <int P> struct SumsM {
   static constexpr int NT = (NC + 15) / 16;
   static constexpr int NCP = 16 * NT;       // columns of the table
 };
-// Samples of one split at most: |(q - 128) d| <= 8192, so int32 holds 262,143 of them.
-constexpr size_t kSumsMaxRange = 262144;
+// Samples of one split at most (a multiple of 256): |(q - 128) d| <= 128 * 64 = 8192, and
+// 8192 * 261,888 < 2^31.
+constexpr size_t kSumsMaxRange = 261888;
 
 struct SumsMExp {
   int e[2 * kMaxP + 1];  // E_f of each linear feature
@@ -2542,12 +2543,12 @@ static int build_sums_tables(saige_engine* e, const std::vector<double>& feat, s
 // The zero-byte table (SpaZeroTab) of an intercept-only engine: the exact sums at the
 // Chebyshev points of unit intervals covering s in [-ln max rho - 40, -ln min rho + 40], the
 // series coefficients by the discrete cosine transform of those values.  Left unbuilt (the
-// passes then take every sample) where some mu is within 1e-12 of 0 or 1, where the range
-// exceeds 1,024 intervals, or for 2^24 samples or more (the carrier queue's index width).
+// passes then take every sample) where some mu is within 1e-12 of 0 or 1, or where the
+// range exceeds 1,024 intervals.
 static int build_spa_zero_table(saige_engine* e, const double* mu) {
   const size_t n = e->n;
   e->zt = SpaZeroTab{};
-  if (!e->intercept_only || n >= (size_t(1) << 24)) return SAIGE_OK;
+  if (!e->intercept_only) return SAIGE_OK;
   double rmin = HUGE_VAL, rmax = 0.0;
   SpaZeroTab T{};
   T.n = (double)n;
