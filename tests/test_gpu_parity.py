@@ -224,16 +224,17 @@ def _null(n, p, seed, eta=-2.5):
     return X, mu, y, a
 
 
-def _compare(got, want, mask_p=1e-10, integer_dosages=True):
+def _compare(got, want, mask_p=1e-10, integer_dosages=True, ac_rtol=1e-12):
     # counts are always exact; allele sums are exact for hard calls and sums of
-    # non-integer dosages agree to rounding (a different summation order)
+    # non-integer dosages agree to rounding (a different summation order; the oracle adds the
+    # n rounded q/127 in sequence, an error that grows with n)
     for f in ("n", "n_cases", "n_controls"):
         np.testing.assert_array_equal(got[f], want[f], err_msg=f)
     for f in ("ac", "af", "mac", "af_cases", "af_controls"):
         if integer_dosages:
             np.testing.assert_array_equal(got[f], want[f], err_msg=f)
         else:
-            np.testing.assert_allclose(got[f], want[f], rtol=1e-12, atol=1e-12, err_msg=f)
+            np.testing.assert_allclose(got[f], want[f], rtol=ac_rtol, atol=1e-12, err_msg=f)
     for f in ("score", "var_t", "var_t_star", "beta"):
         np.testing.assert_allclose(got[f], want[f], rtol=1e-8, atol=1e-12, err_msg=f)
     np.testing.assert_array_equal(got["is_spa"], want["is_spa"])
@@ -304,6 +305,32 @@ def test_score_u8_timed_shape_matches_fixed_forms():
     np.testing.assert_allclose(res[2]["ac"], ac, rtol=1e-14, atol=0)
     for form in (-1, 0):
         _compare(res[form], res[2], integer_dosages=False)
+
+
+def test_step2_full_sample_count_vs_oracle():
+    """Step 2 at the streamed workload's own size: 400,000 samples, an intercept-only null of
+    about 1% cases, uint8 dosages from the same generator and seeds as the benchmark, SPA
+    where |z| > 2.  The first batch times the sums kernels, the second is scored with the
+    kept shape (sample splits of a full-size batch, the zero-byte table, carrier-only SPA
+    passes); both against the oracle."""
+    n, nvar = 400000, 512
+    nf = synth.null_fit(n=n, p=1, eta_mean=-4.6, eta_sd=0.5, seed=3)
+    a = sg.xvx_inv_xv(nf.x, nf.mu * (1 - nf.mu))
+    q, _ = synth.dosage_u8(n, 0, nvar, seed=4)
+    cutoff = 0.04550026389635842  # 2 (1 - Phi(2))
+    e = sg.ScoreTestEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
+                           spa_pval_cutoff=cutoff)
+    assert e.set_spa_split(True)
+    first = e.test_markers_u8(q, 1.0 / 127.0)
+    got = e.test_markers_u8(q, 1.0 / 127.0)
+    eo = oracle.ScoreEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
+                            spa_cutoff=cutoff)
+    want = eo.test_batch_u8(q, 1.0 / 127.0)
+    _say(f"n=400,000: {int(want['is_spa'].sum())} of {nvar} SPA'd")
+    assert want["is_spa"].sum() >= 5
+    # sequential sums of 400,000 rounded terms in the oracle: up to n 2^-53 = 4.4e-11 apart
+    for r in (first, got):
+        _compare(r, want, integer_dosages=False, ac_rtol=5e-11)
 
 
 # With the score near the null mean the saddle point sits near 0 and the Lugannani-Rice form
