@@ -457,6 +457,10 @@ def main():
                          "proposal; loco: leave-one-chromosome-out refits; region: region tests")
     ap.add_argument("--step2-variants", type=int, default=1000000,
                     help="headline: step-2 variants streamed (generated on the device)")
+    ap.add_argument("--step2-engines", type=int, default=2,
+                    help="headline: score-test engines (streams and host threads) of step 2")
+    ap.add_argument("--step2-resident", type=int, default=32,
+                    help="headline: step-2 batches generated and resident per timed chunk")
     ap.add_argument("--cpu-fit-markers", type=int, default=500,
                     help="headline: markers of the CPU baseline's sampled f32 GRM product")
     ap.add_argument("--prod-k", type=int, default=34,
@@ -898,60 +902,79 @@ MEASURED_TRAFFIC_BIOBANK = {(400000, 150000, "int8x2", 224): 1.78e10,
 def step2_stream(args, sg, hip, synth):
     """Step 2 over all of the workload's variants: n = 400,000, intercept-only null with
     mu = logistic(N(-4.6, 0.5)), VR 1.05, SPA where |score/sd| > 2, 4,096-variant uint8
-    batches generated on the device (the seeded generator's device twin) and scored as they
-    come.  Returns the step2 dict and the CPU baseline of step 2."""
+    batches generated on the device (the seeded generator's device twin), in chunks of
+    resident batches.  Each chunk is scored by a pool of engines on their own streams (the
+    timed figure, host wall clock between device synchronisations) and by one engine alone
+    (timed beside it; its results must equal the pool's bit for bit).  Returns the step2 dict
+    and the CPU baseline of step 2."""
     n2, V, B = args.step2_n, args.step2_variants, 4096
     nf = synth.null_fit(n=n2, p=1, eta_mean=-4.6, eta_sd=0.5, seed=3)
     a = sg.xvx_inv_xv(nf.x, nf.mu * (1 - nf.mu))
     cutoff = 0.04550026389635842  # 2 * (1 - Phi(2)): SPA where |score/sd| > 2
-    eng = sg.ScoreTestEngine(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
-                             spa_pval_cutoff=cutoff)
+    pool = sg.ScoreTestPool(nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05, y=nf.y,
+                            spa_pval_cutoff=cutoff, engines=args.step2_engines)
+    one = pool.engines[0]
     mafs = synth.dosage_mafs(0, V, seed=4)
     d_m = hip.DeviceBuffer(mafs.nbytes)
     d_m.upload(mafs)
-    d_q = [hip.DeviceBuffer(B * n2), hip.DeviceBuffer(B * n2)]
-    rsize = C.sizeof(sg._lib.VariantResult)
-    d_r = hip.DeviceBuffer(V * rsize)
-    d_w = hip.DeviceBuffer(B * rsize)
-    es = eng.stream
     nb = -(-V // B)
+    R = max(1, min(args.step2_resident, nb))
+    d_q = [hip.DeviceBuffer(B * n2) for _ in range(R)]
+    rsize = C.sizeof(sg._lib.VariantResult)
+    d_r = [hip.DeviceBuffer(V * rsize), hip.DeviceBuffer(V * rsize)]
+    d_w = hip.DeviceBuffer(B * rsize)
+    es = one.stream
     sizes = sorted({min(B, V - b * B) for b in range(nb)})
     for nv in sizes:  # first use of each batch size tunes the sums kernel's shape: untimed
         synth.dosage_u8_device(n2, 0, nv, d_m.value, d_q[0].value, seed=4, stream=es)
-        eng.test_markers_u8_device(d_q[0].value, n2, nv, 1.0 / 127, d_w.value)
+        pool.test_markers_u8_device([(d_q[0].value, n2, nv, 1.0 / 127, d_w.value)])
     hip.device_synchronize()
-    log(f"step2: {V} variants at n={n2} in {nb} batches of {B}, generated on the device")
-    ev = [(hip.Event(), hip.Event(), hip.Event()) for _ in range(nb)]
-    t = time.perf_counter()
-    for b in range(nb):
-        first = b * B
-        nv = min(B, V - first)
-        g0, g1, s1 = ev[b]
+    log(f"step2: {V} variants at n={n2} in {nb} batches of {B}, generated on the device in "
+        f"chunks of {R} resident batches; {len(pool.engines)} engines")
+    g0, g1 = hip.Event(), hip.Event()
+    gen_ms = score_s = one_s = 0.0
+    for c0 in range(0, nb, R):
+        bl = [(b, b * B, min(B, V - b * B)) for b in range(c0, min(nb, c0 + R))]
         g0.record(es)
-        synth.dosage_u8_device(n2, first, nv, d_m.value + 8 * first, d_q[b % 2].value, seed=4,
-                               stream=es)
+        for b, first, nv in bl:
+            synth.dosage_u8_device(n2, first, nv, d_m.value + 8 * first, d_q[b - c0].value,
+                                   seed=4, stream=es)
         g1.record(es)
-        eng.test_markers_u8_device(d_q[b % 2].value, n2, nv, 1.0 / 127,
-                                   d_r.value + first * rsize)
-        s1.record(es)
-    hip.device_synchronize()
-    wall = time.perf_counter() - t
-    gen_ms = sum(g0.elapsed_ms(g1) for g0, g1, _ in ev)
-    score_ms = sum(g1.elapsed_ms(s1) for _, g1, s1 in ev)
-    host = np.empty(V * rsize, np.uint8)
-    d_r.download(host)
-    out = (sg._lib.VariantResult * V).from_buffer(host)
+        hip.device_synchronize()
+        gen_ms += g0.elapsed_ms(g1)
+        t = time.perf_counter()
+        pool.test_markers_u8_device([(d_q[b - c0].value, n2, nv, 1.0 / 127,
+                                      d_r[0].value + first * rsize) for b, first, nv in bl])
+        hip.device_synchronize()
+        score_s += time.perf_counter() - t
+        t = time.perf_counter()
+        for b, first, nv in bl:
+            one.test_markers_u8_device(d_q[b - c0].value, n2, nv, 1.0 / 127,
+                                       d_r[1].value + first * rsize)
+        hip.device_synchronize()
+        one_s += time.perf_counter() - t
+    score_ms = score_s * 1e3
+    host = [np.empty(V * rsize, np.uint8), np.empty(V * rsize, np.uint8)]
+    d_r[0].download(host[0])
+    d_r[1].download(host[1])
+    same = bool(np.array_equal(host[0], host[1]))
+    out = (sg._lib.VariantResult * V).from_buffer(host[0])
     r = sg.results_to_dict(out)
-    vps = V / (score_ms * 1e-3)
-    step2 = dict(n=n2, variants=V, batches=nb, batch=B, ms=score_ms, variants_per_s=vps,
-                 generation_ms=gen_ms, wall_s_with_generation=wall,
-                 spa_fraction=float(r["is_spa"].mean()),
-                 dosage_gbs=V * n2 / (score_ms * 1e-3) / 1e9,
-                 timing="HIP events on the engine stream around each batch's score-test "
-                        "launches (generation timed apart)")
+    vps = V / score_s
+    step2 = dict(n=n2, variants=V, batches=nb, batch=B, resident_batches=R,
+                 engines=len(pool.engines), ms=score_ms, variants_per_s=vps,
+                 one_engine_ms=one_s * 1e3, one_engine_variants_per_s=V / one_s,
+                 pool_equals_one_engine_bitwise=same,
+                 generation_ms=gen_ms, spa_fraction=float(r["is_spa"].mean()),
+                 dosage_gbs=V * n2 / score_s / 1e9,
+                 timing="host wall clock between device synchronisations around each chunk's "
+                        "score-test calls (dosages resident; generation timed apart)")
     log(f"step2: {V} variants scored in {score_ms:.1f} ms -> {vps:.3g} variants/s "
-        f"({step2['dosage_gbs']:.0f} GB/s of dosages); generation {gen_ms:.1f} ms; SPA'd "
-        f"{r['is_spa'].mean() * 100:.2f}%")
+        f"({step2['dosage_gbs']:.0f} GB/s of dosages) with {len(pool.engines)} engines; one "
+        f"engine {one_s * 1e3:.1f} ms ({V / one_s:.3g}/s); results equal bitwise: {same}; "
+        f"generation {gen_ms:.1f} ms; SPA'd {r['is_spa'].mean() * 100:.2f}%")
+    if not same:
+        raise SystemExit("step 2: the engine pool's results differ from one engine's")
     cpu2 = None
     if not args.no_cpu:
         # the oracle's score test + fast SPA (saige-rs single_variant.rs / fast.rs restated) on
@@ -977,7 +1000,7 @@ def step2_stream(args, sg, hip, synth):
                           f"(assoc_test.rs:165); {dt:.1f} s"}
         log(f"step2 cpu: {nv} variants in {dt:.2f} s on 1 thread -> {nv / dt:.1f} variants/s; "
             f"GPU vs oracle max |d log10 p| = {dlp:.2e}")
-    eng.close()
+    pool.close()
     return step2, cpu2
 
 

@@ -25,7 +25,7 @@ from ._lib import (PREC_FP64, PREC_MIXED, PREC_INT8, PREC_INT8X2, PREC_F32, TRAI
                    SaigeError, check, load)
 
 __all__ = ["OnTheFlyGrm", "fit_ai_reml", "solve_sigma", "xvx_inv_xv", "estimate_variance_ratio",
-           "fit_null", "fit_null_multi", "ScoreTestEngine", "spa_binary_fast", "marker_stats", "decode",
+           "fit_null", "fit_null_multi", "ScoreTestEngine", "ScoreTestPool", "spa_binary_fast", "marker_stats", "decode",
            "PREC_FP64", "PREC_MIXED", "PREC_INT8", "PREC_INT8X2", "PREC_F32", "TRAIT_BINARY", "TRAIT_QUANTITATIVE", "TRAIT_SURVIVAL",
            "SaigeError", "load", "results_to_dict", "SpaRootMlp", "spa_training_set",
            "launch_log_enable", "launch_log_summary", "pcg_graphs",
@@ -517,6 +517,11 @@ class ScoreTestEngine:
         check(load().saige_engine_set_spa_split(self.h, 1 if enable else 0, C.byref(has)))
         return bool(has.value)
 
+    def copy_tuning_from(self, other):
+        """Take other's tuned sums shapes and sums-form and carrier-pass settings (same n and
+        covariate count), so that both engines compute a batch the same way."""
+        check(load().saige_engine_copy_tuning(self.h, other.h))
+
     def test_markers_u8_device(self, d_q, ldq, nvar, scale, d_out):
         check(load().saige_test_markers_u8_device(self.h, d_q, ldq, nvar, scale, d_out))
 
@@ -539,6 +544,75 @@ class ScoreTestEngine:
             self.close()
         except Exception:
             pass
+
+
+class ScoreTestPool:
+    """Several ScoreTestEngines over one null model, each on its own HIP stream and driven by
+    its own host thread, for streams of device-resident uint8 batches.  One engine leaves
+    the GPU partly idle: its SPA searches read the number of variants still searching back to
+    the host between passes, and their last passes are a few microseconds each.  With two
+    engines one batch's HBM-bound sums run beside another batch's SPA passes.
+
+    Batches are dealt round-robin.  The first batch of each new batch size runs alone on
+    engine 0, whose tuned sums shape is then copied to the others, so that every batch is
+    computed the same way and gives the same bits as on a single engine with that tuning."""
+
+    def __init__(self, *args, engines=2, **kw):
+        if engines < 1:
+            raise ValueError("engines >= 1")
+        self.engines = [ScoreTestEngine(*args, **kw) for _ in range(engines)]
+        self.n = self.engines[0].n
+        self._sizes = set()
+
+    def _share_tuning(self):
+        for e in self.engines[1:]:
+            e.copy_tuning_from(self.engines[0])
+
+    def set_sums_form(self, form):
+        for e in self.engines:
+            e.set_sums_form(form)
+
+    def set_spa_split(self, enable):
+        return all([e.set_spa_split(enable) for e in self.engines])
+
+    def test_markers_u8_device(self, batches):
+        """batches: (d_q, ldq, nvar, scale, d_out) tuples, each as for
+        ScoreTestEngine.test_markers_u8_device.  Returns when every batch's launches are
+        issued; each engine's stream then holds its batches' remaining work (synchronize the
+        device before reading the results)."""
+        import threading
+        todo = list(batches)
+        new = [b for b in todo if b[2] not in self._sizes]
+        for b in new:  # the first batch of a size tunes engine 0's shape for it
+            if b[2] not in self._sizes:
+                self.engines[0].test_markers_u8_device(*b)
+                self._sizes.add(b[2])
+                todo.remove(b)
+        if new:
+            self._share_tuning()
+        k = len(self.engines)
+        errors = [None] * k
+
+        def work(j):
+            try:
+                for b in todo[j::k]:
+                    self.engines[j].test_markers_u8_device(*b)
+            except BaseException as ex:  # re-raised on the calling thread
+                errors[j] = ex
+
+        threads = [threading.Thread(target=work, args=(j,)) for j in range(1, k)]
+        for t in threads:
+            t.start()
+        work(0)
+        for t in threads:
+            t.join()
+        for ex in errors:
+            if ex is not None:
+                raise ex
+
+    def close(self):
+        for e in self.engines:
+            e.close()
 
 
 def spa_binary_fast(mu, g, q, qinv, pval_noadj, tol=1e-6, fast=True, device=0):

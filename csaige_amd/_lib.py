@@ -178,6 +178,7 @@ SIGNATURES = [
     ("saige_engine_set_root_proposal", C.c_int, [vp, vp]),
     ("saige_engine_set_sums_form", C.c_int, [vp, C.c_int]),
     ("saige_engine_set_spa_split", C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+    ("saige_engine_copy_tuning", C.c_int, [vp, vp]),
     ("saige_spa_binary", C.c_int, [C.c_int, dp, dp, sz, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_int, C.POINTER(SpaResult)]),
 ]

@@ -2930,6 +2930,18 @@ int saige_engine_set_spa_split(saige_engine* e, int enable, int* has_table) {
   return SAIGE_OK;
 }
 
+int saige_engine_copy_tuning(saige_engine* dst, saige_engine* src) {
+  clear_error();
+  SAIGE_ARG(dst && src, "engine");
+  SAIGE_ARG(dst->n == src->n && dst->p == src->p, "engines of the same sample and covariate count");
+  if (dst == src) return SAIGE_OK;
+  std::scoped_lock lk(dst->mu, src->mu);
+  dst->tuned_shape = src->tuned_shape;
+  dst->sums_form = src->sums_form;
+  dst->spa_split = src->spa_split;
+  return SAIGE_OK;
+}
+
 int saige_engine_set_sums_form(saige_engine* e, int form) {
   clear_error();
   SAIGE_ARG(e, "engine");

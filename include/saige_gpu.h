@@ -483,6 +483,11 @@ int saige_engine_set_sums_form(saige_engine* e, int form);
  * the table.  enable = 1 (the default) uses it, 0 sums over every sample; *has_table (may be
  * NULL) tells whether the engine has the table. */
 int saige_engine_set_spa_split(saige_engine* e, int enable, int* has_table);
+/* Copy src's tuned sums shapes and its sums-form and carrier-pass settings to dst (no
+ * counterpart in the reference: for several engines over one null model, each on its own
+ * stream and host thread, so that each batch is computed the same way, and so gives the same
+ * bits, whichever engine takes it).  Both engines have the same n and covariate count. */
+int saige_engine_copy_tuning(saige_engine* dst, saige_engine* src);
 
 /* ------------------------------------------------------------------------------
  * SPA on one covariate-adjusted genotype vector.

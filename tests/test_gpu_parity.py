@@ -4,6 +4,7 @@ Bars: integer work (decode, allele counts, MAC, marker split) bit-exact; fp64 GR
 to 1e-12 relative; PCG iterate-identical (same iteration counts) with x to 1e-9; tau and
 the variance ratio to 1e-6 relative; step-2 -log10 p to 1e-6 on the parity mask p >= 1e-10.
 """
+import ctypes as C
 import math
 import os
 
@@ -860,3 +861,45 @@ def test_fit_null_categorical_variance_ratio():
     _say(f"categorical VR: {got['categorical_vr']} (want {want}) over {len(used)} markers")
     assert len(want) >= 1
     assert got["categorical_vr"] == want
+
+
+def test_score_pool_equals_one_engine_and_oracle():
+    """A pool of three engines (own streams, own host threads) over device-resident uint8
+    batches of unequal sizes: every result equals, bit for bit, that of one engine holding the
+    same tuned shapes, and the first batch matches the oracle."""
+    from csaige_amd import hip
+    n, sizes = 100000, (512, 512, 300, 512, 512, 512, 173, 300)
+    nf = synth.null_fit(n=n, p=1, eta_mean=-4.6, eta_sd=0.5, seed=31)
+    a = sg.xvx_inv_xv(nf.x, nf.mu * (1 - nf.mu))
+    cutoff = 0.04550026389635842
+    args = (nf.mu, nf.y - nf.mu, (1.0, 0.0), nf.x, a, 1.05)
+    V = sum(sizes)
+    q, _ = synth.dosage_u8(n, 0, V, seed=32, maf=(1e-3, 0.5))
+    d_q = hip.DeviceBuffer(q.nbytes)
+    d_q.upload(q)
+    rs = C.sizeof(sg._lib.VariantResult)
+    d_r = [hip.DeviceBuffer(V * rs), hip.DeviceBuffer(V * rs)]
+    pool = sg.ScoreTestPool(*args, y=nf.y, spa_pval_cutoff=cutoff, engines=3)
+    one = sg.ScoreTestEngine(*args, y=nf.y, spa_pval_cutoff=cutoff)
+    firsts = [int(f) for f in np.concatenate([[0], np.cumsum(sizes)[:-1]])]
+    batches = [[(d_q.value + f * n, n, s, 1.0 / 127, d_r[k].value + f * rs)
+                for f, s in zip(firsts, sizes)] for k in range(2)]
+    pool.test_markers_u8_device(batches[0])
+    hip.device_synchronize()
+    one.copy_tuning_from(pool.engines[0])
+    for b in batches[1]:
+        one.test_markers_u8_device(*b)
+    hip.device_synchronize()
+    host = [np.empty(V * rs, np.uint8), np.empty(V * rs, np.uint8)]
+    for k in range(2):
+        d_r[k].download(host[k])
+    res = [sg.results_to_dict((sg._lib.VariantResult * V).from_buffer(h)) for h in host]
+    _say(f"pool of 3: {int(res[0]['is_spa'].sum())} of {V} SPA'd")
+    assert res[0]["is_spa"].sum() >= 10
+    np.testing.assert_array_equal(host[0], host[1])
+    eo = oracle.ScoreEngine(*args, y=nf.y, spa_cutoff=cutoff)
+    want = eo.test_batch_u8(q[:sizes[0]], 1.0 / 127.0)
+    _compare({k: v[:sizes[0]] for k, v in res[0].items()}, want, integer_dosages=False,
+             ac_rtol=5e-11)
+    pool.close()
+    one.close()
