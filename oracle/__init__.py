@@ -6,9 +6,10 @@ ctypes bindings for ``liboracle_saige.so``, the C++ restatement of the saige-rs 
 the reported CPU baseline.  The product (``csaige_amd``) never imports it.
 
 Functions pinned only by the mathematics or by invariants are marked in their tests;
-the RNG replicas of rand 0.8.5 / rand_chacha 0.3.1 / rand_core 0.6.4 and the statrs
-CDF restatements are "parity unpinned" beyond those checks (no reference test pins
-their draws or last bits).
+the ChaCha core is pinned by OpenSSL's chacha20 keystream and PCG32 (seed_from_u64) by the
+PCG reference library's published output; the rand 0.8.5 draws (gen::<bool>, gen_range,
+shuffle) and the statrs CDF restatements are "parity unpinned" beyond those checks (no
+reference test pins their draws or last bits).
 """
 from __future__ import annotations
 
@@ -106,6 +107,7 @@ def _declare(L):
     L.orc_compute_af.argtypes = [_dp, C.c_size_t, _dp, _dp, _dp, _u64p]
     L.orc_chacha_block.argtypes = [_u32p, C.c_uint64, C.c_uint64, C.c_int, _u32p]
     L.orc_seed_from_u64.argtypes = [C.c_uint64, _u32p]
+    L.orc_pcg32_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, _u32p]
     L.orc_rng_u32_stream.argtypes = [C.c_uint64, C.c_size_t, _u32p]
     L.orc_rademacher.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, _dp]
     L.orc_gen_range.argtypes = [C.c_uint64, C.c_size_t, C.c_uint64, _u64p]
@@ -209,6 +211,14 @@ def seed_from_u64(seed):
     k = np.empty(8, np.uint32)
     lib().orc_seed_from_u64(seed, _p(k, _u32p))
     return k
+
+
+def pcg32_stream(state, inc, count):
+    """PCG32 outputs (XSH-RR of the state before each step) from `state` with increment
+    `inc`; seed_from_u64(x) is this stream from x * 6364136223846793005 + INC."""
+    out = np.empty(count, np.uint32)
+    lib().orc_pcg32_stream(state, inc, count, _p(out, _u32p))
+    return out
 
 
 def rng_u32_stream(seed, count):

@@ -22,9 +22,11 @@
 //   GRM product          : dense brute force Z Z^T v / M (numpy) + symmetry/PSD invariants
 //   PCG                  : KATs decomposition.rs:470-526, pcg.rs:202-223; numpy.linalg.solve
 //   ChaCha8 core         : OpenSSL chacha20 keystream (20 rounds) through the same core
-//   seed_from_u64 / gen_range / gen::<bool> / shuffle : rand 0.8.5 / rand_core 0.6.4
-//                          published algorithms; parity unpinned (no reference test pins
-//                          the draws) beyond invariants (range, permutation, bit balance)
+//   PCG32 (seed_from_u64): the PCG reference library's published demo output
+//                          (pcg32_srandom_r(42, 54)); rand_core's increment per its source
+//   gen_range / gen::<bool> / shuffle : rand 0.8.5 published algorithms; parity unpinned
+//                          (no reference test pins the draws) beyond invariants (range,
+//                          permutation, bit balance)
 //   AI-REML              : dense brute-force REML score equation at convergence (numpy),
 //                          IRLS KAT ai_reml.rs:489-508
 //   variance ratio       : cal_cv / select KATs variance_ratio.rs:344-370, closed forms
@@ -72,16 +74,27 @@ static void chacha_block(const uint32_t key[8], uint64_t counter, uint64_t strea
   for (int i = 0; i < 16; ++i) out[i] = x[i] + s[i];
 }
 
-// rand_core 0.6.4 SeedableRng::seed_from_u64: PCG32 fills the 32-byte seed.
-static void seed_from_u64(uint64_t state, uint32_t key[8]) {
-  const uint64_t MUL = 6364136223846793005ull;
-  const uint64_t INC = 11634580027462260723ull;
-  for (int w = 0; w < 8; ++w) {
-    state = state * MUL + INC;
-    uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
-    uint32_t rot = (uint32_t)(state >> 59);
-    key[w] = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+// PCG32 (O'Neill's pcg32_random_r): 64-bit LCG state, XSH-RR output of the state before
+// each step.
+static const uint64_t kPcgMul = 6364136223846793005ull;
+static inline uint32_t pcg32_output(uint64_t state) {
+  uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
+  uint32_t rot = (uint32_t)(state >> 59);
+  return (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+}
+static void pcg32_stream(uint64_t state, uint64_t inc, size_t count, uint32_t* out) {
+  for (size_t i = 0; i < count; ++i) {
+    out[i] = pcg32_output(state);
+    state = state * kPcgMul + inc;
   }
+}
+
+// rand_core 0.6.4 SeedableRng::seed_from_u64: PCG32 with increment INC fills the 32-byte
+// seed, each word the output of the state after a step, i.e. the PCG32 stream that starts
+// at seed * MUL + INC.
+static void seed_from_u64(uint64_t state, uint32_t key[8]) {
+  const uint64_t INC = 11634580027462260723ull;
+  pcg32_stream(state * kPcgMul + INC, INC, 8, key);
 }
 
 // ChaCha8Rng (rand_chacha 0.3.1): key = seed, stream 0, 64-bit block counter from 0;
@@ -438,6 +451,9 @@ void orc_chacha_block(const uint32_t key[8], uint64_t counter, uint64_t stream, 
   chacha_block(key, counter, stream, rounds, out);
 }
 void orc_seed_from_u64(uint64_t seed, uint32_t key[8]) { seed_from_u64(seed, key); }
+void orc_pcg32_stream(uint64_t state, uint64_t inc, size_t count, uint32_t* out) {
+  pcg32_stream(state, inc, count, out);
+}
 void orc_rng_u32_stream(uint64_t seed, size_t count, uint32_t* out) {
   ChaCha8Rng r(seed);
   for (size_t i = 0; i < count; ++i) out[i] = r.next_u32();

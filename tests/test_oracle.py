@@ -87,6 +87,23 @@ def test_seed_from_u64_pcg32():
     np.testing.assert_array_equal(oracle.seed_from_u64(12345), np.array(want, np.uint32))
 
 
+def test_pcg32_core_vs_published_reference_output():
+    # pcg32-demo of the PCG reference library (pcg-c): pcg32_srandom_r(42, 54) and its first
+    # six pcg32_random_r outputs, as printed by the demo ("Round 1").  Seeding: inc =
+    # 2 * 54 + 1; state = 0, one step, state += 42, one step.
+    mul, inc = 6364136223846793005, 2 * 54 + 1
+    state = ((inc + 42) * mul + inc) % (1 << 64)
+    want = [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+    np.testing.assert_array_equal(oracle.pcg32_stream(state, inc, 6), np.array(want, np.uint32))
+
+
+def test_seed_from_u64_is_pcg32_stream_with_rand_core_increment():
+    inc = 11634580027462260723
+    for seed in (0, 1, 12345, 2**64 - 1):
+        start = (seed * 6364136223846793005 + inc) % (1 << 64)
+        np.testing.assert_array_equal(oracle.seed_from_u64(seed), oracle.pcg32_stream(start, inc, 8))
+
+
 def test_rademacher_is_sign_bit_stream():
     r = oracle.rademacher(7, 50, 3)
     s = oracle.rng_u32_stream(7, 150)
