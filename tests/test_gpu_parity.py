@@ -334,6 +334,26 @@ def test_step2_full_sample_count_vs_oracle():
         _compare(r, want, integer_dosages=False, ac_rtol=5e-11)
 
 
+def test_step2_full_sample_count_with_covariates_vs_oracle():
+    """Step 2 with two covariates beside the intercept at 400,000 samples: the sums kernels
+    of p = 3 (seven linear features, 72 digit columns on the matrix cores) at the real
+    sample-split bound, the full SPA passes of a covariate model, first (timed shapes) and
+    second batch against the oracle."""
+    n, nvar = 400000, 384
+    X, mu, y, a = _null(n, 3, seed=41, eta=-4.0)
+    q, _ = synth.dosage_u8(n, 0, nvar, seed=42, maf=(1e-3, 0.5))
+    e = sg.ScoreTestEngine(mu, y - mu, (1.0, 0.4), X.T, a, 1.02, y=y, spa_pval_cutoff=0.05)
+    first = e.test_markers_u8(q, 1.0 / 127.0)
+    got = e.test_markers_u8(q, 1.0 / 127.0)
+    eo = oracle.ScoreEngine(mu, y - mu, (1.0, 0.4), X.T, a, 1.02, y=y, spa_cutoff=0.05)
+    want = eo.test_batch_u8(q, 1.0 / 127.0)
+    _say(f"n=400,000, p=3: {int(want['is_spa'].sum())} of {nvar} SPA'd")
+    assert want["is_spa"].sum() >= 5
+    for r in (first, got):
+        _compare(r, want, integer_dosages=False, ac_rtol=5e-11)
+    e.close()
+
+
 # With the score near the null mean the saddle point sits near 0 and the Lugannani-Rice form
 # z = w + ln(v / w) / w (fast.rs:135-180) turns a rounding error dK of the summed K(zeta) into
 # dp ~ phi(w) dK / w^3: with dK ~ 1e-13 (sums over 10^3 to 10^5 samples) the p of |z| below
