@@ -267,12 +267,13 @@ def test_score_u8_vs_oracle(p):
     _compare(got, want, integer_dosages=False)
 
 
-@pytest.mark.parametrize("form", [0, 1, 2])
-@pytest.mark.parametrize("p", [1, 3])
+@pytest.mark.parametrize("p,form", [(1, 0), (1, 1), (1, 2), (3, 0), (3, 1), (3, 2), (8, 1),
+                                    (8, 2)])
 def test_score_u8_each_sums_form_vs_oracle(p, form):
-    """Each kernel of the uint8 sums (0 lanes over samples, 1 a lane per variant, 2 the int8
-    matrix cores on a fixed-point expansion of the features) against the oracle, on a sample
-    count that leaves a partial 256-sample block, hard calls and noisy dosages."""
+    """Each kernel of the uint8 sums (0 lanes over samples, at most 3 covariates; 1 a lane per
+    variant; 2 the int8 matrix cores on a fixed-point expansion of the features, 11 column
+    tiles at the largest covariate count, 8) against the oracle, on a sample count that
+    leaves a partial 256-sample block, hard calls and noisy dosages."""
     n, nvar = 5000 + 77, 700
     X, mu, y, a = _null(n, p, seed=60 + p)
     q, _ = synth.dosage_u8(n, 0, nvar, seed=21 + p, maf=(1e-3, 0.5))
