@@ -267,13 +267,33 @@ def test_score_u8_vs_oracle(p):
     _compare(got, want, integer_dosages=False)
 
 
+def _score_u8_padded(e, q, scale=1.0 / 127.0):
+    """Score (nvar, n) uint8 dosages through the device entry point with each row padded to a
+    multiple of 16 bytes: the byte kernels of the sums and the SPA passes take 16-byte-aligned
+    rows only (others take the generic loader kernels)."""
+    from csaige_amd import hip
+    nvar, n = q.shape
+    ld = -(-n // 16) * 16
+    qp = np.zeros((nvar, ld), np.uint8)
+    qp[:, :n] = q
+    rs = C.sizeof(sg._lib.VariantResult)
+    d_q, d_r = hip.DeviceBuffer(qp.nbytes), hip.DeviceBuffer(nvar * rs)
+    d_q.upload(qp)
+    e.test_markers_u8_device(d_q.value, ld, nvar, scale, d_r.value)
+    hip.device_synchronize()
+    host = np.empty(nvar * rs, np.uint8)
+    d_r.download(host)
+    return sg.results_to_dict((sg._lib.VariantResult * nvar).from_buffer(host))
+
+
 @pytest.mark.parametrize("p,form", [(1, 0), (1, 1), (1, 2), (3, 0), (3, 1), (3, 2), (8, 1),
                                     (8, 2)])
 def test_score_u8_each_sums_form_vs_oracle(p, form):
     """Each kernel of the uint8 sums (0 lanes over samples, at most 3 covariates; 1 a lane per
     variant; 2 the int8 matrix cores on a fixed-point expansion of the features, 11 column
     tiles at the largest covariate count, 8) against the oracle, on a sample count that
-    leaves a partial 256-sample block, hard calls and noisy dosages."""
+    leaves a partial 256-sample block (rows padded to 16 bytes, which these kernels need),
+    hard calls and noisy dosages; the generic loader kernel (unpadded rows) beside them."""
     n, nvar = 5000 + 77, 700
     X, mu, y, a = _null(n, p, seed=60 + p)
     q, _ = synth.dosage_u8(n, 0, nvar, seed=21 + p, maf=(1e-3, 0.5))
@@ -282,12 +302,14 @@ def test_score_u8_each_sums_form_vs_oracle(p, form):
         q[v, (y > 0.5) & (rng.random(n) < 0.3)] = 254
     e = sg.ScoreTestEngine(mu, y - mu, (1.0, 0.3), X.T, a, 1.05, y=y, spa_pval_cutoff=0.05)
     e.set_sums_form(form)
-    got = e.test_markers_u8(q, 1.0 / 127.0)
+    got = _score_u8_padded(e, q)
+    generic = e.test_markers_u8(q, 1.0 / 127.0)  # n % 16 != 0: unaligned rows
     eo = oracle.ScoreEngine(mu, y - mu, (1.0, 0.3), X.T, a, 1.05, y=y, spa_cutoff=0.05)
     want = eo.test_batch_u8(q, 1.0 / 127.0)
     _say(f"p={p} form {form}: spa'd {int(want['is_spa'].sum())} of {nvar}")
     assert want["is_spa"].sum() > 10
     _compare(got, want, integer_dosages=False)
+    _compare(generic, want, integer_dosages=False)
 
 
 def test_score_u8_timed_shape_matches_fixed_forms():
