@@ -459,7 +459,7 @@ def main():
                     help="headline: step-2 variants streamed (generated on the device)")
     ap.add_argument("--step2-engines", type=int, default=2,
                     help="headline: score-test engines (streams and host threads) of step 2")
-    ap.add_argument("--step2-resident", type=int, default=32,
+    ap.add_argument("--step2-resident", type=int, default=64,
                     help="headline: step-2 batches generated and resident per timed chunk")
     ap.add_argument("--cpu-fit-markers", type=int, default=500,
                     help="headline: markers of the CPU baseline's sampled f32 GRM product")
